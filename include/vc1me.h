@@ -1,0 +1,219 @@
+/*
+ * vc1me.h — C ABI of the MI355X (gfx950) motion-estimation backend for libvc1.
+ *
+ * The backend occupies the encoder's motion-cost compute slot
+ * (libvc1 src/encoder_internal.h:286-289, EncoderConfig::vulkan_compute and
+ * vulkan_min_batch*) that the experimental Vulkan backend fills today, and adds
+ * a whole-frame entry point in front of Vc1Encoder::analyze_p_picture_core
+ * (src/encoder_motion.cpp:47-442).  Only plain C types cross this boundary:
+ * raw pointers, sizes and POD structs.  No C++ or torch types.
+ *
+ * Status convention: every int-returning entry point returns VC1ME_OK (0) on
+ * success and a negative VC1ME_E* code on failure.  A failure is recoverable:
+ * the caller re-scores on the CPU exactly as the reference does when
+ * VulkanComputeContext::score_motion_batch returns false
+ * (src/vulkan_compute.cpp:486-489, callers e.g. src/encoder_motion.cpp:1938-1942).
+ * The message of the last failure on the calling thread is returned by
+ * vc1me_last_error() (thread-local, like vc1_encoder_last_error,
+ * include/libvc1.h:653).
+ *
+ * Thread safety: one context may be shared by several host threads; every
+ * call that touches the device is serialised by a per-context mutex
+ * (the reference serialises on one mutex too, src/vulkan_compute.cpp:385).
+ */
+#ifndef VC1ME_H
+#define VC1ME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VC1ME_ABI_VERSION 1
+
+enum {
+    VC1ME_OK = 0,
+    VC1ME_EINVAL = -1,     /* bad argument (null pointer, size out of range) */
+    VC1ME_EDEVICE = -2,    /* HIP runtime / device failure */
+    VC1ME_ENOMEM = -3,     /* device allocation failed */
+    VC1ME_ELIMIT = -4,     /* request exceeds a documented limit */
+    VC1ME_EINJECTED = -5,  /* failure forced by VC1ME_FAIL_AT (fault injection) */
+    VC1ME_EKERNEL = -6     /* a kernel reported an internal error flag */
+};
+
+/* Candidate motion vector in quarter-pel units; mirrors VulkanMotionCandidate
+ * (src/vulkan_compute.h:11-14). */
+typedef struct vc1me_candidate {
+    int32_t xq;
+    int32_t yq;
+} vc1me_candidate;
+
+/* Cost metric; mirrors VulkanMotionMetric {Sad=0,Satd=1} (src/vulkan_compute.h:16). */
+enum { VC1ME_METRIC_SAD = 0, VC1ME_METRIC_SATD = 1 };
+
+/* Progressive MV modes; mirrors Vc1Encoder::ProgressiveMvMode
+ * (src/encoder_internal.h:338). */
+enum {
+    VC1ME_MODE_ONE_MV_QPEL = 0,
+    VC1ME_MODE_ONE_MV_HPEL = 1,
+    VC1ME_MODE_ONE_MV_HPEL_BILINEAR = 2,
+    VC1ME_MODE_MIXED_MV = 3
+};
+
+/* Motion-estimation parameters: the EncoderConfig fields the P analysis reads
+ * (src/encoder_internal.h:223-315; ME fields :246-253, :289-302) plus the two
+ * call arguments of analyze_p_picture_core (mode, allow_intra). */
+typedef struct vc1me_params {
+    int32_t width;                      /* coded even raster width  (EncoderConfig::width) */
+    int32_t height;                     /* coded even raster height (EncoderConfig::height) */
+    int32_t display_width;              /* 0 = width  (EncoderConfig::display_width) */
+    int32_t display_height;             /* 0 = height (EncoderConfig::display_height) */
+    int32_t pqindex;                    /* 1..31 */
+    int32_t motion_search_range;        /* 0..1024 px */
+    int32_t motion_local_search_range;  /* px, default 32 */
+    int32_t me_algorithm;               /* 0 = UMH, 1 = diamond (vc1_me_algorithm_e) */
+    int32_t me_quality;                 /* 0 = SAD, 1 = RATE, 2 = SATD (RD is not offered) */
+    int32_t subpixel_refinement;        /* bool */
+    int32_t fast_subpixel_refinement;   /* bool */
+    int32_t subpixel_refinement_range;  /* px, default 1024 */
+    int32_t long_range_search_mode;     /* 0 compare, 1 local-good skip, 2 legacy distant-first */
+    int32_t mode;                       /* VC1ME_MODE_* */
+    int32_t allow_intra;                /* bool */
+    int32_t debug_disable_p_intra;      /* bool */
+    double distant_match_max_mae;       /* default 255.0 */
+    double inter_intra_threshold;       /* default 0.80 */
+} vc1me_params;
+
+/* Fill p with the reference's defaults for a width x height raster
+ * (src/encoder_internal.h:243-253, :295, :289-302). */
+void vc1me_params_default(vc1me_params* p, int32_t width, int32_t height);
+
+/* Per-frame P analysis output; the fields of Vc1Encoder::PAnalysis that ME
+ * produces (src/encoder_internal.h:346-370).  Arrays are caller-owned and
+ * hold one entry per macroblock (mbw*mbh, raster order) unless noted. */
+typedef struct vc1me_p_result {
+    int32_t* mvs;                    /* [mbs][2]  decoder block-0 / 1-MV vector, qpel */
+    int32_t* block_mvs;              /* [mbs][4][2] four luma block vectors, qpel */
+    uint8_t* use_4mv;                /* [mbs] */
+    uint8_t* use_intra;              /* [mbs] */
+    uint64_t* screen_sads;           /* [mbs] */
+    double* inter_intra_cost_ratio;  /* [mbs]  -1 when unavailable */
+    double* distant_local_mae;       /* [mbs]  -1 when unavailable */
+    double* distant_candidate_mae;   /* [mbs]  -1 when unavailable */
+    uint8_t* distant_match_decision; /* [mbs]  vc1_distant_match_decision_e */
+    /* scalar outputs */
+    double mean_abs_residual;
+    double rate_complexity;
+    uint64_t motion_sad;
+    uint64_t moved_macroblocks;
+    uint64_t fractional_chroma_macroblocks;
+    uint64_t four_mv_macroblocks;
+    uint64_t intra_macroblocks;
+    /* diagnostics (not in PAnalysis): the frame translation found by the
+     * global search (src/encoder_motion.cpp:93-101) */
+    int32_t global_dx;
+    int32_t global_dy;
+    uint64_t global_sad;
+} vc1me_p_result;
+
+/* One 8-bit 4:2:0 picture on the host: tight planes, stride = width
+ * (libvc1 Frame, src/encoder_internal.h:67-70). */
+typedef struct vc1me_planes {
+    const uint8_t* y;
+    const uint8_t* u;
+    const uint8_t* v;
+    int32_t width;
+    int32_t height;
+} vc1me_planes;
+
+/* Per-context counters; mirrors VulkanComputeCounters (src/vulkan_compute.h:18-24)
+ * and adds whole-frame work. */
+typedef struct vc1me_counters {
+    uint64_t dispatches;
+    uint64_t candidates;
+    uint64_t frame_uploads;
+    uint64_t frame_upload_bytes;
+    uint64_t submit_wait_nanoseconds;
+    uint64_t frames_analyzed;
+    uint64_t macroblocks_analyzed;
+} vc1me_counters;
+
+typedef struct vc1me_ctx vc1me_ctx;
+typedef struct vc1me_frame vc1me_frame;
+
+/* ---- lifecycle (replaces VulkanComputeContext::create, src/vulkan_compute.cpp:493-505) */
+int vc1me_create(int device, vc1me_ctx** out);
+void vc1me_destroy(vc1me_ctx* ctx);
+const char* vc1me_last_error(void);
+int vc1me_abi_version(void);
+/* Device name of the context (VulkanComputeContext::device_name, src/vulkan_compute.h:33). */
+const char* vc1me_device_name(const vc1me_ctx* ctx);
+
+/* ---- (i) batch scorer: exactly VulkanComputeContext::score_motion_batch
+ * (src/vulkan_compute.h:43-51, src/vulkan_compute.cpp:370-490) with a raw
+ * costs array of n entries.  Planes are luma only, tight (stride = width).
+ * A null plane, non-positive size or n == 0 succeeds with nothing written
+ * (src/vulkan_compute.cpp:376-379).  Limits: n <= 4,194,240 and
+ * width*height <= INT32_MAX (src/vulkan_compute.cpp:380-383).
+ * Host planes are uploaded once per (pointer, size, content checksum). */
+int vc1me_score_batch(vc1me_ctx* ctx, const uint8_t* cur, const uint8_t* ref,
+                      int width, int height, int x0, int y0, int bw, int bh,
+                      int sample_stride, int bilinear, int rnd, int metric,
+                      const vc1me_candidate* candidates, size_t n,
+                      uint64_t* costs_out);
+
+/* ---- frames resident in HBM (explicit handles instead of the reference's
+ * pointer+fingerprint cache, src/vulkan_compute.cpp:194-231). */
+int vc1me_frame_upload(vc1me_ctx* ctx, const vc1me_planes* planes, vc1me_frame** out);
+void vc1me_frame_release(vc1me_ctx* ctx, vc1me_frame* frame);
+
+/* Same contract as vc1me_score_batch on resident frames. */
+int vc1me_score_batch_frames(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                             int x0, int y0, int bw, int bh, int sample_stride,
+                             int bilinear, int rnd, int metric,
+                             const vc1me_candidate* candidates, size_t n,
+                             uint64_t* costs_out);
+
+/* ---- (ii) whole-frame plain P motion estimation: the behaviour of
+ * Vc1Encoder::analyze_p_picture_core(cur, ref, mode, allow_intra,
+ * shared_seed=nullptr, b_reuse=nullptr) (src/encoder_motion.cpp:47-442).
+ * Results are bit-exact with the reference CPU path. */
+int vc1me_analyze_p_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                          const vc1me_params* params, vc1me_p_result* out);
+
+/* ---- (iii) batched multi-stream step: n independent (cur, ref) pairs of
+ * possibly different rasters analysed in one launch sequence on the
+ * context's stream.  Equivalent to n calls of vc1me_analyze_p_frame. */
+typedef struct vc1me_job {
+    const vc1me_frame* cur;
+    const vc1me_frame* ref;
+    const vc1me_params* params;
+    vc1me_p_result* out;      /* may be NULL when results are not wanted on the host */
+} vc1me_job;
+int vc1me_analyze_step(vc1me_ctx* ctx, const vc1me_job* jobs, int n);
+
+/* Timing of the last vc1me_analyze_step / vc1me_analyze_p_frame: device
+ * milliseconds between HIP events recorded on the context stream around the
+ * whole launch sequence, and around every launch of the macroblock search
+ * kernel (sum and count). */
+typedef struct vc1me_timing {
+    double step_ms;
+    double mb_kernel_ms_total;
+    uint64_t mb_kernel_launches;
+    double global_kernel_ms_total;
+    uint64_t global_kernel_launches;
+} vc1me_timing;
+int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
+
+/* ---- counters (VulkanComputeContext::counters / reset_counters,
+ * src/vulkan_compute.h:53-54) */
+int vc1me_counters_get(vc1me_ctx* ctx, vc1me_counters* out);
+void vc1me_counters_reset(vc1me_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VC1ME_H */

@@ -1,0 +1,73 @@
+/*
+ * vc1me_oracle.h — CPU restatement of libvc1's P-frame motion estimation.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker.  The product (libvc1_amd) never links, loads or falls back to it.
+ *
+ * Types are those of the product ABI (include/vc1me.h) so results compare
+ * field by field.
+ */
+#ifndef VC1ME_ORACLE_H
+#define VC1ME_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/vc1me.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Algorithmic operation counters (units of the VALU roofline):
+ *   sad_px      one |a-b| accumulate per sampled pixel
+ *   interp_taps one 4-tap filter application (separable 2D counts both passes)
+ *   satd_ops    112 per 4x4 Hadamard tile
+ *   sig_ops     one signature comparison per long-range index cell scanned
+ *   frame_px    one |a-b| accumulate per sampled pixel of a frame SAD */
+typedef struct oracle_ops {
+    uint64_t sad_px;
+    uint64_t interp_taps;
+    uint64_t satd_ops;
+    uint64_t sig_ops;
+    uint64_t frame_px;
+} oracle_ops;
+
+/* Whole-frame P analysis: Vc1Encoder::analyze_p_picture_core(cur, ref, mode,
+ * allow_intra, nullptr, nullptr), src/encoder_motion.cpp:47-442.
+ * Returns 0, or -1 for an unsupported configuration (me_quality SAD/RD). */
+int oracle_analyze_p_frame(const vc1me_params* p, const vc1me_planes* cur,
+                           const vc1me_planes* ref, vc1me_p_result* out, oracle_ops* ops);
+
+/* Batch scorer semantics of VulkanComputeContext::score_motion_batch as the
+ * GLSL kernel computes it (src/shaders/vulkan_motion_cost.comp:109-183):
+ * clamped fetch, sampled SAD or SATD4x4, no hidden-padding veto. */
+void oracle_score_batch(const uint8_t* cur, const uint8_t* ref, int width, int height,
+                        int x0, int y0, int bw, int bh, int sample_stride, int bilinear,
+                        int rnd, int metric, const vc1me_candidate* c, size_t n,
+                        uint64_t* costs);
+
+/* Primitives exposed for unit tests (same laws as the reference). */
+int oracle_luma_mc_sample(const uint8_t* plane, int w, int h, int x, int y, int xq, int yq,
+                          int mode, int rnd);
+uint64_t oracle_mb_sad(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                       int mx, int my, int dx, int dy, int stride);
+uint64_t oracle_block_cost(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                           int x0, int y0, int bw, int bh, int xq, int yq, int mode, int metric);
+uint64_t oracle_frame_sad(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                          int dx, int dy, int stride);
+void oracle_global_motion(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                          int range, int32_t* dx, int32_t* dy, uint64_t* sad);
+/* MV rate in bits: motion_vector_rate_bits (src/encoder_motion.cpp:3214-3225). */
+int oracle_mv_rate_bits(const vc1me_params* p, int dxq, int dyq, int pxq, int pyq, int mode, int more);
+/* x87 long-double score compare used by the staged sub-pel finalists:
+ * returns -1/0/1 for (sad_a + lambda*rate_a) <=> (sad_b + lambda*rate_b)
+ * evaluated exactly as the reference does (src/encoder_motion.cpp:3465-3471). */
+int oracle_score_cmp(int pqindex, uint64_t sad_a, uint64_t rate_a, uint64_t sad_b, uint64_t rate_b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
