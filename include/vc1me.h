@@ -206,6 +206,11 @@ typedef struct vc1me_timing {
     uint64_t global_kernel_launches;
 } vc1me_timing;
 int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
+/* Record HIP events around every macroblock-kernel launch so that
+ * vc1me_timing.mb_kernel_ms_total is filled (off by default: the events add
+ * a little launch overhead).  No reference counterpart (profiling aid,
+ * like the reference's SpeedProfiler, src/encoder_internal.h:182-221). */
+int vc1me_set_profiling(vc1me_ctx* ctx, int on);
 
 /* ---- counters (VulkanComputeContext::counters / reset_counters,
  * src/vulkan_compute.h:53-54) */

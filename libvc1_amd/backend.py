@@ -1,0 +1,169 @@
+"""Python binding of the C ABI in include/vc1me.h (libvc1_amd/libvc1me.so).
+
+This is plumbing for tests and benchmarks: it loads the in-tree HIP library
+and forwards every call through ctypes.  There is no fallback: if the library
+or a HIP device is missing, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvc1me.so")
+
+# every symbol declared in include/vc1me.h
+EXPORTS = (
+    "vc1me_params_default", "vc1me_create", "vc1me_destroy", "vc1me_last_error", "vc1me_abi_version",
+    "vc1me_device_name", "vc1me_score_batch", "vc1me_frame_upload", "vc1me_frame_release",
+    "vc1me_score_batch_frames", "vc1me_analyze_p_frame", "vc1me_analyze_step", "vc1me_last_timing",
+    "vc1me_set_profiling", "vc1me_counters_get", "vc1me_counters_reset",
+)
+
+
+class BackendError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vc1me error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libvc1me.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: run `python -m libvc1_amd.build` first")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    lib.vc1me_params_default.argtypes = [P(abi.Params), C.c_int32, C.c_int32]
+    lib.vc1me_params_default.restype = None
+    lib.vc1me_create.argtypes = [C.c_int, P(C.c_void_p)]
+    lib.vc1me_destroy.argtypes = [C.c_void_p]
+    lib.vc1me_destroy.restype = None
+    lib.vc1me_last_error.restype = C.c_char_p
+    lib.vc1me_device_name.argtypes = [C.c_void_p]
+    lib.vc1me_device_name.restype = C.c_char_p
+    lib.vc1me_score_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
+    lib.vc1me_frame_upload.argtypes = [C.c_void_p, P(abi.Planes), P(C.c_void_p)]
+    lib.vc1me_frame_release.argtypes = [C.c_void_p, C.c_void_p]
+    lib.vc1me_frame_release.restype = None
+    lib.vc1me_score_batch_frames.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
+    lib.vc1me_analyze_p_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult)]
+    lib.vc1me_analyze_step.argtypes = [C.c_void_p, P(abi.Job), C.c_int]
+    lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
+    lib.vc1me_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    lib.vc1me_counters_get.argtypes = [C.c_void_p, P(abi.Counters)]
+    lib.vc1me_counters_reset.argtypes = [C.c_void_p]
+    lib.vc1me_counters_reset.restype = None
+    _lib = lib
+    return lib
+
+
+class Frame:
+    """A picture resident in device memory (vc1me_frame handle)."""
+
+    def __init__(self, backend: "Backend", handle: int, width: int, height: int):
+        self.backend, self.handle, self.width, self.height = backend, handle, width, height
+
+    def release(self) -> None:
+        if self.handle:
+            self.backend.lib.vc1me_frame_release(self.backend.ctx, self.handle)
+            self.handle = None
+
+
+class Backend:
+    """One vc1me context on a HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        ctx = C.c_void_p()
+        self._check(self.lib.vc1me_create(device, C.byref(ctx)))
+        self.ctx = ctx
+
+    def _check(self, rc: int) -> None:
+        if rc != abi.VC1ME_OK:
+            raise BackendError(rc, self.lib.vc1me_last_error().decode())
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            self.lib.vc1me_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def device_name(self) -> str:
+        return self.lib.vc1me_device_name(self.ctx).decode()
+
+    def upload(self, pic) -> Frame:
+        ps = abi.PlaneSet.of(pic)
+        h = C.c_void_p()
+        self._check(self.lib.vc1me_frame_upload(self.ctx, C.byref(ps.struct), C.byref(h)))
+        return Frame(self, h.value, ps.struct.width, ps.struct.height)
+
+    def analyze(self, cur: Frame, ref: Frame, params: abi.Params) -> abi.ResultArrays:
+        r = abi.ResultArrays(params.width, params.height)
+        self._check(self.lib.vc1me_analyze_p_frame(self.ctx, cur.handle, ref.handle, C.byref(params), C.byref(r.struct)))
+        return r
+
+    def analyze_step(self, pairs, params_list, want_results: bool = True):
+        n = len(pairs)
+        jobs = (abi.Job * n)()
+        results = []
+        keep = []
+        for i, ((cur, ref), p) in enumerate(zip(pairs, params_list)):
+            jobs[i].cur = cur.handle
+            jobs[i].ref = ref.handle
+            jobs[i].params = C.pointer(p)
+            keep.append(p)
+            if want_results:
+                r = abi.ResultArrays(p.width, p.height)
+                results.append(r)
+                jobs[i].out = C.pointer(r.struct)
+            else:
+                jobs[i].out = None
+        self._check(self.lib.vc1me_analyze_step(self.ctx, jobs, n))
+        return results
+
+    def score_batch(self, cur: np.ndarray, ref: np.ndarray, x0: int, y0: int, bw: int, bh: int,
+                    cands: np.ndarray, sample_stride: int = 1, bilinear: int = 0, rnd: int = 0,
+                    metric: int = abi.METRIC_SAD) -> np.ndarray:
+        cur = np.ascontiguousarray(cur, np.uint8)
+        ref = np.ascontiguousarray(ref, np.uint8)
+        h, w = cur.shape
+        cands = np.ascontiguousarray(cands, np.int32).reshape(-1, 2)
+        out = np.zeros(len(cands), np.uint64)
+        self._check(self.lib.vc1me_score_batch(
+            self.ctx, cur.ctypes.data, ref.ctypes.data, w, h, x0, y0, bw, bh, sample_stride, bilinear, rnd,
+            metric, cands.ctypes.data_as(C.POINTER(abi.Candidate)), len(cands),
+            out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
+    def set_profiling(self, on: bool) -> None:
+        self._check(self.lib.vc1me_set_profiling(self.ctx, 1 if on else 0))
+
+    def timing(self) -> abi.Timing:
+        t = abi.Timing()
+        self._check(self.lib.vc1me_last_timing(self.ctx, C.byref(t)))
+        return t
+
+    def counters(self) -> abi.Counters:
+        c = abi.Counters()
+        self._check(self.lib.vc1me_counters_get(self.ctx, C.byref(c)))
+        return c

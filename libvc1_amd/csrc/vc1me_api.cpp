@@ -1,0 +1,679 @@
+// vc1me_api.cpp — host side of the C ABI declared in include/vc1me.h.
+//
+// Owns the HIP device, stream, resident frames and per-job device state, and
+// drives the kernels of vc1me_kernels.hip.  Everything here is plain HIP
+// runtime + C++17; no torch types.
+#include "../../include/vc1me.h"
+#include "vc1me_internal.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using vc1me::GlobalState;
+using vc1me::JobDesc;
+
+struct vc1me_frame {
+    uint8_t* y = nullptr;
+    uint8_t* u = nullptr;
+    uint8_t* v = nullptr;
+    int32_t w = 0, h = 0;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(VC1ME_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Fault injection (VC1ME_FAIL_AT=N makes the N-th device call fail).
+std::atomic<long long> g_calls{0};
+bool injected_failure() {
+    static const long long at = [] {
+        const char* e = std::getenv("VC1ME_FAIL_AT");
+        return e && *e ? std::atoll(e) : 0LL;
+    }();
+    const long long n = ++g_calls;
+    return at > 0 && n == at;
+}
+
+uint64_t fnv64(const uint8_t* p, size_t n) {
+    uint64_t h = 1469598103934665603ull;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        h = (h ^ w) * 1099511628211ull;
+    }
+    for (; i < n; ++i) h = (h ^ p[i]) * 1099511628211ull;
+    return h;
+}
+
+struct JobSlot {
+    size_t mbs_cap = 0, cells_cap = 0;
+    void* mem = nullptr;  // one device allocation carved into the arrays below
+    size_t bytes = 0;
+    int2* mvs = nullptr;
+    int2* bmv = nullptr;
+    uint8_t* flags = nullptr;
+    uint8_t* prop = nullptr;
+    uint64_t* screen = nullptr;
+    double* ratio = nullptr;
+    double* dl = nullptr;
+    double* dc = nullptr;
+    uint8_t* decision = nullptr;
+    uint64_t* chosen = nullptr;
+    uint64_t* temporal = nullptr;
+    uint64_t* sig = nullptr;
+    GlobalState* g = nullptr;
+    int32_t* err = nullptr;
+};
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
+    if (s.mem && s.mbs_cap >= mbs && s.cells_cap >= cells) return hipSuccess;
+    if (s.mem) { (void)hipFree(s.mem); s.mem = nullptr; }
+    mbs = std::max(mbs, s.mbs_cap);
+    cells = std::max(cells, s.cells_cap);
+    size_t off = 0;
+    auto carve = [&](size_t n) { const size_t o = off; off = align_up(off + n, 256); return o; };
+    const size_t o_mvs = carve(mbs * sizeof(int2));
+    const size_t o_bmv = carve(mbs * 4 * sizeof(int2));
+    const size_t o_flags = carve(mbs);
+    const size_t o_prop = carve(mbs);
+    const size_t o_screen = carve(mbs * 8);
+    const size_t o_ratio = carve(mbs * 8);
+    const size_t o_dl = carve(mbs * 8);
+    const size_t o_dc = carve(mbs * 8);
+    const size_t o_dec = carve(mbs);
+    const size_t o_chosen = carve(mbs * 8);
+    const size_t o_temp = carve(mbs * 8);
+    const size_t o_sig = carve(std::max<size_t>(cells, 1) * 8);
+    const size_t o_g = carve(sizeof(GlobalState));
+    const size_t o_err = carve(sizeof(int32_t));
+    hipError_t e = hipMalloc(&s.mem, off);
+    if (e != hipSuccess) { s.mem = nullptr; return e; }
+    uint8_t* b = static_cast<uint8_t*>(s.mem);
+    s.bytes = off;
+    s.mvs = reinterpret_cast<int2*>(b + o_mvs);
+    s.bmv = reinterpret_cast<int2*>(b + o_bmv);
+    s.flags = b + o_flags;
+    s.prop = b + o_prop;
+    s.screen = reinterpret_cast<uint64_t*>(b + o_screen);
+    s.ratio = reinterpret_cast<double*>(b + o_ratio);
+    s.dl = reinterpret_cast<double*>(b + o_dl);
+    s.dc = reinterpret_cast<double*>(b + o_dc);
+    s.decision = b + o_dec;
+    s.chosen = reinterpret_cast<uint64_t*>(b + o_chosen);
+    s.temporal = reinterpret_cast<uint64_t*>(b + o_temp);
+    s.sig = reinterpret_cast<uint64_t*>(b + o_sig);
+    s.g = reinterpret_cast<GlobalState*>(b + o_g);
+    s.err = reinterpret_cast<int32_t*>(b + o_err);
+    s.mbs_cap = mbs;
+    s.cells_cap = cells;
+    return hipSuccess;
+}
+
+struct CachedPlane {
+    const uint8_t* host = nullptr;
+    int w = 0, h = 0;
+    uint64_t sum = 0;
+    uint8_t* dev = nullptr;
+    uint64_t last_use = 0;
+};
+
+}  // namespace
+
+struct vc1me_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string name;
+    vc1me_counters counters{};
+    vc1me_timing timing{};
+    bool profiling = false;
+    std::vector<JobSlot> slots;
+    JobDesc* d_jobs = nullptr;
+    size_t d_jobs_cap = 0;
+    std::vector<JobDesc> h_jobs;
+    // events: 0/1 whole step, 2/3 global search, then pairs per diagonal
+    std::vector<hipEvent_t> events;
+    // score-batch resources
+    CachedPlane planes[8];
+    uint64_t use_clock = 0;
+    int2* d_cands = nullptr;
+    uint32_t* d_costs = nullptr;
+    size_t cand_cap = 0;
+    std::vector<uint32_t> h_costs;
+};
+
+namespace {
+
+int set_device(vc1me_ctx* ctx) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    return VC1ME_OK;
+}
+
+int ensure_events(vc1me_ctx* ctx, size_t n) {
+    while (ctx->events.size() < n) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        ctx->events.push_back(e);
+    }
+    return VC1ME_OK;
+}
+
+// Device copy of a host luma plane for the batch scorer; keyed by pointer,
+// size and a checksum of the whole plane, so in-place edits are never missed.
+int plane_for(vc1me_ctx* ctx, const uint8_t* host, int w, int h, uint8_t** out) {
+    const size_t n = static_cast<size_t>(w) * h;
+    const uint64_t sum = fnv64(host, n);
+    ++ctx->use_clock;
+    for (auto& p : ctx->planes)
+        if (p.dev && p.host == host && p.w == w && p.h == h && p.sum == sum) {
+            p.last_use = ctx->use_clock;
+            *out = p.dev;
+            return VC1ME_OK;
+        }
+    CachedPlane* victim = &ctx->planes[0];
+    for (auto& p : ctx->planes) {
+        if (!p.dev) { victim = &p; break; }
+        if (p.last_use < victim->last_use) victim = &p;
+    }
+    if (victim->dev && (static_cast<size_t>(victim->w) * victim->h < n)) {
+        (void)hipFree(victim->dev);
+        victim->dev = nullptr;
+    }
+    if (!victim->dev) HIP_TRY(hipMalloc(&victim->dev, n));
+    HIP_TRY(hipMemcpyAsync(victim->dev, host, n, hipMemcpyHostToDevice, ctx->stream));
+    victim->host = host; victim->w = w; victim->h = h; victim->sum = sum; victim->last_use = ctx->use_clock;
+    ctx->counters.frame_uploads++;
+    ctx->counters.frame_upload_bytes += n;
+    *out = victim->dev;
+    return VC1ME_OK;
+}
+
+int score_on_device(vc1me_ctx* ctx, const uint8_t* dcur, const uint8_t* dref, int width, int height, int x0, int y0,
+                    int bw, int bh, int sample_stride, int bilinear, int rnd, int metric,
+                    const vc1me_candidate* candidates, size_t n, uint64_t* costs_out) {
+    if (n > ctx->cand_cap) {
+        if (ctx->d_cands) (void)hipFree(ctx->d_cands);
+        if (ctx->d_costs) (void)hipFree(ctx->d_costs);
+        ctx->d_cands = nullptr; ctx->d_costs = nullptr; ctx->cand_cap = 0;
+        const size_t cap = std::max<size_t>(n, 4096);
+        HIP_TRY(hipMalloc(&ctx->d_cands, cap * sizeof(int2)));
+        HIP_TRY(hipMalloc(&ctx->d_costs, cap * sizeof(uint32_t)));
+        ctx->cand_cap = cap;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipMemcpyAsync(ctx->d_cands, candidates, n * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(vc1me::launch_score_batch(dcur, dref, width, height, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
+                                      ctx->d_cands, static_cast<int>(n), ctx->d_costs, ctx->stream));
+    ctx->h_costs.resize(n);
+    HIP_TRY(hipMemcpyAsync(ctx->h_costs.data(), ctx->d_costs, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    for (size_t i = 0; i < n; ++i) costs_out[i] = ctx->h_costs[i];
+    ctx->counters.dispatches++;
+    ctx->counters.candidates += n;
+    ctx->counters.submit_wait_nanoseconds +=
+        static_cast<uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count());
+    return VC1ME_OK;
+}
+
+int validate_score_args(int width, int height, size_t n) {
+    if (n > 64u * 65535u) return fail(VC1ME_ELIMIT, "motion batches above 4,194,240 candidates are not supported");
+    if (static_cast<uint64_t>(width) * static_cast<uint64_t>(height) > static_cast<uint64_t>(INT32_MAX))
+        return fail(VC1ME_ELIMIT, "luma plane exceeds 32-bit addressing");
+    return VC1ME_OK;
+}
+
+int mvrange_of(int r) { return r <= 31 ? 0 : r <= 63 ? 1 : r <= 127 ? 2 : 3; }
+
+int validate_params(const vc1me_params* p, const vc1me_frame* cur, const vc1me_frame* ref) {
+    if (!p || !cur || !ref) return fail(VC1ME_EINVAL, "null argument");
+    if (p->width <= 0 || p->height <= 0) return fail(VC1ME_EINVAL, "raster must be positive");
+    if ((p->width & 1) || (p->height & 1)) return fail(VC1ME_EINVAL, "coded raster must be even (4:2:0)");
+    if (cur->w != p->width || cur->h != p->height || ref->w != p->width || ref->h != p->height)
+        return fail(VC1ME_EINVAL, "P-picture reference dimensions do not match encoder configuration");
+    if (p->pqindex < 1 || p->pqindex > 31) return fail(VC1ME_EINVAL, "PQINDEX must be 1..31");
+    if (p->motion_search_range < 0 || p->motion_search_range > 1024)
+        return fail(VC1ME_EINVAL, "motion search range must be 0..1024 pixels");
+    const int local = std::min(p->motion_search_range, std::max(0, p->motion_local_search_range));
+    if (local > vc1me::kMaxLocalRange)
+        return fail(VC1ME_ELIMIT, "local search range above 64 pixels is not supported by the GPU window");
+    if (p->me_quality != 1 && p->me_quality != 2)
+        return fail(VC1ME_EINVAL, "me_quality must be RATE (1) or SATD (2) on the GPU backend");
+    if (p->me_algorithm != 0 && p->me_algorithm != 1) return fail(VC1ME_EINVAL, "me_algorithm must be UMH or diamond");
+    if (p->mode < 0 || p->mode > 3) return fail(VC1ME_EINVAL, "invalid progressive MV mode");
+    if (p->long_range_search_mode < 0 || p->long_range_search_mode > 2)
+        return fail(VC1ME_EINVAL, "invalid long-range search mode");
+    if (p->subpixel_refinement_range < 0) return fail(VC1ME_EINVAL, "negative sub-pixel refinement range");
+    if (!(p->inter_intra_threshold == p->inter_intra_threshold)) return fail(VC1ME_EINVAL, "NaN intra threshold");
+    return VC1ME_OK;
+}
+
+void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const vc1me_frame* ref, const JobSlot& s) {
+    std::memset(&d, 0, sizeof d);
+    d.cy = cur->y; d.cu = cur->u; d.cv = cur->v;
+    d.ry = ref->y; d.ru = ref->u; d.rv = ref->v;
+    d.W = p->width; d.H = p->height;
+    d.vw = p->display_width > 0 ? p->display_width : p->width;
+    d.vh = p->display_height > 0 ? p->display_height : p->height;
+    d.mbw = (p->width + 15) / 16;
+    d.mbh = (p->height + 15) / 16;
+    d.pq = p->pqindex;
+    d.range = p->motion_search_range;
+    d.local_range = std::min(p->motion_search_range, std::max(0, p->motion_local_search_range));
+    d.algo = p->me_algorithm;
+    d.quality = p->me_quality;
+    d.subpel = p->subpixel_refinement ? 1 : 0;
+    d.fast = p->fast_subpixel_refinement ? 1 : 0;
+    d.subpel_range = p->subpixel_refinement_range;
+    d.lr_mode = p->long_range_search_mode;
+    d.mode = p->mode;
+    d.allow_intra = (p->allow_intra && !p->debug_disable_p_intra) ? 1 : 0;
+    d.mvrange = mvrange_of(p->motion_search_range);
+    // lambda_me = max(1, 0.35*pq) (src/encoder_motion.cpp:3458 and :246)
+    const double l = 0.35 * static_cast<double>(p->pqindex);
+    d.lambda = std::max(1.0, l);
+    int e = 0;
+    const double f = std::frexp(d.lambda, &e);            // lambda = f * 2^e, f in [0.5,1)
+    uint64_t m = static_cast<uint64_t>(std::ldexp(f, 53));  // exact: f has 53 significant bits
+    int k = 53 - e;
+    while (k > 0 && !(m & 1)) { m >>= 1; --k; }
+    d.lam_m = m;
+    d.lam_k = k;
+    d.ncx = (p->width + 7) / 8;
+    d.ncy = (p->height + 7) / 8;
+    d.intra_thr = p->inter_intra_threshold;
+    // distant_match_decision's long-double rejection threshold, per active
+    // pixel count: reject iff (long double)sad > px * max_mae (exact product).
+    for (int px = 0; px <= 256; ++px) {
+        const long double lim = static_cast<long double>(px) * static_cast<long double>(p->distant_match_max_mae);
+        if (!(lim >= 0.0L)) { d.rej_always[px] = 1; d.rej_thr[px] = 0; }
+        else if (lim >= 18446744073709551615.0L) { d.rej_always[px] = 0; d.rej_thr[px] = UINT64_MAX; }
+        else { d.rej_always[px] = 0; d.rej_thr[px] = static_cast<uint64_t>(floorl(lim)); }
+    }
+    d.mvs = s.mvs; d.bmv = s.bmv; d.flags = s.flags; d.prop = s.prop;
+    d.screen = s.screen; d.ratio = s.ratio; d.dl_mae = s.dl; d.dc_mae = s.dc; d.decision = s.decision;
+    d.chosen = s.chosen; d.temporal = s.temporal;
+    d.sig = s.sig;
+    d.g = s.g;
+    d.err = s.err;
+    d.max_diag = d.mbw + 2 * d.mbh - 3;
+}
+
+int mbs_on_diag(int mbw, int mbh, int t) {
+    const int lo = std::max(0, (t - (mbw - 1) + 1) / 2);
+    const int hi = std::min(mbh - 1, t / 2);
+    return std::max(0, hi - lo + 1);
+}
+
+// Copy one job's per-MB arrays back and form the frame scalars
+// (src/encoder_motion.cpp:416-441).
+int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_result* out) {
+    const size_t mbs = static_cast<size_t>(d.mbw) * d.mbh;
+    std::vector<int2> mvs(mbs), bmv(mbs * 4);
+    std::vector<uint8_t> flags(mbs), dec(mbs);
+    std::vector<uint64_t> screen(mbs), chosen(mbs), temporal(mbs);
+    std::vector<double> ratio(mbs), dl(mbs), dc(mbs);
+    GlobalState g;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(mvs.data(), s.mvs, mbs * sizeof(int2), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(bmv.data(), s.bmv, mbs * 4 * sizeof(int2), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(flags.data(), s.flags, mbs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(dec.data(), s.decision, mbs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(screen.data(), s.screen, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(chosen.data(), s.chosen, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(temporal.data(), s.temporal, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ratio.data(), s.ratio, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(dl.data(), s.dl, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(dc.data(), s.dc, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&g, s.g, sizeof g, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    uint64_t total_sad = 0, rate_total = 0, total_px = 0;
+    uint64_t moved = 0, frac = 0, four = 0, intra = 0;
+    for (size_t i = 0; i < mbs; ++i) {
+        const int mx = static_cast<int>(i % d.mbw), my = static_cast<int>(i / d.mbw);
+        const uint64_t px = static_cast<uint64_t>(std::max(0, std::min(16, d.W - mx * 16))) *
+                            static_cast<uint64_t>(std::max(0, std::min(16, d.H - my * 16)));
+        total_px += px;
+        total_sad += chosen[i];
+        rate_total += temporal[i];
+        if (flags[i] & vc1me::kFlag4mvChosen) ++four;
+        if (flags[i] & vc1me::kFlagIntra) ++intra;
+        bool mv_moved = false, mv_frac = false;
+        for (int k = 0; k < 4; ++k) {
+            const int2 v = bmv[i * 4 + k];
+            mv_moved |= v.x != 0 || v.y != 0;
+            mv_frac |= (v.x % 8) != 0 || (v.y % 8) != 0;
+        }
+        moved += mv_moved; frac += mv_frac;
+        if (out->mvs) { out->mvs[2 * i] = mvs[i].x; out->mvs[2 * i + 1] = mvs[i].y; }
+        if (out->block_mvs)
+            for (int k = 0; k < 4; ++k) {
+                out->block_mvs[8 * i + 2 * k] = bmv[i * 4 + k].x;
+                out->block_mvs[8 * i + 2 * k + 1] = bmv[i * 4 + k].y;
+            }
+        if (out->use_4mv) out->use_4mv[i] = (flags[i] & vc1me::kFlag4mvChosen) && !(flags[i] & vc1me::kFlagIntra);
+        if (out->use_intra) out->use_intra[i] = (flags[i] & vc1me::kFlagIntra) ? 1 : 0;
+        if (out->screen_sads) out->screen_sads[i] = screen[i];
+        if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = ratio[i];
+        if (out->distant_local_mae) out->distant_local_mae[i] = dl[i];
+        if (out->distant_candidate_mae) out->distant_candidate_mae[i] = dc[i];
+        if (out->distant_match_decision) out->distant_match_decision[i] = dec[i];
+    }
+    if (d.range == 0) {
+        out->mean_abs_residual = total_px ? static_cast<double>(total_sad) / static_cast<double>(total_px) : 0.0;
+        out->rate_complexity = out->mean_abs_residual;
+        out->motion_sad = 0;
+        out->moved_macroblocks = out->fractional_chroma_macroblocks = out->four_mv_macroblocks = out->intra_macroblocks = 0;
+        out->global_dx = 0; out->global_dy = 0; out->global_sad = 0;
+        return VC1ME_OK;
+    }
+    const double local_mad = total_px ? static_cast<double>(total_sad) / static_cast<double>(total_px) : 0.0;
+    const double global_mad = static_cast<double>(g.full_sad) / static_cast<double>(static_cast<uint64_t>(d.W) * d.H);
+    out->mean_abs_residual = std::min(local_mad, global_mad);
+    const double rate_local = total_px ? static_cast<double>(rate_total) / static_cast<double>(total_px) : 0.0;
+    out->rate_complexity = std::min(rate_local, global_mad);
+    out->motion_sad = total_sad;
+    out->moved_macroblocks = moved;
+    out->fractional_chroma_macroblocks = frac;
+    out->four_mv_macroblocks = four;
+    out->intra_macroblocks = intra;
+    out->global_dx = g.dx;
+    out->global_dy = g.dy;
+    out->global_sad = g.full_sad;
+    return VC1ME_OK;
+}
+
+int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
+    if (n <= 0) return VC1ME_OK;
+    if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
+    for (int i = 0; i < n; ++i) {
+        const int rc = validate_params(jobs[i].params, jobs[i].cur, jobs[i].ref);
+        if (rc) return rc;
+    }
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    if (ctx->slots.size() < static_cast<size_t>(n)) ctx->slots.resize(n);
+    ctx->h_jobs.resize(n);
+    int max_local = 0, max_diag = 0;
+    size_t max_cells = 0;
+    for (int i = 0; i < n; ++i) {
+        const vc1me_params* p = jobs[i].params;
+        const size_t mbs = static_cast<size_t>((p->width + 15) / 16) * ((p->height + 15) / 16);
+        const size_t cells = static_cast<size_t>((p->width + 7) / 8) * ((p->height + 7) / 8);
+        hipError_t e = ensure_slot(ctx->slots[i], mbs, cells);
+        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("device state allocation: ") + hipGetErrorString(e));
+        fill_job(ctx->h_jobs[i], p, jobs[i].cur, jobs[i].ref, ctx->slots[i]);
+        max_local = std::max(max_local, ctx->h_jobs[i].local_range);
+        max_diag = std::max(max_diag, ctx->h_jobs[i].max_diag);
+        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range) max_cells = std::max(max_cells, cells);
+    }
+    if (ctx->d_jobs_cap < static_cast<size_t>(n)) {
+        if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
+        ctx->d_jobs = nullptr;
+        HIP_TRY(hipMalloc(&ctx->d_jobs, sizeof(JobDesc) * n));
+        ctx->d_jobs_cap = n;
+    }
+    hipStream_t st = ctx->stream;
+    if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1))) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs.data(), sizeof(JobDesc) * n, hipMemcpyHostToDevice, st));
+    for (int i = 0; i < n; ++i) HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
+    HIP_TRY(hipEventRecord(ctx->events[0], st));
+    HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st, ctx->events[2], ctx->events[3]));
+    HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
+    const int lds = vc1me::mb_kernel_lds_bytes(max_local);
+    int launches = 0;
+    for (int t = 0; t <= max_diag; ++t) {
+        int per = 0;
+        for (int i = 0; i < n; ++i)
+            if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
+        if (!per) continue;
+        if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[4 + 2 * t], st));
+        HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
+        if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[5 + 2 * t], st));
+        ++launches;
+    }
+    HIP_TRY(hipEventRecord(ctx->events[1], st));
+    HIP_TRY(hipEventSynchronize(ctx->events[1]));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+    ctx->timing.step_ms = ms;
+    float gms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&gms, ctx->events[2], ctx->events[3]));
+    ctx->timing.global_kernel_ms_total = gms;
+    ctx->timing.global_kernel_launches = 9;
+    double mbms = 0.0;
+    if (ctx->profiling) {
+        for (int t = 0; t <= max_diag; ++t) {
+            int per = 0;
+            for (int i = 0; i < n; ++i)
+                if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
+            if (!per) continue;
+            float x = 0.f;
+            HIP_TRY(hipEventElapsedTime(&x, ctx->events[4 + 2 * t], ctx->events[5 + 2 * t]));
+            mbms += x;
+        }
+    }
+    ctx->timing.mb_kernel_ms_total = mbms;
+    ctx->timing.mb_kernel_launches = static_cast<uint64_t>(launches);
+    // kernel-side error flags
+    for (int i = 0; i < n; ++i) {
+        int32_t err = 0;
+        HIP_TRY(hipMemcpy(&err, ctx->slots[i].err, sizeof err, hipMemcpyDeviceToHost));
+        if (err) return fail(VC1ME_EKERNEL, "kernel reported internal error code " + std::to_string(err));
+    }
+    for (int i = 0; i < n; ++i) {
+        ctx->counters.frames_analyzed++;
+        ctx->counters.macroblocks_analyzed += static_cast<uint64_t>(ctx->h_jobs[i].mbw) * ctx->h_jobs[i].mbh;
+        if (jobs[i].out) {
+            const int rc = fetch_result(ctx, ctx->h_jobs[i], ctx->slots[i], jobs[i].out);
+            if (rc) return rc;
+        }
+    }
+    return VC1ME_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void vc1me_params_default(vc1me_params* p, int32_t width, int32_t height) {
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->width = width;
+    p->height = height;
+    p->pqindex = 9;
+    p->motion_search_range = 1024;
+    p->motion_local_search_range = 32;
+    p->me_algorithm = 0;
+    p->me_quality = 2;
+    p->subpixel_refinement = 1;
+    p->fast_subpixel_refinement = 0;
+    p->subpixel_refinement_range = 1024;
+    p->long_range_search_mode = 0;
+    p->mode = VC1ME_MODE_ONE_MV_QPEL;
+    p->allow_intra = 1;
+    p->debug_disable_p_intra = 0;
+    p->distant_match_max_mae = 255.0;
+    p->inter_intra_threshold = 0.80;
+}
+
+int vc1me_abi_version(void) { return VC1ME_ABI_VERSION; }
+
+const char* vc1me_last_error(void) { return g_last_error.c_str(); }
+
+int vc1me_create(int device, vc1me_ctx** out) {
+    if (!out) return fail(VC1ME_EINVAL, "null output pointer");
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(VC1ME_EDEVICE, "no HIP device available");
+    if (device < -1 || device >= count) return fail(VC1ME_EINVAL, "device ordinal out of range");
+    auto* ctx = new vc1me_ctx();
+    ctx->device = device < 0 ? 0 : device;
+    if (int rc = set_device(ctx)) { delete ctx; return rc; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete ctx; return fail(VC1ME_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    *out = ctx;
+    return VC1ME_OK;
+}
+
+void vc1me_destroy(vc1me_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& s : ctx->slots)
+        if (s.mem) (void)hipFree(s.mem);
+    for (auto& p : ctx->planes)
+        if (p.dev) (void)hipFree(p.dev);
+    if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
+    if (ctx->d_cands) (void)hipFree(ctx->d_cands);
+    if (ctx->d_costs) (void)hipFree(ctx->d_costs);
+    for (auto e : ctx->events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* vc1me_device_name(const vc1me_ctx* ctx) { return ctx ? ctx->name.c_str() : ""; }
+
+int vc1me_score_batch(vc1me_ctx* ctx, const uint8_t* cur, const uint8_t* ref, int width, int height, int x0, int y0,
+                      int bw, int bh, int sample_stride, int bilinear, int rnd, int metric,
+                      const vc1me_candidate* candidates, size_t n, uint64_t* costs_out) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (!cur || !ref || width <= 0 || height <= 0 || bw <= 0 || bh <= 0 || !candidates || !n) return VC1ME_OK;
+    if (!costs_out) return fail(VC1ME_EINVAL, "null costs");
+    if (int rc = validate_score_args(width, height, n)) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    uint8_t *dc = nullptr, *dr = nullptr;
+    if (int rc = plane_for(ctx, cur, width, height, &dc)) return rc;
+    if (int rc = plane_for(ctx, ref, width, height, &dr)) return rc;
+    return score_on_device(ctx, dc, dr, width, height, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
+                           candidates, n, costs_out);
+}
+
+int vc1me_frame_upload(vc1me_ctx* ctx, const vc1me_planes* planes, vc1me_frame** out) {
+    if (!ctx || !planes || !out || !planes->y) return fail(VC1ME_EINVAL, "null argument");
+    if (planes->width <= 0 || planes->height <= 0 || (planes->width & 1) || (planes->height & 1))
+        return fail(VC1ME_EINVAL, "frame raster must be positive and even");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const size_t ys = static_cast<size_t>(planes->width) * planes->height;
+    const size_t cs = static_cast<size_t>(planes->width / 2) * (planes->height / 2);
+    auto* f = new vc1me_frame();
+    f->w = planes->width;
+    f->h = planes->height;
+    uint8_t* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, ys + 2 * cs + 64);
+    if (e != hipSuccess) { delete f; return fail(VC1ME_ENOMEM, std::string("frame allocation: ") + hipGetErrorString(e)); }
+    f->y = mem;
+    f->u = mem + ys;
+    f->v = mem + ys + cs;
+    std::vector<uint8_t> grey;
+    const uint8_t* u = planes->u;
+    const uint8_t* v = planes->v;
+    if (!u || !v) { grey.assign(cs, 128); if (!u) u = grey.data(); if (!v) v = grey.data(); }
+    e = hipMemcpy(f->y, planes->y, ys, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(f->u, u, cs, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(f->v, v, cs, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(mem); delete f; return fail(VC1ME_EDEVICE, std::string("frame upload: ") + hipGetErrorString(e)); }
+    ctx->counters.frame_uploads++;
+    ctx->counters.frame_upload_bytes += ys + 2 * cs;
+    *out = f;
+    return VC1ME_OK;
+}
+
+void vc1me_frame_release(vc1me_ctx* ctx, vc1me_frame* frame) {
+    if (!frame) return;
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    (void)hipFree(frame->y);
+    delete frame;
+}
+
+int vc1me_score_batch_frames(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, int x0, int y0, int bw,
+                             int bh, int sample_stride, int bilinear, int rnd, int metric,
+                             const vc1me_candidate* candidates, size_t n, uint64_t* costs_out) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (!cur || !ref || bw <= 0 || bh <= 0 || !candidates || !n) return VC1ME_OK;
+    if (cur->w != ref->w || cur->h != ref->h) return fail(VC1ME_EINVAL, "frame size mismatch");
+    if (!costs_out) return fail(VC1ME_EINVAL, "null costs");
+    if (int rc = validate_score_args(cur->w, cur->h, n)) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    return score_on_device(ctx, cur->y, ref->y, cur->w, cur->h, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
+                           candidates, n, costs_out);
+}
+
+int vc1me_analyze_p_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const vc1me_params* params,
+                          vc1me_p_result* out) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    vc1me_job j{cur, ref, params, out};
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return run_jobs(ctx, &j, 1);
+}
+
+int vc1me_analyze_step(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return run_jobs(ctx, jobs, n);
+}
+
+int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out) {
+    if (!ctx || !out) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = ctx->timing;
+    return VC1ME_OK;
+}
+
+int vc1me_set_profiling(vc1me_ctx* ctx, int on) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->profiling = on != 0;
+    return VC1ME_OK;
+}
+
+int vc1me_counters_get(vc1me_ctx* ctx, vc1me_counters* out) {
+    if (!ctx || !out) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = ctx->counters;
+    return VC1ME_OK;
+}
+
+void vc1me_counters_reset(vc1me_ctx* ctx) {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->counters = vc1me_counters{};
+}
+
+}  // extern "C"

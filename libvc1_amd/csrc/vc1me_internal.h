@@ -1,0 +1,72 @@
+// vc1me_internal.h — types shared by the HIP kernels and the host layer of
+// the MI355X motion-estimation backend.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vc1me {
+
+constexpr uint64_t kBadSad = UINT64_MAX / 4;   // hidden-padding veto cost (src/encoder_motion.cpp:2201-2203)
+constexpr int kWave = 64;
+constexpr int kMaxLocalRange = 64;             // LDS window bound of the integer search
+constexpr int kGlobalMaxPts = 512;
+
+// Frame-level global translation search state (src/encoder_motion.cpp:2025-2141).
+struct GlobalState {
+    int32_t n;                  // points of the current stage
+    int32_t stride;             // frame-SAD sample stride of the stage
+    int32_t pts[2 * kGlobalMaxPts];
+    uint64_t cost[kGlobalMaxPts];
+    // sorted candidate list of the extended search ("top", <= 8 entries)
+    int32_t tn;
+    uint64_t tcost[9];
+    int32_t tdx[9], tdy[9];
+    // result
+    int32_t dx, dy;
+    uint64_t full_sad;
+    int32_t pad;
+};
+
+// One (cur, ref) pair of the batch with its parameters and device state.
+struct JobDesc {
+    const uint8_t* cy; const uint8_t* cu; const uint8_t* cv;
+    const uint8_t* ry; const uint8_t* ru; const uint8_t* rv;
+    int32_t W, H, vw, vh, mbw, mbh;
+    int32_t pq, range, local_range, algo, quality, subpel, fast, subpel_range, lr_mode, mode, allow_intra;
+    int32_t mvrange;
+    double lambda;              // max(1, 0.35*pq) as the reference computes it
+    uint64_t lam_m;             // lambda == lam_m * 2^-lam_k exactly
+    int32_t lam_k;
+    int32_t ncx, ncy;           // long-range index lattice
+    double intra_thr;
+    // distant_match_max_mae threshold per active pixel count: reject iff
+    // sad > rej_thr[px] (rej_always[px] => always reject)
+    uint64_t rej_thr[257];
+    uint8_t rej_always[257];
+    // state / outputs (device), one entry per macroblock
+    int2* mvs; int2* bmv; uint8_t* flags; uint8_t* prop;
+    uint64_t* screen; double* ratio; double* dl_mae; double* dc_mae; uint8_t* decision;
+    uint64_t* chosen; uint64_t* temporal;
+    uint64_t* sig;              // long-range signatures (ncx*ncy)
+    GlobalState* g;
+    int32_t* err;
+    int32_t max_diag;           // mbw + 2*mbh - 2
+};
+
+// Flag bits of JobDesc::flags
+constexpr uint8_t kFlag4mvChosen = 1;
+constexpr uint8_t kFlagIntra = 2;
+
+// Kernel launchers (vc1me_kernels.hip).
+hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc* h_jobs, hipStream_t s,
+                                hipEvent_t ev0, hipEvent_t ev1);
+hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hipStream_t s);
+hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
+                              hipStream_t s);
+int mb_kernel_lds_bytes(int local_range);
+hipError_t launch_score_batch(const uint8_t* cur, const uint8_t* ref, int width, int height, int x0, int y0,
+                              int bw, int bh, int sample_stride, int bilinear, int rnd, int metric,
+                              const int2* cands, int n, uint32_t* costs, hipStream_t s);
+
+}  // namespace vc1me

@@ -1,0 +1,2163 @@
+// vc1me_kernels.hip — gfx950 kernels of the libvc1 motion-estimation backend.
+//
+// Frame level:
+//   k_frame_sads / k_global_*   global translation search
+//                               (Vc1Encoder::global_motion_search, src/encoder_motion.cpp:2025-2141)
+//   k_signatures                long-range content index
+//                               (build_long_range_index, src/encoder_motion.cpp:1615-1625)
+// Macroblock level:
+//   k_mb_diagonal               one 64-lane wave per macroblock; macroblocks of one
+//                               anti-diagonal t = mx + 2*my run concurrently, so the
+//                               left / top / top-left / top-right dependencies of
+//                               analyze_p_picture_core (src/encoder_motion.cpp:113-428)
+//                               are always final.  Candidates of every search stage are
+//                               scored lane-parallel from a reference window staged in
+//                               LDS; stage results merge through wave-wide minimum
+//                               reductions of packed (cost, |mv|, y, x) keys, which is the
+//                               reference's strict total order.
+// Batch scorer:
+//   k_score_batch               VulkanComputeContext::score_motion_batch semantics
+//                               (src/shaders/vulkan_motion_cost.comp).
+//
+// Results are bit-exact with the reference CPU path: every cost is an exact
+// integer, the x87 long-double finalist scores are reproduced by exact 128-bit
+// rounding to a 64-bit significand, and the double decisions are evaluated in
+// IEEE double without contraction.
+#include "vc1me_internal.h"
+
+namespace vc1me {
+
+namespace {
+
+constexpr uint64_t kNone = ~0ull;
+
+// ------------------------------------------------------------------ small helpers
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int floor_div4(int v) { return v >> 2; }  // arithmetic shift == floor division by 4
+__device__ __forceinline__ int arshift(int v, int n) { return v >> n; }  // floor shift (src/encoder_entropy.cpp:377-381)
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const uint32_t lo = __shfl_xor(static_cast<unsigned>(v), m);
+    const uint32_t hi = __shfl_xor(static_cast<unsigned>(v >> 32), m);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        const uint64_t o = shfl_xor64(v, m);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+__device__ __forceinline__ uint32_t group_sum32(uint32_t v, int G) {
+    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m);
+    return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v |= __shfl_xor(v, m);
+    return v;
+}
+
+// llround without library dependence: round half away from zero; x - trunc(x) is exact.
+__device__ __forceinline__ long long llround_exact(double x) {
+    const double t = __builtin_trunc(x);
+    const double f = x - t;
+    long long r = static_cast<long long>(t);
+    if (f >= 0.5) r += 1;
+    else if (f <= -0.5) r -= 1;
+    return r;
+}
+
+// Packed keys of the strict total order (cost, |dx|+|dy|, dy, dx) used by every
+// integer search (src/encoder_motion.cpp:1788-1794).  Costs are < 2^20; the
+// hidden-padding veto (2^62-1) maps to the largest 21-bit value.
+__device__ __forceinline__ uint64_t cost21(uint64_t c) { return c >= (1u << 21) - 1 ? (1u << 21) - 1 : c; }
+__device__ __forceinline__ uint64_t key_int(uint64_t cost, int dx, int dy) {
+    return (cost21(cost) << 34) | (static_cast<uint64_t>(iabs(dx) + iabs(dy)) << 22) |
+           (static_cast<uint64_t>(dy + 1024) << 11) | static_cast<uint64_t>(dx + 1024);
+}
+__device__ __forceinline__ int key_dx(uint64_t k) { return static_cast<int>(k & 2047) - 1024; }
+__device__ __forceinline__ int key_dy(uint64_t k) { return static_cast<int>((k >> 11) & 2047) - 1024; }
+__device__ __forceinline__ uint64_t key_cost(uint64_t k) {
+    const uint64_t c = k >> 34;
+    return c == (1u << 21) - 1 ? kBadSad : c;
+}
+// (cost, |dx|+|dy|, sequence) for the first-found-wins loops of the long-range searches
+__device__ __forceinline__ uint64_t key_seq(uint64_t cost, int man, int seq) {
+    return (cost21(cost) << 34) | (static_cast<uint64_t>(man) << 16) | static_cast<uint64_t>(seq);
+}
+
+// quarter-pel tie order (|x|+|y|, y, x) of the sub-pel stages (src/encoder_motion.cpp:3320-3325)
+__device__ __forceinline__ bool tie_less(int ax, int ay, int bx, int by) {
+    const int am = iabs(ax) + iabs(ay), bm = iabs(bx) + iabs(by);
+    if (am != bm) return am < bm;
+    if (ay != by) return ay < by;
+    return ax < bx;
+}
+
+__device__ __forceinline__ bool mode_halfpel(int m) { return m == 1 || m == 2; }
+__device__ __forceinline__ bool mode_bilinear(int m) { return m == 2; }
+
+// MVDATA code lengths of MV table 0 (ST 421; src/vc1_groupa_tables.h:26-27 row 0)
+__constant__ uint8_t kMvBits0[73] = {
+    6, 7, 7, 8, 14, 6, 5, 6, 7, 14, 14, 6, 6, 6, 8, 9, 10, 9, 7, 7, 7, 14, 14, 10, 9,
+    10, 8, 10, 10, 14, 14, 14, 13, 13, 13, 6, 3, 5, 6, 8, 9, 13, 5, 4, 4, 5, 7, 9, 6, 5,
+    5, 5, 6, 9, 8, 5, 5, 5, 7, 10, 7, 7, 7, 7, 7, 8, 10, 9, 8, 9, 9, 9, 3};
+
+// ------------------------------------------------------------------ hidden padding
+// motion_uses_hidden_luma / _chroma (src/encoder_motion.cpp:2143-2189)
+
+struct Geo {
+    int W, H, vw, vh, mbw, mbh;
+};
+
+__device__ bool hidden_luma(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq, int mode) {
+    const bool rp = g.vw < g.mbw * 16, bp = g.vh < g.mbh * 16;
+    if (!rp && !bp) return false;
+    const int aw = imax(0, imin(bw, g.vw - x0)), ah = imax(0, imin(bh, g.vh - y0));
+    if (!aw || !ah) return false;
+    const int ix = floor_div4(xq), iy = floor_div4(yq);
+    const int fx = xq - ix * 4, fy = yq - iy * 4;
+    const int right = fx ? (mode_bilinear(mode) ? 1 : 2) : 0;
+    const int bottom = fy ? (mode_bilinear(mode) ? 1 : 2) : 0;
+    return (rp && x0 + aw - 1 + ix + right >= g.vw) || (bp && y0 + ah - 1 + iy + bottom >= g.vh);
+}
+__device__ bool hidden_chroma(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq) {
+    const int cw = (g.vw + 1) / 2, ch = (g.vh + 1) / 2;
+    const bool rp = cw < g.mbw * 8, bp = ch < g.mbh * 8;
+    if (!rp && !bp) return false;
+    const int cx = x0 / 2, cy = y0 / 2;
+    const int aw = imax(0, imin(bw / 2, cw - cx)), ah = imax(0, imin(bh / 2, ch - cy));
+    if (!aw || !ah) return false;
+    int uvx = arshift(xq + ((xq & 3) == 3), 1), uvy = arshift(yq + ((yq & 3) == 3), 1);
+    uvx = uvx < 0 ? uvx + (uvx & 1) : uvx - (uvx & 1);
+    uvy = uvy < 0 ? uvy + (uvy & 1) : uvy - (uvy & 1);
+    const int ix = floor_div4(uvx), iy = floor_div4(uvy);
+    const bool fx = (uvx - ix * 4) != 0, fy = (uvy - iy * 4) != 0;
+    return (rp && cx + aw - 1 + ix + (fx ? 1 : 0) >= cw) || (bp && cy + ah - 1 + iy + (fy ? 1 : 0) >= ch);
+}
+__device__ __forceinline__ bool hidden_padding(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq,
+                                               int mode, bool chroma) {
+    return hidden_luma(g, x0, y0, bw, bh, xq, yq, mode) || (chroma && hidden_chroma(g, x0, y0, bw, bh, xq, yq));
+}
+__device__ __forceinline__ bool chroma_block(int x0, int y0, int bw, int bh) {
+    return bw == 16 && bh == 16 && !(x0 & 15) && !(y0 & 15);
+}
+
+// ------------------------------------------------------------------ x87 score emulation
+// score = (long double)cost + (long double)lambda * rate, rounded once to a
+// 64-bit significand (round to nearest even).  lambda = m * 2^-k exactly, so
+// the exact sum is N * 2^-k with N = cost*2^k + m*rate; all scores of one MB
+// share k and compare as rounded N.
+__device__ unsigned __int128 x87_score(uint64_t cost, uint64_t rate, uint64_t m, int k) {
+    unsigned __int128 n = (static_cast<unsigned __int128>(cost) << k) + static_cast<unsigned __int128>(m) * rate;
+    const uint64_t hi = static_cast<uint64_t>(n >> 64);
+    if (!hi) return n;
+    const int bits = 128 - __builtin_clzll(hi);  // bit length of n
+    const int sh = bits - 64;
+    unsigned __int128 q = n >> sh;
+    const unsigned __int128 rem = n - (q << sh);
+    const unsigned __int128 half = static_cast<unsigned __int128>(1) << (sh - 1);
+    if (rem > half || (rem == half && (q & 1))) q += 1;
+    return q << sh;  // q may have become 2^64: still the correctly rounded value
+}
+
+// ------------------------------------------------------------------ LDS layout
+
+struct SubCand {
+    int32_t xq, yq;
+    uint64_t sad;
+    uint64_t satd;
+    uint32_t rate;
+    uint32_t eligible;
+    unsigned __int128 score;
+};
+
+constexpr int kSeenWords = ((2 * kMaxLocalRange + 1) * (2 * kMaxLocalRange + 1) + 31) / 32;
+constexpr int kMaxSub = 32;
+constexpr int kSmallWin = 64 * 68;  // far windows and sub-pel windows
+
+__host__ __device__ inline int win_capacity(int local_range) {
+    const int side = 16 + 2 * local_range;
+    const int pitch = (side + 7) & ~3;
+    const int need = side * pitch;
+    return need > kSmallWin ? need : kSmallWin;
+}
+
+struct Lds {
+    uint8_t* cur;        // 16x16 current block, zero outside the active area
+    uint8_t* win;        // reference window
+    uint32_t* seen_s;    // sampled-SAD dedupe bitset of the UMH search
+    uint32_t* seen_f;    // full-SAD dedupe bitset
+    SubCand* sub;        // staged sub-pel candidate list
+    int2* req;           // pending candidate vectors
+    uint64_t* reqcost;   // their costs
+    int32_t* tbuf;       // SATD transpose buffer (64 lanes x 4)
+    uint8_t* pred;       // 16x16 prediction
+    uint64_t* top16;     // long-range top-16 keys
+    int2* own;           // current MB's four block vectors
+};
+
+__host__ __device__ inline int lds_bytes_for(int local_range) {
+    int off = 0;
+    off += 256;                                    // cur
+    off += (win_capacity(local_range) + 15) & ~15; // win
+    off += kSeenWords * 4 * 2;                     // seen_s, seen_f
+    off = (off + 15) & ~15;
+    off += kMaxSub * static_cast<int>(sizeof(SubCand));
+    off += kMaxSub * 8;                            // req
+    off += kMaxSub * 8;                            // reqcost
+    off += 256 * 4;                                // tbuf
+    off += 256;                                    // pred
+    off += 16 * 8;                                 // top16
+    off += 4 * 8;                                  // own
+    return off + 64;
+}
+
+__device__ Lds carve(uint8_t* smem, int local_range) {
+    Lds L;
+    int off = 0;
+    L.cur = smem + off; off += 256;
+    L.win = smem + off; off += (win_capacity(local_range) + 15) & ~15;
+    L.seen_s = reinterpret_cast<uint32_t*>(smem + off); off += kSeenWords * 4;
+    L.seen_f = reinterpret_cast<uint32_t*>(smem + off); off += kSeenWords * 4;
+    off = (off + 15) & ~15;
+    L.sub = reinterpret_cast<SubCand*>(smem + off); off += kMaxSub * static_cast<int>(sizeof(SubCand));
+    L.req = reinterpret_cast<int2*>(smem + off); off += kMaxSub * 8;
+    L.reqcost = reinterpret_cast<uint64_t*>(smem + off); off += kMaxSub * 8;
+    L.tbuf = reinterpret_cast<int32_t*>(smem + off); off += 256 * 4;
+    L.pred = smem + off; off += 256;
+    L.top16 = reinterpret_cast<uint64_t*>(smem + off); off += 16 * 8;
+    L.own = reinterpret_cast<int2*>(smem + off); off += 4 * 8;
+    return L;
+}
+
+// ------------------------------------------------------------------ per-MB context
+
+struct Ctx {
+    const JobDesc* J;
+    Geo g;
+    const uint8_t* ry;
+    int mx, my, x0, y0, aw, ah, pos, lane;
+    Lds L;
+    int winCap;
+    // current window
+    int wx0, wy0, ww, wh, wp;
+};
+
+__device__ __forceinline__ void wsync() { __syncthreads(); }
+
+__device__ void set_err(Ctx& C, int code) {
+    if (C.lane == 0) atomicMax(C.J->err, code);
+}
+
+// Stage a clamped reference window in LDS (sample_clamped semantics,
+// src/encoder_motion_compensation.cpp:350-353).  Returns false if it does
+// not fit the buffer.
+__device__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int wh) {
+    const int wp = (ww + 7) & ~3;
+    if (wp * wh > C.winCap || ww <= 0 || wh <= 0) return false;
+    wsync();
+    const int W = C.g.W, H = C.g.H;
+    const uint8_t* ry = C.ry;
+    if (wx0 >= 0 && wx0 + wp <= W && !(W & 3) && !(wx0 & 3)) {
+        const int dpr = wp >> 2;
+        const int total = dpr * wh;
+        uint32_t* w32 = reinterpret_cast<uint32_t*>(C.L.win);
+        for (int i = C.lane; i < total; i += kWave) {
+            const int r = i / dpr, d = i - r * dpr;
+            const int sy = clampi(wy0 + r, 0, H - 1);
+            w32[i] = *reinterpret_cast<const uint32_t*>(ry + static_cast<size_t>(sy) * W + wx0 + d * 4);
+        }
+    } else {
+        for (int r = 0; r < wh; ++r) {
+            const int sy = clampi(wy0 + r, 0, H - 1);
+            const uint8_t* row = ry + static_cast<size_t>(sy) * W;
+            for (int x = C.lane; x < wp; x += kWave) C.L.win[r * wp + x] = row[clampi(wx0 + x, 0, W - 1)];
+        }
+    }
+    C.wx0 = wx0; C.wy0 = wy0; C.ww = ww; C.wh = wh; C.wp = wp;
+    wsync();
+    return true;
+}
+
+// ------------------------------------------------------------------ integer SAD from the window
+
+// Partial SAD of sampled rows part, part+G, ... of the 16x16 block at integer
+// offset (dx,dy).  Rows at or beyond the active height and columns at or
+// beyond the active width contribute nothing (mb_sad, src/encoder_motion.cpp:2274-2282).
+__device__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, int stride, int part, int G) {
+    const int ox = C.x0 + dx - C.wx0, oy = C.y0 + dy - C.wy0;
+    const int nrows = (C.ah + stride - 1) / stride;
+    const uint32_t smask = stride == 1 ? 0xffffffffu : (stride == 2 ? 0x00ff00ffu : 0x000000ffu);
+    uint32_t m[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t cm = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (j * 4 + b < C.aw) cm |= 0xffu << (8 * b);
+        m[j] = cm & smask;
+    }
+    if (stride == 4) { m[0] &= 0xff; m[1] &= 0xff; m[2] &= 0xff; m[3] &= 0xff; }
+    const int base = ox & ~3, sh = ox & 3;
+    uint32_t acc = 0;
+    for (int r = part; r < nrows; r += G) {
+        const int y = r * stride;
+        const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + y) * C.wp + base);
+        const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + y * 16);
+        const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
+        const uint32_t r0 = __builtin_amdgcn_alignbyte(d1, d0, sh);
+        const uint32_t r1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
+        const uint32_t r2 = __builtin_amdgcn_alignbyte(d3, d2, sh);
+        const uint32_t r3 = __builtin_amdgcn_alignbyte(d4, d3, sh);
+        acc = __builtin_amdgcn_sad_u8(r0 & m[0], cr[0] & m[0], acc);
+        acc = __builtin_amdgcn_sad_u8(r1 & m[1], cr[1] & m[1], acc);
+        acc = __builtin_amdgcn_sad_u8(r2 & m[2], cr[2] & m[2], acc);
+        acc = __builtin_amdgcn_sad_u8(r3 & m[3], cr[3] & m[3], acc);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ bool in_window(const Ctx& C, int dx, int dy) {
+    const int ox = C.x0 + dx - C.wx0, oy = C.y0 + dy - C.wy0;
+    return ox >= 0 && oy >= 0 && ox + 16 <= C.ww && oy + 16 <= C.wh;
+}
+
+// Sorted list of the eight best (cost, |mv|, y, x) keys seen so far: the
+// top-8 list of integer_motion_umh (src/encoder_motion.cpp:1802-1838).  Keys
+// are unique, so "the eight smallest of all considered points" is exact.
+struct Top8 {
+    uint64_t k[8];
+    int n;
+};
+
+__device__ void top_merge(Top8& t, uint64_t key) {
+    for (int it = 0; it < 8; ++it) {
+        const uint64_t thr = t.n < 8 ? kNone : t.k[7];
+        const uint64_t mk = wave_min64(key < thr ? key : kNone);
+        if (mk == kNone) break;
+        int pos = 0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) pos += (j < t.n && t.k[j] < mk) ? 1 : 0;
+#pragma unroll
+        for (int j = 7; j > 0; --j)
+            if (j > pos) t.k[j] = t.k[j - 1];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j == pos) t.k[j] = mk;
+        t.n = imin(t.n + 1, 8);
+        if (key == mk) key = kNone;
+    }
+}
+
+enum { KEY_TOTAL = 0, KEY_SEQ_MAN = 1, KEY_SEQ = 2 };
+
+// Score n integer candidates (pt(i) -> (dx,dy), already clamped) at the given
+// sample stride from the current window.  seen: dedupe bitset or null.
+// Returns the wave-minimum key; optionally merges all keys into top.
+template <class PtFn>
+__device__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int keymode, int seq0, PtFn pt, Top8* top) {
+    const int rows = (C.ah + stride - 1) / stride;
+    int G = 1;
+    while (G * 2 * n <= kWave && G * 2 <= rows) G *= 2;
+    const int per = kWave / G;
+    const int part = C.lane & (G - 1);
+    uint64_t best = kNone;
+    for (int base = 0; base < n; base += per) {
+        const int i = base + C.lane / G;
+        const bool act = i < n;
+        int dx = 0, dy = 0;
+        if (act) pt(i, dx, dy);
+        int valid = act ? 1 : 0;
+        if (act && seen && part == 0) {
+            const int bit = (dy + kMaxLocalRange) * (2 * kMaxLocalRange + 1) + (dx + kMaxLocalRange);
+            const uint32_t old = atomicOr(&seen[bit >> 5], 1u << (bit & 31));
+            valid = (old >> (bit & 31)) & 1 ? 0 : 1;
+        }
+        if (seen) valid = __shfl(valid, C.lane - part);
+        uint32_t sum = 0;
+        bool hid = false;
+        if (valid) {
+            hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
+            if (!hid) {
+                if (!in_window(C, dx, dy)) {
+                    set_err(C, 2);
+                    hid = true;
+                } else {
+                    sum = sad_int_part(C, dx, dy, stride, part, G);
+                }
+            }
+        }
+        sum = group_sum32(sum, G);
+        uint64_t key = kNone;
+        if (valid && part == 0) {
+            const uint64_t cost = hid ? kBadSad : static_cast<uint64_t>(sum) * static_cast<uint64_t>(stride * stride);
+            if (keymode == KEY_TOTAL) key = key_int(cost, dx, dy);
+            else if (keymode == KEY_SEQ_MAN) key = key_seq(cost, iabs(dx) + iabs(dy), seq0 + i);
+            else key = key_seq(cost, 0, seq0 + i);
+        }
+        if (top) top_merge(*top, key);
+        const uint64_t mk = wave_min64(key);
+        best = mk < best ? mk : best;
+    }
+    return best;
+}
+
+// ------------------------------------------------------------------ geometry helpers
+
+__device__ __forceinline__ uint64_t active_pixels(const Ctx& C) {
+    return static_cast<uint64_t>(C.aw) * static_cast<uint64_t>(C.ah);
+}
+// motion_match_good, src/encoder_motion.cpp:1633-1640
+__device__ __forceinline__ bool match_good(const Ctx& C, uint64_t sad) {
+    const uint64_t px = active_pixels(C);
+    if (!px || sad >= UINT64_MAX / 8) return false;
+    return sad <= px * 6u;
+}
+// distant_match_decision, src/encoder_motion.cpp:1642-1656
+__device__ int distant_decision(const Ctx& C, uint64_t cand, uint64_t local) {
+    const uint64_t px = active_pixels(C);
+    if (!px || cand >= UINT64_MAX / 8) return 0;
+    if (cand <= px * 6u) return 2;
+    if (C.J->rej_always[px] || cand > C.J->rej_thr[px]) return 1;
+    if (local >= UINT64_MAX / 8) return 3;
+    return cand * 4u <= local * 3u ? 3 : 1;
+}
+
+// round_qpel_to_pixel, src/encoder_motion.cpp:1295-1297
+__device__ __forceinline__ int round_qpel_to_pixel(int q) { return q >= 0 ? (q + 2) / 4 : -((-q + 2) / 4); }
+
+struct Bounds {
+    int xmin, xmax, ymin, ymax;
+};
+// motion_search_bounds_px / _qpel, src/encoder_motion.cpp:1238-1255
+__device__ Bounds bounds_px(int requested) {
+    const int r = clampi(requested, 0, 1024);
+    return {-imin(r, 1024), imin(r, 1023), -imin(r, 256), imin(r, 255)};
+}
+__device__ Bounds bounds_qpel(int requested) {
+    const int r = clampi(requested, 0, 1024);
+    Bounds b;
+    b.xmin = -imin(r * 4, 4096);
+    b.xmax = (r >= 1024) ? 4095 : imin(r * 4, 4095);
+    b.ymin = -imin(r * 4, 1024);
+    b.ymax = (r >= 256) ? 1023 : imin(r * 4, 1023);
+    return b;
+}
+
+// ------------------------------------------------------------------ predictors
+
+struct Pred {
+    int px, py, ax, ay, cx, cy;
+    bool a_valid, c_valid, hybrid;
+};
+
+__device__ __forceinline__ int median3(int a, int b, int c) {
+    return a + b + c - imin(a, imin(b, c)) - imax(a, imax(b, c));
+}
+
+__device__ void pullback(int qx, int qy, int X, int Y, int margin, int& px, int& py) {
+    if (qx + px < margin) px = margin - qx;
+    if (qx + px > X) px = X - qx;
+    if (qy + py < margin) py = margin - qy;
+    if (qy + py > Y) py = Y - qy;
+}
+
+// predictor_info, src/encoder_motion.cpp:1312-1352 (reads the final 1-MV field)
+__device__ Pred predictor_info(const Ctx& C) {
+    const JobDesc* J = C.J;
+    const int mbw = C.g.mbw, mbh = C.g.mbh, mx = C.mx, my = C.my;
+    const bool av = my > 0, cv = mx > 0, bv = my > 0 && mbw > 1;
+    int2 A = {0, 0}, B = {0, 0}, Cc = {0, 0};
+    if (av) A = J->mvs[(my - 1) * mbw + mx];
+    if (cv) Cc = J->mvs[my * mbw + mx - 1];
+    if (bv) B = J->mvs[(my - 1) * mbw + ((mx == mbw - 1) ? mx - 1 : mx + 1)];
+    int px = 0, py = 0;
+    if (av) { px = A.x; py = A.y; } else if (cv) { px = Cc.x; py = Cc.y; } else if (bv) { px = B.x; py = B.y; }
+    if ((av ? 1 : 0) + (bv ? 1 : 0) + (cv ? 1 : 0) > 1) { px = median3(A.x, B.x, Cc.x); py = median3(A.y, B.y, Cc.y); }
+    pullback(mx * 64, my * 64, mbw * 64 - 4, mbh * 64 - 4, -60, px, py);
+    bool hyb = false;
+    if (av && cv) {
+        if (iabs(px - A.x) + iabs(py - A.y) > 32) hyb = true;
+        else if (iabs(px - Cc.x) + iabs(py - Cc.y) > 32) hyb = true;
+    }
+    return {px, py, A.x, A.y, Cc.x, Cc.y, av, cv, hyb};
+}
+
+// luma block-vector field with the current macroblock's blocks taken from LDS
+__device__ int2 field_get(const Ctx& C, int x, int y) {
+    if (x < 0 || y < 0 || x >= C.g.mbw * 2 || y >= C.g.mbh * 2) return {0, 0};
+    const int mmx = x >> 1, mmy = y >> 1, k = ((y & 1) << 1) | (x & 1);
+    if (mmx == C.mx && mmy == C.my) return C.L.own[k];
+    return C.J->bmv[(mmy * C.g.mbw + mmx) * 4 + k];
+}
+__device__ bool intra_at(const Ctx& C, int x, int y) {
+    if (x < 0 || y < 0 || x >= C.g.mbw * 2 || y >= C.g.mbh * 2) return false;
+    const int mmx = x >> 1, mmy = y >> 1;
+    if (mmx == C.mx && mmy == C.my) return false;
+    return (C.J->flags[mmy * C.g.mbw + mmx] & kFlagIntra) != 0;
+}
+
+__device__ bool hybrid_flag(const Ctx& C, int px, int py, int bx, int by, int2 A, int2 Cc) {
+    const int suma = intra_at(C, bx, by - 1) ? iabs(px) + iabs(py) : iabs(px - A.x) + iabs(py - A.y);
+    if (suma > 32) return true;
+    const int sumc = intra_at(C, bx - 1, by) ? iabs(px) + iabs(py) : iabs(px - Cc.x) + iabs(py - Cc.y);
+    return sumc > 32;
+}
+
+// predictor_info_mixed_1mv, src/encoder_motion.cpp:1354-1406
+__device__ Pred predictor_mixed_1mv(const Ctx& C) {
+    const int mbw = C.g.mbw, mbh = C.g.mbh, mx = C.mx, my = C.my;
+    const int bx = mx * 2, by = my * 2;
+    const bool av = my > 0, cv = mx > 0, bv = my > 0 && mbw > 1;
+    const int2 A = av ? field_get(C, bx, by - 1) : make_int2(0, 0);
+    const int2 Cc = cv ? field_get(C, bx - 1, by) : make_int2(0, 0);
+    const int2 B = bv ? field_get(C, bx + ((mx == mbw - 1) ? -1 : 2), by - 1) : make_int2(0, 0);
+    int px = 0, py = 0;
+    if (av) { px = A.x; py = A.y; } else if (cv) { px = Cc.x; py = Cc.y; } else if (bv) { px = B.x; py = B.y; }
+    if ((av ? 1 : 0) + (bv ? 1 : 0) + (cv ? 1 : 0) > 1) { px = median3(A.x, B.x, Cc.x); py = median3(A.y, B.y, Cc.y); }
+    pullback(mx * 64, my * 64, mbw * 64 - 4, mbh * 64 - 4, -60, px, py);
+    const bool hyb = av && cv && hybrid_flag(C, px, py, bx, by, A, Cc);
+    return {px, py, A.x, A.y, Cc.x, Cc.y, av, cv, hyb};
+}
+
+// predictor_info_4mv, src/encoder_motion.cpp:1408-1466
+__device__ Pred predictor_4mv(const Ctx& C, int n) {
+    const int mbw = C.g.mbw, mbh = C.g.mbh, mx = C.mx, my = C.my;
+    const int bx = mx * 2 + (n & 1), by = my * 2 + ((n >> 1) & 1);
+    const bool av = my > 0 || n == 2 || n == 3;
+    const bool cv = mx > 0 || n == 1 || n == 3;
+    const bool bv = av;
+    const int2 A = av ? field_get(C, bx, by - 1) : make_int2(0, 0);
+    const int2 Cc = cv ? field_get(C, bx - 1, by) : make_int2(0, 0);
+    int2 B = {0, 0};
+    if (bv) {
+        int bxx = bx;
+        if (n == 0) bxx += mx ? -1 : 1;
+        else if (n == 1) bxx += (mx == mbw - 1) ? -1 : 1;
+        else if (n == 2) bxx += 1;
+        else bxx -= 1;
+        B = field_get(C, bxx, by - 1);
+    }
+    int px = 0, py = 0;
+    if (av) { px = A.x; py = A.y; } else if (cv) { px = Cc.x; py = Cc.y; } else if (bv) { px = B.x; py = B.y; }
+    if ((av ? 1 : 0) + (bv ? 1 : 0) + (cv ? 1 : 0) > 1) { px = median3(A.x, B.x, Cc.x); py = median3(A.y, B.y, Cc.y); }
+    pullback(mx * 64 + ((n == 1 || n == 3) ? 32 : 0), my * 64 + ((n == 2 || n == 3) ? 32 : 0), mbw * 64 - 4,
+             mbh * 64 - 4, -28, px, py);
+    const bool hyb = av && cv && hybrid_flag(C, px, py, bx, by, A, Cc);
+    return {px, py, A.x, A.y, Cc.x, Cc.y, av, cv, hyb};
+}
+
+// ------------------------------------------------------------------ MV rate
+// motion_vector_rate_bits and its syntax helpers, src/encoder_motion.cpp:1099-1197, :3214-3225
+
+__device__ int mv_cat(int v, bool hp) {
+    const int a = iabs(v);
+    if (!a) return 0;
+    if (a <= 2) return 1;
+    if (a <= 6) return 2;
+    if (a <= 14) return 3;
+    if (a <= 30) return 4;
+    if (a <= (hp ? 94 : 158)) return 5;
+    return -1;
+}
+__device__ int mv_component_bits(int cat, bool hp) {
+    const int sz = cat == 0 ? 0 : cat == 1 ? 2 : cat == 2 ? 3 : cat == 3 ? 4 : cat == 4 ? 5 : 8;
+    return sz - ((hp && cat == 5) ? 1 : 0);
+}
+__device__ __forceinline__ int mvrange_kx(int r) { return r == 0 ? 9 : r == 1 ? 10 : r == 2 ? 12 : 13; }
+__device__ __forceinline__ int mvrange_ky(int r) { return r == 0 ? 8 : r == 1 ? 9 : r == 2 ? 10 : 11; }
+__device__ __forceinline__ int mvrange_xq(int r) { return r == 0 ? 256 : r == 1 ? 512 : r == 2 ? 2048 : 4096; }
+__device__ __forceinline__ int mvrange_yq(int r) { return r == 0 ? 128 : r == 1 ? 256 : r == 2 ? 512 : 1024; }
+
+// bits of a differential (dx,dy) (already in mode units), with `more`
+__device__ uint32_t mvdiff_bits(int mvrange, int dx, int dy, bool hp, bool more) {
+    const int xc = mv_cat(dx, hp), yc = mv_cat(dy, hp);
+    int sym;
+    if (xc < 0 || yc < 0) sym = more ? 71 : 34;
+    else {
+        const int idx = yc * 6 + xc;
+        if (!more && idx == 0) sym = 34;
+        else sym = more ? idx + 36 : idx - 1;
+    }
+    int suffix;
+    if (sym == 34 || sym == 71 || xc < 0 || yc < 0) {
+        const int h = hp ? 1 : 0;
+        suffix = (mvrange_kx(mvrange) - h) + (mvrange_ky(mvrange) - h);
+    } else {
+        suffix = mv_component_bits(xc, hp) + mv_component_bits(yc, hp);
+    }
+    return static_cast<uint32_t>(kMvBits0[sym]) + static_cast<uint32_t>(suffix);
+}
+__device__ uint32_t mv_rate_bits(int mvrange, int xq, int yq, int pxq, int pyq, int mode, bool more) {
+    const bool hp = mode_halfpel(mode);
+    int dx = xq - pxq, dy = yq - pyq;
+    if (hp) { dx /= 2; dy /= 2; }
+    return mvdiff_bits(mvrange, dx, dy, hp, more);
+}
+// modular_mv_distance, src/encoder_motion.cpp:1299-1310
+__device__ int mod_dist(int a, int b, int range) {
+    int d = a - b;
+    const int span = range * 2;
+    d = ((d + range) % span + span) % span - range;
+    return iabs(d);
+}
+__device__ int modular_mv_distance(int mvrange, int ax, int ay, int bx, int by) {
+    return mod_dist(ax, bx, mvrange_xq(mvrange)) + mod_dist(ay, by, mvrange_yq(mvrange));
+}
+
+// ------------------------------------------------------------------ interpolation
+// luma_mc_sample / luma_mc_sample_mode, src/encoder_motion_compensation.cpp:360-415,
+// reading the LDS window (which already holds clamped samples).
+
+__device__ __forceinline__ int mspel(int a, int b, int c, int d, int mode) {
+    if (mode == 1) return -4 * a + 53 * b + 18 * c - 3 * d;
+    if (mode == 2) return -a + 9 * b + 9 * c - d;
+    return -3 * a + 18 * b + 53 * c - 4 * d;
+}
+
+// sample at window coordinates (sx, sy) = integer source position, fractional (hm, vm)
+__device__ __forceinline__ int interp_px(const uint8_t* w, int wp, int sx, int sy, int hm, int vm, bool bil) {
+#define S(xx, yy) static_cast<int>(w[(yy) * wp + (xx)])
+    if (bil) {
+        const int a = S(sx, sy);
+        if (!hm && !vm) return a;
+        const int b = S(sx + 1, sy), c = S(sx, sy + 1);
+        if (hm && vm) return (a + b + c + S(sx + 1, sy + 1) + 2) >> 2;
+        if (hm) return (a + b + 1) >> 1;
+        return (a + c + 1) >> 1;
+    }
+    if (!hm && !vm) return S(sx, sy);
+    int v;
+    if (hm && vm) {
+        const int shift = ((hm == 2 ? 1 : 5) + (vm == 2 ? 1 : 5)) >> 1;
+        const int r1 = (1 << (shift - 1)) - 1;
+        int t0 = arshift(mspel(S(sx - 1, sy - 1), S(sx - 1, sy), S(sx - 1, sy + 1), S(sx - 1, sy + 2), vm) + r1, shift);
+        int t1 = arshift(mspel(S(sx, sy - 1), S(sx, sy), S(sx, sy + 1), S(sx, sy + 2), vm) + r1, shift);
+        int t2 = arshift(mspel(S(sx + 1, sy - 1), S(sx + 1, sy), S(sx + 1, sy + 1), S(sx + 1, sy + 2), vm) + r1, shift);
+        int t3 = arshift(mspel(S(sx + 2, sy - 1), S(sx + 2, sy), S(sx + 2, sy + 1), S(sx + 2, sy + 2), vm) + r1, shift);
+        v = arshift(mspel(t0, t1, t2, t3, hm) + 64, 7);
+    } else if (vm) {
+        v = arshift(mspel(S(sx, sy - 1), S(sx, sy), S(sx, sy + 1), S(sx, sy + 2), vm) + (vm == 2 ? 7 : 31),
+                    vm == 2 ? 4 : 6);
+    } else {
+        v = arshift(mspel(S(sx - 1, sy), S(sx, sy), S(sx + 1, sy), S(sx + 2, sy), hm) + (hm == 2 ? 8 : 32),
+                    hm == 2 ? 4 : 6);
+    }
+#undef S
+    return clampi(v, 0, 255);
+}
+
+// Window covering the interpolation footprint of block (bx0,by0,bw,bh) for all
+// vectors in [xq_lo,xq_hi]x[yq_lo,yq_hi] (taps -1..+2 around each integer position).
+__device__ bool qpel_window(Ctx& C, int bx0, int by0, int bw, int bh, int xq_lo, int xq_hi, int yq_lo, int yq_hi) {
+    const int ix_lo = floor_div4(xq_lo), ix_hi = floor_div4(xq_hi);
+    const int iy_lo = floor_div4(yq_lo), iy_hi = floor_div4(yq_hi);
+    const int wx0 = bx0 + ix_lo - 1, wy0 = by0 + iy_lo - 1;
+    const int ww = (ix_hi - ix_lo) + bw + 3, wh = (iy_hi - iy_lo) + bh + 3;
+    if (C.wx0 == wx0 && C.wy0 == wy0 && C.ww >= ww && C.wh >= wh && C.wp * C.wh <= C.winCap) return true;
+    return win_load(C, wx0, wy0, ww, wh);
+}
+
+// SAD (src/encoder_motion.cpp:2605-2667 law) of one candidate on block
+// (bx0,by0,bw,bh) with active (aw,ah); the window must cover it.
+__device__ uint32_t qpel_sad_one(const Ctx& C, int bx0, int by0, int bw, int aw, int ah, int xq, int yq, bool bil) {
+    const int ix = floor_div4(xq), iy = floor_div4(yq);
+    const int hm = xq - ix * 4, vm = yq - iy * 4;
+    uint32_t acc = 0;
+    const int npx = bw * bw;  // square blocks: 16x16 or 8x8
+    for (int p = C.lane; p < npx; p += kWave) {
+        const int x = bw == 16 ? (p & 15) : (p & 7);
+        const int y = bw == 16 ? (p >> 4) : (p >> 3);
+        if (x < aw && y < ah) {
+            const int sx = bx0 + x + ix - C.wx0, sy = by0 + y + iy - C.wy0;
+            const int pred = interp_px(C.L.win, C.wp, sx, sy, hm, vm, bil);
+            const int cv = C.L.cur[(by0 - C.y0 + y) * 16 + (bx0 - C.x0 + x)];
+            acc += static_cast<uint32_t>(iabs(cv - pred));
+        }
+    }
+    return wave_sum32(acc);
+}
+
+// Score n quarter-pel candidates (L.req[0..n)) by interpolated SAD into
+// L.reqcost, applying the hidden-padding veto of block_sad_qpel /
+// block_sad_motion (src/encoder_motion.cpp:2286-2293, :2670-2676).
+__device__ void qpel_sad_batch(Ctx& C, int bx0, int by0, int bw, int bh, int n, int mode) {
+    const bool bil = mode_bilinear(mode);
+    const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
+    const bool chroma = chroma_block(bx0, by0, bw, bh);
+    int xlo = 1 << 30, xhi = -(1 << 30), ylo = 1 << 30, yhi = -(1 << 30);
+    for (int i = 0; i < n; ++i) {
+        const int2 v = C.L.req[i];
+        xlo = imin(xlo, v.x); xhi = imax(xhi, v.x); ylo = imin(ylo, v.y); yhi = imax(yhi, v.y);
+    }
+    bool shared = n > 0 && qpel_window(C, bx0, by0, bw, bh, xlo, xhi, ylo, yhi);
+    for (int i = 0; i < n; ++i) {
+        const int2 v = C.L.req[i];
+        uint64_t cost;
+        if (hidden_padding(C.g, bx0, by0, bw, bh, v.x, v.y, bil ? mode : 0, chroma)) cost = kBadSad;
+        else if (bil && ((v.x & 1) || (v.y & 1))) cost = kBadSad;
+        else {
+            if (!shared) {
+                if (!qpel_window(C, bx0, by0, bw, bh, v.x, v.x, v.y, v.y)) { set_err(C, 3); cost = kBadSad; goto store; }
+            }
+            cost = qpel_sad_one(C, bx0, by0, bw, aw, ah, v.x, v.y, bil);
+        }
+    store:
+        wsync();
+        if (C.lane == 0) C.L.reqcost[i] = cost;
+        wsync();
+        if (!shared && i + 1 < n) {
+            // per-candidate windows: nothing to keep
+        }
+    }
+}
+
+// block_satd_motion, src/encoder_motion.cpp:2962-3042: render the prediction,
+// then sum (|H D H^T| + 1) >> 1 over 4x4 tiles; partial tiles see zero residual.
+__device__ uint64_t satd_one(Ctx& C, int bx0, int by0, int bw, int bh, int xq, int yq, int mode) {
+    const bool bil = mode_bilinear(mode);
+    const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
+    if (!aw || !ah) return 0;
+    if (mode_halfpel(mode) && ((xq & 1) || (yq & 1))) return kBadSad;
+    if (!qpel_window(C, bx0, by0, bw, bh, xq, xq, yq, yq)) { set_err(C, 4); return kBadSad; }
+    const int ix = floor_div4(xq), iy = floor_div4(yq);
+    const int hm = xq - ix * 4, vm = yq - iy * 4;
+    const int npx = bw * bh;
+    for (int p = C.lane; p < npx; p += kWave) {
+        const int x = bw == 16 ? (p & 15) : (p & 7);
+        const int y = bw == 16 ? (p >> 4) : (p >> 3);
+        int pv = 0;
+        if (x < aw && y < ah) pv = interp_px(C.L.win, C.wp, bx0 + x + ix - C.wx0, by0 + y + iy - C.wy0, hm, vm, bil);
+        C.L.pred[y * 16 + x] = static_cast<uint8_t>(pv);
+    }
+    wsync();
+    const int tpr = bw / 4;                 // tiles per row
+    const int ntiles = tpr * (bh / 4);
+    const int tile = C.lane >> 2, r = C.lane & 3;
+    const bool tact = tile < ntiles;
+    int tx = 0, ty = 0;
+    bool tvalid = false;
+    if (tact) {
+        tx = (tile % tpr) * 4;
+        ty = (tile / tpr) * 4;
+        tvalid = tx < aw && ty < ah;
+    }
+    int d[4] = {0, 0, 0, 0};
+    if (tvalid) {
+        const int y = ty + r;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int x = tx + j;
+            if (x < aw && y < ah) {
+                const int cv = C.L.cur[(by0 - C.y0 + y) * 16 + (bx0 - C.x0 + x)];
+                d[j] = cv - static_cast<int>(C.L.pred[y * 16 + x]);
+            }
+        }
+    }
+    const int a0 = d[0] + d[3], a1 = d[1] + d[2], a2 = d[1] - d[2], a3 = d[0] - d[3];
+    C.L.tbuf[C.lane * 4 + 0] = a0 + a1;
+    C.L.tbuf[C.lane * 4 + 1] = a3 + a2;
+    C.L.tbuf[C.lane * 4 + 2] = a0 - a1;
+    C.L.tbuf[C.lane * 4 + 3] = a3 - a2;
+    wsync();
+    uint32_t colsum = 0;
+    if (tvalid) {
+        const int b = (C.lane & ~3) * 4;
+        const int v0 = C.L.tbuf[b + 0 * 4 + r], v1 = C.L.tbuf[b + 1 * 4 + r];
+        const int v2 = C.L.tbuf[b + 2 * 4 + r], v3 = C.L.tbuf[b + 3 * 4 + r];
+        const int c0 = v0 + v3, c1 = v1 + v2, c2 = v1 - v2, c3 = v0 - v3;
+        colsum = static_cast<uint32_t>(iabs(c0 + c1) + iabs(c3 + c2) + iabs(c0 - c1) + iabs(c3 - c2));
+    }
+    uint32_t tsum = colsum;
+    tsum += __shfl_xor(tsum, 1);
+    tsum += __shfl_xor(tsum, 2);
+    const uint32_t contrib = (tvalid && r == 0) ? ((tsum + 1) >> 1) : 0;
+    const uint32_t total = wave_sum32(contrib);
+    wsync();
+    return total;
+}
+
+// ------------------------------------------------------------------ integer searches
+
+struct IRes {
+    int dx, dy;
+    uint64_t sad;
+};
+
+// integer_motion_umh, src/encoder_motion.cpp:1781-2023
+__device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
+    // the whole search stays inside [-R,R]^2: one window serves every stage
+    if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
+        set_err(C, 5);
+        return {0, 0, kBadSad};
+    }
+    for (int i = C.lane; i < kSeenWords; i += kWave) { C.L.seen_s[i] = 0; C.L.seen_f[i] = 0; }
+    wsync();
+    Top8 top;
+    top.n = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) top.k[j] = kNone;
+    auto cl = [R](int v) { return clampi(v, -R, R); };
+
+    // predictor-driven seeds
+    {
+        int sx[5], sy[5], n = 0;
+        sx[n] = cl(round_qpel_to_pixel(pi.px)); sy[n] = cl(round_qpel_to_pixel(pi.py)); ++n;
+        sx[n] = 0; sy[n] = 0; ++n;
+        sx[n] = cl(gdx); sy[n] = cl(gdy); ++n;
+        if (pi.a_valid) { sx[n] = cl(round_qpel_to_pixel(pi.ax)); sy[n] = cl(round_qpel_to_pixel(pi.ay)); ++n; }
+        if (pi.c_valid) { sx[n] = cl(round_qpel_to_pixel(pi.cx)); sy[n] = cl(round_qpel_to_pixel(pi.cy)); ++n; }
+        int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            dx = i == 0 ? sx[0] : i == 1 ? sx[1] : i == 2 ? sx[2] : i == 3 ? sx[3] : sx[4];
+            dy = i == 0 ? sy[0] : i == 1 ? sy[1] : i == 2 ? sy[2] : i == 3 ? sy[3] : sy[4];
+        }, &top);
+    }
+    // local 5x5
+    {
+        const int cx = key_dx(top.k[0]), cy = key_dy(top.k[0]);
+        int_batch(C, 25, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            dx = cl(cx + (i % 5) - 2); dy = cl(cy + (i / 5) - 2);
+        }, &top);
+    }
+    // sparse lattice
+    if (R >= 8) {
+        const int side = (2 * R) / 8 + 1;
+        const int n = side * side + 2 * side;
+        int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            if (i < side * side) { dx = -R + 8 * (i % side); dy = -R + 8 * (i / side); }
+            else {
+                const int j = i - side * side, d = -R + 8 * (j >> 1);
+                if (j & 1) { dx = d; dy = R; } else { dx = R; dy = d; }
+            }
+            dx = cl(dx); dy = cl(dy);
+        }, &top);
+    }
+    // uneven cross
+    {
+        const int cs = R >= 96 ? 8 : (R >= 32 ? 4 : 2);
+        const int cx = key_dx(top.k[0]), cy = key_dy(top.k[0]);
+        const int vlimit = imin(R, imax(8, R / 2));
+        const int nh = R / cs, nv = vlimit / cs;
+        int_batch(C, 2 * (nh + nv), 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            if (i < 2 * nh) {
+                const int d = cs * (i / 2 + 1);
+                dx = (i & 1) ? cx + d : cx - d; dy = cy;
+            } else {
+                const int j = i - 2 * nh, d = cs * (j / 2 + 1);
+                dx = cx; dy = (j & 1) ? cy + d : cy - d;
+            }
+            dx = cl(dx); dy = cl(dy);
+        }, &top);
+    }
+    // multi-hexagon rings
+    {
+        const int hs = R >= 96 ? 8 : (R >= 24 ? 4 : 2);
+        const int hx = key_dx(top.k[0]), hy = key_dy(top.k[0]);
+        const int rings = R / hs;
+        int_batch(C, 6 * rings, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            const int r = hs * (i / 6 + 1), h = (r + 1) / 2, k = i % 6;
+            switch (k) {
+                case 0: dx = hx - r; dy = hy; break;
+                case 1: dx = hx + r; dy = hy; break;
+                case 2: dx = hx - h; dy = hy - r; break;
+                case 3: dx = hx + h; dy = hy - r; break;
+                case 4: dx = hx - h; dy = hy + r; break;
+                default: dx = hx + h; dy = hy + r; break;
+            }
+            dx = cl(dx); dy = cl(dy);
+        }, &top);
+    }
+    // iterative small hexagon
+    for (int iter = 0; iter < 32; ++iter) {
+        const int ox = key_dx(top.k[0]), oy = key_dy(top.k[0]);
+        int_batch(C, 6, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            switch (i) {
+                case 0: dx = ox - 2; dy = oy; break;
+                case 1: dx = ox + 2; dy = oy; break;
+                case 2: dx = ox - 1; dy = oy - 2; break;
+                case 3: dx = ox + 1; dy = oy - 2; break;
+                case 4: dx = ox - 1; dy = oy + 2; break;
+                default: dx = ox + 1; dy = oy + 2; break;
+            }
+            dx = cl(dx); dy = cl(dy);
+        }, &top);
+        if (key_dx(top.k[0]) == ox && key_dy(top.k[0]) == oy) break;
+    }
+    // closing diamond
+    {
+        const int ox = key_dx(top.k[0]), oy = key_dy(top.k[0]);
+        int_batch(C, 4, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            dx = ox + (i == 0 ? -1 : i == 1 ? 1 : 0);
+            dy = oy + (i == 2 ? -1 : i == 3 ? 1 : 0);
+            dx = cl(dx); dy = cl(dy);
+        }, &top);
+    }
+    // full-resolution finalists: 3x3 around the best four
+    const int fin = imin(4, top.n);
+    uint64_t tk[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) tk[j] = top.k[j];
+    uint64_t full = int_batch(C, fin * 9, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        const int f = i / 9, o = i % 9;
+        const uint64_t k = f == 0 ? tk[0] : f == 1 ? tk[1] : f == 2 ? tk[2] : tk[3];
+        dx = cl(key_dx(k) + (o % 3) - 1); dy = cl(key_dy(k) + (o / 3) - 1);
+    }, nullptr);
+    if (full == kNone) {
+        const int bx = key_dx(top.k[0]), by = key_dy(top.k[0]);
+        full = int_batch(C, 1, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int, int& dx, int& dy) { dx = bx; dy = by; }, nullptr);
+    }
+    // rescue of genuinely poor matches
+    if (R > 0 && !match_good(C, key_cost(full))) {
+        const int step = R >= 8 ? 4 : (R >= 3 ? 2 : 1);
+        const int side = (2 * R) / step + 1;
+        const uint64_t coarse = int_batch(C, side * side + 2 * side, 4, nullptr, KEY_TOTAL, 0,
+                                          [&](int i, int& dx, int& dy) {
+            if (i < side * side) { dx = -R + step * (i % side); dy = -R + step * (i / side); }
+            else {
+                const int j = i - side * side, d = -R + step * (j >> 1);
+                if (j & 1) { dx = d; dy = R; } else { dx = R; dy = d; }
+            }
+        }, nullptr);
+        const int cdx = key_dx(coarse), cdy = key_dy(coarse);
+        const int rr = imax(1, step - 1);
+        const int xa = imax(-R, cdx - rr), xb = imin(R, cdx + rr), ya = imax(-R, cdy - rr), yb = imin(R, cdy + rr);
+        const int rw = xb - xa + 1, rh = yb - ya + 1;
+        const uint64_t rescue = int_batch(C, rw * rh, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            dx = xa + i % rw; dy = ya + i / rw;
+        }, nullptr);
+        if (rescue < full) full = rescue;
+        const uint64_t px = active_pixels(C);
+        if (px && key_cost(full) > px * 24u && R <= 32) {
+            const int ds = (2 * R) / 2 + 1;
+            const uint64_t dense = int_batch(C, ds * ds, 2, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+                dx = -R + 2 * (i % ds); dy = -R + 2 * (i / ds);
+            }, nullptr);
+            const int ddx = key_dx(dense), ddy = key_dy(dense);
+            const int xa2 = imax(-R, ddx - 2), xb2 = imin(R, ddx + 2), ya2 = imax(-R, ddy - 2), yb2 = imin(R, ddy + 2);
+            const int w2 = xb2 - xa2 + 1, h2 = yb2 - ya2 + 1;
+            const uint64_t dfull = int_batch(C, w2 * h2, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+                dx = xa2 + i % w2; dy = ya2 + i / w2;
+            }, nullptr);
+            if (dfull < full) full = dfull;
+        }
+    }
+    return {key_dx(full), key_dy(full), key_cost(full)};
+}
+
+// integer_motion_diamond, src/encoder_motion_diamond.cpp:13-73
+__device__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
+    if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
+        set_err(C, 5);
+        return {0, 0, kBadSad};
+    }
+    uint64_t best = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; }, nullptr);
+    if (key_cost(best) == 0) return {0, 0, 0};
+    int sx[5], sy[5], sn = 1;
+    sx[0] = 0; sy[0] = 0;
+    auto seed = [&](int dx, int dy) {
+        dx = clampi(dx, -R, R); dy = clampi(dy, -R, R);
+        for (int i = 0; i < sn; ++i)
+            if (sx[i] == dx && sy[i] == dy) return;
+        sx[sn] = dx; sy[sn] = dy; ++sn;
+    };
+    seed(round_qpel_to_pixel(pi.px), round_qpel_to_pixel(pi.py));
+    seed(gdx, gdy);
+    if (pi.a_valid) seed(round_qpel_to_pixel(pi.ax), round_qpel_to_pixel(pi.ay));
+    if (pi.c_valid) seed(round_qpel_to_pixel(pi.cx), round_qpel_to_pixel(pi.cy));
+    if (sn > 1) {
+        const uint64_t s = int_batch(C, sn - 1, 1, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            const int j = i + 1;
+            dx = j == 1 ? sx[1] : j == 2 ? sx[2] : j == 3 ? sx[3] : sx[4];
+            dy = j == 1 ? sy[1] : j == 2 ? sy[2] : j == 3 ? sy[3] : sy[4];
+        }, nullptr);
+        if (s < best) best = s;
+    }
+    int prevx = key_dx(best), prevy = key_dy(best);
+    bool have_prev = false;
+    for (int step = 0; step <= 4 * R + 1; ++step) {
+        const uint64_t center = best;
+        const int cx = key_dx(center), cy = key_dy(center);
+        int ox[4], oy[4], n = 0;
+        const int offs[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        for (int k = 0; k < 4; ++k) {
+            const int dx = cx + offs[k][0], dy = cy + offs[k][1];
+            if (dx < -R || dx > R || dy < -R || dy > R) continue;
+            if (have_prev && dx == prevx && dy == prevy) continue;
+            ox[n] = dx; oy[n] = dy; ++n;
+        }
+        uint64_t win = center;
+        if (n) {
+            const uint64_t s = int_batch(C, n, 1, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+                dx = i == 0 ? ox[0] : i == 1 ? ox[1] : i == 2 ? ox[2] : ox[3];
+                dy = i == 0 ? oy[0] : i == 1 ? oy[1] : i == 2 ? oy[2] : oy[3];
+            }, nullptr);
+            if (s < win) win = s;
+        }
+        if (win == center) break;
+        prevx = cx; prevy = cy; have_prev = true;
+        best = win;
+        if (key_cost(best) == 0) break;
+    }
+    return {key_dx(best), key_dy(best), key_cost(best)};
+}
+
+// ------------------------------------------------------------------ long-range searches
+
+// Integer SADs of a cluster of far candidates: window over the bounding box.
+template <class PtFn>
+__device__ uint64_t far_batch(Ctx& C, int n, int stride, int keymode, int seq0, PtFn pt) {
+    int xlo = 1 << 30, xhi = -(1 << 30), ylo = 1 << 30, yhi = -(1 << 30);
+    for (int i = 0; i < n; ++i) {
+        int dx, dy;
+        pt(i, dx, dy);
+        xlo = imin(xlo, dx); xhi = imax(xhi, dx); ylo = imin(ylo, dy); yhi = imax(yhi, dy);
+    }
+    if (!win_load(C, C.x0 + xlo, C.y0 + ylo, 16 + xhi - xlo, 16 + yhi - ylo)) {
+        set_err(C, 6);
+        return kNone;
+    }
+    return int_batch(C, n, stride, nullptr, keymode, seq0, pt, nullptr);
+}
+
+// propagated_long_range_motion, src/encoder_motion.cpp:1658-1694
+__device__ IRes propagated(Ctx& C, int range) {
+    const JobDesc* J = C.J;
+    const Bounds b = bounds_px(range);
+    uint64_t best = kNone;
+    const int nbx[4] = {-1, 0, -1, 1}, nby[4] = {0, -1, -1, -1};
+    for (int k = 0; k < 4; ++k) {
+        const int nx = C.mx + nbx[k], ny = C.my + nby[k];
+        if (nx < 0 || ny < 0 || nx >= C.g.mbw) continue;
+        const int pos = ny * C.g.mbw + nx;
+        if (!J->prop[pos]) continue;
+        const int2 f = J->mvs[pos];
+        const int cx = round_qpel_to_pixel(f.x), cy = round_qpel_to_pixel(f.y);
+        const uint64_t s = far_batch(C, 13, 1, KEY_SEQ_MAN, k * 13, [&](int i, int& dx, int& dy) {
+            const int ox = i == 1 || i == 9 || i == 11 ? -1 : i == 2 || i == 10 || i == 12 ? 1 : i == 5 ? -2 : i == 6 ? 2 : 0;
+            const int oy = i == 3 || i == 9 || i == 10 ? -1 : i == 4 || i == 11 || i == 12 ? 1 : i == 7 ? -2 : i == 8 ? 2 : 0;
+            dx = clampi(cx + ox, b.xmin, b.xmax);
+            dy = clampi(cy + oy, b.ymin, b.ymax);
+        });
+        // decode the vector of the winner: recompute its point from the sequence number
+        if (s < best) best = s;
+        if (best != kNone && match_good(C, key_cost(best))) break;
+    }
+    if (best == kNone) return {0, 0, UINT64_MAX};
+    // map the sequence number back to the vector
+    const int seq = static_cast<int>(best & 0xffff);
+    const int k = seq / 13, i = seq % 13;
+    const int2 f = J->mvs[(C.my + nby[k]) * C.g.mbw + (C.mx + nbx[k])];
+    const int cx = round_qpel_to_pixel(f.x), cy = round_qpel_to_pixel(f.y);
+    const int ox = i == 1 || i == 9 || i == 11 ? -1 : i == 2 || i == 10 || i == 12 ? 1 : i == 5 ? -2 : i == 6 ? 2 : 0;
+    const int oy = i == 3 || i == 9 || i == 10 ? -1 : i == 4 || i == 11 || i == 12 ? 1 : i == 7 ? -2 : i == 8 ? 2 : 0;
+    return {clampi(cx + ox, b.xmin, b.xmax), clampi(cy + oy, b.ymin, b.ymax), key_cost(best)};
+}
+
+// long_range_signature of the current block (src/encoder_motion.cpp:1591-1601)
+__device__ uint64_t cur_signature(Ctx& C) {
+    const JobDesc* J = C.J;
+    const int W = C.g.W, H = C.g.H, cw = W / 2, ch = H / 2;
+    uint32_t v[6], n[6];
+    const int lx = C.lane & 7, ly = C.lane >> 3;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        int px, py, pw, ph;
+        const uint8_t* p;
+        if (b < 4) { px = C.x0 + (b & 1) * 8 + lx; py = C.y0 + (b >> 1) * 8 + ly; pw = W; ph = H; p = J->cy; }
+        else { px = C.x0 / 2 + lx; py = C.y0 / 2 + ly; pw = cw; ph = ch; p = b == 4 ? J->cu : J->cv; }
+        const bool in = px < pw && py < ph && px >= 0 && py >= 0;
+        v[b] = wave_sum32(in ? p[static_cast<size_t>(py) * pw + px] : 0u);
+        n[b] = wave_sum32(in ? 1u : 0u);
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const uint32_t m = n[b] ? (v[b] + n[b] / 2) / n[b] : 128u;
+        s |= static_cast<uint64_t>(m & 255u) << (8 * b);
+    }
+    return s;
+}
+
+// long_range_signature_distance, src/encoder_motion.cpp:1603-1613
+__device__ __forceinline__ uint32_t sig_distance(uint64_t a, uint64_t b) {
+    uint32_t d = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        d += static_cast<uint32_t>(iabs(static_cast<int>((a >> (8 * i)) & 255) - static_cast<int>((b >> (8 * i)) & 255))) * 2u;
+    d += static_cast<uint32_t>(iabs(static_cast<int>((a >> 32) & 255) - static_cast<int>((b >> 32) & 255))) * 3u;
+    d += static_cast<uint32_t>(iabs(static_cast<int>((a >> 40) & 255) - static_cast<int>((b >> 40) & 255))) * 3u;
+    return d;
+}
+
+// extended_content_motion, src/encoder_motion.cpp:1696-1779
+__device__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_sad, int* cand_decision) {
+    const JobDesc* J = C.J;
+    const uint64_t want = cur_signature(C);
+    const Bounds b = bounds_px(range);
+    // cells whose offset lies inside the bounds
+    const int cx_lo = imax(0, (C.x0 + b.xmin + 7) >> 3), cx_hi = imin(J->ncx - 1, (C.x0 + b.xmax) >> 3);
+    const int cy_lo = imax(0, (C.y0 + b.ymin + 7) >> 3), cy_hi = imin(J->ncy - 1, (C.y0 + b.ymax) >> 3);
+    uint64_t loc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) loc[j] = kNone;
+    if (cx_lo <= cx_hi && cy_lo <= cy_hi) {
+        const int nxc = cx_hi - cx_lo + 1;
+        const int total = nxc * (cy_hi - cy_lo + 1);
+        for (int i = C.lane; i < total; i += kWave) {
+            const int cyy = cy_lo + i / nxc, cxx = cx_lo + i % nxc;
+            const int dx = cxx * 8 - C.x0, dy = cyy * 8 - C.y0;
+            if (iabs(dx) <= 32 && iabs(dy) <= 32) continue;
+            uint64_t k = key_int(sig_distance(want, J->sig[cyy * J->ncx + cxx]), dx, dy);
+            if (k < loc[15]) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    if (k < loc[j]) { const uint64_t t = loc[j]; loc[j] = k; k = t; }
+                }
+            }
+        }
+    }
+    int tn = 0;
+    for (int r = 0; r < 16; ++r) {
+        const uint64_t mk = wave_min64(loc[0]);
+        if (mk == kNone) break;
+        if (C.lane == 0) C.L.top16[r] = mk;
+        if (loc[0] == mk) {
+#pragma unroll
+            for (int j = 0; j < 15; ++j) loc[j] = loc[j + 1];
+            loc[15] = kNone;
+        }
+        ++tn;
+    }
+    wsync();
+    uint64_t best_sad = UINT64_MAX;
+    int bdx = 0, bdy = 0;
+    for (int t = 0; t < tn; ++t) {
+        const uint64_t ck = C.L.top16[t];
+        const int cdx = key_dx(ck), cdy = key_dy(ck);
+        const uint64_t s1 = far_batch(C, 25, 2, KEY_SEQ, 0, [&](int i, int& dx, int& dy) {
+            dx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
+            dy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
+        });
+        int sx = cdx, sy = cdy;
+        if (s1 != kNone && key_cost(s1) != UINT64_MAX) {
+            const int i = static_cast<int>(s1 & 0xffff);
+            sx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
+            sy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
+        }
+        const uint64_t s2 = far_batch(C, 25, 1, KEY_SEQ_MAN, 0, [&](int i, int& dx, int& dy) {
+            dx = clampi(sx + (i % 5) - 2, b.xmin, b.xmax);
+            dy = clampi(sy + (i / 5) - 2, b.ymin, b.ymax);
+        });
+        if (s2 != kNone) {
+            const uint64_t fs = key_cost(s2);
+            if (fs < best_sad) {
+                const int i = static_cast<int>(s2 & 0xffff);
+                best_sad = fs;
+                bdx = clampi(sx + (i % 5) - 2, b.xmin, b.xmax);
+                bdy = clampi(sy + (i / 5) - 2, b.ymin, b.ymax);
+            }
+        }
+        if (match_good(C, best_sad)) break;
+    }
+    *cand_sad = best_sad;
+    const int decision = best_sad < UINT64_MAX / 8 ? distant_decision(C, best_sad, local.sad) : 0;
+    *cand_decision = decision;
+    if (best_sad >= local.sad || decision < 2) return local;
+    return {bdx, bdy, best_sad};
+}
+
+// ------------------------------------------------------------------ staged sub-pel refinement
+
+struct SubParams {
+    int bx0, by0, bw, bh, mode, range;
+    bool more;
+};
+
+__device__ int find_sub(const Ctx& C, int n, int xq, int yq) {
+    for (int i = 0; i < n; ++i)
+        if (C.L.sub[i].xq == xq && C.L.sub[i].yq == yq) return i;
+    return -1;
+}
+
+// add(mv, known) of refine_motion_block_staged (src/encoder_motion.cpp:3330-3343),
+// batched: candidates not yet in the list are queued and scored together.
+__device__ int sub_add_batch(Ctx& C, const SubParams& sp, int nsub, const int* xs, const int* ys, int n) {
+    int nreq = 0;
+    for (int i = 0; i < n; ++i) {
+        if (find_sub(C, nsub, xs[i], ys[i]) >= 0) continue;
+        bool dup = false;
+        for (int j = 0; j < nreq; ++j)
+            if (C.L.req[j].x == xs[i] && C.L.req[j].y == ys[i]) dup = true;
+        if (dup) continue;
+        wsync();
+        if (C.lane == 0) C.L.req[nreq] = make_int2(xs[i], ys[i]);
+        wsync();
+        ++nreq;
+    }
+    if (!nreq) return nsub;
+    qpel_sad_batch(C, sp.bx0, sp.by0, sp.bw, sp.bh, nreq, sp.mode);
+    for (int j = 0; j < nreq; ++j) {
+        const int2 v = C.L.req[j];
+        const uint64_t cost = C.L.reqcost[j];
+        wsync();
+        if (C.lane == 0) {
+            SubCand& s = C.L.sub[nsub + j];
+            s.xq = v.x; s.yq = v.y; s.sad = cost; s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
+        }
+        wsync();
+    }
+    return nsub + nreq;
+}
+
+__device__ __forceinline__ bool sad_less(const SubCand& a, const SubCand& b) {
+    if (a.sad != b.sad) return a.sad < b.sad;
+    return tie_less(a.xq, a.yq, b.xq, b.yq);
+}
+
+__device__ int make_even(int v) { return (v & 1) ? (v > 0 ? v - 1 : v + 1) : v; }
+
+// refine_motion_integer_only, src/encoder_motion.cpp:3247-3269
+__device__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int syq, uint64_t* best_sad, uint64_t known) {
+    const Bounds qb = bounds_qpel(sp.range);
+    auto snap = [](int q, int lo, int hi) {
+        q = clampi(q, lo, hi);
+        const int nearest = q >= 0 ? (q + 1) / 4 : (q - 1) / 4;
+        return clampi(nearest * 4, lo, hi / 4 * 4);
+    };
+    int ix = snap(sxq, qb.xmin, qb.xmax), iy = snap(syq, qb.ymin, qb.ymax);
+    if (hidden_padding(C.g, sp.bx0, sp.by0, sp.bw, sp.bh, ix, iy, sp.mode, chroma_block(sp.bx0, sp.by0, sp.bw, sp.bh))) {
+        ix = 0; iy = 0;
+    }
+    if (ix == sxq && iy == syq && known != UINT64_MAX) *best_sad = known;
+    else {
+        wsync();
+        if (C.lane == 0) C.L.req[0] = make_int2(ix, iy);
+        wsync();
+        qpel_sad_batch(C, sp.bx0, sp.by0, sp.bw, sp.bh, 1, sp.mode);
+        *best_sad = C.L.reqcost[0];
+    }
+    return make_int2(ix, iy);
+}
+
+// refine_motion_block_staged, src/encoder_motion.cpp:3271-3594 (quality RATE / SATD)
+__device__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred& pi, uint64_t* best_sad,
+                              uint64_t known_seed) {
+    const JobDesc* J = C.J;
+    const int limit_q = 4 * J->subpel_range;
+    if (!J->subpel || limit_q == 0 || imax(iabs(sxq), iabs(syq)) > limit_q)
+        return refine_integer_only(C, sp, sxq, syq, best_sad, known_seed);
+    const Bounds qb = bounds_qpel(sp.range);
+    const int xmin = imax(qb.xmin, -limit_q), xmax = imin(qb.xmax, limit_q);
+    const int ymin = imax(qb.ymin, -limit_q), ymax = imin(qb.ymax, limit_q);
+    const bool hp = mode_halfpel(sp.mode);
+    const bool chroma = chroma_block(sp.bx0, sp.by0, sp.bw, sp.bh);
+    sxq = clampi(sxq, xmin, xmax);
+    syq = clampi(syq, ymin, ymax);
+    if (hp) { sxq = make_even(sxq); syq = make_even(syq); }
+    int nsub = 0;
+    // seed with its known SAD
+    {
+        const bool hid = hidden_padding(C.g, sp.bx0, sp.by0, sp.bw, sp.bh, sxq, syq, sp.mode, chroma);
+        if (hid || known_seed != UINT64_MAX) {
+            wsync();
+            if (C.lane == 0) {
+                SubCand& s = C.L.sub[0];
+                s.xq = sxq; s.yq = syq; s.sad = hid ? kBadSad : known_seed; s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
+            }
+            wsync();
+            nsub = 1;
+        } else {
+            int xs[1] = {sxq}, ys[1] = {syq};
+            nsub = sub_add_batch(C, sp, 0, xs, ys, 1);
+        }
+        if (hid) {
+            int xs[1] = {0}, ys[1] = {0};
+            nsub = sub_add_batch(C, sp, nsub, xs, ys, 1);
+        }
+    }
+    int hbx = sxq, hby = syq;
+    uint64_t hsad = C.L.sub[0].sad;
+    const bool fast = J->fast != 0;
+    int bxs[9], bys[9], nb = 0;
+    if (fast) {
+        bxs[nb] = sxq; bys[nb] = syq; ++nb;
+        const int o4[4][2] = {{-2, 0}, {2, 0}, {0, -2}, {0, 2}};
+        for (int k = 0; k < 4; ++k) {
+            int x = clampi(sxq + o4[k][0], xmin, xmax), y = clampi(syq + o4[k][1], ymin, ymax);
+            if (hp) { x = make_even(x); y = make_even(y); }
+            bxs[nb] = x; bys[nb] = y; ++nb;
+        }
+    } else {
+        for (int oy = -2; oy <= 2; oy += 2)
+            for (int ox = -2; ox <= 2; ox += 2) {
+                int x = clampi(sxq + ox, xmin, xmax), y = clampi(syq + oy, ymin, ymax);
+                if (hp) { x = make_even(x); y = make_even(y); }
+                bxs[nb] = x; bys[nb] = y; ++nb;
+            }
+    }
+    nsub = sub_add_batch(C, sp, nsub, bxs, bys, nb);
+    for (int i = 0; i < nb; ++i) {
+        const int j = find_sub(C, nsub, bxs[i], bys[i]);
+        if (j >= 0) {
+            const SubCand& s = C.L.sub[j];
+            if (s.sad < hsad || (s.sad == hsad && tie_less(s.xq, s.yq, hbx, hby))) { hsad = s.sad; hbx = s.xq; hby = s.yq; }
+        }
+    }
+    if (fast && !(hbx == sxq && hby == syq)) {
+        int dxs[2], dys[2], nd = 0;
+        const int dx = hbx - sxq, dy = hby - syq;
+        if (dx && !dy) {
+            dxs[nd] = hbx; dys[nd] = clampi(syq - 2, ymin, ymax); ++nd;
+            dxs[nd] = hbx; dys[nd] = clampi(syq + 2, ymin, ymax); ++nd;
+        } else if (dy && !dx) {
+            dxs[nd] = clampi(sxq - 2, xmin, xmax); dys[nd] = hby; ++nd;
+            dxs[nd] = clampi(sxq + 2, xmin, xmax); dys[nd] = hby; ++nd;
+        }
+        if (hp)
+            for (int i = 0; i < nd; ++i) { dxs[i] = make_even(dxs[i]); dys[i] = make_even(dys[i]); }
+        nsub = sub_add_batch(C, sp, nsub, dxs, dys, nd);
+        for (int i = 0; i < nd; ++i) {
+            const int j = find_sub(C, nsub, dxs[i], dys[i]);
+            if (j >= 0) {
+                const SubCand& s = C.L.sub[j];
+                if (s.sad < hsad || (s.sad == hsad && tie_less(s.xq, s.yq, hbx, hby))) { hsad = s.sad; hbx = s.xq; hby = s.yq; }
+            }
+        }
+    }
+    if (!hp) {
+        if (fast) {
+            const int qcx = hbx, qcy = hby;
+            int qx[5], qy[5], nq = 0;
+            qx[nq] = qcx; qy[nq] = qcy; ++nq;
+            const int o4[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+            for (int k = 0; k < 4; ++k) {
+                qx[nq] = clampi(qcx + o4[k][0], xmin, xmax); qy[nq] = clampi(qcy + o4[k][1], ymin, ymax); ++nq;
+            }
+            nsub = sub_add_batch(C, sp, nsub, qx, qy, nq);
+            int qbx = qcx, qby = qcy;
+            uint64_t qcost = C.L.sub[find_sub(C, nsub, qcx, qcy)].sad;
+            for (int i = 0; i < nq; ++i) {
+                const int j = find_sub(C, nsub, qx[i], qy[i]);
+                if (j >= 0) {
+                    const SubCand& s = C.L.sub[j];
+                    if (s.sad < qcost || (s.sad == qcost && tie_less(s.xq, s.yq, qbx, qby))) { qbx = s.xq; qby = s.yq; qcost = s.sad; }
+                }
+            }
+            int q2x[2], q2y[2], n2 = 0;
+            const int dx = qbx - qcx, dy = qby - qcy;
+            if (dx && !dy) {
+                q2x[n2] = qbx; q2y[n2] = clampi(qcy - 1, ymin, ymax); ++n2;
+                q2x[n2] = qbx; q2y[n2] = clampi(qcy + 1, ymin, ymax); ++n2;
+            } else if (dy && !dx) {
+                q2x[n2] = clampi(qcx - 1, xmin, xmax); q2y[n2] = qby; ++n2;
+                q2x[n2] = clampi(qcx + 1, xmin, xmax); q2y[n2] = qby; ++n2;
+            }
+            nsub = sub_add_batch(C, sp, nsub, q2x, q2y, n2);
+        } else {
+            int qx[9], qy[9], nq = 0;
+            for (int oy = -1; oy <= 1; ++oy)
+                for (int ox = -1; ox <= 1; ++ox) {
+                    qx[nq] = clampi(hbx + ox, xmin, xmax); qy[nq] = clampi(hby + oy, ymin, ymax); ++nq;
+                }
+            nsub = sub_add_batch(C, sp, nsub, qx, qy, nq);
+        }
+    }
+    // sort by (SAD, tie order) and keep the shortlist
+    wsync();
+    if (C.lane == 0) {
+        for (int i = 1; i < nsub; ++i) {
+            const SubCand x = C.L.sub[i];
+            int j = i - 1;
+            while (j >= 0 && sad_less(x, C.L.sub[j])) { C.L.sub[j + 1] = C.L.sub[j]; --j; }
+            C.L.sub[j + 1] = x;
+        }
+    }
+    wsync();
+    const uint64_t screen = C.L.sub[0].sad;
+    const int keep = fast ? 4 : 8;
+    if (nsub > keep) nsub = keep;
+    // rates and x87 scores
+    const uint64_t lm = J->lam_m;
+    const int lk = J->lam_k;
+    wsync();
+    if (C.lane == 0) {
+        for (int i = 0; i < nsub; ++i) {
+            SubCand& s = C.L.sub[i];
+            int prx = pi.px, pry = pi.py;
+            uint32_t rate = 0;
+            if (pi.hybrid) {
+                const bool use_a = modular_mv_distance(J->mvrange, s.xq, s.yq, pi.ax, pi.ay) <=
+                                   modular_mv_distance(J->mvrange, s.xq, s.yq, pi.cx, pi.cy);
+                prx = use_a ? pi.ax : pi.cx;
+                pry = use_a ? pi.ay : pi.cy;
+                ++rate;
+            }
+            rate += mv_rate_bits(J->mvrange, s.xq, s.yq, prx, pry, sp.mode, sp.more);
+            s.rate = rate;
+            s.score = x87_score(s.sad, rate, lm, lk);
+        }
+    }
+    wsync();
+    int rw = 0;
+    for (int i = 1; i < nsub; ++i) {
+        const SubCand& a = C.L.sub[i];
+        const SubCand& b = C.L.sub[rw];
+        bool less;
+        if (a.score != b.score) less = a.score < b.score;
+        else if (a.sad != b.sad) less = a.sad < b.sad;
+        else less = tie_less(a.xq, a.yq, b.xq, b.yq);
+        if (less) rw = i;
+    }
+    int2 result = make_int2(C.L.sub[rw].xq, C.L.sub[rw].yq);
+    if (J->quality == 2) {
+        const uint64_t rws = C.L.sub[rw].sad;
+        const uint64_t pct3 = (rws / 100) * 3 + (((rws % 100) * 3 + 99) / 100);
+        const uint64_t slack = pct3 > 4 ? pct3 : 4;
+        const uint64_t limit = rws > UINT64_MAX - slack ? UINT64_MAX : rws + slack;
+        for (int i = 0; i < nsub; ++i) {
+            const SubCand s = C.L.sub[i];
+            const bool el = s.sad < UINT64_MAX / 8 && s.sad <= limit;
+            uint64_t satd = kBadSad;
+            if (el) satd = satd_one(C, sp.bx0, sp.by0, sp.bw, sp.bh, s.xq, s.yq, sp.mode);
+            wsync();
+            if (C.lane == 0) {
+                C.L.sub[i].eligible = el ? 1u : 0u;
+                C.L.sub[i].satd = satd;
+                C.L.sub[i].score = x87_score(satd, s.rate, lm, lk);
+            }
+            wsync();
+        }
+        int w = 0;
+        for (int i = 1; i < nsub; ++i) {
+            const SubCand& a = C.L.sub[i];
+            const SubCand& b = C.L.sub[w];
+            bool less;
+            if (a.eligible != b.eligible) less = a.eligible > b.eligible;
+            else if (a.score != b.score) less = a.score < b.score;
+            else if (a.satd != b.satd) less = a.satd < b.satd;
+            else if (a.sad != b.sad) less = a.sad < b.sad;
+            else less = tie_less(a.xq, a.yq, b.xq, b.yq);
+            if (less) w = i;
+        }
+        result = make_int2(C.L.sub[w].xq, C.L.sub[w].yq);
+    }
+    *best_sad = screen;
+    return result;
+}
+
+// ------------------------------------------------------------------ the macroblock kernel
+
+// median4, src/encoder_motion_compensation.cpp:522-529
+__device__ int median4(int a, int b, int c, int d) {
+    if (a < b) {
+        if (c < d) return (imin(b, d) + imax(a, c)) / 2;
+        return (imin(b, c) + imax(a, d)) / 2;
+    }
+    if (c < d) return (imin(a, d) + imax(b, c)) / 2;
+    return (imin(a, c) + imax(b, d)) / 2;
+}
+
+__device__ void analyze_mb(Ctx& C) {
+    const JobDesc* J = C.J;
+    const int mode = J->mode;
+    const bool allow_4mv = mode == 3;
+    const int one_mode = allow_4mv ? 0 : mode;
+    const int range = J->range;
+    const int local_range = J->local_range;
+    const int pos = C.pos;
+
+    // current block to LDS, zero outside the active area
+    for (int i = C.lane; i < 256; i += kWave) {
+        const int x = i & 15, y = i >> 4;
+        C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
+    }
+    wsync();
+
+    uint64_t one_sad = 0;
+    int2 one = {0, 0};
+    double dl_mae = -1.0, dc_mae = -1.0;
+    int decision_out = 0;
+    if (range == 0) {
+        // zero-range mode: zero vectors with their SAD (src/encoder_motion.cpp:73-86)
+        win_load(C, C.x0, C.y0, 16, 16);
+        const uint64_t z = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; }, nullptr);
+        if (C.lane == 0) {
+            J->screen[pos] = key_cost(z);
+            J->chosen[pos] = key_cost(z);
+            J->temporal[pos] = key_cost(z);
+            J->mvs[pos] = make_int2(0, 0);
+            for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = make_int2(0, 0);
+            J->flags[pos] = 0;
+            J->ratio[pos] = -1.0;
+            J->dl_mae[pos] = -1.0;
+            J->dc_mae[pos] = -1.0;
+            J->decision[pos] = 0;
+        }
+        return;
+    }
+
+    if (C.lane < 4) C.L.own[C.lane] = make_int2(0, 0);
+    wsync();
+    const Pred rate_pi = predictor_mixed_1mv(C);
+    {
+        const Pred pi = predictor_info(C);
+        const int gdx = J->g->dx, gdy = J->g->dy;
+        IRes im;
+        bool selected_distant = false;
+        const uint64_t px = active_pixels(C);
+        if (J->lr_mode == 2) {
+            bool sc = false;
+            if (range > local_range) {
+                const IRes pr = propagated(C, range);
+                if (match_good(C, pr.sad)) { im = pr; sc = true; selected_distant = true; }
+            }
+            if (!sc) im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
+        } else {
+            im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
+            const uint64_t local_sad = im.sad;
+            if (px) dl_mae = static_cast<double>(local_sad) / static_cast<double>(px);
+            const bool skip = J->lr_mode == 1 && match_good(C, im.sad);
+            if (range > local_range && !skip) {
+                const IRes pr = propagated(C, range);
+                const int dec = distant_decision(C, pr.sad, local_sad);
+                if (px && pr.sad < UINT64_MAX / 8) {
+                    const double mae = static_cast<double>(pr.sad) / static_cast<double>(px);
+                    if (dc_mae < 0.0 || mae < dc_mae) { dc_mae = mae; decision_out = dec; }
+                }
+                if (key_int(pr.sad == UINT64_MAX ? kBadSad + 1 : pr.sad, pr.dx, pr.dy) < key_int(im.sad, im.dx, im.dy) &&
+                    pr.sad != UINT64_MAX && dec >= 2) {
+                    im = pr;
+                    selected_distant = true;
+                }
+            }
+        }
+        if (range > local_range && !match_good(C, im.sad)) {
+            uint64_t ext_sad = UINT64_MAX;
+            int ext_dec = 0;
+            const IRes ext = extended_content(C, range, im, &ext_sad, &ext_dec);
+            if (J->lr_mode != 2 && px && ext_sad < UINT64_MAX / 8) {
+                const double mae = static_cast<double>(ext_sad) / static_cast<double>(px);
+                if (dc_mae < 0.0 || mae < dc_mae) { dc_mae = mae; decision_out = ext_dec; }
+            }
+            if (ext.dx != im.dx || ext.dy != im.dy) { im = ext; selected_distant = true; }
+        }
+        if (C.lane == 0) J->prop[pos] = selected_distant ? 1 : 0;
+        SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true};
+        one = refine_staged(C, sp, im.dx * 4, im.dy * 4, rate_pi, &one_sad, im.sad);
+    }
+    if (hidden_padding(C.g, C.x0, C.y0, 16, 16, one.x, one.y, one_mode, true)) {
+        one = make_int2(0, 0);
+        wsync();
+        if (C.lane == 0) C.L.req[0] = one;
+        wsync();
+        qpel_sad_batch(C, C.x0, C.y0, 16, 16, 1, one_mode);
+        one_sad = C.L.reqcost[0];
+    }
+    const uint64_t one_screen = one_sad;
+    int2 outmv = one;
+    int2 outb[4] = {one, one, one, one};
+    uint8_t flags = 0;
+    wsync();
+    if (C.lane < 4) C.L.own[C.lane] = one;
+    wsync();
+    const double lambda = J->lambda;
+    uint64_t chosen_sad = one_sad;
+    uint64_t temporal_screen = one_screen;
+    if (allow_4mv) {
+        int2 four[4] = {one, one, one, one};
+        uint64_t four_sad = 0, four_screen = 0, extra_bits = 1;
+        const Bounds qb = bounds_qpel(range);
+        const int limit = 4 * J->subpel_range;
+        for (int k = 0; k < 4; ++k) {
+            const Pred pi4 = predictor_4mv(C, k);
+            const int bx = C.x0 + (k & 1) * 8, by = C.y0 + ((k >> 1) & 1) * 8;
+            auto within = [limit](int2 m) {
+                if (imax(iabs(m.x), iabs(m.y)) > limit) { m.x = (m.x / 4) * 4; m.y = (m.y / 4) * 4; }
+                return m;
+            };
+            // the eleven seed SADs, scored together, then replayed in order
+            int2 cand[11];
+            cand[0] = within(one);
+            const int2 raw[10] = {one, make_int2(pi4.px, pi4.py),
+                                  make_int2(one.x - 8, one.y), make_int2(one.x + 8, one.y),
+                                  make_int2(one.x, one.y - 8), make_int2(one.x, one.y + 8),
+                                  make_int2(one.x - 4, one.y), make_int2(one.x + 4, one.y),
+                                  make_int2(one.x, one.y - 4), make_int2(one.x, one.y + 4)};
+            for (int s = 0; s < 10; ++s) {
+                int2 sd = make_int2(clampi(raw[s].x, qb.xmin, qb.xmax), clampi(raw[s].y, qb.ymin, qb.ymax));
+                cand[s + 1] = within(sd);
+            }
+            // unique vectors -> req
+            int nreq = 0;
+            for (int s = 0; s < 11; ++s) {
+                bool dup = false;
+                for (int j = 0; j < nreq; ++j)
+                    if (C.L.req[j].x == cand[s].x && C.L.req[j].y == cand[s].y) dup = true;
+                if (dup) continue;
+                wsync();
+                if (C.lane == 0) C.L.req[nreq] = cand[s];
+                wsync();
+                ++nreq;
+            }
+            qpel_sad_batch(C, bx, by, 8, 8, nreq, 0);
+            auto cost_of = [&](int2 v) -> uint64_t {
+                for (int j = 0; j < nreq; ++j)
+                    if (C.L.req[j].x == v.x && C.L.req[j].y == v.y) return C.L.reqcost[j];
+                return kBadSad;
+            };
+            int2 best_seed = cand[0];
+            uint64_t seed_cost = cost_of(best_seed);
+            for (int s = 1; s < 11; ++s) {
+                const int2 sd = cand[s];
+                if (sd.x == best_seed.x && sd.y == best_seed.y) continue;
+                const uint64_t cc = cost_of(sd);
+                const int cm = iabs(sd.x) + iabs(sd.y), bm = iabs(best_seed.x) + iabs(best_seed.y);
+                if (cc < seed_cost || (cc == seed_cost && cm < bm)) { seed_cost = cc; best_seed = sd; }
+            }
+            uint64_t bsad = 0;
+            SubParams sp{bx, by, 8, 8, 3, range, true};
+            const int2 mv = refine_staged(C, sp, best_seed.x, best_seed.y, pi4, &bsad, seed_cost);
+            four_screen += bsad;
+            four[k] = mv;
+            wsync();
+            if (C.lane == 0) C.L.own[k] = mv;
+            wsync();
+            four_sad += bsad;
+            int apx = pi4.px, apy = pi4.py;
+            if (pi4.hybrid) {
+                const int da = modular_mv_distance(J->mvrange, mv.x, mv.y, pi4.ax, pi4.ay);
+                const int dc = modular_mv_distance(J->mvrange, mv.x, mv.y, pi4.cx, pi4.cy);
+                apx = da <= dc ? pi4.ax : pi4.cx;
+                apy = da <= dc ? pi4.ay : pi4.cy;
+                ++extra_bits;
+            }
+            if (!(mv.x == apx && mv.y == apy)) extra_bits += mvdiff_bits(J->mvrange, mv.x - apx, mv.y - apy, false, false);
+        }
+        const int cmx = median4(four[0].x, four[1].x, four[2].x, four[3].x);
+        const int cmy = median4(four[0].y, four[1].y, four[2].y, four[3].y);
+        const bool chroma_safe = !hidden_chroma(C.g, C.x0, C.y0, 16, 16, cmx, cmy);
+        bool choose4 = false;
+        if (chroma_safe) {
+            const double lb = __dmul_rn(lambda, static_cast<double>(extra_bits));
+            const double os = __dmul_rn(static_cast<double>(one_sad), 0.985);
+            choose4 = four_sad + static_cast<uint64_t>(llround_exact(lb)) + 8 < static_cast<uint64_t>(llround_exact(os));
+        }
+        if (choose4) {
+            for (int k = 0; k < 4; ++k) outb[k] = four[k];
+            flags |= kFlag4mvChosen;
+            outmv = four[0];
+            chosen_sad = four_sad;
+            temporal_screen = four_screen;
+        } else {
+            for (int k = 0; k < 4; ++k) outb[k] = one;
+            outmv = one;
+            temporal_screen = one_screen;
+        }
+    }
+    const uint64_t temporal_sad = chosen_sad;
+    double ratio = -1.0;
+    if (J->allow_intra) {
+        // visible_block_mean (src/encoder_entropy.cpp:496-509) per 8x8 block and the DC-activity proxy
+        uint64_t intra_sad = 0;
+        const int lx = C.lane & 7, ly = C.lane >> 3;
+        for (int k = 0; k < 4; ++k) {
+            const int bx = C.x0 + (k & 1) * 8, by = C.y0 + ((k >> 1) & 1) * 8;
+            const bool in = bx + lx < C.g.W && by + ly < C.g.H;
+            const uint32_t v = in ? J->cy[static_cast<size_t>(by + ly) * C.g.W + bx + lx] : 0u;
+            const uint32_t sum = wave_sum32(v);
+            const uint32_t n = wave_sum32(in ? 1u : 0u);
+            const int mean = n ? static_cast<int>((sum + n / 2) / n) : 128;
+            intra_sad += wave_sum32(in ? static_cast<uint32_t>(iabs(static_cast<int>(v) - mean)) : 0u);
+        }
+        const uint64_t intra_rate = static_cast<uint64_t>(llround_exact(__dmul_rn(lambda, 72.0)));
+        ratio = temporal_screen > 0 ? __ddiv_rn(static_cast<double>(intra_sad + intra_rate), static_cast<double>(temporal_screen))
+                                    : __builtin_huge_val();
+        const double thr = __dmul_rn(static_cast<double>(temporal_screen), J->intra_thr);
+        if (intra_sad + intra_rate < static_cast<uint64_t>(llround_exact(thr))) {
+            flags |= kFlagIntra;
+            outmv = make_int2(0, 0);
+            for (int k = 0; k < 4; ++k) outb[k] = make_int2(0, 0);
+            chosen_sad = intra_sad;
+        }
+    }
+    if (C.lane == 0) {
+        J->mvs[pos] = outmv;
+        for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = outb[k];
+        J->flags[pos] = flags;
+        J->screen[pos] = temporal_screen;
+        J->ratio[pos] = ratio;
+        J->dl_mae[pos] = dl_mae;
+        J->dc_mae[pos] = dc_mae;
+        J->decision[pos] = static_cast<uint8_t>(decision_out);
+        J->chosen[pos] = chosen_sad;
+        J->temporal[pos] = temporal_sad;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_mb_diagonal(const JobDesc* __restrict__ jobs, int diag) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const JobDesc* J = &jobs[blockIdx.y];
+    const int mbw = J->mbw, mbh = J->mbh;
+    if (diag > J->max_diag) return;
+    // macroblocks on the anti-diagonal mx + 2*my == diag
+    const int my_lo = imax(0, (diag - (mbw - 1) + 1) / 2);
+    const int my = my_lo + static_cast<int>(blockIdx.x);
+    if (my > imin(mbh - 1, diag / 2)) return;
+    const int mx = diag - 2 * my;
+    if (mx < 0 || mx >= mbw) return;
+    Ctx C;
+    C.J = J;
+    C.g = {J->W, J->H, J->vw, J->vh, mbw, mbh};
+    C.ry = J->ry;
+    C.mx = mx; C.my = my; C.x0 = mx * 16; C.y0 = my * 16;
+    C.aw = imax(0, imin(16, J->W - C.x0));
+    C.ah = imax(0, imin(16, J->H - C.y0));
+    C.pos = my * mbw + mx;
+    C.lane = threadIdx.x;
+    C.L = carve(smem, J->local_range);
+    C.winCap = win_capacity(J->local_range);
+    C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
+    analyze_mb(C);
+}
+
+// ------------------------------------------------------------------ global search
+
+// Frame SAD partial sums (frame_sad, src/encoder_motion.cpp:1565-1575):
+// block = (row chunk, point, job); 256 threads, 8 sampled rows per block.
+__global__ void __launch_bounds__(256) k_frame_sads(const JobDesc* __restrict__ jobs) {
+    const JobDesc* J = &jobs[blockIdx.z];
+    GlobalState* g = J->g;
+    const int p = blockIdx.y;
+    if (p >= g->n) return;
+    const int s = g->stride;
+    const int dx = g->pts[2 * p], dy = g->pts[2 * p + 1];
+    const int W = J->W, H = J->H;
+    const int rows = (H + s - 1) / s, cols = (W + s - 1) / s;
+    const int r0 = blockIdx.x * 8;
+    if (r0 >= rows) return;
+    uint32_t acc = 0;
+    for (int r = r0; r < imin(rows, r0 + 8); ++r) {
+        const int y = r * s;
+        const int sy = clampi(y + dy, 0, H - 1);
+        const uint8_t* cr = J->cy + static_cast<size_t>(y) * W;
+        const uint8_t* rr = J->ry + static_cast<size_t>(sy) * W;
+        for (int c = threadIdx.x; c < cols; c += 256) {
+            const int x = c * s;
+            acc += static_cast<uint32_t>(iabs(static_cast<int>(cr[x]) - static_cast<int>(rr[clampi(x + dx, 0, W - 1)])));
+        }
+    }
+    // block reduction
+    __shared__ uint32_t red[4];
+    acc = wave_sum32(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t t = static_cast<uint64_t>(red[0]) + red[1] + red[2] + red[3];
+        atomicAdd(reinterpret_cast<unsigned long long*>(&g->cost[p]), static_cast<unsigned long long>(t));
+    }
+}
+
+__device__ bool g_better(uint64_t ac, int ax, int ay, uint64_t bc, int bx, int by) {
+    if (ac != bc) return ac < bc;
+    const int am = iabs(ax) + iabs(ay), bm = iabs(bx) + iabs(by);
+    if (am != bm) return am < bm;
+    if (ay != by) return ay < by;
+    return ax < bx;
+}
+
+// sorted insertion of the extended search's consider (src/encoder_motion.cpp:2036-2041)
+__device__ void g_consider(GlobalState* g, uint64_t cost, int dx, int dy) {
+    for (int i = 0; i < g->tn; ++i)
+        if (g->tdx[i] == dx && g->tdy[i] == dy) {
+            if (cost < g->tcost[i]) g->tcost[i] = cost;
+            return;
+        }
+    g->tcost[g->tn] = cost; g->tdx[g->tn] = dx; g->tdy[g->tn] = dy; g->tn++;
+    for (int i = 1; i < g->tn; ++i) {
+        const uint64_t c = g->tcost[i];
+        const int x = g->tdx[i], y = g->tdy[i];
+        int j = i - 1;
+        while (j >= 0 && g_better(c, x, y, g->tcost[j], g->tdx[j], g->tdy[j])) {
+            g->tcost[j + 1] = g->tcost[j]; g->tdx[j + 1] = g->tdx[j]; g->tdy[j + 1] = g->tdy[j];
+            --j;
+        }
+        g->tcost[j + 1] = c; g->tdx[j + 1] = x; g->tdy[j + 1] = y;
+    }
+    if (g->tn > 8) g->tn = 8;
+}
+
+__device__ void g_set_stage(GlobalState* g, int stride) {
+    g->stride = stride;
+    for (int i = 0; i < g->n; ++i) g->cost[i] = 0;
+}
+
+__device__ int g_range(const JobDesc* J) { return imin(J->local_range, 32); }
+
+// Stage 0: the first point list.  Extended search (range 32): cross and
+// multi-hex at stride 16; grid search (range 1..31): coarse grid at stride 4.
+__global__ void k_global_begin(const JobDesc* __restrict__ jobs, int njobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const JobDesc* J = &jobs[j];
+    GlobalState* g = J->g;
+    g->tn = 0;
+    g->n = 0;
+    g->dx = 0; g->dy = 0; g->full_sad = 0;
+    if (J->range == 0) return;
+    const int R = g_range(J);
+    if (R <= 0) {
+        g->pts[0] = 0; g->pts[1] = 0; g->n = 1;
+        g_set_stage(g, 1);
+        return;
+    }
+    int n = 0;
+    auto push = [&](int dx, int dy) { g->pts[2 * n] = dx; g->pts[2 * n + 1] = dy; ++n; };
+    if (R > 31) {
+        push(0, 0);
+        const int step = R >= 160 ? 16 : 8;
+        for (int d = step; d <= R; d += step) { push(-d, 0); push(d, 0); push(0, -d); push(0, d); }
+        for (int r = step; r <= R; r += step) {
+            const int h = (r + 1) / 2;
+            push(-r, -h); push(-r, h); push(r, -h); push(r, h);
+            push(-h, -r); push(h, -r); push(-h, r); push(h, r);
+        }
+        for (int i = 0; i < n; ++i) {
+            g->pts[2 * i] = clampi(g->pts[2 * i], -R, R);
+            g->pts[2 * i + 1] = clampi(g->pts[2 * i + 1], -R, R);
+        }
+        g->n = n;
+        g_set_stage(g, 16);
+    } else {
+        const int cs = R >= 4 ? 4 : (R >= 2 ? 2 : 1);
+        for (int dy = -R; dy <= R; dy += cs)
+            for (int dx = -R; dx <= R; dx += cs) push(dx, dy);
+        push(0, 0);
+        for (int d = -R; d <= R; d += cs) { push(R, d); push(d, R); }
+        g->n = n;
+        g_set_stage(g, 4);
+    }
+}
+
+// Replay the sequential bookkeeping of one stage on the computed costs and
+// emit the next stage (src/encoder_motion.cpp:2043-2073 and :2094-2140).
+__global__ void k_global_step(const JobDesc* __restrict__ jobs, int njobs, int stage) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const JobDesc* J = &jobs[j];
+    GlobalState* g = J->g;
+    if (J->range == 0 || g->n == 0) return;
+    const int R = g_range(J);
+    const int s = g->stride;
+    const uint64_t mul = static_cast<uint64_t>(s) * s;
+    if (R <= 0) {
+        g->dx = 0; g->dy = 0; g->full_sad = g->cost[0];
+        g->n = 0;
+        return;
+    }
+    if (R > 31) {
+        const int step = R >= 160 ? 16 : 8;
+        if (stage == 0 || stage == 1 || stage == 2) {
+            for (int i = 0; i < g->n; ++i) g_consider(g, g->cost[i] * (s > 1 ? mul : 1), g->pts[2 * i], g->pts[2 * i + 1]);
+            int n = 0;
+            if (stage == 0) {
+                // densify the best four of the coarse list at stride 8
+                const int tn = g->tn;
+                int cx[4], cy[4];
+                for (int i = 0; i < imin(4, tn); ++i) { cx[i] = g->tdx[i]; cy[i] = g->tdy[i]; }
+                for (int i = 0; i < imin(4, tn); ++i)
+                    for (int oy = -step; oy <= step; oy += 4)
+                        for (int ox = -step; ox <= step; ox += 4) {
+                            g->pts[2 * n] = clampi(cx[i] + ox, -R, R);
+                            g->pts[2 * n + 1] = clampi(cy[i] + oy, -R, R);
+                            ++n;
+                        }
+                g->n = n;
+                g_set_stage(g, 8);
+            } else if (stage == 1) {
+                const int mdx = g->tdx[0], mdy = g->tdy[0];
+                for (int oy = -4; oy <= 4; ++oy)
+                    for (int ox = -4; ox <= 4; ++ox) {
+                        g->pts[2 * n] = clampi(mdx + ox, -R, R);
+                        g->pts[2 * n + 1] = clampi(mdy + oy, -R, R);
+                        ++n;
+                    }
+                g->n = n;
+                g_set_stage(g, 4);
+            } else {
+                const int fdx = g->tdx[0], fdy = g->tdy[0];
+                for (int oy = -1; oy <= 1; ++oy)
+                    for (int ox = -1; ox <= 1; ++ox) {
+                        g->pts[2 * n] = clampi(fdx + ox, -R, R);
+                        g->pts[2 * n + 1] = clampi(fdy + oy, -R, R);
+                        ++n;
+                    }
+                g->n = n;
+                g_set_stage(g, 1);
+            }
+        } else {
+            uint64_t bc = UINT64_MAX;
+            int bx = 0, by = 0;
+            for (int i = 0; i < g->n; ++i)
+                if (g_better(g->cost[i], g->pts[2 * i], g->pts[2 * i + 1], bc, bx, by)) {
+                    bc = g->cost[i]; bx = g->pts[2 * i]; by = g->pts[2 * i + 1];
+                }
+            g->dx = bx; g->dy = by; g->full_sad = bc;
+            g->n = 0;
+        }
+        return;
+    }
+    // grid path (range <= 31)
+    const int cs = R >= 4 ? 4 : (R >= 2 ? 2 : 1);
+    if (stage == 0) {
+        // sorted top-8 of every coarse point (push, sort, truncate)
+        g->tn = 0;
+        for (int i = 0; i < g->n; ++i) {
+            const uint64_t c = g->cost[i] * mul;
+            const int x = g->pts[2 * i], y = g->pts[2 * i + 1];
+            g->tcost[g->tn] = c; g->tdx[g->tn] = x; g->tdy[g->tn] = y; g->tn++;
+            for (int a = g->tn - 1; a > 0 && g_better(g->tcost[a], g->tdx[a], g->tdy[a], g->tcost[a - 1], g->tdx[a - 1], g->tdy[a - 1]); --a) {
+                const uint64_t tc = g->tcost[a]; g->tcost[a] = g->tcost[a - 1]; g->tcost[a - 1] = tc;
+                const int tx = g->tdx[a]; g->tdx[a] = g->tdx[a - 1]; g->tdx[a - 1] = tx;
+                const int ty = g->tdy[a]; g->tdy[a] = g->tdy[a - 1]; g->tdy[a - 1] = ty;
+            }
+            if (g->tn > 8) g->tn = 8;
+        }
+        // neighbourhoods at stride 2 in first-visit order (63x63 seen table)
+        uint64_t seen[63 * 63 / 64 + 1];
+        for (int i = 0; i < 63 * 63 / 64 + 1; ++i) seen[i] = 0;
+        int n = 0;
+        for (int t = 0; t < g->tn; ++t) {
+            const int cx = g->tdx[t], cy = g->tdy[t];
+            for (int dy = imax(-R, cy - cs + 1); dy <= imin(R, cy + cs - 1); ++dy)
+                for (int dx = imax(-R, cx - cs + 1); dx <= imin(R, cx + cs - 1); ++dx) {
+                    const int key = (dy + 31) * 63 + (dx + 31);
+                    if ((seen[key >> 6] >> (key & 63)) & 1) continue;
+                    seen[key >> 6] |= 1ull << (key & 63);
+                    if (n < kGlobalMaxPts) {
+                        g->pts[2 * n] = dx; g->pts[2 * n + 1] = dy;
+                    }
+                    ++n;
+                }
+        }
+        if (n > kGlobalMaxPts) { atomicMax(J->err, 7); n = kGlobalMaxPts; }
+        g->n = n;
+        g_set_stage(g, 2);
+    } else if (stage == 1) {
+        uint64_t bc = UINT64_MAX;
+        int bx = 0, by = 0;
+        for (int i = 0; i < g->n; ++i) {
+            const uint64_t c = g->cost[i] * mul;
+            const int x = g->pts[2 * i], y = g->pts[2 * i + 1];
+            const int cm = iabs(x) + iabs(y), bm = iabs(bx) + iabs(by);
+            if (c < bc || (c == bc && cm < bm)) { bc = c; bx = x; by = y; }
+        }
+        g->pts[0] = bx; g->pts[1] = by;
+        g->n = 1;
+        g_set_stage(g, 1);
+    } else {
+        g->dx = g->pts[0]; g->dy = g->pts[1]; g->full_sad = g->cost[0];
+        g->n = 0;
+    }
+}
+
+// ------------------------------------------------------------------ long-range index
+
+// long_range_signature of every 8-px lattice cell of the reference
+// (src/encoder_motion.cpp:1578-1625), packed 6 bytes per cell.
+__global__ void k_signatures(const JobDesc* __restrict__ jobs, int max_cells) {
+    const JobDesc* J = &jobs[blockIdx.y];
+    if (J->range <= J->local_range || J->range == 0) return;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    const int total = J->ncx * J->ncy;
+    if (cell >= total) return;
+    const int x0 = (cell % J->ncx) * 8, y0 = (cell / J->ncx) * 8;
+    const int W = J->W, H = J->H, cw = W / 2, ch = H / 2;
+    auto mean = [](const uint8_t* p, int pw, int ph, int bx, int by) -> uint32_t {
+        uint32_t sum = 0, n = 0;
+        for (int y = 0; y < 8 && by + y < ph; ++y) {
+            if (by + y < 0) continue;
+            for (int x = 0; x < 8 && bx + x < pw; ++x) {
+                if (bx + x < 0) continue;
+                sum += p[static_cast<size_t>(by + y) * pw + bx + x];
+                ++n;
+            }
+        }
+        return n ? (sum + n / 2) / n : 128u;
+    };
+    uint64_t s = 0;
+    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0, y0));
+    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0 + 8, y0)) << 8;
+    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0, y0 + 8)) << 16;
+    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0 + 8, y0 + 8)) << 24;
+    s |= static_cast<uint64_t>(mean(J->ru, cw, ch, x0 / 2, y0 / 2)) << 32;
+    s |= static_cast<uint64_t>(mean(J->rv, cw, ch, x0 / 2, y0 / 2)) << 40;
+    J->sig[cell] = s;
+}
+
+// ------------------------------------------------------------------ batch scorer
+
+// One lane per candidate (src/shaders/vulkan_motion_cost.comp:178-183):
+// sampled SAD or SATD4x4 of one block, clamped fetch, 32-bit accumulation.
+__global__ void __launch_bounds__(64) k_score_batch(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ ref,
+                                                    int width, int height, int x0, int y0, int bw, int bh, int ss,
+                                                    int bilinear, int rnd, int metric, const int2* __restrict__ cands,
+                                                    int n, uint32_t* __restrict__ costs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 mv = cands[i];
+    const int ix = floor_div4(mv.x), iy = floor_div4(mv.y);
+    const int hm = mv.x - ix * 4, vm = mv.y - iy * 4;
+    const int aw = imax(0, imin(bw, width - x0)), ah = imax(0, imin(bh, height - y0));
+    auto S = [&](int x, int y) {
+        return static_cast<int>(ref[static_cast<size_t>(clampi(y, 0, height - 1)) * width + clampi(x, 0, width - 1)]);
+    };
+    auto pred = [&](int x, int y) -> int {
+        const int sx = x + ix, sy = y + iy;
+        if (bilinear) {
+            const int a = S(sx, sy);
+            if (!hm && !vm) return a;
+            const int b = S(sx + 1, sy), c = S(sx, sy + 1);
+            if (hm && vm) return (a + b + c + S(sx + 1, sy + 1) + (rnd ? 1 : 2)) >> 2;
+            if (hm) return (a + b + (rnd ? 0 : 1)) >> 1;
+            return (a + c + (rnd ? 0 : 1)) >> 1;
+        }
+        if (!hm && !vm) return S(sx, sy);
+        int v;
+        if (hm && vm) {
+            const int shift = ((hm == 2 ? 1 : 5) + (vm == 2 ? 1 : 5)) >> 1;
+            const int r1 = (1 << (shift - 1)) + (rnd ? 1 : 0) - 1;
+            int t[4];
+            for (int k = -1; k <= 2; ++k)
+                t[k + 1] = arshift(mspel(S(sx + k, sy - 1), S(sx + k, sy), S(sx + k, sy + 1), S(sx + k, sy + 2), vm) + r1, shift);
+            v = arshift(mspel(t[0], t[1], t[2], t[3], hm) + 64 - (rnd ? 1 : 0), 7);
+        } else if (vm) {
+            v = arshift(mspel(S(sx, sy - 1), S(sx, sy), S(sx, sy + 1), S(sx, sy + 2), vm) + (vm == 2 ? 7 + (rnd ? 1 : 0) : 31 + (rnd ? 1 : 0)),
+                        vm == 2 ? 4 : 6);
+        } else {
+            v = arshift(mspel(S(sx - 1, sy), S(sx, sy), S(sx + 1, sy), S(sx + 2, sy), hm) + (hm == 2 ? 8 - (rnd ? 1 : 0) : 32 - (rnd ? 1 : 0)),
+                        hm == 2 ? 4 : 6);
+        }
+        return clampi(v, 0, 255);
+    };
+    uint32_t total = 0;
+    if (metric == 0) {
+        const int s = imax(ss, 1);
+        for (int y = 0; y < ah; y += s)
+            for (int x = 0; x < aw; x += s)
+                total += static_cast<uint32_t>(iabs(static_cast<int>(cur[static_cast<size_t>(y0 + y) * width + x0 + x]) - pred(x0 + x, y0 + y)));
+        if (s > 1) total *= static_cast<uint32_t>(s * s);
+    } else {
+        for (int by = 0; by < bh; by += 4)
+            for (int bx = 0; bx < bw; bx += 4) {
+                if (!(bx < aw && by < ah)) continue;
+                int d[16];
+                for (int yy = 0; yy < 4; ++yy)
+                    for (int xx = 0; xx < 4; ++xx) {
+                        const int px = bx + xx, py = by + yy;
+                        d[yy * 4 + xx] = (px < aw && py < ah)
+                                             ? static_cast<int>(cur[static_cast<size_t>(y0 + py) * width + x0 + px]) - pred(x0 + px, y0 + py)
+                                             : 0;
+                    }
+                int t[16];
+                for (int y = 0; y < 4; ++y) {
+                    const int a0 = d[y * 4] + d[y * 4 + 3], a1 = d[y * 4 + 1] + d[y * 4 + 2];
+                    const int a2 = d[y * 4 + 1] - d[y * 4 + 2], a3 = d[y * 4] - d[y * 4 + 3];
+                    t[y * 4] = a0 + a1; t[y * 4 + 1] = a3 + a2; t[y * 4 + 2] = a0 - a1; t[y * 4 + 3] = a3 - a2;
+                }
+                uint32_t tile = 0;
+                for (int x = 0; x < 4; ++x) {
+                    const int a0 = t[x] + t[12 + x], a1 = t[4 + x] + t[8 + x];
+                    const int a2 = t[4 + x] - t[8 + x], a3 = t[x] - t[12 + x];
+                    tile += static_cast<uint32_t>(iabs(a0 + a1) + iabs(a3 + a2) + iabs(a0 - a1) + iabs(a3 - a2));
+                }
+                total += (tile + 1) >> 1;
+            }
+    }
+    costs[i] = total;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+
+int mb_kernel_lds_bytes(int local_range) { return lds_bytes_for(local_range); }
+
+hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc* h_jobs, hipStream_t s,
+                                hipEvent_t ev0, hipEvent_t ev1) {
+    // per-stage upper bounds of point count and sampled-row chunks over the batch
+    int pts[4] = {0, 0, 0, 0}, chunks[4] = {0, 0, 0, 0};
+    for (int i = 0; i < njobs; ++i) {
+        const JobDesc& J = h_jobs[i];
+        if (J.range == 0) continue;
+        const int R = J.local_range < 32 ? J.local_range : 32;
+        int np[4] = {0, 0, 0, 0}, st[4] = {1, 1, 1, 1};
+        if (R <= 0) {
+            np[0] = 1;
+        } else if (R > 31) {
+            const int step = R >= 160 ? 16 : 8;
+            np[0] = 1 + 12 * (R / step); st[0] = 16;
+            np[1] = 4 * (2 * step / 4 + 1) * (2 * step / 4 + 1); st[1] = 8;
+            np[2] = 81; st[2] = 4;
+            np[3] = 9; st[3] = 1;
+        } else {
+            const int cs = R >= 4 ? 4 : (R >= 2 ? 2 : 1);
+            const int side = 2 * R / cs + 1;
+            np[0] = side * side + 1 + 2 * side; st[0] = 4;
+            np[1] = 8 * (2 * cs - 1) * (2 * cs - 1); st[1] = 2;
+            np[2] = 1; st[2] = 1;
+        }
+        for (int k = 0; k < 4; ++k) {
+            if (np[k] > kGlobalMaxPts) return hipErrorInvalidValue;
+            if (!np[k]) continue;
+            pts[k] = np[k] > pts[k] ? np[k] : pts[k];
+            const int rows = (J.H + st[k] - 1) / st[k];
+            const int c = (rows + 7) / 8;
+            chunks[k] = c > chunks[k] ? c : chunks[k];
+        }
+    }
+    const int tb = 64, gb = (njobs + tb - 1) / tb;
+    if (ev0) (void)hipEventRecord(ev0, s);
+    hipLaunchKernelGGL(k_global_begin, dim3(gb), dim3(tb), 0, s, d_jobs, njobs);
+    // up to four stages; each: frame SADs of the stage's points, then the replay
+    for (int stage = 0; stage < 4; ++stage) {
+        if (!pts[stage]) continue;
+        hipLaunchKernelGGL(k_frame_sads, dim3(chunks[stage], pts[stage], njobs), dim3(256), 0, s, d_jobs);
+        hipLaunchKernelGGL(k_global_step, dim3(gb), dim3(tb), 0, s, d_jobs, njobs, stage);
+    }
+    if (ev1) (void)hipEventRecord(ev1, s);
+    return hipGetLastError();
+}
+
+hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hipStream_t s) {
+    if (max_cells <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_signatures, dim3((max_cells + 255) / 256, njobs), dim3(256), 0, s, d_jobs, max_cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_mb_diagonal, dim3(max_per_diag, njobs), dim3(64), lds_bytes, s, d_jobs, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_score_batch(const uint8_t* cur, const uint8_t* ref, int width, int height, int x0, int y0, int bw,
+                              int bh, int sample_stride, int bilinear, int rnd, int metric, const int2* cands, int n,
+                              uint32_t* costs, hipStream_t s) {
+    hipLaunchKernelGGL(k_score_batch, dim3((n + 63) / 64), dim3(64), 0, s, cur, ref, width, height, x0, y0, bw, bh,
+                       sample_stride, bilinear, rnd, metric, cands, n, costs);
+    return hipGetLastError();
+}
+
+}  // namespace vc1me
