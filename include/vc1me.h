@@ -212,6 +212,23 @@ int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
  * like the reference's SpeedProfiler, src/encoder_internal.h:182-221). */
 int vc1me_set_profiling(vc1me_ctx* ctx, int on);
 
+/* Algorithmic work of the last step, summed over its jobs, in the units of
+ * the roofline: one |a-b| accumulate per sampled pixel of a block SAD
+ * (sad_px), one 4-tap filter application per interpolated sample (1D = 1,
+ * separable 2D = 5; interp_taps), 112 per 4x4 Hadamard tile (satd_ops), one
+ * long-range signature comparison per index cell (sig_ops), one |a-b| per
+ * sampled pixel of a frame SAD (frame_px).  These are the operations the
+ * reference algorithm performs on the same inputs.  No reference
+ * counterpart (profiling aid). */
+typedef struct vc1me_work {
+    uint64_t sad_px;
+    uint64_t interp_taps;
+    uint64_t satd_ops;
+    uint64_t sig_ops;
+    uint64_t frame_px;
+} vc1me_work;
+int vc1me_last_work(vc1me_ctx* ctx, vc1me_work* out);
+
 /* ---- counters (VulkanComputeContext::counters / reset_counters,
  * src/vulkan_compute.h:53-54) */
 int vc1me_counters_get(vc1me_ctx* ctx, vc1me_counters* out);

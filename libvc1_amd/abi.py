@@ -87,6 +87,11 @@ class Timing(C.Structure):
                 ("global_kernel_launches", C.c_uint64)]
 
 
+class Work(C.Structure):
+    _fields_ = [("sad_px", C.c_uint64), ("interp_taps", C.c_uint64), ("satd_ops", C.c_uint64),
+                ("sig_ops", C.c_uint64), ("frame_px", C.c_uint64)]
+
+
 class Job(C.Structure):
     _fields_ = [("cur", C.c_void_p), ("ref", C.c_void_p), ("params", C.POINTER(Params)),
                 ("out", C.POINTER(PResult))]

@@ -84,6 +84,7 @@ struct JobSlot {
     uint64_t* sig = nullptr;
     GlobalState* g = nullptr;
     int32_t* err = nullptr;
+    unsigned long long* ops = nullptr;
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -109,6 +110,7 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_sig = carve(std::max<size_t>(cells, 1) * 8);
     const size_t o_g = carve(sizeof(GlobalState));
     const size_t o_err = carve(sizeof(int32_t));
+    const size_t o_ops = carve(8 * sizeof(unsigned long long));
     hipError_t e = hipMalloc(&s.mem, off);
     if (e != hipSuccess) { s.mem = nullptr; return e; }
     uint8_t* b = static_cast<uint8_t*>(s.mem);
@@ -127,6 +129,7 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.sig = reinterpret_cast<uint64_t*>(b + o_sig);
     s.g = reinterpret_cast<GlobalState*>(b + o_g);
     s.err = reinterpret_cast<int32_t*>(b + o_err);
+    s.ops = reinterpret_cast<unsigned long long*>(b + o_ops);
     s.mbs_cap = mbs;
     s.cells_cap = cells;
     return hipSuccess;
@@ -149,6 +152,7 @@ struct vc1me_ctx {
     std::string name;
     vc1me_counters counters{};
     vc1me_timing timing{};
+    vc1me_work work{};
     bool profiling = false;
     std::vector<JobSlot> slots;
     JobDesc* d_jobs = nullptr;
@@ -319,6 +323,7 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.sig = s.sig;
     d.g = s.g;
     d.err = s.err;
+    d.ops = s.ops;
     d.max_diag = d.mbw + 2 * d.mbh - 3;
 }
 
@@ -438,7 +443,10 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
     hipStream_t st = ctx->stream;
     if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1))) return rc;
     HIP_TRY(hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs.data(), sizeof(JobDesc) * n, hipMemcpyHostToDevice, st));
-    for (int i = 0; i < n; ++i) HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
+    for (int i = 0; i < n; ++i) {
+        HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
+        HIP_TRY(hipMemsetAsync(ctx->slots[i].ops, 0, 8 * sizeof(unsigned long long), st));
+    }
     HIP_TRY(hipEventRecord(ctx->events[0], st));
     HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st, ctx->events[2], ctx->events[3]));
     HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
@@ -477,11 +485,19 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
     }
     ctx->timing.mb_kernel_ms_total = mbms;
     ctx->timing.mb_kernel_launches = static_cast<uint64_t>(launches);
-    // kernel-side error flags
+    // kernel-side error flags and work counters
+    ctx->work = vc1me_work{};
     for (int i = 0; i < n; ++i) {
         int32_t err = 0;
         HIP_TRY(hipMemcpy(&err, ctx->slots[i].err, sizeof err, hipMemcpyDeviceToHost));
         if (err) return fail(VC1ME_EKERNEL, "kernel reported internal error code " + std::to_string(err));
+        unsigned long long ops[8];
+        HIP_TRY(hipMemcpy(ops, ctx->slots[i].ops, sizeof ops, hipMemcpyDeviceToHost));
+        ctx->work.sad_px += ops[0];
+        ctx->work.interp_taps += ops[1];
+        ctx->work.satd_ops += ops[2];
+        ctx->work.sig_ops += ops[3];
+        ctx->work.frame_px += ops[4];
     }
     for (int i = 0; i < n; ++i) {
         ctx->counters.frames_analyzed++;
@@ -653,6 +669,13 @@ int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out) {
     if (!ctx || !out) return fail(VC1ME_EINVAL, "null argument");
     std::lock_guard<std::mutex> lock(ctx->mu);
     *out = ctx->timing;
+    return VC1ME_OK;
+}
+
+int vc1me_last_work(vc1me_ctx* ctx, vc1me_work* out) {
+    if (!ctx || !out) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = ctx->work;
     return VC1ME_OK;
 }
 

@@ -51,7 +51,10 @@ struct JobDesc {
     uint64_t* sig;              // long-range signatures (ncx*ncy)
     GlobalState* g;
     int32_t* err;
-    int32_t max_diag;           // mbw + 2*mbh - 2
+    // algorithmic operation counters of this job (sad_px, interp_taps,
+    // satd_ops, sig_ops, frame_px), in the units of the CPU oracle
+    unsigned long long* ops;
+    int32_t max_diag;           // last anti-diagonal index mbw + 2*mbh - 3
 };
 
 // Flag bits of JobDesc::flags

@@ -253,7 +253,15 @@ struct Ctx {
     int winCap;
     // current window
     int wx0, wy0, ww, wh, wp;
+    // algorithmic operation counts of this macroblock (uniform)
+    uint64_t n_sad, n_taps, n_satd, n_sig;
 };
+
+__device__ __forceinline__ uint64_t taps_per_sample(int hm, int vm, bool bil) {
+    if (!hm && !vm) return 0;
+    if (bil) return 1;
+    return (hm && vm) ? 5 : 1;
+}
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }
 
@@ -400,6 +408,11 @@ __device__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int key
             }
         }
         sum = group_sum32(sum, G);
+        {
+            const uint64_t counted = __popcll(__ballot(valid && !hid && part == 0));
+            const uint64_t npx = static_cast<uint64_t>((C.aw + stride - 1) / stride) * ((C.ah + stride - 1) / stride);
+            C.n_sad += counted * npx;
+        }
         uint64_t key = kNone;
         if (valid && part == 0) {
             const uint64_t cost = hid ? kBadSad : static_cast<uint64_t>(sum) * static_cast<uint64_t>(stride * stride);
@@ -713,6 +726,8 @@ __device__ void qpel_sad_batch(Ctx& C, int bx0, int by0, int bw, int bh, int n, 
                 if (!qpel_window(C, bx0, by0, bw, bh, v.x, v.x, v.y, v.y)) { set_err(C, 3); cost = kBadSad; goto store; }
             }
             cost = qpel_sad_one(C, bx0, by0, bw, aw, ah, v.x, v.y, bil);
+            C.n_sad += static_cast<uint64_t>(aw) * ah;
+            C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(v.x & 3, v.y & 3, bil);
         }
     store:
         wsync();
@@ -785,6 +800,8 @@ __device__ uint64_t satd_one(Ctx& C, int bx0, int by0, int bw, int bh, int xq, i
     tsum += __shfl_xor(tsum, 2);
     const uint32_t contrib = (tvalid && r == 0) ? ((tsum + 1) >> 1) : 0;
     const uint32_t total = wave_sum32(contrib);
+    C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(hm, vm, bil);
+    C.n_satd += 112ull * __popcll(__ballot(tvalid && r == 0));
     wsync();
     return total;
 }
@@ -1112,6 +1129,7 @@ __device__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_s
     uint64_t loc[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) loc[j] = kNone;
+    uint32_t scanned = 0;
     if (cx_lo <= cx_hi && cy_lo <= cy_hi) {
         const int nxc = cx_hi - cx_lo + 1;
         const int total = nxc * (cy_hi - cy_lo + 1);
@@ -1119,6 +1137,7 @@ __device__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_s
             const int cyy = cy_lo + i / nxc, cxx = cx_lo + i % nxc;
             const int dx = cxx * 8 - C.x0, dy = cyy * 8 - C.y0;
             if (iabs(dx) <= 32 && iabs(dy) <= 32) continue;
+            ++scanned;
             uint64_t k = key_int(sig_distance(want, J->sig[cyy * J->ncx + cxx]), dx, dy);
             if (k < loc[15]) {
 #pragma unroll
@@ -1128,6 +1147,7 @@ __device__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_s
             }
         }
     }
+    C.n_sig += wave_sum32(scanned);
     int tn = 0;
     for (int r = 0; r < 16; ++r) {
         const uint64_t mk = wave_min64(loc[0]);
@@ -1504,6 +1524,7 @@ __device__ void analyze_mb(Ctx& C) {
             J->dl_mae[pos] = -1.0;
             J->dc_mae[pos] = -1.0;
             J->decision[pos] = 0;
+            atomicAdd(&J->ops[0], static_cast<unsigned long long>(C.n_sad));
         }
         return;
     }
@@ -1703,6 +1724,10 @@ __device__ void analyze_mb(Ctx& C) {
         J->decision[pos] = static_cast<uint8_t>(decision_out);
         J->chosen[pos] = chosen_sad;
         J->temporal[pos] = temporal_sad;
+        atomicAdd(&J->ops[0], static_cast<unsigned long long>(C.n_sad));
+        atomicAdd(&J->ops[1], static_cast<unsigned long long>(C.n_taps));
+        atomicAdd(&J->ops[2], static_cast<unsigned long long>(C.n_satd));
+        atomicAdd(&J->ops[3], static_cast<unsigned long long>(C.n_sig));
     }
 }
 
@@ -1729,6 +1754,7 @@ __global__ void __launch_bounds__(64) k_mb_diagonal(const JobDesc* __restrict__ 
     C.L = carve(smem, J->local_range);
     C.winCap = win_capacity(J->local_range);
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
+    C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
     analyze_mb(C);
 }
 
@@ -1766,6 +1792,7 @@ __global__ void __launch_bounds__(256) k_frame_sads(const JobDesc* __restrict__ 
     if (threadIdx.x == 0) {
         const uint64_t t = static_cast<uint64_t>(red[0]) + red[1] + red[2] + red[3];
         atomicAdd(reinterpret_cast<unsigned long long*>(&g->cost[p]), static_cast<unsigned long long>(t));
+        atomicAdd(&J->ops[4], static_cast<unsigned long long>(imin(rows, r0 + 8) - r0) * static_cast<unsigned long long>(cols));
     }
 }
 

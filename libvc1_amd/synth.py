@@ -45,13 +45,15 @@ def _filter4(a, b, c, d, mode: int):
 
 
 def vc1_luma_translate(plane: np.ndarray, xq: int, yq: int, *, wrap: bool = False,
-                       rnd: int = 0) -> np.ndarray:
+                       rnd: int = 0, out_shape: tuple[int, int] | None = None) -> np.ndarray:
     """Whole-plane VC-1 bicubic prediction at a quarter-pel offset.
 
     out[y, x] = luma_mc_sample(plane, x, y, xq, yq, rnd) with clamped fetch
     (or periodic fetch when ``wrap``), src/encoder_motion_compensation.cpp:367-396.
+    ``out_shape`` (wrap only) asks for a window of another size.
     """
-    h, w = plane.shape
+    ph, pw = plane.shape
+    h, w = out_shape if out_shape is not None else (ph, pw)
     p = plane.astype(np.int32)
     ix, iy = _floor_div4(xq), _floor_div4(yq)
     hm, vm = xq - 4 * ix, yq - 4 * iy
@@ -60,11 +62,11 @@ def vc1_luma_translate(plane: np.ndarray, xq: int, yq: int, *, wrap: bool = Fals
 
     def cols(off):
         v = xs + off
-        return np.mod(v, w) if wrap else np.clip(v, 0, w - 1)
+        return np.mod(v, pw) if wrap else np.clip(v, 0, pw - 1)
 
     def rows(off):
         v = ys + off
-        return np.mod(v, h) if wrap else np.clip(v, 0, h - 1)
+        return np.mod(v, ph) if wrap else np.clip(v, 0, ph - 1)
 
     def S(dx, dy):
         return p[np.ix_(rows(dy), cols(dx))]
@@ -97,10 +99,11 @@ def _chroma_mv(q: int) -> int:
 
 
 def vc1_chroma_translate(plane: np.ndarray, xq: int, yq: int, *, wrap: bool = False,
-                         rnd: int = 0) -> np.ndarray:
+                         rnd: int = 0, out_shape: tuple[int, int] | None = None) -> np.ndarray:
     """Whole-plane VC-1 chroma prediction for a luma quarter-pel vector
     (chroma_mc_sample, src/encoder_motion_compensation.cpp:417-447)."""
-    h, w = plane.shape
+    ph, pw = plane.shape
+    h, w = out_shape if out_shape is not None else (ph, pw)
     p = plane.astype(np.int32)
     uvx, uvy = _chroma_mv(xq), _chroma_mv(yq)
     ix, iy = _floor_div4(uvx), _floor_div4(uvy)
@@ -111,8 +114,8 @@ def vc1_chroma_translate(plane: np.ndarray, xq: int, yq: int, *, wrap: bool = Fa
     def S(dx, dy):
         c = xs + dx
         r = ys + dy
-        c = np.mod(c, w) if wrap else np.clip(c, 0, w - 1)
-        r = np.mod(r, h) if wrap else np.clip(r, 0, h - 1)
+        c = np.mod(c, pw) if wrap else np.clip(c, 0, pw - 1)
+        r = np.mod(r, ph) if wrap else np.clip(r, 0, ph - 1)
         return p[np.ix_(r, c)]
 
     a = S(0, 0)
@@ -237,14 +240,8 @@ def _new_rects(rng, width, height, count, max_obj_px):
 def _sample_periodic(tex: np.ndarray, oxq: int, oyq: int, h: int, w: int, chroma: bool) -> np.ndarray:
     """An h x w window of a periodic texture whose origin sits at quarter-pel
     (oxq, oyq), interpolated with the VC-1 luma or chroma law."""
-    th, tw = tex.shape
-    ix, iy = oxq // 4, oyq // 4
-    rolled = np.roll(tex, shift=(-(iy % th), -(ix % tw)), axis=(0, 1))
-    reps_y = -(-(h + 8) // th)
-    reps_x = -(-(w + 8) // tw)
-    big = np.tile(rolled, (max(1, reps_y), max(1, reps_x)))[: max(h + 8, th), : max(w + 8, tw)]
     fn = vc1_chroma_translate if chroma else vc1_luma_translate
-    return fn(big, oxq - 4 * ix, oyq - 4 * iy, wrap=True)[:h, :w]
+    return fn(tex, oxq, oyq, wrap=True, out_shape=(h, w))
 
 
 def pan_rectangles(seed: int, frames: int, width: int = 1920, height: int = 1080,
@@ -276,6 +273,8 @@ def pan_rectangles(seed: int, frames: int, width: int = 1920, height: int = 1080
             if k in wanted:
                 out[k] = Picture(last.y.copy(), last.u.copy(), last.v.copy())
             continue
+        if k not in wanted and not (k + 1 < frames and dup[k + 1]):
+            continue  # no duplicate depends on this frame: skip it
         t = k - scene_start
         y = _sample_periodic(bg[0], t * pan[0], t * pan[1], height, width, False)
         u = _sample_periodic(bg[1], t * pan[0], t * pan[1], height // 2, width // 2, True)
