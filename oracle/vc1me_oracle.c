@@ -1662,3 +1662,34 @@ int oracle_score_cmp(int pqindex, uint64_t sad_a, uint64_t rate_a, uint64_t sad_
     const long double b = (long double)sad_b + lam * (long double)rate_b;
     return a < b ? -1 : (a > b ? 1 : 0);
 }
+
+int oracle_hidden_padding(const vc1me_params* p, int x0, int y0, int bw, int bh, int xq, int yq, int mode, int chroma) {
+    ctx_t c;
+    memset(&c, 0, sizeof c);
+    c.p = p;
+    c.W = p->width; c.H = p->height;
+    c.vw = p->display_width > 0 ? p->display_width : p->width;
+    c.vh = p->display_height > 0 ? p->display_height : p->height;
+    c.mbw = (p->width + 15) / 16; c.mbh = (p->height + 15) / 16;
+    const mv_t mv = {xq, yq};
+    return hidden_padding(&c, x0, y0, bw, bh, mv, mode, chroma != 0) ? 1 : 0;
+}
+int oracle_distant_decision(const vc1me_params* p, uint64_t cand, uint64_t local, int mx, int my) {
+    ctx_t c;
+    memset(&c, 0, sizeof c);
+    c.p = p;
+    c.W = p->width; c.H = p->height;
+    return distant_decision(&c, cand, local, mx, my);
+}
+void oracle_integer_search(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref, int mx, int my,
+                           int pre_xq, int pre_yq, int gdx, int gdy, int range, int32_t* dx, int32_t* dy,
+                           uint64_t* sad) {
+    ctx_t c;
+    init_ctx(&c, p, cur, ref, NULL);
+    pred_t pi;
+    memset(&pi, 0, sizeof pi);
+    pi.pre.xq = pre_xq; pi.pre.yq = pre_yq;
+    const cand_t g = {0, gdx, gdy};
+    const imr_t r = integer_local(&c, mx, my, &pi, g, range);
+    *dx = r.dx; *dy = r.dy; *sad = r.sad;
+}

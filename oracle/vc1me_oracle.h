@@ -65,6 +65,15 @@ int oracle_mv_rate_bits(const vc1me_params* p, int dxq, int dyq, int pxq, int py
  * returns -1/0/1 for (sad_a + lambda*rate_a) <=> (sad_b + lambda*rate_b)
  * evaluated exactly as the reference does (src/encoder_motion.cpp:3465-3471). */
 int oracle_score_cmp(int pqindex, uint64_t sad_a, uint64_t rate_a, uint64_t sad_b, uint64_t rate_b);
+/* motion_uses_hidden_padding (src/encoder_motion.cpp:2191-2195) */
+int oracle_hidden_padding(const vc1me_params* p, int x0, int y0, int bw, int bh, int xq, int yq, int mode, int chroma);
+/* distant_match_decision (src/encoder_motion.cpp:1642-1656) */
+int oracle_distant_decision(const vc1me_params* p, uint64_t cand, uint64_t local, int mx, int my);
+/* integer_motion_local with a bare predictor (pre only, A/C invalid) and a
+ * given global vector (src/encoder_motion_diamond.cpp:5-11). */
+void oracle_integer_search(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref, int mx, int my,
+                           int pre_xq, int pre_yq, int gdx, int gdy, int range, int32_t* dx, int32_t* dy,
+                           uint64_t* sad);
 
 #ifdef __cplusplus
 }

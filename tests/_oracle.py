@@ -47,6 +47,10 @@ def oracle() -> C.CDLL:
         lib.oracle_global_motion.restype = None
         lib.oracle_mv_rate_bits.argtypes = [P(abi.Params), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         lib.oracle_score_cmp.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
+        lib.oracle_hidden_padding.argtypes = [P(abi.Params)] + [C.c_int] * 8
+        lib.oracle_distant_decision.argtypes = [P(abi.Params), C.c_uint64, C.c_uint64, C.c_int, C.c_int]
+        lib.oracle_integer_search.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_int] * 7 + [P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
+        lib.oracle_integer_search.restype = None
         _oracle = lib
     return _oracle
 

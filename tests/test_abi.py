@@ -1,0 +1,58 @@
+"""The C-ABI library: it loads, exports every symbol include/vc1me.h declares,
+and refuses to run without a device (no silent CPU fallback)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from libvc1_amd import abi, backend
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "vc1me.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(vc1me_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = backend.load_library()
+    names = _declared()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing
+    assert set(names) == set(backend.EXPORTS)
+
+
+def test_abi_version_and_defaults():
+    lib = backend.load_library()
+    assert lib.vc1me_abi_version() == 1
+    p = abi.Params()
+    lib.vc1me_params_default(C.byref(p), 1920, 1080)
+    ref = abi.default_params(1920, 1080)
+    for name, _ in abi.Params._fields_:
+        assert getattr(p, name) == getattr(ref, name), name
+
+
+def test_struct_layouts_match_header():
+    # sizes of the POD structs as the C compiler lays them out
+    assert C.sizeof(abi.Params) == 16 * 4 + 2 * 8
+    assert C.sizeof(abi.Candidate) == 8
+    assert C.sizeof(abi.PResult) == 9 * 8 + 2 * 8 + 5 * 8 + 2 * 4 + 8
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.mark.skipif(_has_gpu(), reason="a device is present")
+def test_create_fails_loudly_without_device():
+    with pytest.raises(backend.BackendError) as e:
+        backend.Backend(0)
+    assert e.value.code == abi.VC1ME_EDEVICE

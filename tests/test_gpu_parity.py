@@ -126,3 +126,54 @@ def test_batched_step_equals_single_frames(backend):
     for (c, r, p), g in zip(jobs, got):
         want = oracle_analyze(p, c, r)
         assert not g.diff(want), describe_diff(g, want, p.width)
+
+
+def _golden_files():
+    import glob
+    import os
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    return sorted(glob.glob(os.path.join(here, "*.npz")))
+
+
+@pytest.mark.parametrize("path", _golden_files(), ids=lambda p: p.rsplit("/", 1)[-1][:-4])
+def test_gpu_matches_reference_fixture(backend, path):
+    """The GPU result against outputs written by the compiled reference itself."""
+    import json
+    z = np.load(path, allow_pickle=False)
+    cur = synth.Picture(z["cur_y"], z["cur_u"], z["cur_v"])
+    ref = synth.Picture(z["ref_y"], z["ref_u"], z["ref_v"])
+    h, w = cur.y.shape
+    p = abi.default_params(w, h, **json.loads(str(z["params"])))
+    _say(f"[case] golden {path.rsplit('/', 1)[-1]}")
+    fc, fr = backend.upload(cur), backend.upload(ref)
+    try:
+        got = backend.analyze(fc, fr, p)
+    finally:
+        fc.release()
+        fr.release()
+    want = abi.ResultArrays(w, h)
+    for k in abi.ResultArrays.ARRAYS:
+        getattr(want, k)[...] = z[f"out_{k}"]
+    for k, v in json.loads(str(z["scalars"])).items():
+        setattr(want.struct, k, v)
+    assert not got.diff(want), describe_diff(got, want, w)
+
+
+def test_work_counters_match_oracle_op_counts(backend):
+    """The kernel's algorithmic op counters equal the oracle's count of the
+    reference algorithm's operations on the same inputs."""
+    from _oracle import OracleOps
+    d = synth.pan_rectangles(31, 3, width=320, height=176, max_pan_px=8, max_obj_px=24, frame_indices=[0, 2])
+    p = abi.default_params(320, 176, pqindex=8, motion_search_range=256, motion_local_search_range=32,
+                           mode=abi.MODE_MIXED_MV)
+    ops = OracleOps()
+    oracle_analyze(p, d[2], d[0], ops)
+    fc, fr = backend.upload(d[2]), backend.upload(d[0])
+    try:
+        backend.analyze(fc, fr, p)
+        w = backend.work()
+    finally:
+        fc.release()
+        fr.release()
+    assert (w.sad_px, w.interp_taps, w.satd_ops, w.sig_ops, w.frame_px) == \
+        (ops.sad_px, ops.interp_taps, ops.satd_ops, ops.sig_ops, ops.frame_px)

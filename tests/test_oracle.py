@@ -1,0 +1,449 @@
+"""The CPU oracle, pinned.
+
+* golden fixtures written from the compiled reference (tests/golden/*.npz and
+  analyze_digests.json, made by tests/golden/make_golden.py);
+* the compiled reference itself (oracle/_ref), when it is present;
+* the reference's own known-answer tests, restated;
+* exact arithmetic: the x87 long-double score law against an independent
+  rational-number evaluation, the VC-1 interpolation law against the
+  synthetic generator's vectorised restatement, sampled SAD against numpy.
+"""
+import ctypes as C
+import glob
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from libvc1_amd import abi, synth
+import _oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+P = C.POINTER
+
+
+def _pics_from_npz(z):
+    cur = synth.Picture(z["cur_y"], z["cur_u"], z["cur_v"])
+    ref = synth.Picture(z["ref_y"], z["ref_u"], z["ref_v"])
+    return cur, ref
+
+
+def _expected(z):
+    h, w = z["cur_y"].shape
+    e = abi.ResultArrays(w, h)
+    for k in abi.ResultArrays.ARRAYS:
+        getattr(e, k)[...] = z[f"out_{k}"]
+    for k, v in json.loads(str(z["scalars"])).items():
+        setattr(e.struct, k, v)
+    return e
+
+
+GOLDEN_FILES = sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN_FILES, ids=[os.path.basename(p)[:-4] for p in GOLDEN_FILES])
+def test_oracle_matches_golden_fixture(path):
+    z = np.load(path, allow_pickle=False)
+    cur, ref = _pics_from_npz(z)
+    h, w = cur.y.shape
+    p = abi.default_params(w, h, **json.loads(str(z["params"])))
+    got = O.oracle_analyze(p, cur, ref)
+    want = _expected(z)
+    assert not got.diff(want), O.describe_diff(got, want, w)
+
+
+def test_golden_fixtures_present():
+    assert len(GOLDEN_FILES) >= 6
+
+
+def _digest_cases():
+    with open(os.path.join(GOLDEN, "analyze_digests.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _digest_cases(), ids=lambda c: c["name"])
+def test_oracle_matches_golden_digest(case):
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import digest, digest_inputs
+    c, r, w, h = digest_inputs(case["kind"], case["args"])
+    p = abi.default_params(w, h, **case["params"])
+    got = O.oracle_analyze(p, c, r)
+    assert digest(got) == case["sha256"], got.scalars()
+
+
+# ---------------------------------------------------------------- against the compiled reference
+
+needs_ref = pytest.mark.skipif(not O.reference_available(), reason="oracle/_ref not built (needs the reference sources)")
+
+
+@needs_ref
+@pytest.mark.parametrize("over", [
+    dict(),
+    dict(me_algorithm=1),
+    dict(fast_subpixel_refinement=1),
+    dict(me_quality=1),
+    dict(long_range_search_mode=1),
+    dict(long_range_search_mode=2),
+    dict(motion_search_range=0),
+    dict(motion_search_range=24, motion_local_search_range=24),
+    dict(motion_search_range=3, motion_local_search_range=3),
+    dict(mode=abi.MODE_ONE_MV_HPEL),
+    dict(mode=abi.MODE_ONE_MV_HPEL_BILINEAR),
+    dict(mode=abi.MODE_ONE_MV_QPEL, allow_intra=0),
+    dict(subpixel_refinement_range=2),
+    dict(subpixel_refinement=0),
+    dict(pqindex=2),
+    dict(pqindex=31),
+    dict(distant_match_max_mae=12.0),
+    dict(inter_intra_threshold=1.7),
+])
+def test_oracle_matches_reference_options(over):
+    d = synth.pan_rectangles(5, 4, width=320, height=176, max_pan_px=6, max_obj_px=20, frame_indices=[0, 2])
+    base = dict(pqindex=8, motion_search_range=128, motion_local_search_range=32, mode=abi.MODE_MIXED_MV)
+    base.update(over)
+    p = abi.default_params(320, 176, **base)
+    a = O.oracle_analyze(p, d[2], d[0])
+    b = O.reference_analyze(p, d[2], d[0])
+    assert not a.diff(b), O.describe_diff(a, b, 320)
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_matches_reference_random_rasters(seed):
+    rng = np.random.default_rng(100 + seed)
+    w = int(rng.integers(1, 12)) * 8 + 16
+    h = int(rng.integers(1, 10)) * 8 + 16
+    dw = w - int(rng.integers(0, 2)) if seed % 2 else 0
+    dh = h - int(rng.integers(0, 3)) if seed % 3 == 0 else 0
+    d = synth.pan_rectangles(200 + seed, 3, width=w, height=h, max_pan_px=5, max_obj_px=12, rects=3,
+                             frame_indices=[0, 2])
+    p = abi.default_params(w, h, display_width=dw, display_height=dh, pqindex=int(rng.integers(1, 32)),
+                           motion_search_range=int(rng.choice([16, 40, 100, 300])),
+                           motion_local_search_range=int(rng.choice([8, 16, 32])), mode=int(rng.integers(0, 4)))
+    a = O.oracle_analyze(p, d[2], d[0])
+    b = O.reference_analyze(p, d[2], d[0])
+    assert not a.diff(b), O.describe_diff(a, b, w)
+
+
+@needs_ref
+def test_oracle_matches_reference_cif_workload():
+    s = synth.cif_translation(9, frames=3)
+    p = abi.default_params(352, 288, pqindex=8, motion_search_range=64, motion_local_search_range=64,
+                           mode=abi.MODE_MIXED_MV)
+    for i in (1, 2):
+        a = O.oracle_analyze(p, s[i], s[i - 1])
+        b = O.reference_analyze(p, s[i], s[i - 1])
+        assert not a.diff(b), O.describe_diff(a, b, 352)
+
+
+@needs_ref
+def test_block_costs_match_reference_over_all_phases():
+    rng = np.random.default_rng(1)
+    w, h = 64, 48
+    cur = synth.Picture(rng.integers(0, 256, (h, w), dtype=np.uint8), np.full((24, 32), 128, np.uint8),
+                        np.full((24, 32), 128, np.uint8))
+    ref = synth.Picture(rng.integers(0, 256, (h, w), dtype=np.uint8), np.full((24, 32), 128, np.uint8),
+                        np.full((24, 32), 128, np.uint8))
+    p = abi.default_params(w, h, display_height=45, motion_search_range=32)
+    cs, rs = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    cands = np.array([(x, y) for y in range(-7, 8) for x in range(-7, 8)], np.int32)
+    n = len(cands)
+    for (x0, y0, bw, bh) in ((16, 16, 16, 16), (48, 32, 16, 16), (8, 40, 8, 8), (0, 0, 8, 8)):
+        for mode in (0, 2, 3):
+            for metric in (0, 1):
+                want = np.zeros(n, np.uint64)
+                O.reference().ref_block_costs(C.byref(p), C.byref(cs.struct), C.byref(rs.struct), x0, y0, bw, bh,
+                                              mode, metric, cands.ctypes.data_as(P(abi.Candidate)), n,
+                                              want.ctypes.data_as(P(C.c_uint64)))
+                got = np.array([O.oracle().oracle_block_cost(C.byref(p), C.byref(cs.struct), C.byref(rs.struct), x0,
+                                                             y0, bw, bh, int(x), int(y), mode, metric)
+                                for x, y in cands], np.uint64)
+                assert np.array_equal(got, want), (x0, y0, mode, metric)
+
+
+# ---------------------------------------------------------------- known answers of the reference's tests
+
+
+def _flat(v, w=16, h=16):
+    return synth.Picture(np.full((h, w), v, np.uint8), np.full((h // 2, w // 2), 128, np.uint8),
+                         np.full((h // 2, w // 2), 128, np.uint8))
+
+
+def _cost(p, cur, ref, x0, y0, bw, bh, xq, yq, mode, metric):
+    cs, rs = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    return O.oracle().oracle_block_cost(C.byref(p), C.byref(cs.struct), C.byref(rs.struct), x0, y0, bw, bh, xq, yq,
+                                        mode, metric)
+
+
+def test_sad_256_and_satd_ordering():
+    # tests/staged_motion_estimation.cpp:27-39
+    p = abi.default_params(16, 16, pqindex=8, motion_search_range=8, motion_local_search_range=8)
+    cur, flat = _flat(128), _flat(127)
+    noisy = _flat(128)
+    state = 1
+    ys = noisy.y.reshape(-1)
+    for i in range(256):
+        state = (state * 1664525 + 1013904223) & 0xffffffff
+        ys[i] = 127 if state & 0x80000000 else 129
+    assert _cost(p, cur, flat, 0, 0, 16, 16, 0, 0, 0, 0) == 256
+    assert _cost(p, cur, noisy, 0, 0, 16, 16, 0, 0, 0, 0) == 256
+    assert _cost(p, cur, noisy, 0, 0, 16, 16, 0, 0, 0, 1) > _cost(p, cur, flat, 0, 0, 16, 16, 0, 0, 0, 1)
+
+
+def test_visible_edge_corner_sad_is_40():
+    # tests/visible_edge_inter.cpp:52-62: an 18x18 picture leaves a 2x2 corner MB
+    p = abi.default_params(18, 18, motion_search_range=2, motion_local_search_range=2)
+    src = synth.Picture(np.full((18, 18), 100, np.uint8), np.full((9, 9), 128, np.uint8), np.full((9, 9), 128, np.uint8))
+    pred = synth.Picture(np.full((18, 18), 90, np.uint8), np.full((9, 9), 128, np.uint8), np.full((9, 9), 128, np.uint8))
+    cs, rs = abi.PlaneSet.of(src), abi.PlaneSet.of(pred)
+    assert O.oracle().oracle_mb_sad(C.byref(p), C.byref(cs.struct), C.byref(rs.struct), 1, 1, 0, 0, 1) == 40
+    assert _cost(p, src, pred, 16, 16, 16, 16, 0, 0, 0, 0) == 40
+    assert _cost(p, src, pred, 16, 16, 16, 16, 0, 0, 2, 0) == 40
+
+
+def test_hidden_padding_truth_table():
+    # tests/hidden_reference_guard.cpp:34-58
+    hp = O.oracle().oracle_hidden_padding
+    c = abi.default_params(160, 120)
+    assert not hp(C.byref(c), 0, 112, 16, 16, 0, 0, 0, 1)
+    assert hp(C.byref(c), 0, 112, 16, 16, 0, 16, 0, 1)
+    assert hp(C.byref(c), 0, 112, 16, 16, 0, 1, 0, 1)
+    assert hp(C.byref(c), 0, 112, 16, 16, 0, 2, 2, 1)
+    assert not hp(C.byref(c), 0, 96, 16, 16, 0, 0, 0, 1)
+    aligned = abi.default_params(160, 128)
+    assert not hp(C.byref(aligned), 0, 112, 16, 16, 0, 16, 0, 1)
+    right = abi.default_params(158, 128)
+    assert hp(C.byref(right), 144, 0, 16, 16, 16, 0, 0, 1)
+    assert not hp(C.byref(right), 144, 0, 16, 16, 0, 0, 0, 1)
+    odd = abi.default_params(66, 50, display_width=65, display_height=49)
+    assert hp(C.byref(odd), 64, 32, 16, 16, 4, 0, 0, 1)
+    assert hp(C.byref(odd), 48, 48, 16, 16, 0, 4, 0, 1)
+    assert not hp(C.byref(odd), 64, 48, 16, 16, 0, 0, 0, 1)
+
+
+def test_hidden_padding_final_vectors_never_read_padding():
+    # tests/hidden_reference_guard.cpp:59-78: final P vectors stay out of the padding
+    z = np.load(os.path.join(GOLDEN, "hidden_160x120_mixed.npz"), allow_pickle=False)
+    cur, ref = _pics_from_npz(z)
+    for mode in (0, 2, 3):
+        p = abi.default_params(160, 120, pqindex=8, motion_search_range=6, motion_local_search_range=6, mode=mode,
+                               allow_intra=0)
+        res = O.oracle_analyze(p, cur, ref)
+        hp = O.oracle().oracle_hidden_padding
+        for pos in range(res.mbs):
+            mx, my = pos % 10, pos // 10
+            if res.use_4mv[pos]:
+                for k in range(4):
+                    bx, by = mx * 16 + (k & 1) * 8, my * 16 + (k >> 1) * 8
+                    xq, yq = res.block_mvs[pos, k]
+                    assert not hp(C.byref(p), bx, by, 8, 8, int(xq), int(yq), 3, 0)
+            else:
+                xq, yq = res.mvs[pos]
+                assert not hp(C.byref(p), mx * 16, my * 16, 16, 16, int(xq), int(yq), mode, 1)
+
+
+def test_distant_match_decision_table():
+    # tests/staged_motion_estimation.cpp:154-159
+    p = abi.default_params(16, 16, distant_match_max_mae=12.0)
+    dd = O.oracle().oracle_distant_decision
+    assert dd(C.byref(p), 5 * 256, 40 * 256, 0, 0) == 2
+    assert dd(C.byref(p), 11 * 256, 20 * 256, 0, 0) == 3
+    assert dd(C.byref(p), 29 * 256, 40 * 256, 0, 0) == 1
+    assert dd(C.byref(p), 11 * 256, 13 * 256, 0, 0) == 1
+
+
+@pytest.mark.parametrize("shift,horizontal", [((4, 0), True), ((0, -3), False)])
+def test_exact_translation_recovery(shift, horizontal):
+    # tests/diamond_motion.cpp:18-41: a pure shift of a gradient is found with SAD 0
+    w, h = 64, 48
+    ys, xs = np.mgrid[0:h, 0:w]
+    ref_y = (20 + 3 * (xs if horizontal else ys)).astype(np.uint8)
+    sx, sy = shift
+    cur_y = ref_y[np.clip(ys + sy, 0, h - 1), np.clip(xs + sx, 0, w - 1)]
+    cu = np.full((h // 2, w // 2), 128, np.uint8)
+    cur, ref = abi.PlaneSet(cur_y, cu, cu), abi.PlaneSet(ref_y, cu, cu)
+    for algo in (1, 0):
+        p = abi.default_params(w, h, motion_search_range=12, motion_local_search_range=12, me_algorithm=algo)
+        dx, dy, sad = C.c_int32(), C.c_int32(), C.c_uint64()
+        O.oracle().oracle_integer_search(C.byref(p), C.byref(cur.struct), C.byref(ref.struct), 1, 1, 0, 0, 0, 0, 12,
+                                         C.byref(dx), C.byref(dy), C.byref(sad))
+        assert (dx.value, dy.value, sad.value) == (sx, sy, 0)
+
+
+def test_diamond_rejects_illegal_edge_predictor():
+    # tests/diamond_motion.cpp:43-56
+    w, h = 64, 48
+    ys, xs = np.mgrid[0:h, 0:w]
+    f = (20 + 3 * xs).astype(np.uint8)
+    cu = np.full((h // 2, w // 2), 128, np.uint8)
+    ps = abi.PlaneSet(f, cu, cu)
+    p = abi.default_params(w, h, display_height=42, motion_search_range=8, motion_local_search_range=8, me_algorithm=1)
+    dx, dy, sad = C.c_int32(), C.c_int32(), C.c_uint64()
+    O.oracle().oracle_integer_search(C.byref(p), C.byref(ps.struct), C.byref(ps.struct), 1, 2, 0, 16, 0, 4, 8,
+                                     C.byref(dx), C.byref(dy), C.byref(sad))
+    assert (dx.value, dy.value, sad.value) == (0, 0, 0)
+    assert O.oracle().oracle_mb_sad(C.byref(p), C.byref(ps.struct), C.byref(ps.struct), 1, 2, 0, 4, 1) >= (2**64 - 1) // 8
+
+
+def test_zero_range_is_zero_vectors():
+    # src/encoder_motion.cpp:73-86
+    d = synth.pan_rectangles(8, 2, width=96, height=64, frame_indices=[0, 1])
+    p = abi.default_params(96, 64, motion_search_range=0, mode=abi.MODE_MIXED_MV)
+    res = O.oracle_analyze(p, d[1], d[0])
+    assert not res.mvs.any() and not res.block_mvs.any() and not res.use_4mv.any()
+    assert res.struct.motion_sad == 0
+
+
+def test_flat_frame_gives_zero_vectors():
+    p = abi.default_params(64, 48, pqindex=8, motion_search_range=32, mode=abi.MODE_MIXED_MV)
+    res = O.oracle_analyze(p, _flat(90, 64, 48), _flat(90, 64, 48))
+    assert not res.mvs.any() and res.struct.motion_sad == 0
+
+
+# ---------------------------------------------------------------- exact arithmetic
+
+
+def _x87_round(num: Fraction) -> Fraction:
+    """Round a positive rational to 64 significant bits, ties to even."""
+    if num == 0:
+        return num
+    e = num.numerator.bit_length() - num.denominator.bit_length()
+    while Fraction(2) ** e > num:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= num:
+        e += 1
+    scale = Fraction(2) ** (63 - e)
+    q = num * scale
+    fl = q.numerator // q.denominator
+    rem = q - fl
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and fl % 2 == 1):
+        fl += 1
+    return Fraction(fl) / scale
+
+
+def _x87_emulation(cost: int, rate: int, pq: int) -> int:
+    """The integer form the GPU kernel uses: round(cost*2^k + m*rate) to 64 bits."""
+    lam = max(1.0, 0.35 * pq)
+    m, e = float(lam).as_integer_ratio()
+    k = e.bit_length() - 1  # e is a power of two
+    n = (cost << k) + m * rate
+    if n < (1 << 64):
+        return n
+    sh = n.bit_length() - 64
+    q, rem = n >> sh, n & ((1 << sh) - 1)
+    half = 1 << (sh - 1)
+    if rem > half or (rem == half and q & 1):
+        q += 1
+    return q << sh
+
+
+def test_x87_score_law():
+    rng = np.random.default_rng(0)
+    cmp = O.oracle().oracle_score_cmp
+    for pq in (1, 2, 3, 5, 8, 9, 13, 20, 31):
+        lam = Fraction(max(1.0, 0.35 * pq))
+        for _ in range(300):
+            a_s, b_s = (int(v) for v in rng.integers(0, 1 << int(rng.integers(4, 22)), size=2))
+            a_r, b_r = (int(v) for v in rng.integers(0, 40, size=2))
+            if rng.random() < 0.3:  # near-ties: the case where rounding decides
+                b_s = a_s + int(rng.integers(-20, 21)) if a_s > 20 else a_s
+                b_r = max(0, a_r + int(rng.integers(-7, 8)))
+            xa = _x87_round(a_s + lam * a_r)
+            xb = _x87_round(b_s + lam * b_r)
+            want = (xa > xb) - (xa < xb)
+            assert cmp(pq, a_s, a_r, b_s, b_r) == want
+            ea, eb = _x87_emulation(a_s, a_r, pq), _x87_emulation(b_s, b_r, pq)
+            assert ((ea > eb) - (ea < eb)) == want
+
+
+def test_x87_near_tie_at_5_lambda():
+    # 5*lambda(pq=8) = 13.999999999999999; at SAD ~ 2^13 the x87 ulp decides the order
+    cmp = O.oracle().oracle_score_cmp
+    for s in range(8180, 8200):
+        assert cmp(8, s + 14, 0, s, 5) in (-1, 0, 1)
+        lam = Fraction(0.35 * 8)
+        want_a, want_b = _x87_round(Fraction(s + 14)), _x87_round(s + lam * 5)
+        assert cmp(8, s + 14, 0, s, 5) == (want_a > want_b) - (want_a < want_b)
+
+
+def test_interpolation_law_matches_generator():
+    rng = np.random.default_rng(2)
+    plane = rng.integers(0, 256, (24, 28), dtype=np.uint8)
+    lib = O.oracle()
+    for xq in range(-9, 10):
+        for yq in range(-9, 10):
+            got = synth.vc1_luma_translate(plane, xq, yq)
+            ys, xs = np.mgrid[0:24, 0:28]
+            pick = [(int(x), int(y)) for x, y in zip(xs.ravel()[::37], ys.ravel()[::37])]
+            for x, y in pick:
+                assert got[y, x] == lib.oracle_luma_mc_sample(plane.ctypes.data, 28, 24, x, y, xq, yq, 0, 0)
+
+
+@needs_ref
+def test_interpolation_law_matches_reference():
+    rng = np.random.default_rng(4)
+    plane = rng.integers(0, 256, (20, 22), dtype=np.uint8)
+    for mode in (0, 2):
+        for _ in range(400):
+            x, y = int(rng.integers(-3, 25)), int(rng.integers(-3, 23))
+            xq, yq = int(rng.integers(-40, 41)), int(rng.integers(-40, 41))
+            if mode == 2:
+                xq, yq = xq & ~1, yq & ~1
+            a = O.oracle().oracle_luma_mc_sample(plane.ctypes.data, 22, 20, x, y, xq, yq, mode, 0)
+            b = O.reference().ref_luma_mc_sample(plane.ctypes.data, 22, 20, x, y, xq, yq, mode, 0)
+            assert a == b
+
+
+def test_sampled_sad_formula():
+    # tests/vulkan_compute_runtime.cpp:17-29, the 48x40 formula frames, 20 vectors
+    w, h = 48, 40
+    ys, xs = np.mgrid[0:h, 0:w]
+    cur = ((xs * 13 + ys * 29 + (xs * ys) % 37 + 17) & 255).astype(np.uint8)
+    ref = ((xs * 7 + ys * 19 + (xs * ys) % 53 + 61) & 255).astype(np.uint8)
+    for stride in (2, 4):
+        for dx, dy in ((0, 0), (1, 0), (-1, 0), (0, 1), (0, -1), (3, 2), (-5, -4), (8, -3)):
+            cands = np.array([[dx * 4, dy * 4]], np.int32)
+            got = O.oracle_score_batch(cur, ref, 5, 7, 16, 16, cands, stride)[0]
+            yy, xx = np.mgrid[0:16:stride, 0:16:stride]
+            rx = np.clip(5 + xx + dx, 0, w - 1)
+            ry = np.clip(7 + yy + dy, 0, h - 1)
+            want = np.abs(cur[7 + yy, 5 + xx].astype(int) - ref[ry, rx].astype(int)).sum() * stride * stride
+            assert got == want
+
+
+def test_score_batch_equals_block_costs_on_vulkan_fixture():
+    # tests/vulkan_compute_runtime.cpp:39-86: kernel law == block_sad_motion / block_satd_motion
+    w, h = 48, 40
+    ys, xs = np.mgrid[0:h, 0:w]
+    cur = ((xs * 13 + ys * 29 + (xs * ys) % 37 + 17) & 255).astype(np.uint8)
+    ref = ((xs * 7 + ys * 19 + (xs * ys) % 53 + 61) & 255).astype(np.uint8)
+    mvs = np.array([(0, 0), (4, 0), (-4, 0), (0, 4), (0, -4), (3, 1), (-3, 2), (7, -5), (-9, -7), (12, 8), (-16, 11),
+                    (1, 3), (2, 2), (-1, -3), (20, -12), (-23, 17), (31, 29), (-32, -27), (5, -19), (-21, 6)], np.int32)
+    p = abi.default_params(w, h)
+    pc, pr = _flat(0, w, h), _flat(0, w, h)
+    pc.y, pr.y = cur, ref
+    for metric in (0, 1):
+        got = O.oracle_score_batch(cur, ref, 5, 7, 16, 16, mvs, 1, 0, 0, metric)
+        want = [_cost(p, pc, pr, 5, 7, 16, 16, int(x), int(y), 0, metric) for x, y in mvs]
+        assert got.tolist() == want
+    even = mvs & ~1
+    got = O.oracle_score_batch(cur, ref, 5, 7, 16, 16, even, 1, 1, 0, 0)
+    want = [_cost(p, pc, pr, 5, 7, 16, 16, int(x), int(y), 2, 0) for x, y in even]
+    assert got.tolist() == want
+
+
+def test_mv_rate_bits_symbols():
+    # motion_vector_rate_bits on hand-derived differentials (table 0, qpel, more=1)
+    p = abi.default_params(352, 288, motion_search_range=64)
+    f = O.oracle().oracle_mv_rate_bits
+    # zero differential, more: symbol 36 -> 3 bits, no suffix
+    assert f(C.byref(p), 0, 0, 0, 0, 0, 1) == 3
+    # (+1, 0): x category 1 -> symbol 37 (5 bits) + 2 suffix bits
+    assert f(C.byref(p), 1, 0, 0, 0, 0, 1) == 5 + 2
+    # (0, +3): y category 2 -> idx 12 -> symbol 48 (6 bits) + 3 suffix bits
+    assert f(C.byref(p), 0, 3, 0, 0, 0, 1) == 6 + 3
+    # escape: |dx| > 158 -> symbol 71 (9 bits) + kx+ky = 12+10 (range 64 -> MVRANGE 2)
+    assert f(C.byref(p), 200, 0, 0, 0, 0, 1) == 9 + 22
