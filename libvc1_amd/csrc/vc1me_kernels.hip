@@ -40,33 +40,72 @@ __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 __device__ __forceinline__ int floor_div4(int v) { return v >> 2; }  // arithmetic shift == floor division by 4
 __device__ __forceinline__ int arshift(int v, int n) { return v >> n; }  // floor shift (src/encoder_entropy.cpp:377-381)
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-    const uint32_t lo = __shfl_xor(static_cast<unsigned>(v), m);
-    const uint32_t hi = __shfl_xor(static_cast<unsigned>(v >> 32), m);
+// DPP cross-lane steps (gfx9 DPP controls): quad_perm, row mirrors and the
+// row broadcasts.  update_dpp(old, src, ...) leaves lanes outside row_mask at
+// `old`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t old, uint32_t src) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(src), CTRL,
+                                                             ROWMASK, 0xf, false));
+}
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // row_half_mirror
+constexpr int kDppMirror = 0x140;     // row_mirror
+constexpr int kDppBcast15 = 0x142;    // row_bcast:15
+constexpr int kDppBcast31 = 0x143;    // row_bcast:31
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint64_t min64_step(uint64_t v) {
+    const uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+    const uint32_t plo = dpp_mov<CTRL, ROWMASK>(lo, lo), phi = dpp_mov<CTRL, ROWMASK>(hi, hi);
+    const uint64_t p = (static_cast<uint64_t>(phi) << 32) | plo;
+    return p < v ? p : v;
+}
+// Minimum over the wave, returned uniformly (read from lane 63).
+__device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
+    v = min64_step<kDppXor1, 0xf>(v);
+    v = min64_step<kDppXor2, 0xf>(v);
+    v = min64_step<kDppHalfMirror, 0xf>(v);
+    v = min64_step<kDppMirror, 0xf>(v);
+    v = min64_step<kDppBcast15, 0xa>(v);
+    v = min64_step<kDppBcast31, 0xc>(v);
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(v), 63);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<int>(v >> 32), 63);
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
-__device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) {
-        const uint64_t o = shfl_xor64(v, m);
-        v = o < v ? o : v;
-    }
-    return v;
-}
+// Sum over the wave, returned uniformly.
 __device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m);
-    return v;
+    v += dpp_mov<kDppXor1, 0xf>(0, v);
+    v += dpp_mov<kDppXor2, 0xf>(0, v);
+    v += dpp_mov<kDppHalfMirror, 0xf>(0, v);
+    v += dpp_mov<kDppMirror, 0xf>(0, v);
+    v += dpp_mov<kDppBcast15, 0xa>(0, v);
+    v += dpp_mov<kDppBcast31, 0xc>(0, v);
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
+// Sum over aligned groups of G lanes (G = 1, 2, 4, 8, 16); every lane of a group gets it.
 __device__ __forceinline__ uint32_t group_sum32(uint32_t v, int G) {
-    for (int m = 1; m < G; m <<= 1) v += __shfl_xor(v, m);
+    if (G >= 2) v += dpp_mov<kDppXor1, 0xf>(0, v);
+    if (G >= 4) v += dpp_mov<kDppXor2, 0xf>(0, v);
+    if (G >= 8) v += dpp_mov<kDppHalfMirror, 0xf>(0, v);
+    if (G >= 16) v += dpp_mov<kDppMirror, 0xf>(0, v);
     return v;
 }
-__device__ __forceinline__ int wave_or(int v) {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) v |= __shfl_xor(v, m);
-    return v;
+// Value of lane q of each quad, in every lane of the quad.
+template <int Q>
+__device__ __forceinline__ int quad_bcast(int v) {
+    return static_cast<int>(dpp_mov<Q | (Q << 2) | (Q << 4) | (Q << 6), 0xf>(0, static_cast<uint32_t>(v)));
 }
+__device__ __forceinline__ int lane_prefix(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0));
+}
+
+// Global-memory view of a plane pointer (lets loads be scheduled past LDS stores).
+typedef const __attribute__((address_space(1))) uint8_t* gptr_u8;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ gptr_u8 as_global(const uint8_t* p) { return (gptr_u8)(p); }
 
 // llround without library dependence: round half away from zero; x - trunc(x) is exact.
 __device__ __forceinline__ long long llround_exact(double x) {
@@ -185,42 +224,45 @@ struct SubCand {
 
 constexpr int kSeenWords = ((2 * kMaxLocalRange + 1) * (2 * kMaxLocalRange + 1) + 31) / 32;
 constexpr int kMaxSub = 32;
-constexpr int kSmallWin = 64 * 68;  // far windows and sub-pel windows
+constexpr int kMaxUmhKeys = 768;     // sampled-SAD points an integer search can consider
+constexpr int kSmallWin = 80 * 96;   // far windows and sub-pel windows
 
 __host__ __device__ inline int win_capacity(int local_range) {
     const int side = 16 + 2 * local_range;
-    const int pitch = (side + 7) & ~3;
-    const int need = side * pitch;
+    const int pitch = (side + 16 + 8 + 15) & ~15;  // alignment of the origin plus read slack
+    const int need = (side + 1) * pitch;
     return need > kSmallWin ? need : kSmallWin;
 }
 
 struct Lds {
     uint8_t* cur;        // 16x16 current block, zero outside the active area
-    uint8_t* win;        // reference window
+    uint8_t* win;        // reference window (16-byte aligned rows)
     uint32_t* seen_s;    // sampled-SAD dedupe bitset of the UMH search
     uint32_t* seen_f;    // full-SAD dedupe bitset
+    uint64_t* ukeys;     // keys of every sampled point the UMH search considered
     SubCand* sub;        // staged sub-pel candidate list
-    int2* req;           // pending candidate vectors
-    uint64_t* reqcost;   // their costs
-    int32_t* tbuf;       // SATD transpose buffer (64 lanes x 4)
-    uint8_t* pred;       // 16x16 prediction
+    SubCand* sub2;       // its sorted copy
+    int2* req;           // a batch of candidate vectors
+    int2* pend;          // vectors waiting to be scored
+    uint64_t* pendcost;  // their costs
+    int32_t* slot;       // per batch entry: index into the list
     uint64_t* top16;     // long-range top-16 keys
     int2* own;           // current MB's four block vectors
 };
 
 __host__ __device__ inline int lds_bytes_for(int local_range) {
     int off = 0;
-    off += 256;                                    // cur
-    off += (win_capacity(local_range) + 15) & ~15; // win
-    off += kSeenWords * 4 * 2;                     // seen_s, seen_f
+    off += 256;                                     // cur
+    off += (win_capacity(local_range) + 15) & ~15;  // win
+    off += kSeenWords * 4 * 2;                      // seen_s, seen_f
     off = (off + 15) & ~15;
-    off += kMaxSub * static_cast<int>(sizeof(SubCand));
-    off += kMaxSub * 8;                            // req
-    off += kMaxSub * 8;                            // reqcost
-    off += 256 * 4;                                // tbuf
-    off += 256;                                    // pred
-    off += 16 * 8;                                 // top16
-    off += 4 * 8;                                  // own
+    off += kMaxUmhKeys * 8;                         // ukeys
+    off += 2 * kMaxSub * static_cast<int>(sizeof(SubCand));
+    off += kMaxSub * 8 * 2;                         // req, pend
+    off += kMaxSub * 8;                             // pendcost
+    off += kMaxSub * 4;                             // slot
+    off += 16 * 8;                                  // top16
+    off += 4 * 8;                                   // own
     return off + 64;
 }
 
@@ -232,11 +274,13 @@ __device__ Lds carve(uint8_t* smem, int local_range) {
     L.seen_s = reinterpret_cast<uint32_t*>(smem + off); off += kSeenWords * 4;
     L.seen_f = reinterpret_cast<uint32_t*>(smem + off); off += kSeenWords * 4;
     off = (off + 15) & ~15;
+    L.ukeys = reinterpret_cast<uint64_t*>(smem + off); off += kMaxUmhKeys * 8;
     L.sub = reinterpret_cast<SubCand*>(smem + off); off += kMaxSub * static_cast<int>(sizeof(SubCand));
+    L.sub2 = reinterpret_cast<SubCand*>(smem + off); off += kMaxSub * static_cast<int>(sizeof(SubCand));
     L.req = reinterpret_cast<int2*>(smem + off); off += kMaxSub * 8;
-    L.reqcost = reinterpret_cast<uint64_t*>(smem + off); off += kMaxSub * 8;
-    L.tbuf = reinterpret_cast<int32_t*>(smem + off); off += 256 * 4;
-    L.pred = smem + off; off += 256;
+    L.pend = reinterpret_cast<int2*>(smem + off); off += kMaxSub * 8;
+    L.pendcost = reinterpret_cast<uint64_t*>(smem + off); off += kMaxSub * 8;
+    L.slot = reinterpret_cast<int32_t*>(smem + off); off += kMaxSub * 4;
     L.top16 = reinterpret_cast<uint64_t*>(smem + off); off += 16 * 8;
     L.own = reinterpret_cast<int2*>(smem + off); off += 4 * 8;
     return L;
@@ -270,31 +314,47 @@ __device__ void set_err(Ctx& C, int code) {
 }
 
 // Stage a clamped reference window in LDS (sample_clamped semantics,
-// src/encoder_motion_compensation.cpp:350-353).  Returns false if it does
-// not fit the buffer.
+// src/encoder_motion_compensation.cpp:350-353) covering frame columns
+// [wx0, wx0+ww) and rows [wy0, wy0+wh).  The stored origin is rounded down to
+// 16 bytes so whole rows move as 16-byte loads.  Returns false if the window
+// does not fit the buffer.
 __device__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int wh) {
-    const int wp = (ww + 7) & ~3;
-    if (wp * wh > C.winCap || ww <= 0 || wh <= 0) return false;
+    const int ax0 = wx0 & ~15;
+    const int aw = wx0 + ww - ax0;
+    const int wp = (aw + 8 + 15) & ~15;
+    if (ww <= 0 || wh <= 0 || wp * wh > C.winCap) return false;
     wsync();
     const int W = C.g.W, H = C.g.H;
-    const uint8_t* ry = C.ry;
-    if (wx0 >= 0 && wx0 + wp <= W && !(W & 3) && !(wx0 & 3)) {
-        const int dpr = wp >> 2;
-        const int total = dpr * wh;
-        uint32_t* w32 = reinterpret_cast<uint32_t*>(C.L.win);
-        for (int i = C.lane; i < total; i += kWave) {
-            const int r = i / dpr, d = i - r * dpr;
-            const int sy = clampi(wy0 + r, 0, H - 1);
-            w32[i] = *reinterpret_cast<const uint32_t*>(ry + static_cast<size_t>(sy) * W + wx0 + d * 4);
+    const int cpr = wp >> 4;
+    const int total = cpr * wh;
+    const bool aligned_rows = (W & 15) == 0;
+    gptr_u8 ry = as_global(C.ry);
+    uint4* w16 = reinterpret_cast<uint4*>(C.L.win);
+#pragma unroll 4
+    for (int i = C.lane; i < total; i += kWave) {
+        const int r = i / cpr, c = i - r * cpr;
+        const int sy = clampi(wy0 + r, 0, H - 1);
+        const int x = ax0 + c * 16;
+        gptr_u8 row = ry + static_cast<size_t>(sy) * W;
+        uint4 v;
+        if (aligned_rows && x >= 0 && x + 16 <= W) {
+            const u32x4 t = *reinterpret_cast<const __attribute__((address_space(1))) u32x4*>(row + x);
+            v = make_uint4(t.x, t.y, t.z, t.w);
+        } else {
+            uint32_t wv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc |= static_cast<uint32_t>(row[clampi(x + q * 4 + b, 0, W - 1)]) << (8 * b);
+                wv[q] = acc;
+            }
+            v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
         }
-    } else {
-        for (int r = 0; r < wh; ++r) {
-            const int sy = clampi(wy0 + r, 0, H - 1);
-            const uint8_t* row = ry + static_cast<size_t>(sy) * W;
-            for (int x = C.lane; x < wp; x += kWave) C.L.win[r * wp + x] = row[clampi(wx0 + x, 0, W - 1)];
-        }
+        w16[i] = v;
     }
-    C.wx0 = wx0; C.wy0 = wy0; C.ww = ww; C.wh = wh; C.wp = wp;
+    C.wx0 = ax0; C.wy0 = wy0; C.ww = aw; C.wh = wh; C.wp = wp;
     wsync();
     return true;
 }
@@ -342,43 +402,18 @@ __device__ __forceinline__ bool in_window(const Ctx& C, int dx, int dy) {
     return ox >= 0 && oy >= 0 && ox + 16 <= C.ww && oy + 16 <= C.wh;
 }
 
-// Sorted list of the eight best (cost, |mv|, y, x) keys seen so far: the
-// top-8 list of integer_motion_umh (src/encoder_motion.cpp:1802-1838).  Keys
-// are unique, so "the eight smallest of all considered points" is exact.
-struct Top8 {
-    uint64_t k[8];
-    int n;
-};
-
-__device__ void top_merge(Top8& t, uint64_t key) {
-    for (int it = 0; it < 8; ++it) {
-        const uint64_t thr = t.n < 8 ? kNone : t.k[7];
-        const uint64_t mk = wave_min64(key < thr ? key : kNone);
-        if (mk == kNone) break;
-        int pos = 0;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) pos += (j < t.n && t.k[j] < mk) ? 1 : 0;
-#pragma unroll
-        for (int j = 7; j > 0; --j)
-            if (j > pos) t.k[j] = t.k[j - 1];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (j == pos) t.k[j] = mk;
-        t.n = imin(t.n + 1, 8);
-        if (key == mk) key = kNone;
-    }
-}
-
 enum { KEY_TOTAL = 0, KEY_SEQ_MAN = 1, KEY_SEQ = 2 };
 
 // Score n integer candidates (pt(i) -> (dx,dy), already clamped) at the given
-// sample stride from the current window.  seen: dedupe bitset or null.
-// Returns the wave-minimum key; optionally merges all keys into top.
+// sample stride from the current window, G lanes per candidate.  seen: dedupe
+// bitset or null.  ukeys/ucount: optional log of every scored key.  Returns
+// the minimum key of the batch (kNone if no candidate was scored).
 template <class PtFn>
-__device__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int keymode, int seq0, PtFn pt, Top8* top) {
+__device__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int keymode, int seq0, PtFn pt,
+                              uint64_t* ukeys = nullptr, int* ucount = nullptr) {
     const int rows = (C.ah + stride - 1) / stride;
     int G = 1;
-    while (G * 2 * n <= kWave && G * 2 <= rows) G *= 2;
+    while (G * 2 * n <= kWave && G * 2 <= rows && G < 16) G *= 2;
     const int per = kWave / G;
     const int part = C.lane & (G - 1);
     uint64_t best = kNone;
@@ -388,12 +423,14 @@ __device__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int key
         int dx = 0, dy = 0;
         if (act) pt(i, dx, dy);
         int valid = act ? 1 : 0;
-        if (act && seen && part == 0) {
-            const int bit = (dy + kMaxLocalRange) * (2 * kMaxLocalRange + 1) + (dx + kMaxLocalRange);
-            const uint32_t old = atomicOr(&seen[bit >> 5], 1u << (bit & 31));
-            valid = (old >> (bit & 31)) & 1 ? 0 : 1;
+        if (seen) {
+            if (act && part == 0) {
+                const int bit = (dy + kMaxLocalRange) * (2 * kMaxLocalRange + 1) + (dx + kMaxLocalRange);
+                const uint32_t old = atomicOr(&seen[bit >> 5], 1u << (bit & 31));
+                valid = (old >> (bit & 31)) & 1 ? 0 : 1;
+            }
+            valid = __shfl(valid, C.lane - part);
         }
-        if (seen) valid = __shfl(valid, C.lane - part);
         uint32_t sum = 0;
         bool hid = false;
         if (valid) {
@@ -414,13 +451,20 @@ __device__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int key
             C.n_sad += counted * npx;
         }
         uint64_t key = kNone;
-        if (valid && part == 0) {
+        const bool leader = valid && part == 0;
+        if (leader) {
             const uint64_t cost = hid ? kBadSad : static_cast<uint64_t>(sum) * static_cast<uint64_t>(stride * stride);
             if (keymode == KEY_TOTAL) key = key_int(cost, dx, dy);
             else if (keymode == KEY_SEQ_MAN) key = key_seq(cost, iabs(dx) + iabs(dy), seq0 + i);
             else key = key_seq(cost, 0, seq0 + i);
         }
-        if (top) top_merge(*top, key);
+        if (ukeys) {
+            const uint64_t m = __ballot(leader);
+            const int pos = *ucount + lane_prefix(m);
+            if (leader && pos < kMaxUmhKeys) ukeys[pos] = key;
+            *ucount += __popcll(m);
+            if (*ucount > kMaxUmhKeys) set_err(C, 8);
+        }
         const uint64_t mk = wave_min64(key);
         best = mk < best ? mk : best;
     }
@@ -640,170 +684,233 @@ __device__ __forceinline__ int mspel(int a, int b, int c, int d, int mode) {
     return -3 * a + 18 * b + 53 * c - 4 * d;
 }
 
-// sample at window coordinates (sx, sy) = integer source position, fractional (hm, vm)
-__device__ __forceinline__ int interp_px(const uint8_t* w, int wp, int sx, int sy, int hm, int vm, bool bil) {
-#define S(xx, yy) static_cast<int>(w[(yy) * wp + (xx)])
-    if (bil) {
-        const int a = S(sx, sy);
-        if (!hm && !vm) return a;
-        const int b = S(sx + 1, sy), c = S(sx, sy + 1);
-        if (hm && vm) return (a + b + c + S(sx + 1, sy + 1) + 2) >> 2;
-        if (hm) return (a + b + 1) >> 1;
-        return (a + c + 1) >> 1;
-    }
-    if (!hm && !vm) return S(sx, sy);
-    int v;
-    if (hm && vm) {
-        const int shift = ((hm == 2 ? 1 : 5) + (vm == 2 ? 1 : 5)) >> 1;
-        const int r1 = (1 << (shift - 1)) - 1;
-        int t0 = arshift(mspel(S(sx - 1, sy - 1), S(sx - 1, sy), S(sx - 1, sy + 1), S(sx - 1, sy + 2), vm) + r1, shift);
-        int t1 = arshift(mspel(S(sx, sy - 1), S(sx, sy), S(sx, sy + 1), S(sx, sy + 2), vm) + r1, shift);
-        int t2 = arshift(mspel(S(sx + 1, sy - 1), S(sx + 1, sy), S(sx + 1, sy + 1), S(sx + 1, sy + 2), vm) + r1, shift);
-        int t3 = arshift(mspel(S(sx + 2, sy - 1), S(sx + 2, sy), S(sx + 2, sy + 1), S(sx + 2, sy + 2), vm) + r1, shift);
-        v = arshift(mspel(t0, t1, t2, t3, hm) + 64, 7);
-    } else if (vm) {
-        v = arshift(mspel(S(sx, sy - 1), S(sx, sy), S(sx, sy + 1), S(sx, sy + 2), vm) + (vm == 2 ? 7 : 31),
-                    vm == 2 ? 4 : 6);
-    } else {
-        v = arshift(mspel(S(sx - 1, sy), S(sx, sy), S(sx + 1, sy), S(sx + 2, sy), hm) + (hm == 2 ? 8 : 32),
-                    hm == 2 ? 4 : 6);
-    }
-#undef S
-    return clampi(v, 0, 255);
-}
-
 // Window covering the interpolation footprint of block (bx0,by0,bw,bh) for all
-// vectors in [xq_lo,xq_hi]x[yq_lo,yq_hi] (taps -1..+2 around each integer position).
+// vectors in [xq_lo,xq_hi]x[yq_lo,yq_hi] (taps -1..+2 around each integer
+// position, plus 8 bytes of row slack for the 4-pixel reads).
 __device__ bool qpel_window(Ctx& C, int bx0, int by0, int bw, int bh, int xq_lo, int xq_hi, int yq_lo, int yq_hi) {
     const int ix_lo = floor_div4(xq_lo), ix_hi = floor_div4(xq_hi);
     const int iy_lo = floor_div4(yq_lo), iy_hi = floor_div4(yq_hi);
-    const int wx0 = bx0 + ix_lo - 1, wy0 = by0 + iy_lo - 1;
-    const int ww = (ix_hi - ix_lo) + bw + 3, wh = (iy_hi - iy_lo) + bh + 3;
-    if (C.wx0 == wx0 && C.wy0 == wy0 && C.ww >= ww && C.wh >= wh && C.wp * C.wh <= C.winCap) return true;
-    return win_load(C, wx0, wy0, ww, wh);
+    const int nx0 = bx0 + ix_lo - 1, ny0 = by0 + iy_lo - 1;
+    const int nx1 = bx0 + ix_hi + bw + 6, ny1 = by0 + iy_hi + bh + 2;  // exclusive
+    if (nx0 >= C.wx0 && nx1 <= C.wx0 + C.ww && ny0 >= C.wy0 && ny1 <= C.wy0 + C.wh) return true;
+    return win_load(C, nx0, ny0, nx1 - nx0, ny1 - ny0);
 }
 
-// SAD (src/encoder_motion.cpp:2605-2667 law) of one candidate on block
-// (bx0,by0,bw,bh) with active (aw,ah); the window must cover it.
-__device__ uint32_t qpel_sad_one(const Ctx& C, int bx0, int by0, int bw, int aw, int ah, int xq, int yq, bool bil) {
-    const int ix = floor_div4(xq), iy = floor_div4(yq);
-    const int hm = xq - ix * 4, vm = yq - iy * 4;
-    uint32_t acc = 0;
-    const int npx = bw * bw;  // square blocks: 16x16 or 8x8
-    for (int p = C.lane; p < npx; p += kWave) {
-        const int x = bw == 16 ? (p & 15) : (p & 7);
-        const int y = bw == 16 ? (p >> 4) : (p >> 3);
-        if (x < aw && y < ah) {
-            const int sx = bx0 + x + ix - C.wx0, sy = by0 + y + iy - C.wy0;
-            const int pred = interp_px(C.L.win, C.wp, sx, sy, hm, vm, bil);
-            const int cv = C.L.cur[(by0 - C.y0 + y) * 16 + (bx0 - C.x0 + x)];
-            acc += static_cast<uint32_t>(iabs(cv - pred));
-        }
+// Four horizontally adjacent predicted samples (luma_mc_sample /
+// luma_mc_sample_mode, src/encoder_motion_compensation.cpp:367-415) packed
+// little-endian; (sx, sy) is the window position of the first sample's
+// integer source pixel.
+__device__ __forceinline__ uint32_t pred4(const uint8_t* w, int wp, int sx, int sy, int hm, int vm, bool bil) {
+    // 8 window bytes of row yy starting at column sx-1
+    auto row8 = [&](int yy, uint32_t& lo, uint32_t& hi) {
+        const int a = sx - 1;
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(w + yy * wp + (a & ~3));
+        const uint32_t d0 = p[0], d1 = p[1], d2 = p[2];
+        lo = __builtin_amdgcn_alignbyte(d1, d0, a & 3);
+        hi = __builtin_amdgcn_alignbyte(d2, d1, a & 3);
+    };
+    auto byte_at = [](uint32_t lo, uint32_t hi, int k) -> int {
+        return static_cast<int>(k < 4 ? (lo >> (8 * k)) & 255u : (hi >> (8 * (k - 4))) & 255u);
+    };
+    if (!hm && !vm) {
+        uint32_t lo, hi;
+        row8(sy, lo, hi);
+        return __builtin_amdgcn_alignbyte(hi, lo, 1);
     }
-    return wave_sum32(acc);
+    uint32_t out = 0;
+    if (bil) {
+        uint32_t l0, h0, l1 = 0, h1 = 0;
+        row8(sy, l0, h0);
+        if (vm) row8(sy + 1, l1, h1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int a = byte_at(l0, h0, j + 1), b = byte_at(l0, h0, j + 2);
+            int v;
+            if (hm && vm) v = (a + b + byte_at(l1, h1, j + 1) + byte_at(l1, h1, j + 2) + 2) >> 2;
+            else if (hm) v = (a + b + 1) >> 1;
+            else v = (a + byte_at(l1, h1, j + 1) + 1) >> 1;
+            out |= static_cast<uint32_t>(v) << (8 * j);
+        }
+        return out;
+    }
+    if (!vm) {
+        uint32_t lo, hi;
+        row8(sy, lo, hi);
+        const int bias = hm == 2 ? 8 : 32, sh = hm == 2 ? 4 : 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int v = arshift(mspel(byte_at(lo, hi, j), byte_at(lo, hi, j + 1), byte_at(lo, hi, j + 2),
+                                        byte_at(lo, hi, j + 3), hm) + bias, sh);
+            out |= static_cast<uint32_t>(clampi(v, 0, 255)) << (8 * j);
+        }
+        return out;
+    }
+    uint32_t L[4], Hh[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) row8(sy - 1 + k, L[k], Hh[k]);
+    if (!hm) {
+        const int bias = vm == 2 ? 7 : 31, sh = vm == 2 ? 4 : 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int v = arshift(mspel(byte_at(L[0], Hh[0], j + 1), byte_at(L[1], Hh[1], j + 1),
+                                        byte_at(L[2], Hh[2], j + 1), byte_at(L[3], Hh[3], j + 1), vm) + bias, sh);
+            out |= static_cast<uint32_t>(clampi(v, 0, 255)) << (8 * j);
+        }
+        return out;
+    }
+    const int shift = ((hm == 2 ? 1 : 5) + (vm == 2 ? 1 : 5)) >> 1;
+    const int r1 = (1 << (shift - 1)) - 1;
+    int t[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c)
+        t[c] = arshift(mspel(byte_at(L[0], Hh[0], c), byte_at(L[1], Hh[1], c), byte_at(L[2], Hh[2], c),
+                             byte_at(L[3], Hh[3], c), vm) + r1, shift);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int v = arshift(mspel(t[j], t[j + 1], t[j + 2], t[j + 3], hm) + 64, 7);
+        out |= static_cast<uint32_t>(clampi(v, 0, 255)) << (8 * j);
+    }
+    return out;
 }
 
-// Score n quarter-pel candidates (L.req[0..n)) by interpolated SAD into
-// L.reqcost, applying the hidden-padding veto of block_sad_qpel /
-// block_sad_motion (src/encoder_motion.cpp:2286-2293, :2670-2676).
-__device__ void qpel_sad_batch(Ctx& C, int bx0, int by0, int bw, int bh, int n, int mode) {
+// Lane layout of a block: lane -> (4x4 tile, tile row); a 16x16 block uses all
+// 64 lanes, an 8x8 block 16 lanes (four blocks or candidates per wave).
+struct LanePix {
+    int x, y;   // first of the lane's four samples, block coordinates
+    int tx, ty; // its tile origin
+    int r;      // row inside the tile
+};
+__device__ __forceinline__ LanePix lane_pix(int lane, int bw) {
+    LanePix p;
+    const int l = bw == 16 ? lane : (lane & 15);
+    const int tile = l >> 2;
+    p.r = l & 3;
+    const int tpr = bw / 4;
+    p.tx = (tile % tpr) * 4;
+    p.ty = (tile / tpr) * 4;
+    p.x = p.tx;
+    p.y = p.ty + p.r;
+    return p;
+}
+
+// Sum over the lanes that hold one candidate (64 for 16x16, 16 for 8x8).
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, int bw) {
+    if (bw == 16) return wave_sum32(v);
+    return group_sum32(v, 16);
+}
+
+// Costs of the candidates vs[0..n) on block (bx0,by0,bw,bh): metric 0 = SAD
+// with the hidden-padding veto of block_sad_qpel / block_sad_motion
+// (src/encoder_motion.cpp:2286-2293, :2670-2676); metric 1 = SATD of
+// block_satd_motion (src/encoder_motion.cpp:2962-3042: rendered prediction,
+// (|H D H^T| + 1) >> 1 per 4x4 tile, zero residual outside the active area).
+// Results in out[0..n); every lane can read them on return.
+__device__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* vs, int n, int mode, int metric,
+                          uint64_t* out) {
+    if (n <= 0) return;
     const bool bil = mode_bilinear(mode);
     const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
     const bool chroma = chroma_block(bx0, by0, bw, bh);
     int xlo = 1 << 30, xhi = -(1 << 30), ylo = 1 << 30, yhi = -(1 << 30);
     for (int i = 0; i < n; ++i) {
-        const int2 v = C.L.req[i];
+        const int2 v = vs[i];
         xlo = imin(xlo, v.x); xhi = imax(xhi, v.x); ylo = imin(ylo, v.y); yhi = imax(yhi, v.y);
     }
-    bool shared = n > 0 && qpel_window(C, bx0, by0, bw, bh, xlo, xhi, ylo, yhi);
-    for (int i = 0; i < n; ++i) {
-        const int2 v = C.L.req[i];
-        uint64_t cost;
-        if (hidden_padding(C.g, bx0, by0, bw, bh, v.x, v.y, bil ? mode : 0, chroma)) cost = kBadSad;
-        else if (bil && ((v.x & 1) || (v.y & 1))) cost = kBadSad;
-        else {
-            if (!shared) {
-                if (!qpel_window(C, bx0, by0, bw, bh, v.x, v.x, v.y, v.y)) { set_err(C, 3); cost = kBadSad; goto store; }
-            }
-            cost = qpel_sad_one(C, bx0, by0, bw, aw, ah, v.x, v.y, bil);
-            C.n_sad += static_cast<uint64_t>(aw) * ah;
-            C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(v.x & 3, v.y & 3, bil);
-        }
-    store:
-        wsync();
-        if (C.lane == 0) C.L.reqcost[i] = cost;
-        wsync();
-        if (!shared && i + 1 < n) {
-            // per-candidate windows: nothing to keep
-        }
-    }
-}
-
-// block_satd_motion, src/encoder_motion.cpp:2962-3042: render the prediction,
-// then sum (|H D H^T| + 1) >> 1 over 4x4 tiles; partial tiles see zero residual.
-__device__ uint64_t satd_one(Ctx& C, int bx0, int by0, int bw, int bh, int xq, int yq, int mode) {
-    const bool bil = mode_bilinear(mode);
-    const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
-    if (!aw || !ah) return 0;
-    if (mode_halfpel(mode) && ((xq & 1) || (yq & 1))) return kBadSad;
-    if (!qpel_window(C, bx0, by0, bw, bh, xq, xq, yq, yq)) { set_err(C, 4); return kBadSad; }
-    const int ix = floor_div4(xq), iy = floor_div4(yq);
-    const int hm = xq - ix * 4, vm = yq - iy * 4;
-    const int npx = bw * bh;
-    for (int p = C.lane; p < npx; p += kWave) {
-        const int x = bw == 16 ? (p & 15) : (p & 7);
-        const int y = bw == 16 ? (p >> 4) : (p >> 3);
-        int pv = 0;
-        if (x < aw && y < ah) pv = interp_px(C.L.win, C.wp, bx0 + x + ix - C.wx0, by0 + y + iy - C.wy0, hm, vm, bil);
-        C.L.pred[y * 16 + x] = static_cast<uint8_t>(pv);
-    }
-    wsync();
-    const int tpr = bw / 4;                 // tiles per row
-    const int ntiles = tpr * (bh / 4);
-    const int tile = C.lane >> 2, r = C.lane & 3;
-    const bool tact = tile < ntiles;
-    int tx = 0, ty = 0;
-    bool tvalid = false;
-    if (tact) {
-        tx = (tile % tpr) * 4;
-        ty = (tile / tpr) * 4;
-        tvalid = tx < aw && ty < ah;
-    }
-    int d[4] = {0, 0, 0, 0};
-    if (tvalid) {
-        const int y = ty + r;
+    const bool shared = qpel_window(C, bx0, by0, bw, bh, xlo, xhi, ylo, yhi);
+    const LanePix lp = lane_pix(C.lane, bw);
+    const int per = shared ? (bw == 16 ? 1 : 4) : 1;
+    const int g = bw == 16 ? 0 : (C.lane >> 4);
+    const uint32_t cur4 = *reinterpret_cast<const uint32_t*>(C.L.cur + (by0 - C.y0 + lp.y) * 16 + (bx0 - C.x0 + lp.x));
+    uint32_t amask = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int x = tx + j;
-            if (x < aw && y < ah) {
-                const int cv = C.L.cur[(by0 - C.y0 + y) * 16 + (bx0 - C.x0 + x)];
-                d[j] = cv - static_cast<int>(C.L.pred[y * 16 + x]);
+    for (int j = 0; j < 4; ++j)
+        if (lp.x + j < aw && lp.y < ah) amask |= 0xffu << (8 * j);
+    const bool tvalid = lp.tx < aw && lp.ty < ah;
+    for (int base = 0; base < n; base += per) {
+        const int c = base + g;
+        const bool mine = c < n && g < per;
+        int2 v = make_int2(0, 0);
+        if (mine) v = vs[c];
+        // uniform per candidate: veto and window
+        bool skip = !mine;
+        uint64_t skip_cost = kBadSad;
+        if (mine && metric == 0) {
+            if (hidden_padding(C.g, bx0, by0, bw, bh, v.x, v.y, bil ? mode : 0, chroma)) skip = true;
+            else if (bil && ((v.x & 1) || (v.y & 1))) skip = true;
+        }
+        if (mine && metric == 1) {
+            if (!aw || !ah) { skip = true; skip_cost = 0; }
+            else if (mode_halfpel(mode) && ((v.x & 1) || (v.y & 1))) skip = true;
+        }
+        if (!shared) {
+            // one candidate at a time, in its own window (the vector is uniform here)
+            const int2 u = vs[base];
+            if (!qpel_window(C, bx0, by0, bw, bh, u.x, u.x, u.y, u.y)) { set_err(C, 3); skip = true; }
+        }
+        const int ix = floor_div4(v.x), iy = floor_div4(v.y);
+        const int hm = v.x - ix * 4, vm = v.y - iy * 4;
+        uint32_t part = 0;
+        if (!skip) {
+            const int sx = bx0 + lp.x + ix - C.wx0, sy = by0 + lp.y + iy - C.wy0;
+            const uint32_t p4 = pred4(C.L.win, C.wp, sx, sy, hm, vm, bil);
+            if (metric == 0) {
+                part = __builtin_amdgcn_sad_u8(p4 & amask, cur4 & amask, 0u);
+            } else {
+                int d[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    d[j] = ((amask >> (8 * j)) & 255u)
+                               ? static_cast<int>((cur4 >> (8 * j)) & 255u) - static_cast<int>((p4 >> (8 * j)) & 255u)
+                               : 0;
+                const int a0 = d[0] + d[3], a1 = d[1] + d[2], a2 = d[1] - d[2], a3 = d[0] - d[3];
+                const int t[4] = {a0 + a1, a3 + a2, a0 - a1, a3 - a2};
+                const int s1 = (lp.r == 0 || lp.r == 1) ? 1 : -1;
+                const int s2 = (lp.r == 0 || lp.r == 3) ? 1 : -1;
+                const int s3 = (lp.r == 0 || lp.r == 2) ? 1 : -1;
+                int acc = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int R0 = quad_bcast<0>(t[k]), R1 = quad_bcast<1>(t[k]);
+                    const int R2 = quad_bcast<2>(t[k]), R3 = quad_bcast<3>(t[k]);
+                    acc += iabs(R0 + s1 * R1 + s2 * R2 + s3 * R3);
+                }
+                uint32_t ts = static_cast<uint32_t>(acc);
+                ts += dpp_mov<kDppXor1, 0xf>(0, ts);
+                ts += dpp_mov<kDppXor2, 0xf>(0, ts);
+                part = (tvalid && lp.r == 0) ? ((ts + 1) >> 1) : 0u;
             }
         }
+        const uint32_t total = block_sum(part, bw);
+        // op counts of the candidates scored in this pass
+        {
+            const uint64_t lead = __ballot(!skip && ((C.lane & (bw == 16 ? 63 : 15)) == 0));
+            const uint64_t cnt = __popcll(lead);
+            C.n_sad += metric == 0 ? cnt * static_cast<uint64_t>(aw) * ah : 0;
+            if (cnt) {
+                // taps depend on the phase of each candidate
+                for (int q = 0; q < per; ++q) {
+                    const int cc = base + q;
+                    if (cc >= n) break;
+                    const int2 vv = vs[cc];
+                    bool sk;
+                    if (metric == 0)
+                        sk = hidden_padding(C.g, bx0, by0, bw, bh, vv.x, vv.y, bil ? mode : 0, chroma) ||
+                             (bil && ((vv.x & 1) || (vv.y & 1)));
+                    else
+                        sk = !aw || !ah || (mode_halfpel(mode) && ((vv.x & 1) || (vv.y & 1)));
+                    if (!sk) {
+                        C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(vv.x & 3, vv.y & 3, bil);
+                        if (metric == 1) {
+                            const uint64_t tiles = static_cast<uint64_t>((aw + 3) / 4) * ((ah + 3) / 4);
+                            C.n_satd += 112ull * tiles;
+                        }
+                    }
+                }
+            }
+        }
+        if (mine && (C.lane & (bw == 16 ? 63 : 15)) == 0) out[c] = skip ? skip_cost : static_cast<uint64_t>(total);
     }
-    const int a0 = d[0] + d[3], a1 = d[1] + d[2], a2 = d[1] - d[2], a3 = d[0] - d[3];
-    C.L.tbuf[C.lane * 4 + 0] = a0 + a1;
-    C.L.tbuf[C.lane * 4 + 1] = a3 + a2;
-    C.L.tbuf[C.lane * 4 + 2] = a0 - a1;
-    C.L.tbuf[C.lane * 4 + 3] = a3 - a2;
     wsync();
-    uint32_t colsum = 0;
-    if (tvalid) {
-        const int b = (C.lane & ~3) * 4;
-        const int v0 = C.L.tbuf[b + 0 * 4 + r], v1 = C.L.tbuf[b + 1 * 4 + r];
-        const int v2 = C.L.tbuf[b + 2 * 4 + r], v3 = C.L.tbuf[b + 3 * 4 + r];
-        const int c0 = v0 + v3, c1 = v1 + v2, c2 = v1 - v2, c3 = v0 - v3;
-        colsum = static_cast<uint32_t>(iabs(c0 + c1) + iabs(c3 + c2) + iabs(c0 - c1) + iabs(c3 - c2));
-    }
-    uint32_t tsum = colsum;
-    tsum += __shfl_xor(tsum, 1);
-    tsum += __shfl_xor(tsum, 2);
-    const uint32_t contrib = (tvalid && r == 0) ? ((tsum + 1) >> 1) : 0;
-    const uint32_t total = wave_sum32(contrib);
-    C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(hm, vm, bil);
-    C.n_satd += 112ull * __popcll(__ballot(tvalid && r == 0));
-    wsync();
-    return total;
 }
 
 // ------------------------------------------------------------------ integer searches
@@ -813,7 +920,12 @@ struct IRes {
     uint64_t sad;
 };
 
-// integer_motion_umh, src/encoder_motion.cpp:1781-2023
+// integer_motion_umh, src/encoder_motion.cpp:1781-2023.  The reference keeps
+// a running best and a top-8 list; with distinct points both are functions of
+// the set of points considered so far, so each stage scores its points
+// together and only the stage minimum updates `best`.  Every considered key
+// is logged, and the finalists (the best four of the top-8) are extracted
+// from the log once, at the end.
 __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
     // the whole search stays inside [-R,R]^2: one window serves every stage
     if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
@@ -822,11 +934,14 @@ __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
     }
     for (int i = C.lane; i < kSeenWords; i += kWave) { C.L.seen_s[i] = 0; C.L.seen_f[i] = 0; }
     wsync();
-    Top8 top;
-    top.n = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) top.k[j] = kNone;
+    uint64_t best = kNone;
+    int nkeys = 0;
+    uint64_t* uk = C.L.ukeys;
     auto cl = [R](int v) { return clampi(v, -R, R); };
+    auto stage = [&](int n, auto pt) {
+        const uint64_t m = int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, pt, uk, &nkeys);
+        best = m < best ? m : best;
+    };
 
     // predictor-driven seeds
     {
@@ -836,38 +951,35 @@ __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
         sx[n] = cl(gdx); sy[n] = cl(gdy); ++n;
         if (pi.a_valid) { sx[n] = cl(round_qpel_to_pixel(pi.ax)); sy[n] = cl(round_qpel_to_pixel(pi.ay)); ++n; }
         if (pi.c_valid) { sx[n] = cl(round_qpel_to_pixel(pi.cx)); sy[n] = cl(round_qpel_to_pixel(pi.cy)); ++n; }
-        int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        stage(n, [&](int i, int& dx, int& dy) {
             dx = i == 0 ? sx[0] : i == 1 ? sx[1] : i == 2 ? sx[2] : i == 3 ? sx[3] : sx[4];
             dy = i == 0 ? sy[0] : i == 1 ? sy[1] : i == 2 ? sy[2] : i == 3 ? sy[3] : sy[4];
-        }, &top);
+        });
     }
     // local 5x5
     {
-        const int cx = key_dx(top.k[0]), cy = key_dy(top.k[0]);
-        int_batch(C, 25, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
-            dx = cl(cx + (i % 5) - 2); dy = cl(cy + (i / 5) - 2);
-        }, &top);
+        const int cx = key_dx(best), cy = key_dy(best);
+        stage(25, [&](int i, int& dx, int& dy) { dx = cl(cx + (i % 5) - 2); dy = cl(cy + (i / 5) - 2); });
     }
     // sparse lattice
     if (R >= 8) {
         const int side = (2 * R) / 8 + 1;
-        const int n = side * side + 2 * side;
-        int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        stage(side * side + 2 * side, [&](int i, int& dx, int& dy) {
             if (i < side * side) { dx = -R + 8 * (i % side); dy = -R + 8 * (i / side); }
             else {
                 const int j = i - side * side, d = -R + 8 * (j >> 1);
                 if (j & 1) { dx = d; dy = R; } else { dx = R; dy = d; }
             }
             dx = cl(dx); dy = cl(dy);
-        }, &top);
+        });
     }
     // uneven cross
     {
         const int cs = R >= 96 ? 8 : (R >= 32 ? 4 : 2);
-        const int cx = key_dx(top.k[0]), cy = key_dy(top.k[0]);
+        const int cx = key_dx(best), cy = key_dy(best);
         const int vlimit = imin(R, imax(8, R / 2));
         const int nh = R / cs, nv = vlimit / cs;
-        int_batch(C, 2 * (nh + nv), 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        stage(2 * (nh + nv), [&](int i, int& dx, int& dy) {
             if (i < 2 * nh) {
                 const int d = cs * (i / 2 + 1);
                 dx = (i & 1) ? cx + d : cx - d; dy = cy;
@@ -876,14 +988,13 @@ __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
                 dx = cx; dy = (j & 1) ? cy + d : cy - d;
             }
             dx = cl(dx); dy = cl(dy);
-        }, &top);
+        });
     }
     // multi-hexagon rings
     {
         const int hs = R >= 96 ? 8 : (R >= 24 ? 4 : 2);
-        const int hx = key_dx(top.k[0]), hy = key_dy(top.k[0]);
-        const int rings = R / hs;
-        int_batch(C, 6 * rings, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        const int hx = key_dx(best), hy = key_dy(best);
+        stage(6 * (R / hs), [&](int i, int& dx, int& dy) {
             const int r = hs * (i / 6 + 1), h = (r + 1) / 2, k = i % 6;
             switch (k) {
                 case 0: dx = hx - r; dy = hy; break;
@@ -894,12 +1005,12 @@ __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
                 default: dx = hx + h; dy = hy + r; break;
             }
             dx = cl(dx); dy = cl(dy);
-        }, &top);
+        });
     }
     // iterative small hexagon
     for (int iter = 0; iter < 32; ++iter) {
-        const int ox = key_dx(top.k[0]), oy = key_dy(top.k[0]);
-        int_batch(C, 6, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        const int ox = key_dx(best), oy = key_dy(best);
+        stage(6, [&](int i, int& dx, int& dy) {
             switch (i) {
                 case 0: dx = ox - 2; dy = oy; break;
                 case 1: dx = ox + 2; dy = oy; break;
@@ -909,31 +1020,46 @@ __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
                 default: dx = ox + 1; dy = oy + 2; break;
             }
             dx = cl(dx); dy = cl(dy);
-        }, &top);
-        if (key_dx(top.k[0]) == ox && key_dy(top.k[0]) == oy) break;
+        });
+        if (key_dx(best) == ox && key_dy(best) == oy) break;
     }
     // closing diamond
     {
-        const int ox = key_dx(top.k[0]), oy = key_dy(top.k[0]);
-        int_batch(C, 4, 2, C.L.seen_s, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
-            dx = ox + (i == 0 ? -1 : i == 1 ? 1 : 0);
-            dy = oy + (i == 2 ? -1 : i == 3 ? 1 : 0);
-            dx = cl(dx); dy = cl(dy);
-        }, &top);
+        const int ox = key_dx(best), oy = key_dy(best);
+        stage(4, [&](int i, int& dx, int& dy) {
+            dx = cl(ox + (i == 0 ? -1 : i == 1 ? 1 : 0));
+            dy = cl(oy + (i == 2 ? -1 : i == 3 ? 1 : 0));
+        });
+    }
+    // the best four considered points (top[0..3] of the reference's list)
+    wsync();
+    uint64_t tk[4] = {kNone, kNone, kNone, kNone};
+    const int nk = imin(nkeys, kMaxUmhKeys);
+    const int fin = imin(4, nk);
+    {
+        uint64_t prev = 0;
+        bool first = true;
+        for (int f = 0; f < fin; ++f) {
+            uint64_t m = kNone;
+            for (int i = C.lane; i < nk; i += kWave) {
+                const uint64_t k = uk[i];
+                if ((first || k > prev) && k < m) m = k;
+            }
+            m = wave_min64(m);
+            if (f == 0) tk[0] = m; else if (f == 1) tk[1] = m; else if (f == 2) tk[2] = m; else tk[3] = m;
+            prev = m;
+            first = false;
+        }
     }
     // full-resolution finalists: 3x3 around the best four
-    const int fin = imin(4, top.n);
-    uint64_t tk[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) tk[j] = top.k[j];
     uint64_t full = int_batch(C, fin * 9, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
         const int f = i / 9, o = i % 9;
         const uint64_t k = f == 0 ? tk[0] : f == 1 ? tk[1] : f == 2 ? tk[2] : tk[3];
         dx = cl(key_dx(k) + (o % 3) - 1); dy = cl(key_dy(k) + (o / 3) - 1);
-    }, nullptr);
+    });
     if (full == kNone) {
-        const int bx = key_dx(top.k[0]), by = key_dy(top.k[0]);
-        full = int_batch(C, 1, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int, int& dx, int& dy) { dx = bx; dy = by; }, nullptr);
+        const int bx = key_dx(best), by = key_dy(best);
+        full = int_batch(C, 1, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int, int& dx, int& dy) { dx = bx; dy = by; });
     }
     // rescue of genuinely poor matches
     if (R > 0 && !match_good(C, key_cost(full))) {
@@ -946,27 +1072,27 @@ __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
                 const int j = i - side * side, d = -R + step * (j >> 1);
                 if (j & 1) { dx = d; dy = R; } else { dx = R; dy = d; }
             }
-        }, nullptr);
+        });
         const int cdx = key_dx(coarse), cdy = key_dy(coarse);
         const int rr = imax(1, step - 1);
         const int xa = imax(-R, cdx - rr), xb = imin(R, cdx + rr), ya = imax(-R, cdy - rr), yb = imin(R, cdy + rr);
         const int rw = xb - xa + 1, rh = yb - ya + 1;
         const uint64_t rescue = int_batch(C, rw * rh, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
             dx = xa + i % rw; dy = ya + i / rw;
-        }, nullptr);
+        });
         if (rescue < full) full = rescue;
         const uint64_t px = active_pixels(C);
         if (px && key_cost(full) > px * 24u && R <= 32) {
             const int ds = (2 * R) / 2 + 1;
             const uint64_t dense = int_batch(C, ds * ds, 2, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
                 dx = -R + 2 * (i % ds); dy = -R + 2 * (i / ds);
-            }, nullptr);
+            });
             const int ddx = key_dx(dense), ddy = key_dy(dense);
             const int xa2 = imax(-R, ddx - 2), xb2 = imin(R, ddx + 2), ya2 = imax(-R, ddy - 2), yb2 = imin(R, ddy + 2);
             const int w2 = xb2 - xa2 + 1, h2 = yb2 - ya2 + 1;
             const uint64_t dfull = int_batch(C, w2 * h2, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
                 dx = xa2 + i % w2; dy = ya2 + i / w2;
-            }, nullptr);
+            });
             if (dfull < full) full = dfull;
         }
     }
@@ -979,7 +1105,7 @@ __device__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) 
         set_err(C, 5);
         return {0, 0, kBadSad};
     }
-    uint64_t best = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; }, nullptr);
+    uint64_t best = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; });
     if (key_cost(best) == 0) return {0, 0, 0};
     int sx[5], sy[5], sn = 1;
     sx[0] = 0; sy[0] = 0;
@@ -998,7 +1124,7 @@ __device__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) 
             const int j = i + 1;
             dx = j == 1 ? sx[1] : j == 2 ? sx[2] : j == 3 ? sx[3] : sx[4];
             dy = j == 1 ? sy[1] : j == 2 ? sy[2] : j == 3 ? sy[3] : sy[4];
-        }, nullptr);
+        });
         if (s < best) best = s;
     }
     int prevx = key_dx(best), prevy = key_dy(best);
@@ -1019,7 +1145,7 @@ __device__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) 
             const uint64_t s = int_batch(C, n, 1, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
                 dx = i == 0 ? ox[0] : i == 1 ? ox[1] : i == 2 ? ox[2] : ox[3];
                 dy = i == 0 ? oy[0] : i == 1 ? oy[1] : i == 2 ? oy[2] : oy[3];
-            }, nullptr);
+            });
             if (s < win) win = s;
         }
         if (win == center) break;
@@ -1045,7 +1171,7 @@ __device__ uint64_t far_batch(Ctx& C, int n, int stride, int keymode, int seq0, 
         set_err(C, 6);
         return kNone;
     }
-    return int_batch(C, n, stride, nullptr, keymode, seq0, pt, nullptr);
+    return int_batch(C, n, stride, nullptr, keymode, seq0, pt);
 }
 
 // propagated_long_range_motion, src/encoder_motion.cpp:1658-1694
@@ -1205,48 +1331,73 @@ struct SubParams {
     bool more;
 };
 
-__device__ int find_sub(const Ctx& C, int n, int xq, int yq) {
-    for (int i = 0; i < n; ++i)
-        if (C.L.sub[i].xq == xq && C.L.sub[i].yq == yq) return i;
-    return -1;
-}
-
-// add(mv, known) of refine_motion_block_staged (src/encoder_motion.cpp:3330-3343),
-// batched: candidates not yet in the list are queued and scored together.
-__device__ int sub_add_batch(Ctx& C, const SubParams& sp, int nsub, const int* xs, const int* ys, int n) {
-    int nreq = 0;
-    for (int i = 0; i < n; ++i) {
-        if (find_sub(C, nsub, xs[i], ys[i]) >= 0) continue;
-        bool dup = false;
-        for (int j = 0; j < nreq; ++j)
-            if (C.L.req[j].x == xs[i] && C.L.req[j].y == ys[i]) dup = true;
-        if (dup) continue;
-        wsync();
-        if (C.lane == 0) C.L.req[nreq] = make_int2(xs[i], ys[i]);
-        wsync();
-        ++nreq;
-    }
-    if (!nreq) return nsub;
-    qpel_sad_batch(C, sp.bx0, sp.by0, sp.bw, sp.bh, nreq, sp.mode);
-    for (int j = 0; j < nreq; ++j) {
-        const int2 v = C.L.req[j];
-        const uint64_t cost = C.L.reqcost[j];
-        wsync();
-        if (C.lane == 0) {
-            SubCand& s = C.L.sub[nsub + j];
-            s.xq = v.x; s.yq = v.y; s.sad = cost; s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
-        }
-        wsync();
-    }
-    return nsub + nreq;
-}
-
-__device__ __forceinline__ bool sad_less(const SubCand& a, const SubCand& b) {
-    if (a.sad != b.sad) return a.sad < b.sad;
-    return tie_less(a.xq, a.yq, b.xq, b.yq);
-}
-
 __device__ int make_even(int v) { return (v & 1) ? (v > 0 ? v - 1 : v + 1) : v; }
+
+// (cost, |x|+|y|, y, x) key of the sub-pel stages (tie order of
+// src/encoder_motion.cpp:3320-3325)
+__device__ __forceinline__ uint64_t key_q(uint64_t cost, int xq, int yq) {
+    return (cost21(cost) << 37) | (static_cast<uint64_t>(iabs(xq) + iabs(yq)) << 24) |
+           (static_cast<uint64_t>(yq + 1024) << 13) | static_cast<uint64_t>(xq + 4096);
+}
+__device__ __forceinline__ int keyq_x(uint64_t k) { return static_cast<int>(k & 8191) - 4096; }
+__device__ __forceinline__ int keyq_y(uint64_t k) { return static_cast<int>((k >> 13) & 2047) - 1024; }
+
+// add(mv) of refine_motion_block_staged (src/encoder_motion.cpp:3330-3343) for
+// the m vectors in L.req: vectors not yet in the list are scored together and
+// appended.  L.slot[i] receives the list index of batch entry i.  Returns the
+// new list length.
+__device__ int sub_add(Ctx& C, const SubParams& sp, int n, int m) {
+    const int i = C.lane;
+    int found = -1;
+    int2 v = make_int2(0, 0);
+    bool isnew = false;
+    if (i < m) {
+        v = C.L.req[i];
+        for (int k = 0; k < n; ++k)
+            if (C.L.sub[k].xq == v.x && C.L.sub[k].yq == v.y) { found = k; break; }
+        if (found < 0) {
+            bool dup = false;
+            for (int j = 0; j < i; ++j) {
+                const int2 u = C.L.req[j];
+                if (u.x == v.x && u.y == v.y) { dup = true; break; }
+            }
+            isnew = !dup;
+        }
+    }
+    const uint64_t nm = __ballot(isnew);
+    const int cnt = __popcll(nm);
+    const int pos = lane_prefix(nm);
+    if (isnew) C.L.pend[pos] = v;
+    wsync();
+    qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, C.L.pend, cnt, sp.mode, 0, C.L.pendcost);
+    if (isnew) {
+        SubCand& s = C.L.sub[n + pos];
+        s.xq = v.x; s.yq = v.y; s.sad = C.L.pendcost[pos];
+        s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
+        found = n + pos;
+    }
+    wsync();
+    if (i < m && found < 0)
+        for (int k = 0; k < n + cnt; ++k)
+            if (C.L.sub[k].xq == v.x && C.L.sub[k].yq == v.y) { found = k; break; }
+    if (i < m) C.L.slot[i] = found;
+    wsync();
+    return n + cnt;
+}
+
+// minimum (SAD, tie order) key over the m entries of the last batch
+__device__ uint64_t batch_min_key(Ctx& C, int m) {
+    uint64_t k = kNone;
+    if (C.lane < m) {
+        const SubCand& s = C.L.sub[C.L.slot[C.lane]];
+        k = key_q(s.sad, s.xq, s.yq);
+    }
+    return wave_min64(k);
+}
+
+__device__ void set_req(Ctx& C, int i, int x, int y) {
+    if (C.lane == i) C.L.req[i] = make_int2(x, y);
+}
 
 // refine_motion_integer_only, src/encoder_motion.cpp:3247-3269
 __device__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int syq, uint64_t* best_sad, uint64_t known) {
@@ -1263,10 +1414,10 @@ __device__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int sy
     if (ix == sxq && iy == syq && known != UINT64_MAX) *best_sad = known;
     else {
         wsync();
-        if (C.lane == 0) C.L.req[0] = make_int2(ix, iy);
+        if (C.lane == 0) C.L.pend[0] = make_int2(ix, iy);
         wsync();
-        qpel_sad_batch(C, sp.bx0, sp.by0, sp.bw, sp.bh, 1, sp.mode);
-        *best_sad = C.L.reqcost[0];
+        qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, C.L.pend, 1, sp.mode, 0, C.L.pendcost);
+        *best_sad = C.L.pendcost[0];
     }
     return make_int2(ix, iy);
 }
@@ -1286,193 +1437,223 @@ __device__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, con
     sxq = clampi(sxq, xmin, xmax);
     syq = clampi(syq, ymin, ymax);
     if (hp) { sxq = make_even(sxq); syq = make_even(syq); }
-    int nsub = 0;
-    // seed with its known SAD
-    {
-        const bool hid = hidden_padding(C.g, sp.bx0, sp.by0, sp.bw, sp.bh, sxq, syq, sp.mode, chroma);
-        if (hid || known_seed != UINT64_MAX) {
-            wsync();
-            if (C.lane == 0) {
-                SubCand& s = C.L.sub[0];
-                s.xq = sxq; s.yq = syq; s.sad = hid ? kBadSad : known_seed; s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
-            }
-            wsync();
-            nsub = 1;
-        } else {
-            int xs[1] = {sxq}, ys[1] = {syq};
-            nsub = sub_add_batch(C, sp, 0, xs, ys, 1);
+    int n = 0;
+    const bool hid = hidden_padding(C.g, sp.bx0, sp.by0, sp.bw, sp.bh, sxq, syq, sp.mode, chroma);
+    wsync();
+    if (hid || known_seed != UINT64_MAX) {
+        if (C.lane == 0) {
+            SubCand& s = C.L.sub[0];
+            s.xq = sxq; s.yq = syq; s.sad = hid ? kBadSad : known_seed; s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
         }
-        if (hid) {
-            int xs[1] = {0}, ys[1] = {0};
-            nsub = sub_add_batch(C, sp, nsub, xs, ys, 1);
-        }
-    }
-    int hbx = sxq, hby = syq;
-    uint64_t hsad = C.L.sub[0].sad;
-    const bool fast = J->fast != 0;
-    int bxs[9], bys[9], nb = 0;
-    if (fast) {
-        bxs[nb] = sxq; bys[nb] = syq; ++nb;
-        const int o4[4][2] = {{-2, 0}, {2, 0}, {0, -2}, {0, 2}};
-        for (int k = 0; k < 4; ++k) {
-            int x = clampi(sxq + o4[k][0], xmin, xmax), y = clampi(syq + o4[k][1], ymin, ymax);
-            if (hp) { x = make_even(x); y = make_even(y); }
-            bxs[nb] = x; bys[nb] = y; ++nb;
-        }
+        wsync();
+        n = 1;
     } else {
-        for (int oy = -2; oy <= 2; oy += 2)
-            for (int ox = -2; ox <= 2; ox += 2) {
-                int x = clampi(sxq + ox, xmin, xmax), y = clampi(syq + oy, ymin, ymax);
+        set_req(C, 0, sxq, syq);
+        wsync();
+        n = sub_add(C, sp, 0, 1);
+    }
+    if (hid) {
+        set_req(C, 0, 0, 0);
+        wsync();
+        n = sub_add(C, sp, n, 1);
+    }
+    uint64_t half = key_q(C.L.sub[0].sad, sxq, syq);
+    const bool fast = J->fast != 0;
+    // half-pel stencil around the seed
+    {
+        int nb;
+        if (fast) {
+            nb = 5;
+            if (C.lane < 5) {
+                int x = sxq, y = syq;
+                if (C.lane == 1) x = sxq - 2; else if (C.lane == 2) x = sxq + 2;
+                else if (C.lane == 3) y = syq - 2; else if (C.lane == 4) y = syq + 2;
+                x = clampi(x, xmin, xmax); y = clampi(y, ymin, ymax);
                 if (hp) { x = make_even(x); y = make_even(y); }
-                bxs[nb] = x; bys[nb] = y; ++nb;
+                C.L.req[C.lane] = make_int2(x, y);
             }
-    }
-    nsub = sub_add_batch(C, sp, nsub, bxs, bys, nb);
-    for (int i = 0; i < nb; ++i) {
-        const int j = find_sub(C, nsub, bxs[i], bys[i]);
-        if (j >= 0) {
-            const SubCand& s = C.L.sub[j];
-            if (s.sad < hsad || (s.sad == hsad && tie_less(s.xq, s.yq, hbx, hby))) { hsad = s.sad; hbx = s.xq; hby = s.yq; }
+        } else {
+            nb = 9;
+            if (C.lane < 9) {
+                int x = clampi(sxq + 2 * (C.lane % 3) - 2, xmin, xmax), y = clampi(syq + 2 * (C.lane / 3) - 2, ymin, ymax);
+                if (hp) { x = make_even(x); y = make_even(y); }
+                C.L.req[C.lane] = make_int2(x, y);
+            }
         }
+        wsync();
+        n = sub_add(C, sp, n, nb);
+        const uint64_t m = batch_min_key(C, nb);
+        half = m < half ? m : half;
     }
+    int hbx = keyq_x(half), hby = keyq_y(half);
     if (fast && !(hbx == sxq && hby == syq)) {
-        int dxs[2], dys[2], nd = 0;
         const int dx = hbx - sxq, dy = hby - syq;
+        int nd = 0;
         if (dx && !dy) {
-            dxs[nd] = hbx; dys[nd] = clampi(syq - 2, ymin, ymax); ++nd;
-            dxs[nd] = hbx; dys[nd] = clampi(syq + 2, ymin, ymax); ++nd;
-        } else if (dy && !dx) {
-            dxs[nd] = clampi(sxq - 2, xmin, xmax); dys[nd] = hby; ++nd;
-            dxs[nd] = clampi(sxq + 2, xmin, xmax); dys[nd] = hby; ++nd;
-        }
-        if (hp)
-            for (int i = 0; i < nd; ++i) { dxs[i] = make_even(dxs[i]); dys[i] = make_even(dys[i]); }
-        nsub = sub_add_batch(C, sp, nsub, dxs, dys, nd);
-        for (int i = 0; i < nd; ++i) {
-            const int j = find_sub(C, nsub, dxs[i], dys[i]);
-            if (j >= 0) {
-                const SubCand& s = C.L.sub[j];
-                if (s.sad < hsad || (s.sad == hsad && tie_less(s.xq, s.yq, hbx, hby))) { hsad = s.sad; hbx = s.xq; hby = s.yq; }
+            nd = 2;
+            if (C.lane < 2) {
+                int x = hbx, y = clampi(syq + (C.lane ? 2 : -2), ymin, ymax);
+                if (hp) { x = make_even(x); y = make_even(y); }
+                C.L.req[C.lane] = make_int2(x, y);
             }
+        } else if (dy && !dx) {
+            nd = 2;
+            if (C.lane < 2) {
+                int x = clampi(sxq + (C.lane ? 2 : -2), xmin, xmax), y = hby;
+                if (hp) { x = make_even(x); y = make_even(y); }
+                C.L.req[C.lane] = make_int2(x, y);
+            }
+        }
+        if (nd) {
+            wsync();
+            n = sub_add(C, sp, n, nd);
+            const uint64_t m = batch_min_key(C, nd);
+            half = m < half ? m : half;
+            hbx = keyq_x(half); hby = keyq_y(half);
         }
     }
     if (!hp) {
         if (fast) {
             const int qcx = hbx, qcy = hby;
-            int qx[5], qy[5], nq = 0;
-            qx[nq] = qcx; qy[nq] = qcy; ++nq;
-            const int o4[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
-            for (int k = 0; k < 4; ++k) {
-                qx[nq] = clampi(qcx + o4[k][0], xmin, xmax); qy[nq] = clampi(qcy + o4[k][1], ymin, ymax); ++nq;
+            if (C.lane < 5) {
+                int x = qcx, y = qcy;
+                if (C.lane == 1) x = qcx - 1; else if (C.lane == 2) x = qcx + 1;
+                else if (C.lane == 3) y = qcy - 1; else if (C.lane == 4) y = qcy + 1;
+                C.L.req[C.lane] = make_int2(clampi(x, xmin, xmax), clampi(y, ymin, ymax));
             }
-            nsub = sub_add_batch(C, sp, nsub, qx, qy, nq);
-            int qbx = qcx, qby = qcy;
-            uint64_t qcost = C.L.sub[find_sub(C, nsub, qcx, qcy)].sad;
-            for (int i = 0; i < nq; ++i) {
-                const int j = find_sub(C, nsub, qx[i], qy[i]);
-                if (j >= 0) {
-                    const SubCand& s = C.L.sub[j];
-                    if (s.sad < qcost || (s.sad == qcost && tie_less(s.xq, s.yq, qbx, qby))) { qbx = s.xq; qby = s.yq; qcost = s.sad; }
-                }
-            }
-            int q2x[2], q2y[2], n2 = 0;
+            wsync();
+            n = sub_add(C, sp, n, 5);
+            // the quarter centre's cost, then the best of the five
+            uint64_t qk = kNone;
+            for (int k = 0; k < n; ++k)
+                if (C.L.sub[k].xq == qcx && C.L.sub[k].yq == qcy) qk = key_q(C.L.sub[k].sad, qcx, qcy);
+            const uint64_t m = batch_min_key(C, 5);
+            qk = m < qk ? m : qk;
+            const int qbx = keyq_x(qk), qby = keyq_y(qk);
             const int dx = qbx - qcx, dy = qby - qcy;
+            int n2 = 0;
             if (dx && !dy) {
-                q2x[n2] = qbx; q2y[n2] = clampi(qcy - 1, ymin, ymax); ++n2;
-                q2x[n2] = qbx; q2y[n2] = clampi(qcy + 1, ymin, ymax); ++n2;
+                n2 = 2;
+                if (C.lane < 2) C.L.req[C.lane] = make_int2(qbx, clampi(qcy + (C.lane ? 1 : -1), ymin, ymax));
             } else if (dy && !dx) {
-                q2x[n2] = clampi(qcx - 1, xmin, xmax); q2y[n2] = qby; ++n2;
-                q2x[n2] = clampi(qcx + 1, xmin, xmax); q2y[n2] = qby; ++n2;
+                n2 = 2;
+                if (C.lane < 2) C.L.req[C.lane] = make_int2(clampi(qcx + (C.lane ? 1 : -1), xmin, xmax), qby);
             }
-            nsub = sub_add_batch(C, sp, nsub, q2x, q2y, n2);
+            if (n2) {
+                wsync();
+                n = sub_add(C, sp, n, n2);
+            }
         } else {
-            int qx[9], qy[9], nq = 0;
-            for (int oy = -1; oy <= 1; ++oy)
-                for (int ox = -1; ox <= 1; ++ox) {
-                    qx[nq] = clampi(hbx + ox, xmin, xmax); qy[nq] = clampi(hby + oy, ymin, ymax); ++nq;
-                }
-            nsub = sub_add_batch(C, sp, nsub, qx, qy, nq);
+            if (C.lane < 9)
+                C.L.req[C.lane] = make_int2(clampi(hbx + (C.lane % 3) - 1, xmin, xmax),
+                                            clampi(hby + (C.lane / 3) - 1, ymin, ymax));
+            wsync();
+            n = sub_add(C, sp, n, 9);
         }
     }
     // sort by (SAD, tie order) and keep the shortlist
-    wsync();
-    if (C.lane == 0) {
-        for (int i = 1; i < nsub; ++i) {
-            const SubCand x = C.L.sub[i];
-            int j = i - 1;
-            while (j >= 0 && sad_less(x, C.L.sub[j])) { C.L.sub[j + 1] = C.L.sub[j]; --j; }
-            C.L.sub[j + 1] = x;
-        }
-    }
-    wsync();
-    const uint64_t screen = C.L.sub[0].sad;
     const int keep = fast ? 4 : 8;
-    if (nsub > keep) nsub = keep;
-    // rates and x87 scores
+    {
+        SubCand mine;
+        uint64_t myk = kNone;
+        if (C.lane < n) {
+            mine = C.L.sub[C.lane];
+            myk = key_q(mine.sad, mine.xq, mine.yq);
+        }
+        int rank = 0;
+        for (int k = 0; k < n; ++k) {
+            const SubCand& o = C.L.sub[k];
+            rank += key_q(o.sad, o.xq, o.yq) < myk ? 1 : 0;
+        }
+        if (C.lane < n && rank < keep) C.L.sub2[rank] = mine;
+        wsync();
+    }
+    const uint64_t screen = C.L.sub2[0].sad;
+    if (n > keep) n = keep;
+    // rates and x87 scores, one candidate per lane
     const uint64_t lm = J->lam_m;
     const int lk = J->lam_k;
-    wsync();
-    if (C.lane == 0) {
-        for (int i = 0; i < nsub; ++i) {
-            SubCand& s = C.L.sub[i];
-            int prx = pi.px, pry = pi.py;
-            uint32_t rate = 0;
-            if (pi.hybrid) {
-                const bool use_a = modular_mv_distance(J->mvrange, s.xq, s.yq, pi.ax, pi.ay) <=
-                                   modular_mv_distance(J->mvrange, s.xq, s.yq, pi.cx, pi.cy);
-                prx = use_a ? pi.ax : pi.cx;
-                pry = use_a ? pi.ay : pi.cy;
-                ++rate;
-            }
-            rate += mv_rate_bits(J->mvrange, s.xq, s.yq, prx, pry, sp.mode, sp.more);
-            s.rate = rate;
-            s.score = x87_score(s.sad, rate, lm, lk);
+    if (C.lane < n) {
+        SubCand& s = C.L.sub2[C.lane];
+        int prx = pi.px, pry = pi.py;
+        uint32_t rate = 0;
+        if (pi.hybrid) {
+            const bool use_a = modular_mv_distance(J->mvrange, s.xq, s.yq, pi.ax, pi.ay) <=
+                               modular_mv_distance(J->mvrange, s.xq, s.yq, pi.cx, pi.cy);
+            prx = use_a ? pi.ax : pi.cx;
+            pry = use_a ? pi.ay : pi.cy;
+            ++rate;
         }
+        rate += mv_rate_bits(J->mvrange, s.xq, s.yq, prx, pry, sp.mode, sp.more);
+        s.rate = rate;
+        s.score = x87_score(s.sad, rate, lm, lk);
     }
     wsync();
-    int rw = 0;
-    for (int i = 1; i < nsub; ++i) {
-        const SubCand& a = C.L.sub[i];
-        const SubCand& b = C.L.sub[rw];
-        bool less;
-        if (a.score != b.score) less = a.score < b.score;
-        else if (a.sad != b.sad) less = a.sad < b.sad;
-        else less = tie_less(a.xq, a.yq, b.xq, b.yq);
-        if (less) rw = i;
+    // the SAD+lambda*R winner: the candidate no other one precedes
+    int rw;
+    {
+        bool first = false;
+        if (C.lane < n) {
+            const SubCand& a = C.L.sub2[C.lane];
+            int before = 0;
+            for (int k = 0; k < n; ++k) {
+                const SubCand& b = C.L.sub2[k];
+                bool less;
+                if (b.score != a.score) less = b.score < a.score;
+                else if (b.sad != a.sad) less = b.sad < a.sad;
+                else less = tie_less(b.xq, b.yq, a.xq, a.yq);
+                before += less ? 1 : 0;
+            }
+            first = before == 0;
+        }
+        rw = __ffsll(__ballot(first)) - 1;
     }
-    int2 result = make_int2(C.L.sub[rw].xq, C.L.sub[rw].yq);
+    int2 result = make_int2(C.L.sub2[rw].xq, C.L.sub2[rw].yq);
     if (J->quality == 2) {
-        const uint64_t rws = C.L.sub[rw].sad;
+        const uint64_t rws = C.L.sub2[rw].sad;
         const uint64_t pct3 = (rws / 100) * 3 + (((rws % 100) * 3 + 99) / 100);
         const uint64_t slack = pct3 > 4 ? pct3 : 4;
         const uint64_t limit = rws > UINT64_MAX - slack ? UINT64_MAX : rws + slack;
-        for (int i = 0; i < nsub; ++i) {
-            const SubCand s = C.L.sub[i];
-            const bool el = s.sad < UINT64_MAX / 8 && s.sad <= limit;
-            uint64_t satd = kBadSad;
-            if (el) satd = satd_one(C, sp.bx0, sp.by0, sp.bw, sp.bh, s.xq, s.yq, sp.mode);
-            wsync();
-            if (C.lane == 0) {
-                C.L.sub[i].eligible = el ? 1u : 0u;
-                C.L.sub[i].satd = satd;
-                C.L.sub[i].score = x87_score(satd, s.rate, lm, lk);
+        bool el = false;
+        int2 mv = make_int2(0, 0);
+        if (C.lane < n) {
+            const SubCand& s = C.L.sub2[C.lane];
+            el = s.sad < UINT64_MAX / 8 && s.sad <= limit;
+            mv = make_int2(s.xq, s.yq);
+        }
+        const uint64_t em = __ballot(el);
+        const int cnt = __popcll(em);
+        const int pos = lane_prefix(em);
+        if (el) C.L.pend[pos] = mv;
+        wsync();
+        qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, C.L.pend, cnt, sp.mode, 1, C.L.pendcost);
+        if (C.lane < n) {
+            SubCand& s = C.L.sub2[C.lane];
+            s.eligible = el ? 1u : 0u;
+            s.satd = el ? C.L.pendcost[pos] : kBadSad;
+            s.score = x87_score(s.satd, s.rate, lm, lk);
+        }
+        wsync();
+        bool first = false;
+        if (C.lane < n) {
+            const SubCand& a = C.L.sub2[C.lane];
+            int before = 0;
+            for (int k = 0; k < n; ++k) {
+                const SubCand& b = C.L.sub2[k];
+                bool less;
+                if (b.eligible != a.eligible) less = b.eligible > a.eligible;
+                else if (b.score != a.score) less = b.score < a.score;
+                else if (b.satd != a.satd) less = b.satd < a.satd;
+                else if (b.sad != a.sad) less = b.sad < a.sad;
+                else less = tie_less(b.xq, b.yq, a.xq, a.yq);
+                before += less ? 1 : 0;
             }
-            wsync();
+            first = before == 0;
         }
-        int w = 0;
-        for (int i = 1; i < nsub; ++i) {
-            const SubCand& a = C.L.sub[i];
-            const SubCand& b = C.L.sub[w];
-            bool less;
-            if (a.eligible != b.eligible) less = a.eligible > b.eligible;
-            else if (a.score != b.score) less = a.score < b.score;
-            else if (a.satd != b.satd) less = a.satd < b.satd;
-            else if (a.sad != b.sad) less = a.sad < b.sad;
-            else less = tie_less(a.xq, a.yq, b.xq, b.yq);
-            if (less) w = i;
-        }
-        result = make_int2(C.L.sub[w].xq, C.L.sub[w].yq);
+        const int w = __ffsll(__ballot(first)) - 1;
+        result = make_int2(C.L.sub2[w].xq, C.L.sub2[w].yq);
     }
+    wsync();
     *best_sad = screen;
     return result;
 }
@@ -1512,7 +1693,7 @@ __device__ void analyze_mb(Ctx& C) {
     if (range == 0) {
         // zero-range mode: zero vectors with their SAD (src/encoder_motion.cpp:73-86)
         win_load(C, C.x0, C.y0, 16, 16);
-        const uint64_t z = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; }, nullptr);
+        const uint64_t z = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; });
         if (C.lane == 0) {
             J->screen[pos] = key_cost(z);
             J->chosen[pos] = key_cost(z);
@@ -1581,10 +1762,10 @@ __device__ void analyze_mb(Ctx& C) {
     if (hidden_padding(C.g, C.x0, C.y0, 16, 16, one.x, one.y, one_mode, true)) {
         one = make_int2(0, 0);
         wsync();
-        if (C.lane == 0) C.L.req[0] = one;
+        if (C.lane == 0) C.L.pend[0] = one;
         wsync();
-        qpel_sad_batch(C, C.x0, C.y0, 16, 16, 1, one_mode);
-        one_sad = C.L.reqcost[0];
+        qpel_eval(C, C.x0, C.y0, 16, 16, C.L.pend, 1, one_mode, 0, C.L.pendcost);
+        one_sad = C.L.pendcost[0];
     }
     const uint64_t one_screen = one_sad;
     int2 outmv = one;
@@ -1608,45 +1789,67 @@ __device__ void analyze_mb(Ctx& C) {
                 if (imax(iabs(m.x), iabs(m.y)) > limit) { m.x = (m.x / 4) * 4; m.y = (m.y / 4) * 4; }
                 return m;
             };
-            // the eleven seed SADs, scored together, then replayed in order
-            int2 cand[11];
-            cand[0] = within(one);
-            const int2 raw[10] = {one, make_int2(pi4.px, pi4.py),
-                                  make_int2(one.x - 8, one.y), make_int2(one.x + 8, one.y),
-                                  make_int2(one.x, one.y - 8), make_int2(one.x, one.y + 8),
-                                  make_int2(one.x - 4, one.y), make_int2(one.x + 4, one.y),
-                                  make_int2(one.x, one.y - 4), make_int2(one.x, one.y + 4)};
-            for (int s = 0; s < 10; ++s) {
-                int2 sd = make_int2(clampi(raw[s].x, qb.xmin, qb.xmax), clampi(raw[s].y, qb.ymin, qb.ymax));
-                cand[s + 1] = within(sd);
+            // the eleven seed SADs, scored together (unique vectors only), then
+            // replayed in the reference's order
+            if (C.lane < 11) {
+                int2 cv = one;
+                if (C.lane > 0) {
+                    const int s = C.lane - 1;
+                    int x = one.x, y = one.y;
+                    switch (s) {
+                        case 1: x = pi4.px; y = pi4.py; break;
+                        case 2: x -= 8; break;
+                        case 3: x += 8; break;
+                        case 4: y -= 8; break;
+                        case 5: y += 8; break;
+                        case 6: x -= 4; break;
+                        case 7: x += 4; break;
+                        case 8: y -= 4; break;
+                        case 9: y += 4; break;
+                        default: break;
+                    }
+                    cv = make_int2(clampi(x, qb.xmin, qb.xmax), clampi(y, qb.ymin, qb.ymax));
+                }
+                C.L.req[C.lane] = within(cv);
             }
-            // unique vectors -> req
-            int nreq = 0;
-            for (int s = 0; s < 11; ++s) {
-                bool dup = false;
-                for (int j = 0; j < nreq; ++j)
-                    if (C.L.req[j].x == cand[s].x && C.L.req[j].y == cand[s].y) dup = true;
-                if (dup) continue;
+            wsync();
+            {
+                int2 v = make_int2(0, 0);
+                bool isnew = false;
+                if (C.lane < 11) {
+                    v = C.L.req[C.lane];
+                    bool dup = false;
+                    for (int j = 0; j < C.lane; ++j) {
+                        const int2 u = C.L.req[j];
+                        if (u.x == v.x && u.y == v.y) { dup = true; break; }
+                    }
+                    isnew = !dup;
+                }
+                const uint64_t nm = __ballot(isnew);
+                const int cnt = __popcll(nm);
+                if (isnew) C.L.pend[lane_prefix(nm)] = v;
                 wsync();
-                if (C.lane == 0) C.L.req[nreq] = cand[s];
+                qpel_eval(C, bx, by, 8, 8, C.L.pend, cnt, 0, 0, C.L.pendcost);
+                if (C.lane < 11) {
+                    uint64_t cost = kBadSad;
+                    for (int j = 0; j < cnt; ++j) {
+                        const int2 u = C.L.pend[j];
+                        if (u.x == v.x && u.y == v.y) { cost = C.L.pendcost[j]; break; }
+                    }
+                    C.L.top16[C.lane] = cost;
+                }
                 wsync();
-                ++nreq;
             }
-            qpel_sad_batch(C, bx, by, 8, 8, nreq, 0);
-            auto cost_of = [&](int2 v) -> uint64_t {
-                for (int j = 0; j < nreq; ++j)
-                    if (C.L.req[j].x == v.x && C.L.req[j].y == v.y) return C.L.reqcost[j];
-                return kBadSad;
-            };
-            int2 best_seed = cand[0];
-            uint64_t seed_cost = cost_of(best_seed);
+            int2 best_seed = C.L.req[0];
+            uint64_t seed_cost = C.L.top16[0];
             for (int s = 1; s < 11; ++s) {
-                const int2 sd = cand[s];
+                const int2 sd = C.L.req[s];
                 if (sd.x == best_seed.x && sd.y == best_seed.y) continue;
-                const uint64_t cc = cost_of(sd);
+                const uint64_t cc = C.L.top16[s];
                 const int cm = iabs(sd.x) + iabs(sd.y), bm = iabs(best_seed.x) + iabs(best_seed.y);
                 if (cc < seed_cost || (cc == seed_cost && cm < bm)) { seed_cost = cc; best_seed = sd; }
             }
+            wsync();
             uint64_t bsad = 0;
             SubParams sp{bx, by, 8, 8, 3, range, true};
             const int2 mv = refine_staged(C, sp, best_seed.x, best_seed.y, pi4, &bsad, seed_cost);
@@ -1731,6 +1934,75 @@ __device__ void analyze_mb(Ctx& C) {
     }
 }
 
+__device__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
+    Ctx C;
+    C.J = J;
+    C.g = {J->W, J->H, J->vw, J->vh, J->mbw, J->mbh};
+    C.ry = J->ry;
+    C.mx = mx; C.my = my; C.x0 = mx * 16; C.y0 = my * 16;
+    C.aw = imax(0, imin(16, J->W - C.x0));
+    C.ah = imax(0, imin(16, J->H - C.y0));
+    C.pos = my * J->mbw + mx;
+    C.lane = threadIdx.x;
+    C.L = carve(smem, J->local_range);
+    C.winCap = win_capacity(J->local_range);
+    C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
+    C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
+    analyze_mb(C);
+}
+
+// 4 s of the 100 MHz real-time counter: far beyond any legitimate wait
+constexpr uint64_t kWaitTicks = 400000000ull;
+
+// Persistent form: each 64-lane workgroup takes tickets until the queue is
+// empty; before a macroblock starts it waits (bounded) until its left,
+// top-left, top and top-right neighbours are final.  A ticket's dependencies
+// always hold smaller tickets, which are held by running workgroups, so the
+// queue drains whatever the residency.
+__global__ void __launch_bounds__(64) k_mb_persistent(const JobDesc* __restrict__ jobs, SchedDesc S) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    for (;;) {
+        int t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(S.next, 1);
+        t = __shfl(t, 0);
+        if (t >= S.total) break;
+        const int code = S.order[t];
+        const JobDesc* J = &jobs[code >> 24];
+        const int pos = code & 0xffffff;
+        const int mbw = J->mbw;
+        const int mx = pos % mbw, my = pos / mbw;
+        bool ok = true;
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        auto wait = [&](int dpos) {
+            for (;;) {
+                const uint32_t v = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(&J->done[dpos], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                if (v == S.epoch) break;
+                if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(S.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                    ok = false;
+                    break;
+                }
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
+                    if (threadIdx.x == 0) atomicExch(S.abort, 1);
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        };
+        if (mx > 0) wait(pos - 1);
+        if (my > 0) {
+            if (mx > 0) wait(pos - mbw - 1);
+            wait(pos - mbw);
+            if (mx + 1 < mbw) wait(pos - mbw + 1);
+        }
+        if (!ok && threadIdx.x == 0) atomicMax(J->err, 9);
+        run_mb(J, mx, my, smem);
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(&J->done[pos], S.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 __global__ void __launch_bounds__(64) k_mb_diagonal(const JobDesc* __restrict__ jobs, int diag) {
     extern __shared__ __align__(16) uint8_t smem[];
     const JobDesc* J = &jobs[blockIdx.y];
@@ -1742,20 +2014,7 @@ __global__ void __launch_bounds__(64) k_mb_diagonal(const JobDesc* __restrict__ 
     if (my > imin(mbh - 1, diag / 2)) return;
     const int mx = diag - 2 * my;
     if (mx < 0 || mx >= mbw) return;
-    Ctx C;
-    C.J = J;
-    C.g = {J->W, J->H, J->vw, J->vh, mbw, mbh};
-    C.ry = J->ry;
-    C.mx = mx; C.my = my; C.x0 = mx * 16; C.y0 = my * 16;
-    C.aw = imax(0, imin(16, J->W - C.x0));
-    C.ah = imax(0, imin(16, J->H - C.y0));
-    C.pos = my * mbw + mx;
-    C.lane = threadIdx.x;
-    C.L = carve(smem, J->local_range);
-    C.winCap = win_capacity(J->local_range);
-    C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
-    C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
-    analyze_mb(C);
+    run_mb(J, mx, my, smem);
 }
 
 // ------------------------------------------------------------------ global search
@@ -2176,6 +2435,12 @@ hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hi
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s) {
     hipLaunchKernelGGL(k_mb_diagonal, dim3(max_per_diag, njobs), dim3(64), lds_bytes, s, d_jobs, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_mb_persistent(const JobDesc* d_jobs, const SchedDesc& sched, int workgroups, int lds_bytes,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_mb_persistent, dim3(workgroups), dim3(64), lds_bytes, s, d_jobs, sched);
     return hipGetLastError();
 }
 

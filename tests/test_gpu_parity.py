@@ -177,3 +177,28 @@ def test_work_counters_match_oracle_op_counts(backend):
         fr.release()
     assert (w.sad_px, w.interp_taps, w.satd_ops, w.sig_ops, w.frame_px) == \
         (ops.sad_px, ops.interp_taps, ops.satd_ops, ops.sig_ops, ops.frame_px)
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["persistent", "diagonal"])
+def test_schedules_give_identical_results(backend, mode):
+    """The persistent work-queue launch and the per-diagonal launches agree
+    with the oracle on a multi-stream step."""
+    jobs = []
+    for i, (w, h) in enumerate(((640, 360), (352, 288))):
+        d = synth.pan_rectangles(40 + i, 4, width=w, height=h, frame_indices=[0, 3])
+        p = abi.default_params(w, h, pqindex=8, motion_search_range=512, motion_local_search_range=32,
+                               mode=abi.MODE_MIXED_MV)
+        jobs.append((d[3], d[0], p))
+    frames = [(backend.upload(c), backend.upload(r)) for c, r, _ in jobs]
+    _say(f"[case] schedule mode {mode}")
+    try:
+        backend.set_schedule(mode, 0)
+        got = backend.analyze_step(frames, [p for _, _, p in jobs])
+    finally:
+        backend.set_schedule(0, 0)
+        for a, b in frames:
+            a.release()
+            b.release()
+    for (c, r, p), g in zip(jobs, got):
+        want = oracle_analyze(p, c, r)
+        assert not g.diff(want), describe_diff(g, want, p.width)
