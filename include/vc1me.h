@@ -183,6 +183,16 @@ int vc1me_score_batch_frames(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
 int vc1me_analyze_p_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
                           const vc1me_params* params, vc1me_p_result* out);
 
+/* Costs of n candidates for one 16x16 or 8x8 block of resident frames under
+ * the search's own laws: metric 0 = block_sad_motion (hidden-padding veto
+ * included, src/encoder_motion.cpp:2670-2933), metric 1 = block_satd_motion
+ * (src/encoder_motion.cpp:2962-3042), for progressive mode `mode`.  Runs the
+ * same device code as the sub-pel stages of vc1me_analyze_p_frame.  No
+ * reference counterpart (a finer-grained form of vc1me_score_batch). */
+int vc1me_block_costs(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const vc1me_params* params,
+                      int x0, int y0, int bw, int bh, int mode, int metric, const vc1me_candidate* candidates,
+                      size_t n, uint64_t* costs_out);
+
 /* ---- (iii) batched multi-stream step: n independent (cur, ref) pairs of
  * possibly different rasters analysed in one launch sequence on the
  * context's stream.  Equivalent to n calls of vc1me_analyze_p_frame. */
@@ -212,12 +222,12 @@ int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
  * like the reference's SpeedProfiler, src/encoder_internal.h:182-221). */
 int vc1me_set_profiling(vc1me_ctx* ctx, int on);
 
-/* Macroblock scheduling of the analysis kernels: mode 0 (default) runs one
- * persistent launch whose 64-lane workgroups take macroblocks from a work
- * queue in anti-diagonal order and wait only for their own neighbours;
- * mode 1 launches one kernel per anti-diagonal.  workgroups = 0 picks 8 per
- * compute unit.  Results are identical in both modes.  No reference
- * counterpart (tuning knob). */
+/* Macroblock scheduling of the analysis kernels: mode 1 (default) launches
+ * one kernel per anti-diagonal; mode 0 runs one persistent launch whose
+ * 64-lane workgroups take macroblocks from a work queue in anti-diagonal
+ * order and wait only for their own neighbours (experimental: it is not
+ * enabled by default).  workgroups = 0 picks 8 per compute unit.  No
+ * reference counterpart (tuning knob). */
 int vc1me_set_schedule(vc1me_ctx* ctx, int mode, int workgroups);
 
 /* Algorithmic work of the last step, summed over its jobs, in the units of

@@ -22,6 +22,7 @@ EXPORTS = (
     "vc1me_device_name", "vc1me_score_batch", "vc1me_frame_upload", "vc1me_frame_release",
     "vc1me_score_batch_frames", "vc1me_analyze_p_frame", "vc1me_analyze_step", "vc1me_last_timing",
     "vc1me_set_profiling", "vc1me_last_work", "vc1me_set_schedule", "vc1me_counters_get", "vc1me_counters_reset",
+    "vc1me_block_costs",
 )
 
 
@@ -66,6 +67,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_last_work.argtypes = [C.c_void_p, P(abi.Work)]
     lib.vc1me_set_schedule.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.vc1me_block_costs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_int, C.c_int, P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
     lib.vc1me_counters_get.argtypes = [C.c_void_p, P(abi.Counters)]
     lib.vc1me_counters_reset.argtypes = [C.c_void_p]
     lib.vc1me_counters_reset.restype = None
@@ -155,6 +158,15 @@ class Backend:
             self.ctx, cur.ctypes.data, ref.ctypes.data, w, h, x0, y0, bw, bh, sample_stride, bilinear, rnd,
             metric, cands.ctypes.data_as(C.POINTER(abi.Candidate)), len(cands),
             out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
+    def block_costs(self, cur: Frame, ref: Frame, params: abi.Params, x0: int, y0: int, bw: int, bh: int,
+                    mode: int, metric: int, cands: np.ndarray) -> np.ndarray:
+        cands = np.ascontiguousarray(cands, np.int32).reshape(-1, 2)
+        out = np.zeros(len(cands), np.uint64)
+        self._check(self.lib.vc1me_block_costs(
+            self.ctx, cur.handle, ref.handle, C.byref(params), x0, y0, bw, bh, mode, metric,
+            cands.ctypes.data_as(C.POINTER(abi.Candidate)), len(cands), out.ctypes.data_as(C.POINTER(C.c_uint64))))
         return out
 
     def set_profiling(self, on: bool) -> None:

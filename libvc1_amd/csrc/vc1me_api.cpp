@@ -173,7 +173,7 @@ struct vc1me_ctx {
     size_t cand_cap = 0;
     std::vector<uint32_t> h_costs;
     // persistent scheduler
-    int sched_mode = 0;        // 0 persistent work queue, 1 one launch per anti-diagonal
+    int sched_mode = 1;        // 0 persistent work queue, 1 one launch per anti-diagonal
     int persist_wgs = 0;       // 0: 8 per compute unit
     int cu_count = 256;
     uint32_t epoch = 0;
@@ -264,6 +264,10 @@ int validate_score_args(int width, int height, size_t n) {
         return fail(VC1ME_ELIMIT, "luma plane exceeds 32-bit addressing");
     return VC1ME_OK;
 }
+
+struct Bounds4 { int xmin, xmax, ymin, ymax; };
+// the widest legal quarter-pel box (motion_search_bounds_qpel at range 1024)
+Bounds4 qpel_limits() { return {-4096, 4095, -1024, 1023}; }
 
 int mvrange_of(int r) { return r <= 31 ? 0 : r <= 63 ? 1 : r <= 127 ? 2 : 3; }
 
@@ -615,7 +619,7 @@ int vc1me_create(int device, vc1me_ctx** out) {
         ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
         if (prop.multiProcessorCount > 0) ctx->cu_count = prop.multiProcessorCount;
     }
-    if (const char* e = std::getenv("VC1ME_SCHED")) ctx->sched_mode = std::string(e) == "diag" ? 1 : 0;
+    if (const char* e = std::getenv("VC1ME_SCHED")) ctx->sched_mode = std::string(e) == "persistent" ? 0 : 1;
     if (const char* e = std::getenv("VC1ME_PERSISTENT_WGS")) ctx->persist_wgs = std::max(0, std::atoi(e));
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete ctx; return fail(VC1ME_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
@@ -715,6 +719,57 @@ int vc1me_score_batch_frames(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
     if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
     return score_on_device(ctx, cur->y, ref->y, cur->w, cur->h, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
                            candidates, n, costs_out);
+}
+
+int vc1me_block_costs(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const vc1me_params* params,
+                      int x0, int y0, int bw, int bh, int mode, int metric, const vc1me_candidate* candidates,
+                      size_t n, uint64_t* costs_out) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (int rc = validate_params(params, cur, ref)) return rc;
+    if (!((bw == 16 && bh == 16) || (bw == 8 && bh == 8))) return fail(VC1ME_EINVAL, "block must be 16x16 or 8x8");
+    if (x0 < 0 || y0 < 0 || x0 >= params->width || y0 >= params->height) return fail(VC1ME_EINVAL, "block outside the raster");
+    if (mode < 0 || mode > 3 || (metric != 0 && metric != 1)) return fail(VC1ME_EINVAL, "bad mode or metric");
+    if (!n) return VC1ME_OK;
+    if (!candidates || !costs_out) return fail(VC1ME_EINVAL, "null candidates or costs");
+    const Bounds4 qb = qpel_limits();
+    for (size_t i = 0; i < n; ++i)
+        if (candidates[i].xq < qb.xmin || candidates[i].xq > qb.xmax || candidates[i].yq < qb.ymin || candidates[i].yq > qb.ymax)
+            return fail(VC1ME_ELIMIT, "candidate outside the legal quarter-pel range");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    if (ctx->slots.empty()) ctx->slots.resize(1);
+    const vc1me_params* p = params;
+    const size_t mbs = static_cast<size_t>((p->width + 15) / 16) * ((p->height + 15) / 16);
+    const size_t cells = static_cast<size_t>((p->width + 7) / 8) * ((p->height + 7) / 8);
+    hipError_t e = ensure_slot(ctx->slots[0], mbs, cells);
+    if (e != hipSuccess) return fail(VC1ME_ENOMEM, "device state allocation");
+    JobDesc d;
+    fill_job(d, p, cur, ref, ctx->slots[0]);
+    if (!ctx->d_jobs) {
+        HIP_TRY(hipMalloc(&ctx->d_jobs, sizeof(JobDesc)));
+        ctx->d_jobs_cap = 1;
+    }
+    int2* dc = nullptr;
+    uint64_t* dout = nullptr;
+    HIP_TRY(hipMalloc(&dc, n * sizeof(int2)));
+    e = hipMalloc(&dout, n * sizeof(uint64_t));
+    if (e != hipSuccess) { (void)hipFree(dc); return fail(VC1ME_ENOMEM, "cost buffer"); }
+    hipStream_t st = ctx->stream;
+    e = hipMemcpyAsync(ctx->d_jobs, &d, sizeof d, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dc, candidates, n * sizeof(int2), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(ctx->slots[0].err, 0, sizeof(int32_t), st);
+    if (e == hipSuccess)
+        e = vc1me::launch_block_costs(ctx->d_jobs, x0, y0, bw, bh, mode, metric, dc, static_cast<int>(n), dout,
+                                      vc1me::mb_kernel_lds_bytes(d.local_range), st);
+    if (e == hipSuccess) e = hipMemcpyAsync(costs_out, dout, n * sizeof(uint64_t), hipMemcpyDeviceToHost, st);
+    int32_t err = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&err, ctx->slots[0].err, sizeof err, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dc);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return fail(VC1ME_EDEVICE, std::string("block costs: ") + hipGetErrorString(e));
+    if (err) return fail(VC1ME_EKERNEL, "kernel reported internal error code " + std::to_string(err));
+    return VC1ME_OK;
 }
 
 int vc1me_analyze_p_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const vc1me_params* params,

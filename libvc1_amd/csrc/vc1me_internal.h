@@ -82,6 +82,8 @@ hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int ma
 hipError_t launch_mb_persistent(const JobDesc* d_jobs, const SchedDesc& sched, int workgroups, int lds_bytes,
                                 hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);
+hipError_t launch_block_costs(const JobDesc* d_job, int x0, int y0, int bw, int bh, int mode, int metric,
+                              const int2* cands, int n, uint64_t* out, int lds_bytes, hipStream_t s);
 hipError_t launch_score_batch(const uint8_t* cur, const uint8_t* ref, int width, int height, int x0, int y0,
                               int bw, int bh, int sample_stride, int bilinear, int rnd, int metric,
                               const int2* cands, int n, uint32_t* costs, hipStream_t s);

@@ -2003,6 +2003,41 @@ __global__ void __launch_bounds__(64) k_mb_persistent(const JobDesc* __restrict_
     }
 }
 
+// Block costs under the search's own laws (block_sad_motion / block_satd_motion,
+// hidden-padding veto included) for an arbitrary candidate list: the device
+// primitive of the sub-pel stages, exposed for direct checking.
+__global__ void __launch_bounds__(64) k_block_costs(const JobDesc* __restrict__ J, int x0, int y0, int bw, int bh,
+                                                    int mode, int metric, const int2* __restrict__ cands, int n,
+                                                    uint64_t* __restrict__ out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    Ctx C;
+    C.J = J;
+    C.g = {J->W, J->H, J->vw, J->vh, J->mbw, J->mbh};
+    C.ry = J->ry;
+    C.mx = x0 >> 4; C.my = y0 >> 4; C.x0 = x0; C.y0 = y0;
+    C.aw = imax(0, imin(16, J->W - x0));
+    C.ah = imax(0, imin(16, J->H - y0));
+    C.pos = 0;
+    C.lane = threadIdx.x;
+    C.L = carve(smem, J->local_range);
+    C.winCap = win_capacity(J->local_range);
+    C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
+    C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
+    for (int i = C.lane; i < 256; i += kWave) {
+        const int x = i & 15, y = i >> 4;
+        C.L.cur[i] = (x0 + x < J->W && y0 + y < J->H) ? J->cy[static_cast<size_t>(y0 + y) * J->W + x0 + x] : 0;
+    }
+    wsync();
+    for (int base = 0; base < n; base += kMaxSub) {
+        const int m = imin(kMaxSub, n - base);
+        if (C.lane < m) C.L.pend[C.lane] = cands[base + C.lane];
+        wsync();
+        qpel_eval(C, x0, y0, bw, bh, C.L.pend, m, mode, metric, C.L.pendcost);
+        if (C.lane < m) out[base + C.lane] = C.L.pendcost[C.lane];
+        wsync();
+    }
+}
+
 __global__ void __launch_bounds__(64) k_mb_diagonal(const JobDesc* __restrict__ jobs, int diag) {
     extern __shared__ __align__(16) uint8_t smem[];
     const JobDesc* J = &jobs[blockIdx.y];
@@ -2441,6 +2476,13 @@ hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int ma
 hipError_t launch_mb_persistent(const JobDesc* d_jobs, const SchedDesc& sched, int workgroups, int lds_bytes,
                                 hipStream_t s) {
     hipLaunchKernelGGL(k_mb_persistent, dim3(workgroups), dim3(64), lds_bytes, s, d_jobs, sched);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_costs(const JobDesc* d_job, int x0, int y0, int bw, int bh, int mode, int metric,
+                              const int2* cands, int n, uint64_t* out, int lds_bytes, hipStream_t s) {
+    hipLaunchKernelGGL(k_block_costs, dim3(1), dim3(64), lds_bytes, s, d_job, x0, y0, bw, bh, mode, metric, cands, n,
+                       out);
     return hipGetLastError();
 }
 
