@@ -222,14 +222,6 @@ int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
  * like the reference's SpeedProfiler, src/encoder_internal.h:182-221). */
 int vc1me_set_profiling(vc1me_ctx* ctx, int on);
 
-/* Macroblock scheduling of the analysis kernels: mode 1 (default) launches
- * one kernel per anti-diagonal; mode 0 runs one persistent launch whose
- * 64-lane workgroups take macroblocks from a work queue in anti-diagonal
- * order and wait only for their own neighbours (experimental: it is not
- * enabled by default).  workgroups = 0 picks 8 per compute unit.  No
- * reference counterpart (tuning knob). */
-int vc1me_set_schedule(vc1me_ctx* ctx, int mode, int workgroups);
-
 /* Algorithmic work of the last step, summed over its jobs, in the units of
  * the roofline: one |a-b| accumulate per sampled pixel of a block SAD
  * (sad_px), one 4-tap filter application per interpolated sample (1D = 1,

@@ -21,7 +21,7 @@ EXPORTS = (
     "vc1me_params_default", "vc1me_create", "vc1me_destroy", "vc1me_last_error", "vc1me_abi_version",
     "vc1me_device_name", "vc1me_score_batch", "vc1me_frame_upload", "vc1me_frame_release",
     "vc1me_score_batch_frames", "vc1me_analyze_p_frame", "vc1me_analyze_step", "vc1me_last_timing",
-    "vc1me_set_profiling", "vc1me_last_work", "vc1me_set_schedule", "vc1me_counters_get", "vc1me_counters_reset",
+    "vc1me_set_profiling", "vc1me_last_work", "vc1me_counters_get", "vc1me_counters_reset",
     "vc1me_block_costs",
 )
 
@@ -66,7 +66,6 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
     lib.vc1me_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_last_work.argtypes = [C.c_void_p, P(abi.Work)]
-    lib.vc1me_set_schedule.argtypes = [C.c_void_p, C.c_int, C.c_int]
     lib.vc1me_block_costs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_int, C.c_int, P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
     lib.vc1me_counters_get.argtypes = [C.c_void_p, P(abi.Counters)]
@@ -171,9 +170,6 @@ class Backend:
 
     def set_profiling(self, on: bool) -> None:
         self._check(self.lib.vc1me_set_profiling(self.ctx, 1 if on else 0))
-
-    def set_schedule(self, mode: int, workgroups: int = 0) -> None:
-        self._check(self.lib.vc1me_set_schedule(self.ctx, mode, workgroups))
 
     def timing(self) -> abi.Timing:
         t = abi.Timing()

@@ -85,7 +85,6 @@ struct JobSlot {
     GlobalState* g = nullptr;
     int32_t* err = nullptr;
     unsigned long long* ops = nullptr;
-    uint32_t* done = nullptr;
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -112,7 +111,6 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_g = carve(sizeof(GlobalState));
     const size_t o_err = carve(sizeof(int32_t));
     const size_t o_ops = carve(8 * sizeof(unsigned long long));
-    const size_t o_done = carve(mbs * sizeof(uint32_t));
     hipError_t e = hipMalloc(&s.mem, off);
     if (e != hipSuccess) { s.mem = nullptr; return e; }
     uint8_t* b = static_cast<uint8_t*>(s.mem);
@@ -132,9 +130,6 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.g = reinterpret_cast<GlobalState*>(b + o_g);
     s.err = reinterpret_cast<int32_t*>(b + o_err);
     s.ops = reinterpret_cast<unsigned long long*>(b + o_ops);
-    s.done = reinterpret_cast<uint32_t*>(b + o_done);
-    e = hipMemset(s.done, 0, mbs * sizeof(uint32_t));
-    if (e != hipSuccess) return e;
     s.mbs_cap = mbs;
     s.cells_cap = cells;
     return hipSuccess;
@@ -172,16 +167,7 @@ struct vc1me_ctx {
     uint32_t* d_costs = nullptr;
     size_t cand_cap = 0;
     std::vector<uint32_t> h_costs;
-    // persistent scheduler
-    int sched_mode = 1;        // 0 persistent work queue, 1 one launch per anti-diagonal
-    int persist_wgs = 0;       // 0: 8 per compute unit
     int cu_count = 256;
-    uint32_t epoch = 0;
-    int32_t* d_next = nullptr;
-    int32_t* d_order = nullptr;
-    size_t order_cap = 0;
-    std::vector<int32_t> h_order;
-    std::vector<int64_t> order_key;
 };
 
 namespace {
@@ -343,7 +329,6 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.g = s.g;
     d.err = s.err;
     d.ops = s.ops;
-    d.done = s.done;
     d.max_diag = d.mbw + 2 * d.mbh - 3;
 }
 
@@ -473,55 +458,15 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
     HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
     const int lds = vc1me::mb_kernel_lds_bytes(max_local);
     int launches = 0;
-    if (ctx->sched_mode == 0) {
-        // ticket order: anti-diagonals across all jobs (cached per set of rasters)
-        std::vector<int64_t> key;
-        for (int i = 0; i < n; ++i) key.push_back((static_cast<int64_t>(ctx->h_jobs[i].mbw) << 32) | ctx->h_jobs[i].mbh);
-        if (key != ctx->order_key) {
-            ctx->h_order.clear();
-            for (int t = 0; t <= max_diag; ++t)
-                for (int i = 0; i < n; ++i) {
-                    const JobDesc& d = ctx->h_jobs[i];
-                    if (t > d.max_diag) continue;
-                    const int lo = std::max(0, (t - (d.mbw - 1) + 1) / 2), hi = std::min(d.mbh - 1, t / 2);
-                    for (int my = lo; my <= hi; ++my)
-                        ctx->h_order.push_back((i << 24) | (my * d.mbw + (t - 2 * my)));
-                }
-            if (ctx->h_order.size() > ctx->order_cap) {
-                if (ctx->d_order) (void)hipFree(ctx->d_order);
-                ctx->d_order = nullptr;
-                ctx->order_cap = 0;
-                HIP_TRY(hipMalloc(&ctx->d_order, ctx->h_order.size() * sizeof(int32_t)));
-                ctx->order_cap = ctx->h_order.size();
-            }
-            HIP_TRY(hipMemcpy(ctx->d_order, ctx->h_order.data(), ctx->h_order.size() * sizeof(int32_t),
-                              hipMemcpyHostToDevice));
-            ctx->order_key = key;
-        }
-        if (!ctx->d_next) HIP_TRY(hipMalloc(&ctx->d_next, 2 * sizeof(int32_t)));
-        HIP_TRY(hipMemsetAsync(ctx->d_next, 0, 2 * sizeof(int32_t), st));
-        vc1me::SchedDesc sd;
-        sd.next = ctx->d_next;
-        sd.order = ctx->d_order;
-        sd.total = static_cast<int32_t>(ctx->h_order.size());
-        sd.epoch = ++ctx->epoch;
-        sd.abort = ctx->d_next + 1;
-        const int wgs = std::max(1, std::min(sd.total, ctx->persist_wgs > 0 ? ctx->persist_wgs : 8 * ctx->cu_count));
-        HIP_TRY(hipEventRecord(ctx->events[4], st));
-        HIP_TRY(vc1me::launch_mb_persistent(ctx->d_jobs, sd, wgs, lds, st));
-        HIP_TRY(hipEventRecord(ctx->events[5], st));
-        launches = 1;
-    } else {
-        for (int t = 0; t <= max_diag; ++t) {
-            int per = 0;
-            for (int i = 0; i < n; ++i)
-                if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
-            if (!per) continue;
-            if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[4 + 2 * t], st));
-            HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
-            if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[5 + 2 * t], st));
-            ++launches;
-        }
+    for (int t = 0; t <= max_diag; ++t) {
+        int per = 0;
+        for (int i = 0; i < n; ++i)
+            if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
+        if (!per) continue;
+        if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[4 + 2 * t], st));
+        HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
+        if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[5 + 2 * t], st));
+        ++launches;
     }
     HIP_TRY(hipEventRecord(ctx->events[1], st));
     HIP_TRY(hipEventSynchronize(ctx->events[1]));
@@ -533,11 +478,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
     ctx->timing.global_kernel_ms_total = gms;
     ctx->timing.global_kernel_launches = 9;
     double mbms = 0.0;
-    if (ctx->sched_mode == 0) {
-        float x = 0.f;
-        HIP_TRY(hipEventElapsedTime(&x, ctx->events[4], ctx->events[5]));
-        mbms = x;
-    } else if (ctx->profiling) {
+    if (ctx->profiling) {
         for (int t = 0; t <= max_diag; ++t) {
             int per = 0;
             for (int i = 0; i < n; ++i)
@@ -619,8 +560,6 @@ int vc1me_create(int device, vc1me_ctx** out) {
         ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
         if (prop.multiProcessorCount > 0) ctx->cu_count = prop.multiProcessorCount;
     }
-    if (const char* e = std::getenv("VC1ME_SCHED")) ctx->sched_mode = std::string(e) == "persistent" ? 0 : 1;
-    if (const char* e = std::getenv("VC1ME_PERSISTENT_WGS")) ctx->persist_wgs = std::max(0, std::atoi(e));
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete ctx; return fail(VC1ME_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
     *out = ctx;
@@ -638,8 +577,6 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
     if (ctx->d_cands) (void)hipFree(ctx->d_cands);
     if (ctx->d_costs) (void)hipFree(ctx->d_costs);
-    if (ctx->d_next) (void)hipFree(ctx->d_next);
-    if (ctx->d_order) (void)hipFree(ctx->d_order);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -806,16 +743,6 @@ int vc1me_set_profiling(vc1me_ctx* ctx, int on) {
     if (!ctx) return fail(VC1ME_EINVAL, "null context");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->profiling = on != 0;
-    return VC1ME_OK;
-}
-
-int vc1me_set_schedule(vc1me_ctx* ctx, int mode, int workgroups) {
-    if (!ctx) return fail(VC1ME_EINVAL, "null context");
-    if (mode != 0 && mode != 1) return fail(VC1ME_EINVAL, "schedule mode must be 0 (persistent) or 1 (diagonal)");
-    if (workgroups < 0 || workgroups > 65536) return fail(VC1ME_EINVAL, "workgroup count out of range");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->sched_mode = mode;
-    ctx->persist_wgs = workgroups;
     return VC1ME_OK;
 }
 

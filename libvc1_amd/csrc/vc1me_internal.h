@@ -54,19 +54,7 @@ struct JobDesc {
     // algorithmic operation counters of this job (sad_px, interp_taps,
     // satd_ops, sig_ops, frame_px), in the units of the CPU oracle
     unsigned long long* ops;
-    uint32_t* done;             // per-MB completion epoch (persistent scheduler)
     int32_t max_diag;           // last anti-diagonal index mbw + 2*mbh - 3
-};
-
-// Work queue of the persistent macroblock kernel: tickets are handed out in
-// anti-diagonal order across all jobs, so every dependency of a ticket has a
-// smaller ticket.
-struct SchedDesc {
-    int32_t* next;              // ticket counter
-    const int32_t* order;       // ticket -> (job << 24) | macroblock index
-    int32_t total;
-    uint32_t epoch;             // value written to JobDesc::done when an MB is final
-    int32_t* abort;             // set when a wait timed out: later waits give up at once
 };
 
 // Flag bits of JobDesc::flags
@@ -79,8 +67,6 @@ hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc*
 hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hipStream_t s);
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s);
-hipError_t launch_mb_persistent(const JobDesc* d_jobs, const SchedDesc& sched, int workgroups, int lds_bytes,
-                                hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);
 hipError_t launch_block_costs(const JobDesc* d_job, int x0, int y0, int bw, int bh, int mode, int metric,
                               const int2* cands, int n, uint64_t* out, int lds_bytes, hipStream_t s);

@@ -696,6 +696,19 @@ __device__ bool qpel_window(Ctx& C, int bx0, int by0, int bw, int bh, int xq_lo,
     return win_load(C, nx0, ny0, nx1 - nx0, ny1 - ny0);
 }
 
+// A filtered sample saturated to [0, 255], ready to be packed with others.
+// The empty asm makes the clamped value opaque: without it the compiler fuses
+// "shift right, saturate to u8, pack two samples" into v_ashr_pk_u8_i32 and
+// then ORs the next samples into bits 16-31 assuming that instruction zeroed
+// them; on gfx950 the upper half keeps the destination register's old bits,
+// which turned bytes 2-3 of a packed prediction into 0xff whenever the
+// reused register held a negative pre-shift value.
+__device__ __forceinline__ uint32_t sat_u8(int v) {
+    uint32_t r = static_cast<uint32_t>(clampi(v, 0, 255));
+    asm("" : "+v"(r));
+    return r;
+}
+
 // Four horizontally adjacent predicted samples (luma_mc_sample /
 // luma_mc_sample_mode, src/encoder_motion_compensation.cpp:367-415) packed
 // little-endian; (sx, sy) is the window position of the first sample's
@@ -741,7 +754,7 @@ __device__ __forceinline__ uint32_t pred4(const uint8_t* w, int wp, int sx, int 
         for (int j = 0; j < 4; ++j) {
             const int v = arshift(mspel(byte_at(lo, hi, j), byte_at(lo, hi, j + 1), byte_at(lo, hi, j + 2),
                                         byte_at(lo, hi, j + 3), hm) + bias, sh);
-            out |= static_cast<uint32_t>(clampi(v, 0, 255)) << (8 * j);
+            out |= sat_u8(v) << (8 * j);
         }
         return out;
     }
@@ -754,7 +767,7 @@ __device__ __forceinline__ uint32_t pred4(const uint8_t* w, int wp, int sx, int 
         for (int j = 0; j < 4; ++j) {
             const int v = arshift(mspel(byte_at(L[0], Hh[0], j + 1), byte_at(L[1], Hh[1], j + 1),
                                         byte_at(L[2], Hh[2], j + 1), byte_at(L[3], Hh[3], j + 1), vm) + bias, sh);
-            out |= static_cast<uint32_t>(clampi(v, 0, 255)) << (8 * j);
+            out |= sat_u8(v) << (8 * j);
         }
         return out;
     }
@@ -768,7 +781,7 @@ __device__ __forceinline__ uint32_t pred4(const uint8_t* w, int wp, int sx, int 
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int v = arshift(mspel(t[j], t[j + 1], t[j + 2], t[j + 3], hm) + 64, 7);
-        out |= static_cast<uint32_t>(clampi(v, 0, 255)) << (8 * j);
+        out |= sat_u8(v) << (8 * j);
     }
     return out;
 }
@@ -911,6 +924,18 @@ __device__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* 
         if (mine && (C.lane & (bw == 16 ? 63 : 15)) == 0) out[c] = skip ? skip_cost : static_cast<uint64_t>(total);
     }
     wsync();
+}
+
+// Op counts of one more SAD evaluation of v on the block, as qpel_eval counts
+// it: for evaluations the reference repeats and this code serves from a table.
+__device__ void count_sad_eval(Ctx& C, int bx0, int by0, int bw, int bh, int2 v, int mode) {
+    const bool bil = mode_bilinear(mode);
+    const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
+    if (hidden_padding(C.g, bx0, by0, bw, bh, v.x, v.y, bil ? mode : 0, chroma_block(bx0, by0, bw, bh)) ||
+        (bil && ((v.x & 1) || (v.y & 1))))
+        return;
+    C.n_sad += static_cast<uint64_t>(aw) * ah;
+    C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(v.x & 3, v.y & 3, bil);
 }
 
 // ------------------------------------------------------------------ integer searches
@@ -1845,6 +1870,11 @@ __device__ void analyze_mb(Ctx& C) {
             for (int s = 1; s < 11; ++s) {
                 const int2 sd = C.L.req[s];
                 if (sd.x == best_seed.x && sd.y == best_seed.y) continue;
+                // the reference scores this seed again if it repeats an earlier one
+                for (int j = 0; j < s; ++j) {
+                    const int2 u = C.L.req[j];
+                    if (u.x == sd.x && u.y == sd.y) { count_sad_eval(C, bx, by, 8, 8, sd, 0); break; }
+                }
                 const uint64_t cc = C.L.top16[s];
                 const int cm = iabs(sd.x) + iabs(sd.y), bm = iabs(best_seed.x) + iabs(best_seed.y);
                 if (cc < seed_cost || (cc == seed_cost && cm < bm)) { seed_cost = cc; best_seed = sd; }
@@ -1949,58 +1979,6 @@ __device__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
     C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
     analyze_mb(C);
-}
-
-// 4 s of the 100 MHz real-time counter: far beyond any legitimate wait
-constexpr uint64_t kWaitTicks = 400000000ull;
-
-// Persistent form: each 64-lane workgroup takes tickets until the queue is
-// empty; before a macroblock starts it waits (bounded) until its left,
-// top-left, top and top-right neighbours are final.  A ticket's dependencies
-// always hold smaller tickets, which are held by running workgroups, so the
-// queue drains whatever the residency.
-__global__ void __launch_bounds__(64) k_mb_persistent(const JobDesc* __restrict__ jobs, SchedDesc S) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    for (;;) {
-        int t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(S.next, 1);
-        t = __shfl(t, 0);
-        if (t >= S.total) break;
-        const int code = S.order[t];
-        const JobDesc* J = &jobs[code >> 24];
-        const int pos = code & 0xffffff;
-        const int mbw = J->mbw;
-        const int mx = pos % mbw, my = pos / mbw;
-        bool ok = true;
-        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        auto wait = [&](int dpos) {
-            for (;;) {
-                const uint32_t v = __builtin_amdgcn_readfirstlane(
-                    __hip_atomic_load(&J->done[dpos], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
-                if (v == S.epoch) break;
-                if (__builtin_amdgcn_readfirstlane(__hip_atomic_load(S.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                    ok = false;
-                    break;
-                }
-                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
-                    if (threadIdx.x == 0) atomicExch(S.abort, 1);
-                    ok = false;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-        };
-        if (mx > 0) wait(pos - 1);
-        if (my > 0) {
-            if (mx > 0) wait(pos - mbw - 1);
-            wait(pos - mbw);
-            if (mx + 1 < mbw) wait(pos - mbw + 1);
-        }
-        if (!ok && threadIdx.x == 0) atomicMax(J->err, 9);
-        run_mb(J, mx, my, smem);
-        __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(&J->done[pos], S.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // Block costs under the search's own laws (block_sad_motion / block_satd_motion,
@@ -2470,12 +2448,6 @@ hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hi
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s) {
     hipLaunchKernelGGL(k_mb_diagonal, dim3(max_per_diag, njobs), dim3(64), lds_bytes, s, d_jobs, diag);
-    return hipGetLastError();
-}
-
-hipError_t launch_mb_persistent(const JobDesc* d_jobs, const SchedDesc& sched, int workgroups, int lds_bytes,
-                                hipStream_t s) {
-    hipLaunchKernelGGL(k_mb_persistent, dim3(workgroups), dim3(64), lds_bytes, s, d_jobs, sched);
     return hipGetLastError();
 }
 
