@@ -160,7 +160,7 @@ struct Geo {
     int W, H, vw, vh, mbw, mbh;
 };
 
-__device__ bool hidden_luma(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq, int mode) {
+__device__ __forceinline__ bool hidden_luma(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq, int mode) {
     const bool rp = g.vw < g.mbw * 16, bp = g.vh < g.mbh * 16;
     if (!rp && !bp) return false;
     const int aw = imax(0, imin(bw, g.vw - x0)), ah = imax(0, imin(bh, g.vh - y0));
@@ -171,7 +171,7 @@ __device__ bool hidden_luma(const Geo& g, int x0, int y0, int bw, int bh, int xq
     const int bottom = fy ? (mode_bilinear(mode) ? 1 : 2) : 0;
     return (rp && x0 + aw - 1 + ix + right >= g.vw) || (bp && y0 + ah - 1 + iy + bottom >= g.vh);
 }
-__device__ bool hidden_chroma(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq) {
+__device__ __forceinline__ bool hidden_chroma(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq) {
     const int cw = (g.vw + 1) / 2, ch = (g.vh + 1) / 2;
     const bool rp = cw < g.mbw * 8, bp = ch < g.mbh * 8;
     if (!rp && !bp) return false;
@@ -198,7 +198,7 @@ __device__ __forceinline__ bool chroma_block(int x0, int y0, int bw, int bh) {
 // 64-bit significand (round to nearest even).  lambda = m * 2^-k exactly, so
 // the exact sum is N * 2^-k with N = cost*2^k + m*rate; all scores of one MB
 // share k and compare as rounded N.
-__device__ unsigned __int128 x87_score(uint64_t cost, uint64_t rate, uint64_t m, int k) {
+__device__ __forceinline__ unsigned __int128 x87_score(uint64_t cost, uint64_t rate, uint64_t m, int k) {
     unsigned __int128 n = (static_cast<unsigned __int128>(cost) << k) + static_cast<unsigned __int128>(m) * rate;
     const uint64_t hi = static_cast<uint64_t>(n >> 64);
     if (!hi) return n;
@@ -266,7 +266,7 @@ __host__ __device__ inline int lds_bytes_for(int local_range) {
     return off + 64;
 }
 
-__device__ Lds carve(uint8_t* smem, int local_range) {
+__device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     Lds L;
     int off = 0;
     L.cur = smem + off; off += 256;
@@ -297,8 +297,11 @@ struct Ctx {
     int winCap;
     // current window
     int wx0, wy0, ww, wh, wp;
-    // algorithmic operation counts of this macroblock (uniform)
+    // algorithmic operation counts of this macroblock: uniform totals, plus
+    // per-lane partial counts of the lane-parallel block scoring (summed
+    // over the wave when the macroblock is done)
     uint64_t n_sad, n_taps, n_satd, n_sig;
+    uint32_t ln_sad, ln_taps, ln_satd;
 };
 
 __device__ __forceinline__ uint64_t taps_per_sample(int hm, int vm, bool bil) {
@@ -309,7 +312,7 @@ __device__ __forceinline__ uint64_t taps_per_sample(int hm, int vm, bool bil) {
 
 __device__ __forceinline__ void wsync() { __syncthreads(); }
 
-__device__ void set_err(Ctx& C, int code) {
+__device__ __forceinline__ void set_err(Ctx& C, int code) {
     if (C.lane == 0) atomicMax(C.J->err, code);
 }
 
@@ -318,7 +321,7 @@ __device__ void set_err(Ctx& C, int code) {
 // [wx0, wx0+ww) and rows [wy0, wy0+wh).  The stored origin is rounded down to
 // 16 bytes so whole rows move as 16-byte loads.  Returns false if the window
 // does not fit the buffer.
-__device__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int wh) {
+__device__ __forceinline__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int wh) {
     const int ax0 = wx0 & ~15;
     const int aw = wx0 + ww - ax0;
     const int wp = (aw + 8 + 15) & ~15;
@@ -364,7 +367,7 @@ __device__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int wh) {
 // Partial SAD of sampled rows part, part+G, ... of the 16x16 block at integer
 // offset (dx,dy).  Rows at or beyond the active height and columns at or
 // beyond the active width contribute nothing (mb_sad, src/encoder_motion.cpp:2274-2282).
-__device__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, int stride, int part, int G) {
+__device__ __forceinline__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, int stride, int part, int G) {
     const int ox = C.x0 + dx - C.wx0, oy = C.y0 + dy - C.wy0;
     const int nrows = (C.ah + stride - 1) / stride;
     const uint32_t smask = stride == 1 ? 0xffffffffu : (stride == 2 ? 0x00ff00ffu : 0x000000ffu);
@@ -409,7 +412,7 @@ enum { KEY_TOTAL = 0, KEY_SEQ_MAN = 1, KEY_SEQ = 2 };
 // bitset or null.  ukeys/ucount: optional log of every scored key.  Returns
 // the minimum key of the batch (kNone if no candidate was scored).
 template <class PtFn>
-__device__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int keymode, int seq0, PtFn pt,
+__device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int keymode, int seq0, PtFn pt,
                               uint64_t* ukeys = nullptr, int* ucount = nullptr) {
     const int rows = (C.ah + stride - 1) / stride;
     int G = 1;
@@ -483,7 +486,7 @@ __device__ __forceinline__ bool match_good(const Ctx& C, uint64_t sad) {
     return sad <= px * 6u;
 }
 // distant_match_decision, src/encoder_motion.cpp:1642-1656
-__device__ int distant_decision(const Ctx& C, uint64_t cand, uint64_t local) {
+__device__ __forceinline__ int distant_decision(const Ctx& C, uint64_t cand, uint64_t local) {
     const uint64_t px = active_pixels(C);
     if (!px || cand >= UINT64_MAX / 8) return 0;
     if (cand <= px * 6u) return 2;
@@ -499,11 +502,11 @@ struct Bounds {
     int xmin, xmax, ymin, ymax;
 };
 // motion_search_bounds_px / _qpel, src/encoder_motion.cpp:1238-1255
-__device__ Bounds bounds_px(int requested) {
+__device__ __forceinline__ Bounds bounds_px(int requested) {
     const int r = clampi(requested, 0, 1024);
     return {-imin(r, 1024), imin(r, 1023), -imin(r, 256), imin(r, 255)};
 }
-__device__ Bounds bounds_qpel(int requested) {
+__device__ __forceinline__ Bounds bounds_qpel(int requested) {
     const int r = clampi(requested, 0, 1024);
     Bounds b;
     b.xmin = -imin(r * 4, 4096);
@@ -524,7 +527,7 @@ __device__ __forceinline__ int median3(int a, int b, int c) {
     return a + b + c - imin(a, imin(b, c)) - imax(a, imax(b, c));
 }
 
-__device__ void pullback(int qx, int qy, int X, int Y, int margin, int& px, int& py) {
+__device__ __forceinline__ void pullback(int qx, int qy, int X, int Y, int margin, int& px, int& py) {
     if (qx + px < margin) px = margin - qx;
     if (qx + px > X) px = X - qx;
     if (qy + py < margin) py = margin - qy;
@@ -532,7 +535,7 @@ __device__ void pullback(int qx, int qy, int X, int Y, int margin, int& px, int&
 }
 
 // predictor_info, src/encoder_motion.cpp:1312-1352 (reads the final 1-MV field)
-__device__ Pred predictor_info(const Ctx& C) {
+__device__ __forceinline__ Pred predictor_info(const Ctx& C) {
     const JobDesc* J = C.J;
     const int mbw = C.g.mbw, mbh = C.g.mbh, mx = C.mx, my = C.my;
     const bool av = my > 0, cv = mx > 0, bv = my > 0 && mbw > 1;
@@ -553,20 +556,20 @@ __device__ Pred predictor_info(const Ctx& C) {
 }
 
 // luma block-vector field with the current macroblock's blocks taken from LDS
-__device__ int2 field_get(const Ctx& C, int x, int y) {
+__device__ __forceinline__ int2 field_get(const Ctx& C, int x, int y) {
     if (x < 0 || y < 0 || x >= C.g.mbw * 2 || y >= C.g.mbh * 2) return {0, 0};
     const int mmx = x >> 1, mmy = y >> 1, k = ((y & 1) << 1) | (x & 1);
     if (mmx == C.mx && mmy == C.my) return C.L.own[k];
     return C.J->bmv[(mmy * C.g.mbw + mmx) * 4 + k];
 }
-__device__ bool intra_at(const Ctx& C, int x, int y) {
+__device__ __forceinline__ bool intra_at(const Ctx& C, int x, int y) {
     if (x < 0 || y < 0 || x >= C.g.mbw * 2 || y >= C.g.mbh * 2) return false;
     const int mmx = x >> 1, mmy = y >> 1;
     if (mmx == C.mx && mmy == C.my) return false;
     return (C.J->flags[mmy * C.g.mbw + mmx] & kFlagIntra) != 0;
 }
 
-__device__ bool hybrid_flag(const Ctx& C, int px, int py, int bx, int by, int2 A, int2 Cc) {
+__device__ __forceinline__ bool hybrid_flag(const Ctx& C, int px, int py, int bx, int by, int2 A, int2 Cc) {
     const int suma = intra_at(C, bx, by - 1) ? iabs(px) + iabs(py) : iabs(px - A.x) + iabs(py - A.y);
     if (suma > 32) return true;
     const int sumc = intra_at(C, bx - 1, by) ? iabs(px) + iabs(py) : iabs(px - Cc.x) + iabs(py - Cc.y);
@@ -574,7 +577,7 @@ __device__ bool hybrid_flag(const Ctx& C, int px, int py, int bx, int by, int2 A
 }
 
 // predictor_info_mixed_1mv, src/encoder_motion.cpp:1354-1406
-__device__ Pred predictor_mixed_1mv(const Ctx& C) {
+__device__ __forceinline__ Pred predictor_mixed_1mv(const Ctx& C) {
     const int mbw = C.g.mbw, mbh = C.g.mbh, mx = C.mx, my = C.my;
     const int bx = mx * 2, by = my * 2;
     const bool av = my > 0, cv = mx > 0, bv = my > 0 && mbw > 1;
@@ -590,7 +593,7 @@ __device__ Pred predictor_mixed_1mv(const Ctx& C) {
 }
 
 // predictor_info_4mv, src/encoder_motion.cpp:1408-1466
-__device__ Pred predictor_4mv(const Ctx& C, int n) {
+__device__ __forceinline__ Pred predictor_4mv(const Ctx& C, int n) {
     const int mbw = C.g.mbw, mbh = C.g.mbh, mx = C.mx, my = C.my;
     const int bx = mx * 2 + (n & 1), by = my * 2 + ((n >> 1) & 1);
     const bool av = my > 0 || n == 2 || n == 3;
@@ -619,7 +622,7 @@ __device__ Pred predictor_4mv(const Ctx& C, int n) {
 // ------------------------------------------------------------------ MV rate
 // motion_vector_rate_bits and its syntax helpers, src/encoder_motion.cpp:1099-1197, :3214-3225
 
-__device__ int mv_cat(int v, bool hp) {
+__device__ __forceinline__ int mv_cat(int v, bool hp) {
     const int a = iabs(v);
     if (!a) return 0;
     if (a <= 2) return 1;
@@ -629,7 +632,7 @@ __device__ int mv_cat(int v, bool hp) {
     if (a <= (hp ? 94 : 158)) return 5;
     return -1;
 }
-__device__ int mv_component_bits(int cat, bool hp) {
+__device__ __forceinline__ int mv_component_bits(int cat, bool hp) {
     const int sz = cat == 0 ? 0 : cat == 1 ? 2 : cat == 2 ? 3 : cat == 3 ? 4 : cat == 4 ? 5 : 8;
     return sz - ((hp && cat == 5) ? 1 : 0);
 }
@@ -639,7 +642,7 @@ __device__ __forceinline__ int mvrange_xq(int r) { return r == 0 ? 256 : r == 1 
 __device__ __forceinline__ int mvrange_yq(int r) { return r == 0 ? 128 : r == 1 ? 256 : r == 2 ? 512 : 1024; }
 
 // bits of a differential (dx,dy) (already in mode units), with `more`
-__device__ uint32_t mvdiff_bits(int mvrange, int dx, int dy, bool hp, bool more) {
+__device__ __forceinline__ uint32_t mvdiff_bits(int mvrange, int dx, int dy, bool hp, bool more) {
     const int xc = mv_cat(dx, hp), yc = mv_cat(dy, hp);
     int sym;
     if (xc < 0 || yc < 0) sym = more ? 71 : 34;
@@ -657,20 +660,20 @@ __device__ uint32_t mvdiff_bits(int mvrange, int dx, int dy, bool hp, bool more)
     }
     return static_cast<uint32_t>(kMvBits0[sym]) + static_cast<uint32_t>(suffix);
 }
-__device__ uint32_t mv_rate_bits(int mvrange, int xq, int yq, int pxq, int pyq, int mode, bool more) {
+__device__ __forceinline__ uint32_t mv_rate_bits(int mvrange, int xq, int yq, int pxq, int pyq, int mode, bool more) {
     const bool hp = mode_halfpel(mode);
     int dx = xq - pxq, dy = yq - pyq;
     if (hp) { dx /= 2; dy /= 2; }
     return mvdiff_bits(mvrange, dx, dy, hp, more);
 }
 // modular_mv_distance, src/encoder_motion.cpp:1299-1310
-__device__ int mod_dist(int a, int b, int range) {
+__device__ __forceinline__ int mod_dist(int a, int b, int range) {
     int d = a - b;
     const int span = range * 2;
     d = ((d + range) % span + span) % span - range;
     return iabs(d);
 }
-__device__ int modular_mv_distance(int mvrange, int ax, int ay, int bx, int by) {
+__device__ __forceinline__ int modular_mv_distance(int mvrange, int ax, int ay, int bx, int by) {
     return mod_dist(ax, bx, mvrange_xq(mvrange)) + mod_dist(ay, by, mvrange_yq(mvrange));
 }
 
@@ -687,7 +690,7 @@ __device__ __forceinline__ int mspel(int a, int b, int c, int d, int mode) {
 // Window covering the interpolation footprint of block (bx0,by0,bw,bh) for all
 // vectors in [xq_lo,xq_hi]x[yq_lo,yq_hi] (taps -1..+2 around each integer
 // position, plus 8 bytes of row slack for the 4-pixel reads).
-__device__ bool qpel_window(Ctx& C, int bx0, int by0, int bw, int bh, int xq_lo, int xq_hi, int yq_lo, int yq_hi) {
+__device__ __forceinline__ bool qpel_window(Ctx& C, int bx0, int by0, int bw, int bh, int xq_lo, int xq_hi, int yq_lo, int yq_hi) {
     const int ix_lo = floor_div4(xq_lo), ix_hi = floor_div4(xq_hi);
     const int iy_lo = floor_div4(yq_lo), iy_hi = floor_div4(yq_hi);
     const int nx0 = bx0 + ix_lo - 1, ny0 = by0 + iy_lo - 1;
@@ -818,7 +821,7 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, int bw) {
 // block_satd_motion (src/encoder_motion.cpp:2962-3042: rendered prediction,
 // (|H D H^T| + 1) >> 1 per 4x4 tile, zero residual outside the active area).
 // Results in out[0..n); every lane can read them on return.
-__device__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* vs, int n, int mode, int metric,
+__device__ __forceinline__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* vs, int n, int mode, int metric,
                           uint64_t* out) {
     if (n <= 0) return;
     const bool bil = mode_bilinear(mode);
@@ -894,32 +897,12 @@ __device__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* 
             }
         }
         const uint32_t total = block_sum(part, bw);
-        // op counts of the candidates scored in this pass
-        {
-            const uint64_t lead = __ballot(!skip && ((C.lane & (bw == 16 ? 63 : 15)) == 0));
-            const uint64_t cnt = __popcll(lead);
-            C.n_sad += metric == 0 ? cnt * static_cast<uint64_t>(aw) * ah : 0;
-            if (cnt) {
-                // taps depend on the phase of each candidate
-                for (int q = 0; q < per; ++q) {
-                    const int cc = base + q;
-                    if (cc >= n) break;
-                    const int2 vv = vs[cc];
-                    bool sk;
-                    if (metric == 0)
-                        sk = hidden_padding(C.g, bx0, by0, bw, bh, vv.x, vv.y, bil ? mode : 0, chroma) ||
-                             (bil && ((vv.x & 1) || (vv.y & 1)));
-                    else
-                        sk = !aw || !ah || (mode_halfpel(mode) && ((vv.x & 1) || (vv.y & 1)));
-                    if (!sk) {
-                        C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(vv.x & 3, vv.y & 3, bil);
-                        if (metric == 1) {
-                            const uint64_t tiles = static_cast<uint64_t>((aw + 3) / 4) * ((ah + 3) / 4);
-                            C.n_satd += 112ull * tiles;
-                        }
-                    }
-                }
-            }
+        // op counts: the lead lane of each scored candidate counts it
+        if (!skip && (C.lane & (bw == 16 ? 63 : 15)) == 0) {
+            const uint32_t px = static_cast<uint32_t>(aw * ah);
+            if (metric == 0) C.ln_sad += px;
+            C.ln_taps += px * static_cast<uint32_t>(taps_per_sample(hm, vm, bil));
+            if (metric == 1) C.ln_satd += 112u * static_cast<uint32_t>(((aw + 3) / 4) * ((ah + 3) / 4));
         }
         if (mine && (C.lane & (bw == 16 ? 63 : 15)) == 0) out[c] = skip ? skip_cost : static_cast<uint64_t>(total);
     }
@@ -928,7 +911,7 @@ __device__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* 
 
 // Op counts of one more SAD evaluation of v on the block, as qpel_eval counts
 // it: for evaluations the reference repeats and this code serves from a table.
-__device__ void count_sad_eval(Ctx& C, int bx0, int by0, int bw, int bh, int2 v, int mode) {
+__device__ __forceinline__ void count_sad_eval(Ctx& C, int bx0, int by0, int bw, int bh, int2 v, int mode) {
     const bool bil = mode_bilinear(mode);
     const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
     if (hidden_padding(C.g, bx0, by0, bw, bh, v.x, v.y, bil ? mode : 0, chroma_block(bx0, by0, bw, bh)) ||
@@ -951,7 +934,7 @@ struct IRes {
 // together and only the stage minimum updates `best`.  Every considered key
 // is logged, and the finalists (the best four of the top-8) are extracted
 // from the log once, at the end.
-__device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
+__device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
     // the whole search stays inside [-R,R]^2: one window serves every stage
     if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
         set_err(C, 5);
@@ -1125,7 +1108,7 @@ __device__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
 }
 
 // integer_motion_diamond, src/encoder_motion_diamond.cpp:13-73
-__device__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
+__device__ __forceinline__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
     if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
         set_err(C, 5);
         return {0, 0, kBadSad};
@@ -1185,7 +1168,7 @@ __device__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) 
 
 // Integer SADs of a cluster of far candidates: window over the bounding box.
 template <class PtFn>
-__device__ uint64_t far_batch(Ctx& C, int n, int stride, int keymode, int seq0, PtFn pt) {
+__device__ __forceinline__ uint64_t far_batch(Ctx& C, int n, int stride, int keymode, int seq0, PtFn pt) {
     int xlo = 1 << 30, xhi = -(1 << 30), ylo = 1 << 30, yhi = -(1 << 30);
     for (int i = 0; i < n; ++i) {
         int dx, dy;
@@ -1200,7 +1183,7 @@ __device__ uint64_t far_batch(Ctx& C, int n, int stride, int keymode, int seq0, 
 }
 
 // propagated_long_range_motion, src/encoder_motion.cpp:1658-1694
-__device__ IRes propagated(Ctx& C, int range) {
+__device__ __forceinline__ IRes propagated(Ctx& C, int range) {
     const JobDesc* J = C.J;
     const Bounds b = bounds_px(range);
     uint64_t best = kNone;
@@ -1234,7 +1217,7 @@ __device__ IRes propagated(Ctx& C, int range) {
 }
 
 // long_range_signature of the current block (src/encoder_motion.cpp:1591-1601)
-__device__ uint64_t cur_signature(Ctx& C) {
+__device__ __forceinline__ uint64_t cur_signature(Ctx& C) {
     const JobDesc* J = C.J;
     const int W = C.g.W, H = C.g.H, cw = W / 2, ch = H / 2;
     uint32_t v[6], n[6];
@@ -1270,7 +1253,7 @@ __device__ __forceinline__ uint32_t sig_distance(uint64_t a, uint64_t b) {
 }
 
 // extended_content_motion, src/encoder_motion.cpp:1696-1779
-__device__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_sad, int* cand_decision) {
+__device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_sad, int* cand_decision) {
     const JobDesc* J = C.J;
     const uint64_t want = cur_signature(C);
     const Bounds b = bounds_px(range);
@@ -1356,7 +1339,7 @@ struct SubParams {
     bool more;
 };
 
-__device__ int make_even(int v) { return (v & 1) ? (v > 0 ? v - 1 : v + 1) : v; }
+__device__ __forceinline__ int make_even(int v) { return (v & 1) ? (v > 0 ? v - 1 : v + 1) : v; }
 
 // (cost, |x|+|y|, y, x) key of the sub-pel stages (tie order of
 // src/encoder_motion.cpp:3320-3325)
@@ -1371,7 +1354,7 @@ __device__ __forceinline__ int keyq_y(uint64_t k) { return static_cast<int>((k >
 // the m vectors in L.req: vectors not yet in the list are scored together and
 // appended.  L.slot[i] receives the list index of batch entry i.  Returns the
 // new list length.
-__device__ int sub_add(Ctx& C, const SubParams& sp, int n, int m) {
+__device__ __forceinline__ int sub_add(Ctx& C, const SubParams& sp, int n, int m) {
     const int i = C.lane;
     int found = -1;
     int2 v = make_int2(0, 0);
@@ -1411,7 +1394,7 @@ __device__ int sub_add(Ctx& C, const SubParams& sp, int n, int m) {
 }
 
 // minimum (SAD, tie order) key over the m entries of the last batch
-__device__ uint64_t batch_min_key(Ctx& C, int m) {
+__device__ __forceinline__ uint64_t batch_min_key(Ctx& C, int m) {
     uint64_t k = kNone;
     if (C.lane < m) {
         const SubCand& s = C.L.sub[C.L.slot[C.lane]];
@@ -1420,12 +1403,12 @@ __device__ uint64_t batch_min_key(Ctx& C, int m) {
     return wave_min64(k);
 }
 
-__device__ void set_req(Ctx& C, int i, int x, int y) {
+__device__ __forceinline__ void set_req(Ctx& C, int i, int x, int y) {
     if (C.lane == i) C.L.req[i] = make_int2(x, y);
 }
 
 // refine_motion_integer_only, src/encoder_motion.cpp:3247-3269
-__device__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int syq, uint64_t* best_sad, uint64_t known) {
+__device__ __forceinline__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int syq, uint64_t* best_sad, uint64_t known) {
     const Bounds qb = bounds_qpel(sp.range);
     auto snap = [](int q, int lo, int hi) {
         q = clampi(q, lo, hi);
@@ -1448,7 +1431,7 @@ __device__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int sy
 }
 
 // refine_motion_block_staged, src/encoder_motion.cpp:3271-3594 (quality RATE / SATD)
-__device__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred& pi, uint64_t* best_sad,
+__device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred& pi, uint64_t* best_sad,
                               uint64_t known_seed) {
     const JobDesc* J = C.J;
     const int limit_q = 4 * J->subpel_range;
@@ -1686,7 +1669,7 @@ __device__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, con
 // ------------------------------------------------------------------ the macroblock kernel
 
 // median4, src/encoder_motion_compensation.cpp:522-529
-__device__ int median4(int a, int b, int c, int d) {
+__device__ __forceinline__ int median4(int a, int b, int c, int d) {
     if (a < b) {
         if (c < d) return (imin(b, d) + imax(a, c)) / 2;
         return (imin(b, c) + imax(a, d)) / 2;
@@ -1695,7 +1678,7 @@ __device__ int median4(int a, int b, int c, int d) {
     return (imin(a, c) + imax(b, d)) / 2;
 }
 
-__device__ void analyze_mb(Ctx& C) {
+__device__ __forceinline__ void analyze_mb(Ctx& C) {
     const JobDesc* J = C.J;
     const int mode = J->mode;
     const bool allow_4mv = mode == 3;
@@ -1946,6 +1929,9 @@ __device__ void analyze_mb(Ctx& C) {
             chosen_sad = intra_sad;
         }
     }
+    C.n_sad += wave_sum32(C.ln_sad);
+    C.n_taps += wave_sum32(C.ln_taps);
+    C.n_satd += wave_sum32(C.ln_satd);
     if (C.lane == 0) {
         J->mvs[pos] = outmv;
         for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = outb[k];
@@ -1964,7 +1950,7 @@ __device__ void analyze_mb(Ctx& C) {
     }
 }
 
-__device__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
+__device__ __forceinline__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
     Ctx C;
     C.J = J;
     C.g = {J->W, J->H, J->vw, J->vh, J->mbw, J->mbh};
@@ -1978,6 +1964,7 @@ __device__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
     C.winCap = win_capacity(J->local_range);
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
     C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
+    C.ln_sad = C.ln_taps = C.ln_satd = 0;
     analyze_mb(C);
 }
 
@@ -2001,6 +1988,7 @@ __global__ void __launch_bounds__(64) k_block_costs(const JobDesc* __restrict__ 
     C.winCap = win_capacity(J->local_range);
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
     C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
+    C.ln_sad = C.ln_taps = C.ln_satd = 0;
     for (int i = C.lane; i < 256; i += kWave) {
         const int x = i & 15, y = i >> 4;
         C.L.cur[i] = (x0 + x < J->W && y0 + y < J->H) ? J->cy[static_cast<size_t>(y0 + y) * J->W + x0 + x] : 0;
@@ -2068,7 +2056,7 @@ __global__ void __launch_bounds__(256) k_frame_sads(const JobDesc* __restrict__ 
     }
 }
 
-__device__ bool g_better(uint64_t ac, int ax, int ay, uint64_t bc, int bx, int by) {
+__device__ __forceinline__ bool g_better(uint64_t ac, int ax, int ay, uint64_t bc, int bx, int by) {
     if (ac != bc) return ac < bc;
     const int am = iabs(ax) + iabs(ay), bm = iabs(bx) + iabs(by);
     if (am != bm) return am < bm;
@@ -2077,7 +2065,7 @@ __device__ bool g_better(uint64_t ac, int ax, int ay, uint64_t bc, int bx, int b
 }
 
 // sorted insertion of the extended search's consider (src/encoder_motion.cpp:2036-2041)
-__device__ void g_consider(GlobalState* g, uint64_t cost, int dx, int dy) {
+__device__ __forceinline__ void g_consider(GlobalState* g, uint64_t cost, int dx, int dy) {
     for (int i = 0; i < g->tn; ++i)
         if (g->tdx[i] == dx && g->tdy[i] == dy) {
             if (cost < g->tcost[i]) g->tcost[i] = cost;
@@ -2097,12 +2085,12 @@ __device__ void g_consider(GlobalState* g, uint64_t cost, int dx, int dy) {
     if (g->tn > 8) g->tn = 8;
 }
 
-__device__ void g_set_stage(GlobalState* g, int stride) {
+__device__ __forceinline__ void g_set_stage(GlobalState* g, int stride) {
     g->stride = stride;
     for (int i = 0; i < g->n; ++i) g->cost[i] = 0;
 }
 
-__device__ int g_range(const JobDesc* J) { return imin(J->local_range, 32); }
+__device__ __forceinline__ int g_range(const JobDesc* J) { return imin(J->local_range, 32); }
 
 // Stage 0: the first point list.  Extended search (range 32): cross and
 // multi-hex at stride 16; grid search (range 1..31): coarse grid at stride 4.
