@@ -25,6 +25,11 @@ struct vc1me_frame {
     uint8_t* u = nullptr;
     uint8_t* v = nullptr;
     int32_t w = 0, h = 0;
+    // interpolated phase planes of this frame as a reference (bicubic 16,
+    // bilinear 4), allocated on first use; their contents are recomputed in
+    // every step that reads them
+    mutable uint8_t* qp = nullptr;
+    mutable uint8_t* bp = nullptr;
 };
 
 namespace {
@@ -330,6 +335,26 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.err = s.err;
     d.ops = s.ops;
     d.max_diag = d.mbw + 2 * d.mbh - 3;
+    d.qp = ref->qp;
+    d.bp = ref->bp;
+    d.pp = vc1me::phase_pitch(ref->w);
+    d.ph = vc1me::phase_rows(ref->h);
+}
+
+// Phase planes of `ref` that a job of mode `mode` reads: allocated here on
+// first use, filled by launch_phases in the step itself.
+hipError_t ensure_phase_alloc(const vc1me_frame* ref, int mode) {
+    const bool bil = mode == 2;
+    uint8_t*& buf = bil ? ref->bp : ref->qp;
+    if (buf) return hipSuccess;
+    const size_t psz = static_cast<size_t>(vc1me::phase_pitch(ref->w)) * vc1me::phase_rows(ref->h);
+    return hipMalloc(&buf, psz * (bil ? 4 : 16));
+}
+
+hipError_t launch_phases(const vc1me_frame* ref, int mode, hipStream_t st) {
+    const bool bil = mode == 2;
+    return vc1me::launch_phase_planes(ref->y, ref->w, ref->h, bil ? ref->bp : ref->qp, vc1me::phase_pitch(ref->w),
+                                      vc1me::phase_rows(ref->h), bil ? 1 : 0, st);
 }
 
 int mbs_on_diag(int mbw, int mbh, int t) {
@@ -435,6 +460,8 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
         const size_t cells = static_cast<size_t>((p->width + 7) / 8) * ((p->height + 7) / 8);
         hipError_t e = ensure_slot(ctx->slots[i], mbs, cells);
         if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("device state allocation: ") + hipGetErrorString(e));
+        e = ensure_phase_alloc(jobs[i].ref, p->mode);
+        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("phase plane allocation: ") + hipGetErrorString(e));
         fill_job(ctx->h_jobs[i], p, jobs[i].cur, jobs[i].ref, ctx->slots[i]);
         max_local = std::max(max_local, ctx->h_jobs[i].local_range);
         max_diag = std::max(max_diag, ctx->h_jobs[i].max_diag);
@@ -454,6 +481,16 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
         HIP_TRY(hipMemsetAsync(ctx->slots[i].ops, 0, 8 * sizeof(unsigned long long), st));
     }
     HIP_TRY(hipEventRecord(ctx->events[0], st));
+    // interpolated references, once per distinct (frame, filter) of the step
+    {
+        std::vector<std::pair<const vc1me_frame*, bool>> done;
+        for (int i = 0; i < n; ++i) {
+            const std::pair<const vc1me_frame*, bool> key(jobs[i].ref, jobs[i].params->mode == 2);
+            if (std::find(done.begin(), done.end(), key) != done.end()) continue;
+            done.push_back(key);
+            HIP_TRY(launch_phases(jobs[i].ref, jobs[i].params->mode, st));
+        }
+    }
     HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st, ctx->events[2], ctx->events[3]));
     HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
     const int lds = vc1me::mb_kernel_lds_bytes(max_local);
@@ -640,6 +677,8 @@ void vc1me_frame_release(vc1me_ctx* ctx, vc1me_frame* frame) {
         (void)hipStreamSynchronize(ctx->stream);
     }
     (void)hipFree(frame->y);
+    if (frame->qp) (void)hipFree(frame->qp);
+    if (frame->bp) (void)hipFree(frame->bp);
     delete frame;
 }
 
@@ -680,6 +719,9 @@ int vc1me_block_costs(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame*
     const size_t cells = static_cast<size_t>((p->width + 7) / 8) * ((p->height + 7) / 8);
     hipError_t e = ensure_slot(ctx->slots[0], mbs, cells);
     if (e != hipSuccess) return fail(VC1ME_ENOMEM, "device state allocation");
+    e = ensure_phase_alloc(ref, mode);
+    if (e != hipSuccess) return fail(VC1ME_ENOMEM, "phase plane allocation");
+    HIP_TRY(launch_phases(ref, mode, ctx->stream));
     JobDesc d;
     fill_job(d, p, cur, ref, ctx->slots[0]);
     if (!ctx->d_jobs) {

@@ -55,7 +55,18 @@ struct JobDesc {
     // satd_ops, sig_ops, frame_px), in the units of the CPU oracle
     unsigned long long* ops;
     int32_t max_diag;           // last anti-diagonal index mbw + 2*mbh - 3
+    // interpolated reference: 16 bicubic (qp) or 4 bilinear (bp) phase planes
+    // of (W+3) x (H+3) samples at pitch pp, row count ph = H + 3
+    const uint8_t* qp;
+    const uint8_t* bp;
+    int32_t pp, ph;
 };
+
+// Phase-plane geometry of a W x H reference: sample (px, py), px in [-2, W],
+// py in [-2, H], lives at row py + 2, column px + 2.  The pitch leaves room
+// for the 8-byte reads of the block scorer.
+inline int32_t phase_pitch(int w) { return (w + 3 + 8 + 15) & ~15; }
+inline int32_t phase_rows(int h) { return h + 3; }
 
 // Flag bits of JobDesc::flags
 constexpr uint8_t kFlag4mvChosen = 1;
@@ -68,6 +79,8 @@ hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hi
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);
+hipError_t launch_phase_planes(const uint8_t* ref, int width, int height, uint8_t* out, int pitch, int rows,
+                               int bilinear, hipStream_t s);
 hipError_t launch_block_costs(const JobDesc* d_job, int x0, int y0, int bw, int bh, int mode, int metric,
                               const int2* cands, int n, uint64_t* out, int lds_bytes, hipStream_t s);
 hipError_t launch_score_batch(const uint8_t* cur, const uint8_t* ref, int width, int height, int x0, int y0,

@@ -687,132 +687,54 @@ __device__ __forceinline__ int mspel(int a, int b, int c, int d, int mode) {
     return -3 * a + 18 * b + 53 * c - 4 * d;
 }
 
-// Window covering the interpolation footprint of block (bx0,by0,bw,bh) for all
-// vectors in [xq_lo,xq_hi]x[yq_lo,yq_hi] (taps -1..+2 around each integer
-// position, plus 8 bytes of row slack for the 4-pixel reads).
-__device__ __forceinline__ bool qpel_window(Ctx& C, int bx0, int by0, int bw, int bh, int xq_lo, int xq_hi, int yq_lo, int yq_hi) {
-    const int ix_lo = floor_div4(xq_lo), ix_hi = floor_div4(xq_hi);
-    const int iy_lo = floor_div4(yq_lo), iy_hi = floor_div4(yq_hi);
-    const int nx0 = bx0 + ix_lo - 1, ny0 = by0 + iy_lo - 1;
-    const int nx1 = bx0 + ix_hi + bw + 6, ny1 = by0 + iy_hi + bh + 2;  // exclusive
-    if (nx0 >= C.wx0 && nx1 <= C.wx0 + C.ww && ny0 >= C.wy0 && ny1 <= C.wy0 + C.wh) return true;
-    return win_load(C, nx0, ny0, nx1 - nx0, ny1 - ny0);
-}
+// ------------------------------------------------------------------ interpolated reference
 
-// A filtered sample saturated to [0, 255], ready to be packed with others.
-// The empty asm makes the clamped value opaque: without it the compiler fuses
-// "shift right, saturate to u8, pack two samples" into v_ashr_pk_u8_i32 and
-// then ORs the next samples into bits 16-31 assuming that instruction zeroed
-// them; on gfx950 the upper half keeps the destination register's old bits,
-// which turned bytes 2-3 of a packed prediction into 0xff whenever the
-// reused register held a negative pre-shift value.
-__device__ __forceinline__ uint32_t sat_u8(int v) {
-    uint32_t r = static_cast<uint32_t>(clampi(v, 0, 255));
-    asm("" : "+v"(r));
-    return r;
-}
-
-// Four horizontally adjacent predicted samples (luma_mc_sample /
-// luma_mc_sample_mode, src/encoder_motion_compensation.cpp:367-415) packed
-// little-endian; (sx, sy) is the window position of the first sample's
-// integer source pixel.
-__device__ __forceinline__ uint32_t pred4(const uint8_t* w, int wp, int sx, int sy, int hm, int vm, bool bil) {
-    // 8 window bytes of row yy starting at column sx-1
-    auto row8 = [&](int yy, uint32_t& lo, uint32_t& hi) {
-        const int a = sx - 1;
-        const uint32_t* p = reinterpret_cast<const uint32_t*>(w + yy * wp + (a & ~3));
-        const uint32_t d0 = p[0], d1 = p[1], d2 = p[2];
-        lo = __builtin_amdgcn_alignbyte(d1, d0, a & 3);
-        hi = __builtin_amdgcn_alignbyte(d2, d1, a & 3);
-    };
-    auto byte_at = [](uint32_t lo, uint32_t hi, int k) -> int {
-        return static_cast<int>(k < 4 ? (lo >> (8 * k)) & 255u : (hi >> (8 * (k - 4))) & 255u);
-    };
-    if (!hm && !vm) {
-        uint32_t lo, hi;
-        row8(sy, lo, hi);
-        return __builtin_amdgcn_alignbyte(hi, lo, 1);
-    }
-    uint32_t out = 0;
+// One interpolated sample (luma_mc_sample / luma_mc_sample_mode,
+// src/encoder_motion_compensation.cpp:367-415) at quarter-pel phase (hm, vm)
+// from its clamped 4x4 integer neighbourhood n[r][c] = ref(py - 1 + r, px - 1 + c).
+__device__ __forceinline__ int mc_sample(const int n[4][4], int hm, int vm, bool bil) {
     if (bil) {
-        uint32_t l0, h0, l1 = 0, h1 = 0;
-        row8(sy, l0, h0);
-        if (vm) row8(sy + 1, l1, h1);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int a = byte_at(l0, h0, j + 1), b = byte_at(l0, h0, j + 2);
-            int v;
-            if (hm && vm) v = (a + b + byte_at(l1, h1, j + 1) + byte_at(l1, h1, j + 2) + 2) >> 2;
-            else if (hm) v = (a + b + 1) >> 1;
-            else v = (a + byte_at(l1, h1, j + 1) + 1) >> 1;
-            out |= static_cast<uint32_t>(v) << (8 * j);
-        }
-        return out;
+        if (hm && vm) return (n[1][1] + n[1][2] + n[2][1] + n[2][2] + 2) >> 2;
+        if (hm) return (n[1][1] + n[1][2] + 1) >> 1;
+        if (vm) return (n[1][1] + n[2][1] + 1) >> 1;
+        return n[1][1];
     }
+    if (!hm && !vm) return n[1][1];
     if (!vm) {
-        uint32_t lo, hi;
-        row8(sy, lo, hi);
         const int bias = hm == 2 ? 8 : 32, sh = hm == 2 ? 4 : 6;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int v = arshift(mspel(byte_at(lo, hi, j), byte_at(lo, hi, j + 1), byte_at(lo, hi, j + 2),
-                                        byte_at(lo, hi, j + 3), hm) + bias, sh);
-            out |= sat_u8(v) << (8 * j);
-        }
-        return out;
+        return clampi(arshift(mspel(n[1][0], n[1][1], n[1][2], n[1][3], hm) + bias, sh), 0, 255);
     }
-    uint32_t L[4], Hh[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) row8(sy - 1 + k, L[k], Hh[k]);
     if (!hm) {
         const int bias = vm == 2 ? 7 : 31, sh = vm == 2 ? 4 : 6;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int v = arshift(mspel(byte_at(L[0], Hh[0], j + 1), byte_at(L[1], Hh[1], j + 1),
-                                        byte_at(L[2], Hh[2], j + 1), byte_at(L[3], Hh[3], j + 1), vm) + bias, sh);
-            out |= sat_u8(v) << (8 * j);
-        }
-        return out;
+        return clampi(arshift(mspel(n[0][1], n[1][1], n[2][1], n[3][1], vm) + bias, sh), 0, 255);
     }
     const int shift = ((hm == 2 ? 1 : 5) + (vm == 2 ? 1 : 5)) >> 1;
     const int r1 = (1 << (shift - 1)) - 1;
-    int t[7];
+    int t[4];
 #pragma unroll
-    for (int c = 0; c < 7; ++c)
-        t[c] = arshift(mspel(byte_at(L[0], Hh[0], c), byte_at(L[1], Hh[1], c), byte_at(L[2], Hh[2], c),
-                             byte_at(L[3], Hh[3], c), vm) + r1, shift);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int v = arshift(mspel(t[j], t[j + 1], t[j + 2], t[j + 3], hm) + 64, 7);
-        out |= sat_u8(v) << (8 * j);
+    for (int c = 0; c < 4; ++c) t[c] = arshift(mspel(n[0][c], n[1][c], n[2][c], n[3][c], vm) + r1, shift);
+    return clampi(arshift(mspel(t[0], t[1], t[2], t[3], hm) + 64, 7), 0, 255);
+}
+
+// A block's candidate costs read the interpolated reference from phase
+// planes: plane (hm, vm) holds the sample of that phase at every integer
+// position px in [-2, W], py in [-2, H].  Every filter tap of a position
+// outside that box reads the clamped frame edge, so the sample at any
+// position is the plane sample at the clamped position.
+
+// Four predicted samples of plane row `row` starting at plane coordinate x0
+// (columns x0 .. x0+3, clamped into [-2, W]), packed little-endian.
+__device__ __forceinline__ uint32_t plane_row4(gptr_u8 row, int x0, int W) {
+    if (x0 >= -2 && x0 + 3 <= W) {
+        const int a = x0 + 2;
+        typedef const __attribute__((address_space(1))) uint32_t* gptr_u32;
+        const gptr_u32 q = reinterpret_cast<gptr_u32>(row + (a & ~3));
+        return __builtin_amdgcn_alignbyte(q[1], q[0], a & 3);
     }
-    return out;
-}
-
-// Lane layout of a block: lane -> (4x4 tile, tile row); a 16x16 block uses all
-// 64 lanes, an 8x8 block 16 lanes (four blocks or candidates per wave).
-struct LanePix {
-    int x, y;   // first of the lane's four samples, block coordinates
-    int tx, ty; // its tile origin
-    int r;      // row inside the tile
-};
-__device__ __forceinline__ LanePix lane_pix(int lane, int bw) {
-    LanePix p;
-    const int l = bw == 16 ? lane : (lane & 15);
-    const int tile = l >> 2;
-    p.r = l & 3;
-    const int tpr = bw / 4;
-    p.tx = (tile % tpr) * 4;
-    p.ty = (tile / tpr) * 4;
-    p.x = p.tx;
-    p.y = p.ty + p.r;
-    return p;
-}
-
-// Sum over the lanes that hold one candidate (64 for 16x16, 16 for 8x8).
-__device__ __forceinline__ uint32_t block_sum(uint32_t v, int bw) {
-    if (bw == 16) return wave_sum32(v);
-    return group_sum32(v, 16);
+    uint32_t r = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) r |= static_cast<uint32_t>(row[clampi(x0 + j, -2, W) + 2]) << (8 * j);
+    return r;
 }
 
 // Costs of the candidates vs[0..n) on block (bx0,by0,bw,bh): metric 0 = SAD
@@ -820,34 +742,37 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, int bw) {
 // (src/encoder_motion.cpp:2286-2293, :2670-2676); metric 1 = SATD of
 // block_satd_motion (src/encoder_motion.cpp:2962-3042: rendered prediction,
 // (|H D H^T| + 1) >> 1 per 4x4 tile, zero residual outside the active area).
+// Each lane owns one 4x4 tile of one candidate: a 16x16 block takes 16 lanes
+// (four candidates per pass), an 8x8 block 4 lanes (sixteen per pass).
 // Results in out[0..n); every lane can read them on return.
-__device__ __forceinline__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* vs, int n, int mode, int metric,
-                          uint64_t* out) {
+__device__ __forceinline__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* vs, int n, int mode,
+                                          int metric, uint64_t* out) {
     if (n <= 0) return;
+    const JobDesc* J = C.J;
     const bool bil = mode_bilinear(mode);
-    const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
+    const int W = C.g.W, H = C.g.H;
+    const int aw = imax(0, imin(bw, W - bx0)), ah = imax(0, imin(bh, H - by0));
     const bool chroma = chroma_block(bx0, by0, bw, bh);
-    int xlo = 1 << 30, xhi = -(1 << 30), ylo = 1 << 30, yhi = -(1 << 30);
-    for (int i = 0; i < n; ++i) {
-        const int2 v = vs[i];
-        xlo = imin(xlo, v.x); xhi = imax(xhi, v.x); ylo = imin(ylo, v.y); yhi = imax(yhi, v.y);
-    }
-    const bool shared = qpel_window(C, bx0, by0, bw, bh, xlo, xhi, ylo, yhi);
-    const LanePix lp = lane_pix(C.lane, bw);
-    const int per = shared ? (bw == 16 ? 1 : 4) : 1;
-    const int g = bw == 16 ? 0 : (C.lane >> 4);
-    const uint32_t cur4 = *reinterpret_cast<const uint32_t*>(C.L.cur + (by0 - C.y0 + lp.y) * 16 + (bx0 - C.x0 + lp.x));
-    uint32_t amask = 0;
+    const int lshift = bw == 16 ? 4 : 2;             // log2(lanes per candidate)
+    const int g = C.lane >> lshift, t = C.lane & ((1 << lshift) - 1);
+    const int tx = (bw == 16 ? (t & 3) : (t & 1)) * 4, ty = (bw == 16 ? (t >> 2) : (t >> 1)) * 4;
+    const int cpp = kWave >> lshift;                 // candidates per pass
+    uint32_t cur[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        cur[r] = *reinterpret_cast<const uint32_t*>(C.L.cur + (by0 - C.y0 + ty + r) * 16 + (bx0 - C.x0 + tx));
+    uint32_t cm = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-        if (lp.x + j < aw && lp.y < ah) amask |= 0xffu << (8 * j);
-    const bool tvalid = lp.tx < aw && lp.ty < ah;
-    for (int base = 0; base < n; base += per) {
+        if (tx + j < aw) cm |= 0xffu << (8 * j);
+    const bool tvalid = tx < aw && ty < ah;
+    const gptr_u8 planes = as_global(bil ? J->bp : J->qp);
+    const size_t psz = static_cast<size_t>(J->pp) * J->ph;
+    for (int base = 0; base < n; base += cpp) {
         const int c = base + g;
-        const bool mine = c < n && g < per;
+        const bool mine = c < n;
         int2 v = make_int2(0, 0);
         if (mine) v = vs[c];
-        // uniform per candidate: veto and window
         bool skip = !mine;
         uint64_t skip_cost = kBadSad;
         if (mine && metric == 0) {
@@ -858,53 +783,55 @@ __device__ __forceinline__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int 
             if (!aw || !ah) { skip = true; skip_cost = 0; }
             else if (mode_halfpel(mode) && ((v.x & 1) || (v.y & 1))) skip = true;
         }
-        if (!shared) {
-            // one candidate at a time, in its own window (the vector is uniform here)
-            const int2 u = vs[base];
-            if (!qpel_window(C, bx0, by0, bw, bh, u.x, u.x, u.y, u.y)) { set_err(C, 3); skip = true; }
-        }
         const int ix = floor_div4(v.x), iy = floor_div4(v.y);
         const int hm = v.x - ix * 4, vm = v.y - iy * 4;
         uint32_t part = 0;
         if (!skip) {
-            const int sx = bx0 + lp.x + ix - C.wx0, sy = by0 + lp.y + iy - C.wy0;
-            const uint32_t p4 = pred4(C.L.win, C.wp, sx, sy, hm, vm, bil);
+            const int phase = bil ? (vm >> 1) * 2 + (hm >> 1) : vm * 4 + hm;
+            const gptr_u8 pl = planes + static_cast<size_t>(phase) * psz;
+            const int x0 = bx0 + tx + ix;
+            uint32_t pr[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int yc = clampi(by0 + ty + r + iy, -2, H) + 2;
+                pr[r] = plane_row4(pl + static_cast<size_t>(yc) * J->pp, x0, W);
+            }
             if (metric == 0) {
-                part = __builtin_amdgcn_sad_u8(p4 & amask, cur4 & amask, 0u);
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (ty + r < ah) part = __builtin_amdgcn_sad_u8(pr[r] & cm, cur[r] & cm, part);
             } else {
-                int d[4];
+                int h[4][4];
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    d[j] = ((amask >> (8 * j)) & 255u)
-                               ? static_cast<int>((cur4 >> (8 * j)) & 255u) - static_cast<int>((p4 >> (8 * j)) & 255u)
-                               : 0;
-                const int a0 = d[0] + d[3], a1 = d[1] + d[2], a2 = d[1] - d[2], a3 = d[0] - d[3];
-                const int t[4] = {a0 + a1, a3 + a2, a0 - a1, a3 - a2};
-                const int s1 = (lp.r == 0 || lp.r == 1) ? 1 : -1;
-                const int s2 = (lp.r == 0 || lp.r == 3) ? 1 : -1;
-                const int s3 = (lp.r == 0 || lp.r == 2) ? 1 : -1;
-                int acc = 0;
+                for (int r = 0; r < 4; ++r) {
+                    int d[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int R0 = quad_bcast<0>(t[k]), R1 = quad_bcast<1>(t[k]);
-                    const int R2 = quad_bcast<2>(t[k]), R3 = quad_bcast<3>(t[k]);
-                    acc += iabs(R0 + s1 * R1 + s2 * R2 + s3 * R3);
+                    for (int j = 0; j < 4; ++j)
+                        d[j] = (ty + r < ah && tx + j < aw)
+                                   ? static_cast<int>((cur[r] >> (8 * j)) & 255u) - static_cast<int>((pr[r] >> (8 * j)) & 255u)
+                                   : 0;
+                    const int a0 = d[0] + d[3], a1 = d[1] + d[2], a2 = d[1] - d[2], a3 = d[0] - d[3];
+                    h[r][0] = a0 + a1; h[r][1] = a3 + a2; h[r][2] = a0 - a1; h[r][3] = a3 - a2;
                 }
-                uint32_t ts = static_cast<uint32_t>(acc);
-                ts += dpp_mov<kDppXor1, 0xf>(0, ts);
-                ts += dpp_mov<kDppXor2, 0xf>(0, ts);
-                part = (tvalid && lp.r == 0) ? ((ts + 1) >> 1) : 0u;
+                uint32_t sum = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int b0 = h[0][j] + h[3][j], b1 = h[1][j] + h[2][j];
+                    const int b2 = h[1][j] - h[2][j], b3 = h[0][j] - h[3][j];
+                    sum += static_cast<uint32_t>(iabs(b0 + b1) + iabs(b3 + b2) + iabs(b0 - b1) + iabs(b3 - b2));
+                }
+                part = tvalid ? ((sum + 1) >> 1) : 0u;
             }
         }
-        const uint32_t total = block_sum(part, bw);
+        const uint32_t total = group_sum32(part, 1 << lshift);
         // op counts: the lead lane of each scored candidate counts it
-        if (!skip && (C.lane & (bw == 16 ? 63 : 15)) == 0) {
+        if (!skip && t == 0) {
             const uint32_t px = static_cast<uint32_t>(aw * ah);
             if (metric == 0) C.ln_sad += px;
             C.ln_taps += px * static_cast<uint32_t>(taps_per_sample(hm, vm, bil));
             if (metric == 1) C.ln_satd += 112u * static_cast<uint32_t>(((aw + 3) / 4) * ((ah + 3) / 4));
         }
-        if (mine && (C.lane & (bw == 16 ? 63 : 15)) == 0) out[c] = skip ? skip_cost : static_cast<uint64_t>(total);
+        if (mine && t == 0) out[c] = skip ? skip_cost : static_cast<uint64_t>(total);
     }
     wsync();
 }
@@ -2430,6 +2357,40 @@ hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc*
 hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hipStream_t s) {
     if (max_cells <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_signatures, dim3((max_cells + 255) / 256, njobs), dim3(256), 0, s, d_jobs, max_cells);
+    return hipGetLastError();
+}
+
+// Phase planes of one reference (see qpel_eval): one thread per plane
+// position, every phase of the filter set.
+__global__ void __launch_bounds__(256) k_phase_planes(const uint8_t* __restrict__ ref, int W, int H,
+                                                      uint8_t* __restrict__ out, int pp, int ph, int bil) {
+    const int px = static_cast<int>(blockIdx.x) * 64 + static_cast<int>(threadIdx.x & 63) - 2;
+    const int py = static_cast<int>(blockIdx.y) * 4 + static_cast<int>(threadIdx.x >> 6) - 2;
+    if (px > W || py > H) return;
+    const gptr_u8 r = as_global(ref);
+    int n[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const size_t row = static_cast<size_t>(clampi(py - 1 + a, 0, H - 1)) * W;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) n[a][b] = r[row + clampi(px - 1 + b, 0, W - 1)];
+    }
+    const size_t psz = static_cast<size_t>(pp) * ph;
+    uint8_t* o = out + static_cast<size_t>(py + 2) * pp + (px + 2);
+    if (bil) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k * psz] = static_cast<uint8_t>(mc_sample(n, (k & 1) * 2, (k >> 1) * 2, true));
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) o[k * psz] = static_cast<uint8_t>(mc_sample(n, k & 3, k >> 2, false));
+    }
+}
+
+hipError_t launch_phase_planes(const uint8_t* ref, int width, int height, uint8_t* out, int pitch, int rows,
+                               int bilinear, hipStream_t s) {
+    if (rows < height + 3 || pitch < width + 3 + 8) return hipErrorInvalidValue;
+    const dim3 grid((width + 3 + 63) / 64, (height + 3 + 3) / 4);
+    hipLaunchKernelGGL(k_phase_planes, grid, dim3(256), 0, s, ref, width, height, out, pitch, rows, bilinear);
     return hipGetLastError();
 }
 
