@@ -102,6 +102,23 @@ __device__ __forceinline__ int lane_prefix(uint64_t mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0));
 }
 
+// Values of another lane: readlane (uniform lane index) and shuffles.
+__device__ __forceinline__ uint64_t lane_read64(uint64_t v, int l) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), l));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ unsigned __int128 lane_read128(unsigned __int128 v, int l) {
+    const uint64_t lo = lane_read64(static_cast<uint64_t>(v), l);
+    const uint64_t hi = lane_read64(static_cast<uint64_t>(v >> 64), l);
+    return (static_cast<unsigned __int128>(hi) << 64) | lo;
+}
+__device__ __forceinline__ uint64_t lane_shfl64(uint64_t v, int src) {
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(v), src));
+    const uint32_t hi = static_cast<uint32_t>(__shfl(static_cast<int>(v >> 32), src));
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Global-memory view of a plane pointer (lets loads be scheduled past LDS stores).
 typedef const __attribute__((address_space(1))) uint8_t* gptr_u8;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -737,17 +754,20 @@ __device__ __forceinline__ uint32_t plane_row4(gptr_u8 row, int x0, int W) {
     return r;
 }
 
-// Costs of the candidates vs[0..n) on block (bx0,by0,bw,bh): metric 0 = SAD
-// with the hidden-padding veto of block_sad_qpel / block_sad_motion
-// (src/encoder_motion.cpp:2286-2293, :2670-2676); metric 1 = SATD of
-// block_satd_motion (src/encoder_motion.cpp:2962-3042: rendered prediction,
-// (|H D H^T| + 1) >> 1 per 4x4 tile, zero residual outside the active area).
-// Each lane owns one 4x4 tile of one candidate: a 16x16 block takes 16 lanes
-// (four candidates per pass), an 8x8 block 4 lanes (sixteen per pass).
-// Results in out[0..n); every lane can read them on return.
-__device__ __forceinline__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, const int2* vs, int n, int mode,
-                                          int metric, uint64_t* out) {
-    if (n <= 0) return;
+// Costs of the candidate vectors (vx, vy) held by lanes first .. first+cnt-1
+// on block (bx0,by0,bw,bh); `want` (per lane) selects the candidates to score,
+// the others cost kBadSad.  metric 0 = SAD with the hidden-padding veto of
+// block_sad_qpel / block_sad_motion (src/encoder_motion.cpp:2286-2293,
+// :2670-2676); metric 1 = SATD of block_satd_motion
+// (src/encoder_motion.cpp:2962-3042: rendered prediction, (|H D H^T| + 1) >> 1
+// per 4x4 tile, zero residual outside the active area).  Each lane owns one
+// 4x4 tile of one candidate: a 16x16 block takes 16 lanes (four candidates per
+// pass), an 8x8 block 4 lanes (sixteen per pass).  Returns the cost of its own
+// candidate in each lane first+c, `keep` in the other lanes.
+__device__ __forceinline__ uint64_t qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, int vx, int vy, bool want,
+                                              int first, int cnt, int mode, int metric, uint64_t keep) {
+    uint64_t res = keep;
+    if (cnt <= 0) return res;
     const JobDesc* J = C.J;
     const bool bil = mode_bilinear(mode);
     const int W = C.g.W, H = C.g.H;
@@ -768,18 +788,20 @@ __device__ __forceinline__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int 
     const bool tvalid = tx < aw && ty < ah;
     const gptr_u8 planes = as_global(bil ? J->bp : J->qp);
     const size_t psz = static_cast<size_t>(J->pp) * J->ph;
-    for (int base = 0; base < n; base += cpp) {
+    const int wanted = want ? 1 : 0;
+    for (int base = 0; base < cnt; base += cpp) {
         const int c = base + g;
-        const bool mine = c < n;
-        int2 v = make_int2(0, 0);
-        if (mine) v = vs[c];
-        bool skip = !mine;
+        const bool mine = c < cnt;
+        const int src = first + (mine ? c : 0);
+        const int2 v = make_int2(__shfl(vx, src), __shfl(vy, src));
+        const bool w = __shfl(wanted, src) != 0;
+        bool skip = !mine || !w;
         uint64_t skip_cost = kBadSad;
-        if (mine && metric == 0) {
+        if (mine && w && metric == 0) {
             if (hidden_padding(C.g, bx0, by0, bw, bh, v.x, v.y, bil ? mode : 0, chroma)) skip = true;
             else if (bil && ((v.x & 1) || (v.y & 1))) skip = true;
         }
-        if (mine && metric == 1) {
+        if (mine && w && metric == 1) {
             if (!aw || !ah) { skip = true; skip_cost = 0; }
             else if (mode_halfpel(mode) && ((v.x & 1) || (v.y & 1))) skip = true;
         }
@@ -831,9 +853,14 @@ __device__ __forceinline__ void qpel_eval(Ctx& C, int bx0, int by0, int bw, int 
             C.ln_taps += px * static_cast<uint32_t>(taps_per_sample(hm, vm, bil));
             if (metric == 1) C.ln_satd += 112u * static_cast<uint32_t>(((aw + 3) / 4) * ((ah + 3) / 4));
         }
-        if (mine && t == 0) out[c] = skip ? skip_cost : static_cast<uint64_t>(total);
+        // hand each cost to the lane that holds its vector
+        const uint64_t val = skip ? skip_cost : static_cast<uint64_t>(total);
+        const int rel = C.lane - first - base;
+        const bool here = rel >= 0 && rel < cpp && base + rel < cnt;
+        const uint64_t got = lane_shfl64(val, (here ? rel : 0) << lshift);
+        if (here) res = got;
     }
-    wsync();
+    return res;
 }
 
 // Op counts of one more SAD evaluation of v on the block, as qpel_eval counts
@@ -1277,65 +1304,46 @@ __device__ __forceinline__ uint64_t key_q(uint64_t cost, int xq, int yq) {
 __device__ __forceinline__ int keyq_x(uint64_t k) { return static_cast<int>(k & 8191) - 4096; }
 __device__ __forceinline__ int keyq_y(uint64_t k) { return static_cast<int>((k >> 13) & 2047) - 1024; }
 
-// add(mv) of refine_motion_block_staged (src/encoder_motion.cpp:3330-3343) for
-// the m vectors in L.req: vectors not yet in the list are scored together and
-// appended.  L.slot[i] receives the list index of batch entry i.  Returns the
-// new list length.
-__device__ __forceinline__ int sub_add(Ctx& C, const SubParams& sp, int n, int m) {
-    const int i = C.lane;
-    int found = -1;
-    int2 v = make_int2(0, 0);
-    bool isnew = false;
-    if (i < m) {
-        v = C.L.req[i];
-        for (int k = 0; k < n; ++k)
-            if (C.L.sub[k].xq == v.x && C.L.sub[k].yq == v.y) { found = k; break; }
-        if (found < 0) {
-            bool dup = false;
-            for (int j = 0; j < i; ++j) {
-                const int2 u = C.L.req[j];
-                if (u.x == v.x && u.y == v.y) { dup = true; break; }
-            }
-            isnew = !dup;
+// The candidate list of refine_motion_block_staged, one entry per lane:
+// lane k < n holds entry k (vector and SAD).  n is uniform.
+struct SubList {
+    int n;
+    int xq, yq;
+    uint64_t sad;
+};
+
+// add(mv) (src/encoder_motion.cpp:3330-3343) for the m vectors (rx, ry) held
+// by lanes 0..m-1: a vector already listed is found, a new one is appended
+// in request order, and the new entries are scored together.  Returns the
+// mask of list slots the batch refers to.
+__device__ __forceinline__ uint64_t sub_add(Ctx& C, const SubParams& sp, SubList& L, int rx, int ry, int m) {
+    uint64_t slots = 0;
+    const int n0 = L.n;
+    int n = n0;
+    for (int j = 0; j < m; ++j) {
+        const int vx = __builtin_amdgcn_readlane(rx, j), vy = __builtin_amdgcn_readlane(ry, j);
+        const uint64_t hit = __ballot(C.lane < n && L.xq == vx && L.yq == vy);
+        if (hit) {
+            slots |= 1ull << (__ffsll(static_cast<long long>(hit)) - 1);
+            continue;
         }
+        if (C.lane == n) { L.xq = vx; L.yq = vy; }
+        slots |= 1ull << n;
+        ++n;
     }
-    const uint64_t nm = __ballot(isnew);
-    const int cnt = __popcll(nm);
-    const int pos = lane_prefix(nm);
-    if (isnew) C.L.pend[pos] = v;
-    wsync();
-    qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, C.L.pend, cnt, sp.mode, 0, C.L.pendcost);
-    if (isnew) {
-        SubCand& s = C.L.sub[n + pos];
-        s.xq = v.x; s.yq = v.y; s.sad = C.L.pendcost[pos];
-        s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
-        found = n + pos;
-    }
-    wsync();
-    if (i < m && found < 0)
-        for (int k = 0; k < n + cnt; ++k)
-            if (C.L.sub[k].xq == v.x && C.L.sub[k].yq == v.y) { found = k; break; }
-    if (i < m) C.L.slot[i] = found;
-    wsync();
-    return n + cnt;
+    L.sad = qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, L.xq, L.yq, true, n0, n - n0, sp.mode, 0, L.sad);
+    L.n = n;
+    return slots;
 }
 
-// minimum (SAD, tie order) key over the m entries of the last batch
-__device__ __forceinline__ uint64_t batch_min_key(Ctx& C, int m) {
-    uint64_t k = kNone;
-    if (C.lane < m) {
-        const SubCand& s = C.L.sub[C.L.slot[C.lane]];
-        k = key_q(s.sad, s.xq, s.yq);
-    }
-    return wave_min64(k);
-}
-
-__device__ __forceinline__ void set_req(Ctx& C, int i, int x, int y) {
-    if (C.lane == i) C.L.req[i] = make_int2(x, y);
+// minimum (SAD, tie order) key over the list slots of a batch
+__device__ __forceinline__ uint64_t batch_min_key(const Ctx& C, const SubList& L, uint64_t slots) {
+    return wave_min64(((slots >> C.lane) & 1) ? key_q(L.sad, L.xq, L.yq) : kNone);
 }
 
 // refine_motion_integer_only, src/encoder_motion.cpp:3247-3269
-__device__ __forceinline__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int syq, uint64_t* best_sad, uint64_t known) {
+__device__ __forceinline__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int syq, uint64_t* best_sad,
+                                                    uint64_t known) {
     const Bounds qb = bounds_qpel(sp.range);
     auto snap = [](int q, int lo, int hi) {
         q = clampi(q, lo, hi);
@@ -1347,19 +1355,13 @@ __device__ __forceinline__ int2 refine_integer_only(Ctx& C, const SubParams& sp,
         ix = 0; iy = 0;
     }
     if (ix == sxq && iy == syq && known != UINT64_MAX) *best_sad = known;
-    else {
-        wsync();
-        if (C.lane == 0) C.L.pend[0] = make_int2(ix, iy);
-        wsync();
-        qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, C.L.pend, 1, sp.mode, 0, C.L.pendcost);
-        *best_sad = C.L.pendcost[0];
-    }
+    else *best_sad = lane_read64(qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, ix, iy, true, 0, 1, sp.mode, 0, 0), 0);
     return make_int2(ix, iy);
 }
 
 // refine_motion_block_staged, src/encoder_motion.cpp:3271-3594 (quality RATE / SATD)
-__device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred& pi, uint64_t* best_sad,
-                              uint64_t known_seed) {
+__device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred& pi,
+                                              uint64_t* best_sad, uint64_t known_seed) {
     const JobDesc* J = C.J;
     const int limit_q = 4 * J->subpel_range;
     if (!J->subpel || limit_q == 0 || imax(iabs(sxq), iabs(syq)) > limit_q)
@@ -1369,80 +1371,53 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
     const int ymin = imax(qb.ymin, -limit_q), ymax = imin(qb.ymax, limit_q);
     const bool hp = mode_halfpel(sp.mode);
     const bool chroma = chroma_block(sp.bx0, sp.by0, sp.bw, sp.bh);
+    const int lane = C.lane;
     sxq = clampi(sxq, xmin, xmax);
     syq = clampi(syq, ymin, ymax);
     if (hp) { sxq = make_even(sxq); syq = make_even(syq); }
-    int n = 0;
+    SubList L;
+    L.n = 0; L.xq = 0; L.yq = 0; L.sad = kNone;
     const bool hid = hidden_padding(C.g, sp.bx0, sp.by0, sp.bw, sp.bh, sxq, syq, sp.mode, chroma);
-    wsync();
     if (hid || known_seed != UINT64_MAX) {
-        if (C.lane == 0) {
-            SubCand& s = C.L.sub[0];
-            s.xq = sxq; s.yq = syq; s.sad = hid ? kBadSad : known_seed; s.satd = 0; s.rate = 0; s.eligible = 1; s.score = 0;
-        }
-        wsync();
-        n = 1;
+        if (lane == 0) { L.xq = sxq; L.yq = syq; L.sad = hid ? kBadSad : known_seed; }
+        L.n = 1;
     } else {
-        set_req(C, 0, sxq, syq);
-        wsync();
-        n = sub_add(C, sp, 0, 1);
+        sub_add(C, sp, L, sxq, syq, 1);
     }
-    if (hid) {
-        set_req(C, 0, 0, 0);
-        wsync();
-        n = sub_add(C, sp, n, 1);
-    }
-    uint64_t half = key_q(C.L.sub[0].sad, sxq, syq);
+    if (hid) sub_add(C, sp, L, 0, 0, 1);
+    uint64_t half = key_q(lane_read64(L.sad, 0), sxq, syq);
     const bool fast = J->fast != 0;
     // half-pel stencil around the seed
     {
-        int nb;
+        int rx = sxq, ry = syq, nb;
         if (fast) {
             nb = 5;
-            if (C.lane < 5) {
-                int x = sxq, y = syq;
-                if (C.lane == 1) x = sxq - 2; else if (C.lane == 2) x = sxq + 2;
-                else if (C.lane == 3) y = syq - 2; else if (C.lane == 4) y = syq + 2;
-                x = clampi(x, xmin, xmax); y = clampi(y, ymin, ymax);
-                if (hp) { x = make_even(x); y = make_even(y); }
-                C.L.req[C.lane] = make_int2(x, y);
-            }
+            if (lane == 1) rx = sxq - 2; else if (lane == 2) rx = sxq + 2;
+            else if (lane == 3) ry = syq - 2; else if (lane == 4) ry = syq + 2;
+            rx = clampi(rx, xmin, xmax); ry = clampi(ry, ymin, ymax);
         } else {
             nb = 9;
-            if (C.lane < 9) {
-                int x = clampi(sxq + 2 * (C.lane % 3) - 2, xmin, xmax), y = clampi(syq + 2 * (C.lane / 3) - 2, ymin, ymax);
-                if (hp) { x = make_even(x); y = make_even(y); }
-                C.L.req[C.lane] = make_int2(x, y);
-            }
+            rx = clampi(sxq + 2 * (lane % 3) - 2, xmin, xmax);
+            ry = clampi(syq + 2 * ((lane / 3) % 3) - 2, ymin, ymax);
         }
-        wsync();
-        n = sub_add(C, sp, n, nb);
-        const uint64_t m = batch_min_key(C, nb);
+        if (hp) { rx = make_even(rx); ry = make_even(ry); }
+        const uint64_t m = batch_min_key(C, L, sub_add(C, sp, L, rx, ry, nb));
         half = m < half ? m : half;
     }
     int hbx = keyq_x(half), hby = keyq_y(half);
     if (fast && !(hbx == sxq && hby == syq)) {
         const int dx = hbx - sxq, dy = hby - syq;
-        int nd = 0;
+        int nd = 0, rx = 0, ry = 0;
         if (dx && !dy) {
             nd = 2;
-            if (C.lane < 2) {
-                int x = hbx, y = clampi(syq + (C.lane ? 2 : -2), ymin, ymax);
-                if (hp) { x = make_even(x); y = make_even(y); }
-                C.L.req[C.lane] = make_int2(x, y);
-            }
+            rx = hbx; ry = clampi(syq + ((lane & 1) ? 2 : -2), ymin, ymax);
         } else if (dy && !dx) {
             nd = 2;
-            if (C.lane < 2) {
-                int x = clampi(sxq + (C.lane ? 2 : -2), xmin, xmax), y = hby;
-                if (hp) { x = make_even(x); y = make_even(y); }
-                C.L.req[C.lane] = make_int2(x, y);
-            }
+            rx = clampi(sxq + ((lane & 1) ? 2 : -2), xmin, xmax); ry = hby;
         }
         if (nd) {
-            wsync();
-            n = sub_add(C, sp, n, nd);
-            const uint64_t m = batch_min_key(C, nd);
+            if (hp) { rx = make_even(rx); ry = make_even(ry); }
+            const uint64_t m = batch_min_key(C, L, sub_add(C, sp, L, rx, ry, nd));
             half = m < half ? m : half;
             hbx = keyq_x(half); hby = keyq_y(half);
         }
@@ -1450,145 +1425,114 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
     if (!hp) {
         if (fast) {
             const int qcx = hbx, qcy = hby;
-            if (C.lane < 5) {
-                int x = qcx, y = qcy;
-                if (C.lane == 1) x = qcx - 1; else if (C.lane == 2) x = qcx + 1;
-                else if (C.lane == 3) y = qcy - 1; else if (C.lane == 4) y = qcy + 1;
-                C.L.req[C.lane] = make_int2(clampi(x, xmin, xmax), clampi(y, ymin, ymax));
-            }
-            wsync();
-            n = sub_add(C, sp, n, 5);
+            int rx = qcx, ry = qcy;
+            if (lane == 1) rx = qcx - 1; else if (lane == 2) rx = qcx + 1;
+            else if (lane == 3) ry = qcy - 1; else if (lane == 4) ry = qcy + 1;
+            rx = clampi(rx, xmin, xmax); ry = clampi(ry, ymin, ymax);
+            const uint64_t sl = sub_add(C, sp, L, rx, ry, 5);
             // the quarter centre's cost, then the best of the five
             uint64_t qk = kNone;
-            for (int k = 0; k < n; ++k)
-                if (C.L.sub[k].xq == qcx && C.L.sub[k].yq == qcy) qk = key_q(C.L.sub[k].sad, qcx, qcy);
-            const uint64_t m = batch_min_key(C, 5);
+            const uint64_t at = __ballot(lane < L.n && L.xq == qcx && L.yq == qcy);
+            if (at) qk = key_q(lane_read64(L.sad, __ffsll(static_cast<long long>(at)) - 1), qcx, qcy);
+            const uint64_t m = batch_min_key(C, L, sl);
             qk = m < qk ? m : qk;
             const int qbx = keyq_x(qk), qby = keyq_y(qk);
             const int dx = qbx - qcx, dy = qby - qcy;
             int n2 = 0;
             if (dx && !dy) {
                 n2 = 2;
-                if (C.lane < 2) C.L.req[C.lane] = make_int2(qbx, clampi(qcy + (C.lane ? 1 : -1), ymin, ymax));
+                rx = qbx; ry = clampi(qcy + ((lane & 1) ? 1 : -1), ymin, ymax);
             } else if (dy && !dx) {
                 n2 = 2;
-                if (C.lane < 2) C.L.req[C.lane] = make_int2(clampi(qcx + (C.lane ? 1 : -1), xmin, xmax), qby);
+                rx = clampi(qcx + ((lane & 1) ? 1 : -1), xmin, xmax); ry = qby;
             }
-            if (n2) {
-                wsync();
-                n = sub_add(C, sp, n, n2);
-            }
+            if (n2) sub_add(C, sp, L, rx, ry, n2);
         } else {
-            if (C.lane < 9)
-                C.L.req[C.lane] = make_int2(clampi(hbx + (C.lane % 3) - 1, xmin, xmax),
-                                            clampi(hby + (C.lane / 3) - 1, ymin, ymax));
-            wsync();
-            n = sub_add(C, sp, n, 9);
+            const int rx = clampi(hbx + (lane % 3) - 1, xmin, xmax), ry = clampi(hby + ((lane / 3) % 3) - 1, ymin, ymax);
+            sub_add(C, sp, L, rx, ry, 9);
         }
     }
-    // sort by (SAD, tie order) and keep the shortlist
+    // sort by (SAD, tie order) and keep the shortlist: the entry of rank r
+    // moves to lane r
     const int keep = fast ? 4 : 8;
+    int n = L.n;
+    int xq, yq;
+    uint64_t sad;
     {
-        SubCand mine;
-        uint64_t myk = kNone;
-        if (C.lane < n) {
-            mine = C.L.sub[C.lane];
-            myk = key_q(mine.sad, mine.xq, mine.yq);
-        }
+        const uint64_t myk = lane < n ? key_q(L.sad, L.xq, L.yq) : kNone;
         int rank = 0;
-        for (int k = 0; k < n; ++k) {
-            const SubCand& o = C.L.sub[k];
-            rank += key_q(o.sad, o.xq, o.yq) < myk ? 1 : 0;
+        for (int k = 0; k < n; ++k) rank += lane_read64(myk, k) < myk ? 1 : 0;
+        if (n > keep) n = keep;
+        int src = 0;
+        for (int r = 0; r < n; ++r) {
+            const uint64_t at = __ballot(lane < L.n && rank == r);
+            if (lane == r) src = __ffsll(static_cast<long long>(at)) - 1;
         }
-        if (C.lane < n && rank < keep) C.L.sub2[rank] = mine;
-        wsync();
+        xq = __shfl(L.xq, src);
+        yq = __shfl(L.yq, src);
+        sad = lane_shfl64(L.sad, src);
     }
-    const uint64_t screen = C.L.sub2[0].sad;
-    if (n > keep) n = keep;
+    const uint64_t screen = lane_read64(sad, 0);
     // rates and x87 scores, one candidate per lane
     const uint64_t lm = J->lam_m;
     const int lk = J->lam_k;
-    if (C.lane < n) {
-        SubCand& s = C.L.sub2[C.lane];
+    const bool in = lane < n;
+    uint32_t rate = 0;
+    unsigned __int128 score = 0;
+    if (in) {
         int prx = pi.px, pry = pi.py;
-        uint32_t rate = 0;
         if (pi.hybrid) {
-            const bool use_a = modular_mv_distance(J->mvrange, s.xq, s.yq, pi.ax, pi.ay) <=
-                               modular_mv_distance(J->mvrange, s.xq, s.yq, pi.cx, pi.cy);
+            const bool use_a = modular_mv_distance(J->mvrange, xq, yq, pi.ax, pi.ay) <=
+                               modular_mv_distance(J->mvrange, xq, yq, pi.cx, pi.cy);
             prx = use_a ? pi.ax : pi.cx;
             pry = use_a ? pi.ay : pi.cy;
             ++rate;
         }
-        rate += mv_rate_bits(J->mvrange, s.xq, s.yq, prx, pry, sp.mode, sp.more);
-        s.rate = rate;
-        s.score = x87_score(s.sad, rate, lm, lk);
+        rate += mv_rate_bits(J->mvrange, xq, yq, prx, pry, sp.mode, sp.more);
+        score = x87_score(sad, rate, lm, lk);
     }
-    wsync();
     // the SAD+lambda*R winner: the candidate no other one precedes
     int rw;
     {
-        bool first = false;
-        if (C.lane < n) {
-            const SubCand& a = C.L.sub2[C.lane];
-            int before = 0;
-            for (int k = 0; k < n; ++k) {
-                const SubCand& b = C.L.sub2[k];
-                bool less;
-                if (b.score != a.score) less = b.score < a.score;
-                else if (b.sad != a.sad) less = b.sad < a.sad;
-                else less = tie_less(b.xq, b.yq, a.xq, a.yq);
-                before += less ? 1 : 0;
-            }
-            first = before == 0;
+        int before = 0;
+        for (int k = 0; k < n; ++k) {
+            const unsigned __int128 bs = lane_read128(score, k);
+            const uint64_t bsad = lane_read64(sad, k);
+            const int bx = __builtin_amdgcn_readlane(xq, k), by = __builtin_amdgcn_readlane(yq, k);
+            bool less;
+            if (bs != score) less = bs < score;
+            else if (bsad != sad) less = bsad < sad;
+            else less = tie_less(bx, by, xq, yq);
+            before += less ? 1 : 0;
         }
-        rw = __ffsll(__ballot(first)) - 1;
+        rw = __ffsll(static_cast<long long>(__ballot(in && before == 0))) - 1;
     }
-    int2 result = make_int2(C.L.sub2[rw].xq, C.L.sub2[rw].yq);
+    int2 result = make_int2(__builtin_amdgcn_readlane(xq, rw), __builtin_amdgcn_readlane(yq, rw));
     if (J->quality == 2) {
-        const uint64_t rws = C.L.sub2[rw].sad;
+        const uint64_t rws = lane_read64(sad, rw);
         const uint64_t pct3 = (rws / 100) * 3 + (((rws % 100) * 3 + 99) / 100);
         const uint64_t slack = pct3 > 4 ? pct3 : 4;
         const uint64_t limit = rws > UINT64_MAX - slack ? UINT64_MAX : rws + slack;
-        bool el = false;
-        int2 mv = make_int2(0, 0);
-        if (C.lane < n) {
-            const SubCand& s = C.L.sub2[C.lane];
-            el = s.sad < UINT64_MAX / 8 && s.sad <= limit;
-            mv = make_int2(s.xq, s.yq);
+        const bool el = in && sad < UINT64_MAX / 8 && sad <= limit;
+        const uint64_t satd = qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, xq, yq, el, 0, n, sp.mode, 1, kBadSad);
+        const unsigned __int128 s2 = x87_score(satd, rate, lm, lk);
+        int before = 0;
+        for (int k = 0; k < n; ++k) {
+            const bool bel = ((__ballot(el) >> k) & 1) != 0;
+            const unsigned __int128 bs = lane_read128(s2, k);
+            const uint64_t bsatd = lane_read64(satd, k), bsad = lane_read64(sad, k);
+            const int bx = __builtin_amdgcn_readlane(xq, k), by = __builtin_amdgcn_readlane(yq, k);
+            bool less;
+            if (bel != el) less = bel;
+            else if (bs != s2) less = bs < s2;
+            else if (bsatd != satd) less = bsatd < satd;
+            else if (bsad != sad) less = bsad < sad;
+            else less = tie_less(bx, by, xq, yq);
+            before += less ? 1 : 0;
         }
-        const uint64_t em = __ballot(el);
-        const int cnt = __popcll(em);
-        const int pos = lane_prefix(em);
-        if (el) C.L.pend[pos] = mv;
-        wsync();
-        qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, C.L.pend, cnt, sp.mode, 1, C.L.pendcost);
-        if (C.lane < n) {
-            SubCand& s = C.L.sub2[C.lane];
-            s.eligible = el ? 1u : 0u;
-            s.satd = el ? C.L.pendcost[pos] : kBadSad;
-            s.score = x87_score(s.satd, s.rate, lm, lk);
-        }
-        wsync();
-        bool first = false;
-        if (C.lane < n) {
-            const SubCand& a = C.L.sub2[C.lane];
-            int before = 0;
-            for (int k = 0; k < n; ++k) {
-                const SubCand& b = C.L.sub2[k];
-                bool less;
-                if (b.eligible != a.eligible) less = b.eligible > a.eligible;
-                else if (b.score != a.score) less = b.score < a.score;
-                else if (b.satd != a.satd) less = b.satd < a.satd;
-                else if (b.sad != a.sad) less = b.sad < a.sad;
-                else less = tie_less(b.xq, b.yq, a.xq, a.yq);
-                before += less ? 1 : 0;
-            }
-            first = before == 0;
-        }
-        const int w = __ffsll(__ballot(first)) - 1;
-        result = make_int2(C.L.sub2[w].xq, C.L.sub2[w].yq);
+        const int w = __ffsll(static_cast<long long>(__ballot(in && before == 0))) - 1;
+        result = make_int2(__builtin_amdgcn_readlane(xq, w), __builtin_amdgcn_readlane(yq, w));
     }
-    wsync();
     *best_sad = screen;
     return result;
 }
@@ -1696,11 +1640,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
     }
     if (hidden_padding(C.g, C.x0, C.y0, 16, 16, one.x, one.y, one_mode, true)) {
         one = make_int2(0, 0);
-        wsync();
-        if (C.lane == 0) C.L.pend[0] = one;
-        wsync();
-        qpel_eval(C, C.x0, C.y0, 16, 16, C.L.pend, 1, one_mode, 0, C.L.pendcost);
-        one_sad = C.L.pendcost[0];
+        one_sad = lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, 0, 0, true, 0, 1, one_mode, 0, 0), 0);
     }
     const uint64_t one_screen = one_sad;
     int2 outmv = one;
@@ -1724,72 +1664,62 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
                 if (imax(iabs(m.x), iabs(m.y)) > limit) { m.x = (m.x / 4) * 4; m.y = (m.y / 4) * 4; }
                 return m;
             };
-            // the eleven seed SADs, scored together (unique vectors only), then
-            // replayed in the reference's order
-            if (C.lane < 11) {
-                int2 cv = one;
-                if (C.lane > 0) {
-                    const int s = C.lane - 1;
-                    int x = one.x, y = one.y;
-                    switch (s) {
-                        case 1: x = pi4.px; y = pi4.py; break;
-                        case 2: x -= 8; break;
-                        case 3: x += 8; break;
-                        case 4: y -= 8; break;
-                        case 5: y += 8; break;
-                        case 6: x -= 4; break;
-                        case 7: x += 4; break;
-                        case 8: y -= 4; break;
-                        case 9: y += 4; break;
-                        default: break;
-                    }
-                    cv = make_int2(clampi(x, qb.xmin, qb.xmax), clampi(y, qb.ymin, qb.ymax));
-                }
-                C.L.req[C.lane] = within(cv);
-            }
-            wsync();
+            // the eleven seed vectors, one per lane: their SADs are scored
+            // together (unique vectors only), then replayed in the
+            // reference's order
+            int sx = one.x, sy = one.y;
             {
-                int2 v = make_int2(0, 0);
-                bool isnew = false;
-                if (C.lane < 11) {
-                    v = C.L.req[C.lane];
-                    bool dup = false;
-                    for (int j = 0; j < C.lane; ++j) {
-                        const int2 u = C.L.req[j];
-                        if (u.x == v.x && u.y == v.y) { dup = true; break; }
-                    }
-                    isnew = !dup;
+                const int s = C.lane - 1;
+                int x = one.x, y = one.y;
+                switch (s) {
+                    case 1: x = pi4.px; y = pi4.py; break;
+                    case 2: x -= 8; break;
+                    case 3: x += 8; break;
+                    case 4: y -= 8; break;
+                    case 5: y += 8; break;
+                    case 6: x -= 4; break;
+                    case 7: x += 4; break;
+                    case 8: y -= 4; break;
+                    case 9: y += 4; break;
+                    default: break;
                 }
-                const uint64_t nm = __ballot(isnew);
-                const int cnt = __popcll(nm);
-                if (isnew) C.L.pend[lane_prefix(nm)] = v;
-                wsync();
-                qpel_eval(C, bx, by, 8, 8, C.L.pend, cnt, 0, 0, C.L.pendcost);
-                if (C.lane < 11) {
-                    uint64_t cost = kBadSad;
-                    for (int j = 0; j < cnt; ++j) {
-                        const int2 u = C.L.pend[j];
-                        if (u.x == v.x && u.y == v.y) { cost = C.L.pendcost[j]; break; }
-                    }
-                    C.L.top16[C.lane] = cost;
-                }
-                wsync();
+                if (C.lane > 0) { sx = clampi(x, qb.xmin, qb.xmax); sy = clampi(y, qb.ymin, qb.ymax); }
+                const int2 w = within(make_int2(sx, sy));
+                sx = w.x; sy = w.y;
             }
-            int2 best_seed = C.L.req[0];
-            uint64_t seed_cost = C.L.top16[0];
+            uint64_t scost;
+            uint32_t repeat = 0;        // bit s: seed s repeats an earlier seed
+            {
+                SubList U;
+                U.n = 0; U.xq = 0; U.yq = 0; U.sad = kNone;
+                int slot = 0;
+                for (int j = 0; j < 11; ++j) {
+                    const int vx = __builtin_amdgcn_readlane(sx, j), vy = __builtin_amdgcn_readlane(sy, j);
+                    const uint64_t hit = __ballot(C.lane < U.n && U.xq == vx && U.yq == vy);
+                    int at = U.n;
+                    if (hit) {
+                        at = __ffsll(static_cast<long long>(hit)) - 1;
+                        repeat |= 1u << j;
+                    } else {
+                        if (C.lane == U.n) { U.xq = vx; U.yq = vy; }
+                        ++U.n;
+                    }
+                    if (C.lane == j) slot = at;
+                }
+                U.sad = qpel_eval(C, bx, by, 8, 8, U.xq, U.yq, true, 0, U.n, 0, 0, kNone);
+                scost = lane_shfl64(U.sad, slot);
+            }
+            int2 best_seed = make_int2(__builtin_amdgcn_readlane(sx, 0), __builtin_amdgcn_readlane(sy, 0));
+            uint64_t seed_cost = lane_read64(scost, 0);
             for (int s = 1; s < 11; ++s) {
-                const int2 sd = C.L.req[s];
+                const int2 sd = make_int2(__builtin_amdgcn_readlane(sx, s), __builtin_amdgcn_readlane(sy, s));
                 if (sd.x == best_seed.x && sd.y == best_seed.y) continue;
                 // the reference scores this seed again if it repeats an earlier one
-                for (int j = 0; j < s; ++j) {
-                    const int2 u = C.L.req[j];
-                    if (u.x == sd.x && u.y == sd.y) { count_sad_eval(C, bx, by, 8, 8, sd, 0); break; }
-                }
-                const uint64_t cc = C.L.top16[s];
+                if ((repeat >> s) & 1) count_sad_eval(C, bx, by, 8, 8, sd, 0);
+                const uint64_t cc = lane_read64(scost, s);
                 const int cm = iabs(sd.x) + iabs(sd.y), bm = iabs(best_seed.x) + iabs(best_seed.y);
                 if (cc < seed_cost || (cc == seed_cost && cm < bm)) { seed_cost = cc; best_seed = sd; }
             }
-            wsync();
             uint64_t bsad = 0;
             SubParams sp{bx, by, 8, 8, 3, range, true};
             const int2 mv = refine_staged(C, sp, best_seed.x, best_seed.y, pi4, &bsad, seed_cost);
@@ -1921,13 +1851,12 @@ __global__ void __launch_bounds__(64) k_block_costs(const JobDesc* __restrict__ 
         C.L.cur[i] = (x0 + x < J->W && y0 + y < J->H) ? J->cy[static_cast<size_t>(y0 + y) * J->W + x0 + x] : 0;
     }
     wsync();
-    for (int base = 0; base < n; base += kMaxSub) {
-        const int m = imin(kMaxSub, n - base);
-        if (C.lane < m) C.L.pend[C.lane] = cands[base + C.lane];
-        wsync();
-        qpel_eval(C, x0, y0, bw, bh, C.L.pend, m, mode, metric, C.L.pendcost);
-        if (C.lane < m) out[base + C.lane] = C.L.pendcost[C.lane];
-        wsync();
+    for (int base = 0; base < n; base += kWave) {
+        const int m = imin(kWave, n - base);
+        int2 v = make_int2(0, 0);
+        if (C.lane < m) v = cands[base + C.lane];
+        const uint64_t cost = qpel_eval(C, x0, y0, bw, bh, v.x, v.y, true, 0, m, mode, metric, 0);
+        if (C.lane < m) out[base + C.lane] = cost;
     }
 }
 
