@@ -164,11 +164,25 @@ __device__ __forceinline__ bool tie_less(int ax, int ay, int bx, int by) {
 __device__ __forceinline__ bool mode_halfpel(int m) { return m == 1 || m == 2; }
 __device__ __forceinline__ bool mode_bilinear(int m) { return m == 2; }
 
-// MVDATA code lengths of MV table 0 (ST 421; src/vc1_groupa_tables.h:26-27 row 0)
-__constant__ uint8_t kMvBits0[73] = {
+// MVDATA code lengths of MV table 0 (ST 421; src/vc1_groupa_tables.h:26-27 row 0),
+// held as sixteen 4-bit entries per 64-bit word so that a lookup is register
+// arithmetic.
+constexpr uint8_t kMvBits0[73] = {
     6, 7, 7, 8, 14, 6, 5, 6, 7, 14, 14, 6, 6, 6, 8, 9, 10, 9, 7, 7, 7, 14, 14, 10, 9,
     10, 8, 10, 10, 14, 14, 14, 13, 13, 13, 6, 3, 5, 6, 8, 9, 13, 5, 4, 4, 5, 7, 9, 6, 5,
     5, 5, 6, 9, 8, 5, 5, 5, 7, 10, 7, 7, 7, 7, 7, 8, 10, 9, 8, 9, 9, 9, 3};
+constexpr uint64_t mv_bits_word(int w) {
+    uint64_t r = 0;
+    for (int i = 0; i < 16; ++i)
+        if (w * 16 + i < 73) r |= static_cast<uint64_t>(kMvBits0[w * 16 + i]) << (4 * i);
+    return r;
+}
+__device__ __forceinline__ uint32_t mv_bits0(int sym) {
+    constexpr uint64_t w0 = mv_bits_word(0), w1 = mv_bits_word(1), w2 = mv_bits_word(2), w3 = mv_bits_word(3),
+                       w4 = mv_bits_word(4);
+    const uint64_t w = sym < 32 ? (sym < 16 ? w0 : w1) : (sym < 48 ? w2 : (sym < 64 ? w3 : w4));
+    return static_cast<uint32_t>((w >> (4 * (sym & 15))) & 15u);
+}
 
 // ------------------------------------------------------------------ hidden padding
 // motion_uses_hidden_luma / _chroma (src/encoder_motion.cpp:2143-2189)
@@ -230,19 +244,18 @@ __device__ __forceinline__ unsigned __int128 x87_score(uint64_t cost, uint64_t r
 
 // ------------------------------------------------------------------ LDS layout
 
-struct SubCand {
-    int32_t xq, yq;
-    uint64_t sad;
-    uint64_t satd;
-    uint32_t rate;
-    uint32_t eligible;
-    unsigned __int128 score;
-};
-
 constexpr int kSeenWords = ((2 * kMaxLocalRange + 1) * (2 * kMaxLocalRange + 1) + 31) / 32;
-constexpr int kMaxSub = 32;
 constexpr int kMaxUmhKeys = 768;     // sampled-SAD points an integer search can consider
-constexpr int kSmallWin = 80 * 96;   // far windows and sub-pel windows
+constexpr int kSmallWin = 80 * 96;   // windows of the far (long-range) candidates
+
+// Phase window: the interpolated reference (every phase) around a macroblock's
+// integer vector, staged in LDS for the sub-pel stages.  kPwSide samples
+// square per phase; rows hold kPwPitch bytes so that 16-byte aligned copies
+// of the phase planes fit whatever the alignment of the window origin.
+constexpr int kPwMargin = 6;
+constexpr int kPwSide = 16 + 2 * kPwMargin;
+constexpr int kPwPitch = 48;
+constexpr int kPwBytes = (16 * kPwSide * kPwPitch + 1023) & ~1023;  // whole 1 KB copy blocks
 
 __host__ __device__ inline int win_capacity(int local_range) {
     const int side = 16 + 2 * local_range;
@@ -254,52 +267,43 @@ __host__ __device__ inline int win_capacity(int local_range) {
 struct Lds {
     uint8_t* cur;        // 16x16 current block, zero outside the active area
     uint8_t* win;        // reference window (16-byte aligned rows)
+    uint8_t* pw;         // phase window
     uint32_t* seen_s;    // sampled-SAD dedupe bitset of the UMH search
     uint32_t* seen_f;    // full-SAD dedupe bitset
     uint64_t* ukeys;     // keys of every sampled point the UMH search considered
-    SubCand* sub;        // staged sub-pel candidate list
-    SubCand* sub2;       // its sorted copy
-    int2* req;           // a batch of candidate vectors
-    int2* pend;          // vectors waiting to be scored
-    uint64_t* pendcost;  // their costs
-    int32_t* slot;       // per batch entry: index into the list
     uint64_t* top16;     // long-range top-16 keys
     int2* own;           // current MB's four block vectors
 };
 
-__host__ __device__ inline int lds_bytes_for(int local_range) {
-    int off = 0;
-    off += 256;                                     // cur
-    off += (win_capacity(local_range) + 15) & ~15;  // win
-    off += kSeenWords * 4 * 2;                      // seen_s, seen_f
+// The integer searches' window, dedupe bitsets and key log are dead once the
+// sub-pel stages start, so the phase window shares their bytes.
+__host__ __device__ inline int int_scratch_bytes(int local_range) {
+    int off = (win_capacity(local_range) + 15) & ~15;  // win
+    off += kSeenWords * 4 * 2;                          // seen_s, seen_f
     off = (off + 15) & ~15;
-    off += kMaxUmhKeys * 8;                         // ukeys
-    off += 2 * kMaxSub * static_cast<int>(sizeof(SubCand));
-    off += kMaxSub * 8 * 2;                         // req, pend
-    off += kMaxSub * 8;                             // pendcost
-    off += kMaxSub * 4;                             // slot
-    off += 16 * 8;                                  // top16
-    off += 4 * 8;                                   // own
-    return off + 64;
+    off += kMaxUmhKeys * 8;                             // ukeys
+    return off;
+}
+
+__host__ __device__ inline int lds_bytes_for(int local_range) {
+    const int shared = int_scratch_bytes(local_range) > kPwBytes ? int_scratch_bytes(local_range) : kPwBytes;
+    return 256 + shared + 16 * 8 + 4 * 8 + 64;          // cur, shared region, top16, own
 }
 
 __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     Lds L;
+    L.cur = smem;
+    uint8_t* const region = smem + 256;
     int off = 0;
-    L.cur = smem + off; off += 256;
-    L.win = smem + off; off += (win_capacity(local_range) + 15) & ~15;
-    L.seen_s = reinterpret_cast<uint32_t*>(smem + off); off += kSeenWords * 4;
-    L.seen_f = reinterpret_cast<uint32_t*>(smem + off); off += kSeenWords * 4;
+    L.win = region + off; off += (win_capacity(local_range) + 15) & ~15;
+    L.seen_s = reinterpret_cast<uint32_t*>(region + off); off += kSeenWords * 4;
+    L.seen_f = reinterpret_cast<uint32_t*>(region + off); off += kSeenWords * 4;
     off = (off + 15) & ~15;
-    L.ukeys = reinterpret_cast<uint64_t*>(smem + off); off += kMaxUmhKeys * 8;
-    L.sub = reinterpret_cast<SubCand*>(smem + off); off += kMaxSub * static_cast<int>(sizeof(SubCand));
-    L.sub2 = reinterpret_cast<SubCand*>(smem + off); off += kMaxSub * static_cast<int>(sizeof(SubCand));
-    L.req = reinterpret_cast<int2*>(smem + off); off += kMaxSub * 8;
-    L.pend = reinterpret_cast<int2*>(smem + off); off += kMaxSub * 8;
-    L.pendcost = reinterpret_cast<uint64_t*>(smem + off); off += kMaxSub * 8;
-    L.slot = reinterpret_cast<int32_t*>(smem + off); off += kMaxSub * 4;
-    L.top16 = reinterpret_cast<uint64_t*>(smem + off); off += 16 * 8;
-    L.own = reinterpret_cast<int2*>(smem + off); off += 4 * 8;
+    L.ukeys = reinterpret_cast<uint64_t*>(region + off);
+    L.pw = region;
+    const int shared = int_scratch_bytes(local_range) > kPwBytes ? int_scratch_bytes(local_range) : kPwBytes;
+    L.top16 = reinterpret_cast<uint64_t*>(region + shared);
+    L.own = reinterpret_cast<int2*>(region + shared + 16 * 8);
     return L;
 }
 
@@ -314,6 +318,10 @@ struct Ctx {
     int winCap;
     // current window
     int wx0, wy0, ww, wh, wp;
+    // phase window: plane coordinates of its sample (0,0), the byte offset of
+    // that sample in each LDS row, and which filter's phases it holds
+    int pwx, pwy, pwsh;
+    bool pw_ok, pw_bil;
     // algorithmic operation counts of this macroblock: uniform totals, plus
     // per-lane partial counts of the lane-parallel block scoring (summed
     // over the wave when the macroblock is done)
@@ -675,7 +683,7 @@ __device__ __forceinline__ uint32_t mvdiff_bits(int mvrange, int dx, int dy, boo
     } else {
         suffix = mv_component_bits(xc, hp) + mv_component_bits(yc, hp);
     }
-    return static_cast<uint32_t>(kMvBits0[sym]) + static_cast<uint32_t>(suffix);
+    return mv_bits0(sym) + static_cast<uint32_t>(suffix);
 }
 __device__ __forceinline__ uint32_t mv_rate_bits(int mvrange, int xq, int yq, int pxq, int pyq, int mode, bool more) {
     const bool hp = mode_halfpel(mode);
@@ -754,6 +762,63 @@ __device__ __forceinline__ uint32_t plane_row4(gptr_u8 row, int x0, int W) {
     return r;
 }
 
+// Stage the phase window with sample (0,0) at plane coordinates (wx, wy):
+// kPwSide x kPwSide samples of every phase of the filter set, copied from the
+// phase planes as 16-byte aligned chunks (a row of the window starts pwsh
+// bytes into its LDS row).
+__device__ __forceinline__ void pw_load(Ctx& C, int wx, int wy, bool bil) {
+    const JobDesc* J = C.J;
+    const int np = bil ? 4 : 16;
+    const gptr_u8 planes = as_global(bil ? J->bp : J->qp);
+    const size_t psz = static_cast<size_t>(J->pp) * J->ph;
+    const int W = C.g.W, H = C.g.H;
+    const int c0 = wx + 2;
+    const int a0 = c0 & ~15;
+    constexpr int kChunks = kPwPitch / 16;
+    const int total = np * kPwSide * kChunks;
+    wsync();
+    if (a0 >= 0 && a0 + kPwPitch <= W + 3) {
+        // every chunk lies inside the plane columns: copy global -> LDS
+        // directly, 64 chunks (1 KB, chunk i at byte 16 i) per instruction
+        typedef const __attribute__((address_space(1))) void* gvoid;
+        typedef __attribute__((address_space(3))) void* lvoid;
+        for (int base = 0; base < total; base += kWave) {
+            const int i = imin(base + C.lane, total - 1);
+            const int ph = i / (kPwSide * kChunks), rem = i - ph * (kPwSide * kChunks);
+            const int r = rem / kChunks, ch = rem - r * kChunks;
+            const int prow = clampi(wy + r, -2, H) + 2;
+            const gptr_u8 src = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp + a0 + ch * 16;
+            __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(C.L.pw + base * 16), 16, 0, 0);
+        }
+    } else
+    for (int i = C.lane; i < total; i += kWave) {
+        const int ph = i / (kPwSide * kChunks), rem = i - ph * (kPwSide * kChunks);
+        const int r = rem / kChunks, ch = rem - r * kChunks;
+        const int prow = clampi(wy + r, -2, H) + 2;
+        const int col = a0 + ch * 16;
+        const gptr_u8 row = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp;
+        uint4 v;
+        if (col >= 0 && col + 16 <= W + 3) {
+            const u32x4 t = *reinterpret_cast<const __attribute__((address_space(1))) u32x4*>(row + col);
+            v = make_uint4(t.x, t.y, t.z, t.w);
+        } else {
+            uint32_t wv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc |= static_cast<uint32_t>(row[clampi(col + q * 4 + b, 0, W + 2)]) << (8 * b);
+                wv[q] = acc;
+            }
+            v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+        }
+        *reinterpret_cast<uint4*>(C.L.pw + (ph * kPwSide + r) * kPwPitch + ch * 16) = v;
+    }
+    C.pwx = wx; C.pwy = wy; C.pwsh = c0 - a0; C.pw_ok = true; C.pw_bil = bil;
+    C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;   // the integer window is overwritten
+    wsync();
+}
+
 // Costs of the candidate vectors (vx, vy) held by lanes first .. first+cnt-1
 // on block (bx0,by0,bw,bh); `want` (per lane) selects the candidates to score,
 // the others cost kBadSad.  metric 0 = SAD with the hidden-padding veto of
@@ -810,13 +875,25 @@ __device__ __forceinline__ uint64_t qpel_eval(Ctx& C, int bx0, int by0, int bw, 
         uint32_t part = 0;
         if (!skip) {
             const int phase = bil ? (vm >> 1) * 2 + (hm >> 1) : vm * 4 + hm;
-            const gptr_u8 pl = planes + static_cast<size_t>(phase) * psz;
             const int x0 = bx0 + tx + ix;
             uint32_t pr[4];
+            const bool in_pw = C.pw_ok && C.pw_bil == bil && bx0 + ix >= C.pwx && bx0 + ix + bw <= C.pwx + kPwSide &&
+                               by0 + iy >= C.pwy && by0 + iy + bh <= C.pwy + kPwSide;
+            if (in_pw) {
+                const int a = x0 - C.pwx + C.pwsh;
+                const uint8_t* base_row = C.L.pw + (phase * kPwSide + (by0 + ty + iy - C.pwy)) * kPwPitch + (a & ~3);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int yc = clampi(by0 + ty + r + iy, -2, H) + 2;
-                pr[r] = plane_row4(pl + static_cast<size_t>(yc) * J->pp, x0, W);
+                for (int r = 0; r < 4; ++r) {
+                    const uint32_t* q = reinterpret_cast<const uint32_t*>(base_row + r * kPwPitch);
+                    pr[r] = __builtin_amdgcn_alignbyte(q[1], q[0], a & 3);
+                }
+            } else {
+                const gptr_u8 pl = planes + static_cast<size_t>(phase) * psz;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int yc = clampi(by0 + ty + r + iy, -2, H) + 2;
+                    pr[r] = plane_row4(pl + static_cast<size_t>(yc) * J->pp, x0, W);
+                }
             }
             if (metric == 0) {
 #pragma unroll
@@ -1635,6 +1712,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
             if (ext.dx != im.dx || ext.dy != im.dy) { im = ext; selected_distant = true; }
         }
         if (C.lane == 0) J->prop[pos] = selected_distant ? 1 : 0;
+        if (J->subpel) pw_load(C, C.x0 + im.dx - kPwMargin, C.y0 + im.dy - kPwMargin, mode_bilinear(one_mode));
         SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true};
         one = refine_staged(C, sp, im.dx * 4, im.dy * 4, rate_pi, &one_sad, im.sad);
     }
@@ -1820,6 +1898,7 @@ __device__ __forceinline__ void run_mb(const JobDesc* J, int mx, int my, uint8_t
     C.L = carve(smem, J->local_range);
     C.winCap = win_capacity(J->local_range);
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
+    C.pw_ok = false; C.pw_bil = false; C.pwx = C.pwy = C.pwsh = 0;
     C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
     C.ln_sad = C.ln_taps = C.ln_satd = 0;
     analyze_mb(C);
@@ -1844,6 +1923,7 @@ __global__ void __launch_bounds__(64) k_block_costs(const JobDesc* __restrict__ 
     C.L = carve(smem, J->local_range);
     C.winCap = win_capacity(J->local_range);
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
+    C.pw_ok = false; C.pw_bil = false; C.pwx = C.pwy = C.pwsh = 0;
     C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
     C.ln_sad = C.ln_taps = C.ln_satd = 0;
     for (int i = C.lane; i < 256; i += kWave) {
