@@ -783,12 +783,13 @@ __device__ __forceinline__ void pw_load(Ctx& C, int wx, int wy, bool bil) {
         typedef const __attribute__((address_space(1))) void* gvoid;
         typedef __attribute__((address_space(3))) void* lvoid;
         for (int base = 0; base < total; base += kWave) {
-            const int i = imin(base + C.lane, total - 1);
+            const int i = base + C.lane;
             const int ph = i / (kPwSide * kChunks), rem = i - ph * (kPwSide * kChunks);
             const int r = rem / kChunks, ch = rem - r * kChunks;
             const int prow = clampi(wy + r, -2, H) + 2;
             const gptr_u8 src = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp + a0 + ch * 16;
-            __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(C.L.pw + base * 16), 16, 0, 0);
+            // lanes past the window issue no copy (a copy lands at byte 16 * lane)
+            if (i < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(C.L.pw + base * 16), 16, 0, 0);
         }
     } else
     for (int i = C.lane; i < total; i += kWave) {
@@ -1283,7 +1284,102 @@ __device__ __forceinline__ uint32_t sig_distance(uint64_t a, uint64_t b) {
     return d;
 }
 
-// extended_content_motion, src/encoder_motion.cpp:1696-1779
+// Windows of the extended search: 36 x 36 frame samples around a candidate
+// (its 5x5 grid at step 4 and the 5x5 refinement around the grid's best stay
+// within +-10 px), rows of 64 bytes holding the 16-byte aligned chunks that
+// cover the window plus the 4-dword read slack of the row SAD.
+constexpr int kXwSide = 36;
+constexpr int kXwPitch = 64;
+constexpr int kXwBytes = kXwSide * kXwPitch;
+constexpr int kXwPerRound = 8;
+
+// Stage the windows of candidates t0 .. t0+cnt-1 (offsets in `cand` lanes
+// t0..) into slots 0..cnt-1 of the shared LDS region.
+__device__ __forceinline__ void xw_load(Ctx& C, uint64_t ckeys, int t0, int cnt) {
+    const JobDesc* J = C.J;
+    const int W = C.g.W, H = C.g.H;
+    uint8_t* const xw = C.L.pw;
+    constexpr int kChunks = kXwPitch / 16;
+    wsync();
+    for (int k = 0; k < cnt; ++k) {
+        const uint64_t ck = lane_read64(ckeys, t0 + k);
+        const int wx = C.x0 + key_dx(ck) - 10, wy = C.y0 + key_dy(ck) - 10;
+        const int a0 = wx & ~15;
+        const gptr_u8 ry = as_global(C.ry);
+        uint8_t* const dst = xw + k * kXwBytes;
+        if (a0 >= 0 && a0 + kXwPitch <= W && (W & 15) == 0) {
+            typedef const __attribute__((address_space(1))) void* gvoid;
+            typedef __attribute__((address_space(3))) void* lvoid;
+#pragma unroll
+            for (int base = 0; base < kXwSide * kChunks; base += kWave) {
+                const int i = base + C.lane;
+                const int r = i / kChunks, ch = i - r * kChunks;
+                const int sy = clampi(wy + r, 0, H - 1);
+                // lanes past the window issue no copy (a copy lands at byte 16 * lane)
+                if (i < kXwSide * kChunks)
+                    __builtin_amdgcn_global_load_lds((gvoid)(ry + static_cast<size_t>(sy) * W + a0 + ch * 16),
+                                                     (lvoid)(dst + base * 16), 16, 0, 0);
+            }
+        } else {
+            for (int i = C.lane; i < kXwSide * kChunks; i += kWave) {
+                const int r = i / kChunks, ch = i - r * kChunks;
+                const int sy = clampi(wy + r, 0, H - 1);
+                const gptr_u8 row = ry + static_cast<size_t>(sy) * W;
+                uint32_t wv[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t acc = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        acc |= static_cast<uint32_t>(row[clampi(a0 + ch * 16 + q * 4 + b, 0, W - 1)]) << (8 * b);
+                    wv[q] = acc;
+                }
+                *reinterpret_cast<uint4*>(dst + r * kXwPitch + ch * 16) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+            }
+        }
+    }
+    C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;   // the integer window is overwritten
+    wsync();
+}
+
+// SAD part of rows part, part+8, ... (sampled at `stride`) of the 16x16 block
+// at offset (dx,dy) read from an extended-search window whose sample (0,0) is
+// frame position (wx, wy) and whose first chunk starts at column wx - sh.
+__device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win, int wx, int wy, int dx, int dy,
+                                                int stride, int part) {
+    const int sh = wx & 15;
+    const int ox = C.x0 + dx - wx + sh, oy = C.y0 + dy - wy;
+    const int nrows = (C.ah + stride - 1) / stride;
+    const uint32_t smask = stride == 1 ? 0xffffffffu : 0x00ff00ffu;
+    uint32_t m[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t cm = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (j * 4 + b < C.aw) cm |= 0xffu << (8 * b);
+        m[j] = cm & smask;
+    }
+    const int base = ox & ~3, al = ox & 3;
+    uint32_t acc = 0;
+    for (int r = part; r < nrows; r += 8) {
+        const int y = r * stride;
+        const uint32_t* wr = reinterpret_cast<const uint32_t*>(win + (oy + y) * kXwPitch + base);
+        const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + y * 16);
+        const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
+        acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, al) & m[0], cr[0] & m[0], acc);
+        acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d2, d1, al) & m[1], cr[1] & m[1], acc);
+        acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d3, d2, al) & m[2], cr[2] & m[2], acc);
+        acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d4, d3, al) & m[3], cr[3] & m[3], acc);
+    }
+    return acc;
+}
+
+// extended_content_motion, src/encoder_motion.cpp:1696-1779.  The candidates
+// are independent of each other, so all of them are searched together (eight
+// per round, one 8-lane group each, from windows staged in LDS) and the
+// reference's sequential loop, with its early exit, is replayed on the
+// results; the op counters count only the candidates the loop reaches.
 __device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_sad, int* cand_decision) {
     const JobDesc* J = C.J;
     const uint64_t want = cur_signature(C);
@@ -1298,26 +1394,44 @@ __device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, 
     if (cx_lo <= cx_hi && cy_lo <= cy_hi) {
         const int nxc = cx_hi - cx_lo + 1;
         const int total = nxc * (cy_hi - cy_lo + 1);
-        for (int i = C.lane; i < total; i += kWave) {
-            const int cyy = cy_lo + i / nxc, cxx = cx_lo + i % nxc;
-            const int dx = cxx * 8 - C.x0, dy = cyy * 8 - C.y0;
-            if (iabs(dx) <= 32 && iabs(dy) <= 32) continue;
-            ++scanned;
-            uint64_t k = key_int(sig_distance(want, J->sig[cyy * J->ncx + cxx]), dx, dy);
-            if (k < loc[15]) {
+        const gptr_u8 sigb = as_global(reinterpret_cast<const uint8_t*>(J->sig));
+        typedef const __attribute__((address_space(1))) uint64_t* gptr_u64;
+        const gptr_u64 sig = reinterpret_cast<gptr_u64>(sigb);
+        constexpr int kU = 4;   // signatures in flight per lane
+        for (int i0 = C.lane; i0 < total; i0 += kU * kWave) {
+            uint64_t sv[kU];
+            int dxs[kU], dys[kU];
+            bool ok[kU];
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    if (k < loc[j]) { const uint64_t t = loc[j]; loc[j] = k; k = t; }
+            for (int u = 0; u < kU; ++u) {
+                const int i = i0 + u * kWave;
+                const int cyy = cy_lo + i / nxc, cxx = cx_lo + i % nxc;
+                dxs[u] = cxx * 8 - C.x0; dys[u] = cyy * 8 - C.y0;
+                ok[u] = i < total && !(iabs(dxs[u]) <= 32 && iabs(dys[u]) <= 32);
+                sv[u] = ok[u] ? sig[static_cast<size_t>(cyy) * J->ncx + cxx] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                if (!ok[u]) continue;
+                ++scanned;
+                uint64_t k = key_int(sig_distance(want, sv[u]), dxs[u], dys[u]);
+                if (k < loc[15]) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        if (k < loc[j]) { const uint64_t t = loc[j]; loc[j] = k; k = t; }
+                    }
                 }
             }
         }
     }
     C.n_sig += wave_sum32(scanned);
+    // the top 16 by (distance, |d|, dy, dx): lane r holds the r-th
+    uint64_t ckeys = kNone;
     int tn = 0;
     for (int r = 0; r < 16; ++r) {
         const uint64_t mk = wave_min64(loc[0]);
         if (mk == kNone) break;
-        if (C.lane == 0) C.L.top16[r] = mk;
+        if (C.lane == r) ckeys = mk;
         if (loc[0] == mk) {
 #pragma unroll
             for (int j = 0; j < 15; ++j) loc[j] = loc[j + 1];
@@ -1325,34 +1439,87 @@ __device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, 
         }
         ++tn;
     }
-    wsync();
+    // per candidate: best key of the step-4 grid (stride-2 SAD) and of the 5x5
+    // full-SAD refinement, with the counts of scored (not vetoed) points
+    const int g = C.lane >> 3, part = C.lane & 7;
+    uint64_t key1 = kNone, key2 = kNone;   // lane t: results of candidate t
+    uint32_t cnt1 = 0, cnt2 = 0;
+    for (int t0 = 0; t0 < tn; t0 += kXwPerRound) {
+        const int cnt = imin(kXwPerRound, tn - t0);
+        xw_load(C, ckeys, t0, cnt);
+        const int t = t0 + g;
+        const bool act = g < cnt;
+        const uint64_t ck = lane_shfl64(ckeys, act ? t : 0);
+        const int cdx = key_dx(ck), cdy = key_dy(ck);
+        const int wx = C.x0 + cdx - 10, wy = C.y0 + cdy - 10;
+        const uint8_t* win = C.L.pw + (act ? g : 0) * kXwBytes;
+        uint64_t b1 = kNone, b2 = kNone;
+        uint32_t c1 = 0, c2 = 0;
+        for (int i = 0; i < 25; ++i) {
+            const int dx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
+            const int dy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
+            const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
+            uint32_t sum = (act && !hid) ? xw_sad_part(C, win, wx, wy, dx, dy, 2, part) : 0u;
+            sum = group_sum32(sum, 8);
+            const uint64_t cost = hid ? kBadSad : static_cast<uint64_t>(sum) * 4u;
+            const uint64_t k = key_seq(cost, 0, i);
+            b1 = k < b1 ? k : b1;
+            c1 += hid ? 0u : 1u;
+        }
+        const int i1 = static_cast<int>(b1 & 0xffff);
+        const int sx = clampi(cdx + 4 * (i1 % 5) - 8, b.xmin, b.xmax);
+        const int sy = clampi(cdy + 4 * (i1 / 5) - 8, b.ymin, b.ymax);
+        for (int i = 0; i < 25; ++i) {
+            const int dx = clampi(sx + (i % 5) - 2, b.xmin, b.xmax);
+            const int dy = clampi(sy + (i / 5) - 2, b.ymin, b.ymax);
+            const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
+            uint32_t sum = (act && !hid) ? xw_sad_part(C, win, wx, wy, dx, dy, 1, part) : 0u;
+            sum = group_sum32(sum, 8);
+            const uint64_t cost = hid ? kBadSad : static_cast<uint64_t>(sum);
+            const uint64_t k = key_seq(cost, iabs(dx) + iabs(dy), i);
+            b2 = k < b2 ? k : b2;
+            c2 += hid ? 0u : 1u;
+        }
+        // results of candidate t to lane t
+        const int rel = C.lane - t0;
+        const bool mine = rel >= 0 && rel < cnt;
+        const int src = (mine ? rel : 0) << 3;
+        const uint64_t r1 = lane_shfl64(b1, src), r2 = lane_shfl64(b2, src);
+        const uint32_t q1 = static_cast<uint32_t>(__shfl(static_cast<int>(c1), src));
+        const uint32_t q2 = static_cast<uint32_t>(__shfl(static_cast<int>(c2), src));
+        if (mine) { key1 = r1; key2 = r2; cnt1 = q1; cnt2 = q2; }
+        // the reference loop stops at the first good candidate: skip the rest
+        bool stop = false;
+        {
+            uint64_t best = UINT64_MAX;
+            for (int u = 0; u < t0 + cnt; ++u) {
+                const uint64_t fs = key_cost(lane_read64(key2, u));
+                if (fs < best) best = fs;
+                if (match_good(C, best)) { stop = true; break; }
+            }
+        }
+        if (stop) break;
+    }
+    // replay the candidate loop in order
     uint64_t best_sad = UINT64_MAX;
     int bdx = 0, bdy = 0;
+    const uint64_t npx2 = static_cast<uint64_t>((C.aw + 1) / 2) * ((C.ah + 1) / 2);
+    const uint64_t npx1 = static_cast<uint64_t>(C.aw) * C.ah;
     for (int t = 0; t < tn; ++t) {
-        const uint64_t ck = C.L.top16[t];
+        const uint64_t ck = lane_read64(ckeys, t);
+        const uint64_t s1 = lane_read64(key1, t), s2 = lane_read64(key2, t);
+        C.n_sad += static_cast<uint64_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt1), t)) * npx2 +
+                   static_cast<uint64_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt2), t)) * npx1;
         const int cdx = key_dx(ck), cdy = key_dy(ck);
-        const uint64_t s1 = far_batch(C, 25, 2, KEY_SEQ, 0, [&](int i, int& dx, int& dy) {
-            dx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
-            dy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
-        });
-        int sx = cdx, sy = cdy;
-        if (s1 != kNone && key_cost(s1) != UINT64_MAX) {
-            const int i = static_cast<int>(s1 & 0xffff);
-            sx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
-            sy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
-        }
-        const uint64_t s2 = far_batch(C, 25, 1, KEY_SEQ_MAN, 0, [&](int i, int& dx, int& dy) {
-            dx = clampi(sx + (i % 5) - 2, b.xmin, b.xmax);
-            dy = clampi(sy + (i / 5) - 2, b.ymin, b.ymax);
-        });
-        if (s2 != kNone) {
-            const uint64_t fs = key_cost(s2);
-            if (fs < best_sad) {
-                const int i = static_cast<int>(s2 & 0xffff);
-                best_sad = fs;
-                bdx = clampi(sx + (i % 5) - 2, b.xmin, b.xmax);
-                bdy = clampi(sy + (i / 5) - 2, b.ymin, b.ymax);
-            }
+        const int i1 = static_cast<int>(s1 & 0xffff);
+        const int sx = clampi(cdx + 4 * (i1 % 5) - 8, b.xmin, b.xmax);
+        const int sy = clampi(cdy + 4 * (i1 / 5) - 8, b.ymin, b.ymax);
+        const uint64_t fs = key_cost(s2);
+        if (fs < best_sad) {
+            const int i = static_cast<int>(s2 & 0xffff);
+            best_sad = fs;
+            bdx = clampi(sx + (i % 5) - 2, b.xmin, b.xmax);
+            bdy = clampi(sy + (i / 5) - 2, b.ymin, b.ymax);
         }
         if (match_good(C, best_sad)) break;
     }
