@@ -1342,11 +1342,11 @@ __device__ __forceinline__ void xw_load(Ctx& C, uint64_t ckeys, int t0, int cnt)
     wsync();
 }
 
-// SAD part of rows part, part+8, ... (sampled at `stride`) of the 16x16 block
+// SAD part of rows part, part+step, ... (sampled at `stride`) of the 16x16 block
 // at offset (dx,dy) read from an extended-search window whose sample (0,0) is
 // frame position (wx, wy) and whose first chunk starts at column wx - sh.
 __device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win, int wx, int wy, int dx, int dy,
-                                                int stride, int part) {
+                                                int stride, int part, int step) {
     const int sh = wx & 15;
     const int ox = C.x0 + dx - wx + sh, oy = C.y0 + dy - wy;
     const int nrows = (C.ah + stride - 1) / stride;
@@ -1362,7 +1362,7 @@ __device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win
     }
     const int base = ox & ~3, al = ox & 3;
     uint32_t acc = 0;
-    for (int r = part; r < nrows; r += 8) {
+    for (int r = part; r < nrows; r += step) {
         const int y = r * stride;
         const uint32_t* wr = reinterpret_cast<const uint32_t*>(win + (oy + y) * kXwPitch + base);
         const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + y * 16);
@@ -1387,38 +1387,64 @@ __device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, 
     // cells whose offset lies inside the bounds
     const int cx_lo = imax(0, (C.x0 + b.xmin + 7) >> 3), cx_hi = imin(J->ncx - 1, (C.x0 + b.xmax) >> 3);
     const int cy_lo = imax(0, (C.y0 + b.ymin + 7) >> 3), cy_hi = imin(J->ncy - 1, (C.y0 + b.ymax) >> 3);
-    uint64_t loc[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) loc[j] = kNone;
+    // Scan: rows of the lattice are staged in LDS (16-byte aligned copies),
+    // every lane keys two cells of a row, and keys below the running 16th
+    // best are inserted into the top list that lanes 0..15 hold in order.
+    uint64_t top = kNone;      // lane r < 16: r-th best key so far
+    uint64_t thr = kNone;      // the 16th best (uniform)
     uint32_t scanned = 0;
     if (cx_lo <= cx_hi && cy_lo <= cy_hi) {
         const int nxc = cx_hi - cx_lo + 1;
-        const int total = nxc * (cy_hi - cy_lo + 1);
+        const int chunks = (8 + nxc * 8 + 15) >> 4;           // per row, worst alignment
+        const int pitch = chunks * 16;
+        const int region = lds_bytes_for(J->local_range) - 256 - 16 * 8 - 4 * 8 - 64;
+        const int rows_per = imax(1, region / pitch);
         const gptr_u8 sigb = as_global(reinterpret_cast<const uint8_t*>(J->sig));
-        typedef const __attribute__((address_space(1))) uint64_t* gptr_u64;
-        const gptr_u64 sig = reinterpret_cast<gptr_u64>(sigb);
-        constexpr int kU = 4;   // signatures in flight per lane
-        for (int i0 = C.lane; i0 < total; i0 += kU * kWave) {
-            uint64_t sv[kU];
-            int dxs[kU], dys[kU];
-            bool ok[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int i = i0 + u * kWave;
-                const int cyy = cy_lo + i / nxc, cxx = cx_lo + i % nxc;
-                dxs[u] = cxx * 8 - C.x0; dys[u] = cyy * 8 - C.y0;
-                ok[u] = i < total && !(iabs(dxs[u]) <= 32 && iabs(dys[u]) <= 32);
-                sv[u] = ok[u] ? sig[static_cast<size_t>(cyy) * J->ncx + cxx] : 0;
+        typedef const __attribute__((address_space(1))) void* gvoid;
+        typedef __attribute__((address_space(3))) void* lvoid;
+        uint8_t* const stage = C.L.pw;
+        for (int r0 = cy_lo; r0 <= cy_hi; r0 += rows_per) {
+            const int nr = imin(rows_per, cy_hi - r0 + 1);
+            wsync();
+            for (int r = 0; r < nr; ++r) {
+                const size_t a = (static_cast<size_t>(r0 + r) * J->ncx + cx_lo) * 8;
+                const gptr_u8 src = sigb + (a & ~static_cast<size_t>(15));
+                for (int base = 0; base < chunks; base += kWave) {
+                    const int idx = base + C.lane;
+                    if (idx < chunks)
+                        __builtin_amdgcn_global_load_lds((gvoid)(src + idx * 16), (lvoid)(stage + r * pitch + base * 16),
+                                                         16, 0, 0);
+                }
             }
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                if (!ok[u]) continue;
-                ++scanned;
-                uint64_t k = key_int(sig_distance(want, sv[u]), dxs[u], dys[u]);
-                if (k < loc[15]) {
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        if (k < loc[j]) { const uint64_t t = loc[j]; loc[j] = k; k = t; }
+            C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;   // the integer window is overwritten
+            wsync();
+            for (int r = 0; r < nr; ++r) {
+                const int cyy = r0 + r;
+                const int xoff = static_cast<int>(((static_cast<size_t>(cyy) * J->ncx + cx_lo) * 8) & 15) >> 3;
+                const uint64_t* row = reinterpret_cast<const uint64_t*>(stage + r * pitch) + xoff;
+                const int dy = cyy * 8 - C.y0;
+                const bool inner_row = iabs(dy) <= 32;
+                for (int xb = 0; xb < nxc; xb += kWave) {
+                    const int xi = xb + C.lane;
+                    const int dx = (cx_lo + xi) * 8 - C.x0;
+                    const bool ok = xi < nxc && !(inner_row && iabs(dx) <= 32);
+                    uint64_t k = kNone;
+                    if (ok) {
+                        ++scanned;
+                        k = key_int(sig_distance(want, row[xi]), dx, dy);
+                    }
+                    uint64_t hits = __ballot(k < thr);
+                    while (hits) {
+                        const int l = __ffsll(static_cast<long long>(hits)) - 1;
+                        hits &= hits - 1;
+                        const uint64_t kk = lane_read64(k, l);
+                        if (kk >= thr) continue;
+                        // insert kk into the ordered list of lanes 0..15
+                        const int pos = __popcll(__ballot(C.lane < 16 && top < kk));
+                        const uint64_t up = lane_shfl64(top, C.lane > 0 ? C.lane - 1 : 0);
+                        if (C.lane < 16 && C.lane > pos) top = up;
+                        if (C.lane == pos) top = kk;
+                        thr = lane_read64(top, 15);
                     }
                 }
             }
@@ -1426,68 +1452,78 @@ __device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, 
     }
     C.n_sig += wave_sum32(scanned);
     // the top 16 by (distance, |d|, dy, dx): lane r holds the r-th
-    uint64_t ckeys = kNone;
-    int tn = 0;
-    for (int r = 0; r < 16; ++r) {
-        const uint64_t mk = wave_min64(loc[0]);
-        if (mk == kNone) break;
-        if (C.lane == r) ckeys = mk;
-        if (loc[0] == mk) {
-#pragma unroll
-            for (int j = 0; j < 15; ++j) loc[j] = loc[j + 1];
-            loc[15] = kNone;
-        }
-        ++tn;
-    }
-    // per candidate: best key of the step-4 grid (stride-2 SAD) and of the 5x5
-    // full-SAD refinement, with the counts of scored (not vetoed) points
-    const int g = C.lane >> 3, part = C.lane & 7;
+    const uint64_t ckeys = top;
+    const int tn = __popcll(__ballot(C.lane < 16 && top != kNone));
+    // Per candidate: the best key of its 5x5 grid at step 4 (stride-2 SAD)
+    // and of the 5x5 full-SAD refinement around the grid's best, with the
+    // counts of scored (not vetoed) points.  Eight candidates per round; each
+    // lane scores whole points, the 25 keys of a candidate meet in LDS.
     uint64_t key1 = kNone, key2 = kNone;   // lane t: results of candidate t
     uint32_t cnt1 = 0, cnt2 = 0;
+    uint64_t* const kbuf = reinterpret_cast<uint64_t*>(C.L.pw + kXwPerRound * kXwBytes);
     for (int t0 = 0; t0 < tn; t0 += kXwPerRound) {
         const int cnt = imin(kXwPerRound, tn - t0);
+        const int m = cnt * 25;
         xw_load(C, ckeys, t0, cnt);
-        const int t = t0 + g;
-        const bool act = g < cnt;
-        const uint64_t ck = lane_shfl64(ckeys, act ? t : 0);
-        const int cdx = key_dx(ck), cdy = key_dy(ck);
-        const int wx = C.x0 + cdx - 10, wy = C.y0 + cdy - 10;
-        const uint8_t* win = C.L.pw + (act ? g : 0) * kXwBytes;
-        uint64_t b1 = kNone, b2 = kNone;
-        uint32_t c1 = 0, c2 = 0;
-        for (int i = 0; i < 25; ++i) {
-            const int dx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
-            const int dy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
-            const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
-            uint32_t sum = (act && !hid) ? xw_sad_part(C, win, wx, wy, dx, dy, 2, part) : 0u;
-            sum = group_sum32(sum, 8);
-            const uint64_t cost = hid ? kBadSad : static_cast<uint64_t>(sum) * 4u;
-            const uint64_t k = key_seq(cost, 0, i);
-            b1 = k < b1 ? k : b1;
-            c1 += hid ? 0u : 1u;
+        // the round's candidate offsets, lane g: candidate t0 + g
+        const uint64_t myck = lane_shfl64(ckeys, imin(t0 + (C.lane & 7), 63));
+        const int mydx = key_dx(myck), mydy = key_dy(myck);
+        int sxv = 0, syv = 0;     // lane g < cnt: the grid winner of candidate t0 + g
+        for (int stage = 0; stage < 2; ++stage) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = C.lane + j * kWave;
+                const bool ok = c < m;
+                const int g = ok ? c / 25 : 0, i = c - g * 25;
+                const int cdx = __shfl(mydx, g), cdy = __shfl(mydy, g);
+                const int ox = stage ? __shfl(sxv, g) : 0, oy = stage ? __shfl(syv, g) : 0;
+                int dx, dy;
+                if (stage == 0) {
+                    dx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
+                    dy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
+                } else {
+                    dx = clampi(ox + (i % 5) - 2, b.xmin, b.xmax);
+                    dy = clampi(oy + (i / 5) - 2, b.ymin, b.ymax);
+                }
+                if (ok) {
+                    const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
+                    uint64_t cost = kBadSad;
+                    if (!hid) {
+                        const int stride = stage ? 1 : 2;
+                        const uint32_t sum = xw_sad_part(C, C.L.pw + g * kXwBytes, C.x0 + cdx - 10, C.y0 + cdy - 10, dx, dy,
+                                                         stride, 0, 1);
+                        cost = static_cast<uint64_t>(sum) * static_cast<uint64_t>(stride * stride);
+                    }
+                    kbuf[c] = stage ? key_seq(cost, iabs(dx) + iabs(dy), i) : key_seq(cost, 0, i);
+                }
+            }
+            wsync();
+            // lane g < cnt: reduce candidate t0 + g
+            uint64_t bk = kNone;
+            uint32_t nc = 0;
+            if (C.lane < cnt) {
+                for (int i = 0; i < 25; ++i) {
+                    const uint64_t k = kbuf[C.lane * 25 + i];
+                    bk = k < bk ? k : bk;
+                    nc += key_cost(k) == kBadSad ? 0u : 1u;
+                }
+            }
+            wsync();
+            if (stage == 0) {
+                const int i1 = static_cast<int>(bk & 0xffff);
+                sxv = clampi(mydx + 4 * (i1 % 5) - 8, b.xmin, b.xmax);
+                syv = clampi(mydy + 4 * (i1 / 5) - 8, b.ymin, b.ymax);
+            }
+            // results of candidate t0 + g to lane t0 + g
+            const int rel = C.lane - t0;
+            const bool mine = rel >= 0 && rel < cnt;
+            const uint64_t rk = lane_shfl64(bk, mine ? rel : 0);
+            const uint32_t rc = static_cast<uint32_t>(__shfl(static_cast<int>(nc), mine ? rel : 0));
+            if (mine) {
+                if (stage == 0) { key1 = rk; cnt1 = rc; }
+                else { key2 = rk; cnt2 = rc; }
+            }
         }
-        const int i1 = static_cast<int>(b1 & 0xffff);
-        const int sx = clampi(cdx + 4 * (i1 % 5) - 8, b.xmin, b.xmax);
-        const int sy = clampi(cdy + 4 * (i1 / 5) - 8, b.ymin, b.ymax);
-        for (int i = 0; i < 25; ++i) {
-            const int dx = clampi(sx + (i % 5) - 2, b.xmin, b.xmax);
-            const int dy = clampi(sy + (i / 5) - 2, b.ymin, b.ymax);
-            const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
-            uint32_t sum = (act && !hid) ? xw_sad_part(C, win, wx, wy, dx, dy, 1, part) : 0u;
-            sum = group_sum32(sum, 8);
-            const uint64_t cost = hid ? kBadSad : static_cast<uint64_t>(sum);
-            const uint64_t k = key_seq(cost, iabs(dx) + iabs(dy), i);
-            b2 = k < b2 ? k : b2;
-            c2 += hid ? 0u : 1u;
-        }
-        // results of candidate t to lane t
-        const int rel = C.lane - t0;
-        const bool mine = rel >= 0 && rel < cnt;
-        const int src = (mine ? rel : 0) << 3;
-        const uint64_t r1 = lane_shfl64(b1, src), r2 = lane_shfl64(b2, src);
-        const uint32_t q1 = static_cast<uint32_t>(__shfl(static_cast<int>(c1), src));
-        const uint32_t q2 = static_cast<uint32_t>(__shfl(static_cast<int>(c2), src));
-        if (mine) { key1 = r1; key2 = r2; cnt1 = q1; cnt2 = q2; }
         // the reference loop stops at the first good candidate: skip the rest
         bool stop = false;
         {
