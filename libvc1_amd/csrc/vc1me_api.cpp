@@ -90,6 +90,9 @@ struct JobSlot {
     GlobalState* g = nullptr;
     int32_t* err = nullptr;
     unsigned long long* ops = nullptr;
+    uint64_t* ext_sad = nullptr;
+    int2* ext_mv = nullptr;
+    uint32_t* ext_ops = nullptr;
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -116,6 +119,9 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_g = carve(sizeof(GlobalState));
     const size_t o_err = carve(sizeof(int32_t));
     const size_t o_ops = carve(8 * sizeof(unsigned long long));
+    const size_t o_exs = carve(mbs * 8);
+    const size_t o_exm = carve(mbs * sizeof(int2));
+    const size_t o_exo = carve(mbs * 2 * sizeof(uint32_t));
     hipError_t e = hipMalloc(&s.mem, off);
     if (e != hipSuccess) { s.mem = nullptr; return e; }
     uint8_t* b = static_cast<uint8_t*>(s.mem);
@@ -135,6 +141,9 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.g = reinterpret_cast<GlobalState*>(b + o_g);
     s.err = reinterpret_cast<int32_t*>(b + o_err);
     s.ops = reinterpret_cast<unsigned long long*>(b + o_ops);
+    s.ext_sad = reinterpret_cast<uint64_t*>(b + o_exs);
+    s.ext_mv = reinterpret_cast<int2*>(b + o_exm);
+    s.ext_ops = reinterpret_cast<uint32_t*>(b + o_exo);
     s.mbs_cap = mbs;
     s.cells_cap = cells;
     return hipSuccess;
@@ -334,6 +343,7 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.g = s.g;
     d.err = s.err;
     d.ops = s.ops;
+    d.ext_sad = s.ext_sad; d.ext_mv = s.ext_mv; d.ext_ops = s.ext_ops;
     d.max_diag = d.mbw + 2 * d.mbh - 3;
     d.qp = ref->qp;
     d.bp = ref->bp;
@@ -494,6 +504,13 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
     HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st, ctx->events[2], ctx->events[3]));
     HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
     const int lds = vc1me::mb_kernel_lds_bytes(max_local);
+    {
+        int max_mbs = 0;
+        for (int i = 0; i < n; ++i)
+            if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range)
+                max_mbs = std::max(max_mbs, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh);
+        HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
+    }
     int launches = 0;
     for (int t = 0; t <= max_diag; ++t) {
         int per = 0;

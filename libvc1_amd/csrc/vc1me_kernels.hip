@@ -1375,12 +1375,16 @@ __device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win
     return acc;
 }
 
-// extended_content_motion, src/encoder_motion.cpp:1696-1779.  The candidates
-// are independent of each other, so all of them are searched together (eight
-// per round, one 8-lane group each, from windows staged in LDS) and the
-// reference's sequential loop, with its early exit, is replayed on the
+// The search of extended_content_motion (src/encoder_motion.cpp:1696-1779):
+// the signature scan for the top-16 cells, then per cell a 5x5 grid at step 4
+// and a 5x5 refinement, stopping at the first good match.  It depends only on
+// the current block, the reference and the bounds, never on other
+// macroblocks' decisions, so k_extended runs it for every macroblock ahead of
+// the wavefront.  The candidates are independent of each other, so all of
+// them are searched together (eight per round from windows staged in LDS) and
+// the reference's sequential loop, with its early exit, is replayed on the
 // results; the op counters count only the candidates the loop reaches.
-__device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, uint64_t* cand_sad, int* cand_decision) {
+__device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out_sad, int* out_dx, int* out_dy) {
     const JobDesc* J = C.J;
     const uint64_t want = cur_signature(C);
     const Bounds b = bounds_px(range);
@@ -1559,11 +1563,24 @@ __device__ __forceinline__ IRes extended_content(Ctx& C, int range, IRes local, 
         }
         if (match_good(C, best_sad)) break;
     }
+    *out_sad = best_sad;
+    *out_dx = bdx;
+    *out_dy = bdy;
+}
+
+// extended_content_motion's decision (src/encoder_motion.cpp:1766-1779) on the
+// search result k_extended left for this macroblock.
+__device__ __forceinline__ IRes extended_content(Ctx& C, IRes local, uint64_t* cand_sad, int* cand_decision) {
+    const JobDesc* J = C.J;
+    const uint64_t best_sad = J->ext_sad[C.pos];
+    const int2 bv = J->ext_mv[C.pos];
+    C.n_sig += J->ext_ops[2 * C.pos];
+    C.n_sad += J->ext_ops[2 * C.pos + 1];
     *cand_sad = best_sad;
     const int decision = best_sad < UINT64_MAX / 8 ? distant_decision(C, best_sad, local.sad) : 0;
     *cand_decision = decision;
     if (best_sad >= local.sad || decision < 2) return local;
-    return {bdx, bdy, best_sad};
+    return {bv.x, bv.y, best_sad};
 }
 
 // ------------------------------------------------------------------ staged sub-pel refinement
@@ -1907,7 +1924,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
         if (range > local_range && !match_good(C, im.sad)) {
             uint64_t ext_sad = UINT64_MAX;
             int ext_dec = 0;
-            const IRes ext = extended_content(C, range, im, &ext_sad, &ext_dec);
+            const IRes ext = extended_content(C, im, &ext_sad, &ext_dec);
             if (J->lr_mode != 2 && px && ext_sad < UINT64_MAX / 8) {
                 const double mae = static_cast<double>(ext_sad) / static_cast<double>(px);
                 if (dc_mae < 0.0 || mae < dc_mae) { dc_mae = mae; decision_out = ext_dec; }
@@ -2088,8 +2105,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
     }
 }
 
-__device__ __forceinline__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
-    Ctx C;
+__device__ __forceinline__ void init_ctx(Ctx& C, const JobDesc* J, int mx, int my, uint8_t* smem) {
     C.J = J;
     C.g = {J->W, J->H, J->vw, J->vh, J->mbw, J->mbh};
     C.ry = J->ry;
@@ -2104,7 +2120,39 @@ __device__ __forceinline__ void run_mb(const JobDesc* J, int mx, int my, uint8_t
     C.pw_ok = false; C.pw_bil = false; C.pwx = C.pwy = C.pwsh = 0;
     C.n_sad = C.n_taps = C.n_satd = C.n_sig = 0;
     C.ln_sad = C.ln_taps = C.ln_satd = 0;
+}
+
+__device__ __forceinline__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
+    Ctx C;
+    init_ctx(C, J, mx, my, smem);
     analyze_mb(C);
+}
+
+// The extended long-range search of every macroblock of jobs with a search
+// range beyond the local range: one 64-lane workgroup per macroblock, all
+// macroblocks at once, ahead of the wavefront (which only reads the result).
+__global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ jobs) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const JobDesc* J = &jobs[blockIdx.y];
+    if (J->range <= J->local_range) return;
+    const int pos = static_cast<int>(blockIdx.x);
+    if (pos >= J->mbw * J->mbh) return;
+    Ctx C;
+    init_ctx(C, J, pos % J->mbw, pos / J->mbw, smem);
+    for (int i = C.lane; i < 256; i += kWave) {
+        const int x = i & 15, y = i >> 4;
+        C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
+    }
+    wsync();
+    uint64_t sad = UINT64_MAX;
+    int dx = 0, dy = 0;
+    extended_search(C, J->range, &sad, &dx, &dy);
+    if (C.lane == 0) {
+        J->ext_sad[pos] = sad;
+        J->ext_mv[pos] = make_int2(dx, dy);
+        J->ext_ops[2 * pos] = static_cast<uint32_t>(C.n_sig);
+        J->ext_ops[2 * pos + 1] = static_cast<uint32_t>(C.n_sad);
+    }
 }
 
 // Block costs under the search's own laws (block_sad_motion / block_satd_motion,
@@ -2603,6 +2651,12 @@ hipError_t launch_phase_planes(const uint8_t* ref, int width, int height, uint8_
     if (rows < height + 3 || pitch < width + 3 + 8) return hipErrorInvalidValue;
     const dim3 grid((width + 3 + 63) / 64, (height + 3 + 3) / 4);
     hipLaunchKernelGGL(k_phase_planes, grid, dim3(256), 0, s, ref, width, height, out, pitch, rows, bilinear);
+    return hipGetLastError();
+}
+
+hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s) {
+    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_extended, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs);
     return hipGetLastError();
 }
 
