@@ -20,6 +20,12 @@ the timed region, and the reference's own multithreaded CPU path timed on
 this host on a bounded sample of the same frames.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-check] [--no-cpu]
+    python bench.py --copy-calibration     # 256 MiB device copy, for HBM counter scaling
+
+`roofline.traffic` (HBM bytes per launch of the macroblock kernel) comes from
+the FETCH_SIZE / WRITE_SIZE counter runs summarised in
+profiles/k_mb_diagonal_1080p_ibbp16_pmc.json, each scaled by what the counter
+reads for the calibration copy; it is null when that summary is absent.
 """
 from __future__ import annotations
 
@@ -79,10 +85,12 @@ def cpu_baseline(pairs, pics, params):
                                           C.POINTER(abi.Planes), C.c_int, C.c_int]
         lib.ref_time_p_frames.restype = C.c_double
         secs = lib.ref_time_p_frames(C.byref(params), n, ca, ra, threads, 1)
+        secs1 = lib.ref_time_p_frames(C.byref(params), 1, ca, ra, 1, 1)
         if secs > 0:
             return {"value": round(n * mbs / secs, 1), "unit": "MB/s", "cores": threads, "kind": "reference",
                     "sample": f"{n} of the 16 1080p P-frame pairs, one pair per thread, reference SIMD auto-selected",
-                    "seconds": round(secs, 3)}
+                    "seconds": round(secs, 3),
+                    "value_1_thread": round(mbs / secs1, 1) if secs1 > 0 else None}
     # the C restatement, single thread, on one pair
     from _oracle import oracle_analyze
     t0 = time.time()
@@ -90,6 +98,34 @@ def cpu_baseline(pairs, pics, params):
     secs = time.time() - t0
     return {"value": round(mbs / secs, 1), "unit": "MB/s", "cores": 1, "kind": "port",
             "sample": "1 of the 16 1080p P-frame pairs, scalar C oracle", "seconds": round(secs, 3)}
+
+
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_mb_diagonal_1080p_ibbp16_pmc.json")
+
+
+def measured_traffic():
+    """HBM bytes per k_mb_diagonal launch from the committed counter summary."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            d = json.load(f)
+        return d["k_mb_diagonal"]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration() -> int:
+    """A plain device-to-device copy of known size (256 MiB, three times):
+    the reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 256 << 20
+    x = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+    return 0
 
 
 def device_peak_ops():
@@ -114,7 +150,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(streams and frame pairs shard across GPUs without communication)", file=sys.stderr)
@@ -181,7 +220,7 @@ def main() -> int:
         "peak": round(peak / 1e12, 2),
         "unit": "TOP/s",
         "frac": round(achieved / peak, 5),
-        "traffic": None,
+        "traffic": measured_traffic(),
         "kernel": "k_mb_diagonal",
         "avg_launch_us": round(avg_launch_us, 2),
         "launches_per_step": mb_launches // max(1, args.steps),
@@ -189,10 +228,15 @@ def main() -> int:
                          "sig_ops": work.sig_ops},
         "peak_basis": f"{cus} CU x 64 lanes x {clk / 1e9:.2f} GHz int32 lane-ops",
     }
-    frame_bytes = WIDTH * HEIGHT * 3 // 2
-    hbm_alg = 2 * frame_bytes * len(jobs)  # each pair reads cur + ref once (algorithmic)
-    roofline_hbm = {"bound": "hbm", "achieved": round(hbm_alg / (ms_per_step / 1e3) / 1e9, 2), "peak": 8000.0,
-                    "unit": "GB/s", "frac": round(hbm_alg / (ms_per_step / 1e3) / 8e12, 6), "traffic": None}
+    # algorithmic HBM bytes: each macroblock's current and reference luma once (2 x 256 B)
+    launches = max(1, mb_launches // max(1, args.steps))
+    alg_bytes_launch = (mbs_per_step / launches) * 512.0
+    launch_s = avg_launch_us / 1e6
+    roofline_hbm = {"bound": "hbm", "achieved": round(alg_bytes_launch / launch_s / 1e9, 3) if launch_s else 0.0,
+                    "peak": 8000.0, "unit": "GB/s",
+                    "frac": round(alg_bytes_launch / launch_s / 8e12, 7) if launch_s else 0.0,
+                    "traffic": measured_traffic(), "algorithmic_bytes_per_launch": round(alg_bytes_launch),
+                    "kernel": "k_mb_diagonal"}
 
     cpu = None
     if not args.no_cpu:
