@@ -244,6 +244,64 @@ int vc1me_last_work(vc1me_ctx* ctx, vc1me_work* out);
 int vc1me_counters_get(vc1me_ctx* ctx, vc1me_counters* out);
 void vc1me_counters_reset(vc1me_ctx* ctx);
 
+/* ---- learned proposals.  No counterpart in the reference: a small network
+ * proposes, per macroblock, two integer-search seeds and a confidence; the
+ * search still judges every vector with exact costs.
+ *
+ * Network: 1/4-resolution luma of cur and ref (2 channels), four 3x3
+ * convolutions to 32 channels (dilations 1, 2, 4, 8) with bias and ReLU, a
+ * mean pool per macroblock and a 1x1 head to 5 outputs per macroblock:
+ * (dx1, dy1, dx2, dy2) in units of 64 pixels and a confidence logit.
+ * Parameters are fp32 masters (layout: per convolution 32 x 9 x cin weights
+ * [out][tap][in] then 32 biases; then 5 x 32 head weights and 5 biases);
+ * the kernels compute in bf16 with fp32 accumulation. */
+typedef struct vc1me_nn vc1me_nn;
+#define VC1ME_NN_OUTPUTS 5
+int vc1me_nn_param_count(void);
+int vc1me_nn_create(vc1me_ctx* ctx, vc1me_nn** out);
+void vc1me_nn_destroy(vc1me_nn* nn);
+/* Copy parameters in (resets the optimizer state) or out, kParams floats. */
+int vc1me_nn_set_params(vc1me_nn* nn, const float* params);
+int vc1me_nn_get_params(vc1me_nn* nn, float* params);
+/* Forward pass on n frame pairs of one raster; out receives
+ * n x mbh x mbw x 5 floats. */
+int vc1me_nn_forward_frames(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs, int n,
+                            float* out);
+/* Forward and backward on a given network input (n x h4 x w4 x 2 floats, as
+ * the downsampling would produce) with a given output gradient
+ * (n x mbh x mbw x 5, mbh = ceil(h4/4), mbw = ceil(w4/4)); returns the outputs
+ * and the parameter gradients.  For gradient checks. */
+int vc1me_nn_forward_backward(vc1me_nn* nn, const float* input, int n, int h4, int w4, const float* dout, float* out,
+                              float* grads);
+/* One optimizer step (Adam) on n frame pairs of one raster: targets are the
+ * plain search's 1-MV vectors (n x mbs x 2, quarter-pel), valid marks the
+ * macroblocks that count (n x mbs).  loss receives the batch loss. */
+int vc1me_nn_train_step(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs,
+                        const int32_t* targets, const uint8_t* valid, int n, float lr, float* loss);
+/* vc1me_analyze_step with learned seeding: the network runs on every pair;
+ * macroblocks whose confidence reaches `threshold` replace the integer search
+ * (UMH stages and the long-range stages) by a search seeded with the two
+ * proposals plus the predictor, zero and the global vector, ending in a unit
+ * diamond; sub-pel refinement, 4-MV and intra run as in the plain search.
+ * The others run the plain search unchanged. */
+int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jobs, int n, float threshold);
+/* The same seeding with proposals from the caller: proposals[i] holds
+ * mbw x mbh x 5 floats for job i in the network's output layout (two vectors
+ * in units of 64 pixels, then a confidence logit). */
+int vc1me_analyze_step_proposals(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* proposals,
+                                 float threshold);
+
+/* ---- cost report (no reference counterpart; judges the learned form).
+ * While on, every analysis records per macroblock C20 = 20 * (SAD + lambda *
+ * R) of its final 1-MV vector: SAD by block_sad_motion at that vector, R the
+ * MV rate against predictor_info_mixed_1mv of the final field including the
+ * HYBRIDPRED bit, lambda = max(1, 0.35 * PQINDEX) (the rate-winner score of
+ * src/encoder_motion.cpp:3458-3471; 20 * lambda = max(20, 7 * PQINDEX) is an
+ * integer).  vc1me_last_mb_costs copies job `job` of the last step: mbs
+ * values, and optionally the 1-MV vectors (mbs x 2 quarter-pel). */
+int vc1me_set_cost_report(vc1me_ctx* ctx, int on);
+int vc1me_last_mb_costs(vc1me_ctx* ctx, int job, uint64_t* cost20, int32_t* one_mvs);
+
 #ifdef __cplusplus
 }
 #endif

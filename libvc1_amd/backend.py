@@ -22,7 +22,9 @@ EXPORTS = (
     "vc1me_device_name", "vc1me_score_batch", "vc1me_frame_upload", "vc1me_frame_release",
     "vc1me_score_batch_frames", "vc1me_analyze_p_frame", "vc1me_analyze_step", "vc1me_last_timing",
     "vc1me_set_profiling", "vc1me_last_work", "vc1me_counters_get", "vc1me_counters_reset",
-    "vc1me_block_costs",
+    "vc1me_block_costs", "vc1me_nn_param_count", "vc1me_nn_create", "vc1me_nn_destroy", "vc1me_nn_set_params",
+    "vc1me_nn_get_params", "vc1me_nn_forward_frames", "vc1me_nn_forward_backward", "vc1me_nn_train_step",
+    "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
 )
 
 
@@ -71,6 +73,21 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_counters_get.argtypes = [C.c_void_p, P(abi.Counters)]
     lib.vc1me_counters_reset.argtypes = [C.c_void_p]
     lib.vc1me_counters_reset.restype = None
+    lib.vc1me_nn_param_count.argtypes = []
+    lib.vc1me_nn_create.argtypes = [C.c_void_p, P(C.c_void_p)]
+    lib.vc1me_nn_destroy.argtypes = [C.c_void_p]
+    lib.vc1me_nn_destroy.restype = None
+    lib.vc1me_nn_set_params.argtypes = [C.c_void_p, P(C.c_float)]
+    lib.vc1me_nn_get_params.argtypes = [C.c_void_p, P(C.c_float)]
+    lib.vc1me_nn_forward_frames.argtypes = [C.c_void_p, P(C.c_void_p), P(C.c_void_p), C.c_int, P(C.c_float)]
+    lib.vc1me_nn_forward_backward.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_int, C.c_int, P(C.c_float),
+                                              P(C.c_float), P(C.c_float)]
+    lib.vc1me_nn_train_step.argtypes = [C.c_void_p, P(C.c_void_p), P(C.c_void_p), P(C.c_int32), P(C.c_uint8), C.c_int,
+                                        C.c_float, P(C.c_float)]
+    lib.vc1me_analyze_step_learned.argtypes = [C.c_void_p, C.c_void_p, P(abi.Job), C.c_int, C.c_float]
+    lib.vc1me_analyze_step_proposals.argtypes = [C.c_void_p, P(abi.Job), C.c_int, P(P(C.c_float)), C.c_float]
+    lib.vc1me_set_cost_report.argtypes = [C.c_void_p, C.c_int]
+    lib.vc1me_last_mb_costs.argtypes = [C.c_void_p, C.c_int, P(C.c_uint64), P(C.c_int32)]
     _lib = lib
     return lib
 
@@ -126,24 +143,60 @@ class Backend:
         self._check(self.lib.vc1me_analyze_p_frame(self.ctx, cur.handle, ref.handle, C.byref(params), C.byref(r.struct)))
         return r
 
-    def analyze_step(self, pairs, params_list, want_results: bool = True):
+    @staticmethod
+    def _jobs(pairs, params_list, want_results: bool):
         n = len(pairs)
         jobs = (abi.Job * n)()
         results = []
-        keep = []
         for i, ((cur, ref), p) in enumerate(zip(pairs, params_list)):
             jobs[i].cur = cur.handle
             jobs[i].ref = ref.handle
             jobs[i].params = C.pointer(p)
-            keep.append(p)
             if want_results:
                 r = abi.ResultArrays(p.width, p.height)
                 results.append(r)
                 jobs[i].out = C.pointer(r.struct)
             else:
                 jobs[i].out = None
-        self._check(self.lib.vc1me_analyze_step(self.ctx, jobs, n))
+        return jobs, results
+
+    def analyze_step(self, pairs, params_list, want_results: bool = True):
+        jobs, results = self._jobs(pairs, params_list, want_results)
+        self._check(self.lib.vc1me_analyze_step(self.ctx, jobs, len(pairs)))
         return results
+
+    def analyze_step_learned(self, net: "ProposalNet", pairs, params_list, threshold: float = 0.5,
+                             want_results: bool = True):
+        """analyze_step with the learned seeding of `net`."""
+        jobs, results = self._jobs(pairs, params_list, want_results)
+        self._check(self.lib.vc1me_analyze_step_learned(self.ctx, net.handle, jobs, len(pairs), threshold))
+        return results
+
+    def analyze_step_proposals(self, pairs, params_list, proposals, threshold: float = 0.5,
+                               want_results: bool = True):
+        """analyze_step seeded by caller-given proposals, one (mbh, mbw, 5)
+        float array per pair in the network's output layout."""
+        jobs, results = self._jobs(pairs, params_list, want_results)
+        n = len(pairs)
+        props = [np.ascontiguousarray(p, np.float32) for p in proposals]
+        for p, prm in zip(props, params_list):
+            mbw, mbh = abi.mb_grid(prm.width, prm.height)
+            assert p.size == mbw * mbh * 5, (p.shape, mbw, mbh)
+        arr = (C.POINTER(C.c_float) * n)(*[p.ctypes.data_as(C.POINTER(C.c_float)) for p in props])
+        self._check(self.lib.vc1me_analyze_step_proposals(self.ctx, jobs, n, arr, threshold))
+        return results
+
+    def set_cost_report(self, on: bool) -> None:
+        self._check(self.lib.vc1me_set_cost_report(self.ctx, 1 if on else 0))
+
+    def last_mb_costs(self, job: int, mbs: int):
+        """(cost20 (mbs,) uint64, 1-MV vectors (mbs, 2) int32) of job `job`
+        of the last step; needs set_cost_report(True) before that step."""
+        c = np.zeros(mbs, np.uint64)
+        v = np.zeros((mbs, 2), np.int32)
+        self._check(self.lib.vc1me_last_mb_costs(self.ctx, job, c.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                                 v.ctypes.data_as(C.POINTER(C.c_int32))))
+        return c, v
 
     def score_batch(self, cur: np.ndarray, ref: np.ndarray, x0: int, y0: int, bw: int, bh: int,
                     cands: np.ndarray, sample_stride: int = 1, bilinear: int = 0, rnd: int = 0,
@@ -185,3 +238,70 @@ class Backend:
         c = abi.Counters()
         self._check(self.lib.vc1me_counters_get(self.ctx, C.byref(c)))
         return c
+
+
+class ProposalNet:
+    """The learned-seeding proposal network of a Backend (vc1me_nn_*)."""
+
+    def __init__(self, backend: Backend):
+        self.backend = backend
+        self.lib = backend.lib
+        h = C.c_void_p()
+        backend._check(self.lib.vc1me_nn_create(backend.ctx, C.byref(h)))
+        self.handle = h
+        self.nparams = int(self.lib.vc1me_nn_param_count())
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.vc1me_nn_destroy(self.handle)
+            self.handle = None
+
+    def set_params(self, params: np.ndarray) -> None:
+        params = np.ascontiguousarray(params, np.float32)
+        assert params.size == self.nparams
+        self.backend._check(self.lib.vc1me_nn_set_params(self.handle, params.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, np.float32)
+        self.backend._check(self.lib.vc1me_nn_get_params(self.handle, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    @staticmethod
+    def grid(width: int, height: int):
+        h4, w4 = (height + 3) // 4, (width + 3) // 4
+        return h4, w4, (h4 + 3) // 4, (w4 + 3) // 4
+
+    def forward_frames(self, pairs) -> np.ndarray:
+        n = len(pairs)
+        curs = (C.c_void_p * n)(*[c.handle for c, _ in pairs])
+        refs = (C.c_void_p * n)(*[r.handle for _, r in pairs])
+        _, _, mbh, mbw = self.grid(pairs[0][0].width, pairs[0][0].height)
+        out = np.zeros((n, mbh, mbw, 5), np.float32)
+        self.backend._check(self.lib.vc1me_nn_forward_frames(self.handle, curs, refs, n,
+                                                            out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def forward_backward(self, x: np.ndarray, dout: np.ndarray):
+        """x: (n, h4, w4, 2) float; dout: (n, mbh, mbw, 5).  Returns (out, grads)."""
+        x = np.ascontiguousarray(x, np.float32)
+        dout = np.ascontiguousarray(dout, np.float32)
+        n, h4, w4, _ = x.shape
+        out = np.zeros_like(dout)
+        grads = np.zeros(self.nparams, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        self.backend._check(self.lib.vc1me_nn_forward_backward(self.handle, fp(x), n, h4, w4, fp(dout), fp(out),
+                                                              fp(grads)))
+        return out, grads
+
+    def train_step(self, pairs, targets: np.ndarray, valid: np.ndarray, lr: float) -> float:
+        n = len(pairs)
+        curs = (C.c_void_p * n)(*[c.handle for c, _ in pairs])
+        refs = (C.c_void_p * n)(*[r.handle for _, r in pairs])
+        targets = np.ascontiguousarray(targets, np.int32)
+        valid = np.ascontiguousarray(valid, np.uint8)
+        loss = C.c_float()
+        self.backend._check(self.lib.vc1me_nn_train_step(
+            self.handle, curs, refs, targets.ctypes.data_as(C.POINTER(C.c_int32)),
+            valid.ctypes.data_as(C.POINTER(C.c_uint8)), n, lr, C.byref(loss)))
+        return float(loss.value)
+

@@ -5,6 +5,7 @@
 // runtime + C++17; no torch types.
 #include "../../include/vc1me.h"
 #include "vc1me_internal.h"
+#include "vc1me_nn_internal.h"
 
 #include <algorithm>
 #include <atomic>
@@ -93,6 +94,9 @@ struct JobSlot {
     uint64_t* ext_sad = nullptr;
     int2* ext_mv = nullptr;
     uint32_t* ext_ops = nullptr;
+    uint64_t* cost20 = nullptr;     // cost report
+    int2* one_mv = nullptr;
+    int mbs_last = 0;               // macroblocks of the last step's job
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -122,6 +126,8 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_exs = carve(mbs * 8);
     const size_t o_exm = carve(mbs * sizeof(int2));
     const size_t o_exo = carve(mbs * 2 * sizeof(uint32_t));
+    const size_t o_c20 = carve(mbs * 8);
+    const size_t o_one = carve(mbs * sizeof(int2));
     hipError_t e = hipMalloc(&s.mem, off);
     if (e != hipSuccess) { s.mem = nullptr; return e; }
     uint8_t* b = static_cast<uint8_t*>(s.mem);
@@ -144,6 +150,8 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.ext_sad = reinterpret_cast<uint64_t*>(b + o_exs);
     s.ext_mv = reinterpret_cast<int2*>(b + o_exm);
     s.ext_ops = reinterpret_cast<uint32_t*>(b + o_exo);
+    s.cost20 = reinterpret_cast<uint64_t*>(b + o_c20);
+    s.one_mv = reinterpret_cast<int2*>(b + o_one);
     s.mbs_cap = mbs;
     s.cells_cap = cells;
     return hipSuccess;
@@ -168,6 +176,8 @@ struct vc1me_ctx {
     vc1me_timing timing{};
     vc1me_work work{};
     bool profiling = false;
+    bool cost_report = false;
+    int last_jobs = 0;
     std::vector<JobSlot> slots;
     JobDesc* d_jobs = nullptr;
     size_t d_jobs_cap = 0;
@@ -182,6 +192,9 @@ struct vc1me_ctx {
     size_t cand_cap = 0;
     std::vector<uint32_t> h_costs;
     int cu_count = 256;
+    // proposals handed in by the host (vc1me_analyze_step_proposals)
+    float* d_props = nullptr;
+    size_t props_cap = 0;
 };
 
 namespace {
@@ -344,6 +357,9 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.err = s.err;
     d.ops = s.ops;
     d.ext_sad = s.ext_sad; d.ext_mv = s.ext_mv; d.ext_ops = s.ext_ops;
+    d.nn_out = nullptr; d.nn_logit = 0.f;
+    d.cost20 = nullptr; d.one_mv = nullptr;
+    d.lam20 = std::max(20, 7 * p->pqindex);
     d.max_diag = d.mbw + 2 * d.mbh - 3;
     d.qp = ref->qp;
     d.bp = ref->bp;
@@ -451,7 +467,7 @@ int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_res
     return VC1ME_OK;
 }
 
-int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
+int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* nn_out = nullptr, float nn_logit = 0.f) {
     if (n <= 0) return VC1ME_OK;
     if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
     if (n > 255) return fail(VC1ME_ELIMIT, "at most 255 jobs per step");
@@ -473,6 +489,15 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
         e = ensure_phase_alloc(jobs[i].ref, p->mode);
         if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("phase plane allocation: ") + hipGetErrorString(e));
         fill_job(ctx->h_jobs[i], p, jobs[i].cur, jobs[i].ref, ctx->slots[i]);
+        if (nn_out) {
+            ctx->h_jobs[i].nn_out = nn_out[i];
+            ctx->h_jobs[i].nn_logit = nn_logit;
+        }
+        if (ctx->cost_report) {
+            ctx->h_jobs[i].cost20 = ctx->slots[i].cost20;
+            ctx->h_jobs[i].one_mv = ctx->slots[i].one_mv;
+        }
+        ctx->slots[i].mbs_last = static_cast<int>(mbs);
         max_local = std::max(max_local, ctx->h_jobs[i].local_range);
         max_diag = std::max(max_diag, ctx->h_jobs[i].max_diag);
         if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range) max_cells = std::max(max_cells, cells);
@@ -489,7 +514,12 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
     for (int i = 0; i < n; ++i) {
         HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
         HIP_TRY(hipMemsetAsync(ctx->slots[i].ops, 0, 8 * sizeof(unsigned long long), st));
+        if (ctx->cost_report) {
+            HIP_TRY(hipMemsetAsync(ctx->slots[i].cost20, 0, sizeof(uint64_t) * ctx->slots[i].mbs_last, st));
+            HIP_TRY(hipMemsetAsync(ctx->slots[i].one_mv, 0, sizeof(int2) * ctx->slots[i].mbs_last, st));
+        }
     }
+    ctx->last_jobs = n;
     HIP_TRY(hipEventRecord(ctx->events[0], st));
     // interpolated references, once per distinct (frame, filter) of the step
     {
@@ -570,6 +600,138 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {
     return VC1ME_OK;
 }
 
+
+// ------------------------------------------------------------------ proposal network
+
+struct NnBuffers {
+    int nb = 0, h4 = 0, w4 = 0;       // capacity
+    uint16_t* x0 = nullptr;            // [nb][h4][w4][2]
+    uint16_t* act[pnet::kLayers] = {};
+    uint16_t* dz[2] = {};
+    float* pooled = nullptr;
+    float* out = nullptr;
+    float* dout = nullptr;
+    float* tgt = nullptr;
+    uint8_t* valid = nullptr;
+    float* in32 = nullptr;
+    float* loss = nullptr;
+};
+
+}  // namespace
+
+struct vc1me_nn {
+    vc1me_ctx* ctx = nullptr;
+    float* p = nullptr;      // fp32 master parameters
+    float* g = nullptr;      // gradients
+    float* m = nullptr;      // Adam moments
+    float* v = nullptr;
+    int step = 0;
+    uint16_t* wt[pnet::kLayers] = {};    // bf16 [co][9][ci]
+    uint16_t* wtt[pnet::kLayers] = {};   // bf16 [9][ci][co] (layers with cin 32)
+    bool packed = false;
+    NnBuffers b;
+    std::vector<float*> props;               // per job of a learned step: 5 floats per MB
+    std::vector<size_t> props_cap;
+};
+
+namespace {
+
+int nn_dims(int W, int H, int& h4, int& w4, int& mbh, int& mbw) {
+    h4 = (H + 3) / 4; w4 = (W + 3) / 4;
+    mbh = (h4 + 3) / 4; mbw = (w4 + 3) / 4;
+    return h4 * w4;
+}
+
+hipError_t nn_reserve(vc1me_nn* nn, int nb, int h4, int w4) {
+    NnBuffers& b = nn->b;
+    if (nb <= b.nb && h4 * w4 <= b.h4 * b.w4 && b.x0) return hipSuccess;
+    auto rel = [](auto*& q) { if (q) (void)hipFree(q); q = nullptr; };
+    rel(b.x0); for (auto& a : b.act) rel(a); rel(b.dz[0]); rel(b.dz[1]); rel(b.pooled); rel(b.out); rel(b.dout);
+    rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss);
+    const size_t px = static_cast<size_t>(nb) * h4 * w4;
+    const size_t mbs = static_cast<size_t>(nb) * ((h4 + 3) / 4) * ((w4 + 3) / 4);
+    hipError_t e = hipMalloc(&b.x0, px * 2 * 2);
+    for (auto& a : b.act) if (e == hipSuccess) e = hipMalloc(&a, px * 32 * 2);
+    if (e == hipSuccess) e = hipMalloc(&b.dz[0], px * 32 * 2);
+    if (e == hipSuccess) e = hipMalloc(&b.dz[1], px * 32 * 2);
+    if (e == hipSuccess) e = hipMalloc(&b.pooled, mbs * 32 * 4);
+    if (e == hipSuccess) e = hipMalloc(&b.out, mbs * 5 * 4);
+    if (e == hipSuccess) e = hipMalloc(&b.dout, mbs * 5 * 4);
+    if (e == hipSuccess) e = hipMalloc(&b.tgt, mbs * 2 * 4);
+    if (e == hipSuccess) e = hipMalloc(&b.valid, mbs);
+    if (e == hipSuccess) e = hipMalloc(&b.in32, px * 2 * 4);
+    if (e == hipSuccess) e = hipMalloc(&b.loss, 4);
+    if (e != hipSuccess) return e;
+    b.nb = nb; b.h4 = h4; b.w4 = w4;
+    return hipSuccess;
+}
+
+hipError_t nn_pack(vc1me_nn* nn, hipStream_t st) {
+    if (nn->packed) return hipSuccess;
+    for (int l = 0; l < pnet::kLayers; ++l) {
+        const hipError_t e = pnet::launch_pack_weights(nn->p + pnet::layer_offset(l), nn->wt[l],
+                                                            l ? nn->wtt[l] : nullptr, pnet::layer_cin(l), st);
+        if (e != hipSuccess) return e;
+    }
+    nn->packed = true;
+    return hipSuccess;
+}
+
+// forward of nb images already in b.x0; outputs in b.out
+hipError_t nn_forward(vc1me_nn* nn, int nb, int h4, int w4, hipStream_t st) {
+    hipError_t e = nn_pack(nn, st);
+    const uint16_t* in = nn->b.x0;
+    for (int l = 0; l < pnet::kLayers && e == hipSuccess; ++l) {
+        const int cin = pnet::layer_cin(l);
+        const float* bias = nn->p + pnet::layer_offset(l) + 32 * 9 * cin;
+        e = pnet::launch_conv_fwd(in, nn->wt[l], bias, nn->b.act[l], nb, h4, w4, cin, pnet::layer_dil(l), st);
+        in = nn->b.act[l];
+    }
+    if (e == hipSuccess)
+        e = pnet::launch_pool_head(nn->b.act[pnet::kLayers - 1], nn->p + pnet::head_offset(), nn->b.pooled,
+                                       nn->b.out, nb, h4, w4, (h4 + 3) / 4, (w4 + 3) / 4, st);
+    return e;
+}
+
+// backward from b.dout into nn->g (zeroed here)
+hipError_t nn_backward(vc1me_nn* nn, int nb, int h4, int w4, hipStream_t st) {
+    using namespace pnet;
+    hipError_t e = hipMemsetAsync(nn->g, 0, sizeof(float) * kParams, st);
+    if (e == hipSuccess)
+        e = launch_pool_head_bwd(nn->b.dout, nn->b.pooled, nn->p + head_offset(), nn->b.act[kLayers - 1],
+                                 nn->g + head_offset(), nn->b.dz[0], nb, h4, w4, (h4 + 3) / 4, (w4 + 3) / 4, st);
+    int cur = 0;
+    for (int l = kLayers - 1; l >= 0 && e == hipSuccess; --l) {
+        const int cin = layer_cin(l);
+        const uint16_t* xin = l ? nn->b.act[l - 1] : nn->b.x0;
+        float* dw = nn->g + layer_offset(l);
+        e = launch_conv_bwd_weight(nn->b.dz[cur], xin, dw, dw + 32 * 9 * cin, nb, h4, w4, cin, layer_dil(l), st);
+        if (e == hipSuccess && l > 0) {
+            e = launch_conv_bwd_data(nn->b.dz[cur], nn->wtt[l], nn->b.act[l - 1], nn->b.dz[1 - cur], nb, h4, w4,
+                                     layer_dil(l), st);
+            cur = 1 - cur;
+        }
+    }
+    return e;
+}
+
+// downsampled inputs of n frame pairs of one raster into b.x0
+int nn_inputs(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs, int n, int& h4, int& w4,
+              int& mbh, int& mbw) {
+    if (!curs || !refs || n <= 0) return fail(VC1ME_EINVAL, "no frames");
+    const int W = curs[0]->w, H = curs[0]->h;
+    for (int i = 0; i < n; ++i)
+        if (!curs[i] || !refs[i] || curs[i]->w != W || refs[i]->w != W || curs[i]->h != H || refs[i]->h != H)
+            return fail(VC1ME_EINVAL, "frames of one batch must share one raster");
+    nn_dims(W, H, h4, w4, mbh, mbw);
+    hipStream_t st = nn->ctx->stream;
+    HIP_TRY(nn_reserve(nn, n, h4, w4));
+    for (int i = 0; i < n; ++i)
+        HIP_TRY(pnet::launch_downsample(curs[i]->y, refs[i]->y, W, H, h4, w4,
+                                            nn->b.x0 + static_cast<size_t>(i) * h4 * w4 * 2, st));
+    return VC1ME_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -631,6 +793,7 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
     if (ctx->d_cands) (void)hipFree(ctx->d_cands);
     if (ctx->d_costs) (void)hipFree(ctx->d_costs);
+    if (ctx->d_props) (void)hipFree(ctx->d_props);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -805,6 +968,25 @@ int vc1me_set_profiling(vc1me_ctx* ctx, int on) {
     return VC1ME_OK;
 }
 
+int vc1me_set_cost_report(vc1me_ctx* ctx, int on) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->cost_report = on != 0;
+    return VC1ME_OK;
+}
+
+int vc1me_last_mb_costs(vc1me_ctx* ctx, int job, uint64_t* cost20, int32_t* one_mvs) {
+    if (!ctx || !cost20) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!ctx->cost_report) return fail(VC1ME_EINVAL, "cost report is off");
+    if (job < 0 || job >= ctx->last_jobs) return fail(VC1ME_EINVAL, "no such job in the last step");
+    if (int rc = set_device(ctx)) return rc;
+    const JobSlot& s = ctx->slots[job];
+    HIP_TRY(hipMemcpy(cost20, s.cost20, sizeof(uint64_t) * s.mbs_last, hipMemcpyDeviceToHost));
+    if (one_mvs) HIP_TRY(hipMemcpy(one_mvs, s.one_mv, sizeof(int2) * s.mbs_last, hipMemcpyDeviceToHost));
+    return VC1ME_OK;
+}
+
 int vc1me_counters_get(vc1me_ctx* ctx, vc1me_counters* out) {
     if (!ctx || !out) return fail(VC1ME_EINVAL, "null argument");
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -816,6 +998,205 @@ void vc1me_counters_reset(vc1me_ctx* ctx) {
     if (!ctx) return;
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->counters = vc1me_counters{};
+}
+
+
+int vc1me_nn_param_count(void) { return pnet::kParams; }
+
+int vc1me_nn_create(vc1me_ctx* ctx, vc1me_nn** out) {
+    if (!ctx || !out) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    vc1me_nn* nn = new vc1me_nn();
+    nn->ctx = ctx;
+    const size_t pb = sizeof(float) * pnet::kParams;
+    hipError_t e = hipMalloc(&nn->p, pb);
+    if (e == hipSuccess) e = hipMalloc(&nn->g, pb);
+    if (e == hipSuccess) e = hipMalloc(&nn->m, pb);
+    if (e == hipSuccess) e = hipMalloc(&nn->v, pb);
+    for (int l = 0; l < pnet::kLayers && e == hipSuccess; ++l) {
+        e = hipMalloc(&nn->wt[l], 2 * 32 * 9 * pnet::layer_cin(l));
+        if (e == hipSuccess && l) e = hipMalloc(&nn->wtt[l], 2 * 32 * 9 * 32);
+    }
+    if (e == hipSuccess) e = hipMemset(nn->p, 0, pb);
+    if (e == hipSuccess) e = hipMemset(nn->m, 0, pb);
+    if (e == hipSuccess) e = hipMemset(nn->v, 0, pb);
+    if (e != hipSuccess) {
+        vc1me_nn_destroy(nn);
+        return fail(VC1ME_ENOMEM, std::string("network allocation: ") + hipGetErrorString(e));
+    }
+    *out = nn;
+    return VC1ME_OK;
+}
+
+void vc1me_nn_destroy(vc1me_nn* nn) {
+    if (!nn) return;
+    if (nn->ctx) {
+        std::lock_guard<std::mutex> lock(nn->ctx->mu);
+        (void)hipStreamSynchronize(nn->ctx->stream);
+    }
+    auto rel = [](auto*& q) { if (q) (void)hipFree(q); q = nullptr; };
+    rel(nn->p); rel(nn->g); rel(nn->m); rel(nn->v);
+    for (int l = 0; l < pnet::kLayers; ++l) { rel(nn->wt[l]); rel(nn->wtt[l]); }
+    NnBuffers& b = nn->b;
+    rel(b.x0); for (auto& a : b.act) rel(a); rel(b.dz[0]); rel(b.dz[1]); rel(b.pooled); rel(b.out); rel(b.dout);
+    rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss);
+    for (auto& q : nn->props) rel(q);
+    delete nn;
+}
+
+int vc1me_nn_set_params(vc1me_nn* nn, const float* params) {
+    if (!nn || !params) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(nn->ctx->mu);
+    if (int rc = set_device(nn->ctx)) return rc;
+    const size_t pb = sizeof(float) * pnet::kParams;
+    HIP_TRY(hipMemcpy(nn->p, params, pb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(nn->m, 0, pb));
+    HIP_TRY(hipMemset(nn->v, 0, pb));
+    nn->step = 0;
+    nn->packed = false;
+    return VC1ME_OK;
+}
+
+int vc1me_nn_get_params(vc1me_nn* nn, float* params) {
+    if (!nn || !params) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(nn->ctx->mu);
+    if (int rc = set_device(nn->ctx)) return rc;
+    HIP_TRY(hipStreamSynchronize(nn->ctx->stream));
+    HIP_TRY(hipMemcpy(params, nn->p, sizeof(float) * pnet::kParams, hipMemcpyDeviceToHost));
+    return VC1ME_OK;
+}
+
+int vc1me_nn_forward_frames(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs, int n,
+                            float* out) {
+    if (!nn || !out) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(nn->ctx->mu);
+    if (int rc = set_device(nn->ctx)) return rc;
+    int h4, w4, mbh, mbw;
+    if (int rc = nn_inputs(nn, curs, refs, n, h4, w4, mbh, mbw)) return rc;
+    hipStream_t st = nn->ctx->stream;
+    HIP_TRY(nn_forward(nn, n, h4, w4, st));
+    HIP_TRY(hipMemcpyAsync(out, nn->b.out, sizeof(float) * 5 * n * mbh * mbw, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return VC1ME_OK;
+}
+
+int vc1me_nn_forward_backward(vc1me_nn* nn, const float* input, int n, int h4, int w4, const float* dout, float* out,
+                              float* grads) {
+    if (!nn || !input || !dout || !out || !grads || n <= 0 || h4 <= 0 || w4 <= 0)
+        return fail(VC1ME_EINVAL, "bad argument");
+    std::lock_guard<std::mutex> lock(nn->ctx->mu);
+    if (int rc = set_device(nn->ctx)) return rc;
+    hipStream_t st = nn->ctx->stream;
+    const int mbh = (h4 + 3) / 4, mbw = (w4 + 3) / 4;
+    const size_t ni = static_cast<size_t>(n) * h4 * w4 * 2, no = static_cast<size_t>(n) * mbh * mbw * 5;
+    HIP_TRY(nn_reserve(nn, n, h4, w4));
+    HIP_TRY(hipMemcpyAsync(nn->b.in32, input, ni * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(pnet::launch_to_bf16(nn->b.in32, nn->b.x0, static_cast<int>(ni), st));
+    HIP_TRY(nn_forward(nn, n, h4, w4, st));
+    HIP_TRY(hipMemcpyAsync(nn->b.dout, dout, no * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(nn_backward(nn, n, h4, w4, st));
+    HIP_TRY(hipMemcpyAsync(out, nn->b.out, no * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(grads, nn->g, sizeof(float) * pnet::kParams, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return VC1ME_OK;
+}
+
+int vc1me_nn_train_step(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs,
+                        const int32_t* targets, const uint8_t* valid, int n, float lr, float* loss) {
+    if (!nn || !targets || !valid || !loss) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(nn->ctx->mu);
+    if (int rc = set_device(nn->ctx)) return rc;
+    int h4, w4, mbh, mbw;
+    if (int rc = nn_inputs(nn, curs, refs, n, h4, w4, mbh, mbw)) return rc;
+    hipStream_t st = nn->ctx->stream;
+    const int mbs = n * mbh * mbw;
+    std::vector<float> t(static_cast<size_t>(mbs) * 2);
+    int count = 0;
+    for (int i = 0; i < mbs; ++i) {
+        t[2 * i] = static_cast<float>(targets[2 * i]) / 256.0f;       // quarter-pel -> 64-px units
+        t[2 * i + 1] = static_cast<float>(targets[2 * i + 1]) / 256.0f;
+        count += valid[i] ? 1 : 0;
+    }
+    HIP_TRY(hipMemcpyAsync(nn->b.tgt, t.data(), t.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(nn->b.valid, valid, mbs, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(nn->b.loss, 0, 4, st));
+    HIP_TRY(nn_forward(nn, n, h4, w4, st));
+    HIP_TRY(pnet::launch_loss(nn->b.out, nn->b.tgt, nn->b.valid, nn->b.dout, nn->b.loss, mbs,
+                                  count ? 1.0f / static_cast<float>(count) : 0.f, st));
+    HIP_TRY(nn_backward(nn, n, h4, w4, st));
+    ++nn->step;
+    const float b1 = 0.9f, b2 = 0.999f;
+    HIP_TRY(pnet::launch_adam(nn->p, nn->g, nn->m, nn->v, pnet::kParams, lr, b1, b2,
+                                  1.0f - std::pow(b1, static_cast<float>(nn->step)),
+                                  1.0f - std::pow(b2, static_cast<float>(nn->step)), 1e-8f, st));
+    nn->packed = false;
+    HIP_TRY(hipMemcpyAsync(loss, nn->b.loss, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return VC1ME_OK;
+}
+
+int vc1me_analyze_step_proposals(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* proposals,
+                                 float threshold) {
+    if (!ctx || !proposals) return fail(VC1ME_EINVAL, "null argument");
+    if (!(threshold > 0.f && threshold < 1.f)) return fail(VC1ME_EINVAL, "threshold must lie in (0, 1)");
+    if (n <= 0) return VC1ME_OK;
+    if (!jobs || n > 255) return fail(VC1ME_EINVAL, "bad job list");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    size_t total = 0;
+    std::vector<size_t> off(n);
+    for (int i = 0; i < n; ++i) {
+        if (!jobs[i].params || !proposals[i]) return fail(VC1ME_EINVAL, "null job field");
+        off[i] = total;
+        total += static_cast<size_t>((jobs[i].params->width + 15) / 16) * ((jobs[i].params->height + 15) / 16) * 5;
+    }
+    if (ctx->props_cap < total) {
+        if (ctx->d_props) (void)hipFree(ctx->d_props);
+        ctx->d_props = nullptr;
+        ctx->props_cap = 0;
+        HIP_TRY(hipMalloc(&ctx->d_props, total * sizeof(float)));
+        ctx->props_cap = total;
+    }
+    std::vector<const float*> dp(n);
+    for (int i = 0; i < n; ++i) {
+        const size_t cnt = (i + 1 < n ? off[i + 1] : total) - off[i];
+        HIP_TRY(hipMemcpyAsync(ctx->d_props + off[i], proposals[i], cnt * sizeof(float), hipMemcpyHostToDevice,
+                               ctx->stream));
+        dp[i] = ctx->d_props + off[i];
+    }
+    const float logit = std::log(threshold / (1.0f - threshold));
+    return run_jobs(ctx, jobs, n, dp.data(), logit);
+}
+
+int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jobs, int n, float threshold) {
+    if (!ctx || !nn || nn->ctx != ctx) return fail(VC1ME_EINVAL, "network and context do not match");
+    if (!(threshold > 0.f && threshold < 1.f)) return fail(VC1ME_EINVAL, "threshold must lie in (0, 1)");
+    if (n <= 0) return VC1ME_OK;
+    if (!jobs || n > 255) return fail(VC1ME_EINVAL, "bad job list");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    hipStream_t st = ctx->stream;
+    if (nn->props.size() < static_cast<size_t>(n)) { nn->props.resize(n, nullptr); nn->props_cap.resize(n, 0); }
+    for (int i = 0; i < n; ++i) {
+        if (!jobs[i].cur || !jobs[i].ref) return fail(VC1ME_EINVAL, "null frame");
+        int h4, w4, mbh, mbw;
+        const vc1me_frame* c = jobs[i].cur;
+        const vc1me_frame* r = jobs[i].ref;
+        if (int rc = nn_inputs(nn, &c, &r, 1, h4, w4, mbh, mbw)) return rc;
+        HIP_TRY(nn_forward(nn, 1, h4, w4, st));
+        const size_t need = static_cast<size_t>(mbh) * mbw * 5;
+        if (nn->props_cap[i] < need) {
+            if (nn->props[i]) (void)hipFree(nn->props[i]);
+            nn->props[i] = nullptr;
+            nn->props_cap[i] = 0;
+            HIP_TRY(hipMalloc(&nn->props[i], need * sizeof(float)));
+            nn->props_cap[i] = need;
+        }
+        HIP_TRY(hipMemcpyAsync(nn->props[i], nn->b.out, need * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    const float logit = std::log(threshold / (1.0f - threshold));
+    return run_jobs(ctx, jobs, n, nn->props.data(), logit);
 }
 
 }  // extern "C"

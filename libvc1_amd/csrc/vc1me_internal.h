@@ -63,6 +63,15 @@ struct JobDesc {
     const uint8_t* qp;
     const uint8_t* bp;
     int32_t pp, ph;
+    // learned seeding: proposals per macroblock (5 floats: two candidates in
+    // 64-px units, confidence logit) or null, and the logit threshold
+    const float* nn_out;
+    float nn_logit;
+    // cost report (null = off): per macroblock 20 * (SAD + lambda * R) of the
+    // final 1-MV vector and that vector; lam20 = 20 * lambda = max(20, 7 * pq)
+    uint64_t* cost20;
+    int2* one_mv;
+    int32_t lam20;
 };
 
 // Phase-plane geometry of a W x H reference: sample (px, py), px in [-2, W],

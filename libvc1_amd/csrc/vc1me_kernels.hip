@@ -1214,6 +1214,81 @@ __device__ __forceinline__ uint64_t far_batch(Ctx& C, int n, int stride, int key
     return int_batch(C, n, stride, nullptr, keymode, seq0, pt);
 }
 
+// ------------------------------------------------------------------ learned seeding
+
+// Full SADs of up to four integer vectors at once, read straight from the
+// reference frame: lane = (candidate, row).  Hidden-padding vectors cost
+// kBadSad.  Returns the minimum key (cost, |d|, dy, dx) over the n vectors
+// held in lanes 0..n-1 of (vx, vy).
+__device__ __forceinline__ uint64_t global_sad_min(Ctx& C, int vx, int vy, int n) {
+    const int W = C.g.W, H = C.g.H;
+    const gptr_u8 ry = as_global(C.ry);
+    uint64_t best = kNone;
+    for (int base = 0; base < n; base += 4) {
+        const int c = base + (C.lane >> 4), r = C.lane & 15;
+        const bool mine = c < n;
+        const int dx = __shfl(vx, mine ? c : 0), dy = __shfl(vy, mine ? c : 0);
+        const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
+        uint32_t sum = 0;
+        if (mine && !hid && r < C.ah) {
+            const gptr_u8 row = ry + static_cast<size_t>(clampi(C.y0 + r + dy, 0, H - 1)) * W;
+            const int x = C.x0 + dx;
+            const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + r * 16);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v |= static_cast<uint32_t>(row[clampi(x + q * 4 + k, 0, W - 1)]) << (8 * k);
+                uint32_t m = 0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (q * 4 + k < C.aw) m |= 0xffu << (8 * k);
+                sum = __builtin_amdgcn_sad_u8(v & m, cr[q] & m, sum);
+            }
+        }
+        sum = group_sum32(sum, 16);
+        if (mine && !hid && r == 0) C.ln_sad += static_cast<uint32_t>(C.aw * C.ah);
+        const uint64_t key = (mine && r == 0) ? key_int(hid ? kBadSad : static_cast<uint64_t>(sum), dx, dy) : kNone;
+        const uint64_t mk = wave_min64(key);
+        best = mk < best ? mk : best;
+    }
+    return best;
+}
+
+// Integer search seeded by the learned proposals (no reference counterpart):
+// full SADs of the two proposals, the predictor, zero and the global vector,
+// then a unit diamond from the best until the centre wins (or 32 steps).
+__device__ __forceinline__ IRes seeded_search(Ctx& C, const Pred& pi, int gdx, int gdy, int range, const float* o) {
+    const Bounds b = bounds_px(range);
+    int vx = 0, vy = 0;
+    {
+        const int k = C.lane;
+        float fx = 0.f, fy = 0.f;
+        if (k == 0) { fx = o[0] * 64.f; fy = o[1] * 64.f; }
+        if (k == 1) { fx = o[2] * 64.f; fy = o[3] * 64.f; }
+        if (k < 2) {
+            vx = static_cast<int>(rintf(fminf(fmaxf(fx, -4096.f), 4096.f)));
+            vy = static_cast<int>(rintf(fminf(fmaxf(fy, -4096.f), 4096.f)));
+        }
+        if (k == 2) { vx = round_qpel_to_pixel(pi.px); vy = round_qpel_to_pixel(pi.py); }
+        if (k == 4) { vx = gdx; vy = gdy; }
+        vx = clampi(vx, b.xmin, b.xmax);
+        vy = clampi(vy, b.ymin, b.ymax);
+    }
+    uint64_t best = global_sad_min(C, vx, vy, 5);
+    for (int it = 0; it < 32; ++it) {
+        const int cx = key_dx(best), cy = key_dy(best);
+        if (!win_load(C, C.x0 + cx - 1, C.y0 + cy - 1, 18, 18)) { set_err(C, 11); break; }
+        const uint64_t m = int_batch(C, 4, 1, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            dx = clampi(cx + (i == 0 ? -1 : i == 1 ? 1 : 0), b.xmin, b.xmax);
+            dy = clampi(cy + (i == 2 ? -1 : i == 3 ? 1 : 0), b.ymin, b.ymax);
+        });
+        if (m >= best) break;
+        best = m;
+    }
+    return {key_dx(best), key_dy(best), key_cost(best)};
+}
+
 // propagated_long_range_motion, src/encoder_motion.cpp:1658-1694
 __device__ __forceinline__ IRes propagated(Ctx& C, int range) {
     const JobDesc* J = C.J;
@@ -1894,20 +1969,23 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
         const int gdx = J->g->dx, gdy = J->g->dy;
         IRes im;
         bool selected_distant = false;
+        // learned seeding: this macroblock's proposals, used when confident
+        const float* nn = J->nn_out ? J->nn_out + static_cast<size_t>(pos) * 5 : nullptr;
+        const bool seeded = nn && nn[4] >= J->nn_logit;
         const uint64_t px = active_pixels(C);
         if (J->lr_mode == 2) {
             bool sc = false;
-            if (range > local_range) {
+            if (range > local_range && !seeded) {
                 const IRes pr = propagated(C, range);
                 if (match_good(C, pr.sad)) { im = pr; sc = true; selected_distant = true; }
             }
-            if (!sc) im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
+            if (!sc) im = seeded ? seeded_search(C, pi, gdx, gdy, range, nn) : J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
         } else {
-            im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
+            im = seeded ? seeded_search(C, pi, gdx, gdy, range, nn) : J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
             const uint64_t local_sad = im.sad;
             if (px) dl_mae = static_cast<double>(local_sad) / static_cast<double>(px);
             const bool skip = J->lr_mode == 1 && match_good(C, im.sad);
-            if (range > local_range && !skip) {
+            if (range > local_range && !skip && !seeded) {
                 const IRes pr = propagated(C, range);
                 const int dec = distant_decision(C, pr.sad, local_sad);
                 if (px && pr.sad < UINT64_MAX / 8) {
@@ -1921,7 +1999,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
                 }
             }
         }
-        if (range > local_range && !match_good(C, im.sad)) {
+        if (range > local_range && !seeded && !match_good(C, im.sad)) {
             uint64_t ext_sad = UINT64_MAX;
             int ext_dec = 0;
             const IRes ext = extended_content(C, im, &ext_sad, &ext_dec);
@@ -1939,6 +2017,29 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
     if (hidden_padding(C.g, C.x0, C.y0, 16, 16, one.x, one.y, one_mode, true)) {
         one = make_int2(0, 0);
         one_sad = lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, 0, 0, true, 0, 1, one_mode, 0, 0), 0);
+    }
+    if (J->cost20) {
+        // judge of the learned form: 20 * (SAD + lambda * R) of the final 1-MV
+        // vector against the rate predictor (the rate-winner score of
+        // src/encoder_motion.cpp:3459-3471), 20 * lambda = lam20; work
+        // counters left as the search alone would leave them
+        const uint32_t s0 = C.ln_sad, s1 = C.ln_taps, s2 = C.ln_satd;
+        const uint64_t s = lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, one.x, one.y, true, 0, 1, one_mode, 0, 0), 0);
+        C.ln_sad = s0; C.ln_taps = s1; C.ln_satd = s2;
+        uint32_t r = 0;
+        int prx = rate_pi.px, pry = rate_pi.py;
+        if (rate_pi.hybrid) {
+            const bool use_a = modular_mv_distance(J->mvrange, one.x, one.y, rate_pi.ax, rate_pi.ay) <=
+                               modular_mv_distance(J->mvrange, one.x, one.y, rate_pi.cx, rate_pi.cy);
+            prx = use_a ? rate_pi.ax : rate_pi.cx;
+            pry = use_a ? rate_pi.ay : rate_pi.cy;
+            ++r;
+        }
+        r += mv_rate_bits(J->mvrange, one.x, one.y, prx, pry, one_mode, true);
+        if (C.lane == 0) {
+            J->cost20[pos] = 20u * s + static_cast<uint64_t>(J->lam20) * r;
+            J->one_mv[pos] = one;
+        }
     }
     const uint64_t one_screen = one_sad;
     int2 outmv = one;
@@ -2137,6 +2238,8 @@ __global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ job
     if (J->range <= J->local_range) return;
     const int pos = static_cast<int>(blockIdx.x);
     if (pos >= J->mbw * J->mbh) return;
+    // macroblocks the learned seeding covers do not search long range
+    if (J->nn_out && J->nn_out[static_cast<size_t>(pos) * 5 + 4] >= J->nn_logit) return;
     Ctx C;
     init_ctx(C, J, pos % J->mbw, pos / J->mbw, smem);
     for (int i = C.lane; i < 256; i += kWave) {

@@ -1693,3 +1693,33 @@ void oracle_integer_search(const vc1me_params* p, const vc1me_planes* cur, const
     const imr_t r = integer_local(&c, mx, my, &pi, g, range);
     *dx = r.dx; *dy = r.dy; *sad = r.sad;
 }
+
+/* Per-macroblock judge of a finished analysis: 20 * (SAD + lambda * R) of the
+ * final 1-MV vector one_mvs[pos] -- block_sad_motion at that vector, R its
+ * rate against predictor_info_mixed_1mv of the final block field and intra
+ * flags, HYBRIDPRED bit included, lambda = max(1, 0.35 * PQINDEX).  This is
+ * the rate-winner score of src/encoder_motion.cpp:3458-3471 (the same terms
+ * refine_staged above forms per candidate) evaluated once per macroblock;
+ * 20 * lambda = max(20, 7 * PQINDEX) keeps it an integer. */
+void oracle_mb_costs20(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                       const int32_t* one_mvs, const int32_t* block_mvs, const uint8_t* use_intra, uint64_t* out) {
+    ctx_t c;
+    init_ctx(&c, p, cur, ref, NULL);
+    const int one_mode = p->mode == VC1ME_MODE_MIXED_MV ? VC1ME_MODE_ONE_MV_QPEL : p->mode;
+    const uint64_t lam20 = (uint64_t)imax(20, 7 * p->pqindex);
+    const mv_t* field = (const mv_t*)block_mvs;
+    for (int my = 0; my < c.mbh; ++my)
+        for (int mx = 0; mx < c.mbw; ++mx) {
+            const size_t pos = (size_t)my * c.mbw + mx;
+            const mv_t one = {one_mvs[2 * pos], one_mvs[2 * pos + 1]};
+            const pred_t pi = predictor_info_mixed_1mv(&c, field, use_intra, mx, my);
+            mv_t pred = pi.pre;
+            uint64_t rate = 0;
+            if (pi.hybrid) {
+                pred = modular_mv_distance(&c, one, pi.a) <= modular_mv_distance(&c, one, pi.c) ? pi.a : pi.c;
+                ++rate;
+            }
+            rate += mv_rate_bits(&c, one, pred, one_mode, true);
+            out[pos] = 20u * block_sad_motion(&c, mx * 16, my * 16, 16, 16, one, one_mode) + lam20 * rate;
+        }
+}

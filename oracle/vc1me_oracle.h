@@ -60,6 +60,11 @@ uint64_t oracle_frame_sad(const vc1me_params* p, const vc1me_planes* cur, const 
 void oracle_global_motion(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
                           int range, int32_t* dx, int32_t* dy, uint64_t* sad);
 /* MV rate in bits: motion_vector_rate_bits (src/encoder_motion.cpp:3214-3225). */
+/* 20 * (SAD + lambda * R) per macroblock of a finished analysis at the given
+ * 1-MV vectors (mbs x 2), with the result's block vectors (mbs x 4 x 2) and
+ * intra flags as the predictor field. */
+void oracle_mb_costs20(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                       const int32_t* one_mvs, const int32_t* block_mvs, const uint8_t* use_intra, uint64_t* out);
 int oracle_mv_rate_bits(const vc1me_params* p, int dxq, int dyq, int pxq, int pyq, int mode, int more);
 /* x87 long-double score compare used by the staged sub-pel finalists:
  * returns -1/0/1 for (sad_a + lambda*rate_a) <=> (sad_b + lambda*rate_b)

@@ -51,6 +51,9 @@ def oracle() -> C.CDLL:
         lib.oracle_distant_decision.argtypes = [P(abi.Params), C.c_uint64, C.c_uint64, C.c_int, C.c_int]
         lib.oracle_integer_search.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_int] * 7 + [P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
         lib.oracle_integer_search.restype = None
+        lib.oracle_mb_costs20.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(C.c_int32), P(C.c_int32),
+                                          P(C.c_uint8), P(C.c_uint64)]
+        lib.oracle_mb_costs20.restype = None
         _oracle = lib
     return _oracle
 
@@ -108,6 +111,20 @@ def oracle_score_batch(cur: np.ndarray, ref: np.ndarray, x0, y0, bw, bh, cands, 
     out = np.zeros(len(cands), np.uint64)
     oracle().oracle_score_batch(cur.ctypes.data, ref.ctypes.data, w, h, x0, y0, bw, bh, stride, bilinear, rnd, metric,
                                 cands.ctypes.data_as(P(abi.Candidate)), len(cands), out.ctypes.data_as(P(C.c_uint64)))
+    return out
+
+
+def oracle_mb_costs20(params: abi.Params, cur, ref, one_mvs: np.ndarray, result: abi.ResultArrays) -> np.ndarray:
+    """20 * (SAD + lambda * R) per macroblock at the given 1-MV vectors, with
+    the result's block vectors and intra flags as the predictor field."""
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    one = np.ascontiguousarray(one_mvs, np.int32)
+    bmv = np.ascontiguousarray(result.block_mvs, np.int32)
+    intra = np.ascontiguousarray(result.use_intra, np.uint8)
+    out = np.zeros(result.mbs, np.uint64)
+    oracle().oracle_mb_costs20(C.byref(params), C.byref(c.struct), C.byref(r.struct), one.ctypes.data_as(P(C.c_int32)),
+                               bmv.ctypes.data_as(P(C.c_int32)), intra.ctypes.data_as(P(C.c_uint8)),
+                               out.ctypes.data_as(P(C.c_uint64)))
     return out
 
 

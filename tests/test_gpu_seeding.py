@@ -1,0 +1,223 @@
+"""Learned seeding on the GPU: the proposal network's HIP kernels against the
+CPU fp64 oracle (oracle/proposal_net.py), the cost report against the C
+oracle, and the seeding's contract with the plain search.
+
+Tolerances: the network computes in bf16 with fp32 accumulation.  Against
+the oracle with the same bf16 roundings (quant=True) outputs and gradients
+agree to 1e-2 relative L2 per tensor (only the accumulation order and rare
+one-ulp rounding flips differ).  Against the exact fp64 network outputs agree
+to 1e-2 and gradients to 0.15 relative L2 per tensor (cosine > 0.99): the
+backward signal is rounded to bf16 at every layer, which costs the first
+layer's gradient about 9% on these inputs in the oracle itself.
+"""
+import numpy as np
+import pytest
+
+from libvc1_amd import abi, synth
+from libvc1_amd.backend import ProposalNet
+from oracle import proposal_net as pn
+from _oracle import oracle_analyze, oracle_mb_costs20, describe_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    d = pn.unpack(pn.init_params(seed).astype(np.float64))
+    for l in range(4):
+        d[f"b{l}"] = rng.normal(0, 0.05, 32)
+    d["wh"] = rng.normal(0, 0.3, (5, 32))
+    d["bh"] = rng.normal(0, 0.1, 5)
+    return pn.pack(d).astype(np.float32)
+
+
+@pytest.fixture
+def net(backend):
+    n = ProposalNet(backend)
+    yield n
+    n.close()
+
+
+@pytest.mark.parametrize("shape", [(2, 13, 22), (2, 30, 45), (1, 68, 120)])
+def test_forward_backward_matches_oracle(net, shape):
+    n, h4, w4 = shape
+    _say(f"[case] proposal net forward/backward n={n} h4={h4} w4={w4}")
+    rng = np.random.default_rng(h4 * 1000 + w4)
+    params = _params(7)
+    net.set_params(params)
+    x = rng.uniform(-0.5, 0.5, size=(n, h4, w4, 2)).astype(np.float32)
+    mbh, mbw = (h4 + 3) // 4, (w4 + 3) // 4
+    dout = rng.normal(size=(n, mbh, mbw, 5)).astype(np.float32)
+    out, grads = net.forward_backward(x, dout)
+    p64 = params.astype(np.float64)
+    wq, cq = pn.forward(p64, x, quant=True)
+    gq = pn.backward(cq, dout.astype(np.float64))
+    we, ce = pn.forward(p64, x)
+    ge = pn.backward(ce, dout.astype(np.float64))
+    assert _rel(out, wq) < 1e-2, _rel(out, wq)
+    assert _rel(out, we) < 1e-2, _rel(out, we)
+    lay, _ = pn.layout()
+    for name, off, shp in lay:
+        sz = int(np.prod(shp))
+        g, q, e = grads[off:off + sz], gq[off:off + sz], ge[off:off + sz]
+        assert _rel(g, q) < 1e-2, (name, _rel(g, q))
+        assert _rel(g, e) < 0.15, (name, _rel(g, e))
+        assert np.dot(g, e) > 0.99 * np.linalg.norm(g) * np.linalg.norm(e), name
+
+
+def test_params_round_trip(net):
+    p = _params(3)
+    net.set_params(p)
+    np.testing.assert_array_equal(net.get_params(), p)
+
+
+def _pair(seed, w, h, gap=1):
+    d = synth.pan_rectangles(seed, gap + 1, width=w, height=h, frame_indices=[0, gap])
+    return d[gap], d[0]
+
+
+def test_forward_frames_matches_oracle(backend, net):
+    params = _params(11)
+    net.set_params(params)
+    pics = [_pair(70 + i, 200, 118) for i in range(2)]
+    frames = [(backend.upload(c), backend.upload(r)) for c, r in pics]
+    _say("[case] proposal net forward on frames 200x118")
+    try:
+        got = net.forward_frames(frames)
+    finally:
+        for a, b in frames:
+            a.release()
+            b.release()
+    x = np.stack([pn.downsample(c.y, r.y) for c, r in pics])
+    want, _ = pn.forward(params.astype(np.float64), x, quant=True)
+    assert got.shape == want.shape == (2, 8, 13, 5)
+    assert _rel(got, want) < 1e-2
+
+
+def test_train_step_loss_and_update(backend, net):
+    params = _params(5)
+    net.set_params(params)
+    w, h = 160, 96
+    pics = [_pair(90 + i, w, h) for i in range(2)]
+    mbw, mbh = abi.mb_grid(w, h)
+    rng = np.random.default_rng(1)
+    tgt = rng.integers(-64, 65, size=(2, mbh * mbw, 2)).astype(np.int32)
+    valid = (rng.uniform(size=(2, mbh * mbw)) < 0.9).astype(np.uint8)
+    frames = [(backend.upload(c), backend.upload(r)) for c, r in pics]
+    lr = 1e-3
+    _say("[case] proposal net train step 160x96")
+    try:
+        loss = net.train_step(frames, tgt, valid, lr)
+    finally:
+        for a, b in frames:
+            a.release()
+            b.release()
+    x = np.stack([pn.downsample(c.y, r.y) for c, r in pics])
+    out, cache = pn.forward(params.astype(np.float64), x, quant=True)
+    want_loss, dout = pn.loss(out.reshape(-1, 5), tgt.reshape(-1, 2) / 256.0, valid.reshape(-1).astype(bool))
+    assert loss == pytest.approx(want_loss, rel=1e-2)
+    g = pn.backward(cache, dout.reshape(out.shape))
+    step = net.get_params().astype(np.float64) - params
+    # Adam's first step moves each parameter by about lr against its gradient
+    strong = np.abs(g) > 0.05 * np.sqrt(np.mean(g * g))
+    agree = np.mean(np.sign(step[strong]) == -np.sign(g[strong]))
+    assert agree > 0.99, agree
+    assert np.all(np.abs(step) <= lr * 1.01 + 1e-7)
+
+
+def _job(seed, w, h, gap):
+    cur, ref = _pair(seed, w, h, gap)
+    p = abi.default_params(w, h, pqindex=8, motion_search_range=512, motion_local_search_range=32,
+                           mode=abi.MODE_MIXED_MV)
+    return cur, ref, p
+
+
+def test_cost_report_matches_oracle(backend):
+    cur, ref, p = _job(31, 352, 288, 2)
+    fc, fr = backend.upload(cur), backend.upload(ref)
+    _say("[case] cost report 352x288")
+    try:
+        backend.set_cost_report(True)
+        got = backend.analyze_step([(fc, fr)], [p])[0]
+        cost, one = backend.last_mb_costs(0, got.mbs)
+    finally:
+        backend.set_cost_report(False)
+        fc.release()
+        fr.release()
+    want = oracle_analyze(p, cur, ref)
+    assert not got.diff(want), describe_diff(got, want, p.width)
+    # where 1-MV won, the reported vector is the result's vector
+    one_mv = got.use_4mv == 0
+    np.testing.assert_array_equal(one[one_mv], got.mvs[one_mv])
+    np.testing.assert_array_equal(cost, oracle_mb_costs20(p, cur, ref, one, got))
+
+
+def test_low_confidence_is_the_plain_search(backend, net):
+    """Below the threshold everywhere, the learned step is the plain step."""
+    params = _params(13)
+    d = pn.unpack(params.astype(np.float64))
+    d["wh"][4] = 0.0
+    d["bh"][4] = -30.0
+    net.set_params(pn.pack(d).astype(np.float32))
+    jobs = [_job(50 + i, w, h, 3) for i, (w, h) in enumerate(((352, 288), (320, 176)))]
+    frames = [(backend.upload(c), backend.upload(r)) for c, r, _ in jobs]
+    _say("[case] learned step with no confident macroblock")
+    try:
+        got = backend.analyze_step_learned(net, frames, [p for _, _, p in jobs])
+    finally:
+        for a, b in frames:
+            a.release()
+            b.release()
+    for (c, r, p), g in zip(jobs, got):
+        want = oracle_analyze(p, c, r)
+        assert not g.diff(want), describe_diff(g, want, p.width)
+
+
+def test_seeded_with_plain_vectors_keeps_cost(backend):
+    """Seeds equal to the plain search's own vectors: every macroblock is
+    seeded, the result stays within the search box, and the summed cost does
+    not rise by more than 1%."""
+    cur, ref, p = _job(61, 640, 360, 3)
+    fc, fr = backend.upload(cur), backend.upload(ref)
+    _say("[case] seeded with plain vectors 640x360")
+    try:
+        backend.set_cost_report(True)
+        plain = backend.analyze_step([(fc, fr)], [p])[0]
+        c_plain, one = backend.last_mb_costs(0, plain.mbs)
+        mbw, mbh = abi.mb_grid(p.width, p.height)
+        prop = np.zeros((mbh, mbw, 5), np.float32)
+        prop[..., 0:2] = (one / 256.0).reshape(mbh, mbw, 2)
+        prop[..., 2:4] = prop[..., 0:2]
+        prop[..., 4] = 10.0
+        seeded = backend.analyze_step_proposals([(fc, fr)], [p], [prop])[0]
+        c_seed, _ = backend.last_mb_costs(0, seeded.mbs)
+    finally:
+        backend.set_cost_report(False)
+        fc.release()
+        fr.release()
+    assert np.all(np.abs(seeded.mvs[:, 0]) <= 4 * 512) and np.all(np.abs(seeded.mvs[:, 1]) <= 4 * 256)
+    rise = c_seed.sum() / c_plain.sum() - 1.0
+    _say(f"seeded-with-plain cost rise {rise:+.4%}")
+    assert rise < 0.01
+
+
+def test_proposal_api_rejects_bad_arguments(backend):
+    cur, ref, p = _job(3, 64, 48, 1)
+    fc, fr = backend.upload(cur), backend.upload(ref)
+    try:
+        prop = np.zeros((3, 4, 5), np.float32)
+        with pytest.raises(Exception):
+            backend.analyze_step_proposals([(fc, fr)], [p], [prop], threshold=1.5)
+        with pytest.raises(Exception):
+            backend.last_mb_costs(0, 12)   # cost report is off
+    finally:
+        fc.release()
+        fr.release()
