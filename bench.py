@@ -14,6 +14,15 @@ One step = vc1me_analyze_step over all 16 (cur, ref) pairs with the frames
 already resident in HBM; the MV fields stay on the device (their download is
 timed separately and reported as `value_with_download`).
 
+The learned-seeded form (`learned`) runs the same steps through
+vc1me_analyze_step_learned with the trained weights of
+libvc1_amd/weights/proposal_net.bin (network forward inside the timed step)
+and is judged by the per-macroblock cost report: the summed SAD + lambda*R of
+the final 1-MV vectors against the plain form's, the fraction of macroblocks
+that got worse and the mean excess.  Two ablations take the network's
+contribution away (every macroblock seeded from the global vector alone; the
+network's confidence gate kept but its candidates removed).
+
 Also reported: the roofline of the dominant kernel (the macroblock search,
 k_mb_diagonal) from HIP events recorded around each of its launches inside
 the timed region, and the reference's own multithreaded CPU path timed on
@@ -143,6 +152,70 @@ def device_peak_ops():
     return cus, clk, cus * 64 * clk
 
 
+def learned_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, threshold=0.5):
+    """The learned-seeded form on the same step: timed like the plain form
+    (network forward included), then judged by the cost report against the
+    plain form, with two ablations that remove the network's contribution."""
+    import numpy as np
+    from libvc1_amd import abi, learned
+    from libvc1_amd.backend import ProposalNet
+
+    params = learned.load_weights()
+    net = ProposalNet(be)
+    net.set_params(params)
+    for _ in range(warmup):
+        be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
+    elapsed = time.perf_counter() - t0
+    value = mbs_per_step * steps / elapsed
+    log(f"learned: {elapsed * 1e3 / steps:.1f} ms/step, {value:,.0f} MB/s ({value / plain_value:.3f}x plain)")
+    judged = learned.cost_compare(be, net, jobs, plist, threshold)
+    log(f"learned cost: {judged}")
+
+    # ablations: the same seeded search without the network's candidates
+    mbw, mbh = abi.mb_grid(plist[0].width, plist[0].height)
+    res = be.analyze_step(jobs, plist)
+    glob = np.array([[r.struct.global_dx, r.struct.global_dy] for r in res], np.int32)
+    outs = net.forward_frames(jobs, glob)
+    abl = {}
+    for name, props in (
+            ("no_network_all_seeded", [np.concatenate([np.zeros((mbh, mbw, 4), np.float32),
+                                                       np.full((mbh, mbw, 1), 30.0, np.float32)], axis=2)
+                                       for _ in jobs]),
+            ("network_gate_without_candidates", [np.concatenate([np.zeros((mbh, mbw, 4), np.float32),
+                                                                 o[..., 4:5]], axis=2) for o in outs])):
+        be.analyze_step_proposals(jobs, plist, props, threshold, want_results=False)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            be.analyze_step_proposals(jobs, plist, props, threshold, want_results=False)
+        v = mbs_per_step * steps / (time.perf_counter() - t0)
+        be.set_cost_report(True)
+        try:
+            be.analyze_step(jobs, plist, want_results=False)
+            cp = np.concatenate([be.last_mb_costs(j, mbw * mbh)[0] for j in range(len(jobs))]).astype(np.float64)
+            be.analyze_step_proposals(jobs, plist, props, threshold, want_results=False)
+            ca = np.concatenate([be.last_mb_costs(j, mbw * mbh)[0] for j in range(len(jobs))]).astype(np.float64)
+        finally:
+            be.set_cost_report(False)
+        abl[name] = {"value": round(v, 1), "speedup_vs_plain": round(v / plain_value, 4),
+                     "cost_rise": round(float(ca.sum() / cp.sum() - 1), 6),
+                     "frac_worse": round(float(np.mean(ca > cp)), 6)}
+        log(f"ablation {name}: {abl[name]}")
+    net.close()
+    meta_path = learned.WEIGHTS + ".json"
+    sha = json.load(open(meta_path)).get("sha256") if os.path.exists(meta_path) else None
+    return {
+        "value": round(value, 1), "unit": "MB/s", "speedup_vs_plain": round(value / plain_value, 4),
+        "ms_per_step": round(elapsed * 1e3 / steps, 3), "threshold": threshold,
+        "cost_rise": round(judged["cost_rise"], 6), "frac_worse": round(judged["frac_worse"], 6),
+        "mean_excess_per_mb": round(judged["mean_excess"], 3), "seeded_fraction": round(judged["seeded_fraction"], 4),
+        "cost": "sum over MBs of SAD + lambda*R of the final 1-MV vector (exact integers 20*C), learned vs plain",
+        "weights_sha256": sha, "ablation": abl,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +224,7 @@ def main() -> int:
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -238,6 +312,10 @@ def main() -> int:
                     "traffic": measured_traffic(), "algorithmic_bytes_per_launch": round(alg_bytes_launch),
                     "kernel": "k_mb_diagonal"}
 
+    learned_out = None
+    if not args.no_learned:
+        learned_out = learned_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step)
+
     cpu = None
     if not args.no_cpu:
         log("timing the reference CPU path ...")
@@ -266,6 +344,7 @@ def main() -> int:
         "roofline": roofline,
         "roofline_hbm": roofline_hbm,
         "cpu_baseline": cpu,
+        "learned": learned_out,
     }
     for f in frames.values():
         f.release()

@@ -236,6 +236,7 @@ typedef struct vc1me_work {
     uint64_t satd_ops;
     uint64_t sig_ops;
     uint64_t frame_px;
+    uint64_t seeded_macroblocks;   /* learned forms: macroblocks searched from proposals */
 } vc1me_work;
 int vc1me_last_work(vc1me_ctx* ctx, vc1me_work* out);
 
@@ -264,9 +265,11 @@ void vc1me_nn_destroy(vc1me_nn* nn);
 int vc1me_nn_set_params(vc1me_nn* nn, const float* params);
 int vc1me_nn_get_params(vc1me_nn* nn, float* params);
 /* Forward pass on n frame pairs of one raster; out receives
- * n x mbh x mbw x 5 floats. */
-int vc1me_nn_forward_frames(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs, int n,
-                            float* out);
+ * n x mbh x mbw x 5 floats.  globals (n x 2 integer pixels, or null for
+ * zero) is each pair's global vector: the network sees ref displaced by it
+ * and its two candidates are residuals from it. */
+int vc1me_nn_forward_frames(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs,
+                            const int32_t* globals, int n, float* out);
 /* Forward and backward on a given network input (n x h4 x w4 x 2 floats, as
  * the downsampling would produce) with a given output gradient
  * (n x mbh x mbw x 5, mbh = ceil(h4/4), mbw = ceil(w4/4)); returns the outputs
@@ -275,10 +278,13 @@ int vc1me_nn_forward_backward(vc1me_nn* nn, const float* input, int n, int h4, i
                               float* grads);
 /* One optimizer step (Adam) on n frame pairs of one raster: targets are the
  * plain search's 1-MV vectors (n x mbs x 2, quarter-pel), valid marks the
- * macroblocks that count (n x mbs).  loss receives the batch loss. */
+ * macroblocks that count (n x mbs), globals as for vc1me_nn_forward_frames
+ * (the plain search's global vectors).  loss receives the batch loss. */
 int vc1me_nn_train_step(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs,
-                        const int32_t* targets, const uint8_t* valid, int n, float lr, float* loss);
-/* vc1me_analyze_step with learned seeding: the network runs on every pair;
+                        const int32_t* globals, const int32_t* targets, const uint8_t* valid, int n, float lr,
+                        float* loss);
+/* vc1me_analyze_step with learned seeding: the network runs on every pair
+ * right after the global search (so it sees ref aligned by the global vector);
  * macroblocks whose confidence reaches `threshold` replace the integer search
  * (UMH stages and the long-range stages) by a search seeded with the two
  * proposals plus the predictor, zero and the global vector, ending in a unit
@@ -287,7 +293,8 @@ int vc1me_nn_train_step(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1m
 int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jobs, int n, float threshold);
 /* The same seeding with proposals from the caller: proposals[i] holds
  * mbw x mbh x 5 floats for job i in the network's output layout (two vectors
- * in units of 64 pixels, then a confidence logit). */
+ * in units of 64 pixels relative to the job's global vector, then a
+ * confidence logit). */
 int vc1me_analyze_step_proposals(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* proposals,
                                  float threshold);
 

@@ -89,7 +89,7 @@ class Timing(C.Structure):
 
 class Work(C.Structure):
     _fields_ = [("sad_px", C.c_uint64), ("interp_taps", C.c_uint64), ("satd_ops", C.c_uint64),
-                ("sig_ops", C.c_uint64), ("frame_px", C.c_uint64)]
+                ("sig_ops", C.c_uint64), ("frame_px", C.c_uint64), ("seeded_macroblocks", C.c_uint64)]
 
 
 class Job(C.Structure):

@@ -79,11 +79,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_nn_destroy.restype = None
     lib.vc1me_nn_set_params.argtypes = [C.c_void_p, P(C.c_float)]
     lib.vc1me_nn_get_params.argtypes = [C.c_void_p, P(C.c_float)]
-    lib.vc1me_nn_forward_frames.argtypes = [C.c_void_p, P(C.c_void_p), P(C.c_void_p), C.c_int, P(C.c_float)]
+    lib.vc1me_nn_forward_frames.argtypes = [C.c_void_p, P(C.c_void_p), P(C.c_void_p), P(C.c_int32), C.c_int,
+                                            P(C.c_float)]
     lib.vc1me_nn_forward_backward.argtypes = [C.c_void_p, P(C.c_float), C.c_int, C.c_int, C.c_int, P(C.c_float),
                                               P(C.c_float), P(C.c_float)]
-    lib.vc1me_nn_train_step.argtypes = [C.c_void_p, P(C.c_void_p), P(C.c_void_p), P(C.c_int32), P(C.c_uint8), C.c_int,
-                                        C.c_float, P(C.c_float)]
+    lib.vc1me_nn_train_step.argtypes = [C.c_void_p, P(C.c_void_p), P(C.c_void_p), P(C.c_int32), P(C.c_int32),
+                                        P(C.c_uint8), C.c_int, C.c_float, P(C.c_float)]
     lib.vc1me_analyze_step_learned.argtypes = [C.c_void_p, C.c_void_p, P(abi.Job), C.c_int, C.c_float]
     lib.vc1me_analyze_step_proposals.argtypes = [C.c_void_p, P(abi.Job), C.c_int, P(P(C.c_float)), C.c_float]
     lib.vc1me_set_cost_report.argtypes = [C.c_void_p, C.c_int]
@@ -271,13 +272,22 @@ class ProposalNet:
         h4, w4 = (height + 3) // 4, (width + 3) // 4
         return h4, w4, (h4 + 3) // 4, (w4 + 3) // 4
 
-    def forward_frames(self, pairs) -> np.ndarray:
+    @staticmethod
+    def _globals(globals_, n):
+        if globals_ is None:
+            return None, None
+        g = np.ascontiguousarray(globals_, np.int32).reshape(n, 2)
+        return g, g.ctypes.data_as(C.POINTER(C.c_int32))
+
+    def forward_frames(self, pairs, globals_=None) -> np.ndarray:
+        """Outputs (n, mbh, mbw, 5); globals_ (n, 2) integer pixels or None."""
         n = len(pairs)
         curs = (C.c_void_p * n)(*[c.handle for c, _ in pairs])
         refs = (C.c_void_p * n)(*[r.handle for _, r in pairs])
         _, _, mbh, mbw = self.grid(pairs[0][0].width, pairs[0][0].height)
         out = np.zeros((n, mbh, mbw, 5), np.float32)
-        self.backend._check(self.lib.vc1me_nn_forward_frames(self.handle, curs, refs, n,
+        g, gp = self._globals(globals_, n)
+        self.backend._check(self.lib.vc1me_nn_forward_frames(self.handle, curs, refs, gp, n,
                                                             out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
@@ -293,15 +303,16 @@ class ProposalNet:
                                                               fp(grads)))
         return out, grads
 
-    def train_step(self, pairs, targets: np.ndarray, valid: np.ndarray, lr: float) -> float:
+    def train_step(self, pairs, targets: np.ndarray, valid: np.ndarray, lr: float, globals_=None) -> float:
         n = len(pairs)
         curs = (C.c_void_p * n)(*[c.handle for c, _ in pairs])
         refs = (C.c_void_p * n)(*[r.handle for _, r in pairs])
         targets = np.ascontiguousarray(targets, np.int32)
         valid = np.ascontiguousarray(valid, np.uint8)
+        g, gp = self._globals(globals_, n)
         loss = C.c_float()
         self.backend._check(self.lib.vc1me_nn_train_step(
-            self.handle, curs, refs, targets.ctypes.data_as(C.POINTER(C.c_int32)),
+            self.handle, curs, refs, gp, targets.ctypes.data_as(C.POINTER(C.c_int32)),
             valid.ctypes.data_as(C.POINTER(C.c_uint8)), n, lr, C.byref(loss)))
         return float(loss.value)
 

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -467,7 +468,10 @@ int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_res
     return VC1ME_OK;
 }
 
-int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* nn_out = nullptr, float nn_logit = 0.f) {
+// after_global, if set, is called once the global search is queued (the
+// learned form runs its network there, on the global vectors in the slots).
+int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* nn_out = nullptr, float nn_logit = 0.f,
+             const std::function<int(hipStream_t)>& after_global = {}) {
     if (n <= 0) return VC1ME_OK;
     if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
     if (n > 255) return fail(VC1ME_ELIMIT, "at most 255 jobs per step");
@@ -532,6 +536,8 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         }
     }
     HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st, ctx->events[2], ctx->events[3]));
+    if (after_global)
+        if (int rc = after_global(st)) return rc;
     HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
     const int lds = vc1me::mb_kernel_lds_bytes(max_local);
     {
@@ -588,6 +594,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         ctx->work.satd_ops += ops[2];
         ctx->work.sig_ops += ops[3];
         ctx->work.frame_px += ops[4];
+        ctx->work.seeded_macroblocks += ops[5];
     }
     for (int i = 0; i < n; ++i) {
         ctx->counters.frames_analyzed++;
@@ -615,6 +622,7 @@ struct NnBuffers {
     uint8_t* valid = nullptr;
     float* in32 = nullptr;
     float* loss = nullptr;
+    int32_t* gxy = nullptr;            // [nb][2] global vectors handed in by the host
 };
 
 }  // namespace
@@ -647,7 +655,7 @@ hipError_t nn_reserve(vc1me_nn* nn, int nb, int h4, int w4) {
     if (nb <= b.nb && h4 * w4 <= b.h4 * b.w4 && b.x0) return hipSuccess;
     auto rel = [](auto*& q) { if (q) (void)hipFree(q); q = nullptr; };
     rel(b.x0); for (auto& a : b.act) rel(a); rel(b.dz[0]); rel(b.dz[1]); rel(b.pooled); rel(b.out); rel(b.dout);
-    rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss);
+    rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss); rel(b.gxy);
     const size_t px = static_cast<size_t>(nb) * h4 * w4;
     const size_t mbs = static_cast<size_t>(nb) * ((h4 + 3) / 4) * ((w4 + 3) / 4);
     hipError_t e = hipMalloc(&b.x0, px * 2 * 2);
@@ -661,6 +669,7 @@ hipError_t nn_reserve(vc1me_nn* nn, int nb, int h4, int w4) {
     if (e == hipSuccess) e = hipMalloc(&b.valid, mbs);
     if (e == hipSuccess) e = hipMalloc(&b.in32, px * 2 * 4);
     if (e == hipSuccess) e = hipMalloc(&b.loss, 4);
+    if (e == hipSuccess) e = hipMalloc(&b.gxy, sizeof(int32_t) * 2 * nb);
     if (e != hipSuccess) return e;
     b.nb = nb; b.h4 = h4; b.w4 = w4;
     return hipSuccess;
@@ -715,9 +724,11 @@ hipError_t nn_backward(vc1me_nn* nn, int nb, int h4, int w4, hipStream_t st) {
     return e;
 }
 
-// downsampled inputs of n frame pairs of one raster into b.x0
+// downsampled inputs of n frame pairs of one raster into b.x0; gdev[i] is
+// pair i's global vector in device memory (null: none), or host_g (n x 2,
+// null: none) is uploaded to b.gxy first
 int nn_inputs(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs, int n, int& h4, int& w4,
-              int& mbh, int& mbw) {
+              int& mbh, int& mbw, const int32_t* host_g, const int32_t* const* gdev = nullptr) {
     if (!curs || !refs || n <= 0) return fail(VC1ME_EINVAL, "no frames");
     const int W = curs[0]->w, H = curs[0]->h;
     for (int i = 0; i < n; ++i)
@@ -726,9 +737,12 @@ int nn_inputs(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* c
     nn_dims(W, H, h4, w4, mbh, mbw);
     hipStream_t st = nn->ctx->stream;
     HIP_TRY(nn_reserve(nn, n, h4, w4));
-    for (int i = 0; i < n; ++i)
-        HIP_TRY(pnet::launch_downsample(curs[i]->y, refs[i]->y, W, H, h4, w4,
+    if (host_g) HIP_TRY(hipMemcpyAsync(nn->b.gxy, host_g, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, st));
+    for (int i = 0; i < n; ++i) {
+        const int32_t* g = gdev ? gdev[i] : host_g ? nn->b.gxy + 2 * i : nullptr;
+        HIP_TRY(pnet::launch_downsample(curs[i]->y, refs[i]->y, W, H, h4, w4, g,
                                             nn->b.x0 + static_cast<size_t>(i) * h4 * w4 * 2, st));
+    }
     return VC1ME_OK;
 }
 
@@ -1040,7 +1054,7 @@ void vc1me_nn_destroy(vc1me_nn* nn) {
     for (int l = 0; l < pnet::kLayers; ++l) { rel(nn->wt[l]); rel(nn->wtt[l]); }
     NnBuffers& b = nn->b;
     rel(b.x0); for (auto& a : b.act) rel(a); rel(b.dz[0]); rel(b.dz[1]); rel(b.pooled); rel(b.out); rel(b.dout);
-    rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss);
+    rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss); rel(b.gxy);
     for (auto& q : nn->props) rel(q);
     delete nn;
 }
@@ -1067,13 +1081,13 @@ int vc1me_nn_get_params(vc1me_nn* nn, float* params) {
     return VC1ME_OK;
 }
 
-int vc1me_nn_forward_frames(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs, int n,
-                            float* out) {
+int vc1me_nn_forward_frames(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs,
+                            const int32_t* globals, int n, float* out) {
     if (!nn || !out) return fail(VC1ME_EINVAL, "null argument");
     std::lock_guard<std::mutex> lock(nn->ctx->mu);
     if (int rc = set_device(nn->ctx)) return rc;
     int h4, w4, mbh, mbw;
-    if (int rc = nn_inputs(nn, curs, refs, n, h4, w4, mbh, mbw)) return rc;
+    if (int rc = nn_inputs(nn, curs, refs, n, h4, w4, mbh, mbw, globals)) return rc;
     hipStream_t st = nn->ctx->stream;
     HIP_TRY(nn_forward(nn, n, h4, w4, st));
     HIP_TRY(hipMemcpyAsync(out, nn->b.out, sizeof(float) * 5 * n * mbh * mbw, hipMemcpyDeviceToHost, st));
@@ -1103,19 +1117,22 @@ int vc1me_nn_forward_backward(vc1me_nn* nn, const float* input, int n, int h4, i
 }
 
 int vc1me_nn_train_step(vc1me_nn* nn, const vc1me_frame* const* curs, const vc1me_frame* const* refs,
-                        const int32_t* targets, const uint8_t* valid, int n, float lr, float* loss) {
+                        const int32_t* globals, const int32_t* targets, const uint8_t* valid, int n, float lr,
+                        float* loss) {
     if (!nn || !targets || !valid || !loss) return fail(VC1ME_EINVAL, "null argument");
     std::lock_guard<std::mutex> lock(nn->ctx->mu);
     if (int rc = set_device(nn->ctx)) return rc;
     int h4, w4, mbh, mbw;
-    if (int rc = nn_inputs(nn, curs, refs, n, h4, w4, mbh, mbw)) return rc;
+    if (int rc = nn_inputs(nn, curs, refs, n, h4, w4, mbh, mbw, globals)) return rc;
     hipStream_t st = nn->ctx->stream;
-    const int mbs = n * mbh * mbw;
+    const int per = mbh * mbw, mbs = n * per;
     std::vector<float> t(static_cast<size_t>(mbs) * 2);
     int count = 0;
     for (int i = 0; i < mbs; ++i) {
-        t[2 * i] = static_cast<float>(targets[2 * i]) / 256.0f;       // quarter-pel -> 64-px units
-        t[2 * i + 1] = static_cast<float>(targets[2 * i + 1]) / 256.0f;
+        // quarter-pel -> residual from the pair's global vector in 64-px units
+        const int gx = globals ? globals[2 * (i / per)] : 0, gy = globals ? globals[2 * (i / per) + 1] : 0;
+        t[2 * i] = static_cast<float>(targets[2 * i] - 4 * gx) / 256.0f;
+        t[2 * i + 1] = static_cast<float>(targets[2 * i + 1] - 4 * gy) / 256.0f;
         count += valid[i] ? 1 : 0;
     }
     HIP_TRY(hipMemcpyAsync(nn->b.tgt, t.data(), t.size() * 4, hipMemcpyHostToDevice, st));
@@ -1176,15 +1193,11 @@ int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jo
     if (!jobs || n > 255) return fail(VC1ME_EINVAL, "bad job list");
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (int rc = set_device(ctx)) return rc;
-    hipStream_t st = ctx->stream;
     if (nn->props.size() < static_cast<size_t>(n)) { nn->props.resize(n, nullptr); nn->props_cap.resize(n, 0); }
     for (int i = 0; i < n; ++i) {
-        if (!jobs[i].cur || !jobs[i].ref) return fail(VC1ME_EINVAL, "null frame");
+        if (!jobs[i].cur || !jobs[i].ref || !jobs[i].params) return fail(VC1ME_EINVAL, "null job field");
         int h4, w4, mbh, mbw;
-        const vc1me_frame* c = jobs[i].cur;
-        const vc1me_frame* r = jobs[i].ref;
-        if (int rc = nn_inputs(nn, &c, &r, 1, h4, w4, mbh, mbw)) return rc;
-        HIP_TRY(nn_forward(nn, 1, h4, w4, st));
+        nn_dims(jobs[i].cur->w, jobs[i].cur->h, h4, w4, mbh, mbw);
         const size_t need = static_cast<size_t>(mbh) * mbw * 5;
         if (nn->props_cap[i] < need) {
             if (nn->props[i]) (void)hipFree(nn->props[i]);
@@ -1193,10 +1206,23 @@ int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jo
             HIP_TRY(hipMalloc(&nn->props[i], need * sizeof(float)));
             nn->props_cap[i] = need;
         }
-        HIP_TRY(hipMemcpyAsync(nn->props[i], nn->b.out, need * sizeof(float), hipMemcpyDeviceToDevice, st));
     }
+    // the network runs inside the step, after the global search it is aligned by
+    auto forward = [&](hipStream_t st) -> int {
+        for (int i = 0; i < n; ++i) {
+            int h4, w4, mbh, mbw;
+            const vc1me_frame* c = jobs[i].cur;
+            const vc1me_frame* r = jobs[i].ref;
+            const int32_t* g = &ctx->slots[i].g->dx;
+            if (int rc = nn_inputs(nn, &c, &r, 1, h4, w4, mbh, mbw, nullptr, &g)) return rc;
+            HIP_TRY(nn_forward(nn, 1, h4, w4, st));
+            HIP_TRY(hipMemcpyAsync(nn->props[i], nn->b.out, sizeof(float) * mbh * mbw * 5, hipMemcpyDeviceToDevice,
+                                   st));
+        }
+        return VC1ME_OK;
+    };
     const float logit = std::log(threshold / (1.0f - threshold));
-    return run_jobs(ctx, jobs, n, nn->props.data(), logit);
+    return run_jobs(ctx, jobs, n, nn->props.data(), logit, forward);
 }
 
 }  // extern "C"

@@ -1255,26 +1255,45 @@ __device__ __forceinline__ uint64_t global_sad_min(Ctx& C, int vx, int vy, int n
     return best;
 }
 
+// A seeded result is kept when its SAD is at most kSeedKeepPerPx per active
+// pixel (a missed match of textured content costs several times more; the
+// reference's own "good match" bound, 6 per pixel, rejects most correct
+// integer matches of quarter-pel motion in noisy content).
+constexpr uint64_t kSeedKeepPerPx = 16;
+__device__ __forceinline__ bool seed_kept(const Ctx& C, uint64_t sad) {
+    const uint64_t px = active_pixels(C);
+    return px && sad < UINT64_MAX / 8 && sad <= px * kSeedKeepPerPx;
+}
+
+// This lane's seed of the learned search: lanes 0 and 1 the two proposals
+// (residuals from the global vector in units of 64 px), lane 2 the predictor
+// (when known, else the global vector), lane 3 zero, lane 4 the global
+// vector; clamped to the search box.
+__device__ __forceinline__ void seed_lane(const Ctx& C, const float* o, int gdx, int gdy, const Bounds& b, int ppx,
+                                          int ppy, int& vx, int& vy) {
+    const int k = C.lane;
+    vx = 0; vy = 0;
+    float fx = 0.f, fy = 0.f;
+    if (k == 0) { fx = o[0] * 64.f + static_cast<float>(gdx); fy = o[1] * 64.f + static_cast<float>(gdy); }
+    if (k == 1) { fx = o[2] * 64.f + static_cast<float>(gdx); fy = o[3] * 64.f + static_cast<float>(gdy); }
+    if (k < 2) {
+        vx = static_cast<int>(rintf(fminf(fmaxf(fx, -4096.f), 4096.f)));
+        vy = static_cast<int>(rintf(fminf(fmaxf(fy, -4096.f), 4096.f)));
+    }
+    if (k == 2) { vx = ppx; vy = ppy; }
+    if (k == 4) { vx = gdx; vy = gdy; }
+    vx = clampi(vx, b.xmin, b.xmax);
+    vy = clampi(vy, b.ymin, b.ymax);
+}
+
 // Integer search seeded by the learned proposals (no reference counterpart):
 // full SADs of the two proposals, the predictor, zero and the global vector,
 // then a unit diamond from the best until the centre wins (or 32 steps).
+// The result's SAD is never above the best seed's.
 __device__ __forceinline__ IRes seeded_search(Ctx& C, const Pred& pi, int gdx, int gdy, int range, const float* o) {
     const Bounds b = bounds_px(range);
-    int vx = 0, vy = 0;
-    {
-        const int k = C.lane;
-        float fx = 0.f, fy = 0.f;
-        if (k == 0) { fx = o[0] * 64.f; fy = o[1] * 64.f; }
-        if (k == 1) { fx = o[2] * 64.f; fy = o[3] * 64.f; }
-        if (k < 2) {
-            vx = static_cast<int>(rintf(fminf(fmaxf(fx, -4096.f), 4096.f)));
-            vy = static_cast<int>(rintf(fminf(fmaxf(fy, -4096.f), 4096.f)));
-        }
-        if (k == 2) { vx = round_qpel_to_pixel(pi.px); vy = round_qpel_to_pixel(pi.py); }
-        if (k == 4) { vx = gdx; vy = gdy; }
-        vx = clampi(vx, b.xmin, b.xmax);
-        vy = clampi(vy, b.ymin, b.ymax);
-    }
+    int vx, vy;
+    seed_lane(C, o, gdx, gdy, b, round_qpel_to_pixel(pi.px), round_qpel_to_pixel(pi.py), vx, vy);
     uint64_t best = global_sad_min(C, vx, vy, 5);
     for (int it = 0; it < 32; ++it) {
         const int cx = key_dx(best), cy = key_dy(best);
@@ -1964,6 +1983,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
     if (C.lane < 4) C.L.own[C.lane] = make_int2(0, 0);
     wsync();
     const Pred rate_pi = predictor_mixed_1mv(C);
+    bool was_seeded = false;
     {
         const Pred pi = predictor_info(C);
         const int gdx = J->g->dx, gdy = J->g->dy;
@@ -1971,21 +1991,28 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
         bool selected_distant = false;
         // learned seeding: this macroblock's proposals, used when confident
         const float* nn = J->nn_out ? J->nn_out + static_cast<size_t>(pos) * 5 : nullptr;
-        const bool seeded = nn && nn[4] >= J->nn_logit;
+        // a confident macroblock keeps the seeded result if it matches well
+        // enough (seed_kept); otherwise it runs the plain search as well
+        bool seeded_ok = false;
+        if (nn && nn[4] >= J->nn_logit) {
+            im = seeded_search(C, pi, gdx, gdy, range, nn);
+            seeded_ok = seed_kept(C, im.sad);
+        }
+        was_seeded = seeded_ok;
         const uint64_t px = active_pixels(C);
         if (J->lr_mode == 2) {
-            bool sc = false;
-            if (range > local_range && !seeded) {
+            bool sc = seeded_ok;
+            if (range > local_range && !sc) {
                 const IRes pr = propagated(C, range);
                 if (match_good(C, pr.sad)) { im = pr; sc = true; selected_distant = true; }
             }
-            if (!sc) im = seeded ? seeded_search(C, pi, gdx, gdy, range, nn) : J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
+            if (!sc) im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
         } else {
-            im = seeded ? seeded_search(C, pi, gdx, gdy, range, nn) : J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
+            if (!seeded_ok) im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
             const uint64_t local_sad = im.sad;
             if (px) dl_mae = static_cast<double>(local_sad) / static_cast<double>(px);
             const bool skip = J->lr_mode == 1 && match_good(C, im.sad);
-            if (range > local_range && !skip && !seeded) {
+            if (range > local_range && !skip && !seeded_ok) {
                 const IRes pr = propagated(C, range);
                 const int dec = distant_decision(C, pr.sad, local_sad);
                 if (px && pr.sad < UINT64_MAX / 8) {
@@ -1999,7 +2026,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
                 }
             }
         }
-        if (range > local_range && !seeded && !match_good(C, im.sad)) {
+        if (range > local_range && !seeded_ok && !match_good(C, im.sad)) {
             uint64_t ext_sad = UINT64_MAX;
             int ext_dec = 0;
             const IRes ext = extended_content(C, im, &ext_sad, &ext_dec);
@@ -2203,6 +2230,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
         atomicAdd(&J->ops[1], static_cast<unsigned long long>(C.n_taps));
         atomicAdd(&J->ops[2], static_cast<unsigned long long>(C.n_satd));
         atomicAdd(&J->ops[3], static_cast<unsigned long long>(C.n_sig));
+        if (was_seeded) atomicAdd(&J->ops[5], 1ull);
     }
 }
 
@@ -2238,8 +2266,6 @@ __global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ job
     if (J->range <= J->local_range) return;
     const int pos = static_cast<int>(blockIdx.x);
     if (pos >= J->mbw * J->mbh) return;
-    // macroblocks the learned seeding covers do not search long range
-    if (J->nn_out && J->nn_out[static_cast<size_t>(pos) * 5 + 4] >= J->nn_logit) return;
     Ctx C;
     init_ctx(C, J, pos % J->mbw, pos / J->mbw, smem);
     for (int i = C.lane; i < 256; i += kWave) {
@@ -2247,6 +2273,16 @@ __global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ job
         C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
     }
     wsync();
+    // learned seeding: the wavefront's seeded search starts from a superset
+    // of these seeds and never raises the SAD, so if one of them already
+    // passes seed_kept the macroblock keeps the seeded result and never
+    // consults the long-range search
+    if (J->nn_out && J->nn_out[static_cast<size_t>(pos) * 5 + 4] >= J->nn_logit) {
+        const int gdx = J->g->dx, gdy = J->g->dy;
+        int vx, vy;
+        seed_lane(C, J->nn_out + static_cast<size_t>(pos) * 5, gdx, gdy, bounds_px(J->range), gdx, gdy, vx, vy);
+        if (seed_kept(C, key_cost(global_sad_min(C, vx, vy, 3)))) return;
+    }
     uint64_t sad = UINT64_MAX;
     int dx = 0, dy = 0;
     extended_search(C, J->range, &sad, &dx, &dy);

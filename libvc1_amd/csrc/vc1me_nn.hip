@@ -43,18 +43,24 @@ __device__ __forceinline__ v4s load4(const uint16_t* p, bool ok) {
 // ------------------------------------------------------------------ input
 
 // 4x4 box mean of cur and ref luma (edge samples repeated past the frame),
-// scaled to [-0.5, 0.5], bf16 [h4][w4][2] with h4 = ceil(H/4), w4 = ceil(W/4)
+// scaled to [-0.5, 0.5], bf16 [h4][w4][2] with h4 = ceil(H/4), w4 = ceil(W/4).
+// The ref block is taken displaced by the global vector gxy = {dx, dy} (px,
+// device memory, null = none), so the network sees ref aligned with the
+// frame's dominant motion and predicts residuals from it.
 __global__ void k_downsample(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ ref, int W, int H, int h4,
-                             int w4, uint16_t* __restrict__ x0) {
+                             int w4, const int32_t* __restrict__ gxy, uint16_t* __restrict__ x0) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= h4 * w4) return;
     const int y = i / w4, x = i % w4;
+    const int gx = gxy ? gxy[0] : 0, gy = gxy ? gxy[1] : 0;
     uint32_t sc = 0, sr = 0;
     for (int dy = 0; dy < 4; ++dy)
         for (int dx = 0; dx < 4; ++dx) {
             const size_t o = static_cast<size_t>(min(y * 4 + dy, H - 1)) * W + min(x * 4 + dx, W - 1);
+            const size_t r = static_cast<size_t>(min(max(y * 4 + dy + gy, 0), H - 1)) * W +
+                             min(max(x * 4 + dx + gx, 0), W - 1);
             sc += cur[o];
-            sr += ref[o];
+            sr += ref[r];
         }
     x0[2 * i] = to_bf16(static_cast<float>(sc) / (16.0f * 255.0f) - 0.5f);
     x0[2 * i + 1] = to_bf16(static_cast<float>(sr) / (16.0f * 255.0f) - 0.5f);
@@ -347,10 +353,10 @@ __global__ void k_pack_weights(const float* __restrict__ wf, uint16_t* __restric
 
 // ------------------------------------------------------------------ launchers
 
-hipError_t launch_downsample(const uint8_t* cur, const uint8_t* ref, int W, int H, int h4, int w4, uint16_t* x0,
-                             hipStream_t s) {
+hipError_t launch_downsample(const uint8_t* cur, const uint8_t* ref, int W, int H, int h4, int w4, const int32_t* gxy,
+                             uint16_t* x0, hipStream_t s) {
     const int n = h4 * w4;
-    hipLaunchKernelGGL(k_downsample, dim3((n + 255) / 256), dim3(256), 0, s, cur, ref, W, H, h4, w4, x0);
+    hipLaunchKernelGGL(k_downsample, dim3((n + 255) / 256), dim3(256), 0, s, cur, ref, W, H, h4, w4, gxy, x0);
     return hipGetLastError();
 }
 hipError_t launch_to_bf16(const float* in, uint16_t* out, int n, hipStream_t s) {

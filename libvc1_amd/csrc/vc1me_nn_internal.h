@@ -17,8 +17,8 @@ inline int layer_dil(int l) { return 1 << l; }
 inline int layer_offset(int l) { return l == 0 ? 0 : (kW1 + 32) + (l - 1) * (kW + 32); }   // weights, then biases
 inline int head_offset() { return (kW1 + 32) + 3 * (kW + 32); }
 
-hipError_t launch_downsample(const uint8_t* cur, const uint8_t* ref, int W, int H, int h4, int w4, uint16_t* x0,
-                             hipStream_t s);
+hipError_t launch_downsample(const uint8_t* cur, const uint8_t* ref, int W, int H, int h4, int w4, const int32_t* gxy,
+                             uint16_t* x0, hipStream_t s);
 hipError_t launch_to_bf16(const float* in, uint16_t* out, int n, hipStream_t s);
 hipError_t launch_conv_fwd(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* out, int nb, int h, int w,
                            int cin, int dil, hipStream_t s);

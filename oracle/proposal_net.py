@@ -148,16 +148,17 @@ def backward(cache, dout: np.ndarray) -> np.ndarray:
     return pack(g)
 
 
-def downsample(cur: np.ndarray, ref: np.ndarray) -> np.ndarray:
+def downsample(cur: np.ndarray, ref: np.ndarray, gx: int = 0, gy: int = 0) -> np.ndarray:
     """Network input of one frame pair: 4x4 box means of the two luma planes
-    (edge samples repeated past the frame), scaled to [-0.5, 0.5] in float32
-    and rounded to bf16.  (h4, w4, 2), h4 = ceil(H / 4), w4 = ceil(W / 4)."""
+    (edge samples repeated past the frame), the ref block displaced by the
+    global vector (gx, gy) px, scaled to [-0.5, 0.5] in float32 and rounded
+    to bf16.  (h4, w4, 2), h4 = ceil(H / 4), w4 = ceil(W / 4)."""
     H, W = cur.shape
     h4, w4 = (H + 3) // 4, (W + 3) // 4
-    ys = np.minimum(np.arange(h4 * 4), H - 1)
-    xs = np.minimum(np.arange(w4 * 4), W - 1)
     out = np.zeros((h4, w4, 2), np.float32)
-    for k, plane in enumerate((cur, ref)):
+    for k, (plane, ox, oy) in enumerate(((cur, 0, 0), (ref, gx, gy))):
+        ys = np.clip(np.arange(h4 * 4) + oy, 0, H - 1)
+        xs = np.clip(np.arange(w4 * 4) + ox, 0, W - 1)
         e = plane.astype(np.int64)[ys][:, xs]
         s = e.reshape(h4, 4, w4, 4).sum(axis=(1, 3))
         out[..., k] = s.astype(np.float32) / np.float32(4080.0) - np.float32(0.5)

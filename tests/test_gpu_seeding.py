@@ -89,14 +89,15 @@ def test_forward_frames_matches_oracle(backend, net):
     net.set_params(params)
     pics = [_pair(70 + i, 200, 118) for i in range(2)]
     frames = [(backend.upload(c), backend.upload(r)) for c, r in pics]
+    g = np.array([[5, -3], [-17, 9]], np.int32)
     _say("[case] proposal net forward on frames 200x118")
     try:
-        got = net.forward_frames(frames)
+        got = net.forward_frames(frames, g)
     finally:
         for a, b in frames:
             a.release()
             b.release()
-    x = np.stack([pn.downsample(c.y, r.y) for c, r in pics])
+    x = np.stack([pn.downsample(c.y, r.y, int(gg[0]), int(gg[1])) for (c, r), gg in zip(pics, g)])
     want, _ = pn.forward(params.astype(np.float64), x, quant=True)
     assert got.shape == want.shape == (2, 8, 13, 5)
     assert _rel(got, want) < 1e-2
@@ -112,17 +113,19 @@ def test_train_step_loss_and_update(backend, net):
     tgt = rng.integers(-64, 65, size=(2, mbh * mbw, 2)).astype(np.int32)
     valid = (rng.uniform(size=(2, mbh * mbw)) < 0.9).astype(np.uint8)
     frames = [(backend.upload(c), backend.upload(r)) for c, r in pics]
+    g = np.array([[3, 1], [-6, 2]], np.int32)
     lr = 1e-3
     _say("[case] proposal net train step 160x96")
     try:
-        loss = net.train_step(frames, tgt, valid, lr)
+        loss = net.train_step(frames, tgt, valid, lr, g)
     finally:
         for a, b in frames:
             a.release()
             b.release()
-    x = np.stack([pn.downsample(c.y, r.y) for c, r in pics])
+    x = np.stack([pn.downsample(c.y, r.y, int(gg[0]), int(gg[1])) for (c, r), gg in zip(pics, g)])
     out, cache = pn.forward(params.astype(np.float64), x, quant=True)
-    want_loss, dout = pn.loss(out.reshape(-1, 5), tgt.reshape(-1, 2) / 256.0, valid.reshape(-1).astype(bool))
+    resid = (tgt - 4 * g[:, None, :]) / 256.0     # targets as residuals from the global vector
+    want_loss, dout = pn.loss(out.reshape(-1, 5), resid.reshape(-1, 2), valid.reshape(-1).astype(bool))
     assert loss == pytest.approx(want_loss, rel=1e-2)
     g = pn.backward(cache, dout.reshape(out.shape))
     step = net.get_params().astype(np.float64) - params
@@ -194,7 +197,8 @@ def test_seeded_with_plain_vectors_keeps_cost(backend):
         c_plain, one = backend.last_mb_costs(0, plain.mbs)
         mbw, mbh = abi.mb_grid(p.width, p.height)
         prop = np.zeros((mbh, mbw, 5), np.float32)
-        prop[..., 0:2] = (one / 256.0).reshape(mbh, mbw, 2)
+        gv = np.array([plain.struct.global_dx, plain.struct.global_dy])
+        prop[..., 0:2] = ((one - 4 * gv) / 256.0).reshape(mbh, mbw, 2)
         prop[..., 2:4] = prop[..., 0:2]
         prop[..., 4] = 10.0
         seeded = backend.analyze_step_proposals([(fc, fr)], [p], [prop])[0]
@@ -204,6 +208,8 @@ def test_seeded_with_plain_vectors_keeps_cost(backend):
         fc.release()
         fr.release()
     assert np.all(np.abs(seeded.mvs[:, 0]) <= 4 * 512) and np.all(np.abs(seeded.mvs[:, 1]) <= 4 * 256)
+    # seeds at the plain vectors: most macroblocks keep the seeded result
+    assert 0.5 * seeded.mbs <= backend.work().seeded_macroblocks <= seeded.mbs
     rise = c_seed.sum() / c_plain.sum() - 1.0
     _say(f"seeded-with-plain cost rise {rise:+.4%}")
     assert rise < 0.01

@@ -137,6 +137,10 @@ def test_downsample_closed_form():
     x = pn.downsample(ramp, ramp)
     last = (12 * 16) / 255.0 - 0.5
     assert x[0, 3, 0] == pytest.approx(last, abs=2 ** -8)
+    # the ref channel displaced by the global vector: ref(x + 4) in cell x is cur's next cell
+    x = pn.downsample(ramp, ramp, gx=4)
+    np.testing.assert_array_equal(x[:, :2, 1], x[:, 1:3, 0])
+    assert x[0, 3, 1] == pytest.approx(last, abs=2 ** -8)   # clamped past the edge
 
 
 def test_adam_first_step_is_sign_step():
