@@ -207,13 +207,16 @@ int vc1me_analyze_step(vc1me_ctx* ctx, const vc1me_job* jobs, int n);
 /* Timing of the last vc1me_analyze_step / vc1me_analyze_p_frame: device
  * milliseconds between HIP events recorded on the context stream around the
  * whole launch sequence, and around every launch of the macroblock search
- * kernel (sum and count). */
+ * kernel (sum and count).  graph_replay: 0 = launched one by one, 1 = a
+ * cached HIP graph of the step was replayed, 2 = the graph was captured by
+ * this step (vc1me_set_graphs). */
 typedef struct vc1me_timing {
     double step_ms;
     double mb_kernel_ms_total;
     uint64_t mb_kernel_launches;
     double global_kernel_ms_total;
     uint64_t global_kernel_launches;
+    uint64_t graph_replay;
 } vc1me_timing;
 int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
 /* Record HIP events around every macroblock-kernel launch so that
@@ -221,6 +224,15 @@ int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
  * a little launch overhead).  No reference counterpart (profiling aid,
  * like the reference's SpeedProfiler, src/encoder_internal.h:182-221). */
 int vc1me_set_profiling(vc1me_ctx* ctx, int on);
+/* Run each plain analysis step as one HIP graph: the step's whole launch
+ * sequence (phase planes, global search, signature index, long-range
+ * pre-pass, one launch per anti-diagonal) is captured the first time a launch
+ * shape (job count, rasters, ranges, modes) is seen and replayed afterwards;
+ * the job table is uploaded before each replay.  Results are identical to
+ * launching one by one.  Not used while profiling is on or for the learned
+ * forms.  No reference counterpart (replaces the per-batch dispatches of
+ * src/vulkan_compute.cpp:370-490). */
+int vc1me_set_graphs(vc1me_ctx* ctx, int on);
 
 /* Algorithmic work of the last step, summed over its jobs, in the units of
  * the roofline: one |a-b| accumulate per sampled pixel of a block SAD

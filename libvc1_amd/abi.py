@@ -84,7 +84,7 @@ class Counters(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("step_ms", C.c_double), ("mb_kernel_ms_total", C.c_double),
                 ("mb_kernel_launches", C.c_uint64), ("global_kernel_ms_total", C.c_double),
-                ("global_kernel_launches", C.c_uint64)]
+                ("global_kernel_launches", C.c_uint64), ("graph_replay", C.c_uint64)]
 
 
 class Work(C.Structure):

@@ -28,8 +28,8 @@ struct vc1me_frame {
     uint8_t* v = nullptr;
     int32_t w = 0, h = 0;
     // interpolated phase planes of this frame as a reference (bicubic 16,
-    // bilinear 4), allocated on first use; their contents are recomputed in
-    // every step that reads them
+    // bilinear 4) for vc1me_block_costs only, allocated on its first use;
+    // analysis steps use the context's step pool instead
     mutable uint8_t* qp = nullptr;
     mutable uint8_t* bp = nullptr;
 };
@@ -178,6 +178,9 @@ struct vc1me_ctx {
     vc1me_work work{};
     bool profiling = false;
     bool cost_report = false;
+    bool use_graphs = false;
+    std::vector<std::pair<uint8_t*, size_t>> phase_pool;   // phase planes of the step's references
+    std::vector<std::pair<std::string, hipGraphExec_t>> graphs;   // captured steps by launch shape
     int last_jobs = 0;
     std::vector<JobSlot> slots;
     JobDesc* d_jobs = nullptr;
@@ -368,8 +371,8 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.ph = vc1me::phase_rows(ref->h);
 }
 
-// Phase planes of `ref` that a job of mode `mode` reads: allocated here on
-// first use, filled by launch_phases in the step itself.
+// Phase planes of `ref` for vc1me_block_costs: allocated here on first use,
+// filled by launch_phases in the call itself.
 hipError_t ensure_phase_alloc(const vc1me_frame* ref, int mode) {
     const bool bil = mode == 2;
     uint8_t*& buf = bil ? ref->bp : ref->qp;
@@ -490,8 +493,6 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         const size_t cells = static_cast<size_t>((p->width + 7) / 8) * ((p->height + 7) / 8);
         hipError_t e = ensure_slot(ctx->slots[i], mbs, cells);
         if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("device state allocation: ") + hipGetErrorString(e));
-        e = ensure_phase_alloc(jobs[i].ref, p->mode);
-        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("phase plane allocation: ") + hipGetErrorString(e));
         fill_job(ctx->h_jobs[i], p, jobs[i].cur, jobs[i].ref, ctx->slots[i]);
         if (nn_out) {
             ctx->h_jobs[i].nn_out = nn_out[i];
@@ -514,6 +515,37 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     }
     hipStream_t st = ctx->stream;
     if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1))) return rc;
+    // interpolated references: one owning job per distinct (frame, filter),
+    // each with a buffer of the context's step pool (planes live for the step)
+    int max_pw = 0, max_ph = 0;
+    {
+        std::vector<std::pair<const vc1me_frame*, bool>> done;
+        for (int i = 0; i < n; ++i) {
+            JobDesc& d = ctx->h_jobs[i];
+            const bool bil = jobs[i].params->mode == 2;
+            const std::pair<const vc1me_frame*, bool> key(jobs[i].ref, bil);
+            const auto it = std::find(done.begin(), done.end(), key);
+            size_t k = static_cast<size_t>(it - done.begin());
+            d.phase_owner = it == done.end() ? 1 : 0;
+            if (d.phase_owner) {
+                done.push_back(key);
+                const size_t need = static_cast<size_t>(d.pp) * d.ph * (bil ? 4 : 16);
+                if (ctx->phase_pool.size() <= k) ctx->phase_pool.resize(k + 1, {nullptr, 0});
+                auto& slot = ctx->phase_pool[k];
+                if (slot.second < need) {
+                    if (slot.first) (void)hipFree(slot.first);
+                    slot = {nullptr, 0};
+                    const hipError_t e = hipMalloc(&slot.first, need);
+                    if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("phase plane allocation: ") + hipGetErrorString(e));
+                    slot.second = need;
+                }
+                max_pw = std::max(max_pw, d.W);
+                max_ph = std::max(max_ph, d.H);
+            }
+            uint8_t* buf = ctx->phase_pool[k].first;
+            if (bil) d.bp = buf; else d.qp = buf;
+        }
+    }
     HIP_TRY(hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs.data(), sizeof(JobDesc) * n, hipMemcpyHostToDevice, st));
     for (int i = 0; i < n; ++i) {
         HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
@@ -524,39 +556,76 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         }
     }
     ctx->last_jobs = n;
-    HIP_TRY(hipEventRecord(ctx->events[0], st));
-    // interpolated references, once per distinct (frame, filter) of the step
-    {
-        std::vector<std::pair<const vc1me_frame*, bool>> done;
-        for (int i = 0; i < n; ++i) {
-            const std::pair<const vc1me_frame*, bool> key(jobs[i].ref, jobs[i].params->mode == 2);
-            if (std::find(done.begin(), done.end(), key) != done.end()) continue;
-            done.push_back(key);
-            HIP_TRY(launch_phases(jobs[i].ref, jobs[i].params->mode, st));
-        }
-    }
-    HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st, ctx->events[2], ctx->events[3]));
-    if (after_global)
-        if (int rc = after_global(st)) return rc;
-    HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
     const int lds = vc1me::mb_kernel_lds_bytes(max_local);
-    {
-        int max_mbs = 0;
-        for (int i = 0; i < n; ++i)
-            if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range)
-                max_mbs = std::max(max_mbs, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh);
-        HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
-    }
-    int launches = 0;
-    for (int t = 0; t <= max_diag; ++t) {
+    int max_mbs = 0;
+    for (int i = 0; i < n; ++i)
+        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range)
+            max_mbs = std::max(max_mbs, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh);
+    auto per_diag = [&](int t) {
         int per = 0;
         for (int i = 0; i < n; ++i)
             if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
-        if (!per) continue;
-        if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[4 + 2 * t], st));
-        HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
-        if (ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[5 + 2 * t], st));
-        ++launches;
+        return per;
+    };
+    // the step's launch sequence; every launch takes the job table and scalars only
+    auto launch_all = [&](bool events) -> int {
+        HIP_TRY(vc1me::launch_phase_planes_jobs(ctx->d_jobs, n, max_pw, max_ph, st));
+        HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st,
+                                            events ? ctx->events[2] : nullptr, events ? ctx->events[3] : nullptr));
+        if (after_global)
+            if (int rc = after_global(st)) return rc;
+        HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
+        HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
+        for (int t = 0; t <= max_diag; ++t) {
+            const int per = per_diag(t);
+            if (!per) continue;
+            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[4 + 2 * t], st));
+            HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
+            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[5 + 2 * t], st));
+        }
+        return VC1ME_OK;
+    };
+    int launches = 0;
+    for (int t = 0; t <= max_diag; ++t) launches += per_diag(t) ? 1 : 0;
+    ctx->timing.graph_replay = 0;
+    if (ctx->use_graphs && !ctx->profiling && !after_global) {
+        // replay the step as one HIP graph, captured once per launch shape
+        std::string key;
+        auto put = [&key](long long v) { key += std::to_string(v); key += ','; };
+        put(reinterpret_cast<long long>(ctx->d_jobs)); put(n); put(lds); put(static_cast<long long>(max_cells));
+        put(max_mbs); put(max_diag); put(max_pw); put(max_ph);
+        for (int i = 0; i < n; ++i) {
+            const JobDesc& d = ctx->h_jobs[i];
+            put(d.W); put(d.H); put(d.range); put(d.local_range); put(d.mbw); put(d.mbh); put(d.max_diag);
+            put(d.mode); put(d.phase_owner);
+        }
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : ctx->graphs)
+            if (g.first == key) exec = g.second;
+        if (!exec) {
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int rc = launch_all(false);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            HIP_TRY(ec);
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIP_TRY(ei);
+            if (ctx->graphs.size() >= 16) {
+                (void)hipGraphExecDestroy(ctx->graphs.front().second);
+                ctx->graphs.erase(ctx->graphs.begin());
+            }
+            ctx->graphs.emplace_back(key, exec);
+            ctx->timing.graph_replay = 2;
+        } else {
+            ctx->timing.graph_replay = 1;
+        }
+        HIP_TRY(hipEventRecord(ctx->events[0], st));
+        HIP_TRY(hipGraphLaunch(exec, st));
+    } else {
+        HIP_TRY(hipEventRecord(ctx->events[0], st));
+        if (int rc = launch_all(true)) return rc;
     }
     HIP_TRY(hipEventRecord(ctx->events[1], st));
     HIP_TRY(hipEventSynchronize(ctx->events[1]));
@@ -564,16 +633,13 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     HIP_TRY(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
     ctx->timing.step_ms = ms;
     float gms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&gms, ctx->events[2], ctx->events[3]));
+    if (!ctx->timing.graph_replay) HIP_TRY(hipEventElapsedTime(&gms, ctx->events[2], ctx->events[3]));
     ctx->timing.global_kernel_ms_total = gms;
     ctx->timing.global_kernel_launches = 9;
     double mbms = 0.0;
     if (ctx->profiling) {
         for (int t = 0; t <= max_diag; ++t) {
-            int per = 0;
-            for (int i = 0; i < n; ++i)
-                if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
-            if (!per) continue;
+            if (!per_diag(t)) continue;
             float x = 0.f;
             HIP_TRY(hipEventElapsedTime(&x, ctx->events[4 + 2 * t], ctx->events[5 + 2 * t]));
             mbms += x;
@@ -808,6 +874,8 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_cands) (void)hipFree(ctx->d_cands);
     if (ctx->d_costs) (void)hipFree(ctx->d_costs);
     if (ctx->d_props) (void)hipFree(ctx->d_props);
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& b : ctx->phase_pool) if (b.first) (void)hipFree(b.first);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -979,6 +1047,13 @@ int vc1me_set_profiling(vc1me_ctx* ctx, int on) {
     if (!ctx) return fail(VC1ME_EINVAL, "null context");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->profiling = on != 0;
+    return VC1ME_OK;
+}
+
+int vc1me_set_graphs(vc1me_ctx* ctx, int on) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->use_graphs = on != 0;
     return VC1ME_OK;
 }
 

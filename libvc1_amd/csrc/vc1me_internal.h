@@ -72,6 +72,9 @@ struct JobDesc {
     uint64_t* cost20;
     int2* one_mv;
     int32_t lam20;
+    // this job computes its reference's phase planes in the step (the first
+    // job of the step with that reference and filter)
+    int32_t phase_owner;
 };
 
 // Phase-plane geometry of a W x H reference: sample (px, py), px in [-2, W],
@@ -92,6 +95,8 @@ hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int ld
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);
+// phase planes of the step's owning jobs (k_phase_planes_jobs)
+hipError_t launch_phase_planes_jobs(const JobDesc* d_jobs, int njobs, int max_w, int max_h, hipStream_t s);
 hipError_t launch_phase_planes(const uint8_t* ref, int width, int height, uint8_t* out, int pitch, int rows,
                                int bilinear, hipStream_t s);
 hipError_t launch_block_costs(const JobDesc* d_job, int x0, int y0, int bw, int bh, int mode, int metric,

@@ -2761,8 +2761,8 @@ hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hi
 
 // Phase planes of one reference (see qpel_eval): one thread per plane
 // position, every phase of the filter set.
-__global__ void __launch_bounds__(256) k_phase_planes(const uint8_t* __restrict__ ref, int W, int H,
-                                                      uint8_t* __restrict__ out, int pp, int ph, int bil) {
+__device__ __forceinline__ void phase_point(const uint8_t* __restrict__ ref, int W, int H, uint8_t* __restrict__ out,
+                                            int pp, int ph, int bil) {
     const int px = static_cast<int>(blockIdx.x) * 64 + static_cast<int>(threadIdx.x & 63) - 2;
     const int py = static_cast<int>(blockIdx.y) * 4 + static_cast<int>(threadIdx.x >> 6) - 2;
     if (px > W || py > H) return;
@@ -2785,11 +2785,32 @@ __global__ void __launch_bounds__(256) k_phase_planes(const uint8_t* __restrict_
     }
 }
 
+__global__ void __launch_bounds__(256) k_phase_planes(const uint8_t* __restrict__ ref, int W, int H,
+                                                      uint8_t* __restrict__ out, int pp, int ph, int bil) {
+    phase_point(ref, W, H, out, pp, ph, bil);
+}
+
+// The phase planes of every job that owns its reference's planes in the step
+// (one job per distinct reference and filter); grid z = job.
+__global__ void __launch_bounds__(256) k_phase_planes_jobs(const JobDesc* __restrict__ jobs) {
+    const JobDesc* J = &jobs[blockIdx.z];
+    if (!J->phase_owner) return;
+    const bool bil = J->mode == 2;
+    phase_point(J->ry, J->W, J->H, const_cast<uint8_t*>(bil ? J->bp : J->qp), J->pp, J->ph, bil ? 1 : 0);
+}
+
 hipError_t launch_phase_planes(const uint8_t* ref, int width, int height, uint8_t* out, int pitch, int rows,
                                int bilinear, hipStream_t s) {
     if (rows < height + 3 || pitch < width + 3 + 8) return hipErrorInvalidValue;
     const dim3 grid((width + 3 + 63) / 64, (height + 3 + 3) / 4);
     hipLaunchKernelGGL(k_phase_planes, grid, dim3(256), 0, s, ref, width, height, out, pitch, rows, bilinear);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_planes_jobs(const JobDesc* d_jobs, int njobs, int max_w, int max_h, hipStream_t s) {
+    if (njobs <= 0 || max_w <= 0) return hipSuccess;
+    const dim3 grid((max_w + 3 + 63) / 64, (max_h + 3 + 3) / 4, njobs);
+    hipLaunchKernelGGL(k_phase_planes_jobs, grid, dim3(256), 0, s, d_jobs);
     return hipGetLastError();
 }
 

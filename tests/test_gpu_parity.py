@@ -253,3 +253,75 @@ def test_pan_variants(backend, mode, rng_):
     d = synth.pan_rectangles(3, 8, width=640, height=360, frame_indices=[0, 3])
     p = abi.default_params(640, 360, pqindex=8, motion_search_range=rng_, motion_local_search_range=32, mode=mode)
     _check_frame(backend, d[3], d[0], p, f"pan variant mode={mode} R={rng_}")
+
+
+def test_graph_replay_matches_eager(backend):
+    """Steps replayed from a captured HIP graph give the launch-by-launch
+    results, over several replays with other steps in between."""
+    shapes = ((352, 288), (320, 176))
+    sets = []
+    for k in range(3):
+        jobs = []
+        for i, (w, h) in enumerate(shapes):
+            d = synth.pan_rectangles(80 + 10 * k + i, 3, width=w, height=h, frame_indices=[0, 2])
+            p = abi.default_params(w, h, pqindex=8, motion_search_range=256, motion_local_search_range=32,
+                                   mode=abi.MODE_MIXED_MV)
+            jobs.append((d[2], d[0], p))
+        sets.append(jobs)
+    up = [[(backend.upload(c), backend.upload(r)) for c, r, _ in s] for s in sets]
+    params = [p for _, _, p in sets[0]]
+    try:
+        eager = []
+        for k in range(3):
+            _say(f"[case] eager step {k}")
+            eager.append(backend.analyze_step(up[k], params))
+        backend.set_graphs(True)
+        flags = []
+        for k in (0, 1, 2, 1, 0):
+            _say(f"[case] graph step on set {k}")
+            got = backend.analyze_step(up[k], params)
+            flags.append(backend.timing().graph_replay)
+            for g, e, p in zip(got, eager[k], params):
+                assert not g.diff(e), describe_diff(g, e, p.width)
+            backend.set_graphs(False)
+            backend.analyze_step(up[(k + 1) % 3], params, want_results=False)   # eager work in between
+            backend.set_graphs(True)
+        assert flags[0] == 2 and all(f == 1 for f in flags[1:]), flags
+    finally:
+        backend.set_graphs(False)
+        for s in up:
+            for a, b in s:
+                a.release()
+                b.release()
+    for (c, r, p), e in zip(sets[1], eager[1]):
+        want = oracle_analyze(p, c, r)
+        assert not e.diff(want), describe_diff(e, want, p.width)
+
+
+def test_batched_streams_runner_matches_oracle(backend):
+    """Streams of uneven raster and length through the batching runner (steps
+    of 3 pairs, graphs on for one raster, off for the other) give each pair
+    the oracle's result."""
+    from libvc1_amd import batching
+    specs = [(300, 176, 144, 4), (301, 320, 176, 3), (302, 176, 144, 2)]
+    streams, pics = [], []
+    for seed, w, h, n in specs:
+        d = synth.pan_rectangles(seed, n, width=w, height=h, max_pan_px=8, max_obj_px=16)
+        p = batching.stream_params(w, h)
+        pics.append([d[k] for k in range(n)])
+        streams.append(batching.Stream(w, h, [backend.upload(x) for x in pics[-1]], p, seed))
+    table = {"settings": {"176x144": {"jobs_per_step": 3, "graphs": True},
+                          "320x176": {"jobs_per_step": 3, "graphs": False}}}
+    _say("[case] batching runner, 3 streams")
+    try:
+        out = batching.run(backend, streams, table, collect=True)
+    finally:
+        for s in streams:
+            for f in s.frames:
+                f.release()
+    assert out["macroblocks"] == sum((n - 1) * ((w + 15) // 16) * ((h + 15) // 16) for _, w, h, n in specs)
+    for si, s in enumerate(streams):
+        for k, (c, r) in enumerate(s.pairs):
+            want = oracle_analyze(s.params, pics[si][c], pics[si][r])
+            got = out["results"][(si, k)]
+            assert not got.diff(want), describe_diff(got, want, s.width)
