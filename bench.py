@@ -233,7 +233,7 @@ def main() -> int:
               "(streams and frame pairs shard across GPUs without communication)", file=sys.stderr)
         return 2
 
-    from libvc1_amd import Backend
+    from libvc1_amd import Backend, abi
 
     params = workload_params()
     pairs, pics = make_frames()
@@ -252,19 +252,34 @@ def main() -> int:
     mb_ms = 0.0
     mb_launches = 0
     work = None
+    passes, analyses = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         be.analyze_step(jobs, plist, want_results=False)
         t = be.timing()
         mb_ms += t.mb_kernel_ms_total
         mb_launches += t.mb_kernel_launches
+        passes.append(int(t.relax_passes))
+        analyses.append(int(t.relax_analyses))
         w = be.work()
         work = w
     elapsed = time.perf_counter() - t0
     be.set_profiling(False)
     ms_per_step = elapsed * 1e3 / args.steps
     value = mbs_per_step * args.steps / elapsed
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s")
+    schedule = {"kind": "relax" if t.schedule == abi.SCHEDULE_RELAX else "diagonal",
+                "passes_per_step": passes, "analyses_per_step": analyses}
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s, schedule {schedule}")
+
+    # the anti-diagonal sweep on the same steps, for comparison
+    be.set_schedule(abi.SCHEDULE_DIAGONAL)
+    be.analyze_step(jobs, plist, want_results=False)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        be.analyze_step(jobs, plist, want_results=False)
+    schedule["diagonal_value"] = round(mbs_per_step * args.steps / (time.perf_counter() - t0), 1)
+    be.set_schedule(abi.SCHEDULE_RELAX)
+    log(f"diagonal sweep: {schedule['diagonal_value']:,.0f} MB/s")
 
     # download-inclusive rate (MV fields back to the host every step)
     t1 = time.perf_counter()
@@ -283,6 +298,7 @@ def main() -> int:
         log(f"parity of pair {pairs[0]} vs oracle: {parity} ({time.time() - t2:.1f}s)")
 
     # roofline of the macroblock kernel
+    mb_kernel = "k_mb_relax" if schedule["kind"] == "relax" else "k_mb_diagonal"
     cus, clk, peak = device_peak_ops()
     mb_ops = work.sad_px + work.interp_taps + work.satd_ops + work.sig_ops
     mb_sec_per_step = (mb_ms / args.steps) / 1e3
@@ -295,7 +311,7 @@ def main() -> int:
         "unit": "TOP/s",
         "frac": round(achieved / peak, 5),
         "traffic": measured_traffic(),
-        "kernel": "k_mb_diagonal",
+        "kernel": mb_kernel,
         "avg_launch_us": round(avg_launch_us, 2),
         "launches_per_step": mb_launches // max(1, args.steps),
         "ops_per_step": {"sad_px": work.sad_px, "interp_taps": work.interp_taps, "satd_ops": work.satd_ops,
@@ -310,7 +326,7 @@ def main() -> int:
                     "peak": 8000.0, "unit": "GB/s",
                     "frac": round(alg_bytes_launch / launch_s / 8e12, 7) if launch_s else 0.0,
                     "traffic": measured_traffic(), "algorithmic_bytes_per_launch": round(alg_bytes_launch),
-                    "kernel": "k_mb_diagonal"}
+                    "kernel": mb_kernel}
 
     learned_out = None
     if not args.no_learned:
@@ -339,6 +355,7 @@ def main() -> int:
                                "PQINDEX 8, plain GPU ME (bit-exact)", "width": WIDTH, "height": HEIGHT,
                    "p_frames_per_step": len(jobs), "macroblocks_per_step": mbs_per_step, "seed": SEED},
         "form": "plain",
+        "schedule": schedule,
         "value_with_download": round(with_dl, 1),
         "parity": parity,
         "roofline": roofline,

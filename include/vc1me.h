@@ -209,7 +209,10 @@ int vc1me_analyze_step(vc1me_ctx* ctx, const vc1me_job* jobs, int n);
  * whole launch sequence, and around every launch of the macroblock search
  * kernel (sum and count).  graph_replay: 0 = launched one by one, 1 = a
  * cached HIP graph of the step was replayed, 2 = the graph was captured by
- * this step (vc1me_set_graphs). */
+ * this step (vc1me_set_graphs).  schedule: the macroblock schedule the step
+ * ran (VC1ME_SCHEDULE_*); relax_passes and relax_analyses: passes of the
+ * relaxation schedule and macroblock analyses over all of them (equal to the
+ * macroblock count when one pass settles), 0 under the diagonal sweep. */
 typedef struct vc1me_timing {
     double step_ms;
     double mb_kernel_ms_total;
@@ -217,6 +220,9 @@ typedef struct vc1me_timing {
     double global_kernel_ms_total;
     uint64_t global_kernel_launches;
     uint64_t graph_replay;
+    uint64_t schedule;
+    uint64_t relax_passes;
+    uint64_t relax_analyses;
 } vc1me_timing;
 int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
 /* Record HIP events around every macroblock-kernel launch so that
@@ -233,6 +239,24 @@ int vc1me_set_profiling(vc1me_ctx* ctx, int on);
  * forms.  No reference counterpart (replaces the per-batch dispatches of
  * src/vulkan_compute.cpp:370-490). */
 int vc1me_set_graphs(vc1me_ctx* ctx, int on);
+
+/* Macroblock schedule of the plain and learned steps.  Both give the
+ * raster-order result of analyze_p_picture_core's macroblock loop
+ * (src/encoder_motion.cpp:113-428) bit for bit.
+ *   VC1ME_SCHEDULE_DIAGONAL: one launch per anti-diagonal mx + 2*my = t, so
+ *     every macroblock runs once, after its left, top-left, top and
+ *     top-right neighbours.
+ *   VC1ME_SCHEDULE_RELAX (default): every macroblock at once from a guessed
+ *     neighbour state (the global vector), then passes that re-analyse only
+ *     the macroblocks whose neighbours changed, until a pass changes
+ *     nothing; the dependency graph is acyclic, so that fixed point is the
+ *     raster-order result.  After 96 passes without settling the step
+ *     finishes with the diagonal sweep.
+ * The relaxation schedule is not captured into HIP graphs: with graphs on
+ * (vc1me_set_graphs) a plain step uses the diagonal sweep.  No reference
+ * counterpart (the reference runs its macroblock loop on one thread). */
+enum { VC1ME_SCHEDULE_DIAGONAL = 0, VC1ME_SCHEDULE_RELAX = 1 };
+int vc1me_set_schedule(vc1me_ctx* ctx, int schedule);
 
 /* Algorithmic work of the last step, summed over its jobs, in the units of
  * the roofline: one |a-b| accumulate per sampled pixel of a block SAD

@@ -25,6 +25,10 @@ MODE_ONE_MV_HPEL = 1
 MODE_ONE_MV_HPEL_BILINEAR = 2
 MODE_MIXED_MV = 3
 
+# macroblock schedules (vc1me_set_schedule)
+SCHEDULE_DIAGONAL = 0
+SCHEDULE_RELAX = 1
+
 
 class Candidate(C.Structure):
     _fields_ = [("xq", C.c_int32), ("yq", C.c_int32)]
@@ -84,7 +88,8 @@ class Counters(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("step_ms", C.c_double), ("mb_kernel_ms_total", C.c_double),
                 ("mb_kernel_launches", C.c_uint64), ("global_kernel_ms_total", C.c_double),
-                ("global_kernel_launches", C.c_uint64), ("graph_replay", C.c_uint64)]
+                ("global_kernel_launches", C.c_uint64), ("graph_replay", C.c_uint64),
+                ("schedule", C.c_uint64), ("relax_passes", C.c_uint64), ("relax_analyses", C.c_uint64)]
 
 
 class Work(C.Structure):

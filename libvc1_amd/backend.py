@@ -25,7 +25,7 @@ EXPORTS = (
     "vc1me_block_costs", "vc1me_nn_param_count", "vc1me_nn_create", "vc1me_nn_destroy", "vc1me_nn_set_params",
     "vc1me_nn_get_params", "vc1me_nn_forward_frames", "vc1me_nn_forward_backward", "vc1me_nn_train_step",
     "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
-    "vc1me_set_graphs",
+    "vc1me_set_graphs", "vc1me_set_schedule",
 )
 
 
@@ -90,6 +90,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_analyze_step_proposals.argtypes = [C.c_void_p, P(abi.Job), C.c_int, P(P(C.c_float)), C.c_float]
     lib.vc1me_set_cost_report.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    lib.vc1me_set_schedule.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_last_mb_costs.argtypes = [C.c_void_p, C.c_int, P(C.c_uint64), P(C.c_int32)]
     _lib = lib
     return lib
@@ -192,6 +193,11 @@ class Backend:
     def set_graphs(self, on: bool) -> None:
         """Replay plain steps as captured HIP graphs (vc1me_set_graphs)."""
         self._check(self.lib.vc1me_set_graphs(self.ctx, 1 if on else 0))
+
+    def set_schedule(self, schedule: int) -> None:
+        """Macroblock schedule of later steps (abi.SCHEDULE_DIAGONAL or
+        abi.SCHEDULE_RELAX; vc1me_set_schedule)."""
+        self._check(self.lib.vc1me_set_schedule(self.ctx, int(schedule)))
 
     def set_cost_report(self, on: bool) -> None:
         self._check(self.lib.vc1me_set_cost_report(self.ctx, 1 if on else 0))

@@ -61,6 +61,14 @@ bool injected_failure() {
     return at > 0 && n == at;
 }
 
+// Pass limit of the relaxation schedule; VC1ME_RELAX_MAX_PASSES lowers it
+// (read at every step) so that the diagonal fallback can be exercised.
+int relax_pass_limit() {
+    const char* e = std::getenv("VC1ME_RELAX_MAX_PASSES");
+    const int v = e && *e ? std::atoi(e) : vc1me::kMaxRelaxPasses;
+    return std::max(1, std::min(v, vc1me::kMaxRelaxPasses));
+}
+
 uint64_t fnv64(const uint8_t* p, size_t n) {
     uint64_t h = 1469598103934665603ull;
     size_t i = 0;
@@ -97,6 +105,10 @@ struct JobSlot {
     uint32_t* ext_ops = nullptr;
     uint64_t* cost20 = nullptr;     // cost report
     int2* one_mv = nullptr;
+    uint64_t* mbops = nullptr;      // per-macroblock op counts of the last analysis
+    int32_t* wl0 = nullptr;         // relaxation worklists and queue stamps
+    int32_t* wl1 = nullptr;
+    uint32_t* mark = nullptr;
     int mbs_last = 0;               // macroblocks of the last step's job
 };
 
@@ -129,6 +141,10 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_exo = carve(mbs * 2 * sizeof(uint32_t));
     const size_t o_c20 = carve(mbs * 8);
     const size_t o_one = carve(mbs * sizeof(int2));
+    const size_t o_mbops = carve(mbs * 5 * sizeof(uint64_t));
+    const size_t o_wl0 = carve(mbs * sizeof(int32_t));
+    const size_t o_wl1 = carve(mbs * sizeof(int32_t));
+    const size_t o_mark = carve(mbs * sizeof(uint32_t));
     hipError_t e = hipMalloc(&s.mem, off);
     if (e != hipSuccess) { s.mem = nullptr; return e; }
     uint8_t* b = static_cast<uint8_t*>(s.mem);
@@ -153,6 +169,10 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.ext_ops = reinterpret_cast<uint32_t*>(b + o_exo);
     s.cost20 = reinterpret_cast<uint64_t*>(b + o_c20);
     s.one_mv = reinterpret_cast<int2*>(b + o_one);
+    s.mbops = reinterpret_cast<uint64_t*>(b + o_mbops);
+    s.wl0 = reinterpret_cast<int32_t*>(b + o_wl0);
+    s.wl1 = reinterpret_cast<int32_t*>(b + o_wl1);
+    s.mark = reinterpret_cast<uint32_t*>(b + o_mark);
     s.mbs_cap = mbs;
     s.cells_cap = cells;
     return hipSuccess;
@@ -199,6 +219,12 @@ struct vc1me_ctx {
     // proposals handed in by the host (vc1me_analyze_step_proposals)
     float* d_props = nullptr;
     size_t props_cap = 0;
+    // macroblock schedule (VC1ME_SCHEDULE_*) and the relaxation's per-pass
+    // worklist lengths: device [pass][job], pinned host copy of one row
+    int schedule = VC1ME_SCHEDULE_RELAX;
+    int32_t* d_wl_n = nullptr;
+    int32_t* h_wl_n = nullptr;
+    size_t wl_jobs_cap = 0;
 };
 
 namespace {
@@ -361,6 +387,8 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.err = s.err;
     d.ops = s.ops;
     d.ext_sad = s.ext_sad; d.ext_mv = s.ext_mv; d.ext_ops = s.ext_ops;
+    d.mbops = s.mbops;
+    d.wl0 = s.wl0; d.wl1 = s.wl1; d.mark = s.mark;
     d.nn_out = nullptr; d.nn_logit = 0.f;
     d.cost20 = nullptr; d.one_mv = nullptr;
     d.lam20 = std::max(20, 7 * p->pqindex);
@@ -514,7 +542,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         ctx->d_jobs_cap = n;
     }
     hipStream_t st = ctx->stream;
-    if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1))) return rc;
+    if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1 + vc1me::kMaxRelaxPasses))) return rc;
     // interpolated references: one owning job per distinct (frame, filter),
     // each with a buffer of the context's step pool (planes live for the step)
     int max_pw = 0, max_ph = 0;
@@ -546,6 +574,24 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             if (bil) d.bp = buf; else d.qp = buf;
         }
     }
+    const bool relax = ctx->schedule == VC1ME_SCHEDULE_RELAX && !(ctx->use_graphs && !ctx->profiling && !after_global);
+    if (relax) {
+        if (ctx->wl_jobs_cap < static_cast<size_t>(n)) {
+            if (ctx->d_wl_n) (void)hipFree(ctx->d_wl_n);
+            if (ctx->h_wl_n) (void)hipHostFree(ctx->h_wl_n);
+            ctx->d_wl_n = nullptr;
+            ctx->h_wl_n = nullptr;
+            ctx->wl_jobs_cap = 0;
+            HIP_TRY(hipMalloc(&ctx->d_wl_n, sizeof(int32_t) * (vc1me::kMaxRelaxPasses + 1) * n));
+            HIP_TRY(hipHostMalloc(&ctx->h_wl_n, sizeof(int32_t) * n, hipHostMallocDefault));
+            ctx->wl_jobs_cap = n;
+        }
+        HIP_TRY(hipMemsetAsync(ctx->d_wl_n, 0, sizeof(int32_t) * (vc1me::kMaxRelaxPasses + 1) * n, st));
+        for (int i = 0; i < n; ++i) {
+            ctx->h_jobs[i].wl_n = ctx->d_wl_n + i;
+            ctx->h_jobs[i].wl_stride = n;
+        }
+    }
     HIP_TRY(hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs.data(), sizeof(JobDesc) * n, hipMemcpyHostToDevice, st));
     for (int i = 0; i < n; ++i) {
         HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
@@ -567,6 +613,60 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
         return per;
     };
+    // one launch per anti-diagonal: every macroblock after its neighbours
+    int mb_launches = 0;
+    std::vector<std::pair<int, int>> mb_events;   // event pairs around macroblock launches
+    auto diagonal_all = [&](bool events) -> int {
+        for (int t = 0; t <= max_diag; ++t) {
+            const int per = per_diag(t);
+            if (!per) continue;
+            const int ev = 4 + 2 * mb_launches;
+            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[ev], st));
+            HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
+            if (events && ctx->profiling) {
+                HIP_TRY(hipEventRecord(ctx->events[ev + 1], st));
+                mb_events.emplace_back(ev, ev + 1);
+            }
+            ++mb_launches;
+        }
+        return VC1ME_OK;
+    };
+    // relaxation: all macroblocks from the guessed state, then worklist passes
+    // until one changes nothing (the host reads each pass's list length)
+    int max_mbs_all = 0;
+    for (int i = 0; i < n; ++i) max_mbs_all = std::max(max_mbs_all, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh);
+    uint64_t relax_passes = 0, relax_analyses = 0;
+    bool relax_fell_back = false;
+    auto relax_all = [&](bool events) -> int {
+        HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
+        const int limit = relax_pass_limit();
+        int grid = max_mbs_all;
+        for (int i = 0; i < n; ++i) relax_analyses += static_cast<uint64_t>(ctx->h_jobs[i].mbw) * ctx->h_jobs[i].mbh;
+        for (int pass = 0;; ++pass) {
+            const int ev = 4 + 2 * mb_launches;
+            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[ev], st));
+            HIP_TRY(vc1me::launch_mb_relax(ctx->d_jobs, n, pass, grid, lds, st));
+            if (events && ctx->profiling) {
+                HIP_TRY(hipEventRecord(ctx->events[ev + 1], st));
+                mb_events.emplace_back(ev, ev + 1);
+            }
+            ++mb_launches;
+            ++relax_passes;
+            HIP_TRY(hipMemcpyAsync(ctx->h_wl_n, ctx->d_wl_n + static_cast<size_t>(pass + 1) * n, sizeof(int32_t) * n,
+                                   hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            grid = 0;
+            for (int i = 0; i < n; ++i) {
+                grid = std::max(grid, ctx->h_wl_n[i]);
+                relax_analyses += static_cast<uint64_t>(ctx->h_wl_n[i]);
+            }
+            if (grid == 0) return VC1ME_OK;
+            if (pass + 1 >= limit) break;
+        }
+        // not settled: the diagonal sweep is exact from any state
+        relax_fell_back = true;
+        return diagonal_all(events);
+    };
     // the step's launch sequence; every launch takes the job table and scalars only
     auto launch_all = [&](bool events) -> int {
         HIP_TRY(vc1me::launch_phase_planes_jobs(ctx->d_jobs, n, max_pw, max_ph, st));
@@ -576,19 +676,16 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             if (int rc = after_global(st)) return rc;
         HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
         HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
-        for (int t = 0; t <= max_diag; ++t) {
-            const int per = per_diag(t);
-            if (!per) continue;
-            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[4 + 2 * t], st));
-            HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
-            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[5 + 2 * t], st));
+        if (relax) {
+            if (int rc = relax_all(events)) return rc;
+        } else {
+            if (int rc = diagonal_all(events)) return rc;
         }
+        HIP_TRY(vc1me::launch_ops_sum(ctx->d_jobs, n, st));
         return VC1ME_OK;
     };
-    int launches = 0;
-    for (int t = 0; t <= max_diag; ++t) launches += per_diag(t) ? 1 : 0;
     ctx->timing.graph_replay = 0;
-    if (ctx->use_graphs && !ctx->profiling && !after_global) {
+    if (!relax && ctx->use_graphs && !ctx->profiling && !after_global) {
         // replay the step as one HIP graph, captured once per launch shape
         std::string key;
         auto put = [&key](long long v) { key += std::to_string(v); key += ','; };
@@ -637,16 +734,20 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     ctx->timing.global_kernel_ms_total = gms;
     ctx->timing.global_kernel_launches = 9;
     double mbms = 0.0;
-    if (ctx->profiling) {
-        for (int t = 0; t <= max_diag; ++t) {
-            if (!per_diag(t)) continue;
-            float x = 0.f;
-            HIP_TRY(hipEventElapsedTime(&x, ctx->events[4 + 2 * t], ctx->events[5 + 2 * t]));
-            mbms += x;
-        }
+    for (const auto& e : mb_events) {
+        float x = 0.f;
+        HIP_TRY(hipEventElapsedTime(&x, ctx->events[e.first], ctx->events[e.second]));
+        mbms += x;
+    }
+    if (ctx->timing.graph_replay) {
+        mb_launches = 0;
+        for (int t = 0; t <= max_diag; ++t) mb_launches += per_diag(t) ? 1 : 0;
     }
     ctx->timing.mb_kernel_ms_total = mbms;
-    ctx->timing.mb_kernel_launches = static_cast<uint64_t>(launches);
+    ctx->timing.mb_kernel_launches = static_cast<uint64_t>(mb_launches);
+    ctx->timing.schedule = relax && !relax_fell_back ? VC1ME_SCHEDULE_RELAX : VC1ME_SCHEDULE_DIAGONAL;
+    ctx->timing.relax_passes = relax_passes;
+    ctx->timing.relax_analyses = relax ? relax_analyses : 0;
     // kernel-side error flags and work counters
     ctx->work = vc1me_work{};
     for (int i = 0; i < n; ++i) {
@@ -874,6 +975,8 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_cands) (void)hipFree(ctx->d_cands);
     if (ctx->d_costs) (void)hipFree(ctx->d_costs);
     if (ctx->d_props) (void)hipFree(ctx->d_props);
+    if (ctx->d_wl_n) (void)hipFree(ctx->d_wl_n);
+    if (ctx->h_wl_n) (void)hipHostFree(ctx->h_wl_n);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& b : ctx->phase_pool) if (b.first) (void)hipFree(b.first);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
@@ -1054,6 +1157,15 @@ int vc1me_set_graphs(vc1me_ctx* ctx, int on) {
     if (!ctx) return fail(VC1ME_EINVAL, "null context");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->use_graphs = on != 0;
+    return VC1ME_OK;
+}
+
+int vc1me_set_schedule(vc1me_ctx* ctx, int schedule) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (schedule != VC1ME_SCHEDULE_DIAGONAL && schedule != VC1ME_SCHEDULE_RELAX)
+        return fail(VC1ME_EINVAL, "unknown schedule " + std::to_string(schedule));
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->schedule = schedule;
     return VC1ME_OK;
 }
 

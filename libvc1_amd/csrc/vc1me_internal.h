@@ -75,7 +75,23 @@ struct JobDesc {
     // this job computes its reference's phase planes in the step (the first
     // job of the step with that reference and filter)
     int32_t phase_owner;
+    // algorithmic operation counts of each macroblock's last analysis:
+    // sad_px, interp_taps, satd_ops, sig_ops, seeded (5 per macroblock),
+    // summed into ops[] once the macroblocks are final
+    uint64_t* mbops;
+    // relaxation schedule: two worklists of macroblock indices (pass p reads
+    // list p & 1, appends to the other), the queue stamp of each macroblock
+    // (p + 1 = already queued for pass p + 1), and the per-pass list lengths
+    // at wl_n[p * wl_stride]
+    int32_t* wl0; int32_t* wl1;
+    uint32_t* mark;
+    int32_t* wl_n;
+    int32_t wl_stride;
 };
+
+// Longest relaxation run before the step falls back to the anti-diagonal
+// sweep (which is exact from any starting state).
+constexpr int kMaxRelaxPasses = 96;
 
 // Phase-plane geometry of a W x H reference: sample (px, py), px in [-2, W],
 // py in [-2, H], lives at row py + 2, column px + 2.  The pitch leaves room
@@ -94,6 +110,11 @@ hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hi
 hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s);
+// relaxation schedule: starting state, one pass over a worklist, and the
+// per-job op totals of the final macroblock analyses
+hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipStream_t s);
+hipError_t launch_mb_relax(const JobDesc* d_jobs, int njobs, int pass, int grid_x, int lds_bytes, hipStream_t s);
+hipError_t launch_ops_sum(const JobDesc* d_jobs, int njobs, hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);
 // phase planes of the step's owning jobs (k_phase_planes_jobs)
 hipError_t launch_phase_planes_jobs(const JobDesc* d_jobs, int njobs, int max_w, int max_h, hipStream_t s);

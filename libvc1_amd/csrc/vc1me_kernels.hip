@@ -1940,7 +1940,16 @@ __device__ __forceinline__ int median4(int a, int b, int c, int d) {
     return (imin(a, c) + imax(b, d)) / 2;
 }
 
-__device__ __forceinline__ void analyze_mb(Ctx& C) {
+// The part of a macroblock's result that its right, bottom-left, bottom and
+// bottom-right neighbours read: 1-MV and block vectors, intra flag and the
+// "propagated" flag of the long-range stage.
+struct NbState {
+    int2 mv;
+    int2 b[4];
+    uint32_t fp;    // intra flag | propagated flag << 8
+};
+
+__device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     const JobDesc* J = C.J;
     const int mode = J->mode;
     const bool allow_4mv = mode == 3;
@@ -1971,12 +1980,17 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
             J->mvs[pos] = make_int2(0, 0);
             for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = make_int2(0, 0);
             J->flags[pos] = 0;
+            J->prop[pos] = 0;
             J->ratio[pos] = -1.0;
             J->dl_mae[pos] = -1.0;
             J->dc_mae[pos] = -1.0;
             J->decision[pos] = 0;
-            atomicAdd(&J->ops[0], static_cast<unsigned long long>(C.n_sad));
+            uint64_t* o = J->mbops + static_cast<size_t>(pos) * 5;
+            o[0] = C.n_sad; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0;
         }
+        st.mv = make_int2(0, 0);
+        for (int k = 0; k < 4; ++k) st.b[k] = make_int2(0, 0);
+        st.fp = 0;
         return;
     }
 
@@ -1984,6 +1998,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
     wsync();
     const Pred rate_pi = predictor_mixed_1mv(C);
     bool was_seeded = false;
+    uint8_t prop_out = 0;
     {
         const Pred pi = predictor_info(C);
         const int gdx = J->g->dx, gdy = J->g->dy;
@@ -2036,7 +2051,8 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
             }
             if (ext.dx != im.dx || ext.dy != im.dy) { im = ext; selected_distant = true; }
         }
-        if (C.lane == 0) J->prop[pos] = selected_distant ? 1 : 0;
+        prop_out = selected_distant ? 1 : 0;
+        if (C.lane == 0) J->prop[pos] = prop_out;
         if (J->subpel) pw_load(C, C.x0 + im.dx - kPwMargin, C.y0 + im.dy - kPwMargin, mode_bilinear(one_mode));
         SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true};
         one = refine_staged(C, sp, im.dx * 4, im.dy * 4, rate_pi, &one_sad, im.sad);
@@ -2226,12 +2242,14 @@ __device__ __forceinline__ void analyze_mb(Ctx& C) {
         J->decision[pos] = static_cast<uint8_t>(decision_out);
         J->chosen[pos] = chosen_sad;
         J->temporal[pos] = temporal_sad;
-        atomicAdd(&J->ops[0], static_cast<unsigned long long>(C.n_sad));
-        atomicAdd(&J->ops[1], static_cast<unsigned long long>(C.n_taps));
-        atomicAdd(&J->ops[2], static_cast<unsigned long long>(C.n_satd));
-        atomicAdd(&J->ops[3], static_cast<unsigned long long>(C.n_sig));
-        if (was_seeded) atomicAdd(&J->ops[5], 1ull);
+        // this analysis's op counts replace any earlier analysis of the
+        // macroblock (the relaxation schedule may analyse it more than once)
+        uint64_t* o = J->mbops + static_cast<size_t>(pos) * 5;
+        o[0] = C.n_sad; o[1] = C.n_taps; o[2] = C.n_satd; o[3] = C.n_sig; o[4] = was_seeded ? 1 : 0;
     }
+    st.mv = outmv;
+    for (int k = 0; k < 4; ++k) st.b[k] = outb[k];
+    st.fp = static_cast<uint32_t>(flags & kFlagIntra) | (static_cast<uint32_t>(prop_out) << 8);
 }
 
 __device__ __forceinline__ void init_ctx(Ctx& C, const JobDesc* J, int mx, int my, uint8_t* smem) {
@@ -2251,10 +2269,12 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const JobDesc* J, int mx, int m
     C.ln_sad = C.ln_taps = C.ln_satd = 0;
 }
 
-__device__ __forceinline__ void run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
+__device__ __forceinline__ NbState run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
     Ctx C;
     init_ctx(C, J, mx, my, smem);
-    analyze_mb(C);
+    NbState st;
+    analyze_mb(C, st);
+    return st;
 }
 
 // The extended long-range search of every macroblock of jobs with a search
@@ -2342,6 +2362,114 @@ __global__ void __launch_bounds__(64) k_mb_diagonal(const JobDesc* __restrict__ 
     const int mx = diag - 2 * my;
     if (mx < 0 || mx >= mbw) return;
     run_mb(J, mx, my, smem);
+}
+
+// ------------------------------------------------------------------ relaxation schedule
+//
+// A macroblock's analysis is a deterministic function of the frames, the
+// frame-level results (global vector, signature index, extended-search
+// results, proposals) and the state of its left, top-left, top and top-right
+// neighbours: their 1-MV and block vectors, intra flag and "propagated"
+// flag.  Those neighbours precede it in raster order, so the dependency
+// graph is acyclic and the raster-order result S is the only state F with
+// F[m] = analysis(m, F[neighbours of m]) for every m.
+//
+// The relaxation schedule starts from a guessed state (every macroblock at
+// the global vector), analyses every macroblock at once, then re-analyses,
+// pass after pass, exactly the macroblocks with a neighbour whose state
+// changed in the previous pass.  When a pass changes nothing, every
+// macroblock's last analysis read its neighbours' final state (a neighbour
+// that changed later would have queued it again), so the state is that
+// fixed point: S, bit for bit.  Reads of a neighbour that is being
+// rewritten in the same pass may see either value; the rewrite queues the
+// reader again, so the race cannot leak into the result.
+
+__device__ __forceinline__ NbState nb_state(const JobDesc* J, int pos) {
+    NbState s;
+    s.mv = J->mvs[pos];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s.b[k] = J->bmv[pos * 4 + k];
+    s.fp = static_cast<uint32_t>(J->flags[pos] & kFlagIntra) | (static_cast<uint32_t>(J->prop[pos]) << 8);
+    return s;
+}
+
+__device__ __forceinline__ bool nb_equal(const NbState& a, const NbState& b) {
+    bool eq = a.mv.x == b.mv.x && a.mv.y == b.mv.y && a.fp == b.fp;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) eq = eq && a.b[k].x == b.b[k].x && a.b[k].y == b.b[k].y;
+    return eq;
+}
+
+// starting state: every macroblock at the frame's global vector, no intra,
+// nothing propagated; no macroblock queued yet
+__global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ jobs) {
+    const JobDesc* J = &jobs[blockIdx.y];
+    const int pos = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (pos >= J->mbw * J->mbh) return;
+    const int2 g = J->range == 0 ? make_int2(0, 0) : make_int2(J->g->dx * 4, J->g->dy * 4);
+    J->mvs[pos] = g;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = g;
+    J->flags[pos] = 0;
+    J->prop[pos] = 0;
+    J->mark[pos] = 0;
+}
+
+// One pass: pass 0 analyses every macroblock, pass p > 0 the worklist the
+// previous pass built; one workgroup per list entry (the host sizes the grid
+// from the list lengths).  The next list gets each dependent of a changed
+// macroblock once (the mark stamp).
+__global__ void __launch_bounds__(64) k_mb_relax(const JobDesc* __restrict__ jobs, int pass) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const JobDesc* J = &jobs[blockIdx.y];
+    const int mbw = J->mbw, mbh = J->mbh;
+    const int n = pass == 0 ? mbw * mbh : J->wl_n[pass * J->wl_stride];
+    const int i = static_cast<int>(blockIdx.x);
+    if (i >= n) return;
+    const int pos = pass == 0 ? i : ((pass & 1) ? J->wl1 : J->wl0)[i];
+    const NbState before = nb_state(J, pos);
+    const NbState after = run_mb(J, pos % mbw, pos / mbw, smem);
+    if (threadIdx.x == 0 && !nb_equal(before, after)) {
+        int32_t* next = (pass & 1) ? J->wl0 : J->wl1;
+        const int mx = pos % mbw, my = pos / mbw;
+        const int dxs[4] = {1, -1, 0, 1}, dys[4] = {0, 1, 1, 1};
+        for (int k = 0; k < 4; ++k) {
+            const int x = mx + dxs[k], y = my + dys[k];
+            if (x < 0 || x >= mbw || y >= mbh) continue;
+            const int d = y * mbw + x;
+            if (atomicExch(&J->mark[d], static_cast<uint32_t>(pass + 1)) != static_cast<uint32_t>(pass + 1)) {
+                const int at = atomicAdd(&J->wl_n[(pass + 1) * J->wl_stride], 1);
+                next[at] = d;
+            }
+        }
+    }
+}
+
+// per-job totals of the final analyses' op counts (block = job)
+__global__ void __launch_bounds__(256) k_ops_sum(const JobDesc* __restrict__ jobs) {
+    const JobDesc* J = &jobs[blockIdx.x];
+    const int mbs = J->mbw * J->mbh;
+    __shared__ unsigned long long part[5][256 / kWave];
+    unsigned long long s[5] = {0, 0, 0, 0, 0};
+    for (int pos = static_cast<int>(threadIdx.x); pos < mbs; pos += 256) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s[k] += J->mbops[static_cast<size_t>(pos) * 5 + k];
+    }
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        unsigned long long v = s[k];
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+        if (lane == 0) part[k][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < 5; ++k)
+            for (int j = 0; j < 256 / kWave; ++j) t[k] += part[k][j];
+        for (int k = 0; k < 4; ++k) atomicAdd(&J->ops[k], t[k]);
+        atomicAdd(&J->ops[5], t[4]);
+    }
 }
 
 // ------------------------------------------------------------------ global search
@@ -2823,6 +2951,25 @@ hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int ld
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s) {
     hipLaunchKernelGGL(k_mb_diagonal, dim3(max_per_diag, njobs), dim3(64), lds_bytes, s, d_jobs, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipStream_t s) {
+    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_relax_init, dim3((max_mbs + 255) / 256, njobs), dim3(256), 0, s, d_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_mb_relax(const JobDesc* d_jobs, int njobs, int pass, int grid_x, int lds_bytes, hipStream_t s) {
+    if (njobs <= 0 || grid_x <= 0) return hipSuccess;
+    if (pass < 0 || pass >= kMaxRelaxPasses) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mb_relax, dim3(grid_x, njobs), dim3(64), lds_bytes, s, d_jobs, pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_ops_sum(const JobDesc* d_jobs, int njobs, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ops_sum, dim3(njobs), dim3(256), 0, s, d_jobs);
     return hipGetLastError();
 }
 
