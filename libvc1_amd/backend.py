@@ -25,7 +25,7 @@ EXPORTS = (
     "vc1me_block_costs", "vc1me_nn_param_count", "vc1me_nn_create", "vc1me_nn_destroy", "vc1me_nn_set_params",
     "vc1me_nn_get_params", "vc1me_nn_forward_frames", "vc1me_nn_forward_backward", "vc1me_nn_train_step",
     "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
-    "vc1me_set_graphs", "vc1me_set_schedule",
+    "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes",
 )
 
 
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_set_cost_report.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_set_graphs.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_set_schedule.argtypes = [C.c_void_p, C.c_int]
+    lib.vc1me_last_relax_passes.argtypes = [C.c_void_p, P(C.c_int64), P(C.c_double), C.c_int]
     lib.vc1me_last_mb_costs.argtypes = [C.c_void_p, C.c_int, P(C.c_uint64), P(C.c_int32)]
     _lib = lib
     return lib
@@ -198,6 +199,16 @@ class Backend:
         """Macroblock schedule of later steps (abi.SCHEDULE_DIAGONAL or
         abi.SCHEDULE_RELAX; vc1me_set_schedule)."""
         self._check(self.lib.vc1me_set_schedule(self.ctx, int(schedule)))
+
+    def relax_passes(self):
+        """[(analyses, device ms or -1)] per relaxation pass of the last step."""
+        cap = 128
+        a = (C.c_int64 * cap)()
+        m = (C.c_double * cap)()
+        n = self.lib.vc1me_last_relax_passes(self.ctx, a, m, cap)
+        if n < 0:
+            self._check(n)
+        return [(int(a[k]), float(m[k])) for k in range(min(n, cap))]
 
     def set_cost_report(self, on: bool) -> None:
         self._check(self.lib.vc1me_set_cost_report(self.ctx, 1 if on else 0))

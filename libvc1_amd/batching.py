@@ -5,16 +5,20 @@ Every P pair of a stream (frame k against frame k-1) is independent of the
 others once the frames exist, so the runner flattens the pairs of all
 streams, groups them by raster, and runs each group as analysis steps of
 `jobs_per_step` pairs: one launch per anti-diagonal covers every macroblock
-of that diagonal in all pairs of the step (vc1me_analyze_step), optionally
-replayed from a captured HIP graph (vc1me_set_graphs).  Grouping by raster
+of the step (vc1me_analyze_step) under one of the two macroblock schedules
+(vc1me_set_schedule): the relaxation schedule (every macroblock at once, then
+worklist passes until nothing changes) or the anti-diagonal sweep, which may
+be replayed from a captured HIP graph (vc1me_set_graphs).  Grouping by raster
 keeps a step's launch count equal to its own frames' diagonal count, so small
 frames never wait on large ones.
 
 The autotuner measures, per raster, macroblock searches per second for each
-candidate step size with graphs on and off, and persists the winners in a
-JSON table per device (``tuning/<device>.json``).  The design has one wave
-per macroblock by construction, so the tuned knobs are the step composition
-and the launch mode; the table changes speed only, never results.
+candidate step size under each launch mode (relaxation; sweep; sweep replayed
+from a graph), and persists the winners in a JSON table per device
+(``tuning/<device>.json``).  The design has one wave per macroblock by
+construction, so the tuned knobs are the step composition, the schedule and
+the launch mode; every mode gives the same results, so the table changes
+speed only.
 
     python -m libvc1_amd.batching tune  [--resolutions 720x480,1280x720,1920x1080,3840x2160]
     python -m libvc1_amd.batching bench [--streams 64] [--seed 5] [--length-scale 1.0]
@@ -37,7 +41,9 @@ from .backend import Backend
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 RESOLUTIONS = ((720, 480), (1280, 720), (1920, 1080), (3840, 2160))
 STEP_CANDIDATES = (4, 8, 16, 32, 64, 128, 255)
-DEFAULT_SETTING = {"jobs_per_step": 16, "graphs": True}
+DEFAULT_SETTING = {"jobs_per_step": 16, "schedule": "relax", "graphs": False}
+# launch modes the tuner tries: (schedule, graphs)
+MODES = (("relax", False), ("diagonal", False), ("diagonal", True))
 
 
 def log(msg: str) -> None:
@@ -99,11 +105,19 @@ def load_table(backend: Backend) -> dict:
 
 
 def setting_for(table: dict, width: int, height: int) -> dict:
-    return table.get("settings", {}).get(f"{width}x{height}", DEFAULT_SETTING)
+    st = dict(table.get("settings", {}).get(f"{width}x{height}", DEFAULT_SETTING))
+    st.setdefault("schedule", "diagonal")      # tables written before the relaxation schedule
+    return st
 
 
-def _run_steps(backend: Backend, jobs, params, per_step: int, graphs: bool, collect: bool = False) -> list:
+def _schedule_code(name: str) -> int:
+    return abi.SCHEDULE_RELAX if name == "relax" else abi.SCHEDULE_DIAGONAL
+
+
+def _run_steps(backend: Backend, jobs, params, per_step: int, graphs: bool, schedule: str = "relax",
+               collect: bool = False) -> list:
     backend.set_graphs(graphs)
+    backend.set_schedule(_schedule_code(schedule))
     out = []
     try:
         for i in range(0, len(jobs), per_step):
@@ -111,13 +125,15 @@ def _run_steps(backend: Backend, jobs, params, per_step: int, graphs: bool, coll
             out += backend.analyze_step(chunk, [params] * len(chunk), want_results=collect)
     finally:
         backend.set_graphs(False)
+        backend.set_schedule(abi.SCHEDULE_RELAX)
     return out
 
 
 def autotune(backend: Backend, resolutions=RESOLUTIONS, candidates=STEP_CANDIDATES, frames: int = 5,
              seed: int = 77, repeats: int = 2) -> dict:
     """Measures MB searches/s per raster for every step size and launch mode
-    on `frames`-frame synthetic clips; returns the table (not yet saved)."""
+    (MODES) on `frames`-frame synthetic clips; returns the table (not yet
+    saved)."""
     settings, trials = {}, {}
     for (w, h) in resolutions:
         pics = synth.pan_rectangles(seed + w, frames, width=w, height=h)
@@ -128,18 +144,19 @@ def autotune(backend: Backend, resolutions=RESOLUTIONS, candidates=STEP_CANDIDAT
         best, res = None, {}
         for b in candidates:
             jobs = (base * ((b + len(base) - 1) // len(base)))[:b]   # b independent jobs
-            for graphs in (False, True):
-                _run_steps(backend, jobs, params, b, graphs)          # warm up (and capture)
+            for schedule, graphs in MODES:
+                _run_steps(backend, jobs, params, b, graphs, schedule)          # warm up (and capture)
                 t0 = time.perf_counter()
                 for _ in range(repeats):
-                    _run_steps(backend, jobs, params, b, graphs)
+                    _run_steps(backend, jobs, params, b, graphs, schedule)
                 rate = repeats * b * mbs / (time.perf_counter() - t0)
-                res[f"{b}{'g' if graphs else ''}"] = round(rate, 1)
+                res[f"{b}{'r' if schedule == 'relax' else ''}{'g' if graphs else ''}"] = round(rate, 1)
                 if best is None or rate > best[0]:
-                    best = (rate, b, graphs)
+                    best = (rate, b, graphs, schedule)
         for f in fr:
             f.release()
-        settings[f"{w}x{h}"] = {"jobs_per_step": best[1], "graphs": best[2], "mbs_per_s": round(best[0], 1)}
+        settings[f"{w}x{h}"] = {"jobs_per_step": best[1], "schedule": best[3], "graphs": best[2],
+                                "mbs_per_s": round(best[0], 1)}
         trials[f"{w}x{h}"] = res
         log(f"{w}x{h}: best {settings[f'{w}x{h}']} of {res}")
     return {"device": backend.device_name, "settings": settings, "trials": trials,
@@ -168,7 +185,8 @@ def run(backend: Backend, streams: list[Stream], table: dict | None = None, coll
     for (w, h), jobs in sorted(groups.items()):
         st = setting_for(table, w, h)
         t1 = time.perf_counter()
-        res = _run_steps(backend, jobs, stream_params(w, h), int(st["jobs_per_step"]), bool(st["graphs"]), collect)
+        res = _run_steps(backend, jobs, stream_params(w, h), int(st["jobs_per_step"]), bool(st["graphs"]),
+                         str(st["schedule"]), collect)
         results.update(zip(keys[(w, h)], res))
         mbs = len(jobs) * int(np.prod(abi.mb_grid(w, h)))
         per[f"{w}x{h}"] = {"pairs": len(jobs), "macroblocks": mbs, "seconds": round(time.perf_counter() - t1, 4),

@@ -103,6 +103,7 @@ struct JobSlot {
     uint64_t* ext_sad = nullptr;
     int2* ext_mv = nullptr;
     uint32_t* ext_ops = nullptr;
+    uint8_t* ext_done = nullptr;
     uint64_t* cost20 = nullptr;     // cost report
     int2* one_mv = nullptr;
     uint64_t* mbops = nullptr;      // per-macroblock op counts of the last analysis
@@ -139,6 +140,7 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_exs = carve(mbs * 8);
     const size_t o_exm = carve(mbs * sizeof(int2));
     const size_t o_exo = carve(mbs * 2 * sizeof(uint32_t));
+    const size_t o_exd = carve(mbs);
     const size_t o_c20 = carve(mbs * 8);
     const size_t o_one = carve(mbs * sizeof(int2));
     const size_t o_mbops = carve(mbs * 5 * sizeof(uint64_t));
@@ -167,6 +169,7 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.ext_sad = reinterpret_cast<uint64_t*>(b + o_exs);
     s.ext_mv = reinterpret_cast<int2*>(b + o_exm);
     s.ext_ops = reinterpret_cast<uint32_t*>(b + o_exo);
+    s.ext_done = b + o_exd;
     s.cost20 = reinterpret_cast<uint64_t*>(b + o_c20);
     s.one_mv = reinterpret_cast<int2*>(b + o_one);
     s.mbops = reinterpret_cast<uint64_t*>(b + o_mbops);
@@ -225,6 +228,10 @@ struct vc1me_ctx {
     int32_t* d_wl_n = nullptr;
     int32_t* h_wl_n = nullptr;
     size_t wl_jobs_cap = 0;
+    // last step's relaxation passes: analyses per pass (summed over jobs) and,
+    // with profiling on, device milliseconds per pass
+    std::vector<int64_t> pass_analyses;
+    std::vector<double> pass_ms;
 };
 
 namespace {
@@ -386,7 +393,7 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.g = s.g;
     d.err = s.err;
     d.ops = s.ops;
-    d.ext_sad = s.ext_sad; d.ext_mv = s.ext_mv; d.ext_ops = s.ext_ops;
+    d.ext_sad = s.ext_sad; d.ext_mv = s.ext_mv; d.ext_ops = s.ext_ops; d.ext_done = s.ext_done;
     d.mbops = s.mbops;
     d.wl0 = s.wl0; d.wl1 = s.wl1; d.mark = s.mark;
     d.nn_out = nullptr; d.nn_logit = 0.f;
@@ -596,6 +603,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     for (int i = 0; i < n; ++i) {
         HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
         HIP_TRY(hipMemsetAsync(ctx->slots[i].ops, 0, 8 * sizeof(unsigned long long), st));
+        HIP_TRY(hipMemsetAsync(ctx->slots[i].ext_done, 0, ctx->slots[i].mbs_last, st));
         if (ctx->cost_report) {
             HIP_TRY(hipMemsetAsync(ctx->slots[i].cost20, 0, sizeof(uint64_t) * ctx->slots[i].mbs_last, st));
             HIP_TRY(hipMemsetAsync(ctx->slots[i].one_mv, 0, sizeof(int2) * ctx->slots[i].mbs_last, st));
@@ -636,12 +644,17 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     int max_mbs_all = 0;
     for (int i = 0; i < n; ++i) max_mbs_all = std::max(max_mbs_all, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh);
     uint64_t relax_passes = 0, relax_analyses = 0;
+    ctx->pass_analyses.clear();
+    ctx->pass_ms.clear();
     bool relax_fell_back = false;
     auto relax_all = [&](bool events) -> int {
         HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
         const int limit = relax_pass_limit();
         int grid = max_mbs_all;
-        for (int i = 0; i < n; ++i) relax_analyses += static_cast<uint64_t>(ctx->h_jobs[i].mbw) * ctx->h_jobs[i].mbh;
+        int64_t first = 0;
+        for (int i = 0; i < n; ++i) first += static_cast<int64_t>(ctx->h_jobs[i].mbw) * ctx->h_jobs[i].mbh;
+        relax_analyses += static_cast<uint64_t>(first);
+        ctx->pass_analyses.push_back(first);
         for (int pass = 0;; ++pass) {
             const int ev = 4 + 2 * mb_launches;
             if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[ev], st));
@@ -656,11 +669,14 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
                                    hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
             grid = 0;
+            int64_t cnt = 0;
             for (int i = 0; i < n; ++i) {
                 grid = std::max(grid, ctx->h_wl_n[i]);
-                relax_analyses += static_cast<uint64_t>(ctx->h_wl_n[i]);
+                cnt += ctx->h_wl_n[i];
             }
             if (grid == 0) return VC1ME_OK;
+            relax_analyses += static_cast<uint64_t>(cnt);
+            ctx->pass_analyses.push_back(cnt);
             if (pass + 1 >= limit) break;
         }
         // not settled: the diagonal sweep is exact from any state
@@ -675,10 +691,10 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         if (after_global)
             if (int rc = after_global(st)) return rc;
         HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
-        HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
         if (relax) {
             if (int rc = relax_all(events)) return rc;
         } else {
+            HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
             if (int rc = diagonal_all(events)) return rc;
         }
         HIP_TRY(vc1me::launch_ops_sum(ctx->d_jobs, n, st));
@@ -734,10 +750,11 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     ctx->timing.global_kernel_ms_total = gms;
     ctx->timing.global_kernel_launches = 9;
     double mbms = 0.0;
-    for (const auto& e : mb_events) {
+    for (size_t k = 0; k < mb_events.size(); ++k) {
         float x = 0.f;
-        HIP_TRY(hipEventElapsedTime(&x, ctx->events[e.first], ctx->events[e.second]));
+        HIP_TRY(hipEventElapsedTime(&x, ctx->events[mb_events[k].first], ctx->events[mb_events[k].second]));
         mbms += x;
+        if (k < relax_passes) ctx->pass_ms.push_back(x);
     }
     if (ctx->timing.graph_replay) {
         mb_launches = 0;
@@ -1158,6 +1175,17 @@ int vc1me_set_graphs(vc1me_ctx* ctx, int on) {
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->use_graphs = on != 0;
     return VC1ME_OK;
+}
+
+int vc1me_last_relax_passes(vc1me_ctx* ctx, int64_t* analyses, double* ms, int cap) {
+    if (!ctx || cap < 0) return fail(VC1ME_EINVAL, "bad argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int np = static_cast<int>(ctx->pass_analyses.size());
+    for (int k = 0; k < np && k < cap; ++k) {
+        if (analyses) analyses[k] = ctx->pass_analyses[k];
+        if (ms) ms[k] = k < static_cast<int>(ctx->pass_ms.size()) ? ctx->pass_ms[k] : -1.0;
+    }
+    return np;
 }
 
 int vc1me_set_schedule(vc1me_ctx* ctx, int schedule) {

@@ -55,9 +55,10 @@ struct JobDesc {
     // satd_ops, sig_ops, frame_px), in the units of the CPU oracle
     unsigned long long* ops;
     int32_t max_diag;           // last anti-diagonal index mbw + 2*mbh - 3
-    // extended long-range search result per macroblock (k_extended): best
-    // SAD, its vector, and the op counts (signature cells, SAD pixels)
-    uint64_t* ext_sad; int2* ext_mv; uint32_t* ext_ops;
+    // extended long-range search result per macroblock, kept for the step
+    // once computed (ext_done): best SAD, its vector, and the op counts
+    // (signature cells, SAD pixels)
+    uint64_t* ext_sad; int2* ext_mv; uint32_t* ext_ops; uint8_t* ext_done;
     // interpolated reference: 16 bicubic (qp) or 4 bilinear (bp) phase planes
     // of (W+3) x (H+3) samples at pitch pp, row count ph = H + 3
     const uint8_t* qp;
@@ -107,9 +108,11 @@ constexpr uint8_t kFlagIntra = 2;
 hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc* h_jobs, hipStream_t s,
                                 hipEvent_t ev0, hipEvent_t ev1);
 hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hipStream_t s);
-hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s);
+// extended long-range search of every macroblock of the plain jobs, ahead of
+// the anti-diagonal sweep (the relaxation schedule computes it on demand)
+hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
 // relaxation schedule: starting state, one pass over a worklist, and the
 // per-job op totals of the final macroblock analyses
 hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipStream_t s);

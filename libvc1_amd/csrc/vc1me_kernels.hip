@@ -1473,8 +1473,9 @@ __device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win
 // the signature scan for the top-16 cells, then per cell a 5x5 grid at step 4
 // and a 5x5 refinement, stopping at the first good match.  It depends only on
 // the current block, the reference and the bounds, never on other
-// macroblocks' decisions, so k_extended runs it for every macroblock ahead of
-// the wavefront.  The candidates are independent of each other, so all of
+// macroblocks' decisions, so its result is computed once per macroblock and
+// step (extended_content) however often the macroblock is analysed.  The
+// candidates are independent of each other, so all of
 // them are searched together (eight per round from windows staged in LDS) and
 // the reference's sequential loop, with its early exit, is replayed on the
 // results; the op counters count only the candidates the loop reaches.
@@ -1662,14 +1663,33 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
     *out_dy = bdy;
 }
 
-// extended_content_motion's decision (src/encoder_motion.cpp:1766-1779) on the
-// search result k_extended left for this macroblock.
+// extended_content_motion (src/encoder_motion.cpp:1696-1779): the search
+// depends only on the macroblock, the reference and the bounds, so it runs
+// the first time an analysis of this macroblock in the step needs it and is
+// kept (result and op counts) for any later analysis of the macroblock.
 __device__ __forceinline__ IRes extended_content(Ctx& C, IRes local, uint64_t* cand_sad, int* cand_decision) {
     const JobDesc* J = C.J;
-    const uint64_t best_sad = J->ext_sad[C.pos];
-    const int2 bv = J->ext_mv[C.pos];
-    C.n_sig += J->ext_ops[2 * C.pos];
-    C.n_sad += J->ext_ops[2 * C.pos + 1];
+    uint64_t best_sad;
+    int2 bv;
+    if (!J->ext_done[C.pos]) {
+        const uint64_t sig0 = C.n_sig, sad0 = C.n_sad;
+        int dx = 0, dy = 0;
+        extended_search(C, J->range, &best_sad, &dx, &dy);
+        bv = make_int2(dx, dy);
+        C.pw_ok = false;   // the staging shared the phase-window bytes
+        if (C.lane == 0) {
+            J->ext_sad[C.pos] = best_sad;
+            J->ext_mv[C.pos] = bv;
+            J->ext_ops[2 * C.pos] = static_cast<uint32_t>(C.n_sig - sig0);
+            J->ext_ops[2 * C.pos + 1] = static_cast<uint32_t>(C.n_sad - sad0);
+            J->ext_done[C.pos] = 1;
+        }
+    } else {
+        best_sad = J->ext_sad[C.pos];
+        bv = J->ext_mv[C.pos];
+        C.n_sig += J->ext_ops[2 * C.pos];
+        C.n_sad += J->ext_ops[2 * C.pos + 1];
+    }
     *cand_sad = best_sad;
     const int decision = best_sad < UINT64_MAX / 8 ? distant_decision(C, best_sad, local.sad) : 0;
     *cand_decision = decision;
@@ -2277,13 +2297,14 @@ __device__ __forceinline__ NbState run_mb(const JobDesc* J, int mx, int my, uint
     return st;
 }
 
-// The extended long-range search of every macroblock of jobs with a search
-// range beyond the local range: one 64-lane workgroup per macroblock, all
-// macroblocks at once, ahead of the wavefront (which only reads the result).
+// The extended long-range search of every macroblock of plain jobs with a
+// search range beyond the local range, one 64-lane workgroup per macroblock
+// at full occupancy: used ahead of the anti-diagonal sweep, whose launches
+// are latency-bound, so that the sweep only reads the kept results.
 __global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t smem[];
     const JobDesc* J = &jobs[blockIdx.y];
-    if (J->range <= J->local_range) return;
+    if (J->range <= J->local_range || J->nn_out) return;
     const int pos = static_cast<int>(blockIdx.x);
     if (pos >= J->mbw * J->mbh) return;
     Ctx C;
@@ -2293,16 +2314,6 @@ __global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ job
         C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
     }
     wsync();
-    // learned seeding: the wavefront's seeded search starts from a superset
-    // of these seeds and never raises the SAD, so if one of them already
-    // passes seed_kept the macroblock keeps the seeded result and never
-    // consults the long-range search
-    if (J->nn_out && J->nn_out[static_cast<size_t>(pos) * 5 + 4] >= J->nn_logit) {
-        const int gdx = J->g->dx, gdy = J->g->dy;
-        int vx, vy;
-        seed_lane(C, J->nn_out + static_cast<size_t>(pos) * 5, gdx, gdy, bounds_px(J->range), gdx, gdy, vx, vy);
-        if (seed_kept(C, key_cost(global_sad_min(C, vx, vy, 3)))) return;
-    }
     uint64_t sad = UINT64_MAX;
     int dx = 0, dy = 0;
     extended_search(C, J->range, &sad, &dx, &dy);
@@ -2311,6 +2322,7 @@ __global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ job
         J->ext_mv[pos] = make_int2(dx, dy);
         J->ext_ops[2 * pos] = static_cast<uint32_t>(C.n_sig);
         J->ext_ops[2 * pos + 1] = static_cast<uint32_t>(C.n_sad);
+        J->ext_done[pos] = 1;
     }
 }
 
@@ -2518,25 +2530,63 @@ __device__ __forceinline__ bool g_better(uint64_t ac, int ax, int ay, uint64_t b
     return ax < bx;
 }
 
-// sorted insertion of the extended search's consider (src/encoder_motion.cpp:2036-2041)
-__device__ __forceinline__ void g_consider(GlobalState* g, uint64_t cost, int dx, int dy) {
-    for (int i = 0; i < g->tn; ++i)
-        if (g->tdx[i] == dx && g->tdy[i] == dy) {
-            if (cost < g->tcost[i]) g->tcost[i] = cost;
-            return;
-        }
-    g->tcost[g->tn] = cost; g->tdx[g->tn] = dx; g->tdy[g->tn] = dy; g->tn++;
-    for (int i = 1; i < g->tn; ++i) {
-        const uint64_t c = g->tcost[i];
-        const int x = g->tdx[i], y = g->tdy[i];
-        int j = i - 1;
-        while (j >= 0 && g_better(c, x, y, g->tcost[j], g->tdx[j], g->tdy[j])) {
-            g->tcost[j + 1] = g->tcost[j]; g->tdx[j + 1] = g->tdx[j]; g->tdy[j + 1] = g->tdy[j];
-            --j;
-        }
-        g->tcost[j + 1] = c; g->tdx[j + 1] = x; g->tdy[j + 1] = y;
+// The extended search's sorted candidate list (src/encoder_motion.cpp:2036-2041)
+// held in registers while a stage is replayed: consider() keeps the lower
+// cost of a repeated point; a new point is appended, the list sorted by the
+// strict order of g_better (points are distinct, so the order is unique) and
+// truncated to 8.
+struct GTop {
+    uint64_t c[9];
+    int x[9], y[9];
+    int n;
+};
+
+__device__ __forceinline__ void gtop_load(GTop& t, const GlobalState* g) {
+    t.n = g->tn;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        t.c[i] = i < t.n ? g->tcost[i] : UINT64_MAX;
+        t.x[i] = i < t.n ? g->tdx[i] : 0;
+        t.y[i] = i < t.n ? g->tdy[i] : 0;
     }
-    if (g->tn > 8) g->tn = 8;
+}
+
+__device__ __forceinline__ void gtop_store(const GTop& t, GlobalState* g) {
+    g->tn = t.n;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        if (i < t.n) { g->tcost[i] = t.c[i]; g->tdx[i] = t.x[i]; g->tdy[i] = t.y[i]; }
+    }
+}
+
+__device__ __forceinline__ void g_consider(GTop& t, uint64_t cost, int dx, int dy) {
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        if (i < t.n && t.x[i] == dx && t.y[i] == dy) {
+            if (cost < t.c[i]) t.c[i] = cost;
+            found = true;
+        }
+    }
+    if (found) return;
+    // append at n (n <= 8), then sort the n + 1 entries (odd-even
+    // transposition: 9 rounds sort any 9 entries)
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+        if (i == t.n) { t.c[i] = cost; t.x[i] = dx; t.y[i] = dy; }
+    const int n = t.n + 1;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+#pragma unroll
+        for (int i = r & 1; i + 1 < 9; i += 2) {
+            if (i + 1 < n && g_better(t.c[i + 1], t.x[i + 1], t.y[i + 1], t.c[i], t.x[i], t.y[i])) {
+                const uint64_t tc = t.c[i]; t.c[i] = t.c[i + 1]; t.c[i + 1] = tc;
+                const int tx = t.x[i]; t.x[i] = t.x[i + 1]; t.x[i + 1] = tx;
+                const int ty = t.y[i]; t.y[i] = t.y[i + 1]; t.y[i + 1] = ty;
+            }
+        }
+    }
+    t.n = n > 8 ? 8 : n;
 }
 
 __device__ __forceinline__ void g_set_stage(GlobalState* g, int stride) {
@@ -2610,7 +2660,11 @@ __global__ void k_global_step(const JobDesc* __restrict__ jobs, int njobs, int s
     if (R > 31) {
         const int step = R >= 160 ? 16 : 8;
         if (stage == 0 || stage == 1 || stage == 2) {
-            for (int i = 0; i < g->n; ++i) g_consider(g, g->cost[i] * (s > 1 ? mul : 1), g->pts[2 * i], g->pts[2 * i + 1]);
+            GTop t;
+            gtop_load(t, g);
+            const int np = g->n;
+            for (int i = 0; i < np; ++i) g_consider(t, g->cost[i] * (s > 1 ? mul : 1), g->pts[2 * i], g->pts[2 * i + 1]);
+            gtop_store(t, g);
             int n = 0;
             if (stage == 0) {
                 // densify the best four of the coarse list at stride 8
@@ -2942,15 +2996,15 @@ hipError_t launch_phase_planes_jobs(const JobDesc* d_jobs, int njobs, int max_w,
     return hipGetLastError();
 }
 
-hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s) {
-    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_extended, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs);
-    return hipGetLastError();
-}
-
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s) {
     hipLaunchKernelGGL(k_mb_diagonal, dim3(max_per_diag, njobs), dim3(64), lds_bytes, s, d_jobs, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s) {
+    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_extended, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs);
     return hipGetLastError();
 }
 

@@ -300,8 +300,8 @@ def test_graph_replay_matches_eager(backend):
 
 def test_batched_streams_runner_matches_oracle(backend):
     """Streams of uneven raster and length through the batching runner (steps
-    of 3 pairs, graphs on for one raster, off for the other) give each pair
-    the oracle's result."""
+    of 3 pairs; the anti-diagonal sweep from a graph for one raster, the
+    relaxation schedule for the other) give each pair the oracle's result."""
     from libvc1_amd import batching
     specs = [(300, 176, 144, 4), (301, 320, 176, 3), (302, 176, 144, 2)]
     streams, pics = [], []
@@ -310,8 +310,10 @@ def test_batched_streams_runner_matches_oracle(backend):
         p = batching.stream_params(w, h)
         pics.append([d[k] for k in range(n)])
         streams.append(batching.Stream(w, h, [backend.upload(x) for x in pics[-1]], p, seed))
+    # one raster as a table written before the schedule knob (sweep + graphs),
+    # the other under the relaxation schedule
     table = {"settings": {"176x144": {"jobs_per_step": 3, "graphs": True},
-                          "320x176": {"jobs_per_step": 3, "graphs": False}}}
+                          "320x176": {"jobs_per_step": 3, "schedule": "relax", "graphs": False}}}
     _say("[case] batching runner, 3 streams")
     try:
         out = batching.run(backend, streams, table, collect=True)
