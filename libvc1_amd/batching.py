@@ -104,8 +104,29 @@ def load_table(backend: Backend) -> dict:
     return {}
 
 
-def setting_for(table: dict, width: int, height: int) -> dict:
-    st = dict(table.get("settings", {}).get(f"{width}x{height}", DEFAULT_SETTING))
+def setting_for(table: dict, width: int, height: int, pairs: int | None = None) -> dict:
+    """The tuned setting of a raster.  With `pairs` (the pairs the runner has
+    for the raster) and the table's trials, the best measured mode among step
+    sizes the pairs can fill: a raster with few pairs gets the mode that was
+    fastest at that size, not the one tuned for full steps."""
+    raster = f"{width}x{height}"
+    trials = table.get("trials", {}).get(raster)
+    if pairs is not None and trials:
+        best = None
+        sizes = sorted({int(re.match(r"\d+", k).group()) for k in trials})
+        fill = [b for b in sizes if b <= pairs] or sizes[:1]
+        for k, rate in trials.items():
+            m = re.fullmatch(r"(\d+)(r?)(g?)", k)
+            if not m or int(m.group(1)) not in fill:
+                continue
+            if best is None or rate > best[0]:
+                best = (rate, int(m.group(1)), "relax" if m.group(2) else "diagonal", bool(m.group(3)))
+        if best is not None:
+            # every trial grew with the step size: take all the pairs (at most
+            # 255 per step) in the mode measured best at the filled size
+            return {"jobs_per_step": max(best[1], min(pairs, 255)), "schedule": best[2], "graphs": best[3],
+                    "mbs_per_s": best[0]}
+    st = dict(table.get("settings", {}).get(raster, DEFAULT_SETTING))
     st.setdefault("schedule", "diagonal")      # tables written before the relaxation schedule
     return st
 
@@ -183,7 +204,7 @@ def run(backend: Backend, streams: list[Stream], table: dict | None = None, coll
     total_mbs, per, results = 0, {}, {}
     t0 = time.perf_counter()
     for (w, h), jobs in sorted(groups.items()):
-        st = setting_for(table, w, h)
+        st = setting_for(table, w, h, len(jobs))
         t1 = time.perf_counter()
         res = _run_steps(backend, jobs, stream_params(w, h), int(st["jobs_per_step"]), bool(st["graphs"]),
                          str(st["schedule"]), collect)

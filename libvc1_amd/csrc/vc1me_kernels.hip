@@ -25,6 +25,10 @@
 // IEEE double without contraction.
 #include "vc1me_internal.h"
 
+// lambdas of device code are inlined like the functions around them (a lambda
+// left out of line passes its captures through scratch memory)
+#define VC1ME_INLINE __attribute__((always_inline))
+
 namespace vc1me {
 
 namespace {
@@ -966,7 +970,7 @@ struct IRes {
 // together and only the stage minimum updates `best`.  Every considered key
 // is logged, and the finalists (the best four of the top-8) are extracted
 // from the log once, at the end.
-__device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
+__device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int gdy, int R) {
     // the whole search stays inside [-R,R]^2: one window serves every stage
     if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
         set_err(C, 5);
@@ -977,34 +981,36 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
     uint64_t best = kNone;
     int nkeys = 0;
     uint64_t* uk = C.L.ukeys;
-    auto cl = [R](int v) { return clampi(v, -R, R); };
-    auto stage = [&](int n, auto pt) {
+    auto cl = [R](int v) VC1ME_INLINE { return clampi(v, -R, R); };
+    auto stage = [&](int n, auto pt) VC1ME_INLINE {
         const uint64_t m = int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, pt, uk, &nkeys);
         best = m < best ? m : best;
     };
 
     // predictor-driven seeds
     {
-        int sx[5], sy[5], n = 0;
-        sx[n] = cl(round_qpel_to_pixel(pi.px)); sy[n] = cl(round_qpel_to_pixel(pi.py)); ++n;
-        sx[n] = 0; sy[n] = 0; ++n;
-        sx[n] = cl(gdx); sy[n] = cl(gdy); ++n;
-        if (pi.a_valid) { sx[n] = cl(round_qpel_to_pixel(pi.ax)); sy[n] = cl(round_qpel_to_pixel(pi.ay)); ++n; }
-        if (pi.c_valid) { sx[n] = cl(round_qpel_to_pixel(pi.cx)); sy[n] = cl(round_qpel_to_pixel(pi.cy)); ++n; }
-        stage(n, [&](int i, int& dx, int& dy) {
-            dx = i == 0 ? sx[0] : i == 1 ? sx[1] : i == 2 ? sx[2] : i == 3 ? sx[3] : sx[4];
-            dy = i == 0 ? sy[0] : i == 1 ? sy[1] : i == 2 ? sy[2] : i == 3 ? sy[3] : sy[4];
+        // predictor, zero, global, then A and C where valid (seed i of lane i;
+        // registers and selects, no indexed array)
+        const int px = cl(round_qpel_to_pixel(pi.px)), py = cl(round_qpel_to_pixel(pi.py));
+        const int ax = cl(round_qpel_to_pixel(pi.ax)), ay = cl(round_qpel_to_pixel(pi.ay));
+        const int qx = cl(round_qpel_to_pixel(pi.cx)), qy = cl(round_qpel_to_pixel(pi.cy));
+        const int gx = cl(gdx), gy = cl(gdy);
+        const int n = 3 + (pi.a_valid ? 1 : 0) + (pi.c_valid ? 1 : 0);
+        stage(n, [=](int i, int& dx, int& dy) VC1ME_INLINE {
+            const bool use_a = i == 3 && pi.a_valid;
+            dx = i == 0 ? px : i == 1 ? 0 : i == 2 ? gx : use_a ? ax : qx;
+            dy = i == 0 ? py : i == 1 ? 0 : i == 2 ? gy : use_a ? ay : qy;
         });
     }
     // local 5x5
     {
         const int cx = key_dx(best), cy = key_dy(best);
-        stage(25, [&](int i, int& dx, int& dy) { dx = cl(cx + (i % 5) - 2); dy = cl(cy + (i / 5) - 2); });
+        stage(25, [=](int i, int& dx, int& dy) VC1ME_INLINE { dx = cl(cx + (i % 5) - 2); dy = cl(cy + (i / 5) - 2); });
     }
     // sparse lattice
     if (R >= 8) {
         const int side = (2 * R) / 8 + 1;
-        stage(side * side + 2 * side, [&](int i, int& dx, int& dy) {
+        stage(side * side + 2 * side, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             if (i < side * side) { dx = -R + 8 * (i % side); dy = -R + 8 * (i / side); }
             else {
                 const int j = i - side * side, d = -R + 8 * (j >> 1);
@@ -1019,7 +1025,7 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
         const int cx = key_dx(best), cy = key_dy(best);
         const int vlimit = imin(R, imax(8, R / 2));
         const int nh = R / cs, nv = vlimit / cs;
-        stage(2 * (nh + nv), [&](int i, int& dx, int& dy) {
+        stage(2 * (nh + nv), [=](int i, int& dx, int& dy) VC1ME_INLINE {
             if (i < 2 * nh) {
                 const int d = cs * (i / 2 + 1);
                 dx = (i & 1) ? cx + d : cx - d; dy = cy;
@@ -1034,7 +1040,7 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
     {
         const int hs = R >= 96 ? 8 : (R >= 24 ? 4 : 2);
         const int hx = key_dx(best), hy = key_dy(best);
-        stage(6 * (R / hs), [&](int i, int& dx, int& dy) {
+        stage(6 * (R / hs), [=](int i, int& dx, int& dy) VC1ME_INLINE {
             const int r = hs * (i / 6 + 1), h = (r + 1) / 2, k = i % 6;
             switch (k) {
                 case 0: dx = hx - r; dy = hy; break;
@@ -1050,7 +1056,7 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
     // iterative small hexagon
     for (int iter = 0; iter < 32; ++iter) {
         const int ox = key_dx(best), oy = key_dy(best);
-        stage(6, [&](int i, int& dx, int& dy) {
+        stage(6, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             switch (i) {
                 case 0: dx = ox - 2; dy = oy; break;
                 case 1: dx = ox + 2; dy = oy; break;
@@ -1066,7 +1072,7 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
     // closing diamond
     {
         const int ox = key_dx(best), oy = key_dy(best);
-        stage(4, [&](int i, int& dx, int& dy) {
+        stage(4, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             dx = cl(ox + (i == 0 ? -1 : i == 1 ? 1 : 0));
             dy = cl(oy + (i == 2 ? -1 : i == 3 ? 1 : 0));
         });
@@ -1092,21 +1098,21 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
         }
     }
     // full-resolution finalists: 3x3 around the best four
-    uint64_t full = int_batch(C, fin * 9, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+    uint64_t full = int_batch(C, fin * 9, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
         const int f = i / 9, o = i % 9;
         const uint64_t k = f == 0 ? tk[0] : f == 1 ? tk[1] : f == 2 ? tk[2] : tk[3];
         dx = cl(key_dx(k) + (o % 3) - 1); dy = cl(key_dy(k) + (o / 3) - 1);
     });
     if (full == kNone) {
         const int bx = key_dx(best), by = key_dy(best);
-        full = int_batch(C, 1, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int, int& dx, int& dy) { dx = bx; dy = by; });
+        full = int_batch(C, 1, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int, int& dx, int& dy) VC1ME_INLINE { dx = bx; dy = by; });
     }
     // rescue of genuinely poor matches
     if (R > 0 && !match_good(C, key_cost(full))) {
         const int step = R >= 8 ? 4 : (R >= 3 ? 2 : 1);
         const int side = (2 * R) / step + 1;
         const uint64_t coarse = int_batch(C, side * side + 2 * side, 4, nullptr, KEY_TOTAL, 0,
-                                          [&](int i, int& dx, int& dy) {
+                                          [=](int i, int& dx, int& dy) VC1ME_INLINE {
             if (i < side * side) { dx = -R + step * (i % side); dy = -R + step * (i / side); }
             else {
                 const int j = i - side * side, d = -R + step * (j >> 1);
@@ -1117,20 +1123,20 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
         const int rr = imax(1, step - 1);
         const int xa = imax(-R, cdx - rr), xb = imin(R, cdx + rr), ya = imax(-R, cdy - rr), yb = imin(R, cdy + rr);
         const int rw = xb - xa + 1, rh = yb - ya + 1;
-        const uint64_t rescue = int_batch(C, rw * rh, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        const uint64_t rescue = int_batch(C, rw * rh, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             dx = xa + i % rw; dy = ya + i / rw;
         });
         if (rescue < full) full = rescue;
         const uint64_t px = active_pixels(C);
         if (px && key_cost(full) > px * 24u && R <= 32) {
             const int ds = (2 * R) / 2 + 1;
-            const uint64_t dense = int_batch(C, ds * ds, 2, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            const uint64_t dense = int_batch(C, ds * ds, 2, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
                 dx = -R + 2 * (i % ds); dy = -R + 2 * (i / ds);
             });
             const int ddx = key_dx(dense), ddy = key_dy(dense);
             const int xa2 = imax(-R, ddx - 2), xb2 = imin(R, ddx + 2), ya2 = imax(-R, ddy - 2), yb2 = imin(R, ddy + 2);
             const int w2 = xb2 - xa2 + 1, h2 = yb2 - ya2 + 1;
-            const uint64_t dfull = int_batch(C, w2 * h2, 1, C.L.seen_f, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+            const uint64_t dfull = int_batch(C, w2 * h2, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
                 dx = xa2 + i % w2; dy = ya2 + i / w2;
             });
             if (dfull < full) full = dfull;
@@ -1140,30 +1146,50 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred& pi, int gdx, int 
 }
 
 // integer_motion_diamond, src/encoder_motion_diamond.cpp:13-73
-__device__ __forceinline__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, int gdy, int R) {
+__device__ __forceinline__ IRes diamond_search(Ctx& C, const Pred pi, int gdx, int gdy, int R) {
     if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
         set_err(C, 5);
         return {0, 0, kBadSad};
     }
-    uint64_t best = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; });
+    uint64_t best = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) VC1ME_INLINE { dx = 0; dy = 0; });
     if (key_cost(best) == 0) return {0, 0, 0};
-    int sx[5], sy[5], sn = 1;
-    sx[0] = 0; sy[0] = 0;
-    auto seed = [&](int dx, int dy) {
-        dx = clampi(dx, -R, R); dy = clampi(dy, -R, R);
-        for (int i = 0; i < sn; ++i)
-            if (sx[i] == dx && sy[i] == dy) return;
-        sx[sn] = dx; sy[sn] = dy; ++sn;
-    };
-    seed(round_qpel_to_pixel(pi.px), round_qpel_to_pixel(pi.py));
-    seed(gdx, gdy);
-    if (pi.a_valid) seed(round_qpel_to_pixel(pi.ax), round_qpel_to_pixel(pi.ay));
-    if (pi.c_valid) seed(round_qpel_to_pixel(pi.cx), round_qpel_to_pixel(pi.cy));
+    // seed 0 is (0,0); seeds 1..4 (predictor, global, A, C) are kept in
+    // order unless they repeat an earlier seed: seed k is slot k's vector if
+    // bit k of `keep` is set
+    const int p1x = clampi(round_qpel_to_pixel(pi.px), -R, R), p1y = clampi(round_qpel_to_pixel(pi.py), -R, R);
+    const int p2x = clampi(gdx, -R, R), p2y = clampi(gdy, -R, R);
+    const int p3x = clampi(round_qpel_to_pixel(pi.ax), -R, R), p3y = clampi(round_qpel_to_pixel(pi.ay), -R, R);
+    const int p4x = clampi(round_qpel_to_pixel(pi.cx), -R, R), p4y = clampi(round_qpel_to_pixel(pi.cy), -R, R);
+    unsigned keep = 0;
+    {
+        const bool k1 = !(p1x == 0 && p1y == 0);
+        const bool k2 = !(p2x == 0 && p2y == 0) && !(k1 && p2x == p1x && p2y == p1y);
+        const bool k3 = pi.a_valid && !(p3x == 0 && p3y == 0) && !(k1 && p3x == p1x && p3y == p1y) &&
+                        !(k2 && p3x == p2x && p3y == p2y);
+        const bool k4 = pi.c_valid && !(p4x == 0 && p4y == 0) && !(k1 && p4x == p1x && p4y == p1y) &&
+                        !(k2 && p4x == p2x && p4y == p2y) && !(k3 && p4x == p3x && p4y == p3y);
+        keep = (k1 ? 1u : 0u) | (k2 ? 2u : 0u) | (k3 ? 4u : 0u) | (k4 ? 8u : 0u);
+    }
+    const int sn = 1 + __builtin_popcount(keep);
+    // kept seeds, compacted in order
+    int s1x = 0, s1y = 0, s2x = 0, s2y = 0, s3x = 0, s3y = 0, s4x = 0, s4y = 0;
+    {
+        unsigned m = keep;
+        auto take = [=](unsigned mm, int& x, int& y) VC1ME_INLINE {
+            const int k = __builtin_ctz(mm);
+            x = k == 0 ? p1x : k == 1 ? p2x : k == 2 ? p3x : p4x;
+            y = k == 0 ? p1y : k == 1 ? p2y : k == 2 ? p3y : p4y;
+        };
+        if (m) { take(m, s1x, s1y); m &= m - 1; }
+        if (m) { take(m, s2x, s2y); m &= m - 1; }
+        if (m) { take(m, s3x, s3y); m &= m - 1; }
+        if (m) { take(m, s4x, s4y); }
+    }
     if (sn > 1) {
-        const uint64_t s = int_batch(C, sn - 1, 1, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        const uint64_t s = int_batch(C, sn - 1, 1, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             const int j = i + 1;
-            dx = j == 1 ? sx[1] : j == 2 ? sx[2] : j == 3 ? sx[3] : sx[4];
-            dy = j == 1 ? sy[1] : j == 2 ? sy[2] : j == 3 ? sy[3] : sy[4];
+            dx = j == 1 ? s1x : j == 2 ? s2x : j == 3 ? s3x : s4x;
+            dy = j == 1 ? s1y : j == 2 ? s2y : j == 3 ? s3y : s4y;
         });
         if (s < best) best = s;
     }
@@ -1172,19 +1198,26 @@ __device__ __forceinline__ IRes diamond_search(Ctx& C, const Pred& pi, int gdx, 
     for (int step = 0; step <= 4 * R + 1; ++step) {
         const uint64_t center = best;
         const int cx = key_dx(center), cy = key_dy(center);
-        int ox[4], oy[4], n = 0;
-        const int offs[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        // the four neighbours left, right, up, down that lie in range and are
+        // not the previous centre, in that order: bit k of `keep`
+        unsigned keep = 0;
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int dx = cx + offs[k][0], dy = cy + offs[k][1];
+            const int dx = cx + (k == 0 ? -1 : k == 1 ? 1 : 0), dy = cy + (k == 2 ? -1 : k == 3 ? 1 : 0);
             if (dx < -R || dx > R || dy < -R || dy > R) continue;
             if (have_prev && dx == prevx && dy == prevy) continue;
-            ox[n] = dx; oy[n] = dy; ++n;
+            keep |= 1u << k;
         }
+        const int n = __builtin_popcount(keep);
         uint64_t win = center;
         if (n) {
-            const uint64_t s = int_batch(C, n, 1, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
-                dx = i == 0 ? ox[0] : i == 1 ? ox[1] : i == 2 ? ox[2] : ox[3];
-                dy = i == 0 ? oy[0] : i == 1 ? oy[1] : i == 2 ? oy[2] : oy[3];
+            const uint64_t s = int_batch(C, n, 1, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
+                // the i-th set bit of keep
+                unsigned m = keep;
+                for (int j = 0; j < i; ++j) m &= m - 1;
+                const int k = __builtin_ctz(m);
+                dx = cx + (k == 0 ? -1 : k == 1 ? 1 : 0);
+                dy = cy + (k == 2 ? -1 : k == 3 ? 1 : 0);
             });
             if (s < win) win = s;
         }
@@ -1290,7 +1323,7 @@ __device__ __forceinline__ void seed_lane(const Ctx& C, const float* o, int gdx,
 // full SADs of the two proposals, the predictor, zero and the global vector,
 // then a unit diamond from the best until the centre wins (or 32 steps).
 // The result's SAD is never above the best seed's.
-__device__ __forceinline__ IRes seeded_search(Ctx& C, const Pred& pi, int gdx, int gdy, int range, const float* o) {
+__device__ __forceinline__ IRes seeded_search(Ctx& C, const Pred pi, int gdx, int gdy, int range, const float* o) {
     const Bounds b = bounds_px(range);
     int vx, vy;
     seed_lane(C, o, gdx, gdy, b, round_qpel_to_pixel(pi.px), round_qpel_to_pixel(pi.py), vx, vy);
@@ -1298,7 +1331,7 @@ __device__ __forceinline__ IRes seeded_search(Ctx& C, const Pred& pi, int gdx, i
     for (int it = 0; it < 32; ++it) {
         const int cx = key_dx(best), cy = key_dy(best);
         if (!win_load(C, C.x0 + cx - 1, C.y0 + cy - 1, 18, 18)) { set_err(C, 11); break; }
-        const uint64_t m = int_batch(C, 4, 1, nullptr, KEY_TOTAL, 0, [&](int i, int& dx, int& dy) {
+        const uint64_t m = int_batch(C, 4, 1, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             dx = clampi(cx + (i == 0 ? -1 : i == 1 ? 1 : 0), b.xmin, b.xmax);
             dy = clampi(cy + (i == 2 ? -1 : i == 3 ? 1 : 0), b.ymin, b.ymax);
         });
@@ -1313,15 +1346,17 @@ __device__ __forceinline__ IRes propagated(Ctx& C, int range) {
     const JobDesc* J = C.J;
     const Bounds b = bounds_px(range);
     uint64_t best = kNone;
-    const int nbx[4] = {-1, 0, -1, 1}, nby[4] = {0, -1, -1, -1};
+    // neighbours left, top, top-left, top-right
+    auto nbx = [](int k) VC1ME_INLINE { return k == 0 ? -1 : k == 1 ? 0 : k == 2 ? -1 : 1; };
+    auto nby = [](int k) VC1ME_INLINE { return k == 0 ? 0 : -1; };
     for (int k = 0; k < 4; ++k) {
-        const int nx = C.mx + nbx[k], ny = C.my + nby[k];
+        const int nx = C.mx + nbx(k), ny = C.my + nby(k);
         if (nx < 0 || ny < 0 || nx >= C.g.mbw) continue;
         const int pos = ny * C.g.mbw + nx;
         if (!J->prop[pos]) continue;
         const int2 f = J->mvs[pos];
         const int cx = round_qpel_to_pixel(f.x), cy = round_qpel_to_pixel(f.y);
-        const uint64_t s = far_batch(C, 13, 1, KEY_SEQ_MAN, k * 13, [&](int i, int& dx, int& dy) {
+        const uint64_t s = far_batch(C, 13, 1, KEY_SEQ_MAN, k * 13, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             const int ox = i == 1 || i == 9 || i == 11 ? -1 : i == 2 || i == 10 || i == 12 ? 1 : i == 5 ? -2 : i == 6 ? 2 : 0;
             const int oy = i == 3 || i == 9 || i == 10 ? -1 : i == 4 || i == 11 || i == 12 ? 1 : i == 7 ? -2 : i == 8 ? 2 : 0;
             dx = clampi(cx + ox, b.xmin, b.xmax);
@@ -1335,7 +1370,7 @@ __device__ __forceinline__ IRes propagated(Ctx& C, int range) {
     // map the sequence number back to the vector
     const int seq = static_cast<int>(best & 0xffff);
     const int k = seq / 13, i = seq % 13;
-    const int2 f = J->mvs[(C.my + nby[k]) * C.g.mbw + (C.mx + nbx[k])];
+    const int2 f = J->mvs[(C.my + nby(k)) * C.g.mbw + (C.mx + nbx(k))];
     const int cx = round_qpel_to_pixel(f.x), cy = round_qpel_to_pixel(f.y);
     const int ox = i == 1 || i == 9 || i == 11 ? -1 : i == 2 || i == 10 || i == 12 ? 1 : i == 5 ? -2 : i == 6 ? 2 : 0;
     const int oy = i == 3 || i == 9 || i == 10 ? -1 : i == 4 || i == 11 || i == 12 ? 1 : i == 7 ? -2 : i == 8 ? 2 : 0;
@@ -1756,7 +1791,7 @@ __device__ __forceinline__ uint64_t batch_min_key(const Ctx& C, const SubList& L
 __device__ __forceinline__ int2 refine_integer_only(Ctx& C, const SubParams& sp, int sxq, int syq, uint64_t* best_sad,
                                                     uint64_t known) {
     const Bounds qb = bounds_qpel(sp.range);
-    auto snap = [](int q, int lo, int hi) {
+    auto snap = [](int q, int lo, int hi) VC1ME_INLINE {
         q = clampi(q, lo, hi);
         const int nearest = q >= 0 ? (q + 1) / 4 : (q - 1) / 4;
         return clampi(nearest * 4, lo, hi / 4 * 4);
@@ -1771,7 +1806,7 @@ __device__ __forceinline__ int2 refine_integer_only(Ctx& C, const SubParams& sp,
 }
 
 // refine_motion_block_staged, src/encoder_motion.cpp:3271-3594 (quality RATE / SATD)
-__device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred& pi,
+__device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred pi,
                                               uint64_t* best_sad, uint64_t known_seed) {
     const JobDesc* J = C.J;
     const int limit_q = 4 * J->subpel_range;
@@ -1992,7 +2027,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     if (range == 0) {
         // zero-range mode: zero vectors with their SAD (src/encoder_motion.cpp:73-86)
         win_load(C, C.x0, C.y0, 16, 16);
-        const uint64_t z = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) { dx = 0; dy = 0; });
+        const uint64_t z = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) VC1ME_INLINE { dx = 0; dy = 0; });
         if (C.lane == 0) {
             J->screen[pos] = key_cost(z);
             J->chosen[pos] = key_cost(z);
@@ -2035,15 +2070,19 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         }
         was_seeded = seeded_ok;
         const uint64_t px = active_pixels(C);
-        if (J->lr_mode == 2) {
-            bool sc = seeded_ok;
-            if (range > local_range && !sc) {
-                const IRes pr = propagated(C, range);
-                if (match_good(C, pr.sad)) { im = pr; sc = true; selected_distant = true; }
-            }
-            if (!sc) im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
-        } else {
-            if (!seeded_ok) im = J->algo == 1 ? diamond_search(C, pi, gdx, gdy, local_range) : umh_search(C, pi, gdx, gdy, local_range);
+        // long-range mode 2 tries the propagated neighbours first and skips
+        // the local search on a good match; the other modes search locally
+        // first (one call site per search keeps the predictor in registers)
+        bool sc = seeded_ok;
+        if (J->lr_mode == 2 && range > local_range && !sc) {
+            const IRes pr = propagated(C, range);
+            if (match_good(C, pr.sad)) { im = pr; sc = true; selected_distant = true; }
+        }
+        if (!sc) {
+            if (J->algo == 1) im = diamond_search(C, pi, gdx, gdy, local_range);
+            else im = umh_search(C, pi, gdx, gdy, local_range);
+        }
+        if (J->lr_mode != 2) {
             const uint64_t local_sad = im.sad;
             if (px) dl_mae = static_cast<double>(local_sad) / static_cast<double>(px);
             const bool skip = J->lr_mode == 1 && match_good(C, im.sad);
@@ -2106,7 +2145,9 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     }
     const uint64_t one_screen = one_sad;
     int2 outmv = one;
-    int2 outb[4] = {one, one, one, one};
+    // block vectors of the decision: one (1-MV), the four 8x8 results (4-MV,
+    // kept in LDS own[]) or zero (intra), held as four registers
+    int2 ob0 = one, ob1 = one, ob2 = one, ob3 = one;
     uint8_t flags = 0;
     wsync();
     if (C.lane < 4) C.L.own[C.lane] = one;
@@ -2115,14 +2156,13 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     uint64_t chosen_sad = one_sad;
     uint64_t temporal_screen = one_screen;
     if (allow_4mv) {
-        int2 four[4] = {one, one, one, one};
         uint64_t four_sad = 0, four_screen = 0, extra_bits = 1;
         const Bounds qb = bounds_qpel(range);
         const int limit = 4 * J->subpel_range;
         for (int k = 0; k < 4; ++k) {
             const Pred pi4 = predictor_4mv(C, k);
             const int bx = C.x0 + (k & 1) * 8, by = C.y0 + ((k >> 1) & 1) * 8;
-            auto within = [limit](int2 m) {
+            auto within = [limit](int2 m) VC1ME_INLINE {
                 if (imax(iabs(m.x), iabs(m.y)) > limit) { m.x = (m.x / 4) * 4; m.y = (m.y / 4) * 4; }
                 return m;
             };
@@ -2186,7 +2226,6 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             SubParams sp{bx, by, 8, 8, 3, range, true};
             const int2 mv = refine_staged(C, sp, best_seed.x, best_seed.y, pi4, &bsad, seed_cost);
             four_screen += bsad;
-            four[k] = mv;
             wsync();
             if (C.lane == 0) C.L.own[k] = mv;
             wsync();
@@ -2201,8 +2240,10 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             }
             if (!(mv.x == apx && mv.y == apy)) extra_bits += mvdiff_bits(J->mvrange, mv.x - apx, mv.y - apy, false, false);
         }
-        const int cmx = median4(four[0].x, four[1].x, four[2].x, four[3].x);
-        const int cmy = median4(four[0].y, four[1].y, four[2].y, four[3].y);
+        // the four block results (lane 0 stored them in own[] after each block)
+        const int2 f0 = C.L.own[0], f1 = C.L.own[1], f2 = C.L.own[2], f3 = C.L.own[3];
+        const int cmx = median4(f0.x, f1.x, f2.x, f3.x);
+        const int cmy = median4(f0.y, f1.y, f2.y, f3.y);
         const bool chroma_safe = !hidden_chroma(C.g, C.x0, C.y0, 16, 16, cmx, cmy);
         bool choose4 = false;
         if (chroma_safe) {
@@ -2211,13 +2252,13 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             choose4 = four_sad + static_cast<uint64_t>(llround_exact(lb)) + 8 < static_cast<uint64_t>(llround_exact(os));
         }
         if (choose4) {
-            for (int k = 0; k < 4; ++k) outb[k] = four[k];
+            ob0 = f0; ob1 = f1; ob2 = f2; ob3 = f3;
             flags |= kFlag4mvChosen;
-            outmv = four[0];
+            outmv = f0;
             chosen_sad = four_sad;
             temporal_screen = four_screen;
         } else {
-            for (int k = 0; k < 4; ++k) outb[k] = one;
+            ob0 = ob1 = ob2 = ob3 = one;
             outmv = one;
             temporal_screen = one_screen;
         }
@@ -2244,7 +2285,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         if (intra_sad + intra_rate < static_cast<uint64_t>(llround_exact(thr))) {
             flags |= kFlagIntra;
             outmv = make_int2(0, 0);
-            for (int k = 0; k < 4; ++k) outb[k] = make_int2(0, 0);
+            ob0 = ob1 = ob2 = ob3 = make_int2(0, 0);
             chosen_sad = intra_sad;
         }
     }
@@ -2253,7 +2294,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     C.n_satd += wave_sum32(C.ln_satd);
     if (C.lane == 0) {
         J->mvs[pos] = outmv;
-        for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = outb[k];
+        J->bmv[pos * 4 + 0] = ob0; J->bmv[pos * 4 + 1] = ob1; J->bmv[pos * 4 + 2] = ob2; J->bmv[pos * 4 + 3] = ob3;
         J->flags[pos] = flags;
         J->screen[pos] = temporal_screen;
         J->ratio[pos] = ratio;
@@ -2268,7 +2309,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         o[0] = C.n_sad; o[1] = C.n_taps; o[2] = C.n_satd; o[3] = C.n_sig; o[4] = was_seeded ? 1 : 0;
     }
     st.mv = outmv;
-    for (int k = 0; k < 4; ++k) st.b[k] = outb[k];
+    st.b[0] = ob0; st.b[1] = ob1; st.b[2] = ob2; st.b[3] = ob3;
     st.fp = static_cast<uint32_t>(flags & kFlagIntra) | (static_cast<uint32_t>(prop_out) << 8);
 }
 
@@ -2614,7 +2655,7 @@ __global__ void k_global_begin(const JobDesc* __restrict__ jobs, int njobs) {
         return;
     }
     int n = 0;
-    auto push = [&](int dx, int dy) { g->pts[2 * n] = dx; g->pts[2 * n + 1] = dy; ++n; };
+    auto push = [&](int dx, int dy) VC1ME_INLINE { g->pts[2 * n] = dx; g->pts[2 * n + 1] = dy; ++n; };
     if (R > 31) {
         push(0, 0);
         const int step = R >= 160 ? 16 : 8;
@@ -2815,10 +2856,10 @@ __global__ void __launch_bounds__(64) k_score_batch(const uint8_t* __restrict__ 
     const int ix = floor_div4(mv.x), iy = floor_div4(mv.y);
     const int hm = mv.x - ix * 4, vm = mv.y - iy * 4;
     const int aw = imax(0, imin(bw, width - x0)), ah = imax(0, imin(bh, height - y0));
-    auto S = [&](int x, int y) {
+    auto S = [&](int x, int y) VC1ME_INLINE {
         return static_cast<int>(ref[static_cast<size_t>(clampi(y, 0, height - 1)) * width + clampi(x, 0, width - 1)]);
     };
-    auto pred = [&](int x, int y) -> int {
+    auto pred = [&](int x, int y) VC1ME_INLINE -> int {
         const int sx = x + ix, sy = y + iy;
         if (bilinear) {
             const int a = S(sx, sy);
