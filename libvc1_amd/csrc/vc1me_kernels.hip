@@ -248,7 +248,10 @@ __device__ __forceinline__ unsigned __int128 x87_score(uint64_t cost, uint64_t r
 
 // ------------------------------------------------------------------ LDS layout
 
-constexpr int kSeenWords = ((2 * kMaxLocalRange + 1) * (2 * kMaxLocalRange + 1) + 31) / 32;
+// dedupe bitset of the integer searches: one bit per point of [-R, R]^2
+__host__ __device__ inline int seen_words(int local_range) {
+    return ((2 * local_range + 1) * (2 * local_range + 1) + 31) / 32;
+}
 constexpr int kMaxUmhKeys = 768;     // sampled-SAD points an integer search can consider
 constexpr int kSmallWin = 80 * 96;   // windows of the far (long-range) candidates
 
@@ -256,7 +259,7 @@ constexpr int kSmallWin = 80 * 96;   // windows of the far (long-range) candidat
 // integer vector, staged in LDS for the sub-pel stages.  kPwSide samples
 // square per phase; rows hold kPwPitch bytes so that 16-byte aligned copies
 // of the phase planes fit whatever the alignment of the window origin.
-constexpr int kPwMargin = 6;
+constexpr int kPwMargin = 3;
 constexpr int kPwSide = 16 + 2 * kPwMargin;
 constexpr int kPwPitch = 48;
 constexpr int kPwBytes = (16 * kPwSide * kPwPitch + 1023) & ~1023;  // whole 1 KB copy blocks
@@ -283,7 +286,7 @@ struct Lds {
 // sub-pel stages start, so the phase window shares their bytes.
 __host__ __device__ inline int int_scratch_bytes(int local_range) {
     int off = (win_capacity(local_range) + 15) & ~15;  // win
-    off += kSeenWords * 4 * 2;                          // seen_s, seen_f
+    off += seen_words(local_range) * 4 * 2;             // seen_s, seen_f
     off = (off + 15) & ~15;
     off += kMaxUmhKeys * 8;                             // ukeys
     return off;
@@ -300,8 +303,8 @@ __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     uint8_t* const region = smem + 256;
     int off = 0;
     L.win = region + off; off += (win_capacity(local_range) + 15) & ~15;
-    L.seen_s = reinterpret_cast<uint32_t*>(region + off); off += kSeenWords * 4;
-    L.seen_f = reinterpret_cast<uint32_t*>(region + off); off += kSeenWords * 4;
+    L.seen_s = reinterpret_cast<uint32_t*>(region + off); off += seen_words(local_range) * 4;
+    L.seen_f = reinterpret_cast<uint32_t*>(region + off); off += seen_words(local_range) * 4;
     off = (off + 15) & ~15;
     L.ukeys = reinterpret_cast<uint64_t*>(region + off);
     L.pw = region;
@@ -457,7 +460,8 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
         int valid = act ? 1 : 0;
         if (seen) {
             if (act && part == 0) {
-                const int bit = (dy + kMaxLocalRange) * (2 * kMaxLocalRange + 1) + (dx + kMaxLocalRange);
+                const int R = C.J->local_range;   // every deduped point lies in [-R, R]^2
+                const int bit = (dy + R) * (2 * R + 1) + (dx + R);
                 const uint32_t old = atomicOr(&seen[bit >> 5], 1u << (bit & 31));
                 valid = (old >> (bit & 31)) & 1 ? 0 : 1;
             }
@@ -976,7 +980,7 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
         set_err(C, 5);
         return {0, 0, kBadSad};
     }
-    for (int i = C.lane; i < kSeenWords; i += kWave) { C.L.seen_s[i] = 0; C.L.seen_f[i] = 0; }
+    for (int i = C.lane; i < seen_words(C.J->local_range); i += kWave) { C.L.seen_s[i] = 0; C.L.seen_f[i] = 0; }
     wsync();
     uint64_t best = kNone;
     int nkeys = 0;
@@ -1420,7 +1424,11 @@ __device__ __forceinline__ uint32_t sig_distance(uint64_t a, uint64_t b) {
 constexpr int kXwSide = 36;
 constexpr int kXwPitch = 64;
 constexpr int kXwBytes = kXwSide * kXwPitch;
-constexpr int kXwPerRound = 8;
+// candidates staged per round: their windows and 25 keys each must fit the
+// shared LDS region, which is at least the phase window's size
+constexpr int kXwPerRound = 6;
+static_assert(kXwPerRound * (kXwBytes + 25 * 8) <= kPwBytes, "extended-search rounds overflow the shared LDS region");
+static_assert(kXwPerRound <= 8 && kXwPerRound * 25 <= 4 * kWave, "one round's points are scored by 4 passes of the wave");
 
 // Stage the windows of candidates t0 .. t0+cnt-1 (offsets in `cand` lanes
 // t0..) into slots 0..cnt-1 of the shared LDS region.
@@ -1511,7 +1519,7 @@ __device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win
 // macroblocks' decisions, so its result is computed once per macroblock and
 // step (extended_content) however often the macroblock is analysed.  The
 // candidates are independent of each other, so all of
-// them are searched together (eight per round from windows staged in LDS) and
+// them are searched together (kXwPerRound per round from windows staged in LDS) and
 // the reference's sequential loop, with its early exit, is replayed on the
 // results; the op counters count only the candidates the loop reaches.
 __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out_sad, int* out_dx, int* out_dy) {
@@ -1590,8 +1598,9 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
     const int tn = __popcll(__ballot(C.lane < 16 && top != kNone));
     // Per candidate: the best key of its 5x5 grid at step 4 (stride-2 SAD)
     // and of the 5x5 full-SAD refinement around the grid's best, with the
-    // counts of scored (not vetoed) points.  Eight candidates per round; each
-    // lane scores whole points, the 25 keys of a candidate meet in LDS.
+    // counts of scored (not vetoed) points.  kXwPerRound candidates per
+    // round; each lane scores whole points, the 25 keys of a candidate meet in
+    // LDS.
     uint64_t key1 = kNone, key2 = kNone;   // lane t: results of candidate t
     uint32_t cnt1 = 0, cnt2 = 0;
     uint64_t* const kbuf = reinterpret_cast<uint64_t*>(C.L.pw + kXwPerRound * kXwBytes);
@@ -2472,7 +2481,7 @@ __global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ 
 // previous pass built; one workgroup per list entry (the host sizes the grid
 // from the list lengths).  The next list gets each dependent of a changed
 // macroblock once (the mark stamp).
-__global__ void __launch_bounds__(64) k_mb_relax(const JobDesc* __restrict__ jobs, int pass) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_relax(const JobDesc* __restrict__ jobs, int pass) {
     extern __shared__ __align__(16) uint8_t smem[];
     const JobDesc* J = &jobs[blockIdx.y];
     const int mbw = J->mbw, mbh = J->mbh;
