@@ -11,8 +11,12 @@ each GOP, each predicted from the anchor three frames back: 16 P frames,
 4-MV 8x8), quarter-pel, SATD finalists, PQINDEX 8, P-intra decision on.
 
 One step = vc1me_analyze_step over all 16 (cur, ref) pairs with the frames
-already resident in HBM; the MV fields stay on the device (their download is
-timed separately and reported as `value_with_download`).
+already resident in HBM, under the default relaxation schedule (every
+macroblock at once, then worklist passes until nothing changes; bit-exact
+with the raster-order reference); the MV fields stay on the device (their
+download is timed separately and reported as `value_with_download`).  The
+same steps under the anti-diagonal sweep are timed for comparison
+(`schedule.diagonal_value`).
 
 The learned-seeded form (`learned`) runs the same steps through
 vc1me_analyze_step_learned with the trained weights of
@@ -23,18 +27,19 @@ that got worse and the mean excess.  Two ablations take the network's
 contribution away (every macroblock seeded from the global vector alone; the
 network's confidence gate kept but its candidates removed).
 
-Also reported: the roofline of the dominant kernel (the macroblock search,
-k_mb_diagonal) from HIP events recorded around each of its launches inside
-the timed region, and the reference's own multithreaded CPU path timed on
-this host on a bounded sample of the same frames.
+Also reported: the roofline of the dominant kernel (the macroblock analysis,
+k_mb_relax: one launch per relaxation pass) from HIP events recorded around
+each of its launches inside the timed region, and the reference's own
+multithreaded CPU path timed on this host on a bounded sample of the same
+frames.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-check] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-check] [--no-cpu] [--no-learned] [--no-compare]
     python bench.py --copy-calibration     # 256 MiB device copy, for HBM counter scaling
 
 `roofline.traffic` (HBM bytes per launch of the macroblock kernel) comes from
 the FETCH_SIZE / WRITE_SIZE counter runs summarised in
-profiles/k_mb_diagonal_1080p_ibbp16_pmc.json, each scaled by what the counter
-reads for the calibration copy; it is null when that summary is absent.
+profiles/<kernel>_1080p_ibbp16_pmc.json, each scaled by what the counter reads
+for the calibration copy; it is null when that summary is absent.
 """
 from __future__ import annotations
 
@@ -109,15 +114,14 @@ def cpu_baseline(pairs, pics, params):
             "sample": "1 of the 16 1080p P-frame pairs, scalar C oracle", "seconds": round(secs, 3)}
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_mb_diagonal_1080p_ibbp16_pmc.json")
-
-
-def measured_traffic():
-    """HBM bytes per k_mb_diagonal launch from the committed counter summary."""
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of the macroblock kernel from the committed
+    counter summary of its schedule (profiles/<kernel>_1080p_ibbp16_pmc.json:
+    FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the calibration copy)."""
     try:
-        with open(PMC_SUMMARY) as f:
+        with open(os.path.join(ROOT, "profiles", f"{kernel}_1080p_ibbp16_pmc.json")) as f:
             d = json.load(f)
-        return d["k_mb_diagonal"]["hbm_bytes_per_launch"]
+        return d[kernel]["hbm_bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -225,6 +229,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the diagonal-sweep comparison steps")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -272,14 +277,15 @@ def main() -> int:
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s, schedule {schedule}")
 
     # the anti-diagonal sweep on the same steps, for comparison
-    be.set_schedule(abi.SCHEDULE_DIAGONAL)
-    be.analyze_step(jobs, plist, want_results=False)
-    t0 = time.perf_counter()
-    for i in range(args.steps):
+    if not args.no_compare:
+        be.set_schedule(abi.SCHEDULE_DIAGONAL)
         be.analyze_step(jobs, plist, want_results=False)
-    schedule["diagonal_value"] = round(mbs_per_step * args.steps / (time.perf_counter() - t0), 1)
-    be.set_schedule(abi.SCHEDULE_RELAX)
-    log(f"diagonal sweep: {schedule['diagonal_value']:,.0f} MB/s")
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            be.analyze_step(jobs, plist, want_results=False)
+        schedule["diagonal_value"] = round(mbs_per_step * args.steps / (time.perf_counter() - t0), 1)
+        be.set_schedule(abi.SCHEDULE_RELAX)
+        log(f"diagonal sweep: {schedule['diagonal_value']:,.0f} MB/s")
 
     # download-inclusive rate (MV fields back to the host every step)
     t1 = time.perf_counter()
@@ -310,7 +316,7 @@ def main() -> int:
         "peak": round(peak / 1e12, 2),
         "unit": "TOP/s",
         "frac": round(achieved / peak, 5),
-        "traffic": measured_traffic(),
+        "traffic": measured_traffic(mb_kernel),
         "kernel": mb_kernel,
         "avg_launch_us": round(avg_launch_us, 2),
         "launches_per_step": mb_launches // max(1, args.steps),
@@ -325,7 +331,7 @@ def main() -> int:
     roofline_hbm = {"bound": "hbm", "achieved": round(alg_bytes_launch / launch_s / 1e9, 3) if launch_s else 0.0,
                     "peak": 8000.0, "unit": "GB/s",
                     "frac": round(alg_bytes_launch / launch_s / 8e12, 7) if launch_s else 0.0,
-                    "traffic": measured_traffic(), "algorithmic_bytes_per_launch": round(alg_bytes_launch),
+                    "traffic": measured_traffic(mb_kernel), "algorithmic_bytes_per_launch": round(alg_bytes_launch),
                     "kernel": mb_kernel}
 
     learned_out = None
