@@ -257,6 +257,12 @@ int vc1me_set_graphs(vc1me_ctx* ctx, int on);
  * counterpart (the reference runs its macroblock loop on one thread). */
 enum { VC1ME_SCHEDULE_DIAGONAL = 0, VC1ME_SCHEDULE_RELAX = 1 };
 int vc1me_set_schedule(vc1me_ctx* ctx, int schedule);
+/* Per-pass record of the last step's relaxation: macroblock analyses of each
+ * pass summed over the jobs and, when profiling is on, the pass's device
+ * milliseconds (-1 otherwise), for the first `cap` passes.  Returns the pass
+ * count (0 under the diagonal sweep) or a negative status.  No reference
+ * counterpart (profiling aid). */
+int vc1me_last_relax_passes(vc1me_ctx* ctx, int64_t* analyses, double* ms, int cap);
 
 /* Algorithmic work of the last step, summed over its jobs, in the units of
  * the roofline: one |a-b| accumulate per sampled pixel of a block SAD
