@@ -2693,12 +2693,25 @@ __global__ void k_global_begin(const JobDesc* __restrict__ jobs, int njobs) {
 
 // Replay the sequential bookkeeping of one stage on the computed costs and
 // emit the next stage (src/encoder_motion.cpp:2043-2073 and :2094-2140).
-__global__ void k_global_step(const JobDesc* __restrict__ jobs, int njobs, int stage) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// One 64-thread block per job: the stage's points and costs are staged in LDS
+// by the block, then one thread replays the stage (its loads are then LDS
+// hits instead of a chain of dependent global loads).
+__global__ void __launch_bounds__(64) k_global_step(const JobDesc* __restrict__ jobs, int njobs, int stage) {
+    __shared__ uint64_t sc[kGlobalMaxPts];
+    __shared__ int32_t sp[2 * kGlobalMaxPts];
+    const int j = static_cast<int>(blockIdx.x);
     if (j >= njobs) return;
     const JobDesc* J = &jobs[j];
     GlobalState* g = J->g;
-    if (J->range == 0 || g->n == 0) return;
+    const int np_in = J->range == 0 ? 0 : imin(g->n, kGlobalMaxPts);
+    for (int i = static_cast<int>(threadIdx.x); i < np_in; i += static_cast<int>(blockDim.x)) {
+        sc[i] = g->cost[i];
+        sp[2 * i] = g->pts[2 * i];
+        sp[2 * i + 1] = g->pts[2 * i + 1];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    if (J->range == 0 || np_in == 0) return;
     const int R = g_range(J);
     const int s = g->stride;
     const uint64_t mul = static_cast<uint64_t>(s) * s;
@@ -2712,8 +2725,8 @@ __global__ void k_global_step(const JobDesc* __restrict__ jobs, int njobs, int s
         if (stage == 0 || stage == 1 || stage == 2) {
             GTop t;
             gtop_load(t, g);
-            const int np = g->n;
-            for (int i = 0; i < np; ++i) g_consider(t, g->cost[i] * (s > 1 ? mul : 1), g->pts[2 * i], g->pts[2 * i + 1]);
+            const int np = np_in;
+            for (int i = 0; i < np; ++i) g_consider(t, sc[i] * (s > 1 ? mul : 1), sp[2 * i], sp[2 * i + 1]);
             gtop_store(t, g);
             int n = 0;
             if (stage == 0) {
@@ -2754,9 +2767,9 @@ __global__ void k_global_step(const JobDesc* __restrict__ jobs, int njobs, int s
         } else {
             uint64_t bc = UINT64_MAX;
             int bx = 0, by = 0;
-            for (int i = 0; i < g->n; ++i)
-                if (g_better(g->cost[i], g->pts[2 * i], g->pts[2 * i + 1], bc, bx, by)) {
-                    bc = g->cost[i]; bx = g->pts[2 * i]; by = g->pts[2 * i + 1];
+            for (int i = 0; i < np_in; ++i)
+                if (g_better(sc[i], sp[2 * i], sp[2 * i + 1], bc, bx, by)) {
+                    bc = sc[i]; bx = sp[2 * i]; by = sp[2 * i + 1];
                 }
             g->dx = bx; g->dy = by; g->full_sad = bc;
             g->n = 0;
@@ -2768,9 +2781,9 @@ __global__ void k_global_step(const JobDesc* __restrict__ jobs, int njobs, int s
     if (stage == 0) {
         // sorted top-8 of every coarse point (push, sort, truncate)
         g->tn = 0;
-        for (int i = 0; i < g->n; ++i) {
-            const uint64_t c = g->cost[i] * mul;
-            const int x = g->pts[2 * i], y = g->pts[2 * i + 1];
+        for (int i = 0; i < np_in; ++i) {
+            const uint64_t c = sc[i] * mul;
+            const int x = sp[2 * i], y = sp[2 * i + 1];
             g->tcost[g->tn] = c; g->tdx[g->tn] = x; g->tdy[g->tn] = y; g->tn++;
             for (int a = g->tn - 1; a > 0 && g_better(g->tcost[a], g->tdx[a], g->tdy[a], g->tcost[a - 1], g->tdx[a - 1], g->tdy[a - 1]); --a) {
                 const uint64_t tc = g->tcost[a]; g->tcost[a] = g->tcost[a - 1]; g->tcost[a - 1] = tc;
@@ -2802,9 +2815,9 @@ __global__ void k_global_step(const JobDesc* __restrict__ jobs, int njobs, int s
     } else if (stage == 1) {
         uint64_t bc = UINT64_MAX;
         int bx = 0, by = 0;
-        for (int i = 0; i < g->n; ++i) {
-            const uint64_t c = g->cost[i] * mul;
-            const int x = g->pts[2 * i], y = g->pts[2 * i + 1];
+        for (int i = 0; i < np_in; ++i) {
+            const uint64_t c = sc[i] * mul;
+            const int x = sp[2 * i], y = sp[2 * i + 1];
             const int cm = iabs(x) + iabs(y), bm = iabs(bx) + iabs(by);
             if (c < bc || (c == bc && cm < bm)) { bc = c; bx = x; by = y; }
         }
@@ -2979,7 +2992,7 @@ hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc*
     for (int stage = 0; stage < 4; ++stage) {
         if (!pts[stage]) continue;
         hipLaunchKernelGGL(k_frame_sads, dim3(chunks[stage], pts[stage], njobs), dim3(256), 0, s, d_jobs);
-        hipLaunchKernelGGL(k_global_step, dim3(gb), dim3(tb), 0, s, d_jobs, njobs, stage);
+        hipLaunchKernelGGL(k_global_step, dim3(njobs), dim3(64), 0, s, d_jobs, njobs, stage);
     }
     if (ev1) (void)hipEventRecord(ev1, s);
     return hipGetLastError();
