@@ -69,6 +69,12 @@ int relax_pass_limit() {
     return std::max(1, std::min(v, vc1me::kMaxRelaxPasses));
 }
 
+// Guess pass before the relaxation's pass 0 (VC1ME_RELAX_GUESS=0 turns it off).
+bool relax_guess() {
+    const char* e = std::getenv("VC1ME_RELAX_GUESS");
+    return !(e && *e == '0');
+}
+
 uint64_t fnv64(const uint8_t* p, size_t n) {
     uint64_t h = 1469598103934665603ull;
     size_t i = 0;
@@ -649,6 +655,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     bool relax_fell_back = false;
     auto relax_all = [&](bool events) -> int {
         HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
+        if (relax_guess()) HIP_TRY(vc1me::launch_mb_guess(ctx->d_jobs, n, max_mbs_all, lds, st));
         const int limit = relax_pass_limit();
         int grid = max_mbs_all;
         int64_t first = 0;

@@ -116,6 +116,7 @@ hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int ld
 // relaxation schedule: starting state, one pass over a worklist, and the
 // per-job op totals of the final macroblock analyses
 hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipStream_t s);
+hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
 hipError_t launch_mb_relax(const JobDesc* d_jobs, int njobs, int pass, int grid_x, int lds_bytes, hipStream_t s);
 hipError_t launch_ops_sum(const JobDesc* d_jobs, int njobs, hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);

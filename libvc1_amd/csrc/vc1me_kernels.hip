@@ -2013,10 +2013,14 @@ struct NbState {
     uint32_t fp;    // intra flag | propagated flag << 8
 };
 
+// kGuess: the relaxation's starting state only (never a result): the
+// analysis without the 4-MV stage and the extended long-range search, the
+// two costliest parts whose outcome rarely changes a neighbour's predictors.
+template <bool kGuess>
 __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     const JobDesc* J = C.J;
     const int mode = J->mode;
-    const bool allow_4mv = mode == 3;
+    const bool allow_4mv = mode == 3 && !kGuess;
     const int one_mode = allow_4mv ? 0 : mode;
     const int range = J->range;
     const int local_range = J->local_range;
@@ -2109,7 +2113,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
                 }
             }
         }
-        if (range > local_range && !seeded_ok && !match_good(C, im.sad)) {
+        if (!kGuess && range > local_range && !seeded_ok && !match_good(C, im.sad)) {
             uint64_t ext_sad = UINT64_MAX;
             int ext_dec = 0;
             const IRes ext = extended_content(C, im, &ext_sad, &ext_dec);
@@ -2339,11 +2343,12 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const JobDesc* J, int mx, int m
     C.ln_sad = C.ln_taps = C.ln_satd = 0;
 }
 
+template <bool kGuess = false>
 __device__ __forceinline__ NbState run_mb(const JobDesc* J, int mx, int my, uint8_t* smem) {
     Ctx C;
     init_ctx(C, J, mx, my, smem);
     NbState st;
-    analyze_mb(C, st);
+    analyze_mb<kGuess>(C, st);
     return st;
 }
 
@@ -2475,6 +2480,17 @@ __global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ 
     J->flags[pos] = 0;
     J->prop[pos] = 0;
     J->mark[pos] = 0;
+}
+
+// The starting state of the relaxation: every macroblock's guess analysis at
+// once (its outputs are all overwritten by pass 0, which analyses every
+// macroblock for real).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_guess(const JobDesc* __restrict__ jobs) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const JobDesc* J = &jobs[blockIdx.y];
+    const int pos = static_cast<int>(blockIdx.x);
+    if (pos >= J->mbw * J->mbh) return;
+    run_mb<true>(J, pos % J->mbw, pos / J->mbw, smem);
 }
 
 // One pass: pass 0 analyses every macroblock, pass p > 0 the worklist the
@@ -3074,6 +3090,12 @@ hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int ld
 hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipStream_t s) {
     if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_relax_init, dim3((max_mbs + 255) / 256, njobs), dim3(256), 0, s, d_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s) {
+    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mb_guess, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs);
     return hipGetLastError();
 }
 
