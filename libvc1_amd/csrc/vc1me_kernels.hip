@@ -252,28 +252,43 @@ __device__ __forceinline__ unsigned __int128 x87_score(uint64_t cost, uint64_t r
 __host__ __device__ inline int seen_words(int local_range) {
     return ((2 * local_range + 1) * (2 * local_range + 1) + 31) / 32;
 }
-constexpr int kMaxUmhKeys = 768;     // sampled-SAD points an integer search can consider
 constexpr int kSmallWin = 80 * 96;   // windows of the far (long-range) candidates
+
+// Sampled points the UMH search of local range R can log: its stages score at
+// most 5 seeds, a 5x5, the sparse lattice, the uneven cross, the hexagon
+// rings, 32 small-hexagon steps of 6 and a closing diamond of 4 (each point
+// logged once: the log only takes points the dedupe bitset has not seen).
+__host__ __device__ inline int umh_key_cap(int R) {
+    const int side = (2 * R) / 8 + 1;
+    const int lattice = R >= 8 ? side * side + 2 * side : 0;
+    const int cs = R >= 96 ? 8 : (R >= 32 ? 4 : 2);
+    const int vlimit = R < (R / 2 > 8 ? R / 2 : 8) ? R : (R / 2 > 8 ? R / 2 : 8);
+    const int cross = 2 * (R / cs + vlimit / cs);
+    const int hs = R >= 96 ? 8 : (R >= 24 ? 4 : 2);
+    return 5 + 25 + lattice + cross + 6 * (R / hs) + 32 * 6 + 4;
+}
 
 // Phase window: the interpolated reference (every phase) around a macroblock's
 // integer vector, staged in LDS for the sub-pel stages.  kPwSide samples
-// square per phase; rows hold kPwPitch bytes so that 16-byte aligned copies
-// of the phase planes fit whatever the alignment of the window origin.
+// square per phase; rows hold kPwPitch bytes so that 4-byte aligned copies
+// of the phase planes (plus the 8-byte read of the block scorer) fit whatever
+// the alignment of the window origin.
 constexpr int kPwMargin = 3;
 constexpr int kPwSide = 16 + 2 * kPwMargin;
-constexpr int kPwPitch = 48;
-constexpr int kPwBytes = (16 * kPwSide * kPwPitch + 1023) & ~1023;  // whole 1 KB copy blocks
+constexpr int kPwPitch = 32;
+static_assert(kPwSide + 3 <= kPwPitch && ((kPwSide - 4 + 3) & ~3) + 8 <= kPwPitch, "phase-window rows too short");
+constexpr int kPwBytes = (16 * kPwSide * kPwPitch + 255) & ~255;   // whole 256-byte copy blocks
 
 __host__ __device__ inline int win_capacity(int local_range) {
     const int side = 16 + 2 * local_range;
-    const int pitch = (side + 16 + 8 + 15) & ~15;  // alignment of the origin plus read slack
+    const int pitch = (side + 3 + 8 + 3) & ~3;     // 4-byte alignment of the origin plus read slack
     const int need = (side + 1) * pitch;
     return need > kSmallWin ? need : kSmallWin;
 }
 
 struct Lds {
     uint8_t* cur;        // 16x16 current block, zero outside the active area
-    uint8_t* win;        // reference window (16-byte aligned rows)
+    uint8_t* win;        // reference window (4-byte aligned rows)
     uint8_t* pw;         // phase window
     uint32_t* seen_s;    // sampled-SAD dedupe bitset of the UMH search
     uint32_t* seen_f;    // full-SAD dedupe bitset
@@ -283,18 +298,20 @@ struct Lds {
 };
 
 // The integer searches' window, dedupe bitsets and key log are dead once the
-// sub-pel stages start, so the phase window shares their bytes.
+// sub-pel stages start, so the phase window shares their bytes; so does the
+// extended search's staging (kXwRoundBytes, defined with it below).
 __host__ __device__ inline int int_scratch_bytes(int local_range) {
     int off = (win_capacity(local_range) + 15) & ~15;  // win
     off += seen_words(local_range) * 4 * 2;             // seen_s, seen_f
     off = (off + 15) & ~15;
-    off += kMaxUmhKeys * 8;                             // ukeys
+    off += umh_key_cap(local_range) * 8;                // ukeys
     return off;
 }
 
+__host__ __device__ inline int shared_bytes(int local_range);
+
 __host__ __device__ inline int lds_bytes_for(int local_range) {
-    const int shared = int_scratch_bytes(local_range) > kPwBytes ? int_scratch_bytes(local_range) : kPwBytes;
-    return 256 + shared + 16 * 8 + 4 * 8 + 64;          // cur, shared region, top16, own
+    return 256 + shared_bytes(local_range) + 16 * 8 + 4 * 8 + 64;   // cur, shared region, top16, own
 }
 
 __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
@@ -308,7 +325,7 @@ __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     off = (off + 15) & ~15;
     L.ukeys = reinterpret_cast<uint64_t*>(region + off);
     L.pw = region;
-    const int shared = int_scratch_bytes(local_range) > kPwBytes ? int_scratch_bytes(local_range) : kPwBytes;
+    const int shared = shared_bytes(local_range);
     L.top16 = reinterpret_cast<uint64_t*>(region + shared);
     L.own = reinterpret_cast<int2*>(region + shared + 16 * 8);
     return L;
@@ -354,40 +371,34 @@ __device__ __forceinline__ void set_err(Ctx& C, int code) {
 // 16 bytes so whole rows move as 16-byte loads.  Returns false if the window
 // does not fit the buffer.
 __device__ __forceinline__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int wh) {
-    const int ax0 = wx0 & ~15;
+    // rows start at the dword holding column wx0 and carry 8 bytes of read
+    // slack for the row SAD; copied one dword per lane
+    const int ax0 = wx0 & ~3;
     const int aw = wx0 + ww - ax0;
-    const int wp = (aw + 8 + 15) & ~15;
+    const int wp = (aw + 8 + 3) & ~3;
     if (ww <= 0 || wh <= 0 || wp * wh > C.winCap) return false;
     wsync();
     const int W = C.g.W, H = C.g.H;
-    const int cpr = wp >> 4;
+    const int cpr = wp >> 2;
     const int total = cpr * wh;
-    const bool aligned_rows = (W & 15) == 0;
+    const bool aligned_rows = (W & 3) == 0;
     gptr_u8 ry = as_global(C.ry);
-    uint4* w16 = reinterpret_cast<uint4*>(C.L.win);
+    uint32_t* w4 = reinterpret_cast<uint32_t*>(C.L.win);
 #pragma unroll 4
     for (int i = C.lane; i < total; i += kWave) {
         const int r = i / cpr, c = i - r * cpr;
         const int sy = clampi(wy0 + r, 0, H - 1);
-        const int x = ax0 + c * 16;
+        const int x = ax0 + c * 4;
         gptr_u8 row = ry + static_cast<size_t>(sy) * W;
-        uint4 v;
-        if (aligned_rows && x >= 0 && x + 16 <= W) {
-            const u32x4 t = *reinterpret_cast<const __attribute__((address_space(1))) u32x4*>(row + x);
-            v = make_uint4(t.x, t.y, t.z, t.w);
+        uint32_t v;
+        if (aligned_rows && x >= 0 && x + 4 <= W) {
+            v = *reinterpret_cast<const __attribute__((address_space(1))) uint32_t*>(row + x);
         } else {
-            uint32_t wv[4];
+            v = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t acc = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    acc |= static_cast<uint32_t>(row[clampi(x + q * 4 + b, 0, W - 1)]) << (8 * b);
-                wv[q] = acc;
-            }
-            v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+            for (int b = 0; b < 4; ++b) v |= static_cast<uint32_t>(row[clampi(x + b, 0, W - 1)]) << (8 * b);
         }
-        w16[i] = v;
+        w4[i] = v;
     }
     C.wx0 = ax0; C.wy0 = wy0; C.ww = aw; C.wh = wh; C.wp = wp;
     wsync();
@@ -497,9 +508,10 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
         if (ukeys) {
             const uint64_t m = __ballot(leader);
             const int pos = *ucount + lane_prefix(m);
-            if (leader && pos < kMaxUmhKeys) ukeys[pos] = key;
+            const int cap = umh_key_cap(C.J->local_range);
+            if (leader && pos < cap) ukeys[pos] = key;
             *ucount += __popcll(m);
-            if (*ucount > kMaxUmhKeys) set_err(C, 8);
+            if (*ucount > cap) set_err(C, 8);
         }
         const uint64_t mk = wave_min64(key);
         best = mk < best ? mk : best;
@@ -772,7 +784,7 @@ __device__ __forceinline__ uint32_t plane_row4(gptr_u8 row, int x0, int W) {
 
 // Stage the phase window with sample (0,0) at plane coordinates (wx, wy):
 // kPwSide x kPwSide samples of every phase of the filter set, copied from the
-// phase planes as 16-byte aligned chunks (a row of the window starts pwsh
+// phase planes as 4-byte aligned dwords (a row of the window starts pwsh
 // bytes into its LDS row).
 __device__ __forceinline__ void pw_load(Ctx& C, int wx, int wy, bool bil) {
     const JobDesc* J = C.J;
@@ -781,47 +793,35 @@ __device__ __forceinline__ void pw_load(Ctx& C, int wx, int wy, bool bil) {
     const size_t psz = static_cast<size_t>(J->pp) * J->ph;
     const int W = C.g.W, H = C.g.H;
     const int c0 = wx + 2;
-    const int a0 = c0 & ~15;
-    constexpr int kChunks = kPwPitch / 16;
-    const int total = np * kPwSide * kChunks;
+    const int a0 = c0 & ~3;
+    constexpr int kDwords = kPwPitch / 4;
+    const int total = np * kPwSide * kDwords;
     wsync();
     if (a0 >= 0 && a0 + kPwPitch <= W + 3) {
-        // every chunk lies inside the plane columns: copy global -> LDS
-        // directly, 64 chunks (1 KB, chunk i at byte 16 i) per instruction
+        // every dword lies inside the plane columns: copy global -> LDS
+        // directly, 64 dwords (256 bytes, dword i at byte 4 i) per instruction
         typedef const __attribute__((address_space(1))) void* gvoid;
         typedef __attribute__((address_space(3))) void* lvoid;
         for (int base = 0; base < total; base += kWave) {
             const int i = base + C.lane;
-            const int ph = i / (kPwSide * kChunks), rem = i - ph * (kPwSide * kChunks);
-            const int r = rem / kChunks, ch = rem - r * kChunks;
+            const int ph = i / (kPwSide * kDwords), rem = i - ph * (kPwSide * kDwords);
+            const int r = rem / kDwords, d = rem - r * kDwords;
             const int prow = clampi(wy + r, -2, H) + 2;
-            const gptr_u8 src = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp + a0 + ch * 16;
-            // lanes past the window issue no copy (a copy lands at byte 16 * lane)
-            if (i < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(C.L.pw + base * 16), 16, 0, 0);
+            const gptr_u8 src = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp + a0 + d * 4;
+            // lanes past the window issue no copy (a copy lands at byte 4 * lane)
+            if (i < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(C.L.pw + base * 4), 4, 0, 0);
         }
     } else
     for (int i = C.lane; i < total; i += kWave) {
-        const int ph = i / (kPwSide * kChunks), rem = i - ph * (kPwSide * kChunks);
-        const int r = rem / kChunks, ch = rem - r * kChunks;
+        const int ph = i / (kPwSide * kDwords), rem = i - ph * (kPwSide * kDwords);
+        const int r = rem / kDwords, d = rem - r * kDwords;
         const int prow = clampi(wy + r, -2, H) + 2;
-        const int col = a0 + ch * 16;
+        const int col = a0 + d * 4;
         const gptr_u8 row = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp;
-        uint4 v;
-        if (col >= 0 && col + 16 <= W + 3) {
-            const u32x4 t = *reinterpret_cast<const __attribute__((address_space(1))) u32x4*>(row + col);
-            v = make_uint4(t.x, t.y, t.z, t.w);
-        } else {
-            uint32_t wv[4];
+        uint32_t v = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t acc = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc |= static_cast<uint32_t>(row[clampi(col + q * 4 + b, 0, W + 2)]) << (8 * b);
-                wv[q] = acc;
-            }
-            v = make_uint4(wv[0], wv[1], wv[2], wv[3]);
-        }
-        *reinterpret_cast<uint4*>(C.L.pw + (ph * kPwSide + r) * kPwPitch + ch * 16) = v;
+        for (int b = 0; b < 4; ++b) v |= static_cast<uint32_t>(row[clampi(col + b, 0, W + 2)]) << (8 * b);
+        *reinterpret_cast<uint32_t*>(C.L.pw + (ph * kPwSide + r) * kPwPitch + d * 4) = v;
     }
     C.pwx = wx; C.pwy = wy; C.pwsh = c0 - a0; C.pw_ok = true; C.pw_bil = bil;
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;   // the integer window is overwritten
@@ -1084,7 +1084,7 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
     // the best four considered points (top[0..3] of the reference's list)
     wsync();
     uint64_t tk[4] = {kNone, kNone, kNone, kNone};
-    const int nk = imin(nkeys, kMaxUmhKeys);
+    const int nk = imin(nkeys, umh_key_cap(C.J->local_range));
     const int fin = imin(4, nk);
     {
         uint64_t prev = 0;
@@ -1426,8 +1426,15 @@ constexpr int kXwPitch = 64;
 constexpr int kXwBytes = kXwSide * kXwPitch;
 // candidates staged per round: their windows and 25 keys each must fit the
 // shared LDS region, which is at least the phase window's size
-constexpr int kXwPerRound = 6;
-static_assert(kXwPerRound * (kXwBytes + 25 * 8) <= kPwBytes, "extended-search rounds overflow the shared LDS region");
+constexpr int kXwPerRound = 5;
+constexpr int kXwRoundBytes = kXwPerRound * (kXwBytes + 25 * 8);   // windows, then 25 keys per candidate
+
+// The shared LDS region: the largest of its three uses.
+__host__ __device__ inline int shared_bytes(int local_range) {
+    const int a = int_scratch_bytes(local_range);
+    const int b = kPwBytes > kXwRoundBytes ? kPwBytes : kXwRoundBytes;
+    return a > b ? a : b;
+}
 static_assert(kXwPerRound <= 8 && kXwPerRound * 25 <= 4 * kWave, "one round's points are scored by 4 passes of the wave");
 
 // Stage the windows of candidates t0 .. t0+cnt-1 (offsets in `cand` lanes
