@@ -295,7 +295,13 @@ struct Lds {
     uint64_t* ukeys;     // keys of every sampled point the UMH search considered
     uint64_t* top16;     // long-range top-16 keys
     int2* own;           // current MB's four block vectors
+    uint32_t* nb;        // neighbour snapshot: kNbWords dwords each of left, top, top-left, top-right
 };
+
+// Neighbour snapshot per neighbour: 1-MV vector (2 dwords), the four block
+// vectors (8), flags (1), propagated flag (1).
+constexpr int kNbWords = 12;
+enum { NB_LEFT = 0, NB_TOP = 1, NB_TOPLEFT = 2, NB_TOPRIGHT = 3 };
 
 // The integer searches' window, dedupe bitsets and key log are dead once the
 // sub-pel stages start, so the phase window shares their bytes; so does the
@@ -311,7 +317,8 @@ __host__ __device__ inline int int_scratch_bytes(int local_range) {
 __host__ __device__ inline int shared_bytes(int local_range);
 
 __host__ __device__ inline int lds_bytes_for(int local_range) {
-    return 256 + shared_bytes(local_range) + 16 * 8 + 4 * 8 + 64;   // cur, shared region, top16, own
+    // cur, shared region, top16, own, neighbour snapshot, slack
+    return 256 + shared_bytes(local_range) + 16 * 8 + 4 * 8 + 4 * 12 * 4 + 64;
 }
 
 __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
@@ -328,6 +335,7 @@ __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     const int shared = shared_bytes(local_range);
     L.top16 = reinterpret_cast<uint64_t*>(region + shared);
     L.own = reinterpret_cast<int2*>(region + shared + 16 * 8);
+    L.nb = reinterpret_cast<uint32_t*>(region + shared + 16 * 8 + 4 * 8);
     return L;
 }
 
@@ -579,15 +587,47 @@ __device__ __forceinline__ void pullback(int qx, int qy, int X, int Y, int margi
     if (qy + py > Y) py = Y - qy;
 }
 
+// The neighbours' state, read once per analysis into LDS (every predictor
+// and the propagated search read this snapshot): lane l loads dword l % 12 of
+// neighbour l / 12; neighbours outside the frame read as zero.
+__device__ __forceinline__ void nb_load(Ctx& C) {
+    const JobDesc* J = C.J;
+    if (C.lane < 4 * kNbWords) {
+        const int n = C.lane / kNbWords, w = C.lane - n * kNbWords;
+        const int nx = C.mx + (n == NB_LEFT || n == NB_TOPLEFT ? -1 : n == NB_TOPRIGHT ? 1 : 0);
+        const int ny = C.my + (n == NB_LEFT ? 0 : -1);
+        uint32_t v = 0;
+        if (nx >= 0 && nx < C.g.mbw && ny >= 0) {
+            const int pos = ny * C.g.mbw + nx;
+            if (w < 2) v = reinterpret_cast<const uint32_t*>(J->mvs)[2 * pos + w];
+            else if (w < 10) v = reinterpret_cast<const uint32_t*>(J->bmv)[8 * pos + (w - 2)];
+            else if (w == 10) v = J->flags[pos];
+            else v = J->prop[pos];
+        }
+        C.L.nb[C.lane] = v;
+    }
+}
+__device__ __forceinline__ int2 nb_mv(const Ctx& C, int n) {
+    return make_int2(static_cast<int>(C.L.nb[n * kNbWords]), static_cast<int>(C.L.nb[n * kNbWords + 1]));
+}
+__device__ __forceinline__ int2 nb_bmv(const Ctx& C, int n, int k) {
+    return make_int2(static_cast<int>(C.L.nb[n * kNbWords + 2 + 2 * k]), static_cast<int>(C.L.nb[n * kNbWords + 3 + 2 * k]));
+}
+__device__ __forceinline__ uint32_t nb_flags(const Ctx& C, int n) { return C.L.nb[n * kNbWords + 10]; }
+__device__ __forceinline__ uint32_t nb_prop(const Ctx& C, int n) { return C.L.nb[n * kNbWords + 11]; }
+// the snapshot slot of macroblock (mx + dx, my + dy): left, top, top-left or top-right
+__device__ __forceinline__ int nb_slot(int dx, int dy) {
+    return dy == 0 ? NB_LEFT : dx < 0 ? NB_TOPLEFT : dx > 0 ? NB_TOPRIGHT : NB_TOP;
+}
+
 // predictor_info, src/encoder_motion.cpp:1312-1352 (reads the final 1-MV field)
 __device__ __forceinline__ Pred predictor_info(const Ctx& C) {
-    const JobDesc* J = C.J;
     const int mbw = C.g.mbw, mbh = C.g.mbh, mx = C.mx, my = C.my;
     const bool av = my > 0, cv = mx > 0, bv = my > 0 && mbw > 1;
     int2 A = {0, 0}, B = {0, 0}, Cc = {0, 0};
-    if (av) A = J->mvs[(my - 1) * mbw + mx];
-    if (cv) Cc = J->mvs[my * mbw + mx - 1];
-    if (bv) B = J->mvs[(my - 1) * mbw + ((mx == mbw - 1) ? mx - 1 : mx + 1)];
+    if (av) A = nb_mv(C, NB_TOP);
+    if (cv) Cc = nb_mv(C, NB_LEFT);
+    if (bv) B = nb_mv(C, mx == mbw - 1 ? NB_TOPLEFT : NB_TOPRIGHT);
     int px = 0, py = 0;
     if (av) { px = A.x; py = A.y; } else if (cv) { px = Cc.x; py = Cc.y; } else if (bv) { px = B.x; py = B.y; }
     if ((av ? 1 : 0) + (bv ? 1 : 0) + (cv ? 1 : 0) > 1) { px = median3(A.x, B.x, Cc.x); py = median3(A.y, B.y, Cc.y); }
@@ -600,18 +640,19 @@ __device__ __forceinline__ Pred predictor_info(const Ctx& C) {
     return {px, py, A.x, A.y, Cc.x, Cc.y, av, cv, hyb};
 }
 
-// luma block-vector field with the current macroblock's blocks taken from LDS
+// luma block-vector field: the current macroblock's blocks from LDS own[],
+// its neighbours' from the snapshot
 __device__ __forceinline__ int2 field_get(const Ctx& C, int x, int y) {
     if (x < 0 || y < 0 || x >= C.g.mbw * 2 || y >= C.g.mbh * 2) return {0, 0};
     const int mmx = x >> 1, mmy = y >> 1, k = ((y & 1) << 1) | (x & 1);
     if (mmx == C.mx && mmy == C.my) return C.L.own[k];
-    return C.J->bmv[(mmy * C.g.mbw + mmx) * 4 + k];
+    return nb_bmv(C, nb_slot(mmx - C.mx, mmy - C.my), k);
 }
 __device__ __forceinline__ bool intra_at(const Ctx& C, int x, int y) {
     if (x < 0 || y < 0 || x >= C.g.mbw * 2 || y >= C.g.mbh * 2) return false;
     const int mmx = x >> 1, mmy = y >> 1;
     if (mmx == C.mx && mmy == C.my) return false;
-    return (C.J->flags[mmy * C.g.mbw + mmx] & kFlagIntra) != 0;
+    return (nb_flags(C, nb_slot(mmx - C.mx, mmy - C.my)) & kFlagIntra) != 0;
 }
 
 __device__ __forceinline__ bool hybrid_flag(const Ctx& C, int px, int py, int bx, int by, int2 A, int2 Cc) {
@@ -1356,9 +1397,8 @@ __device__ __forceinline__ IRes propagated(Ctx& C, int range) {
     for (int k = 0; k < 4; ++k) {
         const int nx = C.mx + nbx(k), ny = C.my + nby(k);
         if (nx < 0 || ny < 0 || nx >= C.g.mbw) continue;
-        const int pos = ny * C.g.mbw + nx;
-        if (!J->prop[pos]) continue;
-        const int2 f = J->mvs[pos];
+        if (!nb_prop(C, k)) continue;           // k: left, top, top-left, top-right
+        const int2 f = nb_mv(C, k);
         const int cx = round_qpel_to_pixel(f.x), cy = round_qpel_to_pixel(f.y);
         const uint64_t s = far_batch(C, 13, 1, KEY_SEQ_MAN, k * 13, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             const int ox = i == 1 || i == 9 || i == 11 ? -1 : i == 2 || i == 10 || i == 12 ? 1 : i == 5 ? -2 : i == 6 ? 2 : 0;
@@ -1374,7 +1414,7 @@ __device__ __forceinline__ IRes propagated(Ctx& C, int range) {
     // map the sequence number back to the vector
     const int seq = static_cast<int>(best & 0xffff);
     const int k = seq / 13, i = seq % 13;
-    const int2 f = J->mvs[(C.my + nby(k)) * C.g.mbw + (C.mx + nbx(k))];
+    const int2 f = nb_mv(C, k);
     const int cx = round_qpel_to_pixel(f.x), cy = round_qpel_to_pixel(f.y);
     const int ox = i == 1 || i == 9 || i == 11 ? -1 : i == 2 || i == 10 || i == 12 ? 1 : i == 5 ? -2 : i == 6 ? 2 : 0;
     const int oy = i == 3 || i == 9 || i == 10 ? -1 : i == 4 || i == 11 || i == 12 ? 1 : i == 7 ? -2 : i == 8 ? 2 : 0;
@@ -1546,7 +1586,7 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
         const int nxc = cx_hi - cx_lo + 1;
         const int chunks = (8 + nxc * 8 + 15) >> 4;           // per row, worst alignment
         const int pitch = chunks * 16;
-        const int region = lds_bytes_for(J->local_range) - 256 - 16 * 8 - 4 * 8 - 64;
+        const int region = shared_bytes(J->local_range);
         const int rows_per = imax(1, region / pitch);
         const gptr_u8 sigb = as_global(reinterpret_cast<const uint8_t*>(J->sig));
         typedef const __attribute__((address_space(1))) void* gvoid;
@@ -2033,11 +2073,13 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     const int local_range = J->local_range;
     const int pos = C.pos;
 
-    // current block to LDS, zero outside the active area
+    // current block to LDS, zero outside the active area; the neighbours'
+    // state snapshot
     for (int i = C.lane; i < 256; i += kWave) {
         const int x = i & 15, y = i >> 4;
         C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
     }
+    nb_load(C);
     wsync();
 
     uint64_t one_sad = 0;
