@@ -273,7 +273,7 @@ __host__ __device__ inline int umh_key_cap(int R) {
 // square per phase; rows hold kPwPitch bytes so that 4-byte aligned copies
 // of the phase planes (plus the 8-byte read of the block scorer) fit whatever
 // the alignment of the window origin.
-constexpr int kPwMargin = 3;
+constexpr int kPwMargin = 4;
 constexpr int kPwSide = 16 + 2 * kPwMargin;
 constexpr int kPwPitch = 32;
 static_assert(kPwSide + 3 <= kPwPitch && ((kPwSide - 4 + 3) & ~3) + 8 <= kPwPitch, "phase-window rows too short");
