@@ -27,6 +27,10 @@ that got worse and the mean excess.  Two ablations take the network's
 contribution away (every macroblock seeded from the global vector alone; the
 network's confidence gate kept but its candidates removed).
 
+The batched leg (`batched`) runs the 64-stream workload of uneven rasters
+and lengths (lengths scaled to keep the run short) through the batching
+runner with the tuned table; it is reported beside the headline, not as it.
+
 Also reported: the roofline of the dominant kernel (the macroblock analysis,
 k_mb_relax: one launch per relaxation pass) from HIP events recorded around
 each of its launches inside the timed region, and the reference's own
@@ -34,6 +38,7 @@ multithreaded CPU path timed on this host on a bounded sample of the same
 frames.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-check] [--no-cpu] [--no-learned] [--no-compare]
+                    [--no-batched] [--batched-scale S]
     python bench.py --copy-calibration     # 256 MiB device copy, for HBM counter scaling
 
 `roofline.traffic` (HBM bytes per launch of the macroblock kernel) comes from
@@ -220,6 +225,64 @@ def learned_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, thres
     }
 
 
+BATCHED_STREAMS, BATCHED_SEED = 64, 5
+
+
+def batched_frames(scale: float):
+    """The 64-stream batched workload's pictures (resolutions uniform over
+    720x480, 1280x720, 1920x1080, 3840x2160; lengths uniform in 8..120
+    frames times `scale`, at least 2; content as the 1080p workload with
+    per-stream seeds), generated in worker processes before the GPU is
+    touched."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from libvc1_amd import batching
+    specs = batching.stream_specs(BATCHED_STREAMS, BATCHED_SEED, scale)
+    t0 = time.time()
+    with ProcessPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)), mp_context=mp.get_context("fork")) as pool:
+        out = list(pool.map(batching._gen, specs, chunksize=1))
+    log(f"batched: generated {sum(len(p) for *_, p in out)} pictures of {len(out)} streams in {time.time() - t0:.1f}s")
+    return out
+
+
+def batched_leg(be, generated, scale: float, check: bool):
+    """Every P pair of the 64 streams through the batching runner (steps and
+    modes from the tuned table, frames resident), timed once after a warm-up
+    run; one 720x480 pair is checked against the oracle."""
+    from libvc1_amd import batching
+    streams = [batching.Stream(w, h, [be.upload(p) for p in pics], batching.stream_params(w, h), seed)
+               for seed, w, h, pics in generated]
+    try:
+        batching.run(be, streams)                      # warm-up (graph capture, allocations)
+        out = batching.run(be, streams)
+        parity = None
+        if check:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from _oracle import oracle_analyze
+            si = next(i for i, s in enumerate(streams) if (s.width, s.height) == (720, 480))
+            s = streams[si]
+            got = be.analyze(s.frames[1], s.frames[0], s.params)
+            want = oracle_analyze(s.params, generated[si][3][1], generated[si][3][0])
+            bad = got.diff(want)
+            parity = f"stream {si} pair (1, 0) bit-exact" if not bad else f"MISMATCH in {bad}"
+    finally:
+        for s in streams:
+            for f in s.frames:
+                f.release()
+    res = {"value": round(out["mbs_per_s"], 1), "unit": "MB/s", "seconds": round(out["seconds"], 4),
+           "macroblocks": out["macroblocks"], "pairs": sum(len(s.pairs) for s in streams), "streams": len(streams),
+           "config": f"{BATCHED_STREAMS} streams (seed {BATCHED_SEED}), rasters uniform over 720x480/1280x720/"
+                     f"1920x1080/3840x2160, lengths uniform 8-120 frames x {scale} (>= 2), P-only, R 256, local 32, "
+                     "Mixed-MV qpel SATD; steps and modes from the tuned table",
+           "per_resolution": {k: {"pairs": v["pairs"], "macroblocks": v["macroblocks"], "seconds": v["seconds"],
+                                  "mode": f"{v['setting']['schedule']}{'+graph' if v['setting']['graphs'] else ''}"
+                                          f" x{v['setting']['jobs_per_step']}"}
+                              for k, v in out["per_resolution"].items()},
+           "parity": parity}
+    log(f"batched: {res['value']:,.0f} MB/s over {res['pairs']} pairs; {res['per_resolution']}; parity {parity}")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +293,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the diagonal-sweep comparison steps")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-stream batched leg")
+    ap.add_argument("--batched-scale", type=float, default=0.1,
+                    help="stream lengths (8..120 frames) are scaled by this in the batched leg")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -242,6 +308,7 @@ def main() -> int:
 
     params = workload_params()
     pairs, pics = make_frames()
+    batched_streams = None if args.no_batched else batched_frames(args.batched_scale)
     be = Backend(0)
     log(f"device {be.device_name}")
     frames = {k: be.upload(v) for k, v in pics.items()}
@@ -338,6 +405,11 @@ def main() -> int:
     if not args.no_learned:
         learned_out = learned_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step)
 
+    batched = None
+    if batched_streams is not None:
+        batched = batched_leg(be, batched_streams, args.batched_scale, not args.no_check)
+        batched_streams = None
+
     cpu = None
     if not args.no_cpu:
         log("timing the reference CPU path ...")
@@ -368,6 +440,7 @@ def main() -> int:
         "roofline_hbm": roofline_hbm,
         "cpu_baseline": cpu,
         "learned": learned_out,
+        "batched": batched,
     }
     for f in frames.values():
         f.release()
