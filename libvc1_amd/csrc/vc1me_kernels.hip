@@ -1375,7 +1375,9 @@ __device__ __forceinline__ IRes seeded_search(Ctx& C, const Pred pi, int gdx, in
     uint64_t best = global_sad_min(C, vx, vy, 5);
     for (int it = 0; it < 32; ++it) {
         const int cx = key_dx(best), cy = key_dy(best);
-        if (!win_load(C, C.x0 + cx - 1, C.y0 + cy - 1, 18, 18)) { set_err(C, 11); break; }
+        // one 28x28 window serves the descent until it walks out of it
+        if (!(in_window(C, cx - 1, cy - 1) && in_window(C, cx + 1, cy + 1)) &&
+            !win_load(C, C.x0 + cx - 6, C.y0 + cy - 6, 28, 28)) { set_err(C, 11); break; }
         const uint64_t m = int_batch(C, 4, 1, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             dx = clampi(cx + (i == 0 ? -1 : i == 1 ? 1 : 0), b.xmin, b.xmax);
             dy = clampi(cy + (i == 2 ? -1 : i == 3 ? 1 : 0), b.ymin, b.ymax);
