@@ -2649,13 +2649,16 @@ __device__ __forceinline__ bool g_better(uint64_t ac, int ax, int ay, uint64_t b
 
 // The extended search's sorted candidate list (src/encoder_motion.cpp:2036-2041)
 // held in registers while a stage is replayed: consider() keeps the lower
-// cost of a repeated point; a new point is appended, the list sorted by the
-// strict order of g_better (points are distinct, so the order is unique) and
-// truncated to 8.
+// cost of a repeated point (without re-sorting, as the reference does); a new
+// point is appended, the list sorted by the strict order of g_better (points
+// are distinct, so the order is unique) and truncated to 8.  `sorted` tracks
+// whether the list is in that order, so that an append to a sorted list is a
+// single insertion; only a lowered repeat can leave it unsorted.
 struct GTop {
     uint64_t c[9];
     int x[9], y[9];
     int n;
+    bool sorted;
 };
 
 __device__ __forceinline__ void gtop_load(GTop& t, const GlobalState* g) {
@@ -2666,6 +2669,10 @@ __device__ __forceinline__ void gtop_load(GTop& t, const GlobalState* g) {
         t.x[i] = i < t.n ? g->tdx[i] : 0;
         t.y[i] = i < t.n ? g->tdy[i] : 0;
     }
+    t.sorted = true;
+#pragma unroll
+    for (int i = 1; i < 9; ++i)
+        if (i < t.n && g_better(t.c[i], t.x[i], t.y[i], t.c[i - 1], t.x[i - 1], t.y[i - 1])) t.sorted = false;
 }
 
 __device__ __forceinline__ void gtop_store(const GTop& t, GlobalState* g) {
@@ -2681,11 +2688,29 @@ __device__ __forceinline__ void g_consider(GTop& t, uint64_t cost, int dx, int d
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         if (i < t.n && t.x[i] == dx && t.y[i] == dy) {
-            if (cost < t.c[i]) t.c[i] = cost;
+            if (cost < t.c[i]) {
+                t.c[i] = cost;
+                if (i > 0 && g_better(t.c[i], t.x[i], t.y[i], t.c[i - 1], t.x[i - 1], t.y[i - 1])) t.sorted = false;
+            }
             found = true;
         }
     }
     if (found) return;
+    if (t.sorted) {
+        // insertion at the new point's rank, shifting the worse entries up
+        int p = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+            if (i < t.n && g_better(t.c[i], t.x[i], t.y[i], cost, dx, dy)) ++p;
+#pragma unroll
+        for (int i = 8; i >= 1; --i)
+            if (i > p && i <= t.n) { t.c[i] = t.c[i - 1]; t.x[i] = t.x[i - 1]; t.y[i] = t.y[i - 1]; }
+#pragma unroll
+        for (int i = 0; i < 9; ++i)
+            if (i == p) { t.c[i] = cost; t.x[i] = dx; t.y[i] = dy; }
+        t.n = t.n + 1 > 8 ? 8 : t.n + 1;
+        return;
+    }
     // append at n (n <= 8), then sort the n + 1 entries (odd-even
     // transposition: 9 rounds sort any 9 entries)
 #pragma unroll
@@ -2704,6 +2729,7 @@ __device__ __forceinline__ void g_consider(GTop& t, uint64_t cost, int dx, int d
         }
     }
     t.n = n > 8 ? 8 : n;
+    t.sorted = true;
 }
 
 __device__ __forceinline__ void g_set_stage(GlobalState* g, int stride) {
