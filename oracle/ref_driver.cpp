@@ -123,6 +123,38 @@ int ref_analyze_p_core(const vc1me_params* p, const vc1me_planes* cur, const vc1
     }
 }
 
+// analyze_p_picture_core with a shared seed built from `seed` (its 1-MV
+// vectors and distant-match fields, src/encoder_motion.cpp:97-101, :120-130)
+int ref_analyze_p_core_seeded(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                              const vc1me_p_result* seed, vc1me_p_result* out, int simd_auto) {
+    try {
+        Vc1Encoder enc(make_config(p, simd_auto != 0));
+        const Frame f = make_frame(cur), r = make_frame(ref);
+        const size_t mbs = static_cast<size_t>((p->width + 15) / 16) * ((p->height + 15) / 16);
+        Vc1Encoder::PAnalysis s;
+        s.mv_mode = Vc1Encoder::ProgressiveMvMode::OneMvQpel;
+        s.mvs.resize(mbs);
+        s.distant_local_mae.resize(mbs);
+        s.distant_candidate_mae.resize(mbs);
+        s.distant_match_decision.resize(mbs);
+        for (size_t i = 0; i < mbs; ++i) {
+            s.mvs[i].xq = seed->mvs[2 * i];
+            s.mvs[i].yq = seed->mvs[2 * i + 1];
+            s.distant_local_mae[i] = seed->distant_local_mae[i];
+            s.distant_candidate_mae[i] = seed->distant_candidate_mae[i];
+            s.distant_match_decision[i] = seed->distant_match_decision[i];
+        }
+        const auto a = enc.analyze_p_picture_core(
+            f, r, static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode), p->allow_intra != 0, &s, nullptr);
+        copy_result(a, out);
+        out->global_dx = 0; out->global_dy = 0; out->global_sad = 0;
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
 uint64_t ref_mb_sad(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
                     int mx, int my, int dx, int dy, int stride) {
     Vc1Encoder enc(make_config(p, false));

@@ -1305,8 +1305,14 @@ static void init_ctx(ctx_t* c, const vc1me_params* p, const vc1me_planes* cur, c
 
 /* Vc1Encoder::analyze_p_picture_core with shared_seed == nullptr and
  * b_reuse == nullptr, src/encoder_motion.cpp:47-442 (Advanced profile). */
-int oracle_analyze_p_frame(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
-                           vc1me_p_result* out, oracle_ops* ops) {
+/* analyze_p_picture_core with shared_seed == seed (NULL: the full search).
+ * A seed supplies each macroblock's 1-MV start vector (clamped to the search
+ * bounds) for the staged refinement instead of the integer search, and its
+ * distant-match fields are the result's (src/encoder_motion.cpp:97-101,
+ * :120-130); there is no global search, so global_mad is the local MAD
+ * (:432-434). */
+static int analyze_core(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                        const vc1me_p_result* seed, vc1me_p_result* out, oracle_ops* ops) {
     if (p->me_quality != 1 && p->me_quality != 2) return -1;
     ctx_t cx;
     init_ctx(&cx, p, cur, ref, ops);
@@ -1349,8 +1355,16 @@ int oracle_analyze_p_frame(const vc1me_params* p, const vc1me_planes* cur, const
     {
         const int range = p->motion_search_range;
         const int local_range = imin(range, imax(0, p->motion_local_search_range));
-        global = global_search(c, local_range);
-        out->global_dx = global.dx; out->global_dy = global.dy; out->global_sad = global.cost;
+        if (!seed) {
+            global = global_search(c, local_range);
+            out->global_dx = global.dx; out->global_dy = global.dy; out->global_sad = global.cost;
+        } else {
+            for (size_t i = 0; i < mbs; ++i) {
+                out->distant_local_mae[i] = seed->distant_local_mae[i];
+                out->distant_candidate_mae[i] = seed->distant_candidate_mae[i];
+                out->distant_match_decision[i] = seed->distant_match_decision[i];
+            }
+        }
         cell_t* cells = NULL;
         int ncells = 0;
         const double lambda = fmax(1.0, 0.35 * (double)p->pqindex);
@@ -1361,7 +1375,11 @@ int oracle_analyze_p_frame(const vc1me_params* p, const vc1me_planes* cur, const
                 const pred_t rate_pi = predictor_info_mixed_1mv(c, bmv, use_intra, mx, my);
                 uint64_t one_sad = 0;
                 mv_t one = {0, 0};
-                {
+                if (seed) {
+                    const bounds_t qb = bounds_qpel(range);
+                    mv_t sv = {clampi(seed->mvs[2 * pos], qb.xmin, qb.xmax), clampi(seed->mvs[2 * pos + 1], qb.ymin, qb.ymax)};
+                    one = refine_staged(c, x0, y0, 16, 16, sv, range, one_mode, &rate_pi, true, &one_sad, U64MAX);
+                } else {
                     const pred_t pi = predictor_info(c, mvs, mx, my);
                     imr_t im;
                     bool selected_distant = false;
@@ -1551,7 +1569,8 @@ int oracle_analyze_p_frame(const vc1me_params* p, const vc1me_planes* cur, const
             }
         free(cells);
         const double local_mad = total_pixels ? (double)total_sad / (double)total_pixels : 0.0;
-        const double global_mad = (c->W > 0 && c->H > 0) ? (double)global.cost / (double)((uint64_t)c->W * c->H) : 0.0;
+        const double global_mad = seed ? local_mad
+                                       : (c->W > 0 && c->H > 0) ? (double)global.cost / (double)((uint64_t)c->W * c->H) : 0.0;
         out->mean_abs_residual = fmin(local_mad, global_mad);
         const double rate_local = total_pixels ? (double)rate_sad_total / (double)total_pixels : 0.0;
         out->rate_complexity = fmin(rate_local, global_mad);
@@ -1570,6 +1589,18 @@ done:
     }
     free(mvs); free(bmv); free(use_4mv); free(use_intra); free(prop);
     return 0;
+}
+
+int oracle_analyze_p_frame(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                           vc1me_p_result* out, oracle_ops* ops) {
+    return analyze_core(p, cur, ref, NULL, out, ops);
+}
+
+int oracle_analyze_p_frame_seeded(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                                  const vc1me_p_result* seed, vc1me_p_result* out, oracle_ops* ops) {
+    if (!seed || !seed->mvs || !seed->distant_local_mae || !seed->distant_candidate_mae || !seed->distant_match_decision)
+        return -1;
+    return analyze_core(p, cur, ref, seed, out, ops);
 }
 
 /* ---------------------------------------------------------------- batch scorer */

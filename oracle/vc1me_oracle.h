@@ -40,6 +40,13 @@ typedef struct oracle_ops {
 int oracle_analyze_p_frame(const vc1me_params* p, const vc1me_planes* cur,
                            const vc1me_planes* ref, vc1me_p_result* out, oracle_ops* ops);
 
+/* The seeded core: analyze_p_picture_core(cur, ref, mode, allow_intra,
+ * &seed, nullptr) (src/encoder_motion.cpp:97-101, :120-130, :432-434): the
+ * seed's 1-MV vectors start each macroblock's staged refinement and its
+ * distant-match fields are copied.  global_* of the result are zero. */
+int oracle_analyze_p_frame_seeded(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                                  const vc1me_p_result* seed, vc1me_p_result* out, oracle_ops* ops);
+
 /* Batch scorer semantics of VulkanComputeContext::score_motion_batch as the
  * GLSL kernel computes it (src/shaders/vulkan_motion_cost.comp:109-183):
  * clamped fetch, sampled SAD or SATD4x4, no hidden-padding veto. */

@@ -54,6 +54,8 @@ def oracle() -> C.CDLL:
         lib.oracle_mb_costs20.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(C.c_int32), P(C.c_int32),
                                           P(C.c_uint8), P(C.c_uint64)]
         lib.oracle_mb_costs20.restype = None
+        lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
+                                                      P(abi.PResult), P(OracleOps)]
         _oracle = lib
     return _oracle
 
@@ -81,6 +83,8 @@ def reference() -> C.CDLL:
         lib.ref_time_p_frames.argtypes = [P(abi.Params), C.c_int, P(abi.Planes), P(abi.Planes), C.c_int, C.c_int]
         lib.ref_time_p_frames.restype = C.c_double
         lib.ref_last_error.restype = C.c_char_p
+        lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
+                                                  P(abi.PResult), C.c_int]
         _ref = lib
     return _ref
 
@@ -98,6 +102,28 @@ def reference_analyze(params: abi.Params, cur, ref) -> abi.ResultArrays:
     c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
     out = abi.ResultArrays(params.width, params.height)
     rc = reference().ref_analyze_p_core(C.byref(params), C.byref(c.struct), C.byref(r.struct), C.byref(out.struct), 0)
+    assert rc == 0, reference().ref_last_error()
+    return out
+
+
+def oracle_analyze_seeded(params: abi.Params, cur, ref, seed: abi.ResultArrays,
+                          ops: OracleOps | None = None) -> abi.ResultArrays:
+    """The seeded core (analyze_p_picture_core with a shared seed): `seed`'s
+    1-MV vectors start the staged refinement of each macroblock."""
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    out = abi.ResultArrays(params.width, params.height)
+    rc = oracle().oracle_analyze_p_frame_seeded(C.byref(params), C.byref(c.struct), C.byref(r.struct),
+                                                C.byref(seed.struct), C.byref(out.struct),
+                                                C.byref(ops) if ops is not None else None)
+    assert rc == 0
+    return out
+
+
+def reference_analyze_seeded(params: abi.Params, cur, ref, seed: abi.ResultArrays) -> abi.ResultArrays:
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    out = abi.ResultArrays(params.width, params.height)
+    rc = reference().ref_analyze_p_core_seeded(C.byref(params), C.byref(c.struct), C.byref(r.struct),
+                                               C.byref(seed.struct), C.byref(out.struct), 0)
     assert rc == 0, reference().ref_last_error()
     return out
 

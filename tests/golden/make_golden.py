@@ -5,7 +5,9 @@ Runs the real libvc1 reference (compiled from its own sources into
 oracle/_ref/libvc1ref.so by `make -C oracle ref`) on small deterministic
 inputs and stores inputs and every PAnalysis output field as .npz fixtures,
 plus SHA-256 digests of the outputs of larger synthetic cases
-(analyze_digests.json).  The fixtures are data only.
+(analyze_digests.json), and the seeded core's fixtures (seeded/*.npz: the
+inputs, the seed's vectors and distant fields, every output field).  The
+fixtures are data only.
 
     python tests/golden/make_golden.py
 """
@@ -24,7 +26,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from libvc1_amd import abi, synth  # noqa: E402
-from _oracle import reference_analyze  # noqa: E402
+from _oracle import reference_analyze, reference_analyze_seeded  # noqa: E402
 
 
 def formula_pair(w, h, a, b):
@@ -86,6 +88,38 @@ def small_cases():
     return out
 
 
+def seeded_cases():
+    """(name, cur, ref, params of the seeded pass, seed) for the seeded core:
+    seeds from the reference's own qpel pass without intra (as the
+    reference's multi-pass analysis makes them), and one of arbitrary vectors
+    partly outside the search bounds."""
+    out = []
+    d = synth.pan_rectangles(101, 3, width=176, height=144, max_pan_px=8, max_obj_px=24, frame_indices=[0, 2])
+    base = dict(pqindex=8, motion_search_range=128, motion_local_search_range=32)
+    q = abi.default_params(176, 144, mode=abi.MODE_ONE_MV_QPEL, allow_intra=0, **base)
+    seed = reference_analyze(q, d[2], d[0])
+    for tag, mode in (("hpel", abi.MODE_ONE_MV_HPEL), ("bilinear", abi.MODE_ONE_MV_HPEL_BILINEAR),
+                      ("mixed", abi.MODE_MIXED_MV)):
+        out.append((f"seeded_pan_176x144_{tag}", d[2], d[0], dict(base, mode=mode), seed))
+    c, r = hidden_pair()
+    hp = dict(pqindex=8, motion_search_range=6, motion_local_search_range=6)
+    hseed = reference_analyze(abi.default_params(160, 120, mode=abi.MODE_ONE_MV_QPEL, allow_intra=0, **hp), c, r)
+    out.append(("seeded_hidden_160x120_mixed", c, r, dict(hp, mode=abi.MODE_MIXED_MV, allow_intra=0), hseed))
+    wild = abi.ResultArrays(176, 144)
+    rng = np.random.default_rng(9)
+    wild.mvs[...] = rng.integers(-700, 701, size=wild.mvs.shape)
+    wild.distant_local_mae[...] = rng.uniform(0, 40, size=wild.mbs)
+    wild.distant_candidate_mae[...] = -1.0
+    wild.distant_match_decision[...] = rng.integers(0, 4, size=wild.mbs)
+    out.append(("seeded_wild_176x144_mixed", d[2], d[0], dict(pqindex=8, motion_search_range=64,
+                                                              motion_local_search_range=32, mode=abi.MODE_MIXED_MV),
+                wild))
+    return out
+
+
+SEED_ARRAYS = ("mvs", "distant_local_mae", "distant_candidate_mae", "distant_match_decision")
+
+
 def digest(res: abi.ResultArrays) -> str:
     h = hashlib.sha256()
     for k in abi.ResultArrays.ARRAYS:
@@ -125,6 +159,17 @@ def main() -> int:
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), cur_y=c.y, cur_u=c.u, cur_v=c.v, ref_y=r.y,
                             ref_u=r.u, ref_v=r.v, params=json.dumps(over), scalars=json.dumps(scal), **arrays)
         print(f"wrote {name}: 4mv={int(res.use_4mv.sum())} intra={int(res.use_intra.sum())}")
+    os.makedirs(os.path.join(HERE, "seeded"), exist_ok=True)
+    for name, c, r, over, seed in seeded_cases():
+        h, w = c.y.shape
+        p = abi.default_params(w, h, **over)
+        res = reference_analyze_seeded(p, c, r, seed)
+        arrays = {f"out_{k}": getattr(res, k) for k in abi.ResultArrays.ARRAYS}
+        arrays.update({f"seed_{k}": getattr(seed, k) for k in SEED_ARRAYS})
+        np.savez_compressed(os.path.join(HERE, "seeded", f"{name}.npz"), cur_y=c.y, cur_u=c.u, cur_v=c.v, ref_y=r.y,
+                            ref_u=r.u, ref_v=r.v, params=json.dumps(over), scalars=json.dumps(res.scalars()),
+                            **arrays)
+        print(f"wrote seeded/{name}: 4mv={int(res.use_4mv.sum())} intra={int(res.use_intra.sum())}")
     digests = []
     for name, kind, a, over in DIGEST_CASES:
         c, r, w, h = digest_inputs(kind, a)
