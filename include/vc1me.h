@@ -204,6 +204,22 @@ typedef struct vc1me_job {
 } vc1me_job;
 int vc1me_analyze_step(vc1me_ctx* ctx, const vc1me_job* jobs, int n);
 
+/* ---- seeded core: Vc1Encoder::analyze_p_picture_core with a shared seed
+ * (src/encoder_motion.cpp:47-442; the seeded branch at :97-101 and :120-130,
+ * the global MAD at :432-434).  Each macroblock's staged quarter-pel
+ * refinement starts from the seed's 1-MV vector (seed->mvs, clamped to the
+ * search bounds) instead of the global and integer search; the seed's
+ * distant_local_mae, distant_candidate_mae and distant_match_decision are
+ * copied into the result; global_dx/dy/sad are 0 and the global MAD is the
+ * local one.  4-MV, intra and the summary run as in vc1me_analyze_p_frame.
+ * The seed is normally the 1-MV quarter-pel result of the same pair (the
+ * reference's shared ME pass of an I/P/B group); its arrays must all be set
+ * (VC1ME_EINVAL otherwise).  A motion_search_range of 0 ignores the seed. */
+int vc1me_analyze_p_frame_seeded(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                 const vc1me_params* params, const vc1me_p_result* seed, vc1me_p_result* out);
+/* n seeded pairs in one step: seeds[i] seeds jobs[i]. */
+int vc1me_analyze_step_seeded(vc1me_ctx* ctx, const vc1me_job* jobs, const vc1me_p_result* const* seeds, int n);
+
 /* Timing of the last vc1me_analyze_step / vc1me_analyze_p_frame: device
  * milliseconds between HIP events recorded on the context stream around the
  * whole launch sequence, and around every launch of the macroblock search

@@ -25,7 +25,8 @@ EXPORTS = (
     "vc1me_block_costs", "vc1me_nn_param_count", "vc1me_nn_create", "vc1me_nn_destroy", "vc1me_nn_set_params",
     "vc1me_nn_get_params", "vc1me_nn_forward_frames", "vc1me_nn_forward_backward", "vc1me_nn_train_step",
     "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
-    "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes",
+    "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
+    "vc1me_analyze_step_seeded",
 )
 
 
@@ -66,6 +67,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                              P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
     lib.vc1me_analyze_p_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult)]
     lib.vc1me_analyze_step.argtypes = [C.c_void_p, P(abi.Job), C.c_int]
+    lib.vc1me_analyze_p_frame_seeded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult),
+                                                 P(abi.PResult)]
+    lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
     lib.vc1me_set_profiling.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_last_work.argtypes = [C.c_void_p, P(abi.Work)]
@@ -168,6 +172,22 @@ class Backend:
     def analyze_step(self, pairs, params_list, want_results: bool = True):
         jobs, results = self._jobs(pairs, params_list, want_results)
         self._check(self.lib.vc1me_analyze_step(self.ctx, jobs, len(pairs)))
+        return results
+
+    def analyze_seeded(self, cur: Frame, ref: Frame, params: abi.Params, seed: abi.ResultArrays) -> abi.ResultArrays:
+        """The seeded core (vc1me_analyze_p_frame_seeded): `seed` is a
+        result of the same raster, normally the 1-MV quarter-pel pass."""
+        r = abi.ResultArrays(params.width, params.height)
+        self._check(self.lib.vc1me_analyze_p_frame_seeded(self.ctx, cur.handle, ref.handle, C.byref(params),
+                                                          C.byref(seed.struct), C.byref(r.struct)))
+        return r
+
+    def analyze_step_seeded(self, pairs, params_list, seeds, want_results: bool = True):
+        """analyze_step of the seeded core, seeds[i] seeding pair i."""
+        jobs, results = self._jobs(pairs, params_list, want_results)
+        n = len(pairs)
+        arr = (C.POINTER(abi.PResult) * n)(*[C.pointer(s.struct) for s in seeds])
+        self._check(self.lib.vc1me_analyze_step_seeded(self.ctx, jobs, arr, n))
         return results
 
     def analyze_step_learned(self, net: "ProposalNet", pairs, params_list, threshold: float = 0.5,

@@ -116,6 +116,7 @@ struct JobSlot {
     int32_t* wl0 = nullptr;         // relaxation worklists and queue stamps
     int32_t* wl1 = nullptr;
     uint32_t* mark = nullptr;
+    int2* seed_mvs = nullptr;       // seeded core: the seed's 1-MV vectors
     int mbs_last = 0;               // macroblocks of the last step's job
 };
 
@@ -153,6 +154,7 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_wl0 = carve(mbs * sizeof(int32_t));
     const size_t o_wl1 = carve(mbs * sizeof(int32_t));
     const size_t o_mark = carve(mbs * sizeof(uint32_t));
+    const size_t o_seed = carve(mbs * sizeof(int2));
     hipError_t e = hipMalloc(&s.mem, off);
     if (e != hipSuccess) { s.mem = nullptr; return e; }
     uint8_t* b = static_cast<uint8_t*>(s.mem);
@@ -182,6 +184,7 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.wl0 = reinterpret_cast<int32_t*>(b + o_wl0);
     s.wl1 = reinterpret_cast<int32_t*>(b + o_wl1);
     s.mark = reinterpret_cast<uint32_t*>(b + o_mark);
+    s.seed_mvs = reinterpret_cast<int2*>(b + o_seed);
     s.mbs_cap = mbs;
     s.cells_cap = cells;
     return hipSuccess;
@@ -403,6 +406,7 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.mbops = s.mbops;
     d.wl0 = s.wl0; d.wl1 = s.wl1; d.mark = s.mark;
     d.nn_out = nullptr; d.nn_logit = 0.f;
+    d.seed_mvs = nullptr;
     d.cost20 = nullptr; d.one_mv = nullptr;
     d.lam20 = std::max(20, 7 * p->pqindex);
     d.max_diag = d.mbw + 2 * d.mbh - 3;
@@ -436,7 +440,10 @@ int mbs_on_diag(int mbw, int mbh, int t) {
 
 // Copy one job's per-MB arrays back and form the frame scalars
 // (src/encoder_motion.cpp:416-441).
-int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_result* out) {
+// `seed`: the seeded core's seed (its distant-match fields are the result's,
+// src/encoder_motion.cpp:97-101), or null
+int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_result* out,
+                 const vc1me_p_result* seed = nullptr) {
     const size_t mbs = static_cast<size_t>(d.mbw) * d.mbh;
     std::vector<int2> mvs(mbs), bmv(mbs * 4);
     std::vector<uint8_t> flags(mbs), dec(mbs);
@@ -484,9 +491,12 @@ int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_res
         if (out->use_intra) out->use_intra[i] = (flags[i] & vc1me::kFlagIntra) ? 1 : 0;
         if (out->screen_sads) out->screen_sads[i] = screen[i];
         if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = ratio[i];
-        if (out->distant_local_mae) out->distant_local_mae[i] = dl[i];
-        if (out->distant_candidate_mae) out->distant_candidate_mae[i] = dc[i];
-        if (out->distant_match_decision) out->distant_match_decision[i] = dec[i];
+        const bool from_seed = seed && d.range != 0;
+        if (out->distant_local_mae) out->distant_local_mae[i] = from_seed ? seed->distant_local_mae[i] : dl[i];
+        if (out->distant_candidate_mae)
+            out->distant_candidate_mae[i] = from_seed ? seed->distant_candidate_mae[i] : dc[i];
+        if (out->distant_match_decision)
+            out->distant_match_decision[i] = from_seed ? seed->distant_match_decision[i] : dec[i];
     }
     if (d.range == 0) {
         out->mean_abs_residual = total_px ? static_cast<double>(total_sad) / static_cast<double>(total_px) : 0.0;
@@ -497,7 +507,10 @@ int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_res
         return VC1ME_OK;
     }
     const double local_mad = total_px ? static_cast<double>(total_sad) / static_cast<double>(total_px) : 0.0;
-    const double global_mad = static_cast<double>(g.full_sad) / static_cast<double>(static_cast<uint64_t>(d.W) * d.H);
+    // the seeded core has no global search: its global MAD is the local one
+    // (src/encoder_motion.cpp:432-434)
+    const double global_mad = seed ? local_mad
+                                   : static_cast<double>(g.full_sad) / static_cast<double>(static_cast<uint64_t>(d.W) * d.H);
     out->mean_abs_residual = std::min(local_mad, global_mad);
     const double rate_local = total_px ? static_cast<double>(rate_total) / static_cast<double>(total_px) : 0.0;
     out->rate_complexity = std::min(rate_local, global_mad);
@@ -506,16 +519,17 @@ int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_res
     out->fractional_chroma_macroblocks = frac;
     out->four_mv_macroblocks = four;
     out->intra_macroblocks = intra;
-    out->global_dx = g.dx;
-    out->global_dy = g.dy;
-    out->global_sad = g.full_sad;
+    out->global_dx = seed ? 0 : g.dx;
+    out->global_dy = seed ? 0 : g.dy;
+    out->global_sad = seed ? 0 : g.full_sad;
     return VC1ME_OK;
 }
 
 // after_global, if set, is called once the global search is queued (the
 // learned form runs its network there, on the global vectors in the slots).
 int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* nn_out = nullptr, float nn_logit = 0.f,
-             const std::function<int(hipStream_t)>& after_global = {}) {
+             const std::function<int(hipStream_t)>& after_global = {},
+             const vc1me_p_result* const* seeds = nullptr) {
     if (n <= 0) return VC1ME_OK;
     if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
     if (n > 255) return fail(VC1ME_ELIMIT, "at most 255 jobs per step");
@@ -546,7 +560,15 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         ctx->slots[i].mbs_last = static_cast<int>(mbs);
         max_local = std::max(max_local, ctx->h_jobs[i].local_range);
         max_diag = std::max(max_diag, ctx->h_jobs[i].max_diag);
-        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range) max_cells = std::max(max_cells, cells);
+        if (seeds) {
+            const vc1me_p_result* sd = seeds[i];
+            if (!sd || !sd->mvs || !sd->distant_local_mae || !sd->distant_candidate_mae || !sd->distant_match_decision)
+                return fail(VC1ME_EINVAL, "seed " + std::to_string(i) + " lacks its vectors or distant-match fields");
+            HIP_TRY(hipMemcpyAsync(ctx->slots[i].seed_mvs, sd->mvs, mbs * sizeof(int2), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            ctx->h_jobs[i].seed_mvs = ctx->slots[i].seed_mvs;
+        }
+        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range && !seeds) max_cells = std::max(max_cells, cells);
     }
     if (ctx->d_jobs_cap < static_cast<size_t>(n)) {
         if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
@@ -717,7 +739,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         for (int i = 0; i < n; ++i) {
             const JobDesc& d = ctx->h_jobs[i];
             put(d.W); put(d.H); put(d.range); put(d.local_range); put(d.mbw); put(d.mbh); put(d.max_diag);
-            put(d.mode); put(d.phase_owner);
+            put(d.mode); put(d.phase_owner); put(d.seed_mvs ? 1 : 0);
         }
         hipGraphExec_t exec = nullptr;
         for (auto& g : ctx->graphs)
@@ -791,7 +813,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         ctx->counters.frames_analyzed++;
         ctx->counters.macroblocks_analyzed += static_cast<uint64_t>(ctx->h_jobs[i].mbw) * ctx->h_jobs[i].mbh;
         if (jobs[i].out) {
-            const int rc = fetch_result(ctx, ctx->h_jobs[i], ctx->slots[i], jobs[i].out);
+            const int rc = fetch_result(ctx, ctx->h_jobs[i], ctx->slots[i], jobs[i].out, seeds ? seeds[i] : nullptr);
             if (rc) return rc;
         }
     }
@@ -1147,6 +1169,23 @@ int vc1me_analyze_p_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_fr
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (int rc = set_device(ctx)) return rc;
     return run_jobs(ctx, &j, 1);
+}
+
+int vc1me_analyze_step_seeded(vc1me_ctx* ctx, const vc1me_job* jobs, const vc1me_p_result* const* seeds, int n) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (n > 0 && !seeds) return fail(VC1ME_EINVAL, "null seeds");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return run_jobs(ctx, jobs, n, nullptr, 0.f, {}, seeds);
+}
+
+int vc1me_analyze_p_frame_seeded(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                 const vc1me_params* params, const vc1me_p_result* seed, vc1me_p_result* out) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    vc1me_job j{cur, ref, params, out};
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return run_jobs(ctx, &j, 1, nullptr, 0.f, {}, &seed);
 }
 
 int vc1me_analyze_step(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {

@@ -88,6 +88,9 @@ struct JobDesc {
     uint32_t* mark;
     int32_t* wl_n;
     int32_t wl_stride;
+    // seeded core (null: the full search): each macroblock's 1-MV start
+    // vector, quarter-pel (analyze_p_picture_core's shared seed)
+    const int2* seed_mvs;
 };
 
 // Longest relaxation run before the step falls back to the anti-diagonal

@@ -2118,7 +2118,19 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     const Pred rate_pi = predictor_mixed_1mv(C);
     bool was_seeded = false;
     uint8_t prop_out = 0;
-    {
+    if (J->seed_mvs) {
+        // the seeded core (src/encoder_motion.cpp:120-130): the seed, clamped
+        // to the search bounds, starts the staged refinement; no integer or
+        // long-range search, nothing propagated
+        const Bounds qb = bounds_qpel(range);
+        const int2 sv = J->seed_mvs[pos];
+        const int sx = clampi(sv.x, qb.xmin, qb.xmax), sy = clampi(sv.y, qb.ymin, qb.ymax);
+        if (C.lane == 0) J->prop[pos] = 0;
+        if (J->subpel) pw_load(C, C.x0 + floor_div4(sx) - kPwMargin, C.y0 + floor_div4(sy) - kPwMargin,
+                               mode_bilinear(one_mode));
+        SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true};
+        one = refine_staged(C, sp, sx, sy, rate_pi, &one_sad, UINT64_MAX);
+    } else {
         const Pred pi = predictor_info(C);
         const int gdx = J->g->dx, gdy = J->g->dy;
         IRes im;
@@ -2410,7 +2422,7 @@ __device__ __forceinline__ NbState run_mb(const JobDesc* J, int mx, int my, uint
 __global__ void __launch_bounds__(64) k_extended(const JobDesc* __restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t smem[];
     const JobDesc* J = &jobs[blockIdx.y];
-    if (J->range <= J->local_range || J->nn_out) return;
+    if (J->range <= J->local_range || J->nn_out || J->seed_mvs) return;
     const int pos = static_cast<int>(blockIdx.x);
     if (pos >= J->mbw * J->mbh) return;
     Ctx C;
@@ -2524,7 +2536,8 @@ __global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ 
     const JobDesc* J = &jobs[blockIdx.y];
     const int pos = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (pos >= J->mbw * J->mbh) return;
-    const int2 g = J->range == 0 ? make_int2(0, 0) : make_int2(J->g->dx * 4, J->g->dy * 4);
+    const int2 g = J->range == 0 ? make_int2(0, 0)
+                   : J->seed_mvs ? J->seed_mvs[pos] : make_int2(J->g->dx * 4, J->g->dy * 4);
     J->mvs[pos] = g;
 #pragma unroll
     for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = g;
@@ -2749,7 +2762,7 @@ __global__ void k_global_begin(const JobDesc* __restrict__ jobs, int njobs) {
     g->tn = 0;
     g->n = 0;
     g->dx = 0; g->dy = 0; g->full_sad = 0;
-    if (J->range == 0) return;
+    if (J->range == 0 || J->seed_mvs) return;   // the seeded core has no global search
     const int R = g_range(J);
     if (R <= 0) {
         g->pts[0] = 0; g->pts[1] = 0; g->n = 1;
@@ -3051,7 +3064,7 @@ hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc*
     int pts[4] = {0, 0, 0, 0}, chunks[4] = {0, 0, 0, 0};
     for (int i = 0; i < njobs; ++i) {
         const JobDesc& J = h_jobs[i];
-        if (J.range == 0) continue;
+        if (J.range == 0 || J.seed_mvs) continue;
         const int R = J.local_range < 32 ? J.local_range : 32;
         int np[4] = {0, 0, 0, 0}, st[4] = {1, 1, 1, 1};
         if (R <= 0) {

@@ -1,0 +1,122 @@
+"""The seeded core on the GPU (vc1me_analyze_p_frame_seeded /
+vc1me_analyze_step_seeded, analyze_p_picture_core with a shared seed,
+src/encoder_motion.cpp:97-101, :120-130, :432-434): bit-exact against the
+fixtures written by the compiled reference and against the oracle, under both
+macroblock schedules, in a batched step and under graph replay."""
+import pytest
+
+from libvc1_amd import abi, synth
+from libvc1_amd.backend import BackendError
+from _oracle import describe_diff, oracle_analyze_seeded
+from test_oracle_seeded import SEEDED_FILES, load_seeded
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _seeded(backend, cur, ref, p, seed):
+    fc, fr = backend.upload(cur), backend.upload(ref)
+    try:
+        return backend.analyze_seeded(fc, fr, p, seed)
+    finally:
+        fc.release()
+        fr.release()
+
+
+@pytest.mark.parametrize("schedule", [abi.SCHEDULE_RELAX, abi.SCHEDULE_DIAGONAL], ids=["relax", "diagonal"])
+@pytest.mark.parametrize("path", SEEDED_FILES, ids=lambda p: p.rsplit("/", 1)[-1][:-4])
+def test_seeded_matches_reference_fixture(backend, path, schedule):
+    cur, ref, p, seed, want = load_seeded(path)
+    _say(f"[case] seeded fixture {path.rsplit('/', 1)[-1]} {p.width}x{p.height} schedule={schedule}")
+    backend.set_schedule(schedule)
+    try:
+        got = _seeded(backend, cur, ref, p, seed)
+    finally:
+        backend.set_schedule(abi.SCHEDULE_RELAX)
+    assert not got.diff(want), describe_diff(got, want, p.width)
+
+
+def _qpel_seed(backend, cur, ref, w, h, **kw):
+    q = abi.default_params(w, h, mode=abi.MODE_ONE_MV_QPEL, allow_intra=0, **kw)
+    fc, fr = backend.upload(cur), backend.upload(ref)
+    try:
+        return backend.analyze(fc, fr, q)
+    finally:
+        fc.release()
+        fr.release()
+
+
+def test_seeded_from_gpu_seed_matches_oracle(backend):
+    """The I/P/B group's shared pass: a 1-MV quarter-pel GPU result seeds a
+    mixed-MV pass with intra; the oracle seeded by the same result agrees."""
+    d = synth.pan_rectangles(61, 4, width=480, height=272, max_pan_px=40, max_obj_px=60, frame_indices=[0, 3])
+    base = dict(pqindex=10, motion_search_range=256, motion_local_search_range=32)
+    seed = _qpel_seed(backend, d[3], d[0], 480, 272, **base)
+    p = abi.default_params(480, 272, mode=abi.MODE_MIXED_MV, allow_intra=1, **base)
+    _say("[case] seeded 480x272 mixed-MV from a GPU seed")
+    got = _seeded(backend, d[3], d[0], p, seed)
+    want = oracle_analyze_seeded(p, d[3], d[0], seed)
+    assert not got.diff(want), describe_diff(got, want, 480)
+    s = got.struct
+    assert (s.global_dx, s.global_dy, s.global_sad) == (0, 0, 0)
+
+
+def test_seeded_step_mixed_rasters_and_graphs(backend):
+    """Three seeded pairs of different rasters and modes in one step, run
+    eagerly, as a captured graph and as its replay: every result equals the
+    oracle's."""
+    pairs, params, seeds, wants = [], [], [], []
+    cases = [(176, 144, abi.MODE_ONE_MV_HPEL, 64), (320, 176, abi.MODE_MIXED_MV, 512),
+             (256, 144, abi.MODE_ONE_MV_HPEL_BILINEAR, 128)]
+    for k, (w, h, mode, rng) in enumerate(cases):
+        d = synth.pan_rectangles(70 + k, 3, width=w, height=h, max_pan_px=20, max_obj_px=30, frame_indices=[0, 2])
+        base = dict(pqindex=8 + 2 * k, motion_search_range=rng, motion_local_search_range=32)
+        seed = _qpel_seed(backend, d[2], d[0], w, h, **base)
+        p = abi.default_params(w, h, mode=mode, allow_intra=k % 2, **base)
+        pairs.append((d[2], d[0]))
+        params.append(p)
+        seeds.append(seed)
+        wants.append(oracle_analyze_seeded(p, d[2], d[0], seed))
+    frames = [(backend.upload(c), backend.upload(r)) for c, r in pairs]
+    try:
+        for label, graphs, schedule in (("relax", False, abi.SCHEDULE_RELAX),
+                                        ("diagonal", False, abi.SCHEDULE_DIAGONAL),
+                                        ("graph capture", True, abi.SCHEDULE_DIAGONAL),
+                                        ("graph replay", True, abi.SCHEDULE_DIAGONAL)):
+            _say(f"[case] seeded step of 3 rasters: {label}")
+            backend.set_schedule(schedule)
+            backend.set_graphs(graphs)
+            got = backend.analyze_step_seeded(frames, params, seeds)
+            for g, want, p in zip(got, wants, params):
+                assert not g.diff(want), f"{label}\n" + describe_diff(g, want, p.width)
+    finally:
+        backend.set_graphs(False)
+        backend.set_schedule(abi.SCHEDULE_RELAX)
+        for a, b in frames:
+            a.release()
+            b.release()
+
+
+def test_seeded_zero_range_ignores_seed(backend):
+    d = synth.pan_rectangles(80, 3, width=176, height=144, frame_indices=[0, 1])
+    p = abi.default_params(176, 144, pqindex=8, motion_search_range=0, motion_local_search_range=32)
+    seed = abi.ResultArrays(176, 144)
+    seed.mvs[...] = 40
+    _say("[case] seeded zero range")
+    got = _seeded(backend, d[1], d[0], p, seed)
+    want = oracle_analyze_seeded(p, d[1], d[0], seed)
+    assert not got.diff(want), describe_diff(got, want, 176)
+    assert not got.mvs.any()
+
+
+def test_seeded_rejects_incomplete_seed(backend):
+    d = synth.pan_rectangles(81, 3, width=176, height=144, frame_indices=[0, 1])
+    p = abi.default_params(176, 144, pqindex=8, motion_search_range=64, motion_local_search_range=32)
+    seed = abi.ResultArrays(176, 144)
+    seed.struct.distant_match_decision = None
+    with pytest.raises(BackendError) as e:
+        _seeded(backend, d[1], d[0], p, seed)
+    assert e.value.code == abi.VC1ME_EINVAL
