@@ -27,6 +27,12 @@ that got worse and the mean excess.  Two ablations take the network's
 contribution away (every macroblock seeded from the global vector alone; the
 network's confidence gate kept but its candidates removed).
 
+The seeded leg (`seeded_core`) runs the seeded mode passes of the
+reference's P-picture analysis on the same pairs: the unseeded 1-MV
+quarter-pel seed pass, the three seeded mode candidates as one 48-job
+vc1me_analyze_step_seeded (its MB/s is `value`), and the final seeded
+Mixed-MV pass with intra.
+
 The batched leg (`batched`) runs the 64-stream workload of uneven rasters
 and lengths (lengths scaled to keep the run short) through the batching
 runner with the tuned table; it is reported beside the headline, not as it.
@@ -38,7 +44,7 @@ multithreaded CPU path timed on this host on a bounded sample of the same
 frames.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-check] [--no-cpu] [--no-learned] [--no-compare]
-                    [--no-batched] [--batched-scale S]
+                    [--no-seeded] [--no-batched] [--batched-scale S]
     python bench.py --copy-calibration     # 256 MiB device copy, for HBM counter scaling
 
 `roofline.traffic` (HBM bytes per launch of the macroblock kernel) comes from
@@ -225,6 +231,89 @@ def learned_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, thres
     }
 
 
+def seeded_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, check_pair=None):
+    """The seeded mode passes of the reference's P-picture analysis
+    (Vc1Encoder::analyze_p_picture, src/encoder_motion.cpp:524-571) on the
+    same 16 pairs: the unseeded 1-MV quarter-pel pass without intra (the
+    shared seed), the three seeded mode candidates (1-MV half-pel, 1-MV
+    half-pel bilinear, Mixed-MV; no intra) as one 48-job step, and the final
+    seeded pass of the chosen mode with intra (Mixed-MV here: the choice is
+    the sampled transform RD, which is not part of this backend).  Reported:
+    the seeded steps' MB/s, the whole multi-pass time per P frame, and the
+    final pass's SAD + lambda*R against the single plain pass."""
+    import numpy as np
+    from libvc1_amd import abi
+
+    base = plist[0]
+    kw = dict(pqindex=base.pqindex, motion_search_range=base.motion_search_range,
+              motion_local_search_range=base.motion_local_search_range, me_quality=base.me_quality)
+    qpel = abi.default_params(base.width, base.height, mode=abi.MODE_ONE_MV_QPEL, allow_intra=0, **kw)
+    cand_modes = (abi.MODE_ONE_MV_HPEL, abi.MODE_ONE_MV_HPEL_BILINEAR, abi.MODE_MIXED_MV)
+    cand_params = [abi.default_params(base.width, base.height, mode=m, allow_intra=0, **kw) for m in cand_modes]
+    final = abi.default_params(base.width, base.height, mode=abi.MODE_MIXED_MV, allow_intra=1, **kw)
+    n = len(jobs)
+    cjobs = [j for _ in cand_modes for j in jobs]
+    cplist = [p for p in cand_params for _ in jobs]
+
+    def seed_step():
+        return be.analyze_step(jobs, [qpel] * n)
+
+    def cand_step(seeds):
+        return be.analyze_step_seeded(cjobs, cplist, seeds * len(cand_modes))
+
+    def final_step(cands, want=False):
+        return be.analyze_step_seeded(jobs, [final] * n, cands[2 * n:3 * n], want_results=want)
+
+    seeds = seed_step()
+    cands = cand_step(seeds)
+    for _ in range(warmup):
+        final_step(cands)
+    t_seed = t_cand = t_final = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        seeds = seed_step()
+        t1 = time.perf_counter()
+        cands = cand_step(seeds)
+        t2 = time.perf_counter()
+        final_step(cands)
+        t3 = time.perf_counter()
+        t_seed += t1 - t0
+        t_cand += t2 - t1
+        t_final += t3 - t2
+    cand_value = len(cand_modes) * mbs_per_step * steps / t_cand
+    final_value = mbs_per_step * steps / t_final
+    per_frame_ms = (t_seed + t_cand + t_final) * 1e3 / (steps * n)
+    log(f"seeded mode passes: seed pass {t_seed * 1e3 / steps:.1f} ms, 3 candidates {t_cand * 1e3 / steps:.1f} ms "
+        f"({cand_value:,.0f} MB/s), final {t_final * 1e3 / steps:.1f} ms ({final_value:,.0f} MB/s); "
+        f"{per_frame_ms:.2f} ms per P frame")
+    mbw, mbh = abi.mb_grid(base.width, base.height)
+    be.set_cost_report(True)
+    try:
+        be.analyze_step(jobs, plist, want_results=False)
+        cp = np.concatenate([be.last_mb_costs(j, mbw * mbh)[0] for j in range(n)]).astype(np.float64)
+        res = final_step(cands, want=True)
+        cs = np.concatenate([be.last_mb_costs(j, mbw * mbh)[0] for j in range(n)]).astype(np.float64)
+    finally:
+        be.set_cost_report(False)
+    parity = None
+    if check_pair is not None:
+        from _oracle import oracle_analyze_seeded
+        cur, ref = check_pair
+        t1 = time.time()
+        bad = cands[2 * n].diff(oracle_analyze_seeded(cand_params[2], cur, ref, seeds[0]))
+        bad += res[0].diff(oracle_analyze_seeded(final, cur, ref, cands[2 * n]))
+        parity = "bit-exact" if not bad else f"MISMATCH in {bad}"
+        log(f"seeded passes of pair 0 vs oracle (Mixed-MV candidate, final): {parity} ({time.time() - t1:.1f}s)")
+    return {"value": round(cand_value, 1), "unit": "MB/s",
+            "value_final_pass": round(final_value, 1), "speedup_vs_plain": round(cand_value / plain_value, 4),
+            "ms_seed_pass": round(t_seed * 1e3 / steps, 3), "ms_candidate_step": round(t_cand * 1e3 / steps, 3),
+            "ms_final_pass": round(t_final * 1e3 / steps, 3), "ms_per_p_frame_all_passes": round(per_frame_ms, 3),
+            "final_cost_vs_plain": round(float(cs.sum() / cp.sum() - 1), 6),
+            "passes": "1-MV qpel seed (unseeded, no intra); 48-job step of seeded Hpel / HpelBilinear / Mixed-MV "
+                      "candidates (no intra); final seeded Mixed-MV with intra",
+            "parity": parity}
+
+
 BATCHED_STREAMS, BATCHED_SEED = 64, 5
 
 
@@ -294,6 +383,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the diagonal-sweep comparison steps")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-stream batched leg")
+    ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-core leg")
     ap.add_argument("--batched-scale", type=float, default=0.1,
                     help="stream lengths (8..120 frames) are scaled by this in the batched leg")
     args = ap.parse_args()
@@ -401,6 +491,11 @@ def main() -> int:
                     "traffic": measured_traffic(mb_kernel), "algorithmic_bytes_per_launch": round(alg_bytes_launch),
                     "kernel": mb_kernel}
 
+    seeded = None
+    if not args.no_seeded:
+        seeded = seeded_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step,
+                            None if args.no_check else (pics[pairs[0][0]], pics[pairs[0][1]]))
+
     learned_out = None
     if not args.no_learned:
         learned_out = learned_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step)
@@ -440,6 +535,7 @@ def main() -> int:
         "roofline_hbm": roofline_hbm,
         "cpu_baseline": cpu,
         "learned": learned_out,
+        "seeded_core": seeded,
         "batched": batched,
     }
     for f in frames.values():
