@@ -220,6 +220,37 @@ int vc1me_analyze_p_frame_seeded(vc1me_ctx* ctx, const vc1me_frame* cur, const v
 /* n seeded pairs in one step: seeds[i] seeds jobs[i]. */
 int vc1me_analyze_step_seeded(vc1me_ctx* ctx, const vc1me_job* jobs, const vc1me_p_result* const* seeds, int n);
 
+/* ---- intensity (fade) compensation of the P analysis
+ * (Vc1Encoder::IntensityComp, src/encoder_internal.h:341-345): LUMSCALE and
+ * LUMSHIFT as coded (6 bits each); enabled = 0 means "no compensation". */
+typedef struct vc1me_intensity {
+    int32_t enabled;
+    int32_t lumscale;
+    int32_t lumshift;
+} vc1me_intensity;
+
+/* Vc1Encoder::estimate_intensity_comp(cur, ref, mvs)
+ * (src/encoder_motion_compensation.cpp:296-348): least-squares fit of the
+ * current luma against the reference luma displaced by each macroblock's
+ * 1-MV vector rounded to whole pixels, sampled every 4th pixel in x and y,
+ * then the (LUMSCALE, LUMSHIFT) of smallest sampled SAD within +-3 of the
+ * fit.  mvs: mbw * mbh quarter-pel (x, y) pairs of cur's raster (the
+ * PAnalysis::mvs of the chosen pass).  cur and ref must share a raster. */
+int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                  const int32_t* mvs, vc1me_intensity* out);
+
+/* Vc1Encoder::intensity_compensated_reference(ref, ic)
+ * (src/encoder_motion_compensation.cpp:195-294, the maps intensity_map_y /
+ * intensity_map_uv at :175-193): a new resident frame whose luma and chroma
+ * are ref's under the compensation (a plain copy when ic->enabled is 0).
+ * Release it with vc1me_frame_release. */
+int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_intensity* ic,
+                                      vc1me_frame** out);
+
+/* Copies a resident frame's planes back to the host (width x height luma,
+ * two width/2 x height/2 chroma planes).  No reference counterpart. */
+int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);
+
 /* Timing of the last vc1me_analyze_step / vc1me_analyze_p_frame: device
  * milliseconds between HIP events recorded on the context stream around the
  * whole launch sequence, and around every launch of the macroblock search

@@ -97,6 +97,14 @@ class Work(C.Structure):
                 ("sig_ops", C.c_uint64), ("frame_px", C.c_uint64), ("seeded_macroblocks", C.c_uint64)]
 
 
+class Intensity(C.Structure):
+    """vc1me_intensity: LUMSCALE / LUMSHIFT as coded, enabled 0 = none."""
+    _fields_ = [("enabled", C.c_int32), ("lumscale", C.c_int32), ("lumshift", C.c_int32)]
+
+    def as_tuple(self):
+        return (int(self.enabled), int(self.lumscale), int(self.lumshift))
+
+
 class Job(C.Structure):
     _fields_ = [("cur", C.c_void_p), ("ref", C.c_void_p), ("params", C.POINTER(Params)),
                 ("out", C.POINTER(PResult))]

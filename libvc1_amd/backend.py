@@ -26,7 +26,8 @@ EXPORTS = (
     "vc1me_nn_get_params", "vc1me_nn_forward_frames", "vc1me_nn_forward_backward", "vc1me_nn_train_step",
     "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
     "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
-    "vc1me_analyze_step_seeded",
+    "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
+    "vc1me_frame_download",
 )
 
 
@@ -69,6 +70,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_analyze_step.argtypes = [C.c_void_p, P(abi.Job), C.c_int]
     lib.vc1me_analyze_p_frame_seeded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult),
                                                  P(abi.PResult)]
+    lib.vc1me_estimate_intensity_comp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(C.c_int32), P(abi.Intensity)]
+    lib.vc1me_frame_intensity_compensated.argtypes = [C.c_void_p, C.c_void_p, P(abi.Intensity), P(C.c_void_p)]
+    lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
     lib.vc1me_set_profiling.argtypes = [C.c_void_p, C.c_int]
@@ -146,6 +150,34 @@ class Backend:
         h = C.c_void_p()
         self._check(self.lib.vc1me_frame_upload(self.ctx, C.byref(ps.struct), C.byref(h)))
         return Frame(self, h.value, ps.struct.width, ps.struct.height)
+
+    def download(self, frame: Frame):
+        """(y, u, v) planes of a resident frame (vc1me_frame_download)."""
+        y = np.empty((frame.height, frame.width), np.uint8)
+        u = np.empty((frame.height // 2, frame.width // 2), np.uint8)
+        v = np.empty_like(u)
+        self._check(self.lib.vc1me_frame_download(self.ctx, frame.handle, y.ctypes.data, u.ctypes.data,
+                                                  v.ctypes.data))
+        return y, u, v
+
+    def estimate_intensity(self, cur: Frame, ref: Frame, mvs) -> tuple:
+        """vc1me_estimate_intensity_comp: (enabled, lumscale, lumshift) from
+        the quarter-pel 1-MV field `mvs` (mbw * mbh pairs)."""
+        m = np.ascontiguousarray(np.asarray(mvs, np.int32).reshape(-1, 2))
+        mbw, mbh = abi.mb_grid(cur.width, cur.height)
+        if m.shape[0] != mbw * mbh:
+            raise ValueError(f"{m.shape[0]} vectors for a {mbw}x{mbh} macroblock grid")
+        out = abi.Intensity()
+        self._check(self.lib.vc1me_estimate_intensity_comp(self.ctx, cur.handle, ref.handle,
+                                                           m.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(out)))
+        return out.as_tuple()
+
+    def intensity_compensated(self, ref: Frame, ic: tuple) -> Frame:
+        """vc1me_frame_intensity_compensated: a new resident frame."""
+        h = C.c_void_p()
+        self._check(self.lib.vc1me_frame_intensity_compensated(self.ctx, ref.handle, C.byref(abi.Intensity(*ic)),
+                                                               C.byref(h)))
+        return Frame(self, h.value, ref.width, ref.height)
 
     def analyze(self, cur: Frame, ref: Frame, params: abi.Params) -> abi.ResultArrays:
         r = abi.ResultArrays(params.width, params.height)

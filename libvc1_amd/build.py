@@ -30,7 +30,7 @@ COMMON = [
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 
-SOURCES = ["vc1me_kernels.hip", "vc1me_nn.hip", "vc1me_api.cpp"]
+SOURCES = ["vc1me_kernels.hip", "vc1me_nn.hip", "vc1me_intensity.hip", "vc1me_api.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

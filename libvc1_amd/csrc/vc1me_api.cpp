@@ -237,6 +237,9 @@ struct vc1me_ctx {
     int32_t* d_wl_n = nullptr;
     int32_t* h_wl_n = nullptr;
     size_t wl_jobs_cap = 0;
+    // intensity estimation: vectors, samples, the four sums and 49 SADs
+    uint8_t* d_ic = nullptr;
+    size_t ic_cap = 0;
     // last step's relaxation passes: analyses per pass (summed over jobs) and,
     // with profiling on, device milliseconds per pass
     std::vector<int64_t> pass_analyses;
@@ -1023,6 +1026,7 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_props) (void)hipFree(ctx->d_props);
     if (ctx->d_wl_n) (void)hipFree(ctx->d_wl_n);
     if (ctx->h_wl_n) (void)hipHostFree(ctx->h_wl_n);
+    if (ctx->d_ic) (void)hipFree(ctx->d_ic);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& b : ctx->phase_pool) if (b.first) (void)hipFree(b.first);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
@@ -1186,6 +1190,155 @@ int vc1me_analyze_p_frame_seeded(vc1me_ctx* ctx, const vc1me_frame* cur, const v
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (int rc = set_device(ctx)) return rc;
     return run_jobs(ctx, &j, 1, nullptr, 0.f, {}, &seed);
+}
+
+// ---- intensity compensation
+
+namespace {
+
+// (scale, shift) of intensity_map_y (src/encoder_motion_compensation.cpp:175-187)
+void intensity_scale_shift(const vc1me_intensity& ic, int* scale, int* shift) {
+    if (ic.lumscale == 0) {
+        *scale = -64;
+        *shift = (255 - ic.lumshift * 2) * 64;
+        if (ic.lumshift > 31) *shift += 128 * 64;
+    } else {
+        *scale = ic.lumscale + 32;
+        *shift = (ic.lumshift > 31 ? ic.lumshift - 64 : ic.lumshift) * 64;
+    }
+}
+
+// the luma and chroma maps of intensity_map_y / intensity_map_uv
+// (src/encoder_motion_compensation.cpp:175-193) as byte tables
+vc1me::IntensityLuts intensity_luts(const vc1me_intensity& ic) {
+    vc1me::IntensityLuts t{};
+    uint8_t* b = reinterpret_cast<uint8_t*>(t.w);
+    int scale = 64, shift = 0;
+    if (ic.enabled) intensity_scale_shift(ic, &scale, &shift);
+    const int uv_scale = ic.lumscale == 0 ? -64 : ic.lumscale + 32;
+    for (int v = 0; v < 256; ++v) {
+        b[v] = ic.enabled ? static_cast<uint8_t>(std::clamp((scale * v + shift + 32) >> 6, 0, 255))
+                          : static_cast<uint8_t>(v);
+        b[256 + v] = ic.enabled ? static_cast<uint8_t>(std::clamp((uv_scale * (v - 128) + 128 * 64 + 32) >> 6, 0, 255))
+                                : static_cast<uint8_t>(v);
+    }
+    return t;
+}
+
+}  // namespace
+
+int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                  const int32_t* mvs, vc1me_intensity* out) {
+    if (!ctx || !cur || !ref || !mvs || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (cur->w != ref->w || cur->h != ref->h) return fail(VC1ME_EINVAL, "cur and ref rasters differ");
+    out->enabled = 0; out->lumscale = 32; out->lumshift = 0;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const int W = cur->w, H = cur->h, mbw = (W + 15) / 16, mbh = (H + 15) / 16;
+    const size_t mbs = static_cast<size_t>(mbw) * mbh;
+    const int n = ((W + 3) / 4) * ((H + 3) / 4);
+    if (n < 16) return VC1ME_OK;      // too few samples: no compensation (:318)
+    const size_t o_mv = 0, o_smp = (mbs * sizeof(int2) + 255) & ~size_t(255);
+    const size_t o_sum = (o_smp + n * sizeof(uint16_t) + 255) & ~size_t(255);
+    const size_t o_sad = o_sum + 4 * sizeof(unsigned long long);
+    const size_t need = o_sad + 49 * sizeof(unsigned long long);
+    if (ctx->ic_cap < need) {
+        if (ctx->d_ic) (void)hipFree(ctx->d_ic);
+        ctx->d_ic = nullptr;
+        ctx->ic_cap = 0;
+        const hipError_t e = hipMalloc(&ctx->d_ic, need);
+        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("intensity buffers: ") + hipGetErrorString(e));
+        ctx->ic_cap = need;
+    }
+    int2* d_mv = reinterpret_cast<int2*>(ctx->d_ic + o_mv);
+    uint16_t* d_smp = reinterpret_cast<uint16_t*>(ctx->d_ic + o_smp);
+    auto* d_sum = reinterpret_cast<unsigned long long*>(ctx->d_ic + o_sum);
+    auto* d_sad = reinterpret_cast<unsigned long long*>(ctx->d_ic + o_sad);
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(d_mv, mvs, mbs * sizeof(int2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemsetAsync(d_sum, 0, (4 + 49) * sizeof(unsigned long long), st));
+    HIP_TRY(vc1me::launch_ic_samples(cur->y, ref->y, W, H, d_mv, mbw, d_smp, d_sum, st));
+    unsigned long long sums[4];
+    HIP_TRY(hipMemcpyAsync(sums, d_sum, sizeof sums, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // the least-squares fit in long double on the exact integer sums
+    // (src/encoder_motion_compensation.cpp:319-328)
+    const long double ln = static_cast<long double>(n), sx = static_cast<long double>(sums[0]),
+                      sy = static_cast<long double>(sums[1]);
+    const long double denom = ln * static_cast<long double>(sums[2]) - sx * sx;
+    long double a = 1.0L, b = 0.0L;
+    if (std::fabs(denom) > 1e-9L) {
+        a = (ln * static_cast<long double>(sums[3]) - sx * sy) / denom;
+        b = (sy - a * sx) / ln;
+    } else {
+        b = (sy - sx) / ln;
+    }
+    const int ls = std::clamp(static_cast<int>(std::llround(a * 64.0L - 32.0L)), 1, 63);
+    const int sh = std::clamp(static_cast<int>(std::llround(b)), -32, 31);
+    const int l0 = std::max(1, ls - 3), l1 = std::min(63, ls + 3);
+    const int z0 = std::max(-32, sh - 3), z1 = std::min(31, sh + 3);
+    HIP_TRY(vc1me::launch_ic_sads(d_smp, n, l0, l1 - l0 + 1, z0, z1 - z0 + 1, d_sad, st));
+    unsigned long long sads[49];
+    HIP_TRY(hipMemcpyAsync(sads, d_sad, sizeof sads, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // smallest SAD, ties to the smaller |scale - 32| + |shift|, then the
+    // first in (scale, shift) order (:337-345)
+    uint64_t best = UINT64_MAX;
+    int best_ls = 32, best_sh = 0;
+    for (int l = l0; l <= l1; ++l)
+        for (int z = z0; z <= z1; ++z) {
+            const uint64_t sad = sads[(l - l0) * 7 + (z - z0)];
+            if (sad < best || (sad == best && std::abs(l - 32) + std::abs(z) < std::abs(best_ls - 32) + std::abs(best_sh))) {
+                best = sad; best_ls = l; best_sh = z;
+            }
+        }
+    if (best_ls == 32 && best_sh == 0) return VC1ME_OK;
+    out->enabled = 1;
+    out->lumscale = best_ls;
+    out->lumshift = best_sh < 0 ? best_sh + 64 : best_sh;
+    return VC1ME_OK;
+}
+
+int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_intensity* ic,
+                                      vc1me_frame** out) {
+    if (!ctx || !ref || !ic || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (ic->enabled && (ic->lumscale < 0 || ic->lumscale > 63 || ic->lumshift < 0 || ic->lumshift > 63))
+        return fail(VC1ME_EINVAL, "LUMSCALE and LUMSHIFT are 6-bit codes");
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const size_t ys = static_cast<size_t>(ref->w) * ref->h;
+    const size_t cs = static_cast<size_t>(ref->w / 2) * (ref->h / 2);
+    auto* f = new vc1me_frame();
+    f->w = ref->w;
+    f->h = ref->h;
+    uint8_t* mem = nullptr;
+    // the same 64-byte tail as an uploaded frame: the remap's 16-byte words
+    // may run past the last plane into it
+    hipError_t e = hipMalloc(&mem, ys + 2 * cs + 64);
+    if (e != hipSuccess) { delete f; return fail(VC1ME_ENOMEM, std::string("frame allocation: ") + hipGetErrorString(e)); }
+    f->y = mem;
+    f->u = mem + ys;
+    f->v = mem + ys + cs;
+    e = vc1me::launch_ic_remap(ref->y, mem, ys + 2 * cs, ys, intensity_luts(*ic), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { (void)hipFree(mem); delete f; return fail(VC1ME_EDEVICE, std::string("intensity remap: ") + hipGetErrorString(e)); }
+    *out = f;
+    return VC1ME_OK;
+}
+
+int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v) {
+    if (!ctx || !frame || !y || !u || !v) return fail(VC1ME_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const size_t ys = static_cast<size_t>(frame->w) * frame->h;
+    const size_t cs = static_cast<size_t>(frame->w / 2) * (frame->h / 2);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(y, frame->y, ys, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(u, frame->u, cs, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(v, frame->v, cs, hipMemcpyDeviceToHost));
+    return VC1ME_OK;
 }
 
 int vc1me_analyze_step(vc1me_ctx* ctx, const vc1me_job* jobs, int n) {

@@ -133,4 +133,16 @@ hipError_t launch_score_batch(const uint8_t* cur, const uint8_t* ref, int width,
                               int bw, int bh, int sample_stride, int bilinear, int rnd, int metric,
                               const int2* cands, int n, uint32_t* costs, hipStream_t s);
 
+// Intensity compensation (vc1me_intensity.hip): the luma table at bytes
+// 0..255 and the chroma table at 256..511.
+struct IntensityLuts {
+    uint32_t w[128];
+};
+hipError_t launch_ic_samples(const uint8_t* cy, const uint8_t* ry, int W, int H, const int2* mvs, int mbw,
+                             uint16_t* samples, unsigned long long* sums, hipStream_t s);
+hipError_t launch_ic_sads(const uint16_t* samples, int n, int l0, int nl, int z0, int nz, unsigned long long* sads,
+                          hipStream_t s);
+hipError_t launch_ic_remap(const uint8_t* src, uint8_t* dst, size_t bytes, size_t luma_bytes,
+                           const IntensityLuts& luts, hipStream_t s);
+
 }  // namespace vc1me

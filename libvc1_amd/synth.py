@@ -322,3 +322,11 @@ def ibbp_p_pairs(frames: int, gop: int = 15, spacing: int = 3) -> list[tuple[int
 
 def p_only_pairs(frames: int) -> list[tuple[int, int]]:
     return [(k, k - 1) for k in range(1, frames)]
+
+
+def fade(pic: Picture, gain: float, offset: float) -> Picture:
+    """A brightness fade of `pic`: luma gain * Y + offset, chroma pulled
+    toward 128 by the same gain, rounded and clipped to 8 bits."""
+    y = np.clip(np.rint(gain * pic.y.astype(np.float64) + offset), 0, 255).astype(np.uint8)
+    uv = lambda p: np.clip(np.rint(128.0 + gain * (p.astype(np.float64) - 128.0)), 0, 255).astype(np.uint8)
+    return Picture(y, uv(pic.u), uv(pic.v))

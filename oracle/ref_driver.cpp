@@ -14,6 +14,8 @@
 //   global_motion_search     src/encoder_motion.cpp:2076-2141
 //   frame_sad                src/encoder_motion.cpp:1499-1576
 //   luma_mc_sample_mode      src/encoder_motion_compensation.cpp:398-415
+//   estimate_intensity_comp  src/encoder_motion_compensation.cpp:296-348
+//   intensity_compensated_reference  src/encoder_motion_compensation.cpp:195-294
 #include "encoder_internal.h"
 
 #include "../include/vc1me.h"
@@ -48,6 +50,19 @@ EncoderConfig make_config(const vc1me_params* p, bool simd_auto) {
     c.debug_disable_p_intra = p->debug_disable_p_intra != 0;
     c.syntax = StreamSyntax::Advanced;
     c.rndctrl = false;
+    if (simd_auto) {
+        static const SimdDispatch dispatch = auto_select_simd(false, false).dispatch;
+        c.simd_dispatch = dispatch;
+    }
+    return c;
+}
+
+// an encoder configuration that only fixes the raster (and SIMD tier)
+EncoderConfig raster_config(int width, int height, bool simd_auto) {
+    EncoderConfig c;
+    c.width = width;
+    c.height = height;
+    c.syntax = StreamSyntax::Advanced;
     if (simd_auto) {
         static const SimdDispatch dispatch = auto_select_simd(false, false).dispatch;
         c.simd_dispatch = dispatch;
@@ -193,6 +208,43 @@ int ref_global_motion(const vc1me_params* p, const vc1me_planes* cur, const vc1m
     const auto g = enc.global_motion_search(make_frame(cur), make_frame(ref), range);
     *dx = g.dx; *dy = g.dy; *sad = g.full_sad;
     return 0;
+}
+
+int ref_estimate_intensity_comp(const vc1me_planes* cur, const vc1me_planes* ref, const int32_t* mvs,
+                                vc1me_intensity* out) {
+    try {
+        Vc1Encoder enc(raster_config(cur->width, cur->height, false));
+        const size_t mbs = static_cast<size_t>((cur->width + 15) / 16) * ((cur->height + 15) / 16);
+        std::vector<Vc1Encoder::MotionVector> mv(mbs);
+        for (size_t i = 0; i < mbs; ++i) { mv[i].xq = mvs[2 * i]; mv[i].yq = mvs[2 * i + 1]; }
+        const auto ic = enc.estimate_intensity_comp(make_frame(cur), make_frame(ref), mv);
+        out->enabled = ic.enabled ? 1 : 0;
+        out->lumscale = ic.lumscale;
+        out->lumshift = ic.lumshift;
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int ref_intensity_compensate(const vc1me_planes* ref, const vc1me_intensity* ic, uint8_t* y, uint8_t* u,
+                             uint8_t* v, int simd_auto) {
+    try {
+        Vc1Encoder enc(raster_config(ref->width, ref->height, simd_auto != 0));
+        Vc1Encoder::IntensityComp c;
+        c.enabled = ic->enabled != 0;
+        c.lumscale = static_cast<uint8_t>(ic->lumscale);
+        c.lumshift = static_cast<uint8_t>(ic->lumshift);
+        const Frame out = enc.intensity_compensated_reference(make_frame(ref), c);
+        std::memcpy(y, out.y.data(), out.y.size());
+        std::memcpy(u, out.u.data(), out.u.size());
+        std::memcpy(v, out.v.data(), out.v.size());
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
 }
 
 int ref_luma_mc_sample(const uint8_t* plane, int w, int h, int x, int y, int xq, int yq, int mode, int rnd) {

@@ -1754,3 +1754,110 @@ void oracle_mb_costs20(const vc1me_params* p, const vc1me_planes* cur, const vc1
             out[pos] = 20u * block_sad_motion(&c, mx * 16, my * 16, 16, 16, one, one_mode) + lam20 * rate;
         }
 }
+
+/* ------------------------------------------------------------------ intensity compensation */
+
+/* Vc1Encoder::intensity_map_y (src/encoder_motion_compensation.cpp:175-187):
+ * LUMSCALE 0 is the inverting map, otherwise scale = LUMSCALE + 32 over 64
+ * and LUMSHIFT a 6-bit two's-complement offset. */
+static void intensity_scale_shift(const vc1me_intensity* ic, int* scale, int* shift) {
+    if (ic->lumscale == 0) {
+        *scale = -64;
+        *shift = (255 - ic->lumshift * 2) * 64;
+        if (ic->lumshift > 31) *shift += 128 * 64;
+    } else {
+        *scale = ic->lumscale + 32;
+        *shift = (ic->lumshift > 31 ? ic->lumshift - 64 : ic->lumshift) * 64;
+    }
+}
+
+uint8_t oracle_intensity_map_y(uint8_t v, const vc1me_intensity* ic) {
+    if (!ic->enabled) return v;
+    int scale, shift;
+    intensity_scale_shift(ic, &scale, &shift);
+    return (uint8_t)clampi((scale * (int)v + shift + 32) >> 6, 0, 255);
+}
+
+/* Vc1Encoder::intensity_map_uv (src/encoder_motion_compensation.cpp:189-193) */
+uint8_t oracle_intensity_map_uv(uint8_t v, const vc1me_intensity* ic) {
+    if (!ic->enabled) return v;
+    const int scale = ic->lumscale == 0 ? -64 : ic->lumscale + 32;
+    return (uint8_t)clampi((scale * ((int)v - 128) + 128 * 64 + 32) >> 6, 0, 255);
+}
+
+/* Vc1Encoder::intensity_compensated_reference
+ * (src/encoder_motion_compensation.cpp:195-294): every sample through the
+ * map of its plane. */
+void oracle_intensity_compensate(const vc1me_planes* ref, const vc1me_intensity* ic, uint8_t* y, uint8_t* u,
+                                 uint8_t* v) {
+    const size_t ys = (size_t)ref->width * ref->height, cs = (size_t)(ref->width / 2) * (ref->height / 2);
+    for (size_t i = 0; i < ys; ++i) y[i] = oracle_intensity_map_y(ref->y[i], ic);
+    for (size_t i = 0; i < cs; ++i) {
+        u[i] = oracle_intensity_map_uv(ref->u ? ref->u[i] : 128, ic);
+        v[i] = oracle_intensity_map_uv(ref->v ? ref->v[i] : 128, ic);
+    }
+}
+
+/* Vc1Encoder::estimate_intensity_comp (src/encoder_motion_compensation.cpp:296-348).
+ * Samples: every 4th pixel in x and y of each macroblock, the reference
+ * sample displaced by the macroblock's vector rounded to whole pixels
+ * (round_qpel_to_pixel, src/encoder_motion.cpp:1295-1297) and clamped to
+ * the raster.  The least-squares fit cur ~ a * ref + b runs in long double
+ * on exact integer sums; LUMSCALE = clamp(round(64a - 32), 1, 63), LUMSHIFT =
+ * clamp(round(b), -32, 31); then the (scale, shift) of smallest sampled SAD
+ * within +-3 of both, ties to the smaller |scale - 32| + |shift| and then to
+ * the first in (scale, shift) order.  (32, 0) means no compensation. */
+int oracle_estimate_intensity_comp(const vc1me_planes* cur, const vc1me_planes* ref, const int32_t* mvs,
+                                   vc1me_intensity* out) {
+    const int W = cur->width, H = cur->height;
+    const int mbw = (W + 15) / 16, mbh = (H + 15) / 16;
+    out->enabled = 0; out->lumscale = 32; out->lumshift = 0;
+    const size_t cap = (size_t)((W + 3) / 4) * ((H + 3) / 4);
+    uint8_t* sx8 = (uint8_t*)malloc(cap);
+    uint8_t* sy8 = (uint8_t*)malloc(cap);
+    uint64_t sx = 0, sy = 0, sxx = 0, sxy = 0, n = 0;
+    for (int my = 0; my < mbh; ++my)
+        for (int mx = 0; mx < mbw; ++mx) {
+            const int32_t xq = mvs[2 * ((size_t)my * mbw + mx)], yq = mvs[2 * ((size_t)my * mbw + mx) + 1];
+            const int dx = xq >= 0 ? (xq + 2) / 4 : -((-xq + 2) / 4);
+            const int dy = yq >= 0 ? (yq + 2) / 4 : -((-yq + 2) / 4);
+            for (int y = 0; y < 16 && my * 16 + y < H; y += 4)
+                for (int x = 0; x < 16 && mx * 16 + x < W; x += 4) {
+                    const int px = mx * 16 + x, py = my * 16 + y;
+                    const uint8_t rv = ref->y[(size_t)clampi(py + dy, 0, H - 1) * W + clampi(px + dx, 0, W - 1)];
+                    const uint8_t cv = cur->y[(size_t)py * W + px];
+                    sx8[n] = rv; sy8[n] = cv;
+                    sx += rv; sy += cv; sxx += (uint64_t)rv * rv; sxy += (uint64_t)rv * cv; ++n;
+                }
+        }
+    if (n < 16) { free(sx8); free(sy8); return 0; }
+    const long double ln = (long double)n, lsx = (long double)sx, lsy = (long double)sy;
+    const long double denom = ln * (long double)sxx - lsx * lsx;
+    long double a = 1.0L, b = 0.0L;
+    if (fabsl(denom) > 1e-9L) {
+        a = (ln * (long double)sxy - lsx * lsy) / denom;
+        b = (lsy - a * lsx) / ln;
+    } else {
+        b = (lsy - lsx) / ln;
+    }
+    const int ls = clampi((int)llroundl(a * 64.0L - 32.0L), 1, 63);
+    const int sh = clampi((int)llroundl(b), -32, 31);
+    uint64_t best = UINT64_MAX;
+    int best_ls = 32, best_sh = 0;
+    for (int l = imax(1, ls - 3); l <= imin(63, ls + 3); ++l)
+        for (int z = imax(-32, sh - 3); z <= imin(31, sh + 3); ++z) {
+            const vc1me_intensity ic = {1, l, z < 0 ? z + 64 : z};
+            uint64_t sad = 0;
+            for (uint64_t i = 0; i < n; ++i) sad += (uint64_t)iabs((int)sy8[i] - (int)oracle_intensity_map_y(sx8[i], &ic));
+            if (sad < best || (sad == best && iabs(l - 32) + iabs(z) < iabs(best_ls - 32) + iabs(best_sh))) {
+                best = sad; best_ls = l; best_sh = z;
+            }
+        }
+    free(sx8);
+    free(sy8);
+    if (best_ls == 32 && best_sh == 0) return 0;
+    out->enabled = 1;
+    out->lumscale = best_ls;
+    out->lumshift = best_sh < 0 ? best_sh + 64 : best_sh;
+    return 0;
+}

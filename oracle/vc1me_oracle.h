@@ -87,6 +87,17 @@ void oracle_integer_search(const vc1me_params* p, const vc1me_planes* cur, const
                            int pre_xq, int pre_yq, int gdx, int gdy, int range, int32_t* dx, int32_t* dy,
                            uint64_t* sad);
 
+/* intensity_map_y / intensity_map_uv (src/encoder_motion_compensation.cpp:175-193) */
+uint8_t oracle_intensity_map_y(uint8_t v, const vc1me_intensity* ic);
+uint8_t oracle_intensity_map_uv(uint8_t v, const vc1me_intensity* ic);
+/* intensity_compensated_reference (src/encoder_motion_compensation.cpp:195-294) */
+void oracle_intensity_compensate(const vc1me_planes* ref, const vc1me_intensity* ic, uint8_t* y, uint8_t* u,
+                                 uint8_t* v);
+/* estimate_intensity_comp (src/encoder_motion_compensation.cpp:296-348); mvs:
+ * mbw * mbh quarter-pel (x, y) pairs */
+int oracle_estimate_intensity_comp(const vc1me_planes* cur, const vc1me_planes* ref, const int32_t* mvs,
+                                   vc1me_intensity* out);
+
 #ifdef __cplusplus
 }
 #endif

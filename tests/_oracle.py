@@ -54,6 +54,14 @@ def oracle() -> C.CDLL:
         lib.oracle_mb_costs20.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(C.c_int32), P(C.c_int32),
                                           P(C.c_uint8), P(C.c_uint64)]
         lib.oracle_mb_costs20.restype = None
+        lib.oracle_estimate_intensity_comp.argtypes = [P(abi.Planes), P(abi.Planes), P(C.c_int32), P(abi.Intensity)]
+        lib.oracle_intensity_compensate.argtypes = [P(abi.Planes), P(abi.Intensity), C.c_void_p, C.c_void_p,
+                                                    C.c_void_p]
+        lib.oracle_intensity_compensate.restype = None
+        lib.oracle_intensity_map_y.argtypes = [C.c_uint8, P(abi.Intensity)]
+        lib.oracle_intensity_map_y.restype = C.c_uint8
+        lib.oracle_intensity_map_uv.argtypes = [C.c_uint8, P(abi.Intensity)]
+        lib.oracle_intensity_map_uv.restype = C.c_uint8
         lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                       P(abi.PResult), P(OracleOps)]
         _oracle = lib
@@ -83,6 +91,9 @@ def reference() -> C.CDLL:
         lib.ref_time_p_frames.argtypes = [P(abi.Params), C.c_int, P(abi.Planes), P(abi.Planes), C.c_int, C.c_int]
         lib.ref_time_p_frames.restype = C.c_double
         lib.ref_last_error.restype = C.c_char_p
+        lib.ref_estimate_intensity_comp.argtypes = [P(abi.Planes), P(abi.Planes), P(C.c_int32), P(abi.Intensity)]
+        lib.ref_intensity_compensate.argtypes = [P(abi.Planes), P(abi.Intensity), C.c_void_p, C.c_void_p,
+                                                 C.c_void_p, C.c_int]
         lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                   P(abi.PResult), C.c_int]
         _ref = lib
@@ -126,6 +137,47 @@ def reference_analyze_seeded(params: abi.Params, cur, ref, seed: abi.ResultArray
                                                C.byref(seed.struct), C.byref(out.struct), 0)
     assert rc == 0, reference().ref_last_error()
     return out
+
+
+def _mv_pairs(mvs: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(mvs, np.int32).reshape(-1, 2))
+
+
+def oracle_estimate_intensity(cur, ref, mvs: np.ndarray) -> tuple:
+    """estimate_intensity_comp: (enabled, lumscale, lumshift)."""
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    m = _mv_pairs(mvs)
+    out = abi.Intensity()
+    assert oracle().oracle_estimate_intensity_comp(C.byref(c.struct), C.byref(r.struct),
+                                                   m.ctypes.data_as(P(C.c_int32)), C.byref(out)) == 0
+    return out.as_tuple()
+
+
+def reference_estimate_intensity(cur, ref, mvs: np.ndarray) -> tuple:
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    m = _mv_pairs(mvs)
+    out = abi.Intensity()
+    rc = reference().ref_estimate_intensity_comp(C.byref(c.struct), C.byref(r.struct),
+                                                 m.ctypes.data_as(P(C.c_int32)), C.byref(out))
+    assert rc == 0, reference().ref_last_error()
+    return out.as_tuple()
+
+
+def _compensate(fn, ref, ic: tuple, *extra):
+    r = abi.PlaneSet.of(ref)
+    y, u, v = np.empty_like(r.y), np.empty_like(r.u), np.empty_like(r.v)
+    rc = fn(C.byref(r.struct), C.byref(abi.Intensity(*ic)), y.ctypes.data, u.ctypes.data, v.ctypes.data, *extra)
+    assert rc in (None, 0)
+    return y, u, v
+
+
+def oracle_intensity_compensate(ref, ic: tuple):
+    """intensity_compensated_reference: the (y, u, v) planes."""
+    return _compensate(oracle().oracle_intensity_compensate, ref, ic)
+
+
+def reference_intensity_compensate(ref, ic: tuple, simd_auto: int = 0):
+    return _compensate(reference().ref_intensity_compensate, ref, ic, simd_auto)
 
 
 def oracle_score_batch(cur: np.ndarray, ref: np.ndarray, x0, y0, bw, bh, cands, stride=1, bilinear=0, rnd=0,

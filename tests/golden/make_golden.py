@@ -5,11 +5,14 @@ Runs the real libvc1 reference (compiled from its own sources into
 oracle/_ref/libvc1ref.so by `make -C oracle ref`) on small deterministic
 inputs and stores inputs and every PAnalysis output field as .npz fixtures,
 plus SHA-256 digests of the outputs of larger synthetic cases
-(analyze_digests.json), and the seeded core's fixtures (seeded/*.npz: the
-inputs, the seed's vectors and distant fields, every output field).  The
-fixtures are data only.
+(analyze_digests.json), the seeded core's fixtures (seeded/*.npz: the
+inputs, the seed's vectors and distant fields, every output field), and the
+intensity-compensation fixtures (intensity.json: estimate_intensity_comp on
+generated fades with their vector fields, and a digest of the luma and
+chroma maps of every LUMSCALE / LUMSHIFT code).  The fixtures are data only.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # everything
+    python tests/golden/make_golden.py intensity    # intensity.json only
 """
 from __future__ import annotations
 
@@ -26,7 +29,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from libvc1_amd import abi, synth  # noqa: E402
-from _oracle import reference_analyze, reference_analyze_seeded  # noqa: E402
+from _oracle import (oracle_analyze, reference_analyze, reference_analyze_seeded,  # noqa: E402
+                     reference_estimate_intensity, reference_intensity_compensate)
 
 
 def formula_pair(w, h, a, b):
@@ -149,7 +153,64 @@ def digest_inputs(kind, a):
     return d[a["cur"]], d[a["ref"]], a["width"], a["height"]
 
 
+def intensity_inputs(case: dict):
+    """(cur, ref, mvs) of an intensity case: a fade of frame 3 of a pan
+    (or of frame 0 for the static kind) against frame 0."""
+    w, h = case["size"]
+    d = synth.pan_rectangles(case["seed"], 4, width=w, height=h, frame_indices=[0, 3])
+    src = d[0] if case["kind"] == "static" else d[3]
+    cur = synth.fade(src, case["gain"], case["offset"])
+    return cur, d[0], np.asarray(case["mvs"], np.int32).reshape(-1, 2)
+
+
+def intensity_map_table(compensate) -> str:
+    """SHA-256 over the compensated planes of a 32x16 picture holding every
+    luma value twice and 128 distinct chroma values per plane, for enabled
+    = 0 and 1 and every (LUMSCALE, LUMSHIFT) code."""
+    pic = synth.Picture((np.arange(512) % 256).reshape(16, 32).astype(np.uint8),
+                        (np.arange(128).reshape(8, 16) * 2).astype(np.uint8),
+                        (np.arange(128).reshape(8, 16) * 2 + 1).astype(np.uint8))
+    hsh = hashlib.sha256()
+    for en in (0, 1):
+        for ls in range(64):
+            for sh in range(64):
+                for plane in compensate(pic, (en, ls, sh)):
+                    hsh.update(plane.tobytes())
+    return hsh.hexdigest()
+
+
+def intensity_fixtures() -> dict:
+    cases = []
+    rng = np.random.default_rng(91)
+    for k, (size, gain, offset, kind) in enumerate([
+            ((320, 176), 0.8, 10.0, "static"), ((320, 176), 1.25, -20.0, "static"),
+            ((320, 176), 1.0, 0.0, "static"), ((320, 176), 0.9, 6.0, "search"),
+            ((320, 176), 1.1, -8.0, "search"), ((200, 120), 0.7, 25.0, "search"),
+            ((200, 120), 1.3, -30.0, "random"), ((96, 64), 0.3, 0.0, "static"),
+            ((96, 64), 1.6, -90.0, "static"), ((176, 144), 1.0, 12.0, "search")]):
+        case = {"seed": 300 + k, "size": list(size), "gain": gain, "offset": offset, "kind": kind}
+        mbw, mbh = abi.mb_grid(*size)
+        case["mvs"] = [0] * (2 * mbw * mbh)
+        cur, ref, _ = intensity_inputs(case)
+        if kind == "search":
+            q = abi.default_params(size[0], size[1], mode=abi.MODE_ONE_MV_QPEL, allow_intra=0, pqindex=8,
+                                   motion_search_range=128)
+            case["mvs"] = oracle_analyze(q, cur, ref).mvs.reshape(-1).tolist()
+        elif kind == "random":
+            case["mvs"] = rng.integers(-60, 60, size=2 * mbw * mbh).tolist()
+        cur, ref, mvs = intensity_inputs(case)
+        case["expected"] = list(reference_estimate_intensity(cur, ref, mvs))
+        cases.append(case)
+        print(f"intensity case {k}: {kind} {size} gain {gain} offset {offset} -> {case['expected']}")
+    return {"estimate": cases,
+            "map_sha256": intensity_map_table(lambda pic, ic: reference_intensity_compensate(pic, ic, ic[0]))}
+
+
 def main() -> int:
+    if sys.argv[1:] == ["intensity"]:
+        with open(os.path.join(HERE, "intensity.json"), "w") as f:
+            json.dump(intensity_fixtures(), f)
+        return 0
     for name, c, r, over in small_cases():
         h, w = c.y.shape
         p = abi.default_params(w, h, **over)
@@ -180,6 +241,8 @@ def main() -> int:
         print(f"digest {name}: {digests[-1]['sha256'][:16]}")
     with open(os.path.join(HERE, "analyze_digests.json"), "w") as f:
         json.dump(digests, f, indent=1)
+    with open(os.path.join(HERE, "intensity.json"), "w") as f:
+        json.dump(intensity_fixtures(), f)
     return 0
 
 
