@@ -304,6 +304,7 @@ def seeded_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, check_
         bad += res[0].diff(oracle_analyze_seeded(final, cur, ref, cands[2 * n]))
         parity = "bit-exact" if not bad else f"MISMATCH in {bad}"
         log(f"seeded passes of pair 0 vs oracle (Mixed-MV candidate, final): {parity} ({time.time() - t1:.1f}s)")
+    intensity = intensity_part(be, jobs, res, check_pair)
     return {"value": round(cand_value, 1), "unit": "MB/s",
             "value_final_pass": round(final_value, 1), "speedup_vs_plain": round(cand_value / plain_value, 4),
             "ms_seed_pass": round(t_seed * 1e3 / steps, 3), "ms_candidate_step": round(t_cand * 1e3 / steps, 3),
@@ -311,7 +312,52 @@ def seeded_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, check_
             "final_cost_vs_plain": round(float(cs.sum() / cp.sum() - 1), 6),
             "passes": "1-MV qpel seed (unseeded, no intra); 48-job step of seeded Hpel / HpelBilinear / Mixed-MV "
                       "candidates (no intra); final seeded Mixed-MV with intra",
-            "parity": parity}
+            "parity": parity, "intensity": intensity}
+
+
+def intensity_part(be, jobs, final_results, check_pair):
+    """The fade path's two kernels on the same pairs (the reference runs them
+    when fade compensation is on, src/encoder_motion.cpp:573-576):
+    estimate_intensity_comp from the final pass's vectors, and the
+    compensated reference.  Host-timed per frame (launches, copies and the
+    host fit included); the remap's bytes are read + written once."""
+    import numpy as np
+    from libvc1_amd import abi
+
+    mvs = [r.mvs.copy() for r in final_results]
+    ics = [be.estimate_intensity(c, r, m) for (c, r), m in zip(jobs, mvs)]
+    t0 = time.perf_counter()
+    for (c, r), m in zip(jobs, mvs):
+        be.estimate_intensity(c, r, m)
+    t_est = (time.perf_counter() - t0) / len(jobs)
+    code = (1, 19, 10)
+    for c, r in jobs[:2]:
+        be.intensity_compensated(r, code).release()
+    t0 = time.perf_counter()
+    made = [be.intensity_compensated(r, code) for _, r in jobs]
+    t_map = (time.perf_counter() - t0) / len(jobs)
+    for f in made:
+        f.release()
+    w, h = jobs[0][0].width, jobs[0][0].height
+    frame_bytes = w * h * 3 // 2
+    parity = None
+    if check_pair is not None:
+        from _oracle import oracle_estimate_intensity, oracle_intensity_compensate
+        cur, ref = check_pair
+        g = be.intensity_compensated(jobs[0][1], code)
+        try:
+            got = be.download(g)
+        finally:
+            g.release()
+        ok = ics[0] == oracle_estimate_intensity(cur, ref, mvs[0]) and \
+            all(np.array_equal(a, b) for a, b in zip(got, oracle_intensity_compensate(ref, code)))
+        parity = "bit-exact" if ok else "MISMATCH"
+        log(f"intensity estimate and remap of pair 0 vs oracle: {parity}")
+    log(f"intensity: estimate {t_est * 1e3:.3f} ms/frame, compensated reference {t_map * 1e3:.3f} ms/frame "
+        f"({2 * frame_bytes / t_map / 1e9:.1f} GB/s host-timed), estimates {sorted(set(ics))}")
+    return {"estimate_ms_per_frame": round(t_est * 1e3, 4), "compensate_ms_per_frame": round(t_map * 1e3, 4),
+            "compensate_gbps_host_timed": round(2 * frame_bytes / t_map / 1e9, 2), "parity": parity,
+            "estimates": sorted({str(x) for x in ics})}
 
 
 BATCHED_STREAMS, BATCHED_SEED = 64, 5

@@ -247,6 +247,65 @@ int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const 
 int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_intensity* ic,
                                       vc1me_frame** out);
 
+/* ---- B-picture analysis: Vc1Encoder::analyze_b_picture_mode
+ * (src/encoder_motion.cpp:753-1004) for one progressive 1-MV mode.  Two
+ * core passes without intra (forward: cur against past, intensity
+ * compensated when forward_intensity is enabled; backward: cur against
+ * future), then per macroblock in raster order the choice among direct
+ * (the future anchor's collocated vector scaled by the temporal fraction),
+ * forward, backward and interpolated prediction by SAD + lambda * MV bits
+ * (lambda = max(1, 0.45 * PQINDEX)), and the B-intra decision.  Modes as
+ * BAnalysis::modes (src/encoder_internal.h:407). */
+enum { VC1ME_B_FORWARD = 0, VC1ME_B_BACKWARD = 1, VC1ME_B_INTERPOLATED = 2, VC1ME_B_DIRECT = 3, VC1ME_B_INTRA = 4 };
+
+typedef struct vc1me_b_params {
+    const int32_t* future_anchor_mvs;   /* mbw * mbh quarter-pel (x, y): the future P anchor's mvs */
+    const uint8_t* future_anchor_4mv;   /* mbw * mbh: the anchor's use_4mv */
+    vc1me_intensity forward_intensity;  /* the anchor's LUMSCALE / LUMSHIFT, applied to past */
+    int32_t fraction_num;               /* temporal position: 1/2, 1/3 or 2/3 */
+    int32_t fraction_den;
+    int32_t allow_intra;                /* !EncoderConfig::debug_disable_b_intra */
+    /* the encoder's minimal zero-motion configuration (search range 0 and no
+     * fade compensation, loop filter, variable transforms, trellis or
+     * adaptive quality, src/encoder_motion.cpp:775-787): every macroblock
+     * forward with zero vectors, no search */
+    int32_t zero_motion_shortcut;
+} vc1me_b_params;
+
+/* BAnalysis (src/encoder_internal.h:408-434), the motion fields; array
+ * pointers may be NULL when not wanted.  Under the zero-motion shortcut the
+ * distant arrays hold -1, -1, 0. */
+typedef struct vc1me_b_result {
+    uint8_t* modes;
+    int32_t* forward_mvs;               /* mbw * mbh quarter-pel (x, y) */
+    int32_t* backward_mvs;
+    double* inter_intra_cost_ratio;
+    double* forward_distant_local_mae;
+    double* forward_distant_candidate_mae;
+    uint8_t* forward_distant_match_decision;
+    double* backward_distant_local_mae;
+    double* backward_distant_candidate_mae;
+    uint8_t* backward_distant_match_decision;
+    uint64_t forward_macroblocks;
+    uint64_t backward_macroblocks;
+    uint64_t interpolated_macroblocks;
+    uint64_t direct_macroblocks;
+    uint64_t intra_macroblocks;
+    uint64_t moved_macroblocks;
+    uint64_t fractional_chroma_macroblocks;
+    double mean_abs_residual;
+    double rate_complexity;
+    uint64_t motion_sad;
+    uint64_t motion_bits;
+} vc1me_b_result;
+
+/* params: the encoder's ME settings; params->mode is the B picture's mode
+ * (VC1ME_MODE_ONE_MV_QPEL, _HPEL or _HPEL_BILINEAR; Mixed-MV is
+ * VC1ME_EINVAL), params->allow_intra is not used (the core passes run
+ * without intra).  cur, past and future share one raster. */
+int vc1me_analyze_b_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* past, const vc1me_frame* future,
+                          const vc1me_params* params, const vc1me_b_params* b, vc1me_b_result* out);
+
 /* Copies a resident frame's planes back to the host (width x height luma,
  * two width/2 x height/2 chroma planes).  No reference counterpart. */
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);

@@ -105,6 +105,95 @@ class Intensity(C.Structure):
         return (int(self.enabled), int(self.lumscale), int(self.lumshift))
 
 
+B_FORWARD, B_BACKWARD, B_INTERPOLATED, B_DIRECT, B_INTRA = 0, 1, 2, 3, 4
+
+
+class BParams(C.Structure):
+    _fields_ = [("future_anchor_mvs", C.POINTER(C.c_int32)), ("future_anchor_4mv", C.POINTER(C.c_uint8)),
+                ("forward_intensity", Intensity), ("fraction_num", C.c_int32), ("fraction_den", C.c_int32),
+                ("allow_intra", C.c_int32), ("zero_motion_shortcut", C.c_int32)]
+
+
+class BResult(C.Structure):
+    _fields_ = [
+        ("modes", C.POINTER(C.c_uint8)), ("forward_mvs", C.POINTER(C.c_int32)),
+        ("backward_mvs", C.POINTER(C.c_int32)), ("inter_intra_cost_ratio", C.POINTER(C.c_double)),
+        ("forward_distant_local_mae", C.POINTER(C.c_double)),
+        ("forward_distant_candidate_mae", C.POINTER(C.c_double)),
+        ("forward_distant_match_decision", C.POINTER(C.c_uint8)),
+        ("backward_distant_local_mae", C.POINTER(C.c_double)),
+        ("backward_distant_candidate_mae", C.POINTER(C.c_double)),
+        ("backward_distant_match_decision", C.POINTER(C.c_uint8)),
+        ("forward_macroblocks", C.c_uint64), ("backward_macroblocks", C.c_uint64),
+        ("interpolated_macroblocks", C.c_uint64), ("direct_macroblocks", C.c_uint64),
+        ("intra_macroblocks", C.c_uint64), ("moved_macroblocks", C.c_uint64),
+        ("fractional_chroma_macroblocks", C.c_uint64),
+        ("mean_abs_residual", C.c_double), ("rate_complexity", C.c_double),
+        ("motion_sad", C.c_uint64), ("motion_bits", C.c_uint64),
+    ]
+
+
+class BInputs:
+    """numpy-backed vc1me_b_params: the future anchor's fields and the rest."""
+
+    def __init__(self, anchor_mvs, anchor_4mv, intensity=(0, 32, 0), fraction=(1, 2), allow_intra=1,
+                 zero_motion_shortcut=0):
+        self.mvs = np.ascontiguousarray(np.asarray(anchor_mvs, np.int32).reshape(-1, 2))
+        self.four = np.ascontiguousarray(np.asarray(anchor_4mv, np.uint8).reshape(-1))
+        self.struct = BParams(self.mvs.ctypes.data_as(C.POINTER(C.c_int32)),
+                              self.four.ctypes.data_as(C.POINTER(C.c_uint8)), Intensity(*intensity),
+                              fraction[0], fraction[1], allow_intra, zero_motion_shortcut)
+
+
+class BResultArrays:
+    """numpy-backed storage for one vc1me_b_result."""
+
+    ARRAYS = ("modes", "forward_mvs", "backward_mvs", "inter_intra_cost_ratio", "forward_distant_local_mae",
+              "forward_distant_candidate_mae", "forward_distant_match_decision", "backward_distant_local_mae",
+              "backward_distant_candidate_mae", "backward_distant_match_decision")
+    SCALARS = ("forward_macroblocks", "backward_macroblocks", "interpolated_macroblocks", "direct_macroblocks",
+               "intra_macroblocks", "moved_macroblocks", "fractional_chroma_macroblocks", "mean_abs_residual",
+               "rate_complexity", "motion_sad", "motion_bits")
+
+    def __init__(self, width: int, height: int):
+        mbw, mbh = mb_grid(width, height)
+        n = mbw * mbh
+        self.mbs = n
+        self.struct = BResult()
+        for k in self.ARRAYS:
+            if k.endswith("mvs"):
+                a = np.zeros((n, 2), np.int32)
+            elif k in ("modes",) or k.endswith("decision"):
+                a = np.zeros(n, np.uint8)
+            else:
+                a = np.zeros(n, np.float64)
+            setattr(self, k, a)
+            ct = {np.int32: C.c_int32, np.uint8: C.c_uint8, np.float64: C.c_double}[a.dtype.type]
+            setattr(self.struct, k, a.ctypes.data_as(C.POINTER(ct)))
+
+    def scalars(self) -> dict:
+        return {k: getattr(self.struct, k) for k in self.SCALARS}
+
+    def diff(self, other: "BResultArrays") -> list[str]:
+        """Names of fields that are not bit-identical (doubles compared as bits)."""
+        bad = []
+        for k in self.ARRAYS:
+            a, b = getattr(self, k), getattr(other, k)
+            if a.dtype == np.float64:
+                if not np.array_equal(a.view(np.uint64), b.view(np.uint64)):
+                    bad.append(k)
+            elif not np.array_equal(a, b):
+                bad.append(k)
+        for k in self.SCALARS:
+            a, b = getattr(self.struct, k), getattr(other.struct, k)
+            if isinstance(a, float):
+                if np.float64(a).view(np.uint64) != np.float64(b).view(np.uint64):
+                    bad.append(k)
+            elif a != b:
+                bad.append(k)
+        return bad
+
+
 class Job(C.Structure):
     _fields_ = [("cur", C.c_void_p), ("ref", C.c_void_p), ("params", C.POINTER(Params)),
                 ("out", C.POINTER(PResult))]

@@ -27,7 +27,7 @@ EXPORTS = (
     "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
     "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
-    "vc1me_frame_download",
+    "vc1me_frame_download", "vc1me_analyze_b_frame",
 )
 
 
@@ -72,6 +72,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                                  P(abi.PResult)]
     lib.vc1me_estimate_intensity_comp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(C.c_int32), P(abi.Intensity)]
     lib.vc1me_frame_intensity_compensated.argtypes = [C.c_void_p, C.c_void_p, P(abi.Intensity), P(C.c_void_p)]
+    lib.vc1me_analyze_b_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params),
+                                          P(abi.BParams), P(abi.BResult)]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
@@ -178,6 +180,14 @@ class Backend:
         self._check(self.lib.vc1me_frame_intensity_compensated(self.ctx, ref.handle, C.byref(abi.Intensity(*ic)),
                                                                C.byref(h)))
         return Frame(self, h.value, ref.width, ref.height)
+
+    def analyze_b(self, cur: Frame, past: Frame, future: Frame, params: abi.Params,
+                  binp: abi.BInputs) -> abi.BResultArrays:
+        """vc1me_analyze_b_frame: params.mode is the B picture's 1-MV mode."""
+        r = abi.BResultArrays(params.width, params.height)
+        self._check(self.lib.vc1me_analyze_b_frame(self.ctx, cur.handle, past.handle, future.handle, C.byref(params),
+                                                   C.byref(binp.struct), C.byref(r.struct)))
+        return r
 
     def analyze(self, cur: Frame, ref: Frame, params: abi.Params) -> abi.ResultArrays:
         r = abi.ResultArrays(params.width, params.height)

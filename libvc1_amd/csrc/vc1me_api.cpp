@@ -240,6 +240,9 @@ struct vc1me_ctx {
     // intensity estimation: vectors, samples, the four sums and 49 SADs
     uint8_t* d_ic = nullptr;
     size_t ic_cap = 0;
+    // B-picture decision buffers
+    uint8_t* d_b = nullptr;
+    size_t b_cap = 0;
     // last step's relaxation passes: analyses per pass (summed over jobs) and,
     // with profiling on, device milliseconds per pass
     std::vector<int64_t> pass_analyses;
@@ -1027,6 +1030,7 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_wl_n) (void)hipFree(ctx->d_wl_n);
     if (ctx->h_wl_n) (void)hipHostFree(ctx->h_wl_n);
     if (ctx->d_ic) (void)hipFree(ctx->d_ic);
+    if (ctx->d_b) (void)hipFree(ctx->d_b);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& b : ctx->phase_pool) if (b.first) (void)hipFree(b.first);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
@@ -1239,10 +1243,11 @@ int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const 
     const size_t mbs = static_cast<size_t>(mbw) * mbh;
     const int n = ((W + 3) / 4) * ((H + 3) / 4);
     if (n < 16) return VC1ME_OK;      // too few samples: no compensation (:318)
+    const int sb = vc1me::ic_sample_blocks(n), gb = vc1me::ic_sad_blocks(n);
     const size_t o_mv = 0, o_smp = (mbs * sizeof(int2) + 255) & ~size_t(255);
     const size_t o_sum = (o_smp + n * sizeof(uint16_t) + 255) & ~size_t(255);
-    const size_t o_sad = o_sum + 4 * sizeof(unsigned long long);
-    const size_t need = o_sad + 49 * sizeof(unsigned long long);
+    const size_t o_sad = o_sum + 4 * sizeof(unsigned long long) * sb;
+    const size_t need = o_sad + 49 * sizeof(unsigned long long) * gb;
     if (ctx->ic_cap < need) {
         if (ctx->d_ic) (void)hipFree(ctx->d_ic);
         ctx->d_ic = nullptr;
@@ -1257,11 +1262,13 @@ int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const 
     auto* d_sad = reinterpret_cast<unsigned long long*>(ctx->d_ic + o_sad);
     hipStream_t st = ctx->stream;
     HIP_TRY(hipMemcpyAsync(d_mv, mvs, mbs * sizeof(int2), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemsetAsync(d_sum, 0, (4 + 49) * sizeof(unsigned long long), st));
     HIP_TRY(vc1me::launch_ic_samples(cur->y, ref->y, W, H, d_mv, mbw, d_smp, d_sum, st));
-    unsigned long long sums[4];
-    HIP_TRY(hipMemcpyAsync(sums, d_sum, sizeof sums, hipMemcpyDeviceToHost, st));
+    std::vector<unsigned long long> part(std::max<size_t>(4 * sb, 49 * gb));
+    HIP_TRY(hipMemcpyAsync(part.data(), d_sum, 4 * sizeof(unsigned long long) * sb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    unsigned long long sums[4] = {0, 0, 0, 0};
+    for (int blk = 0; blk < sb; ++blk)
+        for (int k = 0; k < 4; ++k) sums[k] += part[4 * blk + k];
     // the least-squares fit in long double on the exact integer sums
     // (src/encoder_motion_compensation.cpp:319-328)
     const long double ln = static_cast<long double>(n), sx = static_cast<long double>(sums[0]),
@@ -1278,10 +1285,12 @@ int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const 
     const int sh = std::clamp(static_cast<int>(std::llround(b)), -32, 31);
     const int l0 = std::max(1, ls - 3), l1 = std::min(63, ls + 3);
     const int z0 = std::max(-32, sh - 3), z1 = std::min(31, sh + 3);
-    HIP_TRY(vc1me::launch_ic_sads(d_smp, n, l0, l1 - l0 + 1, z0, z1 - z0 + 1, d_sad, st));
-    unsigned long long sads[49];
-    HIP_TRY(hipMemcpyAsync(sads, d_sad, sizeof sads, hipMemcpyDeviceToHost, st));
+    HIP_TRY(vc1me::launch_ic_sads(d_smp, n, l0, z0, d_sad, st));
+    HIP_TRY(hipMemcpyAsync(part.data(), d_sad, 49 * sizeof(unsigned long long) * gb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    unsigned long long sads[49] = {};
+    for (int blk = 0; blk < gb; ++blk)
+        for (int k = 0; k < 49; ++k) sads[k] += part[49 * blk + k];
     // smallest SAD, ties to the smaller |scale - 32| + |shift|, then the
     // first in (scale, shift) order (:337-345)
     uint64_t best = UINT64_MAX;
@@ -1300,14 +1309,10 @@ int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const 
     return VC1ME_OK;
 }
 
-int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_intensity* ic,
-                                      vc1me_frame** out) {
-    if (!ctx || !ref || !ic || !out) return fail(VC1ME_EINVAL, "null argument");
-    if (ic->enabled && (ic->lumscale < 0 || ic->lumscale > 63 || ic->lumshift < 0 || ic->lumshift > 63))
-        return fail(VC1ME_EINVAL, "LUMSCALE and LUMSHIFT are 6-bit codes");
-    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    if (int rc = set_device(ctx)) return rc;
+namespace {
+
+// a new resident frame holding ref under the compensation (context lock held)
+int compensated_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_intensity* ic, vc1me_frame** out) {
     const size_t ys = static_cast<size_t>(ref->w) * ref->h;
     const size_t cs = static_cast<size_t>(ref->w / 2) * (ref->h / 2);
     auto* f = new vc1me_frame();
@@ -1325,6 +1330,172 @@ int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, co
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) { (void)hipFree(mem); delete f; return fail(VC1ME_EDEVICE, std::string("intensity remap: ") + hipGetErrorString(e)); }
     *out = f;
+    return VC1ME_OK;
+}
+
+bool intensity_codes_ok(const vc1me_intensity& ic) {
+    return !ic.enabled || (ic.lumscale >= 0 && ic.lumscale <= 63 && ic.lumshift >= 0 && ic.lumshift <= 63);
+}
+
+}  // namespace
+
+int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_intensity* ic,
+                                      vc1me_frame** out) {
+    if (!ctx || !ref || !ic || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (!intensity_codes_ok(*ic)) return fail(VC1ME_EINVAL, "LUMSCALE and LUMSHIFT are 6-bit codes");
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return compensated_frame(ctx, ref, ic, out);
+}
+
+// ---- B pictures
+
+int vc1me_analyze_b_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* past, const vc1me_frame* future,
+                          const vc1me_params* params, const vc1me_b_params* b, vc1me_b_result* out) {
+    if (!ctx || !cur || !past || !future || !params || !b || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (!b->future_anchor_mvs || !b->future_anchor_4mv) return fail(VC1ME_EINVAL, "null future anchor fields");
+    if (past->w != cur->w || past->h != cur->h || future->w != cur->w || future->h != cur->h ||
+        params->width != cur->w || params->height != cur->h)
+        return fail(VC1ME_EINVAL, "cur, past, future and params must share one raster");
+    const int fn = b->fraction_num, fd = b->fraction_den;
+    if (!((fd == 2 && fn == 1) || (fd == 3 && (fn == 1 || fn == 2))))
+        return fail(VC1ME_EINVAL, "unsupported B-picture temporal fraction");
+    if (params->mode < VC1ME_MODE_ONE_MV_QPEL || params->mode > VC1ME_MODE_ONE_MV_HPEL_BILINEAR)
+        return fail(VC1ME_EINVAL, "B pictures take a 1-MV mode");
+    if (!intensity_codes_ok(b->forward_intensity)) return fail(VC1ME_EINVAL, "LUMSCALE and LUMSHIFT are 6-bit codes");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const int mbw = (cur->w + 15) / 16, mbh = (cur->h + 15) / 16;
+    const size_t mbs = static_cast<size_t>(mbw) * mbh;
+    out->forward_macroblocks = out->backward_macroblocks = out->interpolated_macroblocks = 0;
+    out->direct_macroblocks = out->intra_macroblocks = out->moved_macroblocks = 0;
+    out->fractional_chroma_macroblocks = 0;
+    out->mean_abs_residual = out->rate_complexity = 0.0;
+    out->motion_sad = out->motion_bits = 0;
+    if (b->zero_motion_shortcut && params->motion_search_range == 0) {
+        // every macroblock forward with zero vectors (src/encoder_motion.cpp:775-787)
+        for (size_t i = 0; i < mbs; ++i) {
+            if (out->modes) out->modes[i] = VC1ME_B_FORWARD;
+            if (out->forward_mvs) out->forward_mvs[2 * i] = out->forward_mvs[2 * i + 1] = 0;
+            if (out->backward_mvs) out->backward_mvs[2 * i] = out->backward_mvs[2 * i + 1] = 0;
+            if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = -1.0;
+            if (out->forward_distant_local_mae) out->forward_distant_local_mae[i] = -1.0;
+            if (out->forward_distant_candidate_mae) out->forward_distant_candidate_mae[i] = -1.0;
+            if (out->forward_distant_match_decision) out->forward_distant_match_decision[i] = 0;
+            if (out->backward_distant_local_mae) out->backward_distant_local_mae[i] = -1.0;
+            if (out->backward_distant_candidate_mae) out->backward_distant_candidate_mae[i] = -1.0;
+            if (out->backward_distant_match_decision) out->backward_distant_match_decision[i] = 0;
+        }
+        out->forward_macroblocks = mbs;
+        return VC1ME_OK;
+    }
+    // the forward reference carries the anchor's intensity compensation
+    vc1me_frame* comp = nullptr;
+    if (b->forward_intensity.enabled)
+        if (int rc = compensated_frame(ctx, past, &b->forward_intensity, &comp)) return rc;
+    struct Release {
+        vc1me_frame* f;
+        ~Release() { if (f) { (void)hipFree(f->y); delete f; } }
+    } release{comp};
+    vc1me_params q = *params;
+    q.allow_intra = 0;
+    vc1me_p_result fo{}, bo{};
+    fo.distant_local_mae = out->forward_distant_local_mae;
+    fo.distant_candidate_mae = out->forward_distant_candidate_mae;
+    fo.distant_match_decision = out->forward_distant_match_decision;
+    bo.distant_local_mae = out->backward_distant_local_mae;
+    bo.distant_candidate_mae = out->backward_distant_candidate_mae;
+    bo.distant_match_decision = out->backward_distant_match_decision;
+    const vc1me_job jobs[2] = {{cur, comp ? comp : past, &q, &fo}, {cur, future, &q, &bo}};
+    if (int rc = run_jobs(ctx, jobs, 2)) return rc;
+    // the decision reads jobs 0 and 1 of the step in place (vectors, screen
+    // SADs, phase planes), so it runs before anything else on the context
+    const size_t o_desc = 0, o_amv = 256, o_a4 = o_amv + ((mbs * sizeof(int2) + 255) & ~size_t(255));
+    const size_t o_cost = o_a4 + ((mbs + 255) & ~size_t(255));
+    const size_t o_isad = o_cost + mbs * 32, o_dir = o_isad + mbs * 8, o_fmv = o_dir + mbs * 16;
+    const size_t o_bmv = o_fmv + mbs * 8, o_ratio = o_bmv + mbs * 8, o_ssad = o_ratio + mbs * 8;
+    const size_t o_rsad = o_ssad + mbs * 8, o_srate = o_rsad + mbs * 8, o_modes = o_srate + mbs * 4;
+    const size_t need = o_modes + mbs;
+    if (ctx->b_cap < need) {
+        if (ctx->d_b) (void)hipFree(ctx->d_b);
+        ctx->d_b = nullptr;
+        ctx->b_cap = 0;
+        const hipError_t e = hipMalloc(&ctx->d_b, need);
+        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("B-picture buffers: ") + hipGetErrorString(e));
+        ctx->b_cap = need;
+    }
+    uint8_t* base = ctx->d_b;
+    vc1me::BDesc d{};
+    d.anchor_mvs = reinterpret_cast<const int2*>(base + o_amv);
+    d.anchor_4mv = base + o_a4;
+    d.mode = params->mode;
+    d.bfrac = fd == 2 ? 128 : (fn == 1 ? 85 : 170);
+    d.below_half = 2 * fn < fd ? 1 : 0;
+    d.allow_intra = b->allow_intra ? 1 : 0;
+    d.lambda = std::max(1.0, 0.45 * static_cast<double>(params->pqindex));
+    d.intra_guard = static_cast<uint64_t>(std::llround(d.lambda * 72.0));
+    d.intra_thr = params->inter_intra_threshold;
+    d.cost = reinterpret_cast<uint64_t*>(base + o_cost);
+    d.intra_sad = reinterpret_cast<uint64_t*>(base + o_isad);
+    d.direct = reinterpret_cast<int2*>(base + o_dir);
+    d.fmv = reinterpret_cast<int2*>(base + o_fmv);
+    d.bmv = reinterpret_cast<int2*>(base + o_bmv);
+    d.ratio = reinterpret_cast<double*>(base + o_ratio);
+    d.sel_sad = reinterpret_cast<uint64_t*>(base + o_ssad);
+    d.rate_sad = reinterpret_cast<uint64_t*>(base + o_rsad);
+    d.sel_rate = reinterpret_cast<uint32_t*>(base + o_srate);
+    d.modes = base + o_modes;
+    static_assert(sizeof(vc1me::BDesc) <= 256, "B descriptor slot");
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(base + o_desc, &d, sizeof d, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(base + o_amv, b->future_anchor_mvs, mbs * sizeof(int2), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(base + o_a4, b->future_anchor_4mv, mbs, hipMemcpyHostToDevice, st));
+    const auto* d_desc = reinterpret_cast<const vc1me::BDesc*>(base + o_desc);
+    HIP_TRY(vc1me::launch_b_costs(ctx->d_jobs, d_desc, static_cast<int>(mbs), st));
+    HIP_TRY(vc1me::launch_b_decide(ctx->d_jobs, d_desc, st));
+    std::vector<uint8_t> modes(mbs);
+    std::vector<int2> fmv(mbs), bmv(mbs);
+    std::vector<double> ratio(mbs);
+    std::vector<uint64_t> ssad(mbs), rsad(mbs);
+    std::vector<uint32_t> srate(mbs);
+    HIP_TRY(hipMemcpyAsync(modes.data(), d.modes, mbs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(fmv.data(), d.fmv, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(bmv.data(), d.bmv, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ratio.data(), d.ratio, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ssad.data(), d.sel_sad, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(rsad.data(), d.rate_sad, mbs * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(srate.data(), d.sel_rate, mbs * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // picture totals (src/encoder_motion.cpp:933-1003)
+    uint64_t sel_sad = 0, rate_sad = 0, sel_rate = 1, sel_px = 0;
+    for (size_t i = 0; i < mbs; ++i) {
+        const int mx = static_cast<int>(i % mbw), my = static_cast<int>(i / mbw);
+        sel_sad += ssad[i];
+        rate_sad += rsad[i];
+        sel_rate += srate[i];
+        sel_px += static_cast<uint64_t>(std::max(0, std::min(16, cur->w - mx * 16))) *
+                  static_cast<uint64_t>(std::max(0, std::min(16, cur->h - my * 16)));
+        switch (modes[i]) {
+            case VC1ME_B_FORWARD: ++out->forward_macroblocks; break;
+            case VC1ME_B_BACKWARD: ++out->backward_macroblocks; break;
+            case VC1ME_B_INTERPOLATED: ++out->interpolated_macroblocks; break;
+            case VC1ME_B_DIRECT: ++out->direct_macroblocks; break;
+            default: ++out->intra_macroblocks; break;
+        }
+        const int2 f = fmv[i], k = bmv[i];
+        if (f.x || f.y || k.x || k.y) ++out->moved_macroblocks;
+        auto frac_chroma = [](int2 v) { return (v.x & 3) || (v.y & 3) || ((v.x / 4) & 1) || ((v.y / 4) & 1); };
+        if (frac_chroma(f) || frac_chroma(k)) ++out->fractional_chroma_macroblocks;
+        if (out->modes) out->modes[i] = modes[i];
+        if (out->forward_mvs) { out->forward_mvs[2 * i] = f.x; out->forward_mvs[2 * i + 1] = f.y; }
+        if (out->backward_mvs) { out->backward_mvs[2 * i] = k.x; out->backward_mvs[2 * i + 1] = k.y; }
+        if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = ratio[i];
+    }
+    out->mean_abs_residual = sel_px ? static_cast<double>(sel_sad) / static_cast<double>(sel_px) : 0.0;
+    out->rate_complexity = sel_px ? static_cast<double>(rate_sad) / static_cast<double>(sel_px) : 0.0;
+    out->motion_sad = sel_sad;
+    out->motion_bits = sel_rate;
     return VC1ME_OK;
 }
 

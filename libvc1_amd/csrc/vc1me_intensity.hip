@@ -14,19 +14,21 @@
 namespace vc1me {
 namespace {
 
-// Sums over a 256-thread block of up to four 64-bit values, added to out[]
-// by thread 0.
-__device__ void block_add64(unsigned long long* out, unsigned long long (&v)[4], int count) {
-    __shared__ unsigned long long part[4][4];
+// Sums over a 256-thread block of `count` 64-bit values, written by thread 0
+// to out[0..count) (the block's partials; the host adds the blocks' rows).
+template <int count>
+__device__ void block_sum64(unsigned long long* out, const unsigned long long (&v)[count]) {
+    __shared__ unsigned long long part[4][count];
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
+#pragma unroll
     for (int k = 0; k < count; ++k) {
         unsigned long long x = v[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
         if (lane == 0) part[wave][k] = x;
     }
     __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < count; ++k) atomicAdd(&out[k], part[0][k] + part[1][k] + part[2][k] + part[3][k]);
+    for (int k = static_cast<int>(threadIdx.x); k < count; k += static_cast<int>(blockDim.x))
+        out[k] = part[0][k] + part[1][k] + part[2][k] + part[3][k];
 }
 
 __device__ __forceinline__ int round_qpel_to_pixel(int q) { return q >= 0 ? (q + 2) / 4 : -((-q + 2) / 4); }
@@ -34,8 +36,8 @@ __device__ __forceinline__ int round_qpel_to_pixel(int q) { return q >= 0 ? (q +
 // One sample per (px, py) with px, py multiples of 4 inside the raster (the
 // union over macroblocks of the reference's per-macroblock 4x4 lattice): the
 // reference luma at the macroblock's vector rounded to whole pixels (clamped
-// fetch) and the current luma.  Stores the pair (ref | cur << 8) and adds
-// sum(ref), sum(cur), sum(ref^2), sum(ref*cur) to sums[0..3].
+// fetch) and the current luma.  Stores the pair (ref | cur << 8) and the
+// block's sum(ref), sum(cur), sum(ref^2), sum(ref*cur) at sums[4 * block].
 __global__ void __launch_bounds__(256) k_ic_samples(const uint8_t* __restrict__ cy, const uint8_t* __restrict__ ry,
                                                     int W, int H, const int2* __restrict__ mvs, int mbw,
                                                     uint16_t* __restrict__ samples, unsigned long long* sums) {
@@ -55,15 +57,16 @@ __global__ void __launch_bounds__(256) k_ic_samples(const uint8_t* __restrict__ 
         acc[2] += r * r;
         acc[3] += r * c;
     }
-    block_add64(sums, acc, 4);
+    block_sum64<4>(sums + 4 * blockIdx.x, acc);
 }
 
 // Sampled SAD of every (LUMSCALE, LUMSHIFT) = (l0 + a, z0 + b), a < nl,
 // b < nz (at most 7 x 7): |cur - map(ref)| with map(v) = clamp(((l + 32) * v
 // + 32 >> 6) + z, 0, 255), which is intensity_map_y for l >= 1 (the shift
-// term 64 z passes through the arithmetic shift unchanged).  sads[a * 7 + b].
-__global__ void __launch_bounds__(256) k_ic_sads(const uint16_t* __restrict__ samples, int n, int l0, int nl,
-                                                 int z0, int nz, unsigned long long* sads) {
+// term 64 z passes through the arithmetic shift unchanged).  The block's
+// sums at sads[49 * block + a * 7 + b].
+__global__ void __launch_bounds__(256) k_ic_sads(const uint16_t* __restrict__ samples, int n, int l0, int z0,
+                                                 unsigned long long* sads) {
     uint32_t acc[7][7];
 #pragma unroll
     for (int a = 0; a < 7; ++a)
@@ -83,20 +86,21 @@ __global__ void __launch_bounds__(256) k_ic_sads(const uint16_t* __restrict__ sa
             }
         }
     }
+    // wave sums in 32 bits (a wave's partial is at most 64 x samples per
+    // lane x 255), then the block's four waves in 64 bits
     __shared__ unsigned long long part[4][49];
     const int lane = static_cast<int>(threadIdx.x) & 63, wave = static_cast<int>(threadIdx.x) >> 6;
 #pragma unroll
     for (int a = 0; a < 7; ++a)
 #pragma unroll
         for (int b = 0; b < 7; ++b) {
-            unsigned long long x = acc[a][b];
+            uint32_t x = acc[a][b];
             for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
             if (lane == 0) part[wave][a * 7 + b] = x;
         }
     __syncthreads();
     const int k = static_cast<int>(threadIdx.x);
-    if (k < 49 && (k / 7) < nl && (k % 7) < nz)
-        atomicAdd(&sads[k], part[0][k] + part[1][k] + part[2][k] + part[3][k]);
+    if (k < 49) sads[49 * blockIdx.x + k] = part[0][k] + part[1][k] + part[2][k] + part[3][k];
 }
 
 // dst = map(src) over `words16` 16-byte words of a frame buffer (luma then
@@ -127,18 +131,19 @@ __global__ void __launch_bounds__(256) k_ic_remap(const uint4* __restrict__ src,
 
 }  // namespace
 
+int ic_sample_blocks(int n) { return std::max(1, std::min((n + 255) / 256, 1024)); }
+int ic_sad_blocks(int n) { return std::max(1, std::min((n + 1023) / 1024, 256)); }
+
 hipError_t launch_ic_samples(const uint8_t* cy, const uint8_t* ry, int W, int H, const int2* mvs, int mbw,
                              uint16_t* samples, unsigned long long* sums, hipStream_t s) {
     const int n = ((W + 3) / 4) * ((H + 3) / 4);
-    const int blocks = std::min((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_ic_samples, dim3(blocks), dim3(256), 0, s, cy, ry, W, H, mvs, mbw, samples, sums);
+    hipLaunchKernelGGL(k_ic_samples, dim3(ic_sample_blocks(n)), dim3(256), 0, s, cy, ry, W, H, mvs, mbw, samples,
+                       sums);
     return hipGetLastError();
 }
 
-hipError_t launch_ic_sads(const uint16_t* samples, int n, int l0, int nl, int z0, int nz, unsigned long long* sads,
-                          hipStream_t s) {
-    const int blocks = std::min((n + 256 * 8 - 1) / (256 * 8), 512);
-    hipLaunchKernelGGL(k_ic_sads, dim3(std::max(blocks, 1)), dim3(256), 0, s, samples, n, l0, nl, z0, nz, sads);
+hipError_t launch_ic_sads(const uint16_t* samples, int n, int l0, int z0, unsigned long long* sads, hipStream_t s) {
+    hipLaunchKernelGGL(k_ic_sads, dim3(ic_sad_blocks(n)), dim3(256), 0, s, samples, n, l0, z0, sads);
     return hipGetLastError();
 }
 

@@ -133,15 +133,48 @@ hipError_t launch_score_batch(const uint8_t* cur, const uint8_t* ref, int width,
                               int bw, int bh, int sample_stride, int bilinear, int rnd, int metric,
                               const int2* cands, int n, uint32_t* costs, hipStream_t s);
 
+// B-picture decision (analyze_b_picture_mode after its two core passes):
+// job 0 of the step is the forward pass (cur vs past), job 1 the backward
+// pass (cur vs future); their vectors, screen SADs and phase planes are
+// read in place.
+struct BDesc {
+    const int2* anchor_mvs;      // future anchor's 1-MV field
+    const uint8_t* anchor_4mv;
+    int32_t mode, bfrac, below_half, allow_intra;
+    double lambda;               // max(1, 0.45 * PQINDEX)
+    uint64_t intra_guard;        // llround(lambda * 72)
+    double intra_thr;
+    // per macroblock: costs of direct, forward, backward, interpolated
+    // (invalid markers and the 4-MV anchor penalty applied), the intra
+    // proxy SAD, the direct vectors (forward, backward)
+    uint64_t* cost;
+    uint64_t* intra_sad;
+    int2* direct;
+    // decisions: mode, the final forward and backward fields, the intra
+    // ratio, and the per-macroblock terms of the picture totals
+    uint8_t* modes;
+    int2* fmv;
+    int2* bmv;
+    double* ratio;
+    uint64_t* sel_sad;
+    uint64_t* rate_sad;
+    uint32_t* sel_rate;
+};
+hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int mbs, hipStream_t s);
+hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, hipStream_t s);
+
 // Intensity compensation (vc1me_intensity.hip): the luma table at bytes
 // 0..255 and the chroma table at 256..511.
 struct IntensityLuts {
     uint32_t w[128];
 };
+// The sampler and the SAD grid write one row of partial sums per block (4
+// and 49 values); ic_*_blocks give the block counts for n samples.
+int ic_sample_blocks(int n);
+int ic_sad_blocks(int n);
 hipError_t launch_ic_samples(const uint8_t* cy, const uint8_t* ry, int W, int H, const int2* mvs, int mbw,
                              uint16_t* samples, unsigned long long* sums, hipStream_t s);
-hipError_t launch_ic_sads(const uint16_t* samples, int n, int l0, int nl, int z0, int nz, unsigned long long* sads,
-                          hipStream_t s);
+hipError_t launch_ic_sads(const uint16_t* samples, int n, int l0, int z0, unsigned long long* sads, hipStream_t s);
 hipError_t launch_ic_remap(const uint8_t* src, uint8_t* dst, size_t bytes, size_t luma_bytes,
                            const IntensityLuts& luts, hipStream_t s);
 

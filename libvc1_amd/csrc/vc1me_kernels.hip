@@ -233,8 +233,7 @@ __device__ __forceinline__ bool chroma_block(int x0, int y0, int bw, int bh) {
 // 64-bit significand (round to nearest even).  lambda = m * 2^-k exactly, so
 // the exact sum is N * 2^-k with N = cost*2^k + m*rate; all scores of one MB
 // share k and compare as rounded N.
-__device__ __forceinline__ unsigned __int128 x87_score(uint64_t cost, uint64_t rate, uint64_t m, int k) {
-    unsigned __int128 n = (static_cast<unsigned __int128>(cost) << k) + static_cast<unsigned __int128>(m) * rate;
+__device__ __forceinline__ unsigned __int128 x87_round(unsigned __int128 n) {
     const uint64_t hi = static_cast<uint64_t>(n >> 64);
     if (!hi) return n;
     const int bits = 128 - __builtin_clzll(hi);  // bit length of n
@@ -244,6 +243,9 @@ __device__ __forceinline__ unsigned __int128 x87_score(uint64_t cost, uint64_t r
     const unsigned __int128 half = static_cast<unsigned __int128>(1) << (sh - 1);
     if (rem > half || (rem == half && (q & 1))) q += 1;
     return q << sh;  // q may have become 2^64: still the correctly rounded value
+}
+__device__ __forceinline__ unsigned __int128 x87_score(uint64_t cost, uint64_t rate, uint64_t m, int k) {
+    return x87_round((static_cast<unsigned __int128>(cost) << k) + static_cast<unsigned __int128>(m) * rate);
 }
 
 // ------------------------------------------------------------------ LDS layout
@@ -3052,9 +3054,241 @@ __global__ void __launch_bounds__(64) k_score_batch(const uint8_t* __restrict__ 
     costs[i] = total;
 }
 
+
+// ------------------------------------------------------------------ B pictures
+// Vc1Encoder::analyze_b_picture_mode (src/encoder_motion.cpp:753-1004) after
+// its two core passes: job 0 = forward (cur vs past), job 1 = backward (cur
+// vs future).
+
+// motion_uses_hidden_chroma with the B future-reference rounding (future:
+// odd chroma components round away from zero, src/encoder_motion.cpp:2176-2184)
+__device__ __forceinline__ bool hidden_chroma_future(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq) {
+    const int cw = (g.vw + 1) / 2, ch = (g.vh + 1) / 2;
+    const bool rp = cw < g.mbw * 8, bp = ch < g.mbh * 8;
+    if (!rp && !bp) return false;
+    const int cx = x0 / 2, cy = y0 / 2;
+    const int aw = imax(0, imin(bw / 2, cw - cx)), ah = imax(0, imin(bh / 2, ch - cy));
+    if (!aw || !ah) return false;
+    int uvx = arshift(xq + ((xq & 3) == 3), 1), uvy = arshift(yq + ((yq & 3) == 3), 1);
+    uvx = uvx < 0 ? uvx - (uvx & 1) : uvx + (uvx & 1);
+    uvy = uvy < 0 ? uvy - (uvy & 1) : uvy + (uvy & 1);
+    const int ix = floor_div4(uvx), iy = floor_div4(uvy);
+    const bool fx = (uvx - ix * 4) != 0, fy = (uvy - iy * 4) != 0;
+    return (rp && cx + aw - 1 + ix + (fx ? 1 : 0) >= cw) || (bp && cy + ah - 1 + iy + (fy ? 1 : 0) >= ch);
+}
+__device__ __forceinline__ bool b_unsafe(const Geo& g, int x0, int y0, int2 v, int mode, bool future) {
+    return hidden_luma(g, x0, y0, 16, 16, v.x, v.y, mode) ||
+           (future ? hidden_chroma_future(g, x0, y0, 16, 16, v.x, v.y) : hidden_chroma(g, x0, y0, 16, 16, v.x, v.y));
+}
+
+// the direct-mode vector: the collocated anchor vector scaled by bfrac / 256
+// (or bfrac - 256), half-pel rounded in half-pel modes, clamped to the
+// picture's MV bounds (src/encoder_motion.cpp:815-833)
+__device__ __forceinline__ int2 b_scale_direct(int2 col, bool inverse, int bfrac, int mode, int mx, int my, int mbw,
+                                               int mbh) {
+    const int n = inverse ? bfrac - 256 : bfrac;
+    int2 mv;
+    if (mode_halfpel(mode)) {
+        mv.x = 2 * arshift(col.x * n + 255, 9);
+        mv.y = 2 * arshift(col.y * n + 255, 9);
+    } else {
+        mv.x = arshift(col.x * n + 128, 8);
+        mv.y = arshift(col.y * n + 128, 8);
+    }
+    mv.x = clampi(mv.x, -60 - mx * 64, mbw * 64 - 4 - mx * 64);
+    mv.y = clampi(mv.y, -60 - my * 64, mbh * 64 - 4 - my * 64);
+    return mv;
+}
+
+// Four predicted samples (row r of the macroblock, columns c0..c0+3) of
+// reference job R at vector v: the job's phase plane of v's phase.
+__device__ __forceinline__ uint32_t b_pred4(const JobDesc* R, int mode, int x, int y, int2 v) {
+    const bool bil = mode_bilinear(mode);
+    const int ix = floor_div4(v.x), iy = floor_div4(v.y);
+    const int hm = v.x - ix * 4, vm = v.y - iy * 4;
+    const int phase = bil ? (vm >> 1) * 2 + (hm >> 1) : vm * 4 + hm;
+    const gptr_u8 pl = as_global(bil ? R->bp : R->qp) + static_cast<size_t>(phase) * R->pp * R->ph;
+    const int yc = clampi(y + iy, -2, R->H) + 2;
+    return plane_row4(pl + static_cast<size_t>(yc) * R->pp, x + ix, R->W);
+}
+
+// rounding average of packed bytes, (a + b + 1) >> 1 per byte
+__device__ __forceinline__ uint32_t avg_u8x4(uint32_t a, uint32_t b) { return (a | b) - (((a ^ b) >> 1) & 0x7f7f7f7fu); }
+
+// One 64-lane wave per macroblock: the direct vectors, the four prediction
+// SADs over the visible pixels (each lane four pixels of one row), their
+// hidden-padding screens, and the B-intra proxy (the four 8x8 blocks' SAD
+// about their visible means).
+__global__ void __launch_bounds__(64) k_b_costs(const JobDesc* __restrict__ jobs, const BDesc* __restrict__ B) {
+    const JobDesc* F = &jobs[0];
+    const JobDesc* K = &jobs[1];
+    const int pos = static_cast<int>(blockIdx.x);
+    const int mbw = F->mbw, mbh = F->mbh, W = F->W, H = F->H, mode = B->mode;
+    const int mx = pos % mbw, my = pos / mbw, x0 = mx * 16, y0 = my * 16;
+    const int lane = static_cast<int>(threadIdx.x);
+    const Geo g = {F->W, F->H, F->vw, F->vh, F->mbw, F->mbh};
+    const int2 col = B->anchor_mvs[pos];
+    const int2 df = b_scale_direct(col, false, B->bfrac, mode, mx, my, mbw, mbh);
+    const int2 db = b_scale_direct(col, true, B->bfrac, mode, mx, my, mbw, mbh);
+    const int2 fm = F->mvs[pos], bm = K->mvs[pos];
+    const int vw = imin(16, W - x0), vh = imin(16, H - y0);
+    const int r = lane >> 2, c0 = (lane & 3) * 4;
+    uint32_t cur = 0, cm = 0;
+    if (r < vh) {
+        const gptr_u8 row = as_global(F->cy) + static_cast<size_t>(y0 + r) * W + x0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (c0 + j < vw) { cur |= static_cast<uint32_t>(row[c0 + j]) << (8 * j); cm |= 0xffu << (8 * j); }
+    }
+    uint32_t sf = 0, sb = 0, si = 0, sd = 0;
+    if (r < vh) {
+        const uint32_t pf = b_pred4(F, mode, x0 + c0, y0 + r, fm) & cm;
+        const uint32_t pb = b_pred4(K, mode, x0 + c0, y0 + r, bm) & cm;
+        const uint32_t pdf = b_pred4(F, mode, x0 + c0, y0 + r, df) & cm;
+        const uint32_t pdb = b_pred4(K, mode, x0 + c0, y0 + r, db) & cm;
+        sf = __builtin_amdgcn_sad_u8(pf, cur, 0u);
+        sb = __builtin_amdgcn_sad_u8(pb, cur, 0u);
+        si = __builtin_amdgcn_sad_u8(avg_u8x4(pf, pb), cur, 0u);
+        sd = __builtin_amdgcn_sad_u8(avg_u8x4(pdf, pdb), cur, 0u);
+    }
+    sf = wave_sum32(sf); sb = wave_sum32(sb); si = wave_sum32(si); sd = wave_sum32(sd);
+    // visible_block_mean (src/encoder_entropy.cpp:496-509) of each 8x8 block and the SAD about it
+    uint64_t intra = 0;
+    const int lx = lane & 7, ly = lane >> 3;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int bx = x0 + (k & 1) * 8, by = y0 + ((k >> 1) & 1) * 8;
+        const bool in = bx + lx < W && by + ly < H;
+        const uint32_t v = in ? F->cy[static_cast<size_t>(by + ly) * W + bx + lx] : 0u;
+        const uint32_t sum = wave_sum32(v), n = wave_sum32(in ? 1u : 0u);
+        const int mean = n ? static_cast<int>((sum + n / 2) / n) : 128;
+        intra += wave_sum32(in ? static_cast<uint32_t>(iabs(static_cast<int>(v) - mean)) : 0u);
+    }
+    if (lane == 0) {
+        constexpr uint64_t kInvalid = uint64_t{1} << 54;
+        const bool ok_fm = !b_unsafe(g, x0, y0, fm, mode, false), ok_bm = !b_unsafe(g, x0, y0, bm, mode, true);
+        const bool ok_df = !b_unsafe(g, x0, y0, df, mode, false), ok_db = !b_unsafe(g, x0, y0, db, mode, true);
+        const uint64_t pen = B->anchor_4mv[pos] ? 1000000000ull : 0ull;
+        uint64_t* c = B->cost + 4 * static_cast<size_t>(pos);
+        c[0] = (ok_df && ok_db ? sd : kInvalid) + pen;
+        c[1] = (ok_fm ? sf : kInvalid) + pen;
+        c[2] = (ok_bm ? sb : kInvalid) + pen;
+        c[3] = ok_fm && ok_bm ? si : kInvalid;
+        B->intra_sad[pos] = intra;
+        B->direct[2 * pos] = df;
+        B->direct[2 * pos + 1] = db;
+    }
+}
+
+// predictor_info(...).pre over a final B field (src/encoder_motion.cpp:1312-1352):
+// the median of the top (A), top-right (B; top-left in the last column) and
+// left (C) vectors, pulled inside the picture's MV bounds
+__device__ __forceinline__ int2 b_predictor(const int2* field, int mbw, int mbh, int mx, int my) {
+    const bool a_valid = my > 0, c_valid = mx > 0, b_valid = my > 0 && mbw > 1;
+    const int2 z = make_int2(0, 0);
+    const int2 A = a_valid ? field[(my - 1) * mbw + mx] : z;
+    const int2 Cv = c_valid ? field[my * mbw + mx - 1] : z;
+    const int2 Bv = b_valid ? field[(my - 1) * mbw + ((mx == mbw - 1) ? mx - 1 : mx + 1)] : z;
+    int2 pre = a_valid ? A : (c_valid ? Cv : (b_valid ? Bv : z));
+    if ((a_valid ? 1 : 0) + (b_valid ? 1 : 0) + (c_valid ? 1 : 0) > 1) {
+        pre.x = median3(A.x, Bv.x, Cv.x);
+        pre.y = median3(A.y, Bv.y, Cv.y);
+    }
+    const int qx = mx * 64, qy = my * 64, X = mbw * 64 - 4, Y = mbh * 64 - 4;
+    if (qx + pre.x < -60) pre.x = -60 - qx;
+    if (qx + pre.x > X) pre.x = X - qx;
+    if (qy + pre.y < -60) pre.y = -60 - qy;
+    if (qy + pre.y > Y) pre.y = Y - qy;
+    return pre;
+}
+
+// (long double)cost + (double)(lambda * bits), rounded to a 64-bit
+// significand, as an integer multiple of 2^-52 (lambda >= 1, so every
+// nonzero double product is a multiple of 2^-52 below 2^11)
+__device__ __forceinline__ unsigned __int128 b_score(uint64_t cost, uint64_t bits, double lambda) {
+    const double prod = __dmul_rn(lambda, static_cast<double>(bits));
+    const uint64_t p52 = static_cast<uint64_t>(__dmul_rn(prod, 4503599627370496.0));
+    return x87_round((static_cast<unsigned __int128>(cost) << 52) + p52);
+}
+
+// The raster-order decision: macroblock (mx, my) needs the final fields of
+// its left, top-left, top and top-right neighbours, so the macroblocks of
+// anti-diagonal t = mx + 2 my are independent once t - 1 is done.  One
+// workgroup walks the diagonals; one thread decides one macroblock.
+__global__ void __launch_bounds__(256) k_b_decide(const JobDesc* __restrict__ jobs, const BDesc* __restrict__ B) {
+    const JobDesc* F = &jobs[0];
+    const JobDesc* K = &jobs[1];
+    const int mbw = F->mbw, mbh = F->mbh, mode = B->mode;
+    const int tmax = (mbw - 1) + 2 * (mbh - 1);
+    const double lambda = B->lambda;
+    const uint32_t marker = imax(static_cast<int>(mv_bits0(35)), static_cast<int>(mv_bits0(72))) + 1;
+    for (int t = 0; t <= tmax; ++t) {
+        const int my_lo = imax(0, (t - (mbw - 1) + 1) / 2), my_hi = imin(mbh - 1, t / 2);
+        for (int my = my_lo + static_cast<int>(threadIdx.x); my <= my_hi; my += static_cast<int>(blockDim.x)) {
+            const int mx = t - 2 * my;
+            const int pos = my * mbw + mx;
+            const int2 fm = F->mvs[pos], bm = K->mvs[pos];
+            const int2 df = B->direct[2 * pos], db = B->direct[2 * pos + 1];
+            const uint64_t* c = B->cost + 4 * static_cast<size_t>(pos);
+            const int2 fp = b_predictor(B->fmv, mbw, mbh, mx, my);
+            const int2 bp = b_predictor(B->bmv, mbw, mbh, mx, my);
+            const uint32_t fbits = mv_rate_bits(F->mvrange, fm.x, fm.y, fp.x, fp.y, mode, false);
+            const uint32_t bbits = mv_rate_bits(F->mvrange, bm.x, bm.y, bp.x, bp.y, mode, false);
+            const uint32_t ibits = mv_rate_bits(F->mvrange, bm.x, bm.y, bp.x, bp.y, mode, true) + fbits + 2;
+            const uint32_t ftype = B->below_half ? 1u : 2u, btype = B->below_half ? 2u : 1u;
+            const uint64_t cost[4] = {c[0], c[1], c[2], c[3]};
+            const uint32_t rate[4] = {0u, fbits + ftype, bbits + btype, ibits};
+            int best = 0;
+            uint64_t best_cost = cost[0];
+            uint32_t best_rate = rate[0];
+            unsigned __int128 bs = b_score(cost[0], rate[0], lambda);
+#pragma unroll
+            for (int k = 1; k < 4; ++k) {
+                const unsigned __int128 sc = b_score(cost[k], rate[k], lambda);
+                if (sc < bs) { bs = sc; best = k; best_cost = cost[k]; best_rate = rate[k]; }
+            }
+            const uint64_t isad = B->intra_sad[pos];
+            uint64_t gate = best_cost;
+            if (F->screen[pos] < gate) gate = F->screen[pos];
+            if (K->screen[pos] < gate) gate = K->screen[pos];
+            const uint64_t guarded = isad + B->intra_guard;
+            B->ratio[pos] = gate > 0 ? __ddiv_rn(static_cast<double>(guarded), static_cast<double>(gate))
+                                     : __builtin_huge_val();
+            const bool intra = B->allow_intra &&
+                               guarded < static_cast<uint64_t>(llround_exact(__dmul_rn(static_cast<double>(gate),
+                                                                                       B->intra_thr)));
+            // candidate order: direct, forward, backward, interpolated
+            int2 fo = make_int2(0, 0), bo = make_int2(0, 0);
+            uint8_t m = 4;
+            if (!intra) {
+                m = best == 0 ? 3 : static_cast<uint8_t>(best - 1);   // direct, forward, backward, interpolated
+                fo = (best == 1 || best == 3) ? fm : df;
+                bo = (best == 2 || best == 3) ? bm : db;
+            }
+            B->modes[pos] = m;
+            B->fmv[pos] = fo;
+            B->bmv[pos] = bo;
+            B->sel_sad[pos] = intra ? isad : best_cost;
+            B->rate_sad[pos] = best_cost;
+            B->sel_rate[pos] = intra ? marker : best_rate;
+        }
+        __syncthreads();
+    }
+}
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+
+hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int mbs, hipStream_t s) {
+    if (mbs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_b_costs, dim3(mbs), dim3(64), 0, s, d_jobs, d_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, hipStream_t s) {
+    hipLaunchKernelGGL(k_b_decide, dim3(1), dim3(256), 0, s, d_jobs, d_b);
+    return hipGetLastError();
+}
 
 int mb_kernel_lds_bytes(int local_range) { return lds_bytes_for(local_range); }
 

@@ -16,6 +16,7 @@
 //   luma_mc_sample_mode      src/encoder_motion_compensation.cpp:398-415
 //   estimate_intensity_comp  src/encoder_motion_compensation.cpp:296-348
 //   intensity_compensated_reference  src/encoder_motion_compensation.cpp:195-294
+//   analyze_b_picture_mode   src/encoder_motion.cpp:753-1004
 #include "encoder_internal.h"
 
 #include "../include/vc1me.h"
@@ -240,6 +241,71 @@ int ref_intensity_compensate(const vc1me_planes* ref, const vc1me_intensity* ic,
         std::memcpy(y, out.y.data(), out.y.size());
         std::memcpy(u, out.u.data(), out.u.size());
         std::memcpy(v, out.v.data(), out.v.size());
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int ref_analyze_b(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* past,
+                  const vc1me_planes* future, const vc1me_b_params* b, vc1me_b_result* out) {
+    try {
+        EncoderConfig cfg = make_config(p, false);
+        cfg.debug_disable_b_intra = b->allow_intra == 0;
+        if (b->zero_motion_shortcut) {
+            cfg.fade_compensation = false;
+            cfg.loop_filter = false;
+            cfg.variable_transforms = false;
+            cfg.trellis = 0;
+            cfg.adaptive_quality = false;
+        }
+        Vc1Encoder enc(cfg);
+        const size_t mbs = static_cast<size_t>((p->width + 15) / 16) * ((p->height + 15) / 16);
+        std::vector<Vc1Encoder::MotionVector> amv(mbs);
+        std::vector<uint8_t> a4(mbs);
+        for (size_t i = 0; i < mbs; ++i) {
+            amv[i].xq = b->future_anchor_mvs[2 * i];
+            amv[i].yq = b->future_anchor_mvs[2 * i + 1];
+            a4[i] = b->future_anchor_4mv[i];
+        }
+        Vc1Encoder::IntensityComp ic;
+        ic.enabled = b->forward_intensity.enabled != 0;
+        ic.lumscale = static_cast<uint8_t>(b->forward_intensity.lumscale);
+        ic.lumshift = static_cast<uint8_t>(b->forward_intensity.lumshift);
+        const auto a = enc.analyze_b_picture_mode(make_frame(cur), make_frame(past), make_frame(future), amv, a4, ic,
+                                                  b->fraction_num, b->fraction_den,
+                                                  static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode), nullptr,
+                                                  nullptr, nullptr);
+        auto dbl = [mbs](const std::vector<double>& v, double* o) {
+            if (o) for (size_t i = 0; i < mbs; ++i) o[i] = v.size() == mbs ? v[i] : -1.0;
+        };
+        auto u8 = [mbs](const std::vector<uint8_t>& v, uint8_t* o) {
+            if (o) for (size_t i = 0; i < mbs; ++i) o[i] = v.size() == mbs ? v[i] : 0;
+        };
+        for (size_t i = 0; i < mbs; ++i) {
+            if (out->modes) out->modes[i] = static_cast<uint8_t>(a.modes[i]);
+            if (out->forward_mvs) { out->forward_mvs[2 * i] = a.forward_mvs[i].xq; out->forward_mvs[2 * i + 1] = a.forward_mvs[i].yq; }
+            if (out->backward_mvs) { out->backward_mvs[2 * i] = a.backward_mvs[i].xq; out->backward_mvs[2 * i + 1] = a.backward_mvs[i].yq; }
+        }
+        dbl(a.inter_intra_cost_ratio, out->inter_intra_cost_ratio);
+        dbl(a.forward_distant_local_mae, out->forward_distant_local_mae);
+        dbl(a.forward_distant_candidate_mae, out->forward_distant_candidate_mae);
+        u8(a.forward_distant_match_decision, out->forward_distant_match_decision);
+        dbl(a.backward_distant_local_mae, out->backward_distant_local_mae);
+        dbl(a.backward_distant_candidate_mae, out->backward_distant_candidate_mae);
+        u8(a.backward_distant_match_decision, out->backward_distant_match_decision);
+        out->forward_macroblocks = a.forward_macroblocks;
+        out->backward_macroblocks = a.backward_macroblocks;
+        out->interpolated_macroblocks = a.interpolated_macroblocks;
+        out->direct_macroblocks = a.direct_macroblocks;
+        out->intra_macroblocks = a.intra_macroblocks;
+        out->moved_macroblocks = a.moved_macroblocks;
+        out->fractional_chroma_macroblocks = a.fractional_chroma_macroblocks;
+        out->mean_abs_residual = a.mean_abs_residual;
+        out->rate_complexity = a.rate_complexity;
+        out->motion_sad = a.motion_sad;
+        out->motion_bits = a.motion_bits;
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();

@@ -1861,3 +1861,217 @@ int oracle_estimate_intensity_comp(const vc1me_planes* cur, const vc1me_planes* 
     out->lumshift = best_sh < 0 ? best_sh + 64 : best_sh;
     return 0;
 }
+
+/* ------------------------------------------------------------------ B pictures */
+
+static void p_result_alloc(vc1me_p_result* r, size_t mbs) {
+    memset(r, 0, sizeof *r);
+    r->mvs = (int32_t*)calloc(mbs * 2, sizeof(int32_t));
+    r->block_mvs = (int32_t*)calloc(mbs * 8, sizeof(int32_t));
+    r->use_4mv = (uint8_t*)calloc(mbs, 1);
+    r->use_intra = (uint8_t*)calloc(mbs, 1);
+    r->screen_sads = (uint64_t*)calloc(mbs, sizeof(uint64_t));
+    r->inter_intra_cost_ratio = (double*)calloc(mbs, sizeof(double));
+    r->distant_local_mae = (double*)calloc(mbs, sizeof(double));
+    r->distant_candidate_mae = (double*)calloc(mbs, sizeof(double));
+    r->distant_match_decision = (uint8_t*)calloc(mbs, 1);
+}
+
+static void p_result_free(vc1me_p_result* r) {
+    free(r->mvs); free(r->block_mvs); free(r->use_4mv); free(r->use_intra); free(r->screen_sads);
+    free(r->inter_intra_cost_ratio); free(r->distant_local_mae); free(r->distant_candidate_mae);
+    free(r->distant_match_decision);
+}
+
+/* The direct-mode vector of a macroblock: the future anchor's collocated
+ * vector scaled by the temporal fraction (bfrac / 256, or bfrac - 256 for
+ * the backward half), rounded on the half-pel grid in half-pel modes, then
+ * clamped to the picture's MV bounds (src/encoder_motion.cpp:815-833). */
+static mv_t b_scale_direct(const ctx_t* c, mv_t col, bool inverse, int bfrac, int mode, int mx, int my) {
+    const int n = inverse ? bfrac - 256 : bfrac;
+    mv_t mv;
+    if (mode_halfpel(mode)) {
+        mv.xq = 2 * arshift(col.xq * n + 255, 9);
+        mv.yq = 2 * arshift(col.yq * n + 255, 9);
+    } else {
+        mv.xq = arshift(col.xq * n + 128, 8);
+        mv.yq = arshift(col.yq * n + 128, 8);
+    }
+    mv.xq = clampi(mv.xq, -60 - mx * 64, c->mbw * 64 - 4 - mx * 64);
+    mv.yq = clampi(mv.yq, -60 - my * 64, c->mbh * 64 - 4 - my * 64);
+    return mv;
+}
+
+/* Vc1Encoder::analyze_b_picture_mode (src/encoder_motion.cpp:753-1004,
+ * Advanced profile): the forward and backward core passes (no intra), then
+ * per macroblock in raster order the four prediction SADs, the mode choice
+ * by (long double)SAD + (double)(lambda * bits) with lambda = max(1, 0.45 *
+ * PQINDEX) against the predictors of the final B fields, and the B-intra
+ * decision with its 20% hysteresis. */
+int oracle_analyze_b_frame(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* past,
+                           const vc1me_planes* future, const vc1me_b_params* b, vc1me_b_result* out) {
+    const int fn = b->fraction_num, fd = b->fraction_den;
+    if (!((fd == 2 && fn == 1) || (fd == 3 && (fn == 1 || fn == 2)))) return -1;
+    if (p->mode == VC1ME_MODE_MIXED_MV || p->me_quality < 1 || p->me_quality > 2) return -1;
+    ctx_t cx;
+    init_ctx(&cx, p, cur, past, NULL);
+    const ctx_t* c = &cx;
+    const int mbw = c->mbw, mbh = c->mbh, mode = p->mode;
+    const size_t mbs = (size_t)mbw * mbh;
+    mv_t* fout = (mv_t*)calloc(mbs, sizeof(mv_t));
+    mv_t* bout = (mv_t*)calloc(mbs, sizeof(mv_t));
+    out->forward_macroblocks = out->backward_macroblocks = out->interpolated_macroblocks = 0;
+    out->direct_macroblocks = out->intra_macroblocks = out->moved_macroblocks = 0;
+    out->fractional_chroma_macroblocks = 0;
+    out->mean_abs_residual = out->rate_complexity = 0.0;
+    out->motion_sad = out->motion_bits = 0;
+    if (b->zero_motion_shortcut && p->motion_search_range == 0) {
+        for (size_t i = 0; i < mbs; ++i) {
+            if (out->modes) out->modes[i] = VC1ME_B_FORWARD;
+            if (out->forward_mvs) { out->forward_mvs[2 * i] = 0; out->forward_mvs[2 * i + 1] = 0; }
+            if (out->backward_mvs) { out->backward_mvs[2 * i] = 0; out->backward_mvs[2 * i + 1] = 0; }
+            if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = -1.0;
+            if (out->forward_distant_local_mae) out->forward_distant_local_mae[i] = -1.0;
+            if (out->forward_distant_candidate_mae) out->forward_distant_candidate_mae[i] = -1.0;
+            if (out->forward_distant_match_decision) out->forward_distant_match_decision[i] = 0;
+            if (out->backward_distant_local_mae) out->backward_distant_local_mae[i] = -1.0;
+            if (out->backward_distant_candidate_mae) out->backward_distant_candidate_mae[i] = -1.0;
+            if (out->backward_distant_match_decision) out->backward_distant_match_decision[i] = 0;
+        }
+        out->forward_macroblocks = mbs;
+        free(fout); free(bout);
+        return 0;
+    }
+    /* the forward reference carries the anchor's intensity compensation */
+    vc1me_planes pw = *past;
+    uint8_t *wy = NULL, *wu = NULL, *wv = NULL;
+    if (b->forward_intensity.enabled) {
+        const size_t ys = (size_t)past->width * past->height, cs = (size_t)(past->width / 2) * (past->height / 2);
+        wy = (uint8_t*)malloc(ys); wu = (uint8_t*)malloc(cs); wv = (uint8_t*)malloc(cs);
+        oracle_intensity_compensate(past, &b->forward_intensity, wy, wu, wv);
+        pw.y = wy; pw.u = wu; pw.v = wv;
+    }
+    vc1me_params q = *p;
+    q.allow_intra = 0;
+    vc1me_p_result fwd, bwd;
+    p_result_alloc(&fwd, mbs);
+    p_result_alloc(&bwd, mbs);
+    int rc = analyze_core(&q, cur, &pw, NULL, &fwd, NULL);
+    if (!rc) rc = analyze_core(&q, cur, future, NULL, &bwd, NULL);
+    if (rc) goto cleanup;
+    for (size_t i = 0; i < mbs; ++i) {
+        if (out->forward_distant_local_mae) out->forward_distant_local_mae[i] = fwd.distant_local_mae[i];
+        if (out->forward_distant_candidate_mae) out->forward_distant_candidate_mae[i] = fwd.distant_candidate_mae[i];
+        if (out->forward_distant_match_decision) out->forward_distant_match_decision[i] = fwd.distant_match_decision[i];
+        if (out->backward_distant_local_mae) out->backward_distant_local_mae[i] = bwd.distant_local_mae[i];
+        if (out->backward_distant_candidate_mae) out->backward_distant_candidate_mae[i] = bwd.distant_candidate_mae[i];
+        if (out->backward_distant_match_decision) out->backward_distant_match_decision[i] = bwd.distant_match_decision[i];
+    }
+    {
+        const int bfrac = fd == 2 ? 128 : (fn == 1 ? 85 : 170);
+        const bool below_half = 2 * fn < fd;
+        const double lambda = fmax(1.0, 0.45 * (double)p->pqindex);
+        const uint64_t intra_guard = (uint64_t)llround(lambda * 72.0);
+        const uint64_t intra_marker_bits = (uint64_t)imax(kMvBits0[35], kMvBits0[72]) + 1u;
+        const uint64_t invalid = (uint64_t)1 << 54;
+        uint64_t sel_sad = 0, rate_sad = 0, sel_rate = 1, sel_px = 0;
+        const int W = c->W, H = c->H;
+        for (int my = 0; my < mbh; ++my)
+            for (int mx = 0; mx < mbw; ++mx) {
+                const size_t pos = (size_t)my * mbw + mx;
+                const mv_t col = {b->future_anchor_mvs[2 * pos], b->future_anchor_mvs[2 * pos + 1]};
+                const mv_t df = b_scale_direct(c, col, false, bfrac, mode, mx, my);
+                const mv_t db = b_scale_direct(c, col, true, bfrac, mode, mx, my);
+                const mv_t fm = {fwd.mvs[2 * pos], fwd.mvs[2 * pos + 1]};
+                const mv_t bm = {bwd.mvs[2 * pos], bwd.mvs[2 * pos + 1]};
+                const int x0 = mx * 16, y0 = my * 16;
+                const int vw = imin(16, W - x0), vh = imin(16, H - y0);
+                const bool safe_fm = !(hidden_luma(c, x0, y0, 16, 16, fm, mode) || hidden_chroma(c, x0, y0, 16, 16, fm, false));
+                const bool safe_bm = !(hidden_luma(c, x0, y0, 16, 16, bm, mode) || hidden_chroma(c, x0, y0, 16, 16, bm, true));
+                const bool safe_df = !(hidden_luma(c, x0, y0, 16, 16, df, mode) || hidden_chroma(c, x0, y0, 16, 16, df, false));
+                const bool safe_db = !(hidden_luma(c, x0, y0, 16, 16, db, mode) || hidden_chroma(c, x0, y0, 16, 16, db, true));
+                uint64_t cf = 0, cb = 0, ci = 0, cdir = 0;
+                for (int y = 0; y < vh; ++y)
+                    for (int x = 0; x < vw; ++x) {
+                        const int src = cur->y[(size_t)(y0 + y) * W + x0 + x];
+                        const int pf = luma_mc_sample_mode(pw.y, W, H, x0 + x, y0 + y, fm.xq, fm.yq, mode, 0);
+                        const int pb = luma_mc_sample_mode(future->y, W, H, x0 + x, y0 + y, bm.xq, bm.yq, mode, 0);
+                        const int pdf = luma_mc_sample_mode(pw.y, W, H, x0 + x, y0 + y, df.xq, df.yq, mode, 0);
+                        const int pdb = luma_mc_sample_mode(future->y, W, H, x0 + x, y0 + y, db.xq, db.yq, mode, 0);
+                        cf += (uint64_t)iabs(src - pf);
+                        cb += (uint64_t)iabs(src - pb);
+                        ci += (uint64_t)iabs(src - ((pf + pb + 1) >> 1));
+                        cdir += (uint64_t)iabs(src - ((pdf + pdb + 1) >> 1));
+                    }
+                if (!safe_fm) cf = invalid;
+                if (!safe_bm) cb = invalid;
+                if (!safe_fm || !safe_bm) ci = invalid;
+                if (!safe_df || !safe_db) cdir = invalid;
+                const mv_t fpred = predictor_info(c, fout, mx, my).pre;
+                const mv_t bpred = predictor_info(c, bout, mx, my).pre;
+                const uint64_t fbits = mv_rate_bits(c, fm, fpred, mode, false);
+                const uint64_t bbits = mv_rate_bits(c, bm, bpred, mode, false);
+                const uint64_t ibits = mv_rate_bits(c, bm, bpred, mode, true) + mv_rate_bits(c, fm, fpred, mode, false) + 2;
+                const uint64_t ftype = below_half ? 1u : 2u, btype = below_half ? 2u : 1u;
+                const uint64_t pen = b->future_anchor_4mv[pos] ? 1000000000ull : 0ull;
+                const int cmode[4] = {VC1ME_B_DIRECT, VC1ME_B_FORWARD, VC1ME_B_BACKWARD, VC1ME_B_INTERPOLATED};
+                const uint64_t ccost[4] = {cdir + pen, cf + pen, cb + pen, ci};
+                const uint64_t crate[4] = {0, fbits + ftype, bbits + btype, ibits};
+                int best = 0;
+                long double best_score = (long double)ccost[0] + lambda * (double)crate[0];
+                for (int k = 0; k < 4; ++k) {
+                    const long double s = (long double)ccost[k] + lambda * (double)crate[k];
+                    if (s < best_score) { best = k; best_score = s; }
+                }
+                uint64_t intra_sad = 0;
+                for (int k = 0; k < 4; ++k) {
+                    const int bx = x0 + (k & 1) * 8, by = y0 + ((k >> 1) & 1) * 8;
+                    const int mean = visible_block_mean(cur->y, W, H, bx, by);
+                    for (int yy = 0; yy < 8 && by + yy < H; ++yy)
+                        for (int xx = 0; xx < 8 && bx + xx < W; ++xx)
+                            intra_sad += (uint64_t)iabs((int)cur->y[(size_t)(by + yy) * W + bx + xx] - mean);
+                }
+                uint64_t gate = ccost[best];
+                if (fwd.screen_sads[pos] < gate) gate = fwd.screen_sads[pos];
+                if (bwd.screen_sads[pos] < gate) gate = bwd.screen_sads[pos];
+                const double ratio = gate > 0 ? (double)(intra_sad + intra_guard) / (double)gate : INFINITY;
+                if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[pos] = ratio;
+                const bool intra = b->allow_intra &&
+                                   intra_sad + intra_guard < (uint64_t)llround((double)gate * p->inter_intra_threshold);
+                sel_sad += intra ? intra_sad : ccost[best];
+                rate_sad += ccost[best];
+                sel_rate += intra ? intra_marker_bits : crate[best];
+                sel_px += (uint64_t)imax(0, imin(16, W - x0)) * (uint64_t)imax(0, imin(16, H - y0));
+                mv_t fo = {0, 0}, bo = {0, 0};
+                int m = VC1ME_B_INTRA;
+                if (intra) {
+                    ++out->intra_macroblocks;
+                } else {
+                    m = cmode[best];
+                    if (m == VC1ME_B_DIRECT) { fo = df; bo = db; ++out->direct_macroblocks; }
+                    else if (m == VC1ME_B_FORWARD) { fo = fm; bo = db; ++out->forward_macroblocks; }
+                    else if (m == VC1ME_B_BACKWARD) { fo = df; bo = bm; ++out->backward_macroblocks; }
+                    else { fo = fm; bo = bm; ++out->interpolated_macroblocks; }
+                }
+                fout[pos] = fo;
+                bout[pos] = bo;
+                if (out->modes) out->modes[pos] = (uint8_t)m;
+                if (out->forward_mvs) { out->forward_mvs[2 * pos] = fo.xq; out->forward_mvs[2 * pos + 1] = fo.yq; }
+                if (out->backward_mvs) { out->backward_mvs[2 * pos] = bo.xq; out->backward_mvs[2 * pos + 1] = bo.yq; }
+                if (fo.xq || fo.yq || bo.xq || bo.yq) ++out->moved_macroblocks;
+#define FRAC_CHROMA(v) (((v).xq & 3) || ((v).yq & 3) || (((v).xq / 4) & 1) || (((v).yq / 4) & 1))
+                if (FRAC_CHROMA(fo) || FRAC_CHROMA(bo)) ++out->fractional_chroma_macroblocks;
+#undef FRAC_CHROMA
+            }
+        out->mean_abs_residual = sel_px ? (double)sel_sad / (double)sel_px : 0.0;
+        out->rate_complexity = sel_px ? (double)rate_sad / (double)sel_px : 0.0;
+        out->motion_sad = sel_sad;
+        out->motion_bits = sel_rate;
+    }
+cleanup:
+    p_result_free(&fwd);
+    p_result_free(&bwd);
+    free(wy); free(wu); free(wv);
+    free(fout); free(bout);
+    return rc;
+}

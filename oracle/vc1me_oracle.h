@@ -98,6 +98,11 @@ void oracle_intensity_compensate(const vc1me_planes* ref, const vc1me_intensity*
 int oracle_estimate_intensity_comp(const vc1me_planes* cur, const vc1me_planes* ref, const int32_t* mvs,
                                    vc1me_intensity* out);
 
+/* analyze_b_picture_mode (src/encoder_motion.cpp:753-1004) for the 1-MV mode
+ * p->mode; -1 for an unsupported fraction or mode */
+int oracle_analyze_b_frame(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* past,
+                           const vc1me_planes* future, const vc1me_b_params* b, vc1me_b_result* out);
+
 #ifdef __cplusplus
 }
 #endif

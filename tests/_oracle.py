@@ -62,6 +62,8 @@ def oracle() -> C.CDLL:
         lib.oracle_intensity_map_y.restype = C.c_uint8
         lib.oracle_intensity_map_uv.argtypes = [C.c_uint8, P(abi.Intensity)]
         lib.oracle_intensity_map_uv.restype = C.c_uint8
+        lib.oracle_analyze_b_frame.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.Planes),
+                                               P(abi.BParams), P(abi.BResult)]
         lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                       P(abi.PResult), P(OracleOps)]
         _oracle = lib
@@ -94,6 +96,8 @@ def reference() -> C.CDLL:
         lib.ref_estimate_intensity_comp.argtypes = [P(abi.Planes), P(abi.Planes), P(C.c_int32), P(abi.Intensity)]
         lib.ref_intensity_compensate.argtypes = [P(abi.Planes), P(abi.Intensity), C.c_void_p, C.c_void_p,
                                                  C.c_void_p, C.c_int]
+        lib.ref_analyze_b.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.Planes), P(abi.BParams),
+                                      P(abi.BResult)]
         lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                   P(abi.PResult), C.c_int]
         _ref = lib
@@ -137,6 +141,24 @@ def reference_analyze_seeded(params: abi.Params, cur, ref, seed: abi.ResultArray
                                                C.byref(seed.struct), C.byref(out.struct), 0)
     assert rc == 0, reference().ref_last_error()
     return out
+
+
+def _b(fn, params, cur, past, future, binp, *err):
+    c, p, f = abi.PlaneSet.of(cur), abi.PlaneSet.of(past), abi.PlaneSet.of(future)
+    out = abi.BResultArrays(params.width, params.height)
+    rc = fn(C.byref(params), C.byref(c.struct), C.byref(p.struct), C.byref(f.struct), C.byref(binp.struct),
+            C.byref(out.struct))
+    assert rc == 0, err[0]() if err else rc
+    return out
+
+
+def oracle_analyze_b(params: abi.Params, cur, past, future, binp: abi.BInputs) -> abi.BResultArrays:
+    """analyze_b_picture_mode in the oracle (params.mode = the B mode)."""
+    return _b(oracle().oracle_analyze_b_frame, params, cur, past, future, binp)
+
+
+def reference_analyze_b(params: abi.Params, cur, past, future, binp: abi.BInputs) -> abi.BResultArrays:
+    return _b(reference().ref_analyze_b, params, cur, past, future, binp, lambda: reference().ref_last_error())
 
 
 def _mv_pairs(mvs: np.ndarray) -> np.ndarray:

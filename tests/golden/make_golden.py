@@ -9,10 +9,14 @@ plus SHA-256 digests of the outputs of larger synthetic cases
 inputs, the seed's vectors and distant fields, every output field), and the
 intensity-compensation fixtures (intensity.json: estimate_intensity_comp on
 generated fades with their vector fields, and a digest of the luma and
-chroma maps of every LUMSCALE / LUMSHIFT code).  The fixtures are data only.
+chroma maps of every LUMSCALE / LUMSHIFT code), and the B-picture fixtures
+(bframes/*.npz: the three pictures, the future anchor's fields, the B
+parameters and every BAnalysis motion output of analyze_b_picture_mode).
+The fixtures are data only.
 
     python tests/golden/make_golden.py              # everything
     python tests/golden/make_golden.py intensity    # intensity.json only
+    python tests/golden/make_golden.py bframes      # bframes/*.npz only
 """
 from __future__ import annotations
 
@@ -29,8 +33,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from libvc1_amd import abi, synth  # noqa: E402
-from _oracle import (oracle_analyze, reference_analyze, reference_analyze_seeded,  # noqa: E402
-                     reference_estimate_intensity, reference_intensity_compensate)
+from _oracle import (oracle_analyze, reference_analyze, reference_analyze_b,  # noqa: E402
+                     reference_analyze_seeded, reference_estimate_intensity, reference_intensity_compensate)
 
 
 def formula_pair(w, h, a, b):
@@ -206,7 +210,63 @@ def intensity_fixtures() -> dict:
             "map_sha256": intensity_map_table(lambda pic, ic: reference_intensity_compensate(pic, ic, ic[0]))}
 
 
+def b_cases():
+    """(name, cur, past, future, params overrides, anchor mvs, anchor 4mv,
+    B parameters) of the B-picture fixtures."""
+    out = []
+    cif = synth.cif_translation(41, frames=4, width=176, height=144, max_px=6)
+    base = dict(pqindex=8, motion_search_range=64, motion_local_search_range=32)
+    anchor = reference_analyze(abi.default_params(176, 144, mode=abi.MODE_MIXED_MV, **base), cif[3], cif[0])
+    out.append(("b_cif_qpel_third", cif[1], cif[0], cif[3], dict(base, mode=abi.MODE_ONE_MV_QPEL),
+                anchor.mvs, anchor.use_4mv, dict(fraction=(1, 3), allow_intra=1)))
+    out.append(("b_cif_bilinear_twothirds_ic", cif[2], cif[0], cif[3], dict(base, mode=abi.MODE_ONE_MV_HPEL_BILINEAR),
+                anchor.mvs, anchor.use_4mv, dict(fraction=(2, 3), allow_intra=1, intensity=(1, 30, 4))))
+    other = synth.pan_rectangles(42, 2, width=176, height=144, frame_indices=[1])[1]
+    out.append(("b_cut_intra_half", other, cif[0], cif[3], dict(base, mode=abi.MODE_ONE_MV_QPEL),
+                anchor.mvs, anchor.use_4mv, dict(fraction=(1, 2), allow_intra=1)))
+    c, r = hidden_pair()
+    rng = np.random.default_rng(43)
+    mbs = 10 * 8
+    out.append(("b_hidden_hpel_half", c, r, c, dict(base, mode=abi.MODE_ONE_MV_HPEL, display_width=154,
+                                                    display_height=114),
+                rng.integers(-40, 40, size=(mbs, 2)).astype(np.int32), rng.integers(0, 2, size=mbs).astype(np.uint8),
+                dict(fraction=(1, 2), allow_intra=1)))
+    # a linear pan at (+7, -5) qpel per frame: the scaled anchor vectors are exact
+    rng = np.random.default_rng(45)
+    tex = synth.blur_reflect(rng.uniform(0, 255, size=(144, 176)), 2.0)
+    tex = np.clip(np.rint(tex), 0, 255).astype(np.uint8)
+    grey = np.full((72, 88), 128, np.uint8)
+    lin = [synth.Picture(synth.vc1_luma_translate(tex, 7 * k, -5 * k, wrap=True), grey, grey) for k in range(4)]
+    lin_anchor = reference_analyze(abi.default_params(176, 144, mode=abi.MODE_ONE_MV_QPEL, allow_intra=0, **base),
+                                   lin[3], lin[0])
+    out.append(("b_linear_direct_third", lin[1], lin[0], lin[3], dict(base, mode=abi.MODE_ONE_MV_QPEL),
+                lin_anchor.mvs, lin_anchor.use_4mv, dict(fraction=(1, 3), allow_intra=1)))
+    small = synth.pan_rectangles(44, 4, width=128, height=96, frame_indices=[0, 1, 3])
+    out.append(("b_zero_range_third", small[1], small[0], small[3], dict(base, motion_search_range=0),
+                np.zeros((48, 2), np.int32), np.zeros(48, np.uint8), dict(fraction=(1, 3), allow_intra=1)))
+    return out
+
+
+def b_fixtures() -> None:
+    os.makedirs(os.path.join(HERE, "bframes"), exist_ok=True)
+    for name, cur, past, fut, over, amv, a4, bkw in b_cases():
+        h, w = cur.y.shape
+        p = abi.default_params(w, h, **over)
+        binp = abi.BInputs(amv, a4, **bkw)
+        res = reference_analyze_b(p, cur, past, fut, binp)
+        arrays = {f"out_{k}": getattr(res, k) for k in abi.BResultArrays.ARRAYS}
+        np.savez_compressed(os.path.join(HERE, "bframes", f"{name}.npz"),
+                            cur_y=cur.y, cur_u=cur.u, cur_v=cur.v, past_y=past.y, past_u=past.u, past_v=past.v,
+                            fut_y=fut.y, fut_u=fut.u, fut_v=fut.v, anchor_mvs=np.asarray(amv, np.int32),
+                            anchor_4mv=np.asarray(a4, np.uint8), params=json.dumps(over), bparams=json.dumps(bkw),
+                            scalars=json.dumps(res.scalars()), **arrays)
+        print(f"wrote bframes/{name}: {res.scalars()}")
+
+
 def main() -> int:
+    if sys.argv[1:] == ["bframes"]:
+        b_fixtures()
+        return 0
     if sys.argv[1:] == ["intensity"]:
         with open(os.path.join(HERE, "intensity.json"), "w") as f:
             json.dump(intensity_fixtures(), f)
@@ -243,6 +303,7 @@ def main() -> int:
         json.dump(digests, f, indent=1)
     with open(os.path.join(HERE, "intensity.json"), "w") as f:
         json.dump(intensity_fixtures(), f)
+    b_fixtures()
     return 0
 
 
