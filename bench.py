@@ -33,6 +33,11 @@ quarter-pel seed pass, the three seeded mode candidates as one 48-job
 vc1me_analyze_step_seeded (its MB/s is `value`), and the final seeded
 Mixed-MV pass with intra.
 
+The B-picture leg (`b_pictures`) runs the B pictures between each GOP's I
+picture and its first P anchor (frames g+1 and g+2 between g and g+3, 8
+pictures) as one vc1me_analyze_b_step, and one by one through
+vc1me_analyze_b_frame, timed per picture.
+
 The batched leg (`batched`) runs the 64-stream workload of uneven rasters
 and lengths (lengths scaled to keep the run short) through the batching
 runner with the tuned table; it is reported beside the headline, not as it.
@@ -44,7 +49,7 @@ multithreaded CPU path timed on this host on a bounded sample of the same
 frames.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-check] [--no-cpu] [--no-learned] [--no-compare]
-                    [--no-seeded] [--no-batched] [--batched-scale S]
+                    [--no-seeded] [--no-bframes] [--no-batched] [--batched-scale S]
     python bench.py --copy-calibration     # 256 MiB device copy, for HBM counter scaling
 
 `roofline.traffic` (HBM bytes per launch of the macroblock kernel) comes from
@@ -81,7 +86,8 @@ def workload_params():
 def make_frames():
     from libvc1_amd import synth
     pairs = synth.ibbp_p_pairs(FRAMES, gop=15, spacing=3)
-    needed = sorted({i for p in pairs for i in p})
+    # the P pairs' frames, plus the two B pictures after each GOP's I picture
+    needed = sorted({i for p in pairs for i in p} | {1, 2, 16, 17, 31, 32, 46, 47})
     t0 = time.time()
     pics = synth.pan_rectangles(SEED, FRAMES, width=WIDTH, height=HEIGHT, max_pan_px=16, max_obj_px=48,
                                 frame_indices=needed)
@@ -315,6 +321,61 @@ def seeded_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, check_
             "parity": parity, "intensity": intensity}
 
 
+def b_leg(be, pics, frames, steps, check):
+    """B pictures of the same IBBP workload: for each GOP's first anchor pair
+    (P at frame 3 predicted from frame 0), the two B pictures at 1/3 and 2/3
+    (frames 1 and 2) through vc1me_analyze_b_frame in 1-MV quarter-pel mode,
+    the anchor's Mixed-MV result supplying the direct-mode vectors.  Times
+    the whole B analysis (two core passes, costs, decision) per picture."""
+    import numpy as np
+    from libvc1_amd import abi
+
+    base = dict(pqindex=8, motion_search_range=512, motion_local_search_range=32, me_quality=2)
+    groups = [(g0, g0 + 3) for g0 in (0, 15, 30, 45) if g0 + 3 in frames and g0 + 1 in frames and g0 + 2 in frames]
+    if not groups:
+        return None
+    anchors = {}
+    for past, fut in groups:
+        anchors[fut] = be.analyze(frames[fut], frames[past],
+                                  abi.default_params(WIDTH, HEIGHT, mode=abi.MODE_MIXED_MV, allow_intra=1, **base))
+    p = abi.default_params(WIDTH, HEIGHT, mode=abi.MODE_ONE_MV_QPEL, **base)
+    work = [(past + k, past, fut, abi.BInputs(anchors[fut].mvs, anchors[fut].use_4mv, fraction=(k, 3)))
+            for past, fut in groups for k in (1, 2)]
+    be.analyze_b(frames[work[0][0]], frames[work[0][1]], frames[work[0][2]], p, work[0][3])
+    t0 = time.perf_counter()
+    n = 0
+    for _ in range(max(1, steps // 2)):
+        for cur, past, fut, binp in work:
+            last = be.analyze_b(frames[cur], frames[past], frames[fut], p, binp)
+            n += 1
+    secs_single = (time.perf_counter() - t0) / n
+    items = [(frames[cur], frames[past], frames[fut], p, binp) for cur, past, fut, binp in work]
+    be.analyze_b_step(items)
+    t0 = time.perf_counter()
+    n = 0
+    for _ in range(max(1, steps // 2)):
+        last = be.analyze_b_step(items)[-1]
+        n += len(items)
+    secs = (time.perf_counter() - t0) / n
+    mbs = ((WIDTH + 15) // 16) * ((HEIGHT + 15) // 16)
+    parity = None
+    if check:
+        from _oracle import oracle_analyze_b
+        cur, past, fut, binp = work[-1]
+        bad = last.diff(oracle_analyze_b(p, pics[cur], pics[past], pics[fut], binp))
+        parity = "bit-exact" if not bad else f"MISMATCH in {bad}"
+        log(f"B picture {cur} vs oracle: {parity}")
+    modes = {k: int(getattr(last.struct, k)) for k in ("direct_macroblocks", "forward_macroblocks",
+                                                        "backward_macroblocks", "interpolated_macroblocks",
+                                                        "intra_macroblocks")}
+    log(f"B pictures: {secs * 1e3:.2f} ms per picture in one step of {len(items)} ({mbs / secs:,.0f} MB/s), "
+        f"{secs_single * 1e3:.2f} ms one by one; last picture {modes}")
+    return {"ms_per_b_picture": round(secs * 1e3, 3), "value": round(mbs / secs, 1), "unit": "MB/s",
+            "ms_per_b_picture_one_by_one": round(secs_single * 1e3, 3), "pictures_per_step": len(items),
+            "pictures": len(work), "mode": "1-MV qpel, anchor Mixed-MV, fractions 1/3 and 2/3", "parity": parity,
+            "last_picture_modes": modes}
+
+
 def intensity_part(be, jobs, final_results, check_pair):
     """The fade path's two kernels on the same pairs (the reference runs them
     when fade compensation is on, src/encoder_motion.cpp:573-576):
@@ -430,6 +491,7 @@ def main() -> int:
     ap.add_argument("--no-compare", action="store_true", help="skip the diagonal-sweep comparison steps")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-stream batched leg")
     ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-core leg")
+    ap.add_argument("--no-bframes", action="store_true", help="skip the B-picture leg")
     ap.add_argument("--batched-scale", type=float, default=0.1,
                     help="stream lengths (8..120 frames) are scaled by this in the batched leg")
     args = ap.parse_args()
@@ -542,6 +604,10 @@ def main() -> int:
         seeded = seeded_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step,
                             None if args.no_check else (pics[pairs[0][0]], pics[pairs[0][1]]))
 
+    bpics = None
+    if not args.no_bframes:
+        bpics = b_leg(be, pics, frames, args.steps, not args.no_check)
+
     learned_out = None
     if not args.no_learned:
         learned_out = learned_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step)
@@ -582,6 +648,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "learned": learned_out,
         "seeded_core": seeded,
+        "b_pictures": bpics,
         "batched": batched,
     }
     for f in frames.values():

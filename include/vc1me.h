@@ -306,6 +306,19 @@ typedef struct vc1me_b_result {
 int vc1me_analyze_b_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* past, const vc1me_frame* future,
                           const vc1me_params* params, const vc1me_b_params* b, vc1me_b_result* out);
 
+/* n B pictures in one step (their 2n core passes run as one batched step,
+ * then every picture's decision): equivalent to n calls of
+ * vc1me_analyze_b_frame; at most 127 pictures. */
+typedef struct vc1me_b_job {
+    const vc1me_frame* cur;
+    const vc1me_frame* past;
+    const vc1me_frame* future;
+    const vc1me_params* params;
+    const vc1me_b_params* b;
+    vc1me_b_result* out;
+} vc1me_b_job;
+int vc1me_analyze_b_step(vc1me_ctx* ctx, const vc1me_b_job* jobs, int n);
+
 /* Copies a resident frame's planes back to the host (width x height luma,
  * two width/2 x height/2 chroma planes).  No reference counterpart. */
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);

@@ -133,6 +133,11 @@ class BResult(C.Structure):
     ]
 
 
+class BJob(C.Structure):
+    _fields_ = [("cur", C.c_void_p), ("past", C.c_void_p), ("future", C.c_void_p), ("params", C.POINTER(Params)),
+                ("b", C.POINTER(BParams)), ("out", C.POINTER(BResult))]
+
+
 class BInputs:
     """numpy-backed vc1me_b_params: the future anchor's fields and the rest."""
 

@@ -27,7 +27,7 @@ EXPORTS = (
     "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
     "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
-    "vc1me_frame_download", "vc1me_analyze_b_frame",
+    "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step",
 )
 
 
@@ -74,6 +74,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_frame_intensity_compensated.argtypes = [C.c_void_p, C.c_void_p, P(abi.Intensity), P(C.c_void_p)]
     lib.vc1me_analyze_b_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params),
                                           P(abi.BParams), P(abi.BResult)]
+    lib.vc1me_analyze_b_step.argtypes = [C.c_void_p, P(abi.BJob), C.c_int]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
@@ -188,6 +189,22 @@ class Backend:
         self._check(self.lib.vc1me_analyze_b_frame(self.ctx, cur.handle, past.handle, future.handle, C.byref(params),
                                                    C.byref(binp.struct), C.byref(r.struct)))
         return r
+
+    def analyze_b_step(self, items) -> list:
+        """vc1me_analyze_b_step over items of (cur, past, future, params,
+        BInputs); one BResultArrays each."""
+        n = len(items)
+        jobs = (abi.BJob * n)()
+        outs = []
+        for i, (cur, past, fut, p, binp) in enumerate(items):
+            r = abi.BResultArrays(p.width, p.height)
+            outs.append(r)
+            jobs[i].cur, jobs[i].past, jobs[i].future = cur.handle, past.handle, fut.handle
+            jobs[i].params = C.pointer(p)
+            jobs[i].b = C.pointer(binp.struct)
+            jobs[i].out = C.pointer(r.struct)
+        self._check(self.lib.vc1me_analyze_b_step(self.ctx, jobs, n))
+        return outs
 
     def analyze(self, cur: Frame, ref: Frame, params: abi.Params) -> abi.ResultArrays:
         r = abi.ResultArrays(params.width, params.height)

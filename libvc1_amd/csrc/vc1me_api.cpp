@@ -38,6 +38,9 @@ namespace {
 
 thread_local std::string g_last_error;
 
+// (cur, ref) pairs of one step
+constexpr int kMaxJobs = 255;
+
 int fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
@@ -538,7 +541,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
              const vc1me_p_result* const* seeds = nullptr) {
     if (n <= 0) return VC1ME_OK;
     if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
-    if (n > 255) return fail(VC1ME_ELIMIT, "at most 255 jobs per step");
+    if (n > kMaxJobs) return fail(VC1ME_ELIMIT, "at most " + std::to_string(kMaxJobs) + " jobs per step");
     for (int i = 0; i < n; ++i) {
         const int rc = validate_params(jobs[i].params, jobs[i].cur, jobs[i].ref);
         if (rc) return rc;
@@ -1351,152 +1354,254 @@ int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, co
 
 // ---- B pictures
 
-int vc1me_analyze_b_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* past, const vc1me_frame* future,
-                          const vc1me_params* params, const vc1me_b_params* b, vc1me_b_result* out) {
-    if (!ctx || !cur || !past || !future || !params || !b || !out) return fail(VC1ME_EINVAL, "null argument");
+namespace {
+
+int check_b_job(const vc1me_b_job& j) {
+    if (!j.cur || !j.past || !j.future || !j.params || !j.b || !j.out) return fail(VC1ME_EINVAL, "null argument");
+    const vc1me_b_params* b = j.b;
     if (!b->future_anchor_mvs || !b->future_anchor_4mv) return fail(VC1ME_EINVAL, "null future anchor fields");
-    if (past->w != cur->w || past->h != cur->h || future->w != cur->w || future->h != cur->h ||
-        params->width != cur->w || params->height != cur->h)
+    const vc1me_frame* cur = j.cur;
+    if (j.past->w != cur->w || j.past->h != cur->h || j.future->w != cur->w || j.future->h != cur->h ||
+        j.params->width != cur->w || j.params->height != cur->h)
         return fail(VC1ME_EINVAL, "cur, past, future and params must share one raster");
     const int fn = b->fraction_num, fd = b->fraction_den;
     if (!((fd == 2 && fn == 1) || (fd == 3 && (fn == 1 || fn == 2))))
         return fail(VC1ME_EINVAL, "unsupported B-picture temporal fraction");
-    if (params->mode < VC1ME_MODE_ONE_MV_QPEL || params->mode > VC1ME_MODE_ONE_MV_HPEL_BILINEAR)
+    if (j.params->mode < VC1ME_MODE_ONE_MV_QPEL || j.params->mode > VC1ME_MODE_ONE_MV_HPEL_BILINEAR)
         return fail(VC1ME_EINVAL, "B pictures take a 1-MV mode");
     if (!intensity_codes_ok(b->forward_intensity)) return fail(VC1ME_EINVAL, "LUMSCALE and LUMSHIFT are 6-bit codes");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    if (int rc = set_device(ctx)) return rc;
-    const int mbw = (cur->w + 15) / 16, mbh = (cur->h + 15) / 16;
-    const size_t mbs = static_cast<size_t>(mbw) * mbh;
+    if ((cur->h + 15) / 16 > vc1me::kBMaxRows)
+        return fail(VC1ME_ELIMIT, "B pictures are limited to " + std::to_string(vc1me::kBMaxRows * 16) + " rows");
+    return VC1ME_OK;
+}
+
+void b_reset_counts(vc1me_b_result* out) {
     out->forward_macroblocks = out->backward_macroblocks = out->interpolated_macroblocks = 0;
     out->direct_macroblocks = out->intra_macroblocks = out->moved_macroblocks = 0;
     out->fractional_chroma_macroblocks = 0;
     out->mean_abs_residual = out->rate_complexity = 0.0;
     out->motion_sad = out->motion_bits = 0;
-    if (b->zero_motion_shortcut && params->motion_search_range == 0) {
-        // every macroblock forward with zero vectors (src/encoder_motion.cpp:775-787)
-        for (size_t i = 0; i < mbs; ++i) {
-            if (out->modes) out->modes[i] = VC1ME_B_FORWARD;
-            if (out->forward_mvs) out->forward_mvs[2 * i] = out->forward_mvs[2 * i + 1] = 0;
-            if (out->backward_mvs) out->backward_mvs[2 * i] = out->backward_mvs[2 * i + 1] = 0;
-            if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = -1.0;
-            if (out->forward_distant_local_mae) out->forward_distant_local_mae[i] = -1.0;
-            if (out->forward_distant_candidate_mae) out->forward_distant_candidate_mae[i] = -1.0;
-            if (out->forward_distant_match_decision) out->forward_distant_match_decision[i] = 0;
-            if (out->backward_distant_local_mae) out->backward_distant_local_mae[i] = -1.0;
-            if (out->backward_distant_candidate_mae) out->backward_distant_candidate_mae[i] = -1.0;
-            if (out->backward_distant_match_decision) out->backward_distant_match_decision[i] = 0;
-        }
-        out->forward_macroblocks = mbs;
-        return VC1ME_OK;
+}
+
+// the encoder's minimal zero-motion configuration: every macroblock forward
+// with zero vectors (src/encoder_motion.cpp:775-787)
+void b_zero_motion(vc1me_b_result* out, size_t mbs) {
+    for (size_t i = 0; i < mbs; ++i) {
+        if (out->modes) out->modes[i] = VC1ME_B_FORWARD;
+        if (out->forward_mvs) out->forward_mvs[2 * i] = out->forward_mvs[2 * i + 1] = 0;
+        if (out->backward_mvs) out->backward_mvs[2 * i] = out->backward_mvs[2 * i + 1] = 0;
+        if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = -1.0;
+        if (out->forward_distant_local_mae) out->forward_distant_local_mae[i] = -1.0;
+        if (out->forward_distant_candidate_mae) out->forward_distant_candidate_mae[i] = -1.0;
+        if (out->forward_distant_match_decision) out->forward_distant_match_decision[i] = 0;
+        if (out->backward_distant_local_mae) out->backward_distant_local_mae[i] = -1.0;
+        if (out->backward_distant_candidate_mae) out->backward_distant_candidate_mae[i] = -1.0;
+        if (out->backward_distant_match_decision) out->backward_distant_match_decision[i] = 0;
     }
-    // the forward reference carries the anchor's intensity compensation
-    vc1me_frame* comp = nullptr;
-    if (b->forward_intensity.enabled)
-        if (int rc = compensated_frame(ctx, past, &b->forward_intensity, &comp)) return rc;
-    struct Release {
-        vc1me_frame* f;
-        ~Release() { if (f) { (void)hipFree(f->y); delete f; } }
-    } release{comp};
-    vc1me_params q = *params;
-    q.allow_intra = 0;
-    vc1me_p_result fo{}, bo{};
-    fo.distant_local_mae = out->forward_distant_local_mae;
-    fo.distant_candidate_mae = out->forward_distant_candidate_mae;
-    fo.distant_match_decision = out->forward_distant_match_decision;
-    bo.distant_local_mae = out->backward_distant_local_mae;
-    bo.distant_candidate_mae = out->backward_distant_candidate_mae;
-    bo.distant_match_decision = out->backward_distant_match_decision;
-    const vc1me_job jobs[2] = {{cur, comp ? comp : past, &q, &fo}, {cur, future, &q, &bo}};
-    if (int rc = run_jobs(ctx, jobs, 2)) return rc;
-    // the decision reads jobs 0 and 1 of the step in place (vectors, screen
-    // SADs, phase planes), so it runs before anything else on the context
-    const size_t o_desc = 0, o_amv = 256, o_a4 = o_amv + ((mbs * sizeof(int2) + 255) & ~size_t(255));
-    const size_t o_cost = o_a4 + ((mbs + 255) & ~size_t(255));
-    const size_t o_isad = o_cost + mbs * 32, o_dir = o_isad + mbs * 8, o_fmv = o_dir + mbs * 16;
-    const size_t o_bmv = o_fmv + mbs * 8, o_ratio = o_bmv + mbs * 8, o_ssad = o_ratio + mbs * 8;
-    const size_t o_rsad = o_ssad + mbs * 8, o_srate = o_rsad + mbs * 8, o_modes = o_srate + mbs * 4;
-    const size_t need = o_modes + mbs;
-    if (ctx->b_cap < need) {
+    out->forward_macroblocks = mbs;
+}
+
+// Device layout of one B picture's decision state in the context's B buffer.
+struct BLayout {
+    size_t amv, a4, cost, isad, dir, fmv, bmv, ratio, ssad, rsad, srate, modes, end;
+    explicit BLayout(size_t base, size_t mbs) {
+        auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+        amv = base;
+        a4 = up(amv + mbs * 8);
+        cost = up(a4 + mbs);
+        isad = cost + mbs * 32;
+        dir = isad + mbs * 8;
+        fmv = dir + mbs * 16;
+        bmv = fmv + mbs * 8;
+        ratio = bmv + mbs * 8;
+        ssad = ratio + mbs * 8;
+        rsad = ssad + mbs * 8;
+        srate = rsad + mbs * 8;
+        modes = srate + mbs * 4;
+        end = up(modes + mbs);
+    }
+};
+
+}  // namespace
+
+int vc1me_analyze_b_step(vc1me_ctx* ctx, const vc1me_b_job* bjobs, int n) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (n < 0 || (n > 0 && !bjobs)) return fail(VC1ME_EINVAL, "bad B job list");
+    if (2 * n > kMaxJobs) return fail(VC1ME_ELIMIT, "at most " + std::to_string(kMaxJobs / 2) + " B pictures per step");
+    for (int i = 0; i < n; ++i)
+        if (int rc = check_b_job(bjobs[i])) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    // pictures that run: not under the zero-motion shortcut
+    std::vector<int> run;
+    for (int i = 0; i < n; ++i) {
+        const vc1me_b_job& j = bjobs[i];
+        b_reset_counts(j.out);
+        const size_t mbs = static_cast<size_t>((j.cur->w + 15) / 16) * ((j.cur->h + 15) / 16);
+        if (j.b->zero_motion_shortcut && j.params->motion_search_range == 0) b_zero_motion(j.out, mbs);
+        else run.push_back(i);
+    }
+    if (run.empty()) return VC1ME_OK;
+    const int nb = static_cast<int>(run.size());
+    // forward references carry the anchor's intensity compensation; one
+    // compensated frame per distinct (past, code)
+    struct Comp { const vc1me_frame* past; vc1me_intensity ic; vc1me_frame* frame; };
+    struct Comps {
+        std::vector<Comp> v;
+        ~Comps() { for (auto& c : v) if (c.frame) { (void)hipFree(c.frame->y); delete c.frame; } }
+    } comps;
+    std::vector<const vc1me_frame*> fwd_ref(nb);
+    for (int k = 0; k < nb; ++k) {
+        const vc1me_b_job& j = bjobs[run[k]];
+        fwd_ref[k] = j.past;
+        const vc1me_intensity& ic = j.b->forward_intensity;
+        if (!ic.enabled) continue;
+        const vc1me_frame* found = nullptr;
+        for (const auto& c : comps.v)
+            if (c.past == j.past && c.ic.lumscale == ic.lumscale && c.ic.lumshift == ic.lumshift) found = c.frame;
+        if (!found) {
+            vc1me_frame* f = nullptr;
+            if (int rc = compensated_frame(ctx, j.past, &ic, &f)) return rc;
+            comps.v.push_back({j.past, ic, f});
+            found = f;
+        }
+        fwd_ref[k] = found;
+    }
+    // the core passes of every picture in one step: job 2k forward, 2k + 1 backward
+    std::vector<vc1me_params> qs(nb);
+    std::vector<vc1me_p_result> pres(2 * static_cast<size_t>(nb));
+    std::vector<vc1me_job> jobs(2 * static_cast<size_t>(nb));
+    for (int k = 0; k < nb; ++k) {
+        const vc1me_b_job& j = bjobs[run[k]];
+        qs[k] = *j.params;
+        qs[k].allow_intra = 0;
+        vc1me_p_result& fo = pres[2 * k];
+        vc1me_p_result& bo = pres[2 * k + 1];
+        fo = vc1me_p_result{};
+        bo = vc1me_p_result{};
+        fo.distant_local_mae = j.out->forward_distant_local_mae;
+        fo.distant_candidate_mae = j.out->forward_distant_candidate_mae;
+        fo.distant_match_decision = j.out->forward_distant_match_decision;
+        bo.distant_local_mae = j.out->backward_distant_local_mae;
+        bo.distant_candidate_mae = j.out->backward_distant_candidate_mae;
+        bo.distant_match_decision = j.out->backward_distant_match_decision;
+        jobs[2 * k] = {j.cur, fwd_ref[k], &qs[k], &fo};
+        jobs[2 * k + 1] = {j.cur, j.future, &qs[k], &bo};
+    }
+    if (int rc = run_jobs(ctx, jobs.data(), 2 * nb)) return rc;
+    // the decision reads the step's jobs in place (vectors, screen SADs, phase
+    // planes), so it runs before anything else on the context
+    const size_t o_desc = 0;
+    size_t off = ((sizeof(vc1me::BDesc) * nb) + 255) & ~size_t(255);
+    std::vector<BLayout> lay;
+    int max_mbs = 0, max_mbh = 0;
+    for (int k = 0; k < nb; ++k) {
+        const vc1me_frame* cur = bjobs[run[k]].cur;
+        const int mbw = (cur->w + 15) / 16, mbh = (cur->h + 15) / 16;
+        lay.emplace_back(off, static_cast<size_t>(mbw) * mbh);
+        off = lay.back().end;
+        max_mbs = std::max(max_mbs, mbw * mbh);
+        max_mbh = std::max(max_mbh, mbh);
+    }
+    if (ctx->b_cap < off) {
         if (ctx->d_b) (void)hipFree(ctx->d_b);
         ctx->d_b = nullptr;
         ctx->b_cap = 0;
-        const hipError_t e = hipMalloc(&ctx->d_b, need);
+        const hipError_t e = hipMalloc(&ctx->d_b, off);
         if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("B-picture buffers: ") + hipGetErrorString(e));
-        ctx->b_cap = need;
+        ctx->b_cap = off;
     }
     uint8_t* base = ctx->d_b;
-    vc1me::BDesc d{};
-    d.anchor_mvs = reinterpret_cast<const int2*>(base + o_amv);
-    d.anchor_4mv = base + o_a4;
-    d.mode = params->mode;
-    d.bfrac = fd == 2 ? 128 : (fn == 1 ? 85 : 170);
-    d.below_half = 2 * fn < fd ? 1 : 0;
-    d.allow_intra = b->allow_intra ? 1 : 0;
-    d.lambda = std::max(1.0, 0.45 * static_cast<double>(params->pqindex));
-    d.intra_guard = static_cast<uint64_t>(std::llround(d.lambda * 72.0));
-    d.intra_thr = params->inter_intra_threshold;
-    d.cost = reinterpret_cast<uint64_t*>(base + o_cost);
-    d.intra_sad = reinterpret_cast<uint64_t*>(base + o_isad);
-    d.direct = reinterpret_cast<int2*>(base + o_dir);
-    d.fmv = reinterpret_cast<int2*>(base + o_fmv);
-    d.bmv = reinterpret_cast<int2*>(base + o_bmv);
-    d.ratio = reinterpret_cast<double*>(base + o_ratio);
-    d.sel_sad = reinterpret_cast<uint64_t*>(base + o_ssad);
-    d.rate_sad = reinterpret_cast<uint64_t*>(base + o_rsad);
-    d.sel_rate = reinterpret_cast<uint32_t*>(base + o_srate);
-    d.modes = base + o_modes;
-    static_assert(sizeof(vc1me::BDesc) <= 256, "B descriptor slot");
     hipStream_t st = ctx->stream;
-    HIP_TRY(hipMemcpyAsync(base + o_desc, &d, sizeof d, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(base + o_amv, b->future_anchor_mvs, mbs * sizeof(int2), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(base + o_a4, b->future_anchor_4mv, mbs, hipMemcpyHostToDevice, st));
-    const auto* d_desc = reinterpret_cast<const vc1me::BDesc*>(base + o_desc);
-    HIP_TRY(vc1me::launch_b_costs(ctx->d_jobs, d_desc, static_cast<int>(mbs), st));
-    HIP_TRY(vc1me::launch_b_decide(ctx->d_jobs, d_desc, st));
-    std::vector<uint8_t> modes(mbs);
-    std::vector<int2> fmv(mbs), bmv(mbs);
-    std::vector<double> ratio(mbs);
-    std::vector<uint64_t> ssad(mbs), rsad(mbs);
-    std::vector<uint32_t> srate(mbs);
-    HIP_TRY(hipMemcpyAsync(modes.data(), d.modes, mbs, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(fmv.data(), d.fmv, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(bmv.data(), d.bmv, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ratio.data(), d.ratio, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ssad.data(), d.sel_sad, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(rsad.data(), d.rate_sad, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(srate.data(), d.sel_rate, mbs * 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    // picture totals (src/encoder_motion.cpp:933-1003)
-    uint64_t sel_sad = 0, rate_sad = 0, sel_rate = 1, sel_px = 0;
-    for (size_t i = 0; i < mbs; ++i) {
-        const int mx = static_cast<int>(i % mbw), my = static_cast<int>(i / mbw);
-        sel_sad += ssad[i];
-        rate_sad += rsad[i];
-        sel_rate += srate[i];
-        sel_px += static_cast<uint64_t>(std::max(0, std::min(16, cur->w - mx * 16))) *
-                  static_cast<uint64_t>(std::max(0, std::min(16, cur->h - my * 16)));
-        switch (modes[i]) {
-            case VC1ME_B_FORWARD: ++out->forward_macroblocks; break;
-            case VC1ME_B_BACKWARD: ++out->backward_macroblocks; break;
-            case VC1ME_B_INTERPOLATED: ++out->interpolated_macroblocks; break;
-            case VC1ME_B_DIRECT: ++out->direct_macroblocks; break;
-            default: ++out->intra_macroblocks; break;
-        }
-        const int2 f = fmv[i], k = bmv[i];
-        if (f.x || f.y || k.x || k.y) ++out->moved_macroblocks;
-        auto frac_chroma = [](int2 v) { return (v.x & 3) || (v.y & 3) || ((v.x / 4) & 1) || ((v.y / 4) & 1); };
-        if (frac_chroma(f) || frac_chroma(k)) ++out->fractional_chroma_macroblocks;
-        if (out->modes) out->modes[i] = modes[i];
-        if (out->forward_mvs) { out->forward_mvs[2 * i] = f.x; out->forward_mvs[2 * i + 1] = f.y; }
-        if (out->backward_mvs) { out->backward_mvs[2 * i] = k.x; out->backward_mvs[2 * i + 1] = k.y; }
-        if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = ratio[i];
+    std::vector<vc1me::BDesc> descs(nb);
+    for (int k = 0; k < nb; ++k) {
+        const vc1me_b_job& j = bjobs[run[k]];
+        const BLayout& L = lay[k];
+        const size_t mbs = static_cast<size_t>((j.cur->w + 15) / 16) * ((j.cur->h + 15) / 16);
+        const int fn = j.b->fraction_num, fd = j.b->fraction_den;
+        vc1me::BDesc& d = descs[k];
+        d = vc1me::BDesc{};
+        d.anchor_mvs = reinterpret_cast<const int2*>(base + L.amv);
+        d.anchor_4mv = base + L.a4;
+        d.mode = j.params->mode;
+        d.bfrac = fd == 2 ? 128 : (fn == 1 ? 85 : 170);
+        d.below_half = 2 * fn < fd ? 1 : 0;
+        d.allow_intra = j.b->allow_intra ? 1 : 0;
+        d.lambda = std::max(1.0, 0.45 * static_cast<double>(j.params->pqindex));
+        d.intra_guard = static_cast<uint64_t>(std::llround(d.lambda * 72.0));
+        d.intra_thr = j.params->inter_intra_threshold;
+        d.cost = reinterpret_cast<uint64_t*>(base + L.cost);
+        d.intra_sad = reinterpret_cast<uint64_t*>(base + L.isad);
+        d.direct = reinterpret_cast<int2*>(base + L.dir);
+        d.fmv = reinterpret_cast<int2*>(base + L.fmv);
+        d.bmv = reinterpret_cast<int2*>(base + L.bmv);
+        d.ratio = reinterpret_cast<double*>(base + L.ratio);
+        d.sel_sad = reinterpret_cast<uint64_t*>(base + L.ssad);
+        d.rate_sad = reinterpret_cast<uint64_t*>(base + L.rsad);
+        d.sel_rate = reinterpret_cast<uint32_t*>(base + L.srate);
+        d.modes = base + L.modes;
+        HIP_TRY(hipMemcpyAsync(base + L.amv, j.b->future_anchor_mvs, mbs * sizeof(int2), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(base + L.a4, j.b->future_anchor_4mv, mbs, hipMemcpyHostToDevice, st));
     }
-    out->mean_abs_residual = sel_px ? static_cast<double>(sel_sad) / static_cast<double>(sel_px) : 0.0;
-    out->rate_complexity = sel_px ? static_cast<double>(rate_sad) / static_cast<double>(sel_px) : 0.0;
-    out->motion_sad = sel_sad;
-    out->motion_bits = sel_rate;
+    HIP_TRY(hipMemcpyAsync(base + o_desc, descs.data(), sizeof(vc1me::BDesc) * nb, hipMemcpyHostToDevice, st));
+    const auto* d_desc = reinterpret_cast<const vc1me::BDesc*>(base + o_desc);
+    HIP_TRY(vc1me::launch_b_costs(ctx->d_jobs, d_desc, nb, max_mbs, st));
+    HIP_TRY(vc1me::launch_b_decide(ctx->d_jobs, d_desc, nb, max_mbh, st));
+    // results: one copy of each picture's decision block
+    for (int k = 0; k < nb; ++k) {
+        const vc1me_b_job& j = bjobs[run[k]];
+        const BLayout& L = lay[k];
+        const int mbw = (j.cur->w + 15) / 16;
+        const size_t mbs = static_cast<size_t>(mbw) * ((j.cur->h + 15) / 16);
+        std::vector<uint8_t> host(L.end - L.fmv);
+        HIP_TRY(hipMemcpyAsync(host.data(), base + L.fmv, host.size(), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        auto at = [&](size_t o) { return host.data() + (o - L.fmv); };
+        const int2* fmv = reinterpret_cast<const int2*>(at(L.fmv));
+        const int2* bmv = reinterpret_cast<const int2*>(at(L.bmv));
+        const double* ratio = reinterpret_cast<const double*>(at(L.ratio));
+        const uint64_t* ssad = reinterpret_cast<const uint64_t*>(at(L.ssad));
+        const uint64_t* rsad = reinterpret_cast<const uint64_t*>(at(L.rsad));
+        const uint32_t* srate = reinterpret_cast<const uint32_t*>(at(L.srate));
+        const uint8_t* modes = at(L.modes);
+        vc1me_b_result* out = j.out;
+        // picture totals (src/encoder_motion.cpp:933-1003)
+        uint64_t sel_sad = 0, rate_sad = 0, sel_rate = 1, sel_px = 0;
+        for (size_t i = 0; i < mbs; ++i) {
+            const int mx = static_cast<int>(i % mbw), my = static_cast<int>(i / mbw);
+            sel_sad += ssad[i];
+            rate_sad += rsad[i];
+            sel_rate += srate[i];
+            sel_px += static_cast<uint64_t>(std::max(0, std::min(16, j.cur->w - mx * 16))) *
+                      static_cast<uint64_t>(std::max(0, std::min(16, j.cur->h - my * 16)));
+            switch (modes[i]) {
+                case VC1ME_B_FORWARD: ++out->forward_macroblocks; break;
+                case VC1ME_B_BACKWARD: ++out->backward_macroblocks; break;
+                case VC1ME_B_INTERPOLATED: ++out->interpolated_macroblocks; break;
+                case VC1ME_B_DIRECT: ++out->direct_macroblocks; break;
+                default: ++out->intra_macroblocks; break;
+            }
+            const int2 f = fmv[i], g = bmv[i];
+            if (f.x || f.y || g.x || g.y) ++out->moved_macroblocks;
+            auto frac_chroma = [](int2 v) { return (v.x & 3) || (v.y & 3) || ((v.x / 4) & 1) || ((v.y / 4) & 1); };
+            if (frac_chroma(f) || frac_chroma(g)) ++out->fractional_chroma_macroblocks;
+            if (out->modes) out->modes[i] = modes[i];
+            if (out->forward_mvs) { out->forward_mvs[2 * i] = f.x; out->forward_mvs[2 * i + 1] = f.y; }
+            if (out->backward_mvs) { out->backward_mvs[2 * i] = g.x; out->backward_mvs[2 * i + 1] = g.y; }
+            if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = ratio[i];
+        }
+        out->mean_abs_residual = sel_px ? static_cast<double>(sel_sad) / static_cast<double>(sel_px) : 0.0;
+        out->rate_complexity = sel_px ? static_cast<double>(rate_sad) / static_cast<double>(sel_px) : 0.0;
+        out->motion_sad = sel_sad;
+        out->motion_bits = sel_rate;
+    }
     return VC1ME_OK;
+}
+
+int vc1me_analyze_b_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* past, const vc1me_frame* future,
+                          const vc1me_params* params, const vc1me_b_params* b, vc1me_b_result* out) {
+    const vc1me_b_job j{cur, past, future, params, b, out};
+    return vc1me_analyze_b_step(ctx, &j, 1);
 }
 
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v) {

@@ -134,9 +134,9 @@ hipError_t launch_score_batch(const uint8_t* cur, const uint8_t* ref, int width,
                               const int2* cands, int n, uint32_t* costs, hipStream_t s);
 
 // B-picture decision (analyze_b_picture_mode after its two core passes):
-// job 0 of the step is the forward pass (cur vs past), job 1 the backward
-// pass (cur vs future); their vectors, screen SADs and phase planes are
-// read in place.
+// for picture i, job 2i of the step is the forward pass (cur vs past), job
+// 2i + 1 the backward pass (cur vs future); their vectors, screen SADs and
+// phase planes are read in place.
 struct BDesc {
     const int2* anchor_mvs;      // future anchor's 1-MV field
     const uint8_t* anchor_4mv;
@@ -160,8 +160,12 @@ struct BDesc {
     uint64_t* rate_sad;
     uint32_t* sel_rate;
 };
-hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int mbs, hipStream_t s);
-hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, hipStream_t s);
+// nb B pictures: picture i uses jobs 2i (forward) and 2i + 1 (backward) and
+// descriptor i; the decision runs one workgroup per picture with one thread
+// per macroblock row
+constexpr int kBMaxRows = 1024;
+hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbs, hipStream_t s);
+hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbh, hipStream_t s);
 
 // Intensity compensation (vc1me_intensity.hip): the luma table at bytes
 // 0..255 and the chroma table at 256..511.

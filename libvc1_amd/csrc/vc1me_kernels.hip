@@ -3119,10 +3119,12 @@ __device__ __forceinline__ uint32_t avg_u8x4(uint32_t a, uint32_t b) { return (a
 // SADs over the visible pixels (each lane four pixels of one row), their
 // hidden-padding screens, and the B-intra proxy (the four 8x8 blocks' SAD
 // about their visible means).
-__global__ void __launch_bounds__(64) k_b_costs(const JobDesc* __restrict__ jobs, const BDesc* __restrict__ B) {
-    const JobDesc* F = &jobs[0];
-    const JobDesc* K = &jobs[1];
+__global__ void __launch_bounds__(64) k_b_costs(const JobDesc* __restrict__ jobs, const BDesc* __restrict__ Bs) {
+    const BDesc* B = &Bs[blockIdx.y];
+    const JobDesc* F = &jobs[2 * blockIdx.y];
+    const JobDesc* K = &jobs[2 * blockIdx.y + 1];
     const int pos = static_cast<int>(blockIdx.x);
+    if (pos >= F->mbw * F->mbh) return;
     const int mbw = F->mbw, mbh = F->mbh, W = F->W, H = F->H, mode = B->mode;
     const int mx = pos % mbw, my = pos / mbw, x0 = mx * 16, y0 = my * 16;
     const int lane = static_cast<int>(threadIdx.x);
@@ -3180,15 +3182,16 @@ __global__ void __launch_bounds__(64) k_b_costs(const JobDesc* __restrict__ jobs
     }
 }
 
-// predictor_info(...).pre over a final B field (src/encoder_motion.cpp:1312-1352):
-// the median of the top (A), top-right (B; top-left in the last column) and
-// left (C) vectors, pulled inside the picture's MV bounds
-__device__ __forceinline__ int2 b_predictor(const int2* field, int mbw, int mbh, int mx, int my) {
-    const bool a_valid = my > 0, c_valid = mx > 0, b_valid = my > 0 && mbw > 1;
+// predictor_info(...).pre of a final B field (src/encoder_motion.cpp:1312-1352)
+// from its top (A), top-right (B; top-left in the last column) and left (C)
+// vectors: their median (or the one valid of them), pulled inside the
+// picture's MV bounds
+__device__ __forceinline__ int2 b_predictor(int2 A, bool a_valid, int2 Bv, bool b_valid, int2 Cv, bool c_valid, int mx,
+                                            int my, int mbw, int mbh) {
     const int2 z = make_int2(0, 0);
-    const int2 A = a_valid ? field[(my - 1) * mbw + mx] : z;
-    const int2 Cv = c_valid ? field[my * mbw + mx - 1] : z;
-    const int2 Bv = b_valid ? field[(my - 1) * mbw + ((mx == mbw - 1) ? mx - 1 : mx + 1)] : z;
+    if (!a_valid) A = z;
+    if (!b_valid) Bv = z;
+    if (!c_valid) Cv = z;
     int2 pre = a_valid ? A : (c_valid ? Cv : (b_valid ? Bv : z));
     if ((a_valid ? 1 : 0) + (b_valid ? 1 : 0) + (c_valid ? 1 : 0) > 1) {
         pre.x = median3(A.x, Bv.x, Cv.x);
@@ -3214,29 +3217,66 @@ __device__ __forceinline__ unsigned __int128 b_score(uint64_t cost, uint64_t bit
 // The raster-order decision: macroblock (mx, my) needs the final fields of
 // its left, top-left, top and top-right neighbours, so the macroblocks of
 // anti-diagonal t = mx + 2 my are independent once t - 1 is done.  One
-// workgroup walks the diagonals; one thread decides one macroblock.
-__global__ void __launch_bounds__(256) k_b_decide(const JobDesc* __restrict__ jobs, const BDesc* __restrict__ B) {
-    const JobDesc* F = &jobs[0];
-    const JobDesc* K = &jobs[1];
+// workgroup walks the diagonals with one thread per macroblock row (row my
+// decides macroblock t - 2 my on diagonal t).  The final fields of the last
+// three diagonals live in an LDS ring (entry [t & 3][my]: forward and
+// backward vectors), which holds every neighbour a decision reads; each
+// thread loads its next macroblock's costs and vectors while the current one
+// is decided.
+struct BMb {
+    int2 fm, bm, df, db;
+    uint64_t c0, c1, c2, c3, isad, sf, sb;
+};
+__device__ __forceinline__ BMb b_load(const JobDesc* F, const JobDesc* K, const BDesc* B, int pos) {
+    BMb m;
+    m.fm = F->mvs[pos];
+    m.bm = K->mvs[pos];
+    m.df = B->direct[2 * pos];
+    m.db = B->direct[2 * pos + 1];
+    const uint64_t* c = B->cost + 4 * static_cast<size_t>(pos);
+    m.c0 = c[0]; m.c1 = c[1]; m.c2 = c[2]; m.c3 = c[3];
+    m.isad = B->intra_sad[pos];
+    m.sf = F->screen[pos];
+    m.sb = K->screen[pos];
+    return m;
+}
+
+__global__ void __launch_bounds__(1024) k_b_decide(const JobDesc* __restrict__ jobs, const BDesc* __restrict__ Bs) {
+    extern __shared__ int4 ring[];   // [4][mbh]: (forward x, y, backward x, y)
+    const BDesc* B = &Bs[blockIdx.x];
+    const JobDesc* F = &jobs[2 * blockIdx.x];
+    const JobDesc* K = &jobs[2 * blockIdx.x + 1];
     const int mbw = F->mbw, mbh = F->mbh, mode = B->mode;
     const int tmax = (mbw - 1) + 2 * (mbh - 1);
     const double lambda = B->lambda;
     const uint32_t marker = imax(static_cast<int>(mv_bits0(35)), static_cast<int>(mv_bits0(72))) + 1;
+    const uint32_t ftype = B->below_half ? 1u : 2u, btype = B->below_half ? 2u : 1u;
+    const int my = static_cast<int>(threadIdx.x);
+    const bool row = my < mbh;
+    BMb cur{};
+    if (row) cur = b_load(F, K, B, my * mbw);
     for (int t = 0; t <= tmax; ++t) {
-        const int my_lo = imax(0, (t - (mbw - 1) + 1) / 2), my_hi = imin(mbh - 1, t / 2);
-        for (int my = my_lo + static_cast<int>(threadIdx.x); my <= my_hi; my += static_cast<int>(blockDim.x)) {
-            const int mx = t - 2 * my;
+        const int mx = t - 2 * my;
+        if (row && mx >= 0 && mx < mbw) {
             const int pos = my * mbw + mx;
-            const int2 fm = F->mvs[pos], bm = K->mvs[pos];
-            const int2 df = B->direct[2 * pos], db = B->direct[2 * pos + 1];
-            const uint64_t* c = B->cost + 4 * static_cast<size_t>(pos);
-            const int2 fp = b_predictor(B->fmv, mbw, mbh, mx, my);
-            const int2 bp = b_predictor(B->bmv, mbw, mbh, mx, my);
-            const uint32_t fbits = mv_rate_bits(F->mvrange, fm.x, fm.y, fp.x, fp.y, mode, false);
-            const uint32_t bbits = mv_rate_bits(F->mvrange, bm.x, bm.y, bp.x, bp.y, mode, false);
-            const uint32_t ibits = mv_rate_bits(F->mvrange, bm.x, bm.y, bp.x, bp.y, mode, true) + fbits + 2;
-            const uint32_t ftype = B->below_half ? 1u : 2u, btype = B->below_half ? 2u : 1u;
-            const uint64_t cost[4] = {c[0], c[1], c[2], c[3]};
+            const BMb m = cur;
+            if (mx + 1 < mbw) cur = b_load(F, K, B, pos + 1);
+            // neighbours: left (t-1, my), top (t-2, my-1), top-right (t-1, my-1),
+            // top-left in the last column (t-3, my-1)
+            const bool a_valid = my > 0, c_valid = mx > 0, b_valid = my > 0 && mbw > 1;
+            const int4 L = c_valid ? ring[((t - 1) & 3) * mbh + my] : make_int4(0, 0, 0, 0);
+            const int4 T = a_valid ? ring[((t - 2) & 3) * mbh + my - 1] : make_int4(0, 0, 0, 0);
+            const int4 D = !b_valid ? make_int4(0, 0, 0, 0)
+                                    : (mx == mbw - 1 ? ring[((t - 3) & 3) * mbh + my - 1] : ring[((t - 1) & 3) * mbh + my - 1]);
+            const int2 fp = b_predictor(make_int2(T.x, T.y), a_valid, make_int2(D.x, D.y), b_valid, make_int2(L.x, L.y),
+                                        c_valid, mx, my, mbw, mbh);
+            const int2 bp = b_predictor(make_int2(T.z, T.w), a_valid, make_int2(D.z, D.w), b_valid, make_int2(L.z, L.w),
+                                        c_valid, mx, my, mbw, mbh);
+            const uint32_t fbits = mv_rate_bits(F->mvrange, m.fm.x, m.fm.y, fp.x, fp.y, mode, false);
+            const uint32_t bbits = mv_rate_bits(F->mvrange, m.bm.x, m.bm.y, bp.x, bp.y, mode, false);
+            const uint32_t ibits = mv_rate_bits(F->mvrange, m.bm.x, m.bm.y, bp.x, bp.y, mode, true) + fbits + 2;
+            // candidates in the reference's order: direct, forward, backward, interpolated
+            const uint64_t cost[4] = {m.c0, m.c1, m.c2, m.c3};
             const uint32_t rate[4] = {0u, fbits + ftype, bbits + btype, ibits};
             int best = 0;
             uint64_t best_cost = cost[0];
@@ -3247,28 +3287,27 @@ __global__ void __launch_bounds__(256) k_b_decide(const JobDesc* __restrict__ jo
                 const unsigned __int128 sc = b_score(cost[k], rate[k], lambda);
                 if (sc < bs) { bs = sc; best = k; best_cost = cost[k]; best_rate = rate[k]; }
             }
-            const uint64_t isad = B->intra_sad[pos];
             uint64_t gate = best_cost;
-            if (F->screen[pos] < gate) gate = F->screen[pos];
-            if (K->screen[pos] < gate) gate = K->screen[pos];
-            const uint64_t guarded = isad + B->intra_guard;
+            if (m.sf < gate) gate = m.sf;
+            if (m.sb < gate) gate = m.sb;
+            const uint64_t guarded = m.isad + B->intra_guard;
             B->ratio[pos] = gate > 0 ? __ddiv_rn(static_cast<double>(guarded), static_cast<double>(gate))
                                      : __builtin_huge_val();
             const bool intra = B->allow_intra &&
                                guarded < static_cast<uint64_t>(llround_exact(__dmul_rn(static_cast<double>(gate),
                                                                                        B->intra_thr)));
-            // candidate order: direct, forward, backward, interpolated
             int2 fo = make_int2(0, 0), bo = make_int2(0, 0);
-            uint8_t m = 4;
+            uint8_t mm = 4;
             if (!intra) {
-                m = best == 0 ? 3 : static_cast<uint8_t>(best - 1);   // direct, forward, backward, interpolated
-                fo = (best == 1 || best == 3) ? fm : df;
-                bo = (best == 2 || best == 3) ? bm : db;
+                mm = best == 0 ? 3 : static_cast<uint8_t>(best - 1);   // direct, forward, backward, interpolated
+                fo = (best == 1 || best == 3) ? m.fm : m.df;
+                bo = (best == 2 || best == 3) ? m.bm : m.db;
             }
-            B->modes[pos] = m;
+            ring[(t & 3) * mbh + my] = make_int4(fo.x, fo.y, bo.x, bo.y);
+            B->modes[pos] = mm;
             B->fmv[pos] = fo;
             B->bmv[pos] = bo;
-            B->sel_sad[pos] = intra ? isad : best_cost;
+            B->sel_sad[pos] = intra ? m.isad : best_cost;
             B->rate_sad[pos] = best_cost;
             B->sel_rate[pos] = intra ? marker : best_rate;
         }
@@ -3279,14 +3318,17 @@ __global__ void __launch_bounds__(256) k_b_decide(const JobDesc* __restrict__ jo
 
 // ------------------------------------------------------------------ launchers
 
-hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int mbs, hipStream_t s) {
-    if (mbs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_b_costs, dim3(mbs), dim3(64), 0, s, d_jobs, d_b);
+hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbs, hipStream_t s) {
+    if (max_mbs <= 0 || nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_b_costs, dim3(max_mbs, nb), dim3(64), 0, s, d_jobs, d_b);
     return hipGetLastError();
 }
 
-hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, hipStream_t s) {
-    hipLaunchKernelGGL(k_b_decide, dim3(1), dim3(256), 0, s, d_jobs, d_b);
+hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbh, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    if (max_mbh <= 0 || max_mbh > kBMaxRows) return hipErrorInvalidValue;
+    const int threads = (max_mbh + kWave - 1) / kWave * kWave;
+    hipLaunchKernelGGL(k_b_decide, dim3(nb), dim3(threads), 4 * max_mbh * sizeof(int4), s, d_jobs, d_b);
     return hipGetLastError();
 }
 

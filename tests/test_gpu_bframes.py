@@ -40,21 +40,34 @@ def test_b_matches_reference_fixture(backend, path, schedule):
 
 def test_b_ibbp_group_matches_oracle(backend):
     """An IBBP group at 480x272: the P anchor analysed on the GPU, then both B
-    pictures (1/3 and 2/3) against the oracle fed the same anchor."""
+    pictures (1/3 qpel and 2/3 bilinear half-pel) in one batched step with a
+    fixture's picture of another raster, each against the oracle (or the
+    reference-written fixture)."""
     d = synth.pan_rectangles(70, 4, width=480, height=272, max_pan_px=12, max_obj_px=30, frame_indices=[0, 1, 2, 3])
     base = dict(pqindex=9, motion_search_range=256, motion_local_search_range=32)
     fs = {k: backend.upload(v) for k, v in d.items()}
+    fc, fp, ff, fp_, fbin, fwant = load_b(B_FILES[0])
+    extra = [backend.upload(x) for x in (fc, fp, ff)]
     try:
         anchor = backend.analyze(fs[3], fs[0], abi.default_params(480, 272, mode=abi.MODE_MIXED_MV, **base))
+        items, wants = [], []
         for k, frac, mode in ((1, (1, 3), abi.MODE_ONE_MV_QPEL), (2, (2, 3), abi.MODE_ONE_MV_HPEL_BILINEAR)):
             p = abi.default_params(480, 272, mode=mode, **base)
             binp = abi.BInputs(anchor.mvs, anchor.use_4mv, fraction=frac)
-            _say(f"[case] B picture {k} of the group, mode {mode}")
-            got = backend.analyze_b(fs[k], fs[0], fs[3], p, binp)
-            want = oracle_analyze_b(p, d[k], d[0], d[3], binp)
-            assert not got.diff(want), describe_b(got, want)
+            items.append((fs[k], fs[0], fs[3], p, binp))
+            wants.append(oracle_analyze_b(p, d[k], d[0], d[3], binp))
+        items.append((extra[0], extra[1], extra[2], fp_, fbin))
+        wants.append(fwant)
+        _say("[case] B step: two 480x272 pictures and one 176x144 fixture")
+        got = backend.analyze_b_step(items)
+        for g, w in zip(got, wants):
+            assert not g.diff(w), describe_b(g, w)
+        _say("[case] the same pictures one by one")
+        for (c, pa, fu, p, binp), w in zip(items, wants):
+            g = backend.analyze_b(c, pa, fu, p, binp)
+            assert not g.diff(w), describe_b(g, w)
     finally:
-        for f in fs.values():
+        for f in list(fs.values()) + extra:
             f.release()
 
 
