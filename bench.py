@@ -118,10 +118,34 @@ def cpu_baseline(pairs, pics, params):
         secs = lib.ref_time_p_frames(C.byref(params), n, ca, ra, threads, 1)
         secs1 = lib.ref_time_p_frames(C.byref(params), 1, ca, ra, 1, 1)
         if secs > 0:
-            return {"value": round(n * mbs / secs, 1), "unit": "MB/s", "cores": threads, "kind": "reference",
-                    "sample": f"{n} of the 16 1080p P-frame pairs, one pair per thread, reference SIMD auto-selected",
-                    "seconds": round(secs, 3),
-                    "value_1_thread": round(mbs / secs1, 1) if secs1 > 0 else None}
+            out = {"value": round(n * mbs / secs, 1), "unit": "MB/s", "cores": threads, "kind": "reference",
+                   "sample": f"{n} of the 16 1080p P-frame pairs, one pair per thread, reference SIMD auto-selected",
+                   "seconds": round(secs, 3),
+                   "value_1_thread": round(mbs / secs1, 1) if secs1 > 0 else None}
+            if all(k in pics for k in (0, 1, 3)):
+                # one B picture (frame 1 between anchors 0 and 3, 1-MV qpel) on one thread
+                lib.ref_analyze_p_core.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.Planes),
+                                                   C.POINTER(abi.Planes), C.POINTER(abi.PResult), C.c_int]
+                lib.ref_time_b_frame.argtypes = [C.POINTER(abi.Params), C.POINTER(abi.Planes), C.POINTER(abi.Planes),
+                                                 C.POINTER(abi.Planes), C.POINTER(abi.BParams)]
+                lib.ref_time_b_frame.restype = C.c_double
+                p0, p1, p3 = (abi.PlaneSet.of(pics[k]) for k in (0, 1, 3))
+                anchor = abi.ResultArrays(WIDTH, HEIGHT)
+                ap = abi.default_params(WIDTH, HEIGHT, pqindex=params.pqindex,
+                                        motion_search_range=params.motion_search_range,
+                                        motion_local_search_range=params.motion_local_search_range,
+                                        mode=abi.MODE_MIXED_MV, allow_intra=1)
+                lib.ref_analyze_p_core(C.byref(ap), C.byref(p3.struct), C.byref(p0.struct), C.byref(anchor.struct), 1)
+                bp = abi.default_params(WIDTH, HEIGHT, pqindex=params.pqindex,
+                                        motion_search_range=params.motion_search_range,
+                                        motion_local_search_range=params.motion_local_search_range,
+                                        mode=abi.MODE_ONE_MV_QPEL)
+                binp = abi.BInputs(anchor.mvs, anchor.use_4mv, fraction=(1, 3))
+                bs = lib.ref_time_b_frame(C.byref(bp), C.byref(p1.struct), C.byref(p0.struct), C.byref(p3.struct),
+                                          C.byref(binp.struct))
+                if bs > 0:
+                    out["b_picture_ms_1_thread"] = round(bs * 1e3, 2)
+            return out
     # the C restatement, single thread, on one pair
     from _oracle import oracle_analyze
     t0 = time.time()

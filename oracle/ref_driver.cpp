@@ -313,6 +313,35 @@ int ref_analyze_b(const vc1me_params* p, const vc1me_planes* cur, const vc1me_pl
     }
 }
 
+// seconds of one analyze_b_picture_mode call on this thread, SIMD tier
+// auto-selected (the CPU baseline of the B-picture path)
+double ref_time_b_frame(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* past,
+                        const vc1me_planes* future, const vc1me_b_params* b) {
+    try {
+        Vc1Encoder enc(make_config(p, true));
+        const size_t mbs = static_cast<size_t>((p->width + 15) / 16) * ((p->height + 15) / 16);
+        std::vector<Vc1Encoder::MotionVector> amv(mbs);
+        std::vector<uint8_t> a4(mbs);
+        for (size_t i = 0; i < mbs; ++i) {
+            amv[i].xq = b->future_anchor_mvs[2 * i];
+            amv[i].yq = b->future_anchor_mvs[2 * i + 1];
+            a4[i] = b->future_anchor_4mv[i];
+        }
+        Vc1Encoder::IntensityComp ic;
+        const Frame f = make_frame(cur), pa = make_frame(past), fu = make_frame(future);
+        const auto t0 = std::chrono::steady_clock::now();
+        const auto a = enc.analyze_b_picture_mode(f, pa, fu, amv, a4, ic, b->fraction_num, b->fraction_den,
+                                                  static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode), nullptr,
+                                                  nullptr, nullptr);
+        const auto t1 = std::chrono::steady_clock::now();
+        (void)a;
+        return std::chrono::duration<double>(t1 - t0).count();
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1.0;
+    }
+}
+
 int ref_luma_mc_sample(const uint8_t* plane, int w, int h, int x, int y, int xq, int yq, int mode, int rnd) {
     std::vector<uint8_t> p(plane, plane + static_cast<size_t>(w) * h);
     return Vc1Encoder::luma_mc_sample_mode(p, w, h, x, y, xq, yq,
