@@ -319,6 +319,19 @@ typedef struct vc1me_b_job {
 } vc1me_b_job;
 int vc1me_analyze_b_step(vc1me_ctx* ctx, const vc1me_b_job* jobs, int n);
 
+/* ---- scene cuts: Vc1Encoder::scene_change_score (src/encoder_motion.cpp:599-676).
+ * The global-translation MAD of cur against ref (search range
+ * params->motion_search_range, clamped to 32 by the search), the
+ * intensity-compensated one when fade_compensation is set, and for a score
+ * at or above scene_threshold the 1-MV quarter-pel core's MAD if the
+ * interior vector field is a coherent zoom (translation + isotropic scale
+ * within 2.5 px for 60% of vectors, 0.005 <= |scale| <= 0.08) and that MAD
+ * is below 0.72 x the score and 0.86 x the threshold.  The encoder inserts
+ * an I picture when the score reaches its scene threshold (default 28). */
+int vc1me_scene_change_score(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                             const vc1me_params* params, int fade_compensation, double scene_threshold,
+                             double* score);
+
 /* Copies a resident frame's planes back to the host (width x height luma,
  * two width/2 x height/2 chroma planes).  No reference counterpart. */
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);

@@ -27,7 +27,7 @@ EXPORTS = (
     "vc1me_analyze_step_learned", "vc1me_analyze_step_proposals", "vc1me_set_cost_report", "vc1me_last_mb_costs",
     "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
-    "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step",
+    "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
 )
 
 
@@ -75,6 +75,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_analyze_b_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params),
                                           P(abi.BParams), P(abi.BResult)]
     lib.vc1me_analyze_b_step.argtypes = [C.c_void_p, P(abi.BJob), C.c_int]
+    lib.vc1me_scene_change_score.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_int, C.c_double,
+                                             P(C.c_double)]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
@@ -205,6 +207,14 @@ class Backend:
             jobs[i].out = C.pointer(r.struct)
         self._check(self.lib.vc1me_analyze_b_step(self.ctx, jobs, n))
         return outs
+
+    def scene_change_score(self, cur: Frame, ref: Frame, params: abi.Params, fade: bool = True,
+                           threshold: float = 28.0) -> float:
+        """vc1me_scene_change_score."""
+        out = C.c_double()
+        self._check(self.lib.vc1me_scene_change_score(self.ctx, cur.handle, ref.handle, C.byref(params), int(fade),
+                                                      float(threshold), C.byref(out)))
+        return out.value
 
     def analyze(self, cur: Frame, ref: Frame, params: abi.Params) -> abi.ResultArrays:
         r = abi.ResultArrays(params.width, params.height)

@@ -1234,14 +1234,12 @@ vc1me::IntensityLuts intensity_luts(const vc1me_intensity& ic) {
 
 }  // namespace
 
-int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
-                                  const int32_t* mvs, vc1me_intensity* out) {
-    if (!ctx || !cur || !ref || !mvs || !out) return fail(VC1ME_EINVAL, "null argument");
-    if (cur->w != ref->w || cur->h != ref->h) return fail(VC1ME_EINVAL, "cur and ref rasters differ");
+namespace {
+
+// estimate_intensity_comp on resident frames (context lock held)
+int estimate_ic(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const int32_t* mvs,
+                vc1me_intensity* out) {
     out->enabled = 0; out->lumscale = 32; out->lumshift = 0;
-    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    if (int rc = set_device(ctx)) return rc;
     const int W = cur->w, H = cur->h, mbw = (W + 15) / 16, mbh = (H + 15) / 16;
     const size_t mbs = static_cast<size_t>(mbw) * mbh;
     const int n = ((W + 3) / 4) * ((H + 3) / 4);
@@ -1310,6 +1308,19 @@ int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const 
     out->lumscale = best_ls;
     out->lumshift = best_sh < 0 ? best_sh + 64 : best_sh;
     return VC1ME_OK;
+}
+
+}  // namespace
+
+int vc1me_estimate_intensity_comp(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                  const int32_t* mvs, vc1me_intensity* out) {
+    if (!ctx || !cur || !ref || !mvs || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (cur->w != ref->w || cur->h != ref->h) return fail(VC1ME_EINVAL, "cur and ref rasters differ");
+    out->enabled = 0; out->lumscale = 32; out->lumshift = 0;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return estimate_ic(ctx, cur, ref, mvs, out);
 }
 
 namespace {
@@ -1602,6 +1613,127 @@ int vc1me_analyze_b_frame(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_fr
                           const vc1me_params* params, const vc1me_b_params* b, vc1me_b_result* out) {
     const vc1me_b_job j{cur, past, future, params, b, out};
     return vc1me_analyze_b_step(ctx, &j, 1);
+}
+
+// ---- scene cuts
+
+namespace {
+
+// global_motion_search(cur, ref, range) on its own (range clamped to 32 by the
+// search itself, src/encoder_motion.cpp:2076-2141); context lock held
+int global_only(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const vc1me_params* p, int range,
+                vc1me::GlobalState* out) {
+    vc1me_params q = *p;
+    // a job whose global search runs at min(range, 32); range 0 is the plain
+    // frame SAD at (0, 0)
+    q.motion_search_range = std::max(range, 1);
+    q.motion_local_search_range = std::max(range, 0);
+    const size_t mbs = static_cast<size_t>((cur->w + 15) / 16) * ((cur->h + 15) / 16);
+    const size_t cells = static_cast<size_t>((cur->w + 7) / 8) * ((cur->h + 7) / 8);
+    if (ctx->slots.empty()) ctx->slots.resize(1);
+    if (ctx->h_jobs.empty()) ctx->h_jobs.resize(1);
+    hipError_t e = ensure_slot(ctx->slots[0], mbs, cells);
+    if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("device state allocation: ") + hipGetErrorString(e));
+    if (ctx->d_jobs_cap < 1) {
+        HIP_TRY(hipMalloc(&ctx->d_jobs, sizeof(JobDesc)));
+        ctx->d_jobs_cap = 1;
+    }
+    JobDesc& d = ctx->h_jobs[0];
+    fill_job(d, &q, cur, ref, ctx->slots[0]);
+    hipStream_t st = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(ctx->d_jobs, &d, sizeof(JobDesc), hipMemcpyHostToDevice, st));
+    HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, 1, &d, st, nullptr, nullptr));
+    HIP_TRY(hipMemcpyAsync(out, ctx->slots[0].g, sizeof(vc1me::GlobalState), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    ctx->last_jobs = 0;
+    return VC1ME_OK;
+}
+
+}  // namespace
+
+int vc1me_scene_change_score(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                             const vc1me_params* params, int fade_compensation, double scene_threshold,
+                             double* score) {
+    if (!ctx || !cur || !ref || !params || !score) return fail(VC1ME_EINVAL, "null argument");
+    if (cur->w != ref->w || cur->h != ref->h || params->width != cur->w || params->height != cur->h)
+        return fail(VC1ME_EINVAL, "scene-analysis dimensions do not match the parameters");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const int W = cur->w, H = cur->h, mbw = (W + 15) / 16, mbh = (H + 15) / 16;
+    const size_t mbs = static_cast<size_t>(mbw) * mbh;
+    const uint64_t pixels = static_cast<uint64_t>(W) * H;
+    vc1me::GlobalState g;
+    if (int rc = global_only(ctx, cur, ref, params, params->motion_search_range, &g)) return rc;
+    double best = pixels ? static_cast<double>(g.full_sad) / static_cast<double>(pixels) : 0.0;
+    if (fade_compensation) {
+        // the fade fit on the uniform global field, then the global search on
+        // the compensated reference (src/encoder_motion.cpp:606-615)
+        std::vector<int32_t> uni(2 * mbs);
+        for (size_t i = 0; i < mbs; ++i) { uni[2 * i] = g.dx * 4; uni[2 * i + 1] = g.dy * 4; }
+        vc1me_intensity ic;
+        if (int rc = estimate_ic(ctx, cur, ref, uni.data(), &ic)) return rc;
+        if (ic.enabled) {
+            vc1me_frame* comp = nullptr;
+            if (int rc = compensated_frame(ctx, ref, &ic, &comp)) return rc;
+            vc1me::GlobalState wg;
+            const int rc = global_only(ctx, cur, comp, params, params->motion_search_range, &wg);
+            (void)hipFree(comp->y);
+            delete comp;
+            if (rc) return rc;
+            const double wm = pixels ? static_cast<double>(wg.full_sad) / static_cast<double>(pixels) : 0.0;
+            best = std::min(best, wm);
+        }
+    }
+    if (best >= scene_threshold) {
+        // a zoom rescue: the 1-MV quarter-pel core, its interior field fitted
+        // to translation + isotropic zoom (src/encoder_motion.cpp:627-673)
+        vc1me_params q = *params;
+        q.mode = VC1ME_MODE_ONE_MV_QPEL;
+        q.allow_intra = 0;
+        std::vector<int32_t> mv(2 * mbs);
+        vc1me_p_result local{};
+        local.mvs = mv.data();
+        const vc1me_job job{cur, ref, &q, &local};
+        if (int rc = run_jobs(ctx, &job, 1)) return rc;
+        const double lm = local.mean_abs_residual;
+        long double mx = 0.0L, my = 0.0L, mdx = 0.0L, mdy = 0.0L;
+        size_t samples = 0;
+        for (int by = 1; by < mbh - 1; ++by)
+            for (int bx = 1; bx < mbw - 1; ++bx) {
+                const size_t pos = static_cast<size_t>(by) * mbw + bx;
+                mx += bx * 16.0L + 8.0L; my += by * 16.0L + 8.0L;
+                mdx += mv[2 * pos] / 4.0L; mdy += mv[2 * pos + 1] / 4.0L;
+                ++samples;
+            }
+        bool coherent = false;
+        if (samples >= 12) {
+            mx /= samples; my /= samples; mdx /= samples; mdy /= samples;
+            long double num = 0.0L, den = 0.0L;
+            for (int by = 1; by < mbh - 1; ++by)
+                for (int bx = 1; bx < mbw - 1; ++bx) {
+                    const size_t pos = static_cast<size_t>(by) * mbw + bx;
+                    const long double x = bx * 16.0L + 8.0L - mx, y = by * 16.0L + 8.0L - my;
+                    const long double dx = mv[2 * pos] / 4.0L - mdx, dy = mv[2 * pos + 1] / 4.0L - mdy;
+                    num += x * dx + y * dy;
+                    den += x * x + y * y;
+                }
+            const long double z = den > 0.0L ? num / den : 0.0L;
+            size_t inliers = 0;
+            for (int by = 1; by < mbh - 1; ++by)
+                for (int bx = 1; bx < mbw - 1; ++bx) {
+                    const size_t pos = static_cast<size_t>(by) * mbw + bx;
+                    const long double x = bx * 16.0L + 8.0L - mx, y = by * 16.0L + 8.0L - my;
+                    const long double ex = mv[2 * pos] / 4.0L - (mdx + z * x);
+                    const long double ey = mv[2 * pos + 1] / 4.0L - (mdy + z * y);
+                    if (ex * ex + ey * ey <= 6.25L) ++inliers;
+                }
+            const long double az = std::fabs(z);
+            coherent = inliers * 5 >= samples * 3 && az >= 0.005L && az <= 0.08L;
+        }
+        if (coherent && lm < best * 0.72 && lm < scene_threshold * 0.86) best = lm;
+    }
+    *score = best;
+    return VC1ME_OK;
 }
 
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v) {

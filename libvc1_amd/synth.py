@@ -330,3 +330,21 @@ def fade(pic: Picture, gain: float, offset: float) -> Picture:
     y = np.clip(np.rint(gain * pic.y.astype(np.float64) + offset), 0, 255).astype(np.uint8)
     uv = lambda p: np.clip(np.rint(128.0 + gain * (p.astype(np.float64) - 128.0)), 0, 255).astype(np.uint8)
     return Picture(y, uv(pic.u), uv(pic.v))
+
+
+def zoom(pic: Picture, scale: float) -> Picture:
+    """`pic` magnified by `scale` about its centre (bilinear resampling,
+    clamped edges): a camera zoom that no single translation models."""
+    def resample(p: np.ndarray) -> np.ndarray:
+        h, w = p.shape
+        ys, xs = np.mgrid[0:h, 0:w].astype(np.float64)
+        sy = (ys - (h - 1) / 2.0) / scale + (h - 1) / 2.0
+        sx = (xs - (w - 1) / 2.0) / scale + (w - 1) / 2.0
+        y0 = np.clip(np.floor(sy).astype(int), 0, h - 1)
+        x0 = np.clip(np.floor(sx).astype(int), 0, w - 1)
+        y1, x1 = np.clip(y0 + 1, 0, h - 1), np.clip(x0 + 1, 0, w - 1)
+        fy, fx = np.clip(sy - y0, 0, 1), np.clip(sx - x0, 0, 1)
+        a = p.astype(np.float64)
+        v = (a[y0, x0] * (1 - fx) + a[y0, x1] * fx) * (1 - fy) + (a[y1, x0] * (1 - fx) + a[y1, x1] * fx) * fy
+        return np.clip(np.rint(v), 0, 255).astype(np.uint8)
+    return Picture(resample(pic.y), resample(pic.u), resample(pic.v))

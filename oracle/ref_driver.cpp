@@ -17,6 +17,7 @@
 //   estimate_intensity_comp  src/encoder_motion_compensation.cpp:296-348
 //   intensity_compensated_reference  src/encoder_motion_compensation.cpp:195-294
 //   analyze_b_picture_mode   src/encoder_motion.cpp:753-1004
+//   scene_change_score       src/encoder_motion.cpp:599-676
 #include "encoder_internal.h"
 
 #include "../include/vc1me.h"
@@ -339,6 +340,21 @@ double ref_time_b_frame(const vc1me_params* p, const vc1me_planes* cur, const vc
     } catch (const std::exception& e) {
         g_error = e.what();
         return -1.0;
+    }
+}
+
+int ref_scene_change_score(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                           int fade_compensation, double scene_threshold, double* out) {
+    try {
+        EncoderConfig cfg = make_config(p, false);
+        cfg.fade_compensation = fade_compensation != 0;
+        cfg.scene_threshold = scene_threshold;
+        Vc1Encoder enc(cfg);
+        *out = enc.scene_change_score(make_frame(cur), make_frame(ref));
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
     }
 }
 

@@ -2075,3 +2075,97 @@ cleanup:
     free(fout); free(bout);
     return rc;
 }
+
+/* ------------------------------------------------------------------ scene cuts */
+
+/* global_motion_search(cur, ref, range) as a frame-level call (the search
+ * range is clamped to 32, src/encoder_motion.cpp:2076-2141) */
+static cand_t global_for(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref, int range) {
+    ctx_t cx;
+    init_ctx(&cx, p, cur, ref, NULL);
+    return global_search(&cx, range);
+}
+
+/* Vc1Encoder::scene_change_score (src/encoder_motion.cpp:599-676): the
+ * global-translation MAD, the intensity-compensated one when fade
+ * compensation is on, and, for a score at or above the threshold, the 1-MV
+ * quarter-pel core's MAD if the interior vector field fits translation plus
+ * isotropic zoom (>= 60% of vectors within 2.5 px, 0.005 <= |zoom| <= 0.08)
+ * and that MAD is clearly lower. */
+int oracle_scene_change_score(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                              int fade_compensation, double scene_threshold, double* out) {
+    const int W = p->width, H = p->height;
+    if (cur->width != W || cur->height != H || ref->width != W || ref->height != H) return -1;
+    const cand_t g = global_for(p, cur, ref, p->motion_search_range);
+    const uint64_t pixels = (uint64_t)W * H;
+    double best = pixels ? (double)g.cost / (double)pixels : 0.0;
+    const int mbw = (W + 15) / 16, mbh = (H + 15) / 16;
+    const size_t mbs = (size_t)mbw * mbh;
+    if (fade_compensation) {
+        int32_t* uni = (int32_t*)malloc(mbs * 2 * sizeof(int32_t));
+        for (size_t i = 0; i < mbs; ++i) { uni[2 * i] = g.dx * 4; uni[2 * i + 1] = g.dy * 4; }
+        vc1me_intensity ic;
+        oracle_estimate_intensity_comp(cur, ref, uni, &ic);
+        free(uni);
+        if (ic.enabled) {
+            const size_t ys = (size_t)W * H, cs = (size_t)(W / 2) * (H / 2);
+            uint8_t* wy = (uint8_t*)malloc(ys);
+            uint8_t* wu = (uint8_t*)malloc(cs);
+            uint8_t* wv = (uint8_t*)malloc(cs);
+            oracle_intensity_compensate(ref, &ic, wy, wu, wv);
+            const vc1me_planes wp = {wy, wu, wv, W, H};
+            const cand_t wg = global_for(p, cur, &wp, p->motion_search_range);
+            const double wm = pixels ? (double)wg.cost / (double)pixels : 0.0;
+            if (wm < best) best = wm;
+            free(wy); free(wu); free(wv);
+        }
+    }
+    if (best >= scene_threshold) {
+        vc1me_params q = *p;
+        q.mode = VC1ME_MODE_ONE_MV_QPEL;
+        q.allow_intra = 0;
+        vc1me_p_result local;
+        p_result_alloc(&local, mbs);
+        const int rc = analyze_core(&q, cur, ref, NULL, &local, NULL);
+        if (rc) { p_result_free(&local); return rc; }
+        const double lm = local.mean_abs_residual;
+        long double mx = 0.0L, my = 0.0L, mdx = 0.0L, mdy = 0.0L;
+        size_t samples = 0;
+        for (int by = 1; by < mbh - 1; ++by)
+            for (int bx = 1; bx < mbw - 1; ++bx) {
+                const size_t pos = (size_t)by * mbw + bx;
+                mx += bx * 16.0L + 8.0L; my += by * 16.0L + 8.0L;
+                mdx += local.mvs[2 * pos] / 4.0L; mdy += local.mvs[2 * pos + 1] / 4.0L;
+                ++samples;
+            }
+        bool coherent = false;
+        if (samples >= 12) {
+            mx /= samples; my /= samples; mdx /= samples; mdy /= samples;
+            long double num = 0.0L, den = 0.0L;
+            for (int by = 1; by < mbh - 1; ++by)
+                for (int bx = 1; bx < mbw - 1; ++bx) {
+                    const size_t pos = (size_t)by * mbw + bx;
+                    const long double x = bx * 16.0L + 8.0L - mx, y = by * 16.0L + 8.0L - my;
+                    const long double dx = local.mvs[2 * pos] / 4.0L - mdx, dy = local.mvs[2 * pos + 1] / 4.0L - mdy;
+                    num += x * dx + y * dy;
+                    den += x * x + y * y;
+                }
+            const long double z = den > 0.0L ? num / den : 0.0L;
+            size_t inliers = 0;
+            for (int by = 1; by < mbh - 1; ++by)
+                for (int bx = 1; bx < mbw - 1; ++bx) {
+                    const size_t pos = (size_t)by * mbw + bx;
+                    const long double x = bx * 16.0L + 8.0L - mx, y = by * 16.0L + 8.0L - my;
+                    const long double ex = local.mvs[2 * pos] / 4.0L - (mdx + z * x);
+                    const long double ey = local.mvs[2 * pos + 1] / 4.0L - (mdy + z * y);
+                    if (ex * ex + ey * ey <= 6.25L) ++inliers;
+                }
+            const long double az = fabsl(z);
+            coherent = inliers * 5 >= samples * 3 && az >= 0.005L && az <= 0.08L;
+        }
+        if (coherent && lm < best * 0.72 && lm < scene_threshold * 0.86) best = lm;
+        p_result_free(&local);
+    }
+    *out = best;
+    return 0;
+}

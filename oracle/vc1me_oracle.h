@@ -103,6 +103,11 @@ int oracle_estimate_intensity_comp(const vc1me_planes* cur, const vc1me_planes* 
 int oracle_analyze_b_frame(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* past,
                            const vc1me_planes* future, const vc1me_b_params* b, vc1me_b_result* out);
 
+/* scene_change_score (src/encoder_motion.cpp:599-676) under the ME settings
+ * p, fade_compensation and scene_threshold of the encoder */
+int oracle_scene_change_score(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                              int fade_compensation, double scene_threshold, double* out);
+
 #ifdef __cplusplus
 }
 #endif

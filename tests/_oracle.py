@@ -64,6 +64,8 @@ def oracle() -> C.CDLL:
         lib.oracle_intensity_map_uv.restype = C.c_uint8
         lib.oracle_analyze_b_frame.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.Planes),
                                                P(abi.BParams), P(abi.BResult)]
+        lib.oracle_scene_change_score.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), C.c_int, C.c_double,
+                                                  P(C.c_double)]
         lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                       P(abi.PResult), P(OracleOps)]
         _oracle = lib
@@ -98,6 +100,8 @@ def reference() -> C.CDLL:
                                                  C.c_void_p, C.c_int]
         lib.ref_analyze_b.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.Planes), P(abi.BParams),
                                       P(abi.BResult)]
+        lib.ref_scene_change_score.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), C.c_int, C.c_double,
+                                               P(C.c_double)]
         lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                   P(abi.PResult), C.c_int]
         _ref = lib
@@ -159,6 +163,24 @@ def oracle_analyze_b(params: abi.Params, cur, past, future, binp: abi.BInputs) -
 
 def reference_analyze_b(params: abi.Params, cur, past, future, binp: abi.BInputs) -> abi.BResultArrays:
     return _b(reference().ref_analyze_b, params, cur, past, future, binp, lambda: reference().ref_last_error())
+
+
+def _scene(fn, params, cur, ref, fade, threshold, *err):
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    out = C.c_double()
+    rc = fn(C.byref(params), C.byref(c.struct), C.byref(r.struct), int(fade), float(threshold), C.byref(out))
+    assert rc == 0, err[0]() if err else rc
+    return out.value
+
+
+def oracle_scene_change_score(params: abi.Params, cur, ref, fade: bool = True, threshold: float = 28.0) -> float:
+    """scene_change_score in the oracle under the ME settings `params`."""
+    return _scene(oracle().oracle_scene_change_score, params, cur, ref, fade, threshold)
+
+
+def reference_scene_change_score(params: abi.Params, cur, ref, fade: bool = True, threshold: float = 28.0) -> float:
+    return _scene(reference().ref_scene_change_score, params, cur, ref, fade, threshold,
+                  lambda: reference().ref_last_error())
 
 
 def _mv_pairs(mvs: np.ndarray) -> np.ndarray:

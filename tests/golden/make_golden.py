@@ -17,6 +17,7 @@ The fixtures are data only.
     python tests/golden/make_golden.py              # everything
     python tests/golden/make_golden.py intensity    # intensity.json only
     python tests/golden/make_golden.py bframes      # bframes/*.npz only
+    python tests/golden/make_golden.py scene        # scene.json only (scene_change_score)
 """
 from __future__ import annotations
 
@@ -33,7 +34,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from libvc1_amd import abi, synth  # noqa: E402
-from _oracle import (oracle_analyze, reference_analyze, reference_analyze_b,  # noqa: E402
+from _scene_cases import SCENE_CASES, scene_inputs  # noqa: E402
+from _oracle import (oracle_analyze, reference_analyze, reference_analyze_b, reference_scene_change_score,  # noqa: E402
                      reference_analyze_seeded, reference_estimate_intensity, reference_intensity_compensate)
 
 
@@ -263,7 +265,23 @@ def b_fixtures() -> None:
         print(f"wrote bframes/{name}: {res.scalars()}")
 
 
+def scene_fixtures() -> list:
+    p = abi.default_params(320, 176, pqindex=8, motion_search_range=128, motion_local_search_range=32)
+    out = []
+    for name, fade, thr in SCENE_CASES:
+        cur, ref = scene_inputs(name)
+        v = reference_scene_change_score(p, cur, ref, fade, thr)
+        out.append({"case": name, "fade": fade, "threshold": thr, "score": v,
+                    "score_bits": int(np.float64(v).view(np.uint64))})
+        print(f"scene {name} fade={fade} thr={thr}: {v}")
+    return out
+
+
 def main() -> int:
+    if sys.argv[1:] == ["scene"]:
+        with open(os.path.join(HERE, "scene.json"), "w") as f:
+            json.dump(scene_fixtures(), f, indent=1)
+        return 0
     if sys.argv[1:] == ["bframes"]:
         b_fixtures()
         return 0
@@ -304,6 +322,8 @@ def main() -> int:
     with open(os.path.join(HERE, "intensity.json"), "w") as f:
         json.dump(intensity_fixtures(), f)
     b_fixtures()
+    with open(os.path.join(HERE, "scene.json"), "w") as f:
+        json.dump(scene_fixtures(), f, indent=1)
     return 0
 
 
