@@ -332,6 +332,20 @@ int vc1me_scene_change_score(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
                              const vc1me_params* params, int fade_compensation, double scene_threshold,
                              double* score);
 
+/* ---- motion failure: Vc1Encoder::has_good_motion_match
+ * (src/encoder_motion.cpp:678-751).  *good = 1 when some macroblock of cur
+ * has a usable match in ref (against the fade-compensated reference when
+ * fade_compensation finds a code on the global field): the P search's
+ * integer and long-range stages refined by the SAD-only refine_qpel_block
+ * (:3596-3685), usable at <= 12 SAD per pixel or <= 90% of the zero-vector
+ * SAD.  The encoder inserts an emergency I picture when it is 0.
+ * field (NULL when not wanted; no reference counterpart): every
+ * macroblock's refined quarter-pel vector, mbw * mbh (x, y) pairs, as the
+ * reference's scan would compute them if it did not stop at the first
+ * usable match. */
+int vc1me_has_good_motion_match(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                const vc1me_params* params, int fade_compensation, int* good, int32_t* field);
+
 /* Copies a resident frame's planes back to the host (width x height luma,
  * two width/2 x height/2 chroma planes).  No reference counterpart. */
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);

@@ -28,6 +28,7 @@ EXPORTS = (
     "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
     "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
+    "vc1me_has_good_motion_match",
 )
 
 
@@ -77,6 +78,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_analyze_b_step.argtypes = [C.c_void_p, P(abi.BJob), C.c_int]
     lib.vc1me_scene_change_score.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_int, C.c_double,
                                              P(C.c_double)]
+    lib.vc1me_has_good_motion_match.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_int,
+                                                P(C.c_int), C.c_void_p]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
@@ -207,6 +210,17 @@ class Backend:
             jobs[i].out = C.pointer(r.struct)
         self._check(self.lib.vc1me_analyze_b_step(self.ctx, jobs, n))
         return outs
+
+    def has_good_motion_match(self, cur: Frame, ref: Frame, params: abi.Params, fade: bool = True,
+                              want_field: bool = False):
+        """vc1me_has_good_motion_match: the answer, or (answer, field) with
+        every macroblock's refined vector."""
+        out = C.c_int()
+        mbw, mbh = abi.mb_grid(cur.width, cur.height)
+        field = np.zeros((mbw * mbh, 2), np.int32) if want_field else None
+        self._check(self.lib.vc1me_has_good_motion_match(self.ctx, cur.handle, ref.handle, C.byref(params), int(fade),
+                                                         C.byref(out), field.ctypes.data if want_field else None))
+        return (bool(out.value), field) if want_field else bool(out.value)
 
     def scene_change_score(self, cur: Frame, ref: Frame, params: abi.Params, fade: bool = True,
                            threshold: float = 28.0) -> float:

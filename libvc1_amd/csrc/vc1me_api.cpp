@@ -416,6 +416,7 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.wl0 = s.wl0; d.wl1 = s.wl1; d.mark = s.mark;
     d.nn_out = nullptr; d.nn_logit = 0.f;
     d.seed_mvs = nullptr;
+    d.motion_check = 0;
     d.cost20 = nullptr; d.one_mv = nullptr;
     d.lam20 = std::max(20, 7 * p->pqindex);
     d.max_diag = d.mbw + 2 * d.mbh - 3;
@@ -538,7 +539,7 @@ int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_res
 // learned form runs its network there, on the global vectors in the slots).
 int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* nn_out = nullptr, float nn_logit = 0.f,
              const std::function<int(hipStream_t)>& after_global = {},
-             const vc1me_p_result* const* seeds = nullptr) {
+             const vc1me_p_result* const* seeds = nullptr, bool motion_check = false) {
     if (n <= 0) return VC1ME_OK;
     if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
     if (n > kMaxJobs) return fail(VC1ME_ELIMIT, "at most " + std::to_string(kMaxJobs) + " jobs per step");
@@ -558,6 +559,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         hipError_t e = ensure_slot(ctx->slots[i], mbs, cells);
         if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("device state allocation: ") + hipGetErrorString(e));
         fill_job(ctx->h_jobs[i], p, jobs[i].cur, jobs[i].ref, ctx->slots[i]);
+        ctx->h_jobs[i].motion_check = motion_check ? 1 : 0;
         if (nn_out) {
             ctx->h_jobs[i].nn_out = nn_out[i];
             ctx->h_jobs[i].nn_logit = nn_logit;
@@ -748,7 +750,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         for (int i = 0; i < n; ++i) {
             const JobDesc& d = ctx->h_jobs[i];
             put(d.W); put(d.H); put(d.range); put(d.local_range); put(d.mbw); put(d.mbh); put(d.max_diag);
-            put(d.mode); put(d.phase_owner); put(d.seed_mvs ? 1 : 0);
+            put(d.mode); put(d.phase_owner); put(d.seed_mvs ? 1 : 0); put(d.motion_check);
         }
         hipGraphExec_t exec = nullptr;
         for (auto& g : ctx->graphs)
@@ -1733,6 +1735,51 @@ int vc1me_scene_change_score(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
         if (coherent && lm < best * 0.72 && lm < scene_threshold * 0.86) best = lm;
     }
     *score = best;
+    return VC1ME_OK;
+}
+
+int vc1me_has_good_motion_match(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                const vc1me_params* params, int fade_compensation, int* good, int32_t* field) {
+    if (!ctx || !cur || !ref || !params || !good) return fail(VC1ME_EINVAL, "null argument");
+    if (cur->w != ref->w || cur->h != ref->h || params->width != cur->w || params->height != cur->h)
+        return fail(VC1ME_EINVAL, "motion-failure analysis dimensions do not match the parameters");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    *good = 0;
+    const int W = cur->w, H = cur->h, mbw = (W + 15) / 16, mbh = (H + 15) / 16;
+    const size_t mbs = static_cast<size_t>(mbw) * mbh;
+    const int range = std::max(0, params->motion_search_range);
+    const int local_range = std::min(range, std::max(0, params->motion_local_search_range));
+    // the fade fit on the uniform global field picks the prediction reference
+    // (src/encoder_motion.cpp:684-697); the step's own global search then runs
+    // on that reference, as the reference's second global search does
+    vc1me_frame* comp = nullptr;
+    if (fade_compensation) {
+        vc1me::GlobalState g;
+        if (int rc = global_only(ctx, cur, ref, params, local_range, &g)) return rc;
+        std::vector<int32_t> uni(2 * mbs);
+        for (size_t i = 0; i < mbs; ++i) { uni[2 * i] = g.dx * 4; uni[2 * i + 1] = g.dy * 4; }
+        vc1me_intensity ic;
+        if (int rc = estimate_ic(ctx, cur, ref, uni.data(), &ic)) return rc;
+        if (ic.enabled)
+            if (int rc = compensated_frame(ctx, ref, &ic, &comp)) return rc;
+    }
+    struct Release {
+        vc1me_frame* f;
+        ~Release() { if (f) { (void)hipFree(f->y); delete f; } }
+    } release{comp};
+    vc1me_params q = *params;
+    q.mode = VC1ME_MODE_ONE_MV_QPEL;
+    q.allow_intra = 0;
+    q.motion_search_range = range;
+    const vc1me_job job{cur, comp ? comp : ref, &q, nullptr};
+    if (int rc = run_jobs(ctx, &job, 1, nullptr, 0.f, {}, nullptr, true)) return rc;
+    // a usable match anywhere: the raster scan's early return finds the first
+    // one, and every macroblock's analysis equals the scan's up to there
+    std::vector<uint8_t> flags(mbs);
+    HIP_TRY(hipMemcpy(flags.data(), ctx->slots[0].flags, mbs, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < mbs && !*good; ++i) *good = (flags[i] & vc1me::kFlagGoodMatch) ? 1 : 0;
+    if (field) HIP_TRY(hipMemcpy(field, ctx->slots[0].mvs, mbs * sizeof(int2), hipMemcpyDeviceToHost));
     return VC1ME_OK;
 }
 

@@ -91,6 +91,9 @@ struct JobDesc {
     // seeded core (null: the full search): each macroblock's 1-MV start
     // vector, quarter-pel (analyze_p_picture_core's shared seed)
     const int2* seed_mvs;
+    // motion-failure check (has_good_motion_match): the integer and long-range
+    // stages, then refine_qpel_block; flags carry kFlagGoodMatch
+    int32_t motion_check;
 };
 
 // Longest relaxation run before the step falls back to the anti-diagonal
@@ -106,6 +109,7 @@ inline int32_t phase_rows(int h) { return h + 3; }
 // Flag bits of JobDesc::flags
 constexpr uint8_t kFlag4mvChosen = 1;
 constexpr uint8_t kFlagIntra = 2;
+constexpr uint8_t kFlagGoodMatch = 4;   // motion-failure check: a usable match
 
 // Kernel launchers (vc1me_kernels.hip).
 hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc* h_jobs, hipStream_t s,

@@ -1865,6 +1865,94 @@ __device__ __forceinline__ int2 refine_integer_only(Ctx& C, const SubParams& sp,
     return make_int2(ix, iy);
 }
 
+// refine_qpel_block, src/encoder_motion.cpp:3596-3685: the SAD-only
+// quarter-pel refinement of the motion-failure check.  Each stage scores its
+// candidates at once (one per lane group) and takes the least in the
+// (SAD, |mv|, y, x) order, which is the reference's sequential scan; a
+// candidate equal to the seed is skipped, so the seed keeps its given cost as
+// in the scan's visited list (other repeats score equal to their first visit
+// and cannot win).
+__device__ __forceinline__ bool qpel_key_less(uint64_t s, int x, int y, uint64_t bs, int bx, int by) {
+    if (s != bs) return s < bs;
+    const int m = iabs(x) + iabs(y), bm = iabs(bx) + iabs(by);
+    if (m != bm) return m < bm;
+    if (y != by) return y < by;
+    return x < bx;
+}
+template <typename F>
+__device__ __forceinline__ void qpel_sad_stage(Ctx& C, int n, F cand, int xmin, int xmax, int ymin, int ymax, int2 seed,
+                                               uint64_t& cost, int2& best) {
+    int cx = 0, cy = 0;
+    if (C.lane < n) {
+        cand(C.lane, cx, cy);
+        cx = clampi(cx, xmin, xmax);
+        cy = clampi(cy, ymin, ymax);
+    }
+    const bool skip = C.lane >= n || (cx == seed.x && cy == seed.y);
+    const uint64_t sad = qpel_eval(C, C.x0, C.y0, 16, 16, cx, cy, !skip, 0, n, 0, 0, 0);
+    for (int k = 0; k < n; ++k) {
+        const int kx = __shfl(cx, k), ky = __shfl(cy, k);
+        const bool ks = __shfl(skip ? 1 : 0, k) != 0;
+        const uint64_t v = lane_read64(sad, k);
+        if (!ks && qpel_key_less(v, kx, ky, cost, best.x, best.y)) { cost = v; best = make_int2(kx, ky); }
+    }
+}
+__device__ __forceinline__ int2 refine_qpel_sad(Ctx& C, int range, int sxq, int syq, uint64_t known,
+                                                uint64_t* best_sad) {
+    const JobDesc* J = C.J;
+    const int limit_q = 4 * J->subpel_range;
+    if (!J->subpel || limit_q == 0 || imax(iabs(sxq), iabs(syq)) > limit_q) {
+        const SubParams sp{C.x0, C.y0, 16, 16, 0, range, true};
+        return refine_integer_only(C, sp, sxq, syq, best_sad, known);
+    }
+    const Bounds qb = bounds_qpel(range);
+    const int xmin = imax(qb.xmin, -limit_q), xmax = imin(qb.xmax, limit_q);
+    const int ymin = imax(qb.ymin, -limit_q), ymax = imin(qb.ymax, limit_q);
+    int2 seed = make_int2(clampi(sxq, xmin, xmax), clampi(syq, ymin, ymax));
+    if (hidden_padding(C.g, C.x0, C.y0, 16, 16, seed.x, seed.y, 0, true)) {
+        seed = make_int2(0, 0);
+        known = UINT64_MAX;
+    }
+    int2 best = seed;
+    uint64_t cost = known != UINT64_MAX
+                        ? known
+                        : lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, seed.x, seed.y, true, 0, 1, 0, 0, 0), 0);
+    if (J->fast) {
+        if (cost == 0 && best.x == 0 && best.y == 0) { *best_sad = 0; return best; }
+        qpel_sad_stage(C, 4, [&](int k, int& x, int& y) VC1ME_INLINE {
+            x = seed.x + (k == 0 ? -2 : k == 1 ? 2 : 0); y = seed.y + (k == 2 ? -2 : k == 3 ? 2 : 0); },
+            xmin, xmax, ymin, ymax, seed, cost, best);
+        const int2 axis = best;
+        const int dx = axis.x - seed.x, dy = axis.y - seed.y;
+        if (dx && !dy)
+            qpel_sad_stage(C, 2, [&](int k, int& x, int& y) VC1ME_INLINE { x = axis.x; y = axis.y + (k ? 2 : -2); },
+                           xmin, xmax, ymin, ymax, seed, cost, best);
+        else if (dy && !dx)
+            qpel_sad_stage(C, 2, [&](int k, int& x, int& y) VC1ME_INLINE { x = axis.x + (k ? 2 : -2); y = axis.y; },
+                           xmin, xmax, ymin, ymax, seed, cost, best);
+        const int2 half = best;
+        qpel_sad_stage(C, 4, [&](int k, int& x, int& y) VC1ME_INLINE {
+            x = half.x + (k == 0 ? -1 : k == 1 ? 1 : 0); y = half.y + (k == 2 ? -1 : k == 3 ? 1 : 0); },
+            xmin, xmax, ymin, ymax, seed, cost, best);
+        const int2 qaxis = best;
+        const int qdx = qaxis.x - half.x, qdy = qaxis.y - half.y;
+        if (qdx && !qdy)
+            qpel_sad_stage(C, 2, [&](int k, int& x, int& y) VC1ME_INLINE { x = qaxis.x; y = qaxis.y + (k ? 1 : -1); },
+                           xmin, xmax, ymin, ymax, seed, cost, best);
+        else if (qdy && !qdx)
+            qpel_sad_stage(C, 2, [&](int k, int& x, int& y) VC1ME_INLINE { x = qaxis.x + (k ? 1 : -1); y = qaxis.y; },
+                           xmin, xmax, ymin, ymax, seed, cost, best);
+    } else {
+        qpel_sad_stage(C, 9, [&](int k, int& x, int& y) VC1ME_INLINE {
+            x = seed.x + (k % 3 - 1) * 2; y = seed.y + (k / 3 - 1) * 2; }, xmin, xmax, ymin, ymax, seed, cost, best);
+        const int2 half = best;
+        qpel_sad_stage(C, 9, [&](int k, int& x, int& y) VC1ME_INLINE {
+            x = half.x + (k % 3 - 1); y = half.y + (k / 3 - 1); }, xmin, xmax, ymin, ymax, seed, cost, best);
+    }
+    *best_sad = cost;
+    return best;
+}
+
 // refine_motion_block_staged, src/encoder_motion.cpp:3271-3594 (quality RATE / SATD)
 __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int sxq, int syq, const Pred pi,
                                               uint64_t* best_sad, uint64_t known_seed) {
@@ -2094,13 +2182,17 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         // zero-range mode: zero vectors with their SAD (src/encoder_motion.cpp:73-86)
         win_load(C, C.x0, C.y0, 16, 16);
         const uint64_t z = int_batch(C, 1, 1, nullptr, KEY_TOTAL, 0, [](int, int& dx, int& dy) VC1ME_INLINE { dx = 0; dy = 0; });
+        // the motion-failure check at range 0: the refined and the zero-vector
+        // SAD are both this SAD (src/encoder_motion.cpp:741-748)
+        const uint64_t zs = key_cost(z), zpx = active_pixels(C);
+        const bool zgood = J->motion_check && zpx && (zs <= zpx * 12u || (zs > 0 && zs * 10u <= zs * 9u));
         if (C.lane == 0) {
             J->screen[pos] = key_cost(z);
             J->chosen[pos] = key_cost(z);
             J->temporal[pos] = key_cost(z);
             J->mvs[pos] = make_int2(0, 0);
             for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = make_int2(0, 0);
-            J->flags[pos] = 0;
+            J->flags[pos] = zgood ? kFlagGoodMatch : 0;
             J->prop[pos] = 0;
             J->ratio[pos] = -1.0;
             J->dl_mae[pos] = -1.0;
@@ -2190,6 +2282,36 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         }
         prop_out = selected_distant ? 1 : 0;
         if (C.lane == 0) J->prop[pos] = prop_out;
+        if (J->motion_check) {
+            // the motion-failure check (src/encoder_motion.cpp:731-748): the
+            // SAD-only refinement, then the usable-match test against 12 per
+            // pixel and 90% of the zero-vector SAD (u64 arithmetic as written)
+            uint64_t refined = im.sad;
+            if (J->subpel) pw_load(C, C.x0 + im.dx - kPwMargin, C.y0 + im.dy - kPwMargin, false);
+            const int2 r = refine_qpel_sad(C, range, im.dx * 4, im.dy * 4, im.sad, &refined);
+            const uint64_t zero = lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, 0, 0, true, 0, 1, 0, 0, 0), 0);
+            const bool good = px && (refined <= px * 12u || (zero > 0 && refined * 10u <= zero * 9u));
+            C.n_sad += wave_sum32(C.ln_sad);
+            C.n_taps += wave_sum32(C.ln_taps);
+            if (C.lane == 0) {
+                J->mvs[pos] = r;
+                for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = r;
+                J->flags[pos] = good ? kFlagGoodMatch : 0;
+                J->screen[pos] = refined;
+                J->chosen[pos] = refined;
+                J->temporal[pos] = refined;
+                J->ratio[pos] = -1.0;
+                J->dl_mae[pos] = dl_mae;
+                J->dc_mae[pos] = dc_mae;
+                J->decision[pos] = static_cast<uint8_t>(decision_out);
+                uint64_t* o = J->mbops + static_cast<size_t>(pos) * 5;
+                o[0] = C.n_sad; o[1] = C.n_taps; o[2] = 0; o[3] = C.n_sig; o[4] = 0;
+            }
+            st.mv = r;
+            for (int k = 0; k < 4; ++k) st.b[k] = r;
+            st.fp = static_cast<uint32_t>(prop_out) << 8;
+            return;
+        }
         if (J->subpel) pw_load(C, C.x0 + im.dx - kPwMargin, C.y0 + im.dy - kPwMargin, mode_bilinear(one_mode));
         SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true};
         one = refine_staged(C, sp, im.dx * 4, im.dy * 4, rate_pi, &one_sad, im.sad);

@@ -18,6 +18,7 @@
 //   intensity_compensated_reference  src/encoder_motion_compensation.cpp:195-294
 //   analyze_b_picture_mode   src/encoder_motion.cpp:753-1004
 //   scene_change_score       src/encoder_motion.cpp:599-676
+//   has_good_motion_match    src/encoder_motion.cpp:678-751
 #include "encoder_internal.h"
 
 #include "../include/vc1me.h"
@@ -351,6 +352,37 @@ int ref_scene_change_score(const vc1me_params* p, const vc1me_planes* cur, const
         cfg.scene_threshold = scene_threshold;
         Vc1Encoder enc(cfg);
         *out = enc.scene_change_score(make_frame(cur), make_frame(ref));
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int ref_has_good_motion_match(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                              int fade_compensation, int* out) {
+    try {
+        EncoderConfig cfg = make_config(p, false);
+        cfg.fade_compensation = fade_compensation != 0;
+        Vc1Encoder enc(cfg);
+        *out = enc.has_good_motion_match(make_frame(cur), make_frame(ref)) ? 1 : 0;
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int ref_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref, int x0, int y0,
+                          int bw, int bh, int seed_xq, int seed_yq, int range, uint64_t known_seed_sad,
+                          int32_t* xq, int32_t* yq, uint64_t* sad) {
+    try {
+        Vc1Encoder enc(make_config(p, false));
+        const Vc1Encoder::MotionVector m = enc.refine_qpel_block(make_frame(cur), make_frame(ref), x0, y0, bw, bh,
+                                                                  Vc1Encoder::MotionVector{seed_xq, seed_yq}, range,
+                                                                  sad, known_seed_sad);
+        *xq = m.xq;
+        *yq = m.yq;
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();

@@ -2169,3 +2169,192 @@ int oracle_scene_change_score(const vc1me_params* p, const vc1me_planes* cur, co
     *out = best;
     return 0;
 }
+
+/* ------------------------------------------------------------------ motion failure */
+
+/* the (SAD, |mv|, y, x) order of refine_qpel_block */
+static bool qpel_better(uint64_t sad, mv_t mv, uint64_t best, mv_t bmv) {
+    const int mag = iabs(mv.xq) + iabs(mv.yq), bmag = iabs(bmv.xq) + iabs(bmv.yq);
+    if (sad != best) return sad < best;
+    if (mag != bmag) return mag < bmag;
+    if (mv.yq != bmv.yq) return mv.yq < bmv.yq;
+    return mv.xq < bmv.xq;
+}
+
+/* Vc1Encoder::refine_qpel_block (src/encoder_motion.cpp:3596-3685): the
+ * SAD-only quarter-pel refinement (half-pel 3x3 around the seed, then
+ * quarter-pel 3x3 around the half-pel winner; the fast mode probes
+ * cardinals, then the corners beside the winning axis, at each scale). */
+static mv_t refine_qpel_sad(const ctx_t* c, int x0, int y0, int bw, int bh, mv_t seed, int range, uint64_t* best_sad,
+                            uint64_t known_seed_sad) {
+    const vc1me_params* P = c->p;
+    const int limit_q = 4 * P->subpixel_refinement_range;
+    if (!P->subpixel_refinement || limit_q == 0 || imax(iabs(seed.xq), iabs(seed.yq)) > limit_q)
+        return refine_integer_only(c, x0, y0, bw, bh, seed, range, VC1ME_MODE_ONE_MV_QPEL, best_sad, known_seed_sad);
+    const bounds_t qb = bounds_qpel(range);
+    const int xmin = imax(qb.xmin, -limit_q), xmax = imin(qb.xmax, limit_q);
+    const int ymin = imax(qb.ymin, -limit_q), ymax = imin(qb.ymax, limit_q);
+    seed.xq = clampi(seed.xq, xmin, xmax);
+    seed.yq = clampi(seed.yq, ymin, ymax);
+    if (hidden_padding(c, x0, y0, bw, bh, seed, VC1ME_MODE_ONE_MV_QPEL, chroma_block(x0, y0, bw, bh))) {
+        seed.xq = seed.yq = 0;
+        known_seed_sad = U64MAX;
+    }
+    mv_t best = seed;
+    uint64_t cost = known_seed_sad != U64MAX ? known_seed_sad
+                                             : block_sad_motion(c, x0, y0, bw, bh, best, VC1ME_MODE_ONE_MV_QPEL);
+    mv_t seen[20];
+    int nseen = 1;
+    seen[0] = seed;
+#define EVAL(mx_, my_)                                                                         \
+    do {                                                                                       \
+        mv_t mv_ = {clampi((mx_), xmin, xmax), clampi((my_), ymin, ymax)};                     \
+        bool dup_ = false;                                                                     \
+        for (int s_ = 0; s_ < nseen; ++s_) dup_ |= same_mv(seen[s_], mv_);                     \
+        if (!dup_) {                                                                           \
+            if (nseen < 20) seen[nseen++] = mv_;                                               \
+            const uint64_t sad_ = block_sad_motion(c, x0, y0, bw, bh, mv_, VC1ME_MODE_ONE_MV_QPEL); \
+            if (qpel_better(sad_, mv_, cost, best)) { cost = sad_; best = mv_; }                \
+        }                                                                                      \
+    } while (0)
+    if (P->fast_subpixel_refinement) {
+        if (cost == 0 && best.xq == 0 && best.yq == 0) {
+            if (best_sad) *best_sad = 0;
+            return best;
+        }
+        static const int card[4][2] = {{-2, 0}, {2, 0}, {0, -2}, {0, 2}};
+        for (int k = 0; k < 4; ++k) EVAL(seed.xq + card[k][0], seed.yq + card[k][1]);
+        const mv_t axis = best;
+        const int dx = axis.xq - seed.xq, dy = axis.yq - seed.yq;
+        if (dx && !dy) { EVAL(axis.xq, axis.yq - 2); EVAL(axis.xq, axis.yq + 2); }
+        else if (dy && !dx) { EVAL(axis.xq - 2, axis.yq); EVAL(axis.xq + 2, axis.yq); }
+        const mv_t half = best;
+        static const int quar[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+        for (int k = 0; k < 4; ++k) EVAL(half.xq + quar[k][0], half.yq + quar[k][1]);
+        const mv_t qaxis = best;
+        const int qdx = qaxis.xq - half.xq, qdy = qaxis.yq - half.yq;
+        if (qdx && !qdy) { EVAL(qaxis.xq, qaxis.yq - 1); EVAL(qaxis.xq, qaxis.yq + 1); }
+        else if (qdy && !qdx) { EVAL(qaxis.xq - 1, qaxis.yq); EVAL(qaxis.xq + 1, qaxis.yq); }
+    } else {
+        for (int oy = -2; oy <= 2; oy += 2)
+            for (int ox = -2; ox <= 2; ox += 2) EVAL(seed.xq + ox, seed.yq + oy);
+        const mv_t half = best;
+        for (int oy = -1; oy <= 1; ++oy)
+            for (int ox = -1; ox <= 1; ++ox) EVAL(half.xq + ox, half.yq + oy);
+    }
+#undef EVAL
+    if (best_sad) *best_sad = cost;
+    return best;
+}
+
+/* Vc1Encoder::has_good_motion_match (src/encoder_motion.cpp:678-751): the
+ * P search's integer and long-range stages per macroblock in raster order
+ * (against the fade-compensated reference when fade compensation finds a
+ * code on the uniform global field), each refined by refine_qpel_block; true
+ * at the first macroblock whose refined SAD is at most 12 per pixel or at
+ * most 90% of its zero-vector SAD. */
+int oracle_has_good_motion_match(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                                 int fade_compensation, int* out, int32_t* field_out) {
+    const int W = p->width, H = p->height;
+    if (cur->width != W || cur->height != H || ref->width != W || ref->height != H) return -1;
+    const int mbw = (W + 15) / 16, mbh = (H + 15) / 16;
+    const size_t mbs = (size_t)mbw * mbh;
+    const int range = imax(0, p->motion_search_range);
+    const int local_range = imin(range, imax(0, p->motion_local_search_range));
+    cand_t global = global_for(p, cur, ref, local_range);
+    vc1me_planes pw = *ref;
+    uint8_t *wy = NULL, *wu = NULL, *wv = NULL;
+    if (fade_compensation) {
+        int32_t* uni = (int32_t*)malloc(mbs * 2 * sizeof(int32_t));
+        for (size_t i = 0; i < mbs; ++i) { uni[2 * i] = global.dx * 4; uni[2 * i + 1] = global.dy * 4; }
+        vc1me_intensity ic;
+        oracle_estimate_intensity_comp(cur, ref, uni, &ic);
+        free(uni);
+        if (ic.enabled) {
+            const size_t ys = (size_t)W * H, cs = (size_t)(W / 2) * (H / 2);
+            wy = (uint8_t*)malloc(ys); wu = (uint8_t*)malloc(cs); wv = (uint8_t*)malloc(cs);
+            oracle_intensity_compensate(ref, &ic, wy, wu, wv);
+            pw.y = wy; pw.u = wu; pw.v = wv;
+            global = global_for(p, cur, &pw, local_range);
+        }
+    }
+    ctx_t cx;
+    init_ctx(&cx, p, cur, &pw, NULL);
+    const ctx_t* c = &cx;
+    mv_t* field = (mv_t*)calloc(mbs, sizeof(mv_t));
+    uint8_t* prop = (uint8_t*)calloc(mbs, 1);
+    cell_t* cells = NULL;
+    int ncells = 0;
+    int good = 0;
+    /* with field_out the scan continues past the first usable match and
+     * reports every macroblock's refined vector */
+    for (int my = 0; my < mbh && (!good || field_out); ++my)
+        for (int mx = 0; mx < mbw && (!good || field_out); ++mx) {
+            const size_t pos = (size_t)my * mbw + mx;
+            const pred_t pi = predictor_info(c, field, mx, my);
+            imr_t im;
+            bool selected_distant = false;
+            if (p->long_range_search_mode == 2) {
+                bool done = false;
+                if (range > local_range) {
+                    const imr_t pr = propagated_long_range(c, mx, my, field, prop, range);
+                    if (match_good(c, pr.sad, mx, my)) { im = pr; done = true; selected_distant = true; }
+                }
+                if (!done) im = integer_local(c, mx, my, &pi, global, local_range);
+            } else {
+                im = integer_local(c, mx, my, &pi, global, local_range);
+                const bool skip = p->long_range_search_mode == 1 && match_good(c, im.sad, mx, my);
+                if (range > local_range && !skip) {
+                    const imr_t pr = propagated_long_range(c, mx, my, field, prop, range);
+                    if (better4(pr.sad, pr.dx, pr.dy, im.sad, im.dx, im.dy) &&
+                        distant_decision(c, pr.sad, im.sad, mx, my) >= 2) {
+                        im = pr;
+                        selected_distant = true;
+                    }
+                }
+            }
+            if (range > local_range && !match_good(c, im.sad, mx, my)) {
+                if (!cells) {
+                    const int nx = (W + 7) / 8, ny = (H + 7) / 8;
+                    cells = (cell_t*)malloc(sizeof(cell_t) * (size_t)nx * ny);
+                    for (int y = 0; y < H; y += 8)
+                        for (int x = 0; x < W; x += 8) {
+                            cells[ncells].x = x;
+                            cells[ncells].y = y;
+                            cells[ncells].s = signature(c->ry, c->ru, c->rv, W, H, x, y);
+                            ++ncells;
+                        }
+                }
+                uint64_t ext_sad = U64MAX;
+                int ext_dec = 0;
+                const imr_t ext = extended_content(c, mx, my, range, cells, ncells, im, &ext_sad, &ext_dec);
+                if (ext.dx != im.dx || ext.dy != im.dy) { im = ext; selected_distant = true; }
+            }
+            if (selected_distant) prop[pos] = 1;
+            uint64_t refined = im.sad;
+            const mv_t seed = {im.dx * 4, im.dy * 4};
+            field[pos] = refine_qpel_sad(c, mx * 16, my * 16, 16, 16, seed, range, &refined, im.sad);
+            const uint64_t pixels = active_mb_pixels(c, mx, my);
+            const uint64_t zero_sad = mb_sad(c, mx, my, 0, 0, 1);
+            if (pixels && (refined <= pixels * 12u || (zero_sad > 0 && refined * 10u <= zero_sad * 9u))) good = 1;
+        }
+    if (field_out)
+        for (size_t i = 0; i < mbs; ++i) { field_out[2 * i] = field[i].xq; field_out[2 * i + 1] = field[i].yq; }
+    free(field); free(prop); free(cells);
+    free(wy); free(wu); free(wv);
+    *out = good;
+    return 0;
+}
+
+/* refine_qpel_block on one block (for its tests) */
+int oracle_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref, int x0, int y0,
+                             int bw, int bh, int seed_xq, int seed_yq, int range, uint64_t known_seed_sad,
+                             int32_t* xq, int32_t* yq, uint64_t* sad) {
+    ctx_t cx;
+    init_ctx(&cx, p, cur, ref, NULL);
+    const mv_t seed = {seed_xq, seed_yq};
+    const mv_t r = refine_qpel_sad(&cx, x0, y0, bw, bh, seed, range, sad, known_seed_sad);
+    *xq = r.xq;
+    *yq = r.yq;
+    return 0;
+}

@@ -108,6 +108,17 @@ int oracle_analyze_b_frame(const vc1me_params* p, const vc1me_planes* cur, const
 int oracle_scene_change_score(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
                               int fade_compensation, double scene_threshold, double* out);
 
+/* has_good_motion_match (src/encoder_motion.cpp:678-751): *out = 1 when some
+ * macroblock has a usable motion match; field_out (or NULL): every
+ * macroblock's refined vector, the scan continuing past the first match */
+int oracle_has_good_motion_match(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref,
+                                 int fade_compensation, int* out, int32_t* field_out);
+
+/* refine_qpel_block (src/encoder_motion.cpp:3596-3685) on one block */
+int oracle_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, const vc1me_planes* ref, int x0, int y0,
+                             int bw, int bh, int seed_xq, int seed_yq, int range, uint64_t known_seed_sad,
+                             int32_t* xq, int32_t* yq, uint64_t* sad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,10 @@ def oracle() -> C.CDLL:
                                                P(abi.BParams), P(abi.BResult)]
         lib.oracle_scene_change_score.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), C.c_int, C.c_double,
                                                   P(C.c_double)]
+        lib.oracle_has_good_motion_match.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), C.c_int, P(C.c_int),
+                                                     C.c_void_p]
+        lib.oracle_refine_qpel_block.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_int] * 7 + \
+            [C.c_uint64, P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
         lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                       P(abi.PResult), P(OracleOps)]
         _oracle = lib
@@ -102,6 +106,9 @@ def reference() -> C.CDLL:
                                       P(abi.BResult)]
         lib.ref_scene_change_score.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), C.c_int, C.c_double,
                                                P(C.c_double)]
+        lib.ref_has_good_motion_match.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), C.c_int, P(C.c_int)]
+        lib.ref_refine_qpel_block.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_int] * 7 + \
+            [C.c_uint64, P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
         lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                   P(abi.PResult), C.c_int]
         _ref = lib
@@ -181,6 +188,49 @@ def oracle_scene_change_score(params: abi.Params, cur, ref, fade: bool = True, t
 def reference_scene_change_score(params: abi.Params, cur, ref, fade: bool = True, threshold: float = 28.0) -> float:
     return _scene(reference().ref_scene_change_score, params, cur, ref, fade, threshold,
                   lambda: reference().ref_last_error())
+
+
+def _good(fn, params, cur, ref, fade, *extra, err=None):
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    out = C.c_int()
+    rc = fn(C.byref(params), C.byref(c.struct), C.byref(r.struct), int(fade), C.byref(out), *extra)
+    assert rc == 0, err() if err else rc
+    return bool(out.value)
+
+
+def oracle_has_good_motion_match(params: abi.Params, cur, ref, fade: bool = True) -> bool:
+    """has_good_motion_match in the oracle under the ME settings `params`."""
+    return _good(oracle().oracle_has_good_motion_match, params, cur, ref, fade, None)
+
+
+def oracle_motion_check_field(params: abi.Params, cur, ref, fade: bool = True):
+    """(answer, every macroblock's refined vector) of the motion-failure scan
+    run to the end."""
+    mbw, mbh = abi.mb_grid(params.width, params.height)
+    f = np.zeros((mbw * mbh, 2), np.int32)
+    g = _good(oracle().oracle_has_good_motion_match, params, cur, ref, fade, f.ctypes.data)
+    return g, f
+
+
+def reference_has_good_motion_match(params: abi.Params, cur, ref, fade: bool = True) -> bool:
+    return _good(reference().ref_has_good_motion_match, params, cur, ref, fade, err=lambda: reference().ref_last_error())
+
+
+def _refine_qpel(fn, params, cur, ref, x0, y0, bw, bh, seed, rng, known):
+    c, r = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    x, y, s = C.c_int32(), C.c_int32(), C.c_uint64()
+    assert fn(C.byref(params), C.byref(c.struct), C.byref(r.struct), x0, y0, bw, bh, seed[0], seed[1], rng,
+              known, C.byref(x), C.byref(y), C.byref(s)) == 0
+    return (x.value, y.value, s.value)
+
+
+def oracle_refine_qpel_block(params, cur, ref, x0, y0, bw, bh, seed, rng, known=2**64 - 1):
+    """refine_qpel_block: (xq, yq, sad)."""
+    return _refine_qpel(oracle().oracle_refine_qpel_block, params, cur, ref, x0, y0, bw, bh, seed, rng, known)
+
+
+def reference_refine_qpel_block(params, cur, ref, x0, y0, bw, bh, seed, rng, known=2**64 - 1):
+    return _refine_qpel(reference().ref_refine_qpel_block, params, cur, ref, x0, y0, bw, bh, seed, rng, known)
 
 
 def _mv_pairs(mvs: np.ndarray) -> np.ndarray:

@@ -18,6 +18,8 @@ The fixtures are data only.
     python tests/golden/make_golden.py intensity    # intensity.json only
     python tests/golden/make_golden.py bframes      # bframes/*.npz only
     python tests/golden/make_golden.py scene        # scene.json only (scene_change_score)
+    python tests/golden/make_golden.py motion       # motion_check.json only (has_good_motion_match,
+                                                    # refine_qpel_block)
 """
 from __future__ import annotations
 
@@ -35,7 +37,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from libvc1_amd import abi, synth  # noqa: E402
 from _scene_cases import SCENE_CASES, scene_inputs  # noqa: E402
+from _motion_cases import MOTION_CASES, motion_inputs  # noqa: E402
 from _oracle import (oracle_analyze, reference_analyze, reference_analyze_b, reference_scene_change_score,  # noqa: E402
+                     reference_has_good_motion_match, reference_refine_qpel_block,
                      reference_analyze_seeded, reference_estimate_intensity, reference_intensity_compensate)
 
 
@@ -277,7 +281,40 @@ def scene_fixtures() -> list:
     return out
 
 
+def motion_fixtures() -> dict:
+    from _motion_cases import W, H
+    good = []
+    for name, fade, lr, rng in MOTION_CASES:
+        cur, ref = motion_inputs(name)
+        p = abi.default_params(W, H, pqindex=8, motion_search_range=rng, long_range_search_mode=lr)
+        v = reference_has_good_motion_match(p, cur, ref, fade)
+        good.append({"case": name, "fade": fade, "long_range_search_mode": lr, "range": rng, "good": bool(v)})
+        print(f"motion {name} fade={fade} lr={lr} range={rng}: {v}")
+    refine = []
+    cur, ref = motion_inputs("pan")
+    rs = np.random.default_rng(78)
+    for t in range(240):
+        fast, sub = t % 2, 0 if t % 11 == 0 else 1
+        bw = 8 if t % 3 == 0 else 16
+        case = {"fast": fast, "subpel": sub, "subpel_range": int(rs.choice([1024, 8, 2])),
+                "display_width": W - 6 if t % 4 == 1 else 0, "x0": int(rs.integers(0, W // bw)) * bw,
+                "y0": int(rs.integers(0, H // bw)) * bw, "bw": bw,
+                "seed": [int(rs.integers(-90, 90)), int(rs.integers(-90, 90))],
+                "range": int(rs.choice([0, 3, 32, 256])),
+                "known": 2**64 - 1 if t % 2 else int(rs.integers(0, 6000))}
+        p = abi.default_params(W, H, pqindex=8, fast_subpixel_refinement=fast, subpixel_refinement=sub,
+                               subpixel_refinement_range=case["subpel_range"], display_width=case["display_width"])
+        case["expected"] = list(reference_refine_qpel_block(p, cur, ref, case["x0"], case["y0"], bw, bw,
+                                                            tuple(case["seed"]), case["range"], case["known"]))
+        refine.append(case)
+    return {"good": good, "refine": refine}
+
+
 def main() -> int:
+    if sys.argv[1:] == ["motion"]:
+        with open(os.path.join(HERE, "motion_check.json"), "w") as f:
+            json.dump(motion_fixtures(), f)
+        return 0
     if sys.argv[1:] == ["scene"]:
         with open(os.path.join(HERE, "scene.json"), "w") as f:
             json.dump(scene_fixtures(), f, indent=1)
@@ -324,6 +361,8 @@ def main() -> int:
     b_fixtures()
     with open(os.path.join(HERE, "scene.json"), "w") as f:
         json.dump(scene_fixtures(), f, indent=1)
+    with open(os.path.join(HERE, "motion_check.json"), "w") as f:
+        json.dump(motion_fixtures(), f)
     return 0
 
 
