@@ -346,6 +346,16 @@ int vc1me_scene_change_score(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
 int vc1me_has_good_motion_match(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
                                 const vc1me_params* params, int fade_compensation, int* good, int32_t* field);
 
+/* ---- motion compensation for coding: the P prediction picture of an
+ * analysis, motion_compensate_4mv (src/encoder_motion_compensation.cpp:536-571)
+ * for macroblocks with use_4mv (four bicubic luma blocks at block_mvs,
+ * chroma at their median-4 vector) and motion_compensate_mb (:474-521) with
+ * params->mode at mvs for the others (intra macroblocks carry zero vectors,
+ * so they get the co-located reference).  A new resident frame; release it
+ * with vc1me_frame_release. */
+int vc1me_predict_p_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params,
+                          const vc1me_p_result* analysis, vc1me_frame** out);
+
 /* Copies a resident frame's planes back to the host (width x height luma,
  * two width/2 x height/2 chroma planes).  No reference counterpart. */
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);

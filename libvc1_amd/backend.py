@@ -28,7 +28,7 @@ EXPORTS = (
     "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
     "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
-    "vc1me_has_good_motion_match",
+    "vc1me_has_good_motion_match", "vc1me_predict_p_frame",
 )
 
 
@@ -80,6 +80,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                              P(C.c_double)]
     lib.vc1me_has_good_motion_match.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_int,
                                                 P(C.c_int), C.c_void_p]
+    lib.vc1me_predict_p_frame.argtypes = [C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult), P(C.c_void_p)]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
@@ -179,6 +180,13 @@ class Backend:
         self._check(self.lib.vc1me_estimate_intensity_comp(self.ctx, cur.handle, ref.handle,
                                                            m.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(out)))
         return out.as_tuple()
+
+    def predict_p_frame(self, ref: Frame, params: abi.Params, result: abi.ResultArrays) -> Frame:
+        """vc1me_predict_p_frame: the prediction picture as a new resident frame."""
+        h = C.c_void_p()
+        self._check(self.lib.vc1me_predict_p_frame(self.ctx, ref.handle, C.byref(params), C.byref(result.struct),
+                                                   C.byref(h)))
+        return Frame(self, h.value, ref.width, ref.height)
 
     def intensity_compensated(self, ref: Frame, ic: tuple) -> Frame:
         """vc1me_frame_intensity_compensated: a new resident frame."""

@@ -1365,6 +1365,54 @@ int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, co
     return compensated_frame(ctx, ref, ic, out);
 }
 
+// ---- motion compensation
+
+int vc1me_predict_p_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params,
+                          const vc1me_p_result* analysis, vc1me_frame** out) {
+    if (!ctx || !ref || !params || !analysis || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (!analysis->mvs || !analysis->block_mvs || !analysis->use_4mv)
+        return fail(VC1ME_EINVAL, "the analysis lacks mvs, block_mvs or use_4mv");
+    if (params->width != ref->w || params->height != ref->h) return fail(VC1ME_EINVAL, "raster differs from the parameters");
+    if (params->mode < VC1ME_MODE_ONE_MV_QPEL || params->mode > VC1ME_MODE_MIXED_MV) return fail(VC1ME_EINVAL, "bad mode");
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const int W = ref->w, H = ref->h;
+    const size_t mbs = static_cast<size_t>((W + 15) / 16) * ((H + 15) / 16);
+    const size_t o_mv = 0, o_bmv = (mbs * 8 + 255) & ~size_t(255), o_four = o_bmv + ((mbs * 32 + 255) & ~size_t(255));
+    const size_t need = o_four + mbs;
+    if (ctx->b_cap < need) {
+        if (ctx->d_b) (void)hipFree(ctx->d_b);
+        ctx->d_b = nullptr;
+        ctx->b_cap = 0;
+        const hipError_t e = hipMalloc(&ctx->d_b, need);
+        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("prediction buffers: ") + hipGetErrorString(e));
+        ctx->b_cap = need;
+    }
+    const size_t ys = static_cast<size_t>(W) * H, cs = static_cast<size_t>(W / 2) * (H / 2);
+    auto* f = new vc1me_frame();
+    f->w = W;
+    f->h = H;
+    uint8_t* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, ys + 2 * cs + 64);
+    if (e != hipSuccess) { delete f; return fail(VC1ME_ENOMEM, std::string("frame allocation: ") + hipGetErrorString(e)); }
+    f->y = mem;
+    f->u = mem + ys;
+    f->v = mem + ys + cs;
+    hipStream_t st = ctx->stream;
+    uint8_t* base = ctx->d_b;
+    e = hipMemcpyAsync(base + o_mv, analysis->mvs, mbs * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_bmv, analysis->block_mvs, mbs * 32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_four, analysis->use_4mv, mbs, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = vc1me::launch_predict_p(ref->y, ref->u, ref->v, W, H, params->mode, reinterpret_cast<const int2*>(base + o_mv),
+                                    reinterpret_cast<const int2*>(base + o_bmv), base + o_four, f->y, f->u, f->v, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { (void)hipFree(mem); delete f; return fail(VC1ME_EDEVICE, std::string("prediction: ") + hipGetErrorString(e)); }
+    *out = f;
+    return VC1ME_OK;
+}
+
 // ---- B pictures
 
 namespace {

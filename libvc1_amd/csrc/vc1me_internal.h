@@ -171,6 +171,11 @@ constexpr int kBMaxRows = 1024;
 hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbs, hipStream_t s);
 hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbh, hipStream_t s);
 
+// The P prediction picture of an analysis (k_predict_luma, k_predict_chroma).
+hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t* rv, int W, int H, int mode,
+                            const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du, uint8_t* dv,
+                            hipStream_t s);
+
 // Intensity compensation (vc1me_intensity.hip): the luma table at bytes
 // 0..255 and the chroma table at 256..511.
 struct IntensityLuts {

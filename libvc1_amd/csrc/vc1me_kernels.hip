@@ -3436,9 +3436,96 @@ __global__ void __launch_bounds__(1024) k_b_decide(const JobDesc* __restrict__ j
         __syncthreads();
     }
 }
+
+// ------------------------------------------------------------------ motion compensation
+// The P prediction picture of an analysis: motion_compensate_mb /
+// motion_compensate_4mv (src/encoder_motion_compensation.cpp:474-571), one
+// thread per output sample.
+
+// luma: the picture's mode for 1-MV macroblocks, bicubic for the four
+// vectors of a 4-MV macroblock (luma_mc_sample_mode / luma_mc_sample)
+__global__ void __launch_bounds__(256) k_predict_luma(const uint8_t* __restrict__ ref, int W, int H, int mode,
+                                                      const int2* __restrict__ mvs, const int2* __restrict__ bmv,
+                                                      const uint8_t* __restrict__ four, uint8_t* __restrict__ dst) {
+    const int px = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), py = static_cast<int>(blockIdx.y);
+    if (px >= W || py >= H) return;
+    const int mbw = (W + 15) / 16;
+    const int pos = (py >> 4) * mbw + (px >> 4);
+    int2 v;
+    bool bil = mode == 2;
+    if (four[pos]) {
+        v = bmv[pos * 4 + ((py >> 3) & 1) * 2 + ((px >> 3) & 1)];
+        bil = false;
+    } else {
+        v = mvs[pos];
+    }
+    const int ix = floor_div4(v.x), iy = floor_div4(v.y);
+    const int hm = v.x - ix * 4, vm = v.y - iy * 4;
+    int n[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int yy = clampi(py + iy - 1 + r, 0, H - 1);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) n[r][c] = ref[static_cast<size_t>(yy) * W + clampi(px + ix - 1 + c, 0, W - 1)];
+    }
+    dst[static_cast<size_t>(py) * W + px] = static_cast<uint8_t>(mc_sample(n, hm, vm, bil));
+}
+
+// chroma_mc_sample (src/encoder_motion_compensation.cpp:417-447), P form,
+// at the macroblock's vector or, for 4-MV, the median-4 vector (:522-535)
+__global__ void __launch_bounds__(256) k_predict_chroma(const uint8_t* __restrict__ ru, const uint8_t* __restrict__ rv,
+                                                        int W, int H, const int2* __restrict__ mvs,
+                                                        const int2* __restrict__ bmv, const uint8_t* __restrict__ four,
+                                                        uint8_t* __restrict__ du, uint8_t* __restrict__ dv) {
+    const int cw = W / 2, ch = H / 2;
+    const int px = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), py = static_cast<int>(blockIdx.y);
+    if (px >= cw || py >= ch) return;
+    const int mbw = (W + 15) / 16;
+    const int pos = (py >> 3) * mbw + (px >> 3);
+    int2 v;
+    if (four[pos]) {
+        const int2* b = bmv + pos * 4;
+        v = make_int2(median4(b[0].x, b[1].x, b[2].x, b[3].x), median4(b[0].y, b[1].y, b[2].y, b[3].y));
+    } else {
+        v = mvs[pos];
+    }
+    int uvx = arshift(v.x + ((v.x & 3) == 3), 1), uvy = arshift(v.y + ((v.y & 3) == 3), 1);
+    uvx = uvx < 0 ? uvx + (uvx & 1) : uvx - (uvx & 1);
+    uvy = uvy < 0 ? uvy + (uvy & 1) : uvy - (uvy & 1);
+    const int ix = floor_div4(uvx), iy = floor_div4(uvy);
+    const int fx = (uvx - ix * 4) * 2, fy = (uvy - iy * 4) * 2;
+    const int x0 = clampi(px + ix, 0, cw - 1), x1 = clampi(px + ix + 1, 0, cw - 1);
+    const int y0 = clampi(py + iy, 0, ch - 1), y1 = clampi(py + iy + 1, 0, ch - 1);
+    const size_t o = static_cast<size_t>(py) * cw + px;
+    const uint8_t* planes[2] = {ru, rv};
+    uint8_t* outs[2] = {du, dv};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const uint8_t* p = planes[k];
+        const int a = p[static_cast<size_t>(y0) * cw + x0];
+        int val = a;
+        if (fx || fy) {
+            const int b = p[static_cast<size_t>(y0) * cw + x1], c = p[static_cast<size_t>(y1) * cw + x0];
+            const int d = p[static_cast<size_t>(y1) * cw + x1];
+            val = (a * (8 - fx) * (8 - fy) + b * fx * (8 - fy) + c * (8 - fx) * fy + d * fx * fy + 32) >> 6;
+        }
+        outs[k][o] = static_cast<uint8_t>(val);
+    }
+}
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+
+hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t* rv, int W, int H, int mode,
+                            const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du, uint8_t* dv,
+                            hipStream_t s) {
+    if (W <= 0 || H <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_predict_luma, dim3((W + 255) / 256, H), dim3(256), 0, s, ry, W, H, mode, mvs, bmv, four, dy);
+    if (W / 2 > 0 && H / 2 > 0)
+        hipLaunchKernelGGL(k_predict_chroma, dim3((W / 2 + 255) / 256, H / 2), dim3(256), 0, s, ru, rv, W, H, mvs, bmv,
+                           four, du, dv);
+    return hipGetLastError();
+}
 
 hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbs, hipStream_t s) {
     if (max_mbs <= 0 || nb <= 0) return hipSuccess;

@@ -19,10 +19,12 @@
 //   analyze_b_picture_mode   src/encoder_motion.cpp:753-1004
 //   scene_change_score       src/encoder_motion.cpp:599-676
 //   has_good_motion_match    src/encoder_motion.cpp:678-751
+//   motion_compensate_mb / _4mv  src/encoder_motion_compensation.cpp:474-571
 #include "encoder_internal.h"
 
 #include "../include/vc1me.h"
 
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -383,6 +385,35 @@ int ref_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, const 
                                                                   sad, known_seed_sad);
         *xq = m.xq;
         *yq = m.yq;
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int ref_predict_p_frame(const vc1me_params* p, const vc1me_planes* ref, const int32_t* mvs, const int32_t* block_mvs,
+                        const uint8_t* use_4mv, uint8_t* y, uint8_t* u, uint8_t* v, int simd_auto) {
+    try {
+        Vc1Encoder enc(make_config(p, simd_auto != 0));
+        const Frame r = make_frame(ref);
+        Frame dst = r;
+        const int mbw = (p->width + 15) / 16, mbh = (p->height + 15) / 16;
+        for (int my = 0; my < mbh; ++my)
+            for (int mx = 0; mx < mbw; ++mx) {
+                const size_t pos = static_cast<size_t>(my) * mbw + mx;
+                if (use_4mv[pos]) {
+                    std::array<Vc1Encoder::MotionVector, 4> b;
+                    for (int k = 0; k < 4; ++k) b[k] = {block_mvs[8 * pos + 2 * k], block_mvs[8 * pos + 2 * k + 1]};
+                    enc.motion_compensate_4mv(dst, r, mx, my, b);
+                } else {
+                    enc.motion_compensate_mb(dst, r, mx, my, {mvs[2 * pos], mvs[2 * pos + 1]},
+                                             static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode));
+                }
+            }
+        std::memcpy(y, dst.y.data(), dst.y.size());
+        std::memcpy(u, dst.u.data(), dst.u.size());
+        std::memcpy(v, dst.v.data(), dst.v.size());
         return 0;
     } catch (const std::exception& e) {
         g_error = e.what();

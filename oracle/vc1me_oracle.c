@@ -2358,3 +2358,61 @@ int oracle_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, con
     *yq = r.yq;
     return 0;
 }
+
+/* ------------------------------------------------------------------ motion compensation */
+
+/* Vc1Encoder::chroma_mc_sample (src/encoder_motion_compensation.cpp:417-447),
+ * P form (not the B future reference), rounding control off */
+static int chroma_mc_sample(const uint8_t* p, int w, int h, int x, int y, int mvx, int mvy) {
+    int uvx = arshift(mvx + ((mvx & 3) == 3), 1), uvy = arshift(mvy + ((mvy & 3) == 3), 1);
+    uvx = uvx < 0 ? uvx + (uvx & 1) : uvx - (uvx & 1);
+    uvy = uvy < 0 ? uvy + (uvy & 1) : uvy - (uvy & 1);
+    const int ix = floor_div4(uvx), iy = floor_div4(uvy);
+    const int fx = (uvx - ix * 4) * 2, fy = (uvy - iy * 4) * 2;
+    const int sx = x + ix, sy = y + iy;
+    const int a = sample_clamped(p, w, h, sx, sy);
+    if (!fx && !fy) return a;
+    const int b = sample_clamped(p, w, h, sx + 1, sy), cc = sample_clamped(p, w, h, sx, sy + 1);
+    const int d = sample_clamped(p, w, h, sx + 1, sy + 1);
+    return (a * (8 - fx) * (8 - fy) + b * fx * (8 - fy) + cc * (8 - fx) * fy + d * fx * fy + 32) >> 6;
+}
+
+/* The P prediction picture of an analysis: motion_compensate_4mv
+ * (src/encoder_motion_compensation.cpp:536-571) for 4-MV macroblocks (four
+ * bicubic luma blocks, chroma at the median-4 vector), motion_compensate_mb
+ * (:474-521) with the picture's mode for the others. */
+void oracle_predict_p_frame(const vc1me_planes* ref, int mode, const int32_t* mvs, const int32_t* block_mvs,
+                            const uint8_t* use_4mv, uint8_t* y, uint8_t* u, uint8_t* v) {
+    const int W = ref->width, H = ref->height, cw = W / 2, ch = H / 2;
+    const int mbw = (W + 15) / 16;
+    for (int py = 0; py < H; ++py)
+        for (int px = 0; px < W; ++px) {
+            const size_t pos = (size_t)(py / 16) * mbw + px / 16;
+            int mvx, mvy, m = mode;
+            if (use_4mv[pos]) {
+                const int k = ((py / 8) & 1) * 2 + ((px / 8) & 1);
+                mvx = block_mvs[8 * pos + 2 * k];
+                mvy = block_mvs[8 * pos + 2 * k + 1];
+                m = VC1ME_MODE_ONE_MV_QPEL;
+            } else {
+                mvx = mvs[2 * pos];
+                mvy = mvs[2 * pos + 1];
+            }
+            y[(size_t)py * W + px] = (uint8_t)luma_mc_sample_mode(ref->y, W, H, px, py, mvx, mvy, m, 0);
+        }
+    for (int py = 0; py < ch; ++py)
+        for (int px = 0; px < cw; ++px) {
+            const size_t pos = (size_t)(py / 8) * mbw + px / 8;
+            int mvx, mvy;
+            if (use_4mv[pos]) {
+                const int32_t* b = block_mvs + 8 * pos;
+                mvx = median4(b[0], b[2], b[4], b[6]);
+                mvy = median4(b[1], b[3], b[5], b[7]);
+            } else {
+                mvx = mvs[2 * pos];
+                mvy = mvs[2 * pos + 1];
+            }
+            u[(size_t)py * cw + px] = (uint8_t)chroma_mc_sample(ref->u, cw, ch, px, py, mvx, mvy);
+            v[(size_t)py * cw + px] = (uint8_t)chroma_mc_sample(ref->v, cw, ch, px, py, mvx, mvy);
+        }
+}

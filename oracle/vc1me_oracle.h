@@ -119,6 +119,11 @@ int oracle_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, con
                              int bw, int bh, int seed_xq, int seed_yq, int range, uint64_t known_seed_sad,
                              int32_t* xq, int32_t* yq, uint64_t* sad);
 
+/* the P prediction picture of an analysis (motion_compensate_mb /
+ * motion_compensate_4mv, src/encoder_motion_compensation.cpp:474-571) */
+void oracle_predict_p_frame(const vc1me_planes* ref, int mode, const int32_t* mvs, const int32_t* block_mvs,
+                            const uint8_t* use_4mv, uint8_t* y, uint8_t* u, uint8_t* v);
+
 #ifdef __cplusplus
 }
 #endif

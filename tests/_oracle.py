@@ -70,6 +70,9 @@ def oracle() -> C.CDLL:
                                                      C.c_void_p]
         lib.oracle_refine_qpel_block.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_int] * 7 + \
             [C.c_uint64, P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
+        lib.oracle_predict_p_frame.argtypes = [P(abi.Planes), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]
+        lib.oracle_predict_p_frame.restype = None
         lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                       P(abi.PResult), P(OracleOps)]
         _oracle = lib
@@ -109,6 +112,7 @@ def reference() -> C.CDLL:
         lib.ref_has_good_motion_match.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), C.c_int, P(C.c_int)]
         lib.ref_refine_qpel_block.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_int] * 7 + \
             [C.c_uint64, P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
+        lib.ref_predict_p_frame.argtypes = [P(abi.Params), P(abi.Planes)] + [C.c_void_p] * 6 + [C.c_int]
         lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                   P(abi.PResult), C.c_int]
         _ref = lib
@@ -231,6 +235,31 @@ def oracle_refine_qpel_block(params, cur, ref, x0, y0, bw, bh, seed, rng, known=
 
 def reference_refine_qpel_block(params, cur, ref, x0, y0, bw, bh, seed, rng, known=2**64 - 1):
     return _refine_qpel(reference().ref_refine_qpel_block, params, cur, ref, x0, y0, bw, bh, seed, rng, known)
+
+
+def _pred_arrays(ref, result):
+    r = abi.PlaneSet.of(ref)
+    y, u, v = np.empty_like(r.y), np.empty_like(r.u), np.empty_like(r.v)
+    mvs = np.ascontiguousarray(result.mvs, np.int32)
+    bmv = np.ascontiguousarray(result.block_mvs, np.int32)
+    four = np.ascontiguousarray(result.use_4mv, np.uint8)
+    return r, y, u, v, mvs, bmv, four
+
+
+def oracle_predict_p_frame(ref, mode: int, result: abi.ResultArrays):
+    """The P prediction picture (y, u, v) of an analysis result."""
+    r, y, u, v, mvs, bmv, four = _pred_arrays(ref, result)
+    oracle().oracle_predict_p_frame(C.byref(r.struct), mode, mvs.ctypes.data, bmv.ctypes.data, four.ctypes.data,
+                                    y.ctypes.data, u.ctypes.data, v.ctypes.data)
+    return y, u, v
+
+
+def reference_predict_p_frame(params: abi.Params, ref, result: abi.ResultArrays, simd_auto: int = 0):
+    r, y, u, v, mvs, bmv, four = _pred_arrays(ref, result)
+    rc = reference().ref_predict_p_frame(C.byref(params), C.byref(r.struct), mvs.ctypes.data, bmv.ctypes.data,
+                                         four.ctypes.data, y.ctypes.data, u.ctypes.data, v.ctypes.data, simd_auto)
+    assert rc == 0, reference().ref_last_error()
+    return y, u, v
 
 
 def _mv_pairs(mvs: np.ndarray) -> np.ndarray:

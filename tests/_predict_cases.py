@@ -1,0 +1,30 @@
+"""Inputs of the P prediction-picture cases (tests and fixture writer)."""
+import numpy as np
+
+from libvc1_amd import abi, synth
+
+PREDICT_CASES = [("pan_qpel", 240, 136, abi.MODE_ONE_MV_QPEL), ("pan_hpel", 240, 136, abi.MODE_ONE_MV_HPEL),
+                 ("pan_bilinear", 240, 136, abi.MODE_ONE_MV_HPEL_BILINEAR), ("pan_mixed", 240, 136, abi.MODE_MIXED_MV),
+                 ("odd_mixed", 202, 118, abi.MODE_MIXED_MV)]
+
+
+def predict_inputs(name, w, h, mode, analyze):
+    """(ref, params, analysis result) of a case; `analyze(params, cur, ref)`
+    supplies the vector field (the reference's or the oracle's analysis)."""
+    d = synth.pan_rectangles(sum(map(ord, name)) % 1000, 4, width=w, height=h, frame_indices=[0, 3])
+    p = abi.default_params(w, h, pqindex=8, motion_search_range=256, mode=mode)
+    res = analyze(p, d[3], d[0])
+    if mode == abi.MODE_MIXED_MV:
+        rng = np.random.default_rng(len(name))
+        res.block_mvs[::3] = rng.integers(-300, 300, size=res.block_mvs[::3].shape)
+        res.use_4mv[::3] = 1
+    return d[0], p, res
+
+
+def result_from_fixture(f):
+    """An analysis result carrying a fixture's vector field."""
+    res = abi.ResultArrays(f["width"], f["height"])
+    res.mvs[...] = np.asarray(f["mvs"], np.int32).reshape(res.mvs.shape)
+    res.block_mvs[...] = np.asarray(f["block_mvs"], np.int32).reshape(res.block_mvs.shape)
+    res.use_4mv[...] = np.asarray(f["use_4mv"], np.uint8)
+    return res

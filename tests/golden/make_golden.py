@@ -20,6 +20,7 @@ The fixtures are data only.
     python tests/golden/make_golden.py scene        # scene.json only (scene_change_score)
     python tests/golden/make_golden.py motion       # motion_check.json only (has_good_motion_match,
                                                     # refine_qpel_block)
+    python tests/golden/make_golden.py predict      # predict.json only (P prediction pictures)
 """
 from __future__ import annotations
 
@@ -39,7 +40,7 @@ from libvc1_amd import abi, synth  # noqa: E402
 from _scene_cases import SCENE_CASES, scene_inputs  # noqa: E402
 from _motion_cases import MOTION_CASES, motion_inputs  # noqa: E402
 from _oracle import (oracle_analyze, reference_analyze, reference_analyze_b, reference_scene_change_score,  # noqa: E402
-                     reference_has_good_motion_match, reference_refine_qpel_block,
+                     reference_has_good_motion_match, reference_refine_qpel_block, reference_predict_p_frame,
                      reference_analyze_seeded, reference_estimate_intensity, reference_intensity_compensate)
 
 
@@ -310,7 +311,27 @@ def motion_fixtures() -> dict:
     return {"good": good, "refine": refine}
 
 
+def predict_fixtures() -> list:
+    """Vector fields from the reference's analysis of generated pairs (some
+    4-MV vectors replaced by wild ones) and SHA-256 digests of the
+    reference's prediction planes."""
+    from _predict_cases import PREDICT_CASES, predict_inputs
+    out = []
+    for name, w, h, mode in PREDICT_CASES:
+        ref, p, res = predict_inputs(name, w, h, mode, reference_analyze)
+        y, u, v = reference_predict_p_frame(p, ref, res, 1)
+        out.append({"case": name, "width": w, "height": h, "mode": mode, "mvs": res.mvs.reshape(-1).tolist(),
+                    "block_mvs": res.block_mvs.reshape(-1).tolist(), "use_4mv": res.use_4mv.tolist(),
+                    "sha256": hashlib.sha256(y.tobytes() + u.tobytes() + v.tobytes()).hexdigest()})
+        print(f"predict {name} {w}x{h} mode {mode}: 4-MV {int(res.use_4mv.sum())}")
+    return out
+
+
 def main() -> int:
+    if sys.argv[1:] == ["predict"]:
+        with open(os.path.join(HERE, "predict.json"), "w") as f:
+            json.dump(predict_fixtures(), f)
+        return 0
     if sys.argv[1:] == ["motion"]:
         with open(os.path.join(HERE, "motion_check.json"), "w") as f:
             json.dump(motion_fixtures(), f)
@@ -363,6 +384,8 @@ def main() -> int:
         json.dump(scene_fixtures(), f, indent=1)
     with open(os.path.join(HERE, "motion_check.json"), "w") as f:
         json.dump(motion_fixtures(), f)
+    with open(os.path.join(HERE, "predict.json"), "w") as f:
+        json.dump(predict_fixtures(), f)
     return 0
 
 
