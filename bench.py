@@ -335,6 +335,13 @@ def seeded_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, check_
         parity = "bit-exact" if not bad else f"MISMATCH in {bad}"
         log(f"seeded passes of pair 0 vs oracle (Mixed-MV candidate, final): {parity} ({time.time() - t1:.1f}s)")
     intensity = intensity_part(be, jobs, res, check_pair)
+    # the prediction pictures of the final pass (vc1me_predict_p_frame)
+    be.predict_p_frame(jobs[0][1], final, res[0]).release()
+    t0 = time.perf_counter()
+    for (c, r), rr in zip(jobs, res):
+        be.predict_p_frame(r, final, rr).release()
+    pred_ms = (time.perf_counter() - t0) * 1e3 / n
+    log(f"prediction pictures: {pred_ms:.3f} ms per frame (host-timed, allocation included)")
     return {"value": round(cand_value, 1), "unit": "MB/s",
             "value_final_pass": round(final_value, 1), "speedup_vs_plain": round(cand_value / plain_value, 4),
             "ms_seed_pass": round(t_seed * 1e3 / steps, 3), "ms_candidate_step": round(t_cand * 1e3 / steps, 3),
@@ -342,7 +349,7 @@ def seeded_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, check_
             "final_cost_vs_plain": round(float(cs.sum() / cp.sum() - 1), 6),
             "passes": "1-MV qpel seed (unseeded, no intra); 48-job step of seeded Hpel / HpelBilinear / Mixed-MV "
                       "candidates (no intra); final seeded Mixed-MV with intra",
-            "parity": parity, "intensity": intensity}
+            "parity": parity, "intensity": intensity, "prediction_ms_per_frame": round(pred_ms, 4)}
 
 
 def b_leg(be, pics, frames, steps, check):

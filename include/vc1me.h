@@ -356,6 +356,14 @@ int vc1me_has_good_motion_match(vc1me_ctx* ctx, const vc1me_frame* cur, const vc
 int vc1me_predict_p_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params,
                           const vc1me_p_result* analysis, vc1me_frame** out);
 
+/* The bi-directional prediction picture: motion_compensate_bi_mb
+ * (src/encoder_motion_compensation.cpp:659-711) for every macroblock, with
+ * fmvs against past and bmvs against future (mbw * mbh quarter-pel (x, y)
+ * each; params->mode a 1-MV mode).  The interpolated and direct B
+ * macroblocks' prediction; a new resident frame. */
+int vc1me_predict_bi_frame(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future,
+                           const vc1me_params* params, const int32_t* fmvs, const int32_t* bmvs, vc1me_frame** out);
+
 /* Copies a resident frame's planes back to the host (width x height luma,
  * two width/2 x height/2 chroma planes).  No reference counterpart. */
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);

@@ -28,7 +28,7 @@ EXPORTS = (
     "vc1me_set_graphs", "vc1me_set_schedule", "vc1me_last_relax_passes", "vc1me_analyze_p_frame_seeded",
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
     "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
-    "vc1me_has_good_motion_match", "vc1me_predict_p_frame",
+    "vc1me_has_good_motion_match", "vc1me_predict_p_frame", "vc1me_predict_bi_frame",
 )
 
 
@@ -81,6 +81,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_has_good_motion_match.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_int,
                                                 P(C.c_int), C.c_void_p]
     lib.vc1me_predict_p_frame.argtypes = [C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult), P(C.c_void_p)]
+    lib.vc1me_predict_bi_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_void_p, C.c_void_p,
+                                           P(C.c_void_p)]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
@@ -187,6 +189,15 @@ class Backend:
         self._check(self.lib.vc1me_predict_p_frame(self.ctx, ref.handle, C.byref(params), C.byref(result.struct),
                                                    C.byref(h)))
         return Frame(self, h.value, ref.width, ref.height)
+
+    def predict_bi_frame(self, past: Frame, future: Frame, params: abi.Params, fmvs, bmvs) -> Frame:
+        """vc1me_predict_bi_frame: a new resident frame."""
+        f = np.ascontiguousarray(np.asarray(fmvs, np.int32))
+        b = np.ascontiguousarray(np.asarray(bmvs, np.int32))
+        h = C.c_void_p()
+        self._check(self.lib.vc1me_predict_bi_frame(self.ctx, past.handle, future.handle, C.byref(params),
+                                                    f.ctypes.data, b.ctypes.data, C.byref(h)))
+        return Frame(self, h.value, past.width, past.height)
 
     def intensity_compensated(self, ref: Frame, ic: tuple) -> Frame:
         """vc1me_frame_intensity_compensated: a new resident frame."""

@@ -1413,6 +1413,50 @@ int vc1me_predict_p_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_pa
     return VC1ME_OK;
 }
 
+int vc1me_predict_bi_frame(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future,
+                           const vc1me_params* params, const int32_t* fmvs, const int32_t* bmvs, vc1me_frame** out) {
+    if (!ctx || !past || !future || !params || !fmvs || !bmvs || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (params->width != past->w || params->height != past->h || future->w != past->w || future->h != past->h)
+        return fail(VC1ME_EINVAL, "raster differs from the parameters");
+    if (params->mode < VC1ME_MODE_ONE_MV_QPEL || params->mode > VC1ME_MODE_ONE_MV_HPEL_BILINEAR)
+        return fail(VC1ME_EINVAL, "bi-directional prediction takes a 1-MV mode");
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    const int W = past->w, H = past->h;
+    const size_t mbs = static_cast<size_t>((W + 15) / 16) * ((H + 15) / 16);
+    const size_t o_f = 0, o_b = (mbs * 8 + 255) & ~size_t(255), need = o_b + mbs * 8;
+    if (ctx->b_cap < need) {
+        if (ctx->d_b) (void)hipFree(ctx->d_b);
+        ctx->d_b = nullptr;
+        ctx->b_cap = 0;
+        const hipError_t e = hipMalloc(&ctx->d_b, need);
+        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("prediction buffers: ") + hipGetErrorString(e));
+        ctx->b_cap = need;
+    }
+    const size_t ys = static_cast<size_t>(W) * H, cs = static_cast<size_t>(W / 2) * (H / 2);
+    auto* f = new vc1me_frame();
+    f->w = W;
+    f->h = H;
+    uint8_t* mem = nullptr;
+    hipError_t e = hipMalloc(&mem, ys + 2 * cs + 64);
+    if (e != hipSuccess) { delete f; return fail(VC1ME_ENOMEM, std::string("frame allocation: ") + hipGetErrorString(e)); }
+    f->y = mem;
+    f->u = mem + ys;
+    f->v = mem + ys + cs;
+    hipStream_t st = ctx->stream;
+    uint8_t* base = ctx->d_b;
+    e = hipMemcpyAsync(base + o_f, fmvs, mbs * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_b, bmvs, mbs * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = vc1me::launch_predict_bi(past->y, future->y, W, H, params->mode, reinterpret_cast<const int2*>(base + o_f),
+                                     reinterpret_cast<const int2*>(base + o_b), f->y, f->u, f->v, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { (void)hipFree(mem); delete f; return fail(VC1ME_EDEVICE, std::string("prediction: ") + hipGetErrorString(e)); }
+    *out = f;
+    return VC1ME_OK;
+}
+
 // ---- B pictures
 
 namespace {

@@ -176,6 +176,11 @@ hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t*
                             const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du, uint8_t* dv,
                             hipStream_t s);
 
+// The bi-directional prediction picture (k_predict_bi); past and fut are
+// frame buffers with the two chroma planes after luma.
+hipError_t launch_predict_bi(const uint8_t* past, const uint8_t* fut, int W, int H, int mode, const int2* fmv,
+                             const int2* bmv, uint8_t* dy, uint8_t* du, uint8_t* dv, hipStream_t s);
+
 // Intensity compensation (vc1me_intensity.hip): the luma table at bytes
 // 0..255 and the chroma table at 256..511.
 struct IntensityLuts {

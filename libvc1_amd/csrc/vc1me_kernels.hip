@@ -3512,9 +3512,77 @@ __global__ void __launch_bounds__(256) k_predict_chroma(const uint8_t* __restric
         outs[k][o] = static_cast<uint8_t>(val);
     }
 }
+
+// The bi-directional prediction picture: motion_compensate_bi_mb
+// (src/encoder_motion_compensation.cpp:659-711) for every macroblock, the
+// rounding average of the past prediction at fmvs and the future prediction
+// at bmvs; the future chroma vector rounds odd components away from the even
+// value (:420-435).
+__device__ __forceinline__ int chroma_sample(const uint8_t* p, int cw, int ch, int px, int py, int2 v, bool future) {
+    int uvx = arshift(v.x + ((v.x & 3) == 3), 1), uvy = arshift(v.y + ((v.y & 3) == 3), 1);
+    if (future) {
+        uvx = uvx < 0 ? uvx - (uvx & 1) : uvx + (uvx & 1);
+        uvy = uvy < 0 ? uvy - (uvy & 1) : uvy + (uvy & 1);
+    } else {
+        uvx = uvx < 0 ? uvx + (uvx & 1) : uvx - (uvx & 1);
+        uvy = uvy < 0 ? uvy + (uvy & 1) : uvy - (uvy & 1);
+    }
+    const int ix = floor_div4(uvx), iy = floor_div4(uvy);
+    const int fx = (uvx - ix * 4) * 2, fy = (uvy - iy * 4) * 2;
+    const int x0 = clampi(px + ix, 0, cw - 1), x1 = clampi(px + ix + 1, 0, cw - 1);
+    const int y0 = clampi(py + iy, 0, ch - 1), y1 = clampi(py + iy + 1, 0, ch - 1);
+    const int a = p[static_cast<size_t>(y0) * cw + x0];
+    if (!fx && !fy) return a;
+    const int b = p[static_cast<size_t>(y0) * cw + x1], c = p[static_cast<size_t>(y1) * cw + x0];
+    const int d = p[static_cast<size_t>(y1) * cw + x1];
+    return (a * (8 - fx) * (8 - fy) + b * fx * (8 - fy) + c * (8 - fx) * fy + d * fx * fy + 32) >> 6;
+}
+__device__ __forceinline__ int luma_sample(const uint8_t* ref, int W, int H, int px, int py, int2 v, bool bil) {
+    const int ix = floor_div4(v.x), iy = floor_div4(v.y);
+    int n[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int yy = clampi(py + iy - 1 + r, 0, H - 1);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) n[r][c] = ref[static_cast<size_t>(yy) * W + clampi(px + ix - 1 + c, 0, W - 1)];
+    }
+    return mc_sample(n, v.x - ix * 4, v.y - iy * 4, bil);
+}
+__global__ void __launch_bounds__(256) k_predict_bi(const uint8_t* __restrict__ past, const uint8_t* __restrict__ fut,
+                                                    int W, int H, int mode, const int2* __restrict__ fmv,
+                                                    const int2* __restrict__ bmv, uint8_t* __restrict__ dy,
+                                                    uint8_t* __restrict__ du, uint8_t* __restrict__ dv) {
+    const int px = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), py = static_cast<int>(blockIdx.y);
+    const int mbw = (W + 15) / 16, cw = W / 2, ch = H / 2;
+    if (blockIdx.z == 0) {
+        if (px >= W || py >= H) return;
+        const int pos = (py >> 4) * mbw + (px >> 4);
+        const int a = luma_sample(past, W, H, px, py, fmv[pos], mode == 2);
+        const int z = luma_sample(fut, W, H, px, py, bmv[pos], mode == 2);
+        dy[static_cast<size_t>(py) * W + px] = static_cast<uint8_t>((a + z + 1) >> 1);
+        return;
+    }
+    if (px >= cw || py >= ch) return;
+    const int pos = (py >> 3) * mbw + (px >> 3);
+    const size_t ys = static_cast<size_t>(W) * H, cs = static_cast<size_t>(cw) * ch;
+    const uint8_t* pp = past + ys + (blockIdx.z == 2 ? cs : 0);
+    const uint8_t* fp = fut + ys + (blockIdx.z == 2 ? cs : 0);
+    uint8_t* out = blockIdx.z == 2 ? dv : du;
+    const int a = chroma_sample(pp, cw, ch, px, py, fmv[pos], false);
+    const int b = chroma_sample(fp, cw, ch, px, py, bmv[pos], true);
+    out[static_cast<size_t>(py) * cw + px] = static_cast<uint8_t>((a + b + 1) >> 1);
+}
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+
+hipError_t launch_predict_bi(const uint8_t* past, const uint8_t* fut, int W, int H, int mode, const int2* fmv,
+                             const int2* bmv, uint8_t* dy, uint8_t* du, uint8_t* dv, hipStream_t s) {
+    if (W <= 0 || H <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_predict_bi, dim3((W + 255) / 256, H, 3), dim3(256), 0, s, past, fut, W, H, mode, fmv, bmv, dy,
+                       du, dv);
+    return hipGetLastError();
+}
 
 hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t* rv, int W, int H, int mode,
                             const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du, uint8_t* dv,

@@ -20,6 +20,7 @@
 //   scene_change_score       src/encoder_motion.cpp:599-676
 //   has_good_motion_match    src/encoder_motion.cpp:678-751
 //   motion_compensate_mb / _4mv  src/encoder_motion_compensation.cpp:474-571
+//   motion_compensate_bi_mb  src/encoder_motion_compensation.cpp:659-711
 #include "encoder_internal.h"
 
 #include "../include/vc1me.h"
@@ -410,6 +411,30 @@ int ref_predict_p_frame(const vc1me_params* p, const vc1me_planes* ref, const in
                     enc.motion_compensate_mb(dst, r, mx, my, {mvs[2 * pos], mvs[2 * pos + 1]},
                                              static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode));
                 }
+            }
+        std::memcpy(y, dst.y.data(), dst.y.size());
+        std::memcpy(u, dst.u.data(), dst.u.size());
+        std::memcpy(v, dst.v.data(), dst.v.size());
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+int ref_predict_bi_frame(const vc1me_params* p, const vc1me_planes* past, const vc1me_planes* future,
+                         const int32_t* fmvs, const int32_t* bmvs, uint8_t* y, uint8_t* u, uint8_t* v, int simd_auto) {
+    try {
+        Vc1Encoder enc(make_config(p, simd_auto != 0));
+        const Frame pa = make_frame(past), fu = make_frame(future);
+        Frame dst = pa;
+        const int mbw = (p->width + 15) / 16, mbh = (p->height + 15) / 16;
+        for (int my = 0; my < mbh; ++my)
+            for (int mx = 0; mx < mbw; ++mx) {
+                const size_t pos = static_cast<size_t>(my) * mbw + mx;
+                enc.motion_compensate_bi_mb(dst, pa, fu, mx, my, {fmvs[2 * pos], fmvs[2 * pos + 1]},
+                                            {bmvs[2 * pos], bmvs[2 * pos + 1]},
+                                            static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode));
             }
         std::memcpy(y, dst.y.data(), dst.y.size());
         std::memcpy(u, dst.u.data(), dst.u.size());

@@ -2362,11 +2362,17 @@ int oracle_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, con
 /* ------------------------------------------------------------------ motion compensation */
 
 /* Vc1Encoder::chroma_mc_sample (src/encoder_motion_compensation.cpp:417-447),
- * P form (not the B future reference), rounding control off */
-static int chroma_mc_sample(const uint8_t* p, int w, int h, int x, int y, int mvx, int mvy) {
+ * rounding control off; future: the B future-reference rounding of odd
+ * chroma components (away from the even value) */
+static int chroma_mc_sample_dir(const uint8_t* p, int w, int h, int x, int y, int mvx, int mvy, bool future) {
     int uvx = arshift(mvx + ((mvx & 3) == 3), 1), uvy = arshift(mvy + ((mvy & 3) == 3), 1);
-    uvx = uvx < 0 ? uvx + (uvx & 1) : uvx - (uvx & 1);
-    uvy = uvy < 0 ? uvy + (uvy & 1) : uvy - (uvy & 1);
+    if (future) {
+        uvx = uvx < 0 ? uvx - (uvx & 1) : uvx + (uvx & 1);
+        uvy = uvy < 0 ? uvy - (uvy & 1) : uvy + (uvy & 1);
+    } else {
+        uvx = uvx < 0 ? uvx + (uvx & 1) : uvx - (uvx & 1);
+        uvy = uvy < 0 ? uvy + (uvy & 1) : uvy - (uvy & 1);
+    }
     const int ix = floor_div4(uvx), iy = floor_div4(uvy);
     const int fx = (uvx - ix * 4) * 2, fy = (uvy - iy * 4) * 2;
     const int sx = x + ix, sy = y + iy;
@@ -2375,6 +2381,9 @@ static int chroma_mc_sample(const uint8_t* p, int w, int h, int x, int y, int mv
     const int b = sample_clamped(p, w, h, sx + 1, sy), cc = sample_clamped(p, w, h, sx, sy + 1);
     const int d = sample_clamped(p, w, h, sx + 1, sy + 1);
     return (a * (8 - fx) * (8 - fy) + b * fx * (8 - fy) + cc * (8 - fx) * fy + d * fx * fy + 32) >> 6;
+}
+static int chroma_mc_sample(const uint8_t* p, int w, int h, int x, int y, int mvx, int mvy) {
+    return chroma_mc_sample_dir(p, w, h, x, y, mvx, mvy, false);
 }
 
 /* The P prediction picture of an analysis: motion_compensate_4mv
@@ -2414,5 +2423,32 @@ void oracle_predict_p_frame(const vc1me_planes* ref, int mode, const int32_t* mv
             }
             u[(size_t)py * cw + px] = (uint8_t)chroma_mc_sample(ref->u, cw, ch, px, py, mvx, mvy);
             v[(size_t)py * cw + px] = (uint8_t)chroma_mc_sample(ref->v, cw, ch, px, py, mvx, mvy);
+        }
+}
+
+/* The bi-directional prediction picture: motion_compensate_bi_mb
+ * (src/encoder_motion_compensation.cpp:659-711) for every macroblock, the
+ * rounding average of the past prediction at fmvs and the future prediction
+ * at bmvs (chroma with the future-reference rounding). */
+void oracle_predict_bi_frame(const vc1me_planes* past, const vc1me_planes* future, int mode, const int32_t* fmvs,
+                             const int32_t* bmvs, uint8_t* y, uint8_t* u, uint8_t* v) {
+    const int W = past->width, H = past->height, cw = W / 2, ch = H / 2;
+    const int mbw = (W + 15) / 16;
+    for (int py = 0; py < H; ++py)
+        for (int px = 0; px < W; ++px) {
+            const size_t pos = (size_t)(py / 16) * mbw + px / 16;
+            const int a = luma_mc_sample_mode(past->y, W, H, px, py, fmvs[2 * pos], fmvs[2 * pos + 1], mode, 0);
+            const int z = luma_mc_sample_mode(future->y, W, H, px, py, bmvs[2 * pos], bmvs[2 * pos + 1], mode, 0);
+            y[(size_t)py * W + px] = (uint8_t)((a + z + 1) >> 1);
+        }
+    for (int py = 0; py < ch; ++py)
+        for (int px = 0; px < cw; ++px) {
+            const size_t pos = (size_t)(py / 8) * mbw + px / 8;
+            const int fx = fmvs[2 * pos], fy = fmvs[2 * pos + 1], bx = bmvs[2 * pos], by = bmvs[2 * pos + 1];
+            const size_t o = (size_t)py * cw + px;
+            u[o] = (uint8_t)((chroma_mc_sample_dir(past->u, cw, ch, px, py, fx, fy, false) +
+                              chroma_mc_sample_dir(future->u, cw, ch, px, py, bx, by, true) + 1) >> 1);
+            v[o] = (uint8_t)((chroma_mc_sample_dir(past->v, cw, ch, px, py, fx, fy, false) +
+                              chroma_mc_sample_dir(future->v, cw, ch, px, py, bx, by, true) + 1) >> 1);
         }
 }

@@ -124,6 +124,11 @@ int oracle_refine_qpel_block(const vc1me_params* p, const vc1me_planes* cur, con
 void oracle_predict_p_frame(const vc1me_planes* ref, int mode, const int32_t* mvs, const int32_t* block_mvs,
                             const uint8_t* use_4mv, uint8_t* y, uint8_t* u, uint8_t* v);
 
+/* the bi-directional prediction picture (motion_compensate_bi_mb,
+ * src/encoder_motion_compensation.cpp:659-711, every macroblock) */
+void oracle_predict_bi_frame(const vc1me_planes* past, const vc1me_planes* future, int mode, const int32_t* fmvs,
+                             const int32_t* bmvs, uint8_t* y, uint8_t* u, uint8_t* v);
+
 #ifdef __cplusplus
 }
 #endif

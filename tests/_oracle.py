@@ -73,6 +73,8 @@ def oracle() -> C.CDLL:
         lib.oracle_predict_p_frame.argtypes = [P(abi.Planes), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p]
         lib.oracle_predict_p_frame.restype = None
+        lib.oracle_predict_bi_frame.argtypes = [P(abi.Planes), P(abi.Planes), C.c_int] + [C.c_void_p] * 5
+        lib.oracle_predict_bi_frame.restype = None
         lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                       P(abi.PResult), P(OracleOps)]
         _oracle = lib
@@ -113,6 +115,7 @@ def reference() -> C.CDLL:
         lib.ref_refine_qpel_block.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_int] * 7 + \
             [C.c_uint64, P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
         lib.ref_predict_p_frame.argtypes = [P(abi.Params), P(abi.Planes)] + [C.c_void_p] * 6 + [C.c_int]
+        lib.ref_predict_bi_frame.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_void_p] * 5 + [C.c_int]
         lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                   P(abi.PResult), C.c_int]
         _ref = lib
@@ -258,6 +261,33 @@ def reference_predict_p_frame(params: abi.Params, ref, result: abi.ResultArrays,
     r, y, u, v, mvs, bmv, four = _pred_arrays(ref, result)
     rc = reference().ref_predict_p_frame(C.byref(params), C.byref(r.struct), mvs.ctypes.data, bmv.ctypes.data,
                                          four.ctypes.data, y.ctypes.data, u.ctypes.data, v.ctypes.data, simd_auto)
+    assert rc == 0, reference().ref_last_error()
+    return y, u, v
+
+
+def _bi(fn, head, past, future, fmvs, bmvs, *tail):
+    pa, fu = abi.PlaneSet.of(past), abi.PlaneSet.of(future)
+    y, u, v = np.empty_like(pa.y), np.empty_like(pa.u), np.empty_like(pa.v)
+    f = np.ascontiguousarray(fmvs, np.int32)
+    b = np.ascontiguousarray(bmvs, np.int32)
+    rc = fn(*head, C.byref(pa.struct), C.byref(fu.struct), *tail[:1], f.ctypes.data, b.ctypes.data, y.ctypes.data,
+            u.ctypes.data, v.ctypes.data, *tail[1:])
+    assert rc in (None, 0)
+    return y, u, v
+
+
+def oracle_predict_bi_frame(past, future, mode: int, fmvs, bmvs):
+    """The bi-directional prediction picture (y, u, v)."""
+    return _bi(oracle().oracle_predict_bi_frame, (), past, future, fmvs, bmvs, mode)
+
+
+def reference_predict_bi_frame(params: abi.Params, past, future, fmvs, bmvs, simd_auto: int = 0):
+    pa, fu = abi.PlaneSet.of(past), abi.PlaneSet.of(future)
+    y, u, v = np.empty_like(pa.y), np.empty_like(pa.u), np.empty_like(pa.v)
+    f = np.ascontiguousarray(fmvs, np.int32)
+    b = np.ascontiguousarray(bmvs, np.int32)
+    rc = reference().ref_predict_bi_frame(C.byref(params), C.byref(pa.struct), C.byref(fu.struct), f.ctypes.data,
+                                          b.ctypes.data, y.ctypes.data, u.ctypes.data, v.ctypes.data, simd_auto)
     assert rc == 0, reference().ref_last_error()
     return y, u, v
 

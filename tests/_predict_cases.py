@@ -28,3 +28,21 @@ def result_from_fixture(f):
     res.block_mvs[...] = np.asarray(f["block_mvs"], np.int32).reshape(res.block_mvs.shape)
     res.use_4mv[...] = np.asarray(f["use_4mv"], np.uint8)
     return res
+
+
+BI_CASES = [("bi_qpel", 240, 136, abi.MODE_ONE_MV_QPEL), ("bi_hpel", 240, 136, abi.MODE_ONE_MV_HPEL),
+            ("bi_bilinear_odd", 202, 118, abi.MODE_ONE_MV_HPEL_BILINEAR)]
+
+
+def bi_inputs(name, w, h, mode):
+    """(past, future, forward field, backward field) of a bi-directional case:
+    odd quarter-pel components included, so both chroma roundings occur."""
+    d = synth.pan_rectangles(sum(map(ord, name)) % 1000, 4, width=w, height=h, frame_indices=[0, 3])
+    mbw, mbh = abi.mb_grid(w, h)
+    rng = np.random.default_rng(len(name) + mode)
+    f = rng.integers(-200, 200, size=(mbw * mbh, 2)).astype(np.int32)
+    b = rng.integers(-200, 200, size=(mbw * mbh, 2)).astype(np.int32)
+    if mode != abi.MODE_ONE_MV_QPEL:
+        f &= ~1
+        b &= ~1
+    return d[0], d[3], f, b

@@ -41,6 +41,7 @@ from _scene_cases import SCENE_CASES, scene_inputs  # noqa: E402
 from _motion_cases import MOTION_CASES, motion_inputs  # noqa: E402
 from _oracle import (oracle_analyze, reference_analyze, reference_analyze_b, reference_scene_change_score,  # noqa: E402
                      reference_has_good_motion_match, reference_refine_qpel_block, reference_predict_p_frame,
+                     reference_predict_bi_frame,
                      reference_analyze_seeded, reference_estimate_intensity, reference_intensity_compensate)
 
 
@@ -327,10 +328,26 @@ def predict_fixtures() -> list:
     return out
 
 
+def predict_bi_fixtures() -> list:
+    """Random forward/backward fields and digests of the reference's
+    bi-directional prediction pictures."""
+    from _predict_cases import BI_CASES, bi_inputs
+    out = []
+    for name, w, h, mode in BI_CASES:
+        past, fut, fmv, bmv = bi_inputs(name, w, h, mode)
+        y, u, v = reference_predict_bi_frame(abi.default_params(w, h, mode=mode), past, fut, fmv, bmv, 1)
+        out.append({"case": name, "width": w, "height": h, "mode": mode,
+                    "sha256": hashlib.sha256(y.tobytes() + u.tobytes() + v.tobytes()).hexdigest()})
+        print(f"predict bi {name} {w}x{h} mode {mode}")
+    return out
+
+
 def main() -> int:
     if sys.argv[1:] == ["predict"]:
         with open(os.path.join(HERE, "predict.json"), "w") as f:
             json.dump(predict_fixtures(), f)
+        with open(os.path.join(HERE, "predict_bi.json"), "w") as f:
+            json.dump(predict_bi_fixtures(), f, indent=1)
         return 0
     if sys.argv[1:] == ["motion"]:
         with open(os.path.join(HERE, "motion_check.json"), "w") as f:
@@ -386,6 +403,8 @@ def main() -> int:
         json.dump(motion_fixtures(), f)
     with open(os.path.join(HERE, "predict.json"), "w") as f:
         json.dump(predict_fixtures(), f)
+    with open(os.path.join(HERE, "predict_bi.json"), "w") as f:
+        json.dump(predict_bi_fixtures(), f, indent=1)
     return 0
 
 

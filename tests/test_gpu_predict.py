@@ -6,8 +6,8 @@ import pytest
 
 import _oracle as O
 from libvc1_amd import abi, synth
-from _predict_cases import result_from_fixture
-from test_oracle_predict import FIX, digest, fixture_ref
+from _predict_cases import bi_inputs, result_from_fixture
+from test_oracle_predict import BI_FIX, FIX, digest, fixture_ref
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +51,22 @@ def test_prediction_of_gpu_analysis_matches_oracle(backend):
     res.block_mvs[:] = rng.integers(-9000, 9000, size=res.block_mvs.shape)
     print("[case] prediction 1080p, wild vectors", flush=True)
     assert digest(_predict(backend, d[0], p, res)) == digest(O.oracle_predict_p_frame(d[0], p.mode, res))
+
+
+@pytest.mark.parametrize("k", range(len(BI_FIX)), ids=lambda k: BI_FIX[k]["case"])
+def test_bi_prediction_matches_fixture_and_oracle(backend, k):
+    f = BI_FIX[k]
+    print(f"[case] bi prediction {f['case']}", flush=True)
+    past, fut, fmv, bmv = bi_inputs(f["case"], f["width"], f["height"], f["mode"])
+    fp, ff = backend.upload(past), backend.upload(fut)
+    try:
+        g = backend.predict_bi_frame(fp, ff, abi.default_params(f["width"], f["height"], mode=f["mode"]), fmv, bmv)
+        try:
+            got = backend.download(g)
+        finally:
+            g.release()
+    finally:
+        fp.release()
+        ff.release()
+    assert digest(got) == f["sha256"]
+    assert digest(got) == digest(O.oracle_predict_bi_frame(past, fut, f["mode"], fmv, bmv))
