@@ -248,6 +248,21 @@ def learned_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, thres
                      "cost_rise": round(float(ca.sum() / cp.sum() - 1), 6),
                      "frac_worse": round(float(np.mean(ca > cp)), 6)}
         log(f"ablation {name}: {abl[name]}")
+    # the network as a confidence gate only (vc1me_set_learned_candidates(0))
+    be.set_learned_candidates(False)
+    try:
+        be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
+        gv = mbs_per_step * steps / (time.perf_counter() - t0)
+        gj = learned.cost_compare(be, net, jobs, plist, threshold)
+    finally:
+        be.set_learned_candidates(True)
+    gate_only = {"value": round(gv, 1), "speedup_vs_plain": round(gv / plain_value, 4),
+                 "cost_rise": round(gj["cost_rise"], 6), "frac_worse": round(gj["frac_worse"], 6),
+                 "seeded_fraction": round(gj["seeded_fraction"], 4)}
+    log(f"learned, gate only: {gate_only}")
     net.close()
     meta_path = learned.WEIGHTS + ".json"
     sha = json.load(open(meta_path)).get("sha256") if os.path.exists(meta_path) else None
@@ -257,7 +272,7 @@ def learned_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, thres
         "cost_rise": round(judged["cost_rise"], 6), "frac_worse": round(judged["frac_worse"], 6),
         "mean_excess_per_mb": round(judged["mean_excess"], 3), "seeded_fraction": round(judged["seeded_fraction"], 4),
         "cost": "sum over MBs of SAD + lambda*R of the final 1-MV vector (exact integers 20*C), learned vs plain",
-        "weights_sha256": sha, "ablation": abl,
+        "weights_sha256": sha, "ablation": abl, "gate_only": gate_only,
     }
 
 

@@ -503,6 +503,11 @@ int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jo
  * confidence logit). */
 int vc1me_analyze_step_proposals(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* proposals,
                                  float threshold);
+/* Learned seeding without the proposals' vectors (on = 0): confident
+ * macroblocks run the seeded search from the predictor, zero and the global
+ * vector only, the network acting as a confidence gate; on = 1 (the
+ * default) uses its two candidates too.  No reference counterpart. */
+int vc1me_set_learned_candidates(vc1me_ctx* ctx, int on);
 
 /* ---- cost report (no reference counterpart; judges the learned form).
  * While on, every analysis records per macroblock C20 = 20 * (SAD + lambda *

@@ -29,6 +29,7 @@ EXPORTS = (
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
     "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
     "vc1me_has_good_motion_match", "vc1me_predict_p_frame", "vc1me_predict_bi_frame",
+    "vc1me_set_learned_candidates",
 )
 
 
@@ -83,6 +84,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_predict_p_frame.argtypes = [C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult), P(C.c_void_p)]
     lib.vc1me_predict_bi_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_void_p, C.c_void_p,
                                            P(C.c_void_p)]
+    lib.vc1me_set_learned_candidates.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
     lib.vc1me_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
@@ -312,6 +314,10 @@ class Backend:
         arr = (C.POINTER(C.c_float) * n)(*[p.ctypes.data_as(C.POINTER(C.c_float)) for p in props])
         self._check(self.lib.vc1me_analyze_step_proposals(self.ctx, jobs, n, arr, threshold))
         return results
+
+    def set_learned_candidates(self, on: bool) -> None:
+        """Use the proposals' vectors in learned seeding (vc1me_set_learned_candidates)."""
+        self._check(self.lib.vc1me_set_learned_candidates(self.ctx, 1 if on else 0))
 
     def set_graphs(self, on: bool) -> None:
         """Replay plain steps as captured HIP graphs (vc1me_set_graphs)."""

@@ -243,6 +243,9 @@ struct vc1me_ctx {
     // intensity estimation: vectors, samples, the four sums and 49 SADs
     uint8_t* d_ic = nullptr;
     size_t ic_cap = 0;
+    // learned seeding: use the proposals' vectors (1) or only their
+    // confidence gate (0)
+    int learned_candidates = 1;
     // B-picture decision buffers
     uint8_t* d_b = nullptr;
     size_t b_cap = 0;
@@ -414,7 +417,7 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.ext_sad = s.ext_sad; d.ext_mv = s.ext_mv; d.ext_ops = s.ext_ops; d.ext_done = s.ext_done;
     d.mbops = s.mbops;
     d.wl0 = s.wl0; d.wl1 = s.wl1; d.mark = s.mark;
-    d.nn_out = nullptr; d.nn_logit = 0.f;
+    d.nn_out = nullptr; d.nn_logit = 0.f; d.nn_candidates = 1;
     d.seed_mvs = nullptr;
     d.motion_check = 0;
     d.cost20 = nullptr; d.one_mv = nullptr;
@@ -563,6 +566,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         if (nn_out) {
             ctx->h_jobs[i].nn_out = nn_out[i];
             ctx->h_jobs[i].nn_logit = nn_logit;
+            ctx->h_jobs[i].nn_candidates = ctx->learned_candidates ? 1 : 0;
         }
         if (ctx->cost_report) {
             ctx->h_jobs[i].cost20 = ctx->slots[i].cost20;
@@ -1913,6 +1917,13 @@ int vc1me_set_profiling(vc1me_ctx* ctx, int on) {
     if (!ctx) return fail(VC1ME_EINVAL, "null context");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->profiling = on != 0;
+    return VC1ME_OK;
+}
+
+int vc1me_set_learned_candidates(vc1me_ctx* ctx, int on) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->learned_candidates = on ? 1 : 0;
     return VC1ME_OK;
 }
 

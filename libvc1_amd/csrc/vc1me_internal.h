@@ -68,6 +68,7 @@ struct JobDesc {
     // 64-px units, confidence logit) or null, and the logit threshold
     const float* nn_out;
     float nn_logit;
+    int32_t nn_candidates;      // 0: the proposals' vectors are not used (confidence gate only)
     // cost report (null = off): per macroblock 20 * (SAD + lambda * R) of the
     // final 1-MV vector and that vector; lam20 = 20 * lambda = max(20, 7 * pq)
     uint64_t* cost20;

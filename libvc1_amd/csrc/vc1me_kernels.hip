@@ -1354,8 +1354,10 @@ __device__ __forceinline__ void seed_lane(const Ctx& C, const float* o, int gdx,
     const int k = C.lane;
     vx = 0; vy = 0;
     float fx = 0.f, fy = 0.f;
-    if (k == 0) { fx = o[0] * 64.f + static_cast<float>(gdx); fy = o[1] * 64.f + static_cast<float>(gdy); }
-    if (k == 1) { fx = o[2] * 64.f + static_cast<float>(gdx); fy = o[3] * 64.f + static_cast<float>(gdy); }
+    // gate-only mode: both proposals sit on the global vector
+    const float use = C.J->nn_candidates ? 1.f : 0.f;
+    if (k == 0) { fx = use * o[0] * 64.f + static_cast<float>(gdx); fy = use * o[1] * 64.f + static_cast<float>(gdy); }
+    if (k == 1) { fx = use * o[2] * 64.f + static_cast<float>(gdx); fy = use * o[3] * 64.f + static_cast<float>(gdy); }
     if (k < 2) {
         vx = static_cast<int>(rintf(fminf(fmaxf(fx, -4096.f), 4096.f)));
         vy = static_cast<int>(rintf(fminf(fmaxf(fy, -4096.f), 4096.f)));
