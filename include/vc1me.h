@@ -504,9 +504,11 @@ int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jo
 int vc1me_analyze_step_proposals(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* proposals,
                                  float threshold);
 /* Learned seeding without the proposals' vectors (on = 0): confident
- * macroblocks run the seeded search from the predictor, zero and the global
- * vector only, the network acting as a confidence gate; on = 1 (the
- * default) uses its two candidates too.  No reference counterpart. */
+ * macroblocks of vc1me_analyze_step_learned run the seeded search from the
+ * predictor, zero and the global vector only, the network acting as a
+ * confidence gate; on = 1 (the default) uses its two candidates too.  Only
+ * vc1me_analyze_step_learned reads this setting: vc1me_analyze_step_proposals
+ * always uses the caller's vectors.  No reference counterpart. */
 int vc1me_set_learned_candidates(vc1me_ctx* ctx, int on);
 
 /* ---- cost report (no reference counterpart; judges the learned form).

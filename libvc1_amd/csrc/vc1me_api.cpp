@@ -542,7 +542,7 @@ int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_res
 // learned form runs its network there, on the global vectors in the slots).
 int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* nn_out = nullptr, float nn_logit = 0.f,
              const std::function<int(hipStream_t)>& after_global = {},
-             const vc1me_p_result* const* seeds = nullptr, bool motion_check = false) {
+             const vc1me_p_result* const* seeds = nullptr, bool motion_check = false, bool nn_candidates = true) {
     if (n <= 0) return VC1ME_OK;
     if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
     if (n > kMaxJobs) return fail(VC1ME_ELIMIT, "at most " + std::to_string(kMaxJobs) + " jobs per step");
@@ -566,7 +566,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         if (nn_out) {
             ctx->h_jobs[i].nn_out = nn_out[i];
             ctx->h_jobs[i].nn_logit = nn_logit;
-            ctx->h_jobs[i].nn_candidates = ctx->learned_candidates ? 1 : 0;
+            ctx->h_jobs[i].nn_candidates = nn_candidates ? 1 : 0;
         }
         if (ctx->cost_report) {
             ctx->h_jobs[i].cost20 = ctx->slots[i].cost20;
@@ -839,7 +839,10 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
 // ------------------------------------------------------------------ proposal network
 
 struct NnBuffers {
-    int nb = 0, h4 = 0, w4 = 0;       // capacity
+    // capacities: images (gxy), samples over all images (x0, act, dz, in32)
+    // and macroblock cells over all images (pooled, out, dout, tgt, valid);
+    // kept apart because a raster with fewer samples can have more cells
+    size_t nb = 0, px = 0, mbs = 0;
     uint16_t* x0 = nullptr;            // [nb][h4][w4][2]
     uint16_t* act[pnet::kLayers] = {};
     uint16_t* dz[2] = {};
@@ -878,14 +881,20 @@ int nn_dims(int W, int H, int& h4, int& w4, int& mbh, int& mbw) {
     return h4 * w4;
 }
 
-hipError_t nn_reserve(vc1me_nn* nn, int nb, int h4, int w4) {
+hipError_t nn_reserve(vc1me_nn* nn, int nb_, int h4, int w4) {
     NnBuffers& b = nn->b;
-    if (nb <= b.nb && h4 * w4 <= b.h4 * b.w4 && b.x0) return hipSuccess;
+    const size_t nb = static_cast<size_t>(nb_);
+    size_t px = nb * h4 * w4;
+    size_t mbs = nb * ((h4 + 3) / 4) * ((w4 + 3) / 4);
+    if (b.x0 && nb <= b.nb && px <= b.px && mbs <= b.mbs) return hipSuccess;
     auto rel = [](auto*& q) { if (q) (void)hipFree(q); q = nullptr; };
     rel(b.x0); for (auto& a : b.act) rel(a); rel(b.dz[0]); rel(b.dz[1]); rel(b.pooled); rel(b.out); rel(b.dout);
     rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss); rel(b.gxy);
-    const size_t px = static_cast<size_t>(nb) * h4 * w4;
-    const size_t mbs = static_cast<size_t>(nb) * ((h4 + 3) / 4) * ((w4 + 3) / 4);
+    // never shrink: the next request may need the old capacity again
+    const size_t nbc = std::max(nb, b.nb);
+    px = std::max(px, b.px);
+    mbs = std::max(mbs, b.mbs);
+    b.nb = b.px = b.mbs = 0;
     hipError_t e = hipMalloc(&b.x0, px * 2 * 2);
     for (auto& a : b.act) if (e == hipSuccess) e = hipMalloc(&a, px * 32 * 2);
     if (e == hipSuccess) e = hipMalloc(&b.dz[0], px * 32 * 2);
@@ -897,9 +906,9 @@ hipError_t nn_reserve(vc1me_nn* nn, int nb, int h4, int w4) {
     if (e == hipSuccess) e = hipMalloc(&b.valid, mbs);
     if (e == hipSuccess) e = hipMalloc(&b.in32, px * 2 * 4);
     if (e == hipSuccess) e = hipMalloc(&b.loss, 4);
-    if (e == hipSuccess) e = hipMalloc(&b.gxy, sizeof(int32_t) * 2 * nb);
+    if (e == hipSuccess) e = hipMalloc(&b.gxy, sizeof(int32_t) * 2 * nbc);
     if (e != hipSuccess) return e;
-    b.nb = nb; b.h4 = h4; b.w4 = w4;
+    b.nb = nbc; b.px = px; b.mbs = mbs;
     return hipSuccess;
 }
 
@@ -2194,7 +2203,7 @@ int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jo
         return VC1ME_OK;
     };
     const float logit = std::log(threshold / (1.0f - threshold));
-    return run_jobs(ctx, jobs, n, nn->props.data(), logit, forward);
+    return run_jobs(ctx, jobs, n, nn->props.data(), logit, forward, nullptr, false, ctx->learned_candidates != 0);
 }
 
 }  // extern "C"

@@ -1354,10 +1354,17 @@ __device__ __forceinline__ void seed_lane(const Ctx& C, const float* o, int gdx,
     const int k = C.lane;
     vx = 0; vy = 0;
     float fx = 0.f, fy = 0.f;
-    // gate-only mode: both proposals sit on the global vector
-    const float use = C.J->nn_candidates ? 1.f : 0.f;
-    if (k == 0) { fx = use * o[0] * 64.f + static_cast<float>(gdx); fy = use * o[1] * 64.f + static_cast<float>(gdy); }
-    if (k == 1) { fx = use * o[2] * 64.f + static_cast<float>(gdx); fy = use * o[3] * 64.f + static_cast<float>(gdy); }
+    // gate-only mode: both proposals sit on the global vector (selected, not
+    // multiplied by zero, so a non-finite output cannot leak into the seed)
+    const bool use = C.J->nn_candidates != 0;
+    if (k == 0) {
+        fx = (use ? o[0] * 64.f : 0.f) + static_cast<float>(gdx);
+        fy = (use ? o[1] * 64.f : 0.f) + static_cast<float>(gdy);
+    }
+    if (k == 1) {
+        fx = (use ? o[2] * 64.f : 0.f) + static_cast<float>(gdx);
+        fy = (use ? o[3] * 64.f : 0.f) + static_cast<float>(gdy);
+    }
     if (k < 2) {
         vx = static_cast<int>(rintf(fminf(fmaxf(fx, -4096.f), 4096.f)));
         vy = static_cast<int>(rintf(fminf(fmaxf(fy, -4096.f), 4096.f)));

@@ -143,10 +143,13 @@ def blur_reflect(img: np.ndarray, sigma: float) -> np.ndarray:
     return sum(k[i] * a[i:i + img.shape[0], :] for i in range(len(k)))
 
 
-def periodic_texture(rng: np.random.Generator, h: int, w: int, sigma: float) -> np.ndarray:
+def periodic_texture(rng: np.random.Generator, h: int, w: int, sigma: float, make: bool = True):
     """Uniform noise blurred by a periodic Gaussian (FFT), stretched back to
-    roughly the full 8-bit range, rounded.  Tiles seamlessly."""
+    roughly the full 8-bit range, rounded.  Tiles seamlessly.  make=False
+    only consumes the random draws (returns None)."""
     noise = rng.uniform(0.0, 255.0, size=(h, w))
+    if not make:
+        return None
     fy = np.fft.fftfreq(h)[:, None]
     fx = np.fft.fftfreq(w)[None, :]
     gain = np.exp(-2.0 * (np.pi * sigma) ** 2 * (fx * fx + fy * fy))
@@ -209,16 +212,16 @@ class _Layer:
     h: int
 
 
-def _new_background(rng, width, height):
+def _new_background(rng, width, height, make=True):
     sigma = float(rng.uniform(2.0, 6.0))
     th = 1 << int(np.ceil(np.log2(height + 64)))
     tw = 1 << int(np.ceil(np.log2(width + 64)))
-    return (periodic_texture(rng, th, tw, sigma),
-            periodic_texture(rng, th // 2, tw // 2, sigma / 2),
-            periodic_texture(rng, th // 2, tw // 2, sigma / 2))
+    return (periodic_texture(rng, th, tw, sigma, make),
+            periodic_texture(rng, th // 2, tw // 2, sigma / 2, make),
+            periodic_texture(rng, th // 2, tw // 2, sigma / 2, make))
 
 
-def _new_rects(rng, width, height, count, max_obj_px):
+def _new_rects(rng, width, height, count, max_obj_px, make=True):
     rects = []
     for _ in range(count):
         w = int(rng.integers(32, 257)) * 2
@@ -227,9 +230,9 @@ def _new_rects(rng, width, height, count, max_obj_px):
         tw = 1 << int(np.ceil(np.log2(w)))
         th = 1 << int(np.ceil(np.log2(h)))
         rects.append(_Layer(
-            periodic_texture(rng, th, tw, sigma),
-            periodic_texture(rng, th // 2, tw // 2, sigma / 2),
-            periodic_texture(rng, th // 2, tw // 2, sigma / 2),
+            periodic_texture(rng, th, tw, sigma, make),
+            periodic_texture(rng, th // 2, tw // 2, sigma / 2, make),
+            periodic_texture(rng, th // 2, tw // 2, sigma / 2, make),
             int(rng.integers(-w // 2, width)) * 4, int(rng.integers(-h // 2, height)) * 4,
             int(rng.integers(-4 * max_obj_px, 4 * max_obj_px + 1)),
             int(rng.integers(-4 * max_obj_px, 4 * max_obj_px + 1)),
@@ -305,6 +308,24 @@ def pan_rectangles(seed: int, frames: int, width: int = 1920, height: int = 1080
         if k in wanted:
             out[k] = pic
     return out
+
+
+def scene_events(seed: int, frames: int, width: int = 1920, height: int = 1080, max_pan_px: int = 16,
+                 max_obj_px: int = 48, rects: int = 8, cut_every: int = 0,
+                 dup_fraction: float = 0.0) -> dict[str, list[int]]:
+    """The frames of pan_rectangles(seed, frames, ...) that start a new scene
+    ("cuts") and that repeat the previous frame exactly ("duplicates"),
+    found by replaying the generator's random draws without building any
+    texture."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    _new_background(rng, width, height, make=False)
+    rng.integers(-4 * max_pan_px, 4 * max_pan_px + 1)
+    rng.integers(-4 * max_pan_px, 4 * max_pan_px + 1)
+    _new_rects(rng, width, height, rects, max_obj_px, make=False)
+    dup = rng.uniform(0.0, 1.0, size=frames) < dup_fraction
+    dup[0] = False
+    cuts = [k for k in range(1, frames) if cut_every and k % cut_every == 0]
+    return {"cuts": cuts, "duplicates": [int(k) for k in np.nonzero(dup)[0]]}
 
 
 def ibbp_p_pairs(frames: int, gop: int = 15, spacing: int = 3) -> list[tuple[int, int]]:

@@ -21,6 +21,8 @@ The fixtures are data only.
     python tests/golden/make_golden.py motion       # motion_check.json only (has_good_motion_match,
                                                     # refine_qpel_block)
     python tests/golden/make_golden.py predict      # predict.json only (P prediction pictures)
+    python tests/golden/make_golden.py configs      # config_digests.json only (benchmark configurations'
+                                                    # pairs at full size, tests/_config_cases.py)
 """
 from __future__ import annotations
 
@@ -342,7 +344,50 @@ def predict_bi_fixtures() -> list:
     return out
 
 
+def _config_case(job):
+    """(key, sha256, scalars) of one benchmark-configuration pair analysed by
+    the compiled reference (run in a worker process)."""
+    import _config_cases as cc
+    kind, arg, label = job
+    if kind == "cfg2":
+        d = cc.cfg2_frames()
+        c, r = d[arg[0]], d[arg[1]]
+        p = cc.cfg2_params()
+        key = f"cfg2/{arg[0]}-{arg[1]}"
+    elif kind == "cfg3":
+        st = cc.cfg3_stream(arg)
+        w, h = cc.CFG3_RASTERS[arg]
+        c, r = st[label[0]], st[label[1]]
+        p = cc.cfg3_params(w, h)
+        key = f"cfg3/{w}x{h}/{label[0]}-{label[1]}"
+    else:
+        seed, pairs = cc.cfg5_stream(arg)
+        c, r = pairs[label]
+        p = cc.cfg5_params()
+        key = f"cfg5/{seed}/{label}"
+    res = reference_analyze(p, c, r)
+    return key, cc.digest(res), res.scalars()
+
+
+def config_fixtures() -> dict:
+    """Digests of the reference's analysis of the benchmark configurations'
+    pairs (tests/_config_cases.py), written to config_digests.json."""
+    import _config_cases as cc
+    jobs = [("cfg2", pr, None) for pr in cc.CFG2_PAIRS]
+    jobs += [("cfg3", k, pr) for k in range(len(cc.CFG3_RASTERS)) for pr in ((1, 0), (2, 1))]
+    jobs += [("cfg5", 0, "cut"), ("cfg5", 0, "duplicate"), ("cfg5", 1, "after_cut"), ("cfg5", 2, "cut")]
+    out = {}
+    for key, sha, scal in cc.generate_parallel(_config_case, jobs, int(os.environ.get("GOLDEN_WORKERS", "8"))):
+        out[key] = {"sha256": sha, "scalars": scal}
+        print(f"config {key}: {sha[:16]} {scal}", flush=True)
+    return out
+
+
 def main() -> int:
+    if sys.argv[1:] == ["configs"]:
+        with open(os.path.join(HERE, "config_digests.json"), "w") as f:
+            json.dump(config_fixtures(), f, indent=1, sort_keys=True)
+        return 0
     if sys.argv[1:] == ["predict"]:
         with open(os.path.join(HERE, "predict.json"), "w") as f:
             json.dump(predict_fixtures(), f)

@@ -103,6 +103,29 @@ def test_forward_frames_matches_oracle(backend, net):
     assert _rel(got, want) < 1e-2
 
 
+def test_forward_frames_fewer_samples_more_cells(backend, net):
+    """176x144 then 168x148 on one network: the second raster has fewer
+    downsampled samples (37x42 < 36x44) but more macroblock cells (10x11 >
+    9x11), so the per-cell buffers must grow although the sample buffers
+    need not."""
+    params = _params(17)
+    net.set_params(params)
+    for k, (w, h) in enumerate(((176, 144), (168, 148))):
+        c, r = _pair(80 + k, w, h)
+        fc, fr = backend.upload(c), backend.upload(r)
+        g = np.array([[2 - k, 1 + k]], np.int32)
+        _say(f"[case] proposal net forward {w}x{h} after a larger-sample raster")
+        try:
+            got = net.forward_frames([(fc, fr)], g)
+        finally:
+            fc.release()
+            fr.release()
+        x = pn.downsample(c.y, r.y, int(g[0, 0]), int(g[0, 1]))[None]
+        want, _ = pn.forward(params.astype(np.float64), x, quant=True)
+        assert got.shape == want.shape
+        assert _rel(got, want) < 1e-2
+
+
 def test_train_step_loss_and_update(backend, net):
     params = _params(5)
     net.set_params(params)
