@@ -176,6 +176,25 @@ int vc1me_score_batch_frames(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
                              const vc1me_candidate* candidates, size_t n,
                              uint64_t* costs_out);
 
+/* Same contract as vc1me_score_batch_frames, but the call allocates nothing:
+ * no hipMalloc, no hipFree, no allocating library routine.  No counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code around HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The candidate list and its 32-bit costs are staged in a
+ * fixed scratch array of the library (768 KiB per GPU, one copy per GPU,
+ * held until the process ends), so n is limited to
+ * VC1ME_NOALLOC_MAX_CANDIDATES (VC1ME_ELIMIT above it, checked before any
+ * device work).  Every call on one GPU shares that GPU's array, which keeps
+ * the last call's contents (each call overwrites all it reads); the library
+ * serialises such calls with a per-GPU lock, so calls that would overlap on
+ * one GPU wait for each other instead of racing. */
+#define VC1ME_NOALLOC_MAX_CANDIDATES 65536
+int vc1me_score_batch_frames_noalloc(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
+                                     int x0, int y0, int bw, int bh, int sample_stride,
+                                     int bilinear, int rnd, int metric,
+                                     const vc1me_candidate* candidates, size_t n,
+                                     uint64_t* costs_out);
+
 /* ---- (ii) whole-frame plain P motion estimation: the behaviour of
  * Vc1Encoder::analyze_p_picture_core(cur, ref, mode, allow_intra,
  * shared_seed=nullptr, b_reuse=nullptr) (src/encoder_motion.cpp:47-442).

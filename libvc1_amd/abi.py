@@ -16,6 +16,7 @@ VC1ME_ENOMEM = -3
 VC1ME_ELIMIT = -4
 VC1ME_EINJECTED = -5
 VC1ME_EKERNEL = -6
+VC1ME_NOALLOC_MAX_CANDIDATES = 65536   # include/vc1me.h
 
 METRIC_SAD = 0
 METRIC_SATD = 1

@@ -29,7 +29,7 @@ EXPORTS = (
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
     "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
     "vc1me_has_good_motion_match", "vc1me_predict_p_frame", "vc1me_predict_bi_frame",
-    "vc1me_set_learned_candidates",
+    "vc1me_set_learned_candidates", "vc1me_score_batch_frames_noalloc",
 )
 
 
@@ -68,6 +68,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_score_batch_frames.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                              C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                              P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
+    lib.vc1me_score_batch_frames_noalloc.argtypes = lib.vc1me_score_batch_frames.argtypes
     lib.vc1me_analyze_p_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult)]
     lib.vc1me_analyze_step.argtypes = [C.c_void_p, P(abi.Job), C.c_int]
     lib.vc1me_analyze_p_frame_seeded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult),

@@ -78,6 +78,41 @@ bool relax_guess() {
     return !(e && *e == '0');
 }
 
+// ---- scratch of the allocation-free batch scorer
+// (vc1me_score_batch_frames_noalloc): the candidate list staged for the
+// kernel, then its 32-bit costs.  One copy per GPU, held until the process
+// ends; host code never uses the array's name as an address, only the
+// pointer hipGetSymbolAddress returned for the current device.
+constexpr size_t kNoAllocCandBytes = static_cast<size_t>(VC1ME_NOALLOC_MAX_CANDIDATES) * sizeof(int2);
+constexpr size_t kNoAllocScratchBytes =
+    kNoAllocCandBytes + static_cast<size_t>(VC1ME_NOALLOC_MAX_CANDIDATES) * sizeof(uint32_t);
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+struct DeviceScratch {
+    std::mutex mu;          // serialises the allocation-free calls on this GPU
+    void* ptr = nullptr;    // device address of this GPU's copy of `scratch`
+    bool ready = false;
+};
+DeviceScratch g_scratch[kMaxScratchDevices];
+
+// Looks up this device's copy of the scratch array once (the device must be
+// current).  Returns its address, or null with the error in `err`.
+void* scratch_for_device(int device, hipError_t* err) {
+    *err = hipSuccess;
+    if (device < 0 || device >= kMaxScratchDevices) { *err = hipErrorInvalidDevice; return nullptr; }
+    DeviceScratch& d = g_scratch[device];
+    std::lock_guard<std::mutex> lock(d.mu);
+    if (!d.ready) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) { *err = e != hipSuccess ? e : hipErrorInvalidSymbol; return nullptr; }
+        d.ptr = ptr;
+        d.ready = true;
+    }
+    return d.ptr;
+}
+
 uint64_t fnv64(const uint8_t* p, size_t n) {
     uint64_t h = 1469598103934665603ull;
     size_t i = 0;
@@ -253,6 +288,10 @@ struct vc1me_ctx {
     // with profiling on, device milliseconds per pass
     std::vector<int64_t> pass_analyses;
     std::vector<double> pass_ms;
+    // this device's copy of the allocation-free scorer's scratch array (null
+    // if its lookup failed at creation; scratch_error says why)
+    void* scratch = nullptr;
+    hipError_t scratch_error = hipSuccess;
 };
 
 namespace {
@@ -1029,6 +1068,9 @@ int vc1me_create(int device, vc1me_ctx** out) {
     }
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete ctx; return fail(VC1ME_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    // the allocation-free scorer's scratch array on this device (looked up
+    // once per device, with the device current)
+    ctx->scratch = scratch_for_device(ctx->device, &ctx->scratch_error);
     *out = ctx;
     return VC1ME_OK;
 }
@@ -1132,6 +1174,46 @@ int vc1me_score_batch_frames(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
     if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
     return score_on_device(ctx, cur->y, ref->y, cur->w, cur->h, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
                            candidates, n, costs_out);
+}
+
+int vc1me_score_batch_frames_noalloc(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, int x0, int y0,
+                                     int bw, int bh, int sample_stride, int bilinear, int rnd, int metric,
+                                     const vc1me_candidate* candidates, size_t n, uint64_t* costs_out) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (!cur || !ref || bw <= 0 || bh <= 0 || !candidates || !n) return VC1ME_OK;
+    if (cur->w != ref->w || cur->h != ref->h) return fail(VC1ME_EINVAL, "frame size mismatch");
+    if (!costs_out) return fail(VC1ME_EINVAL, "null costs");
+    if (n > static_cast<size_t>(VC1ME_NOALLOC_MAX_CANDIDATES))
+        return fail(VC1ME_ELIMIT, "the allocation-free scorer takes at most " +
+                                      std::to_string(VC1ME_NOALLOC_MAX_CANDIDATES) + " candidates per call");
+    if (int rc = validate_score_args(cur->w, cur->h, n)) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    if (!ctx->scratch)
+        return fail(VC1ME_EDEVICE, std::string("scratch array unavailable: ") + hipGetErrorString(ctx->scratch_error));
+    // every call on this GPU shares its scratch array: one at a time
+    std::lock_guard<std::mutex> dev_lock(g_scratch[ctx->device].mu);
+    int2* d_cands = static_cast<int2*>(ctx->scratch);
+    uint32_t* d_costs = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ctx->scratch) + kNoAllocCandBytes);
+    // the 32-bit costs land in the upper half of costs_out and are widened
+    // in place from the front: entry i is read before any write reaches it
+    // (8i + 8 <= 4n + 4i + 4 for every i < n)
+    uint32_t* staged = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(costs_out) + 4 * n);
+    hipStream_t st = ctx->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(hipMemcpyAsync(d_cands, candidates, n * sizeof(int2), hipMemcpyHostToDevice, st));
+    HIP_TRY(vc1me::launch_score_batch(cur->y, ref->y, cur->w, cur->h, x0, y0, bw, bh, sample_stride, bilinear, rnd,
+                                      metric, d_cands, static_cast<int>(n), d_costs, st));
+    HIP_TRY(hipMemcpyAsync(staged, d_costs, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const auto t1 = std::chrono::steady_clock::now();
+    for (size_t i = 0; i < n; ++i) costs_out[i] = staged[i];
+    ctx->counters.dispatches++;
+    ctx->counters.candidates += n;
+    ctx->counters.submit_wait_nanoseconds +=
+        static_cast<uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count());
+    return VC1ME_OK;
 }
 
 int vc1me_block_costs(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const vc1me_params* params,
