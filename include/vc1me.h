@@ -383,6 +383,26 @@ int vc1me_predict_p_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_pa
 int vc1me_predict_bi_frame(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future,
                            const vc1me_params* params, const int32_t* fmvs, const int32_t* bmvs, vc1me_frame** out);
 
+/* The padded prediction pictures of the loop reconstruction
+ * (src/encoder_encode.cpp:1183, :1195, :1681-1687): the same pictures as
+ * vc1me_predict_p_frame / vc1me_predict_bi_frame built by
+ * motion_compensate_4mv_padded, motion_compensate_mb_padded
+ * (src/encoder_motion_compensation.cpp:573-657) and
+ * motion_compensate_bi_mb_padded (:712-763).  The new resident frame covers
+ * the whole macroblock raster of params (mbw*16 x mbh*16, the encoder's
+ * empty_padded_frame, src/encoder_encode.cpp:1949-1956), every sample of
+ * every macroblock written; each reference is sampled with clamping to its
+ * own raster, which may be the coded raster or the padded one (the
+ * encoder's padded loop reference).  A B picture's padded reconstruction
+ * takes forward and backward macroblocks from vc1me_predict_p_frame_padded
+ * of each reference and interpolated or direct ones from
+ * vc1me_predict_bi_frame_padded. */
+int vc1me_predict_p_frame_padded(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params,
+                                 const vc1me_p_result* analysis, vc1me_frame** out);
+int vc1me_predict_bi_frame_padded(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future,
+                                  const vc1me_params* params, const int32_t* fmvs, const int32_t* bmvs,
+                                  vc1me_frame** out);
+
 /* Copies a resident frame's planes back to the host (width x height luma,
  * two width/2 x height/2 chroma planes).  No reference counterpart. */
 int vc1me_frame_download(vc1me_ctx* ctx, const vc1me_frame* frame, uint8_t* y, uint8_t* u, uint8_t* v);

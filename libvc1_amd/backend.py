@@ -29,7 +29,8 @@ EXPORTS = (
     "vc1me_analyze_step_seeded", "vc1me_estimate_intensity_comp", "vc1me_frame_intensity_compensated",
     "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
     "vc1me_has_good_motion_match", "vc1me_predict_p_frame", "vc1me_predict_bi_frame",
-    "vc1me_set_learned_candidates", "vc1me_score_batch_frames_noalloc",
+    "vc1me_set_learned_candidates", "vc1me_score_batch_frames_noalloc", "vc1me_predict_p_frame_padded",
+    "vc1me_predict_bi_frame_padded",
 )
 
 
@@ -85,6 +86,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_predict_p_frame.argtypes = [C.c_void_p, C.c_void_p, P(abi.Params), P(abi.PResult), P(C.c_void_p)]
     lib.vc1me_predict_bi_frame.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, P(abi.Params), C.c_void_p, C.c_void_p,
                                            P(C.c_void_p)]
+    lib.vc1me_predict_p_frame_padded.argtypes = lib.vc1me_predict_p_frame.argtypes
+    lib.vc1me_predict_bi_frame_padded.argtypes = lib.vc1me_predict_bi_frame.argtypes
     lib.vc1me_set_learned_candidates.argtypes = [C.c_void_p, C.c_int]
     lib.vc1me_frame_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.vc1me_analyze_step_seeded.argtypes = [C.c_void_p, P(abi.Job), C.POINTER(P(abi.PResult)), C.c_int]
@@ -201,6 +204,26 @@ class Backend:
         self._check(self.lib.vc1me_predict_bi_frame(self.ctx, past.handle, future.handle, C.byref(params),
                                                     f.ctypes.data, b.ctypes.data, C.byref(h)))
         return Frame(self, h.value, past.width, past.height)
+
+    def predict_p_frame_padded(self, ref: Frame, params: abi.Params, result: abi.ResultArrays) -> Frame:
+        """vc1me_predict_p_frame_padded: the padded picture (the macroblock
+        raster of params) as a new resident frame."""
+        h = C.c_void_p()
+        self._check(self.lib.vc1me_predict_p_frame_padded(self.ctx, ref.handle, C.byref(params),
+                                                          C.byref(result.struct), C.byref(h)))
+        mbw, mbh = abi.mb_grid(params.width, params.height)
+        return Frame(self, h.value, mbw * 16, mbh * 16)
+
+    def predict_bi_frame_padded(self, past: Frame, future: Frame, params: abi.Params, fmvs, bmvs) -> Frame:
+        """vc1me_predict_bi_frame_padded: a new resident frame of the
+        macroblock raster of params."""
+        f = np.ascontiguousarray(np.asarray(fmvs, np.int32))
+        b = np.ascontiguousarray(np.asarray(bmvs, np.int32))
+        h = C.c_void_p()
+        self._check(self.lib.vc1me_predict_bi_frame_padded(self.ctx, past.handle, future.handle, C.byref(params),
+                                                           f.ctypes.data, b.ctypes.data, C.byref(h)))
+        mbw, mbh = abi.mb_grid(params.width, params.height)
+        return Frame(self, h.value, mbw * 16, mbh * 16)
 
     def intensity_compensated(self, ref: Frame, ic: tuple) -> Frame:
         """vc1me_frame_intensity_compensated: a new resident frame."""

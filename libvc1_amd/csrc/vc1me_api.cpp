@@ -1462,94 +1462,153 @@ int vc1me_frame_intensity_compensated(vc1me_ctx* ctx, const vc1me_frame* ref, co
 
 // ---- motion compensation
 
-int vc1me_predict_p_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params,
-                          const vc1me_p_result* analysis, vc1me_frame** out) {
-    if (!ctx || !ref || !params || !analysis || !out) return fail(VC1ME_EINVAL, "null argument");
-    if (!analysis->mvs || !analysis->block_mvs || !analysis->use_4mv)
-        return fail(VC1ME_EINVAL, "the analysis lacks mvs, block_mvs or use_4mv");
-    if (params->width != ref->w || params->height != ref->h) return fail(VC1ME_EINVAL, "raster differs from the parameters");
-    if (params->mode < VC1ME_MODE_ONE_MV_QPEL || params->mode > VC1ME_MODE_MIXED_MV) return fail(VC1ME_EINVAL, "bad mode");
-    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    if (int rc = set_device(ctx)) return rc;
-    const int W = ref->w, H = ref->h;
-    const size_t mbs = static_cast<size_t>((W + 15) / 16) * ((H + 15) / 16);
-    const size_t o_mv = 0, o_bmv = (mbs * 8 + 255) & ~size_t(255), o_four = o_bmv + ((mbs * 32 + 255) & ~size_t(255));
-    const size_t need = o_four + mbs;
-    if (ctx->b_cap < need) {
-        if (ctx->d_b) (void)hipFree(ctx->d_b);
-        ctx->d_b = nullptr;
-        ctx->b_cap = 0;
-        const hipError_t e = hipMalloc(&ctx->d_b, need);
-        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("prediction buffers: ") + hipGetErrorString(e));
-        ctx->b_cap = need;
-    }
-    const size_t ys = static_cast<size_t>(W) * H, cs = static_cast<size_t>(W / 2) * (H / 2);
+namespace {
+
+// a new resident frame of w x h (chroma planes after luma, the 64-byte tail
+// of an uploaded frame)
+int new_frame(int w, int h, vc1me_frame** out) {
+    const size_t ys = static_cast<size_t>(w) * h, cs = static_cast<size_t>(w / 2) * (h / 2);
     auto* f = new vc1me_frame();
-    f->w = W;
-    f->h = H;
+    f->w = w;
+    f->h = h;
     uint8_t* mem = nullptr;
-    hipError_t e = hipMalloc(&mem, ys + 2 * cs + 64);
+    const hipError_t e = hipMalloc(&mem, ys + 2 * cs + 64);
     if (e != hipSuccess) { delete f; return fail(VC1ME_ENOMEM, std::string("frame allocation: ") + hipGetErrorString(e)); }
     f->y = mem;
     f->u = mem + ys;
     f->v = mem + ys + cs;
-    hipStream_t st = ctx->stream;
-    uint8_t* base = ctx->d_b;
-    e = hipMemcpyAsync(base + o_mv, analysis->mvs, mbs * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_bmv, analysis->block_mvs, mbs * 32, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_four, analysis->use_4mv, mbs, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = vc1me::launch_predict_p(ref->y, ref->u, ref->v, W, H, params->mode, reinterpret_cast<const int2*>(base + o_mv),
-                                    reinterpret_cast<const int2*>(base + o_bmv), base + o_four, f->y, f->u, f->v, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) { (void)hipFree(mem); delete f; return fail(VC1ME_EDEVICE, std::string("prediction: ") + hipGetErrorString(e)); }
     *out = f;
     return VC1ME_OK;
 }
 
-int vc1me_predict_bi_frame(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future,
-                           const vc1me_params* params, const int32_t* fmvs, const int32_t* bmvs, vc1me_frame** out) {
+void drop_frame(vc1me_frame* f) {
+    (void)hipFree(f->y);
+    delete f;
+}
+
+// the context's prediction scratch of at least `need` bytes
+int ensure_b_scratch(vc1me_ctx* ctx, size_t need) {
+    if (ctx->b_cap >= need) return VC1ME_OK;
+    if (ctx->d_b) (void)hipFree(ctx->d_b);
+    ctx->d_b = nullptr;
+    ctx->b_cap = 0;
+    const hipError_t e = hipMalloc(&ctx->d_b, need);
+    if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("prediction buffers: ") + hipGetErrorString(e));
+    ctx->b_cap = need;
+    return VC1ME_OK;
+}
+
+// The P prediction picture (padded = false: the coded raster of params;
+// padded = true: its macroblock raster), context lock held.
+int predict_p(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params, const vc1me_p_result* analysis,
+              bool padded, vc1me_frame** out) {
+    const int W = params->width, H = params->height;
+    const size_t mbs = static_cast<size_t>((W + 15) / 16) * ((H + 15) / 16);
+    const int DW = padded ? (W + 15) / 16 * 16 : W, DH = padded ? (H + 15) / 16 * 16 : H;
+    const size_t o_mv = 0, o_bmv = (mbs * 8 + 255) & ~size_t(255), o_four = o_bmv + ((mbs * 32 + 255) & ~size_t(255));
+    if (int rc = ensure_b_scratch(ctx, o_four + mbs)) return rc;
+    vc1me_frame* f = nullptr;
+    if (int rc = new_frame(DW, DH, &f)) return rc;
+    hipStream_t st = ctx->stream;
+    uint8_t* base = ctx->d_b;
+    hipError_t e = hipMemcpyAsync(base + o_mv, analysis->mvs, mbs * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_bmv, analysis->block_mvs, mbs * 32, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_four, analysis->use_4mv, mbs, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = vc1me::launch_predict_p(ref->y, ref->u, ref->v, ref->w, ref->h, DW, DH, params->mode,
+                                    reinterpret_cast<const int2*>(base + o_mv),
+                                    reinterpret_cast<const int2*>(base + o_bmv), base + o_four, f->y, f->u, f->v, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { drop_frame(f); return fail(VC1ME_EDEVICE, std::string("prediction: ") + hipGetErrorString(e)); }
+    *out = f;
+    return VC1ME_OK;
+}
+
+int check_predict_p(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params, const vc1me_p_result* analysis,
+                    vc1me_frame** out) {
+    if (!ctx || !ref || !params || !analysis || !out) return fail(VC1ME_EINVAL, "null argument");
+    if (!analysis->mvs || !analysis->block_mvs || !analysis->use_4mv)
+        return fail(VC1ME_EINVAL, "the analysis lacks mvs, block_mvs or use_4mv");
+    if (params->width <= 0 || params->height <= 0 || (params->width & 1) || (params->height & 1))
+        return fail(VC1ME_EINVAL, "raster must be positive and even");
+    if (params->mode < VC1ME_MODE_ONE_MV_QPEL || params->mode > VC1ME_MODE_MIXED_MV) return fail(VC1ME_EINVAL, "bad mode");
+    return VC1ME_OK;
+}
+
+// The bi-directional prediction picture, context lock held.
+int predict_bi(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future, const vc1me_params* params,
+               const int32_t* fmvs, const int32_t* bmvs, bool padded, vc1me_frame** out) {
+    const int W = params->width, H = params->height;
+    const size_t mbs = static_cast<size_t>((W + 15) / 16) * ((H + 15) / 16);
+    const int DW = padded ? (W + 15) / 16 * 16 : W, DH = padded ? (H + 15) / 16 * 16 : H;
+    const size_t o_f = 0, o_b = (mbs * 8 + 255) & ~size_t(255);
+    if (int rc = ensure_b_scratch(ctx, o_b + mbs * 8)) return rc;
+    vc1me_frame* f = nullptr;
+    if (int rc = new_frame(DW, DH, &f)) return rc;
+    hipStream_t st = ctx->stream;
+    uint8_t* base = ctx->d_b;
+    hipError_t e = hipMemcpyAsync(base + o_f, fmvs, mbs * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + o_b, bmvs, mbs * 8, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = vc1me::launch_predict_bi(past->y, past->w, past->h, future->y, future->w, future->h, DW, DH, params->mode,
+                                     reinterpret_cast<const int2*>(base + o_f),
+                                     reinterpret_cast<const int2*>(base + o_b), f->y, f->u, f->v, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) { drop_frame(f); return fail(VC1ME_EDEVICE, std::string("prediction: ") + hipGetErrorString(e)); }
+    *out = f;
+    return VC1ME_OK;
+}
+
+int check_predict_bi(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future, const vc1me_params* params,
+                     const int32_t* fmvs, const int32_t* bmvs, vc1me_frame** out) {
     if (!ctx || !past || !future || !params || !fmvs || !bmvs || !out) return fail(VC1ME_EINVAL, "null argument");
-    if (params->width != past->w || params->height != past->h || future->w != past->w || future->h != past->h)
-        return fail(VC1ME_EINVAL, "raster differs from the parameters");
+    if (params->width <= 0 || params->height <= 0 || (params->width & 1) || (params->height & 1))
+        return fail(VC1ME_EINVAL, "raster must be positive and even");
     if (params->mode < VC1ME_MODE_ONE_MV_QPEL || params->mode > VC1ME_MODE_ONE_MV_HPEL_BILINEAR)
         return fail(VC1ME_EINVAL, "bi-directional prediction takes a 1-MV mode");
+    return VC1ME_OK;
+}
+
+}  // namespace
+
+int vc1me_predict_p_frame(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params,
+                          const vc1me_p_result* analysis, vc1me_frame** out) {
+    if (int rc = check_predict_p(ctx, ref, params, analysis, out)) return rc;
+    if (params->width != ref->w || params->height != ref->h) return fail(VC1ME_EINVAL, "raster differs from the parameters");
     if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (int rc = set_device(ctx)) return rc;
-    const int W = past->w, H = past->h;
-    const size_t mbs = static_cast<size_t>((W + 15) / 16) * ((H + 15) / 16);
-    const size_t o_f = 0, o_b = (mbs * 8 + 255) & ~size_t(255), need = o_b + mbs * 8;
-    if (ctx->b_cap < need) {
-        if (ctx->d_b) (void)hipFree(ctx->d_b);
-        ctx->d_b = nullptr;
-        ctx->b_cap = 0;
-        const hipError_t e = hipMalloc(&ctx->d_b, need);
-        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("prediction buffers: ") + hipGetErrorString(e));
-        ctx->b_cap = need;
-    }
-    const size_t ys = static_cast<size_t>(W) * H, cs = static_cast<size_t>(W / 2) * (H / 2);
-    auto* f = new vc1me_frame();
-    f->w = W;
-    f->h = H;
-    uint8_t* mem = nullptr;
-    hipError_t e = hipMalloc(&mem, ys + 2 * cs + 64);
-    if (e != hipSuccess) { delete f; return fail(VC1ME_ENOMEM, std::string("frame allocation: ") + hipGetErrorString(e)); }
-    f->y = mem;
-    f->u = mem + ys;
-    f->v = mem + ys + cs;
-    hipStream_t st = ctx->stream;
-    uint8_t* base = ctx->d_b;
-    e = hipMemcpyAsync(base + o_f, fmvs, mbs * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(base + o_b, bmvs, mbs * 8, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = vc1me::launch_predict_bi(past->y, future->y, W, H, params->mode, reinterpret_cast<const int2*>(base + o_f),
-                                     reinterpret_cast<const int2*>(base + o_b), f->y, f->u, f->v, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) { (void)hipFree(mem); delete f; return fail(VC1ME_EDEVICE, std::string("prediction: ") + hipGetErrorString(e)); }
-    *out = f;
-    return VC1ME_OK;
+    return predict_p(ctx, ref, params, analysis, false, out);
+}
+
+int vc1me_predict_p_frame_padded(vc1me_ctx* ctx, const vc1me_frame* ref, const vc1me_params* params,
+                                 const vc1me_p_result* analysis, vc1me_frame** out) {
+    if (int rc = check_predict_p(ctx, ref, params, analysis, out)) return rc;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return predict_p(ctx, ref, params, analysis, true, out);
+}
+
+int vc1me_predict_bi_frame(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future,
+                           const vc1me_params* params, const int32_t* fmvs, const int32_t* bmvs, vc1me_frame** out) {
+    if (int rc = check_predict_bi(ctx, past, future, params, fmvs, bmvs, out)) return rc;
+    if (params->width != past->w || params->height != past->h || future->w != past->w || future->h != past->h)
+        return fail(VC1ME_EINVAL, "raster differs from the parameters");
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return predict_bi(ctx, past, future, params, fmvs, bmvs, false, out);
+}
+
+int vc1me_predict_bi_frame_padded(vc1me_ctx* ctx, const vc1me_frame* past, const vc1me_frame* future,
+                                  const vc1me_params* params, const int32_t* fmvs, const int32_t* bmvs,
+                                  vc1me_frame** out) {
+    if (int rc = check_predict_bi(ctx, past, future, params, fmvs, bmvs, out)) return rc;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    return predict_bi(ctx, past, future, params, fmvs, bmvs, true, out);
 }
 
 // ---- B pictures

@@ -172,15 +172,20 @@ constexpr int kBMaxRows = 1024;
 hipError_t launch_b_costs(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbs, hipStream_t s);
 hipError_t launch_b_decide(const JobDesc* d_jobs, const BDesc* d_b, int nb, int max_mbh, hipStream_t s);
 
-// The P prediction picture of an analysis (k_predict_luma, k_predict_chroma).
-hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t* rv, int W, int H, int mode,
-                            const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du, uint8_t* dv,
-                            hipStream_t s);
+// The P prediction picture of an analysis (k_predict_luma, k_predict_chroma):
+// a DW x DH picture (the coded raster, or the macroblock raster for the
+// padded picture of the loop reconstruction) sampled from a reference of
+// raster RW x RH.
+hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t* rv, int RW, int RH, int DW, int DH,
+                            int mode, const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du,
+                            uint8_t* dv, hipStream_t s);
 
 // The bi-directional prediction picture (k_predict_bi); past and fut are
-// frame buffers with the two chroma planes after luma.
-hipError_t launch_predict_bi(const uint8_t* past, const uint8_t* fut, int W, int H, int mode, const int2* fmv,
-                             const int2* bmv, uint8_t* dy, uint8_t* du, uint8_t* dv, hipStream_t s);
+// frame buffers with the two chroma planes after luma, of rasters PW x PH
+// and FW x FH; the picture is DW x DH.
+hipError_t launch_predict_bi(const uint8_t* past, int PW, int PH, const uint8_t* fut, int FW, int FH, int DW, int DH,
+                             int mode, const int2* fmv, const int2* bmv, uint8_t* dy, uint8_t* du, uint8_t* dv,
+                             hipStream_t s);
 
 // Intensity compensation (vc1me_intensity.hip): the luma table at bytes
 // 0..255 and the chroma table at 256..511.

@@ -3449,16 +3449,21 @@ __global__ void __launch_bounds__(1024) k_b_decide(const JobDesc* __restrict__ j
 // ------------------------------------------------------------------ motion compensation
 // The P prediction picture of an analysis: motion_compensate_mb /
 // motion_compensate_4mv (src/encoder_motion_compensation.cpp:474-571), one
-// thread per output sample.
+// thread per output sample.  The same kernels write the padded pictures of
+// the loop reconstruction, motion_compensate_mb_padded / _4mv_padded
+// (:573-657): those cover the whole macroblock raster (DW x DH = mbw*16 x
+// mbh*16) while the samples clamp to the reference's own raster (RW x RH),
+// which is the coded raster or the padded one.  The plain picture is the
+// case DW = RW = W, DH = RH = H.
 
 // luma: the picture's mode for 1-MV macroblocks, bicubic for the four
 // vectors of a 4-MV macroblock (luma_mc_sample_mode / luma_mc_sample)
-__global__ void __launch_bounds__(256) k_predict_luma(const uint8_t* __restrict__ ref, int W, int H, int mode,
-                                                      const int2* __restrict__ mvs, const int2* __restrict__ bmv,
-                                                      const uint8_t* __restrict__ four, uint8_t* __restrict__ dst) {
+__global__ void __launch_bounds__(256) k_predict_luma(const uint8_t* __restrict__ ref, int RW, int RH, int DW, int DH,
+                                                      int mbw, int mode, const int2* __restrict__ mvs,
+                                                      const int2* __restrict__ bmv, const uint8_t* __restrict__ four,
+                                                      uint8_t* __restrict__ dst) {
     const int px = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), py = static_cast<int>(blockIdx.y);
-    if (px >= W || py >= H) return;
-    const int mbw = (W + 15) / 16;
+    if (px >= DW || py >= DH) return;
     const int pos = (py >> 4) * mbw + (px >> 4);
     int2 v;
     bool bil = mode == 2;
@@ -3473,60 +3478,16 @@ __global__ void __launch_bounds__(256) k_predict_luma(const uint8_t* __restrict_
     int n[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int yy = clampi(py + iy - 1 + r, 0, H - 1);
+        const int yy = clampi(py + iy - 1 + r, 0, RH - 1);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) n[r][c] = ref[static_cast<size_t>(yy) * W + clampi(px + ix - 1 + c, 0, W - 1)];
+        for (int c = 0; c < 4; ++c) n[r][c] = ref[static_cast<size_t>(yy) * RW + clampi(px + ix - 1 + c, 0, RW - 1)];
     }
-    dst[static_cast<size_t>(py) * W + px] = static_cast<uint8_t>(mc_sample(n, hm, vm, bil));
+    dst[static_cast<size_t>(py) * DW + px] = static_cast<uint8_t>(mc_sample(n, hm, vm, bil));
 }
 
-// chroma_mc_sample (src/encoder_motion_compensation.cpp:417-447), P form,
-// at the macroblock's vector or, for 4-MV, the median-4 vector (:522-535)
-__global__ void __launch_bounds__(256) k_predict_chroma(const uint8_t* __restrict__ ru, const uint8_t* __restrict__ rv,
-                                                        int W, int H, const int2* __restrict__ mvs,
-                                                        const int2* __restrict__ bmv, const uint8_t* __restrict__ four,
-                                                        uint8_t* __restrict__ du, uint8_t* __restrict__ dv) {
-    const int cw = W / 2, ch = H / 2;
-    const int px = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), py = static_cast<int>(blockIdx.y);
-    if (px >= cw || py >= ch) return;
-    const int mbw = (W + 15) / 16;
-    const int pos = (py >> 3) * mbw + (px >> 3);
-    int2 v;
-    if (four[pos]) {
-        const int2* b = bmv + pos * 4;
-        v = make_int2(median4(b[0].x, b[1].x, b[2].x, b[3].x), median4(b[0].y, b[1].y, b[2].y, b[3].y));
-    } else {
-        v = mvs[pos];
-    }
-    int uvx = arshift(v.x + ((v.x & 3) == 3), 1), uvy = arshift(v.y + ((v.y & 3) == 3), 1);
-    uvx = uvx < 0 ? uvx + (uvx & 1) : uvx - (uvx & 1);
-    uvy = uvy < 0 ? uvy + (uvy & 1) : uvy - (uvy & 1);
-    const int ix = floor_div4(uvx), iy = floor_div4(uvy);
-    const int fx = (uvx - ix * 4) * 2, fy = (uvy - iy * 4) * 2;
-    const int x0 = clampi(px + ix, 0, cw - 1), x1 = clampi(px + ix + 1, 0, cw - 1);
-    const int y0 = clampi(py + iy, 0, ch - 1), y1 = clampi(py + iy + 1, 0, ch - 1);
-    const size_t o = static_cast<size_t>(py) * cw + px;
-    const uint8_t* planes[2] = {ru, rv};
-    uint8_t* outs[2] = {du, dv};
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const uint8_t* p = planes[k];
-        const int a = p[static_cast<size_t>(y0) * cw + x0];
-        int val = a;
-        if (fx || fy) {
-            const int b = p[static_cast<size_t>(y0) * cw + x1], c = p[static_cast<size_t>(y1) * cw + x0];
-            const int d = p[static_cast<size_t>(y1) * cw + x1];
-            val = (a * (8 - fx) * (8 - fy) + b * fx * (8 - fy) + c * (8 - fx) * fy + d * fx * fy + 32) >> 6;
-        }
-        outs[k][o] = static_cast<uint8_t>(val);
-    }
-}
-
-// The bi-directional prediction picture: motion_compensate_bi_mb
-// (src/encoder_motion_compensation.cpp:659-711) for every macroblock, the
-// rounding average of the past prediction at fmvs and the future prediction
-// at bmvs; the future chroma vector rounds odd components away from the even
-// value (:420-435).
+// chroma_mc_sample (src/encoder_motion_compensation.cpp:417-447): `future`
+// rounds an odd half-chroma component away from the even value (the
+// future reference of a bi-directional prediction), otherwise toward it
 __device__ __forceinline__ int chroma_sample(const uint8_t* p, int cw, int ch, int px, int py, int2 v, bool future) {
     int uvx = arshift(v.x + ((v.x & 3) == 3), 1), uvy = arshift(v.y + ((v.y & 3) == 3), 1);
     if (future) {
@@ -3546,6 +3507,35 @@ __device__ __forceinline__ int chroma_sample(const uint8_t* p, int cw, int ch, i
     const int d = p[static_cast<size_t>(y1) * cw + x1];
     return (a * (8 - fx) * (8 - fy) + b * fx * (8 - fy) + c * (8 - fx) * fy + d * fx * fy + 32) >> 6;
 }
+
+// chroma, P form, at the macroblock's vector or, for 4-MV, the median-4
+// vector (:522-535); reference chroma RCW x RCH, destination DCW x DCH
+__global__ void __launch_bounds__(256) k_predict_chroma(const uint8_t* __restrict__ ru, const uint8_t* __restrict__ rv,
+                                                        int RCW, int RCH, int DCW, int DCH, int mbw,
+                                                        const int2* __restrict__ mvs, const int2* __restrict__ bmv,
+                                                        const uint8_t* __restrict__ four, uint8_t* __restrict__ du,
+                                                        uint8_t* __restrict__ dv) {
+    const int px = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), py = static_cast<int>(blockIdx.y);
+    if (px >= DCW || py >= DCH) return;
+    const int pos = (py >> 3) * mbw + (px >> 3);
+    int2 v;
+    if (four[pos]) {
+        const int2* b = bmv + pos * 4;
+        v = make_int2(median4(b[0].x, b[1].x, b[2].x, b[3].x), median4(b[0].y, b[1].y, b[2].y, b[3].y));
+    } else {
+        v = mvs[pos];
+    }
+    const size_t o = static_cast<size_t>(py) * DCW + px;
+    du[o] = static_cast<uint8_t>(chroma_sample(ru, RCW, RCH, px, py, v, false));
+    dv[o] = static_cast<uint8_t>(chroma_sample(rv, RCW, RCH, px, py, v, false));
+}
+
+// The bi-directional prediction picture: motion_compensate_bi_mb
+// (src/encoder_motion_compensation.cpp:659-711) or, over the macroblock
+// raster, motion_compensate_bi_mb_padded (:712-763) for every macroblock:
+// the rounding average of the past prediction at fmvs and the future
+// prediction at bmvs, each reference clamped to its own raster (PW x PH,
+// FW x FH; planes contiguous, chroma after luma).
 __device__ __forceinline__ int luma_sample(const uint8_t* ref, int W, int H, int px, int py, int2 v, bool bil) {
     const int ix = floor_div4(v.x), iy = floor_div4(v.y);
     int n[4][4];
@@ -3557,50 +3547,54 @@ __device__ __forceinline__ int luma_sample(const uint8_t* ref, int W, int H, int
     }
     return mc_sample(n, v.x - ix * 4, v.y - iy * 4, bil);
 }
-__global__ void __launch_bounds__(256) k_predict_bi(const uint8_t* __restrict__ past, const uint8_t* __restrict__ fut,
-                                                    int W, int H, int mode, const int2* __restrict__ fmv,
+__global__ void __launch_bounds__(256) k_predict_bi(const uint8_t* __restrict__ past, int PW, int PH,
+                                                    const uint8_t* __restrict__ fut, int FW, int FH, int DW, int DH,
+                                                    int mbw, int mode, const int2* __restrict__ fmv,
                                                     const int2* __restrict__ bmv, uint8_t* __restrict__ dy,
                                                     uint8_t* __restrict__ du, uint8_t* __restrict__ dv) {
     const int px = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x), py = static_cast<int>(blockIdx.y);
-    const int mbw = (W + 15) / 16, cw = W / 2, ch = H / 2;
     if (blockIdx.z == 0) {
-        if (px >= W || py >= H) return;
+        if (px >= DW || py >= DH) return;
         const int pos = (py >> 4) * mbw + (px >> 4);
-        const int a = luma_sample(past, W, H, px, py, fmv[pos], mode == 2);
-        const int z = luma_sample(fut, W, H, px, py, bmv[pos], mode == 2);
-        dy[static_cast<size_t>(py) * W + px] = static_cast<uint8_t>((a + z + 1) >> 1);
+        const int a = luma_sample(past, PW, PH, px, py, fmv[pos], mode == 2);
+        const int z = luma_sample(fut, FW, FH, px, py, bmv[pos], mode == 2);
+        dy[static_cast<size_t>(py) * DW + px] = static_cast<uint8_t>((a + z + 1) >> 1);
         return;
     }
-    if (px >= cw || py >= ch) return;
+    const int dcw = DW / 2, dch = DH / 2;
+    if (px >= dcw || py >= dch) return;
     const int pos = (py >> 3) * mbw + (px >> 3);
-    const size_t ys = static_cast<size_t>(W) * H, cs = static_cast<size_t>(cw) * ch;
-    const uint8_t* pp = past + ys + (blockIdx.z == 2 ? cs : 0);
-    const uint8_t* fp = fut + ys + (blockIdx.z == 2 ? cs : 0);
+    const int pcw = PW / 2, pch = PH / 2, fcw = FW / 2, fch = FH / 2;
+    const uint8_t* pp = past + static_cast<size_t>(PW) * PH + (blockIdx.z == 2 ? static_cast<size_t>(pcw) * pch : 0);
+    const uint8_t* fp = fut + static_cast<size_t>(FW) * FH + (blockIdx.z == 2 ? static_cast<size_t>(fcw) * fch : 0);
     uint8_t* out = blockIdx.z == 2 ? dv : du;
-    const int a = chroma_sample(pp, cw, ch, px, py, fmv[pos], false);
-    const int b = chroma_sample(fp, cw, ch, px, py, bmv[pos], true);
-    out[static_cast<size_t>(py) * cw + px] = static_cast<uint8_t>((a + b + 1) >> 1);
+    const int a = chroma_sample(pp, pcw, pch, px, py, fmv[pos], false);
+    const int b = chroma_sample(fp, fcw, fch, px, py, bmv[pos], true);
+    out[static_cast<size_t>(py) * dcw + px] = static_cast<uint8_t>((a + b + 1) >> 1);
 }
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
 
-hipError_t launch_predict_bi(const uint8_t* past, const uint8_t* fut, int W, int H, int mode, const int2* fmv,
-                             const int2* bmv, uint8_t* dy, uint8_t* du, uint8_t* dv, hipStream_t s) {
-    if (W <= 0 || H <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_predict_bi, dim3((W + 255) / 256, H, 3), dim3(256), 0, s, past, fut, W, H, mode, fmv, bmv, dy,
-                       du, dv);
+hipError_t launch_predict_bi(const uint8_t* past, int PW, int PH, const uint8_t* fut, int FW, int FH, int DW, int DH,
+                             int mode, const int2* fmv, const int2* bmv, uint8_t* dy, uint8_t* du, uint8_t* dv,
+                             hipStream_t s) {
+    if (DW <= 0 || DH <= 0 || PW <= 0 || PH <= 0 || FW <= 0 || FH <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_predict_bi, dim3((DW + 255) / 256, DH, 3), dim3(256), 0, s, past, PW, PH, fut, FW, FH, DW, DH,
+                       (DW + 15) / 16, mode, fmv, bmv, dy, du, dv);
     return hipGetLastError();
 }
 
-hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t* rv, int W, int H, int mode,
-                            const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du, uint8_t* dv,
-                            hipStream_t s) {
-    if (W <= 0 || H <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_predict_luma, dim3((W + 255) / 256, H), dim3(256), 0, s, ry, W, H, mode, mvs, bmv, four, dy);
-    if (W / 2 > 0 && H / 2 > 0)
-        hipLaunchKernelGGL(k_predict_chroma, dim3((W / 2 + 255) / 256, H / 2), dim3(256), 0, s, ru, rv, W, H, mvs, bmv,
-                           four, du, dv);
+hipError_t launch_predict_p(const uint8_t* ry, const uint8_t* ru, const uint8_t* rv, int RW, int RH, int DW, int DH,
+                            int mode, const int2* mvs, const int2* bmv, const uint8_t* four, uint8_t* dy, uint8_t* du,
+                            uint8_t* dv, hipStream_t s) {
+    if (DW <= 0 || DH <= 0 || RW <= 0 || RH <= 0) return hipSuccess;
+    const int mbw = (DW + 15) / 16;
+    hipLaunchKernelGGL(k_predict_luma, dim3((DW + 255) / 256, DH), dim3(256), 0, s, ry, RW, RH, DW, DH, mbw, mode, mvs,
+                       bmv, four, dy);
+    if (DW / 2 > 0 && DH / 2 > 0 && RW / 2 > 0 && RH / 2 > 0)
+        hipLaunchKernelGGL(k_predict_chroma, dim3((DW / 2 + 255) / 256, DH / 2), dim3(256), 0, s, ru, rv, RW / 2,
+                           RH / 2, DW / 2, DH / 2, mbw, mvs, bmv, four, du, dv);
     return hipGetLastError();
 }
 

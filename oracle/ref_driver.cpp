@@ -21,6 +21,9 @@
 //   has_good_motion_match    src/encoder_motion.cpp:678-751
 //   motion_compensate_mb / _4mv  src/encoder_motion_compensation.cpp:474-571
 //   motion_compensate_bi_mb  src/encoder_motion_compensation.cpp:659-711
+//   motion_compensate_mb_padded / _4mv_padded  src/encoder_motion_compensation.cpp:573-657
+//   motion_compensate_bi_mb_padded  src/encoder_motion_compensation.cpp:712-763
+//   empty_padded_frame       src/encoder_encode.cpp:1949-1956
 #include "encoder_internal.h"
 
 #include "../include/vc1me.h"
@@ -435,6 +438,67 @@ int ref_predict_bi_frame(const vc1me_params* p, const vc1me_planes* past, const 
                 enc.motion_compensate_bi_mb(dst, pa, fu, mx, my, {fmvs[2 * pos], fmvs[2 * pos + 1]},
                                             {bmvs[2 * pos], bmvs[2 * pos + 1]},
                                             static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode));
+            }
+        std::memcpy(y, dst.y.data(), dst.y.size());
+        std::memcpy(u, dst.u.data(), dst.u.size());
+        std::memcpy(v, dst.v.data(), dst.v.size());
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+/* The loop-reconstruction prediction of a P picture: every macroblock of
+ * p's raster through motion_compensate_4mv_padded or
+ * motion_compensate_mb_padded into the encoder's own empty padded frame
+ * (mbw*16 x mbh*16), from `ref` at whatever raster it has. */
+int ref_predict_p_frame_padded(const vc1me_params* p, const vc1me_planes* ref, const int32_t* mvs,
+                               const int32_t* block_mvs, const uint8_t* use_4mv, uint8_t* y, uint8_t* u, uint8_t* v,
+                               int simd_auto) {
+    try {
+        Vc1Encoder enc(make_config(p, simd_auto != 0));
+        const Frame r = make_frame(ref);
+        const int mbw = (p->width + 15) / 16, mbh = (p->height + 15) / 16;
+        Frame dst = enc.empty_padded_frame(mbw, mbh);
+        for (int my = 0; my < mbh; ++my)
+            for (int mx = 0; mx < mbw; ++mx) {
+                const size_t pos = static_cast<size_t>(my) * mbw + mx;
+                if (use_4mv[pos]) {
+                    std::array<Vc1Encoder::MotionVector, 4> b;
+                    for (int k = 0; k < 4; ++k) b[k] = {block_mvs[8 * pos + 2 * k], block_mvs[8 * pos + 2 * k + 1]};
+                    enc.motion_compensate_4mv_padded(dst, r, mx, my, b);
+                } else {
+                    enc.motion_compensate_mb_padded(dst, r, mx, my, {mvs[2 * pos], mvs[2 * pos + 1]},
+                                                    static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode));
+                }
+            }
+        std::memcpy(y, dst.y.data(), dst.y.size());
+        std::memcpy(u, dst.u.data(), dst.u.size());
+        std::memcpy(v, dst.v.data(), dst.v.size());
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+/* The loop-reconstruction bi-directional prediction: every macroblock
+ * through motion_compensate_bi_mb_padded into an empty padded frame. */
+int ref_predict_bi_frame_padded(const vc1me_params* p, const vc1me_planes* past, const vc1me_planes* future,
+                                const int32_t* fmvs, const int32_t* bmvs, uint8_t* y, uint8_t* u, uint8_t* v,
+                                int simd_auto) {
+    try {
+        Vc1Encoder enc(make_config(p, simd_auto != 0));
+        const Frame pa = make_frame(past), fu = make_frame(future);
+        const int mbw = (p->width + 15) / 16, mbh = (p->height + 15) / 16;
+        Frame dst = enc.empty_padded_frame(mbw, mbh);
+        for (int my = 0; my < mbh; ++my)
+            for (int mx = 0; mx < mbw; ++mx) {
+                const size_t pos = static_cast<size_t>(my) * mbw + mx;
+                enc.motion_compensate_bi_mb_padded(dst, pa, fu, mx, my, {fmvs[2 * pos], fmvs[2 * pos + 1]},
+                                                   {bmvs[2 * pos], bmvs[2 * pos + 1]},
+                                                   static_cast<Vc1Encoder::ProgressiveMvMode>(p->mode));
             }
         std::memcpy(y, dst.y.data(), dst.y.size());
         std::memcpy(u, dst.u.data(), dst.u.size());

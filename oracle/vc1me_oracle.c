@@ -2452,3 +2452,73 @@ void oracle_predict_bi_frame(const vc1me_planes* past, const vc1me_planes* futur
                               chroma_mc_sample_dir(future->v, cw, ch, px, py, bx, by, true) + 1) >> 1);
         }
 }
+
+/* The padded P prediction picture: motion_compensate_mb_padded and
+ * motion_compensate_4mv_padded (src/encoder_motion_compensation.cpp:573-657)
+ * for every macroblock of an mbw x mbh raster.  Unlike the picture above,
+ * every sample of each 16x16 luma and 8x8 chroma macroblock is written, into
+ * a picture of mbw*16 x mbh*16 (chroma mbw*8 x mbh*8), and the samples clamp
+ * to the reference's own raster, which may itself be the padded one. */
+void oracle_predict_p_frame_padded(const vc1me_planes* ref, int mbw, int mbh, int mode, const int32_t* mvs,
+                                   const int32_t* block_mvs, const uint8_t* use_4mv, uint8_t* y, uint8_t* u,
+                                   uint8_t* v) {
+    const int RW = ref->width, RH = ref->height, rcw = RW / 2, rch = RH / 2;
+    const int DW = mbw * 16, DH = mbh * 16, dcw = DW / 2, dch = DH / 2;
+    for (int py = 0; py < DH; ++py)
+        for (int px = 0; px < DW; ++px) {
+            const size_t pos = (size_t)(py / 16) * mbw + px / 16;
+            int mvx, mvy, m = mode;
+            if (use_4mv[pos]) {
+                /* the four 8x8 blocks are bicubic whatever the mode (:573-595) */
+                const int k = ((py / 8) & 1) * 2 + ((px / 8) & 1);
+                mvx = block_mvs[8 * pos + 2 * k];
+                mvy = block_mvs[8 * pos + 2 * k + 1];
+                m = VC1ME_MODE_ONE_MV_QPEL;
+            } else {
+                mvx = mvs[2 * pos];
+                mvy = mvs[2 * pos + 1];
+            }
+            y[(size_t)py * DW + px] = (uint8_t)luma_mc_sample_mode(ref->y, RW, RH, px, py, mvx, mvy, m, 0);
+        }
+    for (int py = 0; py < dch; ++py)
+        for (int px = 0; px < dcw; ++px) {
+            const size_t pos = (size_t)(py / 8) * mbw + px / 8;
+            int mvx, mvy;
+            if (use_4mv[pos]) {
+                const int32_t* b = block_mvs + 8 * pos;
+                mvx = median4(b[0], b[2], b[4], b[6]);
+                mvy = median4(b[1], b[3], b[5], b[7]);
+            } else {
+                mvx = mvs[2 * pos];
+                mvy = mvs[2 * pos + 1];
+            }
+            u[(size_t)py * dcw + px] = (uint8_t)chroma_mc_sample(ref->u, rcw, rch, px, py, mvx, mvy);
+            v[(size_t)py * dcw + px] = (uint8_t)chroma_mc_sample(ref->v, rcw, rch, px, py, mvx, mvy);
+        }
+}
+
+/* The padded bi-directional picture: motion_compensate_bi_mb_padded
+ * (src/encoder_motion_compensation.cpp:712-763) for every macroblock of an
+ * mbw x mbh raster, each reference sampled with clamping to its own raster. */
+void oracle_predict_bi_frame_padded(const vc1me_planes* past, const vc1me_planes* future, int mbw, int mbh, int mode,
+                                    const int32_t* fmvs, const int32_t* bmvs, uint8_t* y, uint8_t* u, uint8_t* v) {
+    const int PW = past->width, PH = past->height, FW = future->width, FH = future->height;
+    const int DW = mbw * 16, DH = mbh * 16, dcw = DW / 2, dch = DH / 2;
+    for (int py = 0; py < DH; ++py)
+        for (int px = 0; px < DW; ++px) {
+            const size_t pos = (size_t)(py / 16) * mbw + px / 16;
+            const int a = luma_mc_sample_mode(past->y, PW, PH, px, py, fmvs[2 * pos], fmvs[2 * pos + 1], mode, 0);
+            const int z = luma_mc_sample_mode(future->y, FW, FH, px, py, bmvs[2 * pos], bmvs[2 * pos + 1], mode, 0);
+            y[(size_t)py * DW + px] = (uint8_t)((a + z + 1) >> 1);
+        }
+    for (int py = 0; py < dch; ++py)
+        for (int px = 0; px < dcw; ++px) {
+            const size_t pos = (size_t)(py / 8) * mbw + px / 8;
+            const int fx = fmvs[2 * pos], fy = fmvs[2 * pos + 1], bx = bmvs[2 * pos], by = bmvs[2 * pos + 1];
+            const size_t o = (size_t)py * dcw + px;
+            u[o] = (uint8_t)((chroma_mc_sample_dir(past->u, PW / 2, PH / 2, px, py, fx, fy, false) +
+                              chroma_mc_sample_dir(future->u, FW / 2, FH / 2, px, py, bx, by, true) + 1) >> 1);
+            v[o] = (uint8_t)((chroma_mc_sample_dir(past->v, PW / 2, PH / 2, px, py, fx, fy, false) +
+                              chroma_mc_sample_dir(future->v, FW / 2, FH / 2, px, py, bx, by, true) + 1) >> 1);
+        }
+}

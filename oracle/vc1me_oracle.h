@@ -129,6 +129,18 @@ void oracle_predict_p_frame(const vc1me_planes* ref, int mode, const int32_t* mv
 void oracle_predict_bi_frame(const vc1me_planes* past, const vc1me_planes* future, int mode, const int32_t* fmvs,
                              const int32_t* bmvs, uint8_t* y, uint8_t* u, uint8_t* v);
 
+/* the padded P prediction picture (motion_compensate_mb_padded /
+ * motion_compensate_4mv_padded, src/encoder_motion_compensation.cpp:573-657):
+ * an mbw*16 x mbh*16 picture, samples clamped to ref's own raster */
+void oracle_predict_p_frame_padded(const vc1me_planes* ref, int mbw, int mbh, int mode, const int32_t* mvs,
+                                   const int32_t* block_mvs, const uint8_t* use_4mv, uint8_t* y, uint8_t* u,
+                                   uint8_t* v);
+
+/* the padded bi-directional picture (motion_compensate_bi_mb_padded,
+ * src/encoder_motion_compensation.cpp:712-763) */
+void oracle_predict_bi_frame_padded(const vc1me_planes* past, const vc1me_planes* future, int mbw, int mbh, int mode,
+                                    const int32_t* fmvs, const int32_t* bmvs, uint8_t* y, uint8_t* u, uint8_t* v);
+
 #ifdef __cplusplus
 }
 #endif

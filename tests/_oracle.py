@@ -75,6 +75,11 @@ def oracle() -> C.CDLL:
         lib.oracle_predict_p_frame.restype = None
         lib.oracle_predict_bi_frame.argtypes = [P(abi.Planes), P(abi.Planes), C.c_int] + [C.c_void_p] * 5
         lib.oracle_predict_bi_frame.restype = None
+        lib.oracle_predict_p_frame_padded.argtypes = [P(abi.Planes), C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
+        lib.oracle_predict_p_frame_padded.restype = None
+        lib.oracle_predict_bi_frame_padded.argtypes = [P(abi.Planes), P(abi.Planes), C.c_int, C.c_int, C.c_int] + \
+            [C.c_void_p] * 5
+        lib.oracle_predict_bi_frame_padded.restype = None
         lib.oracle_analyze_p_frame_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                       P(abi.PResult), P(OracleOps)]
         _oracle = lib
@@ -116,6 +121,9 @@ def reference() -> C.CDLL:
             [C.c_uint64, P(C.c_int32), P(C.c_int32), P(C.c_uint64)]
         lib.ref_predict_p_frame.argtypes = [P(abi.Params), P(abi.Planes)] + [C.c_void_p] * 6 + [C.c_int]
         lib.ref_predict_bi_frame.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + [C.c_void_p] * 5 + [C.c_int]
+        lib.ref_predict_p_frame_padded.argtypes = [P(abi.Params), P(abi.Planes)] + [C.c_void_p] * 6 + [C.c_int]
+        lib.ref_predict_bi_frame_padded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes)] + \
+            [C.c_void_p] * 5 + [C.c_int]
         lib.ref_analyze_p_core_seeded.argtypes = [P(abi.Params), P(abi.Planes), P(abi.Planes), P(abi.PResult),
                                                   P(abi.PResult), C.c_int]
         _ref = lib
@@ -288,6 +296,65 @@ def reference_predict_bi_frame(params: abi.Params, past, future, fmvs, bmvs, sim
     b = np.ascontiguousarray(bmvs, np.int32)
     rc = reference().ref_predict_bi_frame(C.byref(params), C.byref(pa.struct), C.byref(fu.struct), f.ctypes.data,
                                           b.ctypes.data, y.ctypes.data, u.ctypes.data, v.ctypes.data, simd_auto)
+    assert rc == 0, reference().ref_last_error()
+    return y, u, v
+
+
+def _padded_planes(width: int, height: int):
+    """Empty y, u, v planes of the macroblock raster of a width x height
+    picture (mbw*16 x mbh*16)."""
+    mbw, mbh = abi.mb_grid(width, height)
+    return (np.empty((mbh * 16, mbw * 16), np.uint8), np.empty((mbh * 8, mbw * 8), np.uint8),
+            np.empty((mbh * 8, mbw * 8), np.uint8))
+
+
+def oracle_predict_p_frame_padded(ref, width: int, height: int, mode: int, result: abi.ResultArrays):
+    """The padded P prediction picture (y, u, v) of a width x height
+    analysis, the reference at its own raster."""
+    r = abi.PlaneSet.of(ref)
+    y, u, v = _padded_planes(width, height)
+    mbw, mbh = abi.mb_grid(width, height)
+    mvs = np.ascontiguousarray(result.mvs, np.int32)
+    bmv = np.ascontiguousarray(result.block_mvs, np.int32)
+    four = np.ascontiguousarray(result.use_4mv, np.uint8)
+    oracle().oracle_predict_p_frame_padded(C.byref(r.struct), mbw, mbh, mode, mvs.ctypes.data, bmv.ctypes.data,
+                                           four.ctypes.data, y.ctypes.data, u.ctypes.data, v.ctypes.data)
+    return y, u, v
+
+
+def reference_predict_p_frame_padded(params: abi.Params, ref, result: abi.ResultArrays, simd_auto: int = 0):
+    r = abi.PlaneSet.of(ref)
+    y, u, v = _padded_planes(params.width, params.height)
+    mvs = np.ascontiguousarray(result.mvs, np.int32)
+    bmv = np.ascontiguousarray(result.block_mvs, np.int32)
+    four = np.ascontiguousarray(result.use_4mv, np.uint8)
+    rc = reference().ref_predict_p_frame_padded(C.byref(params), C.byref(r.struct), mvs.ctypes.data, bmv.ctypes.data,
+                                                four.ctypes.data, y.ctypes.data, u.ctypes.data, v.ctypes.data,
+                                                simd_auto)
+    assert rc == 0, reference().ref_last_error()
+    return y, u, v
+
+
+def oracle_predict_bi_frame_padded(past, future, width: int, height: int, mode: int, fmvs, bmvs):
+    """The padded bi-directional prediction picture (y, u, v)."""
+    pa, fu = abi.PlaneSet.of(past), abi.PlaneSet.of(future)
+    y, u, v = _padded_planes(width, height)
+    mbw, mbh = abi.mb_grid(width, height)
+    f = np.ascontiguousarray(fmvs, np.int32)
+    b = np.ascontiguousarray(bmvs, np.int32)
+    oracle().oracle_predict_bi_frame_padded(C.byref(pa.struct), C.byref(fu.struct), mbw, mbh, mode, f.ctypes.data,
+                                            b.ctypes.data, y.ctypes.data, u.ctypes.data, v.ctypes.data)
+    return y, u, v
+
+
+def reference_predict_bi_frame_padded(params: abi.Params, past, future, fmvs, bmvs, simd_auto: int = 0):
+    pa, fu = abi.PlaneSet.of(past), abi.PlaneSet.of(future)
+    y, u, v = _padded_planes(params.width, params.height)
+    f = np.ascontiguousarray(fmvs, np.int32)
+    b = np.ascontiguousarray(bmvs, np.int32)
+    rc = reference().ref_predict_bi_frame_padded(C.byref(params), C.byref(pa.struct), C.byref(fu.struct),
+                                                 f.ctypes.data, b.ctypes.data, y.ctypes.data, u.ctypes.data,
+                                                 v.ctypes.data, simd_auto)
     assert rc == 0, reference().ref_last_error()
     return y, u, v
 

@@ -46,3 +46,53 @@ def bi_inputs(name, w, h, mode):
         f &= ~1
         b &= ~1
     return d[0], d[3], f, b
+
+
+# padded (loop-reconstruction) pictures: (name, width, height, mode, rasters
+# of the references: True = the padded macroblock raster, False = the coded one)
+PADDED_CASES = [("pad_qpel_coded", 202, 118, abi.MODE_ONE_MV_QPEL, False),
+                ("pad_hpel_padded", 240, 136, abi.MODE_ONE_MV_HPEL, True),
+                ("pad_bilinear_padded", 202, 118, abi.MODE_ONE_MV_HPEL_BILINEAR, True),
+                ("pad_mixed_coded", 240, 136, abi.MODE_MIXED_MV, False),
+                ("pad_mixed_padded", 202, 118, abi.MODE_MIXED_MV, True)]
+BI_PADDED_CASES = [("bipad_qpel_padded", 202, 118, abi.MODE_ONE_MV_QPEL, (True, True)),
+                   ("bipad_hpel_coded", 240, 136, abi.MODE_ONE_MV_HPEL, (False, False)),
+                   ("bipad_bilinear_mixed_rasters", 202, 118, abi.MODE_ONE_MV_HPEL_BILINEAR, (True, False))]
+
+
+def _ref_picture(seed, w, h, padded):
+    mbw, mbh = abi.mb_grid(w, h)
+    rw, rh = (mbw * 16, mbh * 16) if padded else (w, h)
+    return synth.pan_rectangles(seed, 1, width=rw, height=rh, frame_indices=[0])[0]
+
+
+def padded_inputs(name, w, h, mode, padded):
+    """(ref, analysis result with a random field) of a padded P case: vectors
+    up to +-75 px (even for the half-pel modes), a third of the macroblocks
+    4-MV in Mixed-MV mode."""
+    seed = sum(map(ord, name)) % 1000
+    ref = _ref_picture(seed, w, h, padded)
+    rng = np.random.default_rng(seed)
+    res = abi.ResultArrays(w, h)
+    res.mvs[...] = rng.integers(-300, 301, size=res.mvs.shape)
+    res.block_mvs[...] = rng.integers(-300, 301, size=res.block_mvs.shape)
+    if mode in (abi.MODE_ONE_MV_HPEL, abi.MODE_ONE_MV_HPEL_BILINEAR):
+        res.mvs &= ~1
+    if mode == abi.MODE_MIXED_MV:
+        res.use_4mv[::3] = 1
+    return ref, res
+
+
+def bi_padded_inputs(name, w, h, mode, padded):
+    """(past, future, forward field, backward field) of a padded bi case."""
+    seed = sum(map(ord, name)) % 1000
+    past = _ref_picture(seed, w, h, padded[0])
+    fut = _ref_picture(seed + 1, w, h, padded[1])
+    mbw, mbh = abi.mb_grid(w, h)
+    rng = np.random.default_rng(seed + mode)
+    f = rng.integers(-200, 200, size=(mbw * mbh, 2)).astype(np.int32)
+    b = rng.integers(-200, 200, size=(mbw * mbh, 2)).astype(np.int32)
+    if mode != abi.MODE_ONE_MV_QPEL:
+        f &= ~1
+        b &= ~1
+    return past, fut, f, b

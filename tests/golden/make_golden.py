@@ -21,6 +21,7 @@ The fixtures are data only.
     python tests/golden/make_golden.py motion       # motion_check.json only (has_good_motion_match,
                                                     # refine_qpel_block)
     python tests/golden/make_golden.py predict      # predict.json only (P prediction pictures)
+    python tests/golden/make_golden.py predict_padded   # predict_padded.json only (padded pictures)
     python tests/golden/make_golden.py configs      # config_digests.json only (benchmark configurations'
                                                     # pairs at full size, tests/_config_cases.py)
 """
@@ -344,6 +345,33 @@ def predict_bi_fixtures() -> list:
     return out
 
 
+def predict_padded_fixtures() -> list:
+    """Digests of the reference's padded (loop-reconstruction) prediction
+    pictures, motion_compensate_mb_padded / _4mv_padded and
+    motion_compensate_bi_mb_padded, its scalar and SIMD paths agreeing."""
+    from _predict_cases import BI_PADDED_CASES, PADDED_CASES, bi_padded_inputs, padded_inputs
+    from _oracle import reference_predict_bi_frame_padded, reference_predict_p_frame_padded
+    out = []
+    for name, w, h, mode, padded in PADDED_CASES:
+        ref, res = padded_inputs(name, w, h, mode, padded)
+        p = abi.default_params(w, h, mode=mode)
+        shas = {hashlib.sha256(b"".join(x.tobytes() for x in reference_predict_p_frame_padded(p, ref, res, simd)))
+                .hexdigest() for simd in (0, 1)}
+        assert len(shas) == 1, f"{name}: scalar and SIMD paths differ"
+        out.append({"case": name, "kind": "p", "width": w, "height": h, "mode": mode, "sha256": shas.pop()})
+        print(f"predict padded {name}: ref {ref.width}x{ref.height}")
+    for name, w, h, mode, padded in BI_PADDED_CASES:
+        past, fut, f, b = bi_padded_inputs(name, w, h, mode, padded)
+        p = abi.default_params(w, h, mode=mode)
+        shas = {hashlib.sha256(b"".join(x.tobytes() for x in reference_predict_bi_frame_padded(p, past, fut, f, b,
+                                                                                                 simd)))
+                .hexdigest() for simd in (0, 1)}
+        assert len(shas) == 1, f"{name}: scalar and SIMD paths differ"
+        out.append({"case": name, "kind": "bi", "width": w, "height": h, "mode": mode, "sha256": shas.pop()})
+        print(f"predict padded {name}: past {past.width}x{past.height}, future {fut.width}x{fut.height}")
+    return out
+
+
 def _config_case(job):
     """(key, sha256, scalars) of one benchmark-configuration pair analysed by
     the compiled reference (run in a worker process)."""
@@ -393,6 +421,12 @@ def main() -> int:
             json.dump(predict_fixtures(), f)
         with open(os.path.join(HERE, "predict_bi.json"), "w") as f:
             json.dump(predict_bi_fixtures(), f, indent=1)
+        with open(os.path.join(HERE, "predict_padded.json"), "w") as f:
+            json.dump(predict_padded_fixtures(), f, indent=1)
+        return 0
+    if sys.argv[1:] == ["predict_padded"]:
+        with open(os.path.join(HERE, "predict_padded.json"), "w") as f:
+            json.dump(predict_padded_fixtures(), f, indent=1)
         return 0
     if sys.argv[1:] == ["motion"]:
         with open(os.path.join(HERE, "motion_check.json"), "w") as f:
@@ -450,6 +484,8 @@ def main() -> int:
         json.dump(predict_fixtures(), f)
     with open(os.path.join(HERE, "predict_bi.json"), "w") as f:
         json.dump(predict_bi_fixtures(), f, indent=1)
+    with open(os.path.join(HERE, "predict_padded.json"), "w") as f:
+        json.dump(predict_padded_fixtures(), f, indent=1)
     return 0
 
 

@@ -99,3 +99,84 @@ def test_bi_prediction_closed_form():
     res.mvs[:] = (8, -4)
     assert digest(O.oracle_predict_bi_frame(a, a, abi.MODE_ONE_MV_QPEL, w, w)) == \
         digest(O.oracle_predict_p_frame(a, abi.MODE_ONE_MV_QPEL, res))
+
+
+# ---------------------------------------------------------------- padded pictures
+
+from _predict_cases import BI_PADDED_CASES, PADDED_CASES, bi_padded_inputs, padded_inputs  # noqa: E402
+
+PAD_FIX = json.load(open(os.path.join(GOLDEN, "predict_padded.json")))
+P_CASES = {c[0]: c for c in PADDED_CASES}
+B_CASES = {c[0]: c for c in BI_PADDED_CASES}
+
+
+@pytest.mark.parametrize("k", range(len(PAD_FIX)), ids=lambda k: PAD_FIX[k]["case"])
+def test_oracle_padded_prediction_matches_fixture(k):
+    f = PAD_FIX[k]
+    if f["kind"] == "p":
+        name, w, h, mode, padded = P_CASES[f["case"]]
+        ref, res = padded_inputs(name, w, h, mode, padded)
+        got = O.oracle_predict_p_frame_padded(ref, w, h, mode, res)
+    else:
+        name, w, h, mode, padded = B_CASES[f["case"]]
+        past, fut, fm, bm = bi_padded_inputs(name, w, h, mode, padded)
+        got = O.oracle_predict_bi_frame_padded(past, fut, w, h, mode, fm, bm)
+    assert digest(got) == f["sha256"]
+
+
+def test_padded_prediction_closed_forms():
+    """On the coded raster the padded picture is the plain one (both clamp to
+    the reference's raster); zero vectors copy a reference that has the
+    padded raster and replicate the edges of one that has the coded raster."""
+    rng = np.random.default_rng(12)
+    w, h = 58, 38                                           # 4 x 3 macroblocks: padded 64 x 48
+    ref = synth.Picture(rng.integers(0, 256, size=(h, w)).astype(np.uint8),
+                        rng.integers(0, 256, size=(h // 2, w // 2)).astype(np.uint8),
+                        rng.integers(0, 256, size=(h // 2, w // 2)).astype(np.uint8))
+    res = abi.ResultArrays(w, h)
+    res.mvs[:] = rng.integers(-90, 90, size=res.mvs.shape)
+    res.block_mvs[:] = rng.integers(-90, 90, size=res.block_mvs.shape)
+    res.use_4mv[::2] = 1
+    py, pu, pv = O.oracle_predict_p_frame_padded(ref, w, h, abi.MODE_MIXED_MV, res)
+    assert py.shape == (48, 64) and pu.shape == (24, 32)
+    y, u, v = O.oracle_predict_p_frame(ref, abi.MODE_MIXED_MV, res)
+    assert np.array_equal(py[:h, :w], y) and np.array_equal(pu[:h // 2, :w // 2], u)
+    assert np.array_equal(pv[:h // 2, :w // 2], v)
+    zero = abi.ResultArrays(w, h)
+    zy, zu, _ = O.oracle_predict_p_frame_padded(ref, w, h, abi.MODE_ONE_MV_QPEL, zero)
+    assert np.array_equal(zy, ref.y[np.ix_(np.clip(np.arange(48), 0, h - 1), np.clip(np.arange(64), 0, w - 1))])
+    assert np.array_equal(zu, ref.u[np.ix_(np.clip(np.arange(24), 0, h // 2 - 1), np.clip(np.arange(32), 0, w // 2 - 1))])
+    big = synth.Picture(zy, zu, zu.copy())                   # a reference at the padded raster
+    again = O.oracle_predict_p_frame_padded(big, w, h, abi.MODE_ONE_MV_QPEL, zero)
+    assert np.array_equal(again[0], zy) and np.array_equal(again[1], zu)
+    fm = rng.integers(-90, 90, size=(12, 2)).astype(np.int32)
+    bm = rng.integers(-90, 90, size=(12, 2)).astype(np.int32)
+    other = synth.Picture(*(rng.integers(0, 256, size=x.shape).astype(np.uint8) for x in (ref.y, ref.u, ref.v)))
+    by, bu, _ = O.oracle_predict_bi_frame_padded(ref, other, w, h, abi.MODE_ONE_MV_QPEL, fm, bm)
+    ty, tu, _ = O.oracle_predict_bi_frame(ref, other, abi.MODE_ONE_MV_QPEL, fm, bm)
+    assert np.array_equal(by[:h, :w], ty) and np.array_equal(bu[:h // 2, :w // 2], tu)
+
+
+@needs_ref
+def test_oracle_padded_prediction_matches_reference_wild_fields():
+    """Vectors far outside the picture, references at the coded and at the
+    padded raster, every mode."""
+    rng = np.random.default_rng(13)
+    w, h = 170, 98
+    mbw, mbh = abi.mb_grid(w, h)
+    coded = synth.pan_rectangles(41, 1, width=w, height=h, frame_indices=[0])[0]
+    padded = synth.pan_rectangles(42, 1, width=mbw * 16, height=mbh * 16, frame_indices=[0])[0]
+    for mode in (0, 1, 2, 3):
+        res = abi.ResultArrays(w, h)
+        res.mvs[:] = rng.integers(-900, 900, size=res.mvs.shape)
+        res.block_mvs[:] = rng.integers(-900, 900, size=res.block_mvs.shape)
+        res.use_4mv[:] = rng.integers(0, 2, size=res.use_4mv.shape) if mode == 3 else 0
+        p = abi.default_params(w, h, mode=mode)
+        for ref in (coded, padded):
+            want = O.reference_predict_p_frame_padded(p, ref, res, 1)
+            assert digest(O.oracle_predict_p_frame_padded(ref, w, h, mode, res)) == digest(want), (mode, ref.width)
+        if mode < 3:
+            fm = rng.integers(-900, 900, size=(mbw * mbh, 2)).astype(np.int32)
+            bm = rng.integers(-900, 900, size=(mbw * mbh, 2)).astype(np.int32)
+            want = O.reference_predict_bi_frame_padded(p, padded, coded, fm, bm, 1)
+            assert digest(O.oracle_predict_bi_frame_padded(padded, coded, w, h, mode, fm, bm)) == digest(want), mode
