@@ -434,10 +434,13 @@ int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
  * like the reference's SpeedProfiler, src/encoder_internal.h:182-221). */
 int vc1me_set_profiling(vc1me_ctx* ctx, int on);
 /* Run each plain analysis step as one HIP graph: the step's whole launch
- * sequence (phase planes, global search, signature index, long-range
- * pre-pass, one launch per anti-diagonal) is captured the first time a launch
- * shape (job count, rasters, ranges, modes) is seen and replayed afterwards;
- * the job table is uploaded before each replay.  Results are identical to
+ * sequence (counter clearing, phase planes, global search, signature index,
+ * then either the relaxation's guess pass and its device-driven passes or
+ * the long-range pre-pass and one launch per anti-diagonal, then the step's
+ * summary and packed results) is captured the first time a launch shape
+ * (job count, rasters, ranges, modes, results wanted, relaxation passes) is
+ * seen and replayed afterwards; the job table is uploaded before each replay
+ * and the results come back in one copy after it.  Results are identical to
  * launching one by one.  Not used while profiling is on or for the learned
  * forms.  No reference counterpart (replaces the per-batch dispatches of
  * src/vulkan_compute.cpp:370-490). */
@@ -453,11 +456,14 @@ int vc1me_set_graphs(vc1me_ctx* ctx, int on);
  *     neighbour state (the global vector), then passes that re-analyse only
  *     the macroblocks whose neighbours changed, until a pass changes
  *     nothing; the dependency graph is acyclic, so that fixed point is the
- *     raster-order result.  After 96 passes without settling the step
- *     finishes with the diagonal sweep.
- * The relaxation schedule is not captured into HIP graphs: with graphs on
- * (vc1me_set_graphs) a plain step uses the diagonal sweep.  No reference
- * counterpart (the reference runs its macroblock loop on one thread). */
+ *     raster-order result.  The passes run from device-side worklists
+ *     without host reads: a step launches as many passes as its launch
+ *     shape has needed before (plus two; 24 for a new shape) and launches
+ *     more only if the last one still changed something.  After 96 passes
+ *     without settling the step finishes with the diagonal sweep.
+ * Both schedules are captured into HIP graphs with vc1me_set_graphs.  No
+ * reference counterpart (the reference runs its macroblock loop on one
+ * thread). */
 enum { VC1ME_SCHEDULE_DIAGONAL = 0, VC1ME_SCHEDULE_RELAX = 1 };
 int vc1me_set_schedule(vc1me_ctx* ctx, int schedule);
 /* Per-pass record of the last step's relaxation: macroblock analyses of each

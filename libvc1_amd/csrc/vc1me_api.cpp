@@ -17,6 +17,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 using vc1me::GlobalState;
@@ -69,6 +70,15 @@ bool injected_failure() {
 int relax_pass_limit() {
     const char* e = std::getenv("VC1ME_RELAX_MAX_PASSES");
     const int v = e && *e ? std::atoi(e) : vc1me::kMaxRelaxPasses;
+    return std::max(1, std::min(v, vc1me::kMaxRelaxPasses));
+}
+
+// Relaxation passes a launch shape starts with before its need is known
+// (VC1ME_RELAX_FIRST_PASSES lowers or raises it; read at every step), so
+// that the continuation of an unsettled step can be exercised.
+int relax_first_passes() {
+    const char* e = std::getenv("VC1ME_RELAX_FIRST_PASSES");
+    const int v = e && *e ? std::atoi(e) : 24;
     return std::max(1, std::min(v, vc1me::kMaxRelaxPasses));
 }
 
@@ -151,9 +161,7 @@ struct JobSlot {
     uint64_t* cost20 = nullptr;     // cost report
     int2* one_mv = nullptr;
     uint64_t* mbops = nullptr;      // per-macroblock op counts of the last analysis
-    int32_t* wl0 = nullptr;         // relaxation worklists and queue stamps
-    int32_t* wl1 = nullptr;
-    uint32_t* mark = nullptr;
+    uint32_t* mark = nullptr;       // relaxation queue stamps
     int2* seed_mvs = nullptr;       // seeded core: the seed's 1-MV vectors
     int mbs_last = 0;               // macroblocks of the last step's job
 };
@@ -189,8 +197,6 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     const size_t o_c20 = carve(mbs * 8);
     const size_t o_one = carve(mbs * sizeof(int2));
     const size_t o_mbops = carve(mbs * 5 * sizeof(uint64_t));
-    const size_t o_wl0 = carve(mbs * sizeof(int32_t));
-    const size_t o_wl1 = carve(mbs * sizeof(int32_t));
     const size_t o_mark = carve(mbs * sizeof(uint32_t));
     const size_t o_seed = carve(mbs * sizeof(int2));
     hipError_t e = hipMalloc(&s.mem, off);
@@ -219,8 +225,6 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
     s.cost20 = reinterpret_cast<uint64_t*>(b + o_c20);
     s.one_mv = reinterpret_cast<int2*>(b + o_one);
     s.mbops = reinterpret_cast<uint64_t*>(b + o_mbops);
-    s.wl0 = reinterpret_cast<int32_t*>(b + o_wl0);
-    s.wl1 = reinterpret_cast<int32_t*>(b + o_wl1);
     s.mark = reinterpret_cast<uint32_t*>(b + o_mark);
     s.seed_mvs = reinterpret_cast<int2*>(b + o_seed);
     s.mbs_cap = mbs;
@@ -269,12 +273,20 @@ struct vc1me_ctx {
     // proposals handed in by the host (vc1me_analyze_step_proposals)
     float* d_props = nullptr;
     size_t props_cap = 0;
-    // macroblock schedule (VC1ME_SCHEDULE_*) and the relaxation's per-pass
-    // worklist lengths: device [pass][job], pinned host copy of one row
+    // macroblock schedule (VC1ME_SCHEDULE_*); the relaxation's two
+    // worklists over all jobs of a step (2 x relax_wl_cap entries), its
+    // work-queue grid, and the pass count each launch shape last needed
     int schedule = VC1ME_SCHEDULE_RELAX;
-    int32_t* d_wl_n = nullptr;
-    int32_t* h_wl_n = nullptr;
-    size_t wl_jobs_cap = 0;
+    int32_t* d_relax_wl = nullptr;
+    size_t relax_wl_cap = 0;
+    int relax_grid = 0, relax_grid_lds = -1;
+    std::vector<std::pair<std::string, int>> pass_hints;
+    // a step's results: the relaxation counters, one summary per job and the
+    // packed per-macroblock outputs, on the device and in pinned host memory
+    // (one copy per step)
+    uint8_t* d_out = nullptr;
+    uint8_t* h_out = nullptr;
+    size_t out_cap = 0;
     // intensity estimation: vectors, samples, the four sums and 49 SADs
     uint8_t* d_ic = nullptr;
     size_t ic_cap = 0;
@@ -455,7 +467,8 @@ void fill_job(JobDesc& d, const vc1me_params* p, const vc1me_frame* cur, const v
     d.ops = s.ops;
     d.ext_sad = s.ext_sad; d.ext_mv = s.ext_mv; d.ext_ops = s.ext_ops; d.ext_done = s.ext_done;
     d.mbops = s.mbops;
-    d.wl0 = s.wl0; d.wl1 = s.wl1; d.mark = s.mark;
+    d.mark = s.mark;
+    d.pack = nullptr;
     d.nn_out = nullptr; d.nn_logit = 0.f; d.nn_candidates = 1;
     d.seed_mvs = nullptr;
     d.motion_check = 0;
@@ -490,92 +503,59 @@ int mbs_on_diag(int mbw, int mbh, int t) {
     return std::max(0, hi - lo + 1);
 }
 
-// Copy one job's per-MB arrays back and form the frame scalars
-// (src/encoder_motion.cpp:416-441).
-// `seed`: the seeded core's seed (its distant-match fields are the result's,
-// src/encoder_motion.cpp:97-101), or null
-int fetch_result(vc1me_ctx* ctx, const JobDesc& d, const JobSlot& s, vc1me_p_result* out,
-                 const vc1me_p_result* seed = nullptr) {
-    const size_t mbs = static_cast<size_t>(d.mbw) * d.mbh;
-    std::vector<int2> mvs(mbs), bmv(mbs * 4);
-    std::vector<uint8_t> flags(mbs), dec(mbs);
-    std::vector<uint64_t> screen(mbs), chosen(mbs), temporal(mbs);
-    std::vector<double> ratio(mbs), dl(mbs), dc(mbs);
-    GlobalState g;
-    hipStream_t st = ctx->stream;
-    HIP_TRY(hipMemcpyAsync(mvs.data(), s.mvs, mbs * sizeof(int2), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(bmv.data(), s.bmv, mbs * 4 * sizeof(int2), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(flags.data(), s.flags, mbs, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(dec.data(), s.decision, mbs, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(screen.data(), s.screen, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(chosen.data(), s.chosen, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(temporal.data(), s.temporal, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ratio.data(), s.ratio, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(dl.data(), s.dl, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(dc.data(), s.dc, mbs * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(&g, s.g, sizeof g, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    uint64_t total_sad = 0, rate_total = 0, total_px = 0;
-    uint64_t moved = 0, frac = 0, four = 0, intra = 0;
-    for (size_t i = 0; i < mbs; ++i) {
-        const int mx = static_cast<int>(i % d.mbw), my = static_cast<int>(i / d.mbw);
-        const uint64_t px = static_cast<uint64_t>(std::max(0, std::min(16, d.W - mx * 16))) *
-                            static_cast<uint64_t>(std::max(0, std::min(16, d.H - my * 16)));
-        total_px += px;
-        total_sad += chosen[i];
-        rate_total += temporal[i];
-        if (flags[i] & vc1me::kFlag4mvChosen) ++four;
-        if (flags[i] & vc1me::kFlagIntra) ++intra;
-        bool mv_moved = false, mv_frac = false;
-        for (int k = 0; k < 4; ++k) {
-            const int2 v = bmv[i * 4 + k];
-            mv_moved |= v.x != 0 || v.y != 0;
-            mv_frac |= (v.x % 8) != 0 || (v.y % 8) != 0;
-        }
-        moved += mv_moved; frac += mv_frac;
-        if (out->mvs) { out->mvs[2 * i] = mvs[i].x; out->mvs[2 * i + 1] = mvs[i].y; }
-        if (out->block_mvs)
-            for (int k = 0; k < 4; ++k) {
-                out->block_mvs[8 * i + 2 * k] = bmv[i * 4 + k].x;
-                out->block_mvs[8 * i + 2 * k + 1] = bmv[i * 4 + k].y;
-            }
-        if (out->use_4mv) out->use_4mv[i] = (flags[i] & vc1me::kFlag4mvChosen) && !(flags[i] & vc1me::kFlagIntra);
-        if (out->use_intra) out->use_intra[i] = (flags[i] & vc1me::kFlagIntra) ? 1 : 0;
-        if (out->screen_sads) out->screen_sads[i] = screen[i];
-        if (out->inter_intra_cost_ratio) out->inter_intra_cost_ratio[i] = ratio[i];
-        const bool from_seed = seed && d.range != 0;
-        if (out->distant_local_mae) out->distant_local_mae[i] = from_seed ? seed->distant_local_mae[i] : dl[i];
-        if (out->distant_candidate_mae)
-            out->distant_candidate_mae[i] = from_seed ? seed->distant_candidate_mae[i] : dc[i];
-        if (out->distant_match_decision)
-            out->distant_match_decision[i] = from_seed ? seed->distant_match_decision[i] : dec[i];
-    }
+// Fill one job's caller arrays from the step's packed copy and form the
+// frame scalars from its exact integer aggregates
+// (src/encoder_motion.cpp:416-441).  `seed`: the seeded core's seed (its
+// distant-match fields are the result's, src/encoder_motion.cpp:97-101), or
+// null.
+void scatter_result(const JobDesc& d, const vc1me::StepSummary& sum, const uint8_t* packed, vc1me_p_result* out,
+                    const vc1me_p_result* seed) {
+    const int mbs = d.mbw * d.mbh;
+    const size_t n = static_cast<size_t>(mbs);
+    const vc1me::PackedOut P = vc1me::pack_layout(const_cast<uint8_t*>(packed), mbs);
+    if (out->mvs) std::memcpy(out->mvs, P.mvs, n * 8);
+    if (out->block_mvs) std::memcpy(out->block_mvs, P.bmv, n * 32);
+    if (out->use_4mv) std::memcpy(out->use_4mv, P.four, n);
+    if (out->use_intra) std::memcpy(out->use_intra, P.intra, n);
+    if (out->screen_sads) std::memcpy(out->screen_sads, P.screen, n * 8);
+    if (out->inter_intra_cost_ratio) std::memcpy(out->inter_intra_cost_ratio, P.ratio, n * 8);
+    const bool from_seed = seed && d.range != 0;
+    if (out->distant_local_mae) std::memcpy(out->distant_local_mae, from_seed ? seed->distant_local_mae : P.dl, n * 8);
+    if (out->distant_candidate_mae)
+        std::memcpy(out->distant_candidate_mae, from_seed ? seed->distant_candidate_mae : P.dc, n * 8);
+    if (out->distant_match_decision)
+        std::memcpy(out->distant_match_decision, from_seed ? seed->distant_match_decision : P.decision, n);
+    const uint64_t total_px = sum.total_px;
     if (d.range == 0) {
-        out->mean_abs_residual = total_px ? static_cast<double>(total_sad) / static_cast<double>(total_px) : 0.0;
+        out->mean_abs_residual = total_px ? static_cast<double>(sum.total_sad) / static_cast<double>(total_px) : 0.0;
         out->rate_complexity = out->mean_abs_residual;
         out->motion_sad = 0;
         out->moved_macroblocks = out->fractional_chroma_macroblocks = out->four_mv_macroblocks = out->intra_macroblocks = 0;
         out->global_dx = 0; out->global_dy = 0; out->global_sad = 0;
-        return VC1ME_OK;
+        return;
     }
-    const double local_mad = total_px ? static_cast<double>(total_sad) / static_cast<double>(total_px) : 0.0;
+    const double local_mad = total_px ? static_cast<double>(sum.total_sad) / static_cast<double>(total_px) : 0.0;
     // the seeded core has no global search: its global MAD is the local one
     // (src/encoder_motion.cpp:432-434)
     const double global_mad = seed ? local_mad
-                                   : static_cast<double>(g.full_sad) / static_cast<double>(static_cast<uint64_t>(d.W) * d.H);
+                                   : static_cast<double>(sum.global_sad) /
+                                         static_cast<double>(static_cast<uint64_t>(d.W) * d.H);
     out->mean_abs_residual = std::min(local_mad, global_mad);
-    const double rate_local = total_px ? static_cast<double>(rate_total) / static_cast<double>(total_px) : 0.0;
+    const double rate_local = total_px ? static_cast<double>(sum.rate_total) / static_cast<double>(total_px) : 0.0;
     out->rate_complexity = std::min(rate_local, global_mad);
-    out->motion_sad = total_sad;
-    out->moved_macroblocks = moved;
-    out->fractional_chroma_macroblocks = frac;
-    out->four_mv_macroblocks = four;
-    out->intra_macroblocks = intra;
-    out->global_dx = seed ? 0 : g.dx;
-    out->global_dy = seed ? 0 : g.dy;
-    out->global_sad = seed ? 0 : g.full_sad;
-    return VC1ME_OK;
+    out->motion_sad = sum.total_sad;
+    out->moved_macroblocks = sum.moved;
+    out->fractional_chroma_macroblocks = sum.frac;
+    out->four_mv_macroblocks = sum.four;
+    out->intra_macroblocks = sum.intra;
+    out->global_dx = seed ? 0 : sum.gdx;
+    out->global_dy = seed ? 0 : sum.gdy;
+    out->global_sad = seed ? 0 : sum.global_sad;
 }
+
+constexpr int kRelaxCounters = 2 * (vc1me::kMaxRelaxPasses + 1);   // list lengths, then tickets
+constexpr size_t kOutSummaryAt = 1024;                              // byte offset of the summaries
+static_assert(kRelaxCounters * sizeof(int32_t) <= kOutSummaryAt, "relaxation counters overflow their region");
 
 // after_global, if set, is called once the global search is queued (the
 // learned form runs its network there, on the global vectors in the slots).
@@ -592,8 +572,9 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
     if (ctx->slots.size() < static_cast<size_t>(n)) ctx->slots.resize(n);
     ctx->h_jobs.resize(n);
+    hipStream_t st = ctx->stream;
     int max_local = 0, max_diag = 0;
-    size_t max_cells = 0;
+    size_t max_cells = 0, total_mbs = 0;
     for (int i = 0; i < n; ++i) {
         const vc1me_params* p = jobs[i].params;
         const size_t mbs = static_cast<size_t>((p->width + 15) / 16) * ((p->height + 15) / 16);
@@ -612,14 +593,14 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             ctx->h_jobs[i].one_mv = ctx->slots[i].one_mv;
         }
         ctx->slots[i].mbs_last = static_cast<int>(mbs);
+        total_mbs += mbs;
         max_local = std::max(max_local, ctx->h_jobs[i].local_range);
         max_diag = std::max(max_diag, ctx->h_jobs[i].max_diag);
         if (seeds) {
             const vc1me_p_result* sd = seeds[i];
             if (!sd || !sd->mvs || !sd->distant_local_mae || !sd->distant_candidate_mae || !sd->distant_match_decision)
                 return fail(VC1ME_EINVAL, "seed " + std::to_string(i) + " lacks its vectors or distant-match fields");
-            HIP_TRY(hipMemcpyAsync(ctx->slots[i].seed_mvs, sd->mvs, mbs * sizeof(int2), hipMemcpyHostToDevice,
-                                   ctx->stream));
+            HIP_TRY(hipMemcpyAsync(ctx->slots[i].seed_mvs, sd->mvs, mbs * sizeof(int2), hipMemcpyHostToDevice, st));
             ctx->h_jobs[i].seed_mvs = ctx->slots[i].seed_mvs;
         }
         if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range && !seeds) max_cells = std::max(max_cells, cells);
@@ -630,7 +611,6 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         HIP_TRY(hipMalloc(&ctx->d_jobs, sizeof(JobDesc) * n));
         ctx->d_jobs_cap = n;
     }
-    hipStream_t st = ctx->stream;
     if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1 + vc1me::kMaxRelaxPasses))) return rc;
     // interpolated references: one owning job per distinct (frame, filter),
     // each with a buffer of the context's step pool (planes live for the step)
@@ -663,111 +643,131 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             if (bil) d.bp = buf; else d.qp = buf;
         }
     }
-    const bool relax = ctx->schedule == VC1ME_SCHEDULE_RELAX && !(ctx->use_graphs && !ctx->profiling && !after_global);
+    // the step's result buffer: relaxation counters, summaries, then the
+    // packed outputs of every job whose results the caller wants
+    const size_t sum_at = kOutSummaryAt;
+    const size_t pack_at = vc1me::pack_align(sum_at + sizeof(vc1me::StepSummary) * n);
+    std::vector<size_t> pack_off(n, 0);
+    size_t out_bytes = pack_at;
+    bool want_any = false;
+    for (int i = 0; i < n; ++i) {
+        if (!jobs[i].out) continue;
+        want_any = true;
+        pack_off[i] = out_bytes;
+        out_bytes += vc1me::pack_layout(nullptr, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh).bytes;
+    }
+    if (ctx->out_cap < out_bytes) {
+        const size_t cap = std::max(out_bytes, ctx->out_cap * 2);
+        if (ctx->d_out) (void)hipFree(ctx->d_out);
+        if (ctx->h_out) (void)hipHostFree(ctx->h_out);
+        ctx->d_out = nullptr;
+        ctx->h_out = nullptr;
+        ctx->out_cap = 0;
+        const hipError_t e = hipMalloc(&ctx->d_out, cap);
+        if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("result buffer: ") + hipGetErrorString(e));
+        HIP_TRY(hipHostMalloc(&ctx->h_out, cap, hipHostMallocDefault));
+        ctx->out_cap = cap;
+    }
+    for (int i = 0; i < n; ++i) ctx->h_jobs[i].pack = jobs[i].out ? ctx->d_out + pack_off[i] : nullptr;
+    int32_t* d_cnt = reinterpret_cast<int32_t*>(ctx->d_out);
+    auto* d_sum = reinterpret_cast<vc1me::StepSummary*>(ctx->d_out + sum_at);
+    const bool relax = ctx->schedule == VC1ME_SCHEDULE_RELAX;
+    vc1me::RelaxQueue Q{};
+    const int lds = vc1me::mb_kernel_lds_bytes(max_local);
     if (relax) {
-        if (ctx->wl_jobs_cap < static_cast<size_t>(n)) {
-            if (ctx->d_wl_n) (void)hipFree(ctx->d_wl_n);
-            if (ctx->h_wl_n) (void)hipHostFree(ctx->h_wl_n);
-            ctx->d_wl_n = nullptr;
-            ctx->h_wl_n = nullptr;
-            ctx->wl_jobs_cap = 0;
-            HIP_TRY(hipMalloc(&ctx->d_wl_n, sizeof(int32_t) * (vc1me::kMaxRelaxPasses + 1) * n));
-            HIP_TRY(hipHostMalloc(&ctx->h_wl_n, sizeof(int32_t) * n, hipHostMallocDefault));
-            ctx->wl_jobs_cap = n;
+        if (ctx->relax_wl_cap < total_mbs) {
+            if (ctx->d_relax_wl) (void)hipFree(ctx->d_relax_wl);
+            ctx->d_relax_wl = nullptr;
+            ctx->relax_wl_cap = 0;
+            const hipError_t e = hipMalloc(&ctx->d_relax_wl, 2 * total_mbs * sizeof(int32_t));
+            if (e != hipSuccess) return fail(VC1ME_ENOMEM, std::string("relaxation lists: ") + hipGetErrorString(e));
+            ctx->relax_wl_cap = total_mbs;
         }
-        HIP_TRY(hipMemsetAsync(ctx->d_wl_n, 0, sizeof(int32_t) * (vc1me::kMaxRelaxPasses + 1) * n, st));
-        for (int i = 0; i < n; ++i) {
-            ctx->h_jobs[i].wl_n = ctx->d_wl_n + i;
-            ctx->h_jobs[i].wl_stride = n;
+        if (ctx->relax_grid_lds != lds) {
+            ctx->relax_grid = vc1me::relax_queue_grid(lds, ctx->cu_count);
+            ctx->relax_grid_lds = lds;
         }
+        Q.cnt = d_cnt;
+        Q.tk = d_cnt + (vc1me::kMaxRelaxPasses + 1);
+        Q.wl[0] = ctx->d_relax_wl;
+        Q.wl[1] = ctx->d_relax_wl + total_mbs;
+        Q.cap = static_cast<int32_t>(total_mbs);
     }
     HIP_TRY(hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs.data(), sizeof(JobDesc) * n, hipMemcpyHostToDevice, st));
-    for (int i = 0; i < n; ++i) {
-        HIP_TRY(hipMemsetAsync(ctx->slots[i].err, 0, sizeof(int32_t), st));
-        HIP_TRY(hipMemsetAsync(ctx->slots[i].ops, 0, 8 * sizeof(unsigned long long), st));
-        HIP_TRY(hipMemsetAsync(ctx->slots[i].ext_done, 0, ctx->slots[i].mbs_last, st));
-        if (ctx->cost_report) {
-            HIP_TRY(hipMemsetAsync(ctx->slots[i].cost20, 0, sizeof(uint64_t) * ctx->slots[i].mbs_last, st));
-            HIP_TRY(hipMemsetAsync(ctx->slots[i].one_mv, 0, sizeof(int2) * ctx->slots[i].mbs_last, st));
-        }
-    }
     ctx->last_jobs = n;
-    const int lds = vc1me::mb_kernel_lds_bytes(max_local);
-    int max_mbs = 0;
-    for (int i = 0; i < n; ++i)
-        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range)
-            max_mbs = std::max(max_mbs, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh);
+    int max_mbs = 0, max_mbs_all = 0;
+    for (int i = 0; i < n; ++i) {
+        const int m = ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh;
+        max_mbs_all = std::max(max_mbs_all, m);
+        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range) max_mbs = std::max(max_mbs, m);
+    }
     auto per_diag = [&](int t) {
         int per = 0;
         for (int i = 0; i < n; ++i)
             if (t <= ctx->h_jobs[i].max_diag) per = std::max(per, mbs_on_diag(ctx->h_jobs[i].mbw, ctx->h_jobs[i].mbh, t));
         return per;
     };
-    // one launch per anti-diagonal: every macroblock after its neighbours
+    // the launch shape: rasters, ranges and modes of the jobs (graphs and
+    // the relaxation's pass count are kept per shape)
+    std::string shape;
+    {
+        auto put = [&shape](long long v) { shape += std::to_string(v); shape += ','; };
+        put(n); put(lds); put(static_cast<long long>(max_cells)); put(max_mbs); put(max_diag); put(max_pw); put(max_ph);
+        put(relax ? 1 : 0);
+        for (int i = 0; i < n; ++i) {
+            const JobDesc& d = ctx->h_jobs[i];
+            put(d.W); put(d.H); put(d.range); put(d.local_range); put(d.mbw); put(d.mbh); put(d.max_diag);
+            put(d.mode); put(d.phase_owner); put(d.seed_mvs ? 1 : 0); put(d.motion_check); put(d.pack ? 1 : 0);
+            put(d.nn_out ? 1 : 0); put(d.cost20 ? 1 : 0);
+        }
+    }
+    // passes launched without a host read: the most this shape has needed
+    // so far plus two (each pass after the state settles costs a launch of
+    // empty workgroups), or a fixed start for a new shape
+    const int limit = relax_pass_limit();
+    int passes = std::min(limit, relax_first_passes());
+    int seen_need = 0;
+    for (const auto& h : ctx->pass_hints)
+        if (h.first == shape) seen_need = h.second;
+    if (seen_need) passes = std::min(limit, seen_need + 2);
+    // launches with events around them when profiling
     int mb_launches = 0;
-    std::vector<std::pair<int, int>> mb_events;   // event pairs around macroblock launches
+    std::vector<std::pair<int, int>> mb_events;
+    auto timed = [&](bool events, const std::function<hipError_t()>& launch) -> int {
+        const int ev = 4 + 2 * mb_launches;
+        const bool rec = events && ctx->profiling && static_cast<size_t>(ev + 1) < ctx->events.size();
+        if (rec) HIP_TRY(hipEventRecord(ctx->events[ev], st));
+        HIP_TRY(launch());
+        if (rec) {
+            HIP_TRY(hipEventRecord(ctx->events[ev + 1], st));
+            mb_events.emplace_back(ev, ev + 1);
+        }
+        ++mb_launches;
+        return VC1ME_OK;
+    };
+    // one launch per anti-diagonal: every macroblock after its neighbours
     auto diagonal_all = [&](bool events) -> int {
         for (int t = 0; t <= max_diag; ++t) {
             const int per = per_diag(t);
             if (!per) continue;
-            const int ev = 4 + 2 * mb_launches;
-            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[ev], st));
-            HIP_TRY(vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st));
-            if (events && ctx->profiling) {
-                HIP_TRY(hipEventRecord(ctx->events[ev + 1], st));
-                mb_events.emplace_back(ev, ev + 1);
-            }
-            ++mb_launches;
+            if (int rc = timed(events, [&] { return vc1me::launch_mb_diagonal(ctx->d_jobs, n, t, per, lds, st); }))
+                return rc;
         }
         return VC1ME_OK;
     };
-    // relaxation: all macroblocks from the guessed state, then worklist passes
-    // until one changes nothing (the host reads each pass's list length)
-    int max_mbs_all = 0;
-    for (int i = 0; i < n; ++i) max_mbs_all = std::max(max_mbs_all, ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh);
-    uint64_t relax_passes = 0, relax_analyses = 0;
-    ctx->pass_analyses.clear();
-    ctx->pass_ms.clear();
-    bool relax_fell_back = false;
-    auto relax_all = [&](bool events) -> int {
-        HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
-        if (relax_guess()) HIP_TRY(vc1me::launch_mb_guess(ctx->d_jobs, n, max_mbs_all, lds, st));
-        const int limit = relax_pass_limit();
-        int grid = max_mbs_all;
-        int64_t first = 0;
-        for (int i = 0; i < n; ++i) first += static_cast<int64_t>(ctx->h_jobs[i].mbw) * ctx->h_jobs[i].mbh;
-        relax_analyses += static_cast<uint64_t>(first);
-        ctx->pass_analyses.push_back(first);
-        for (int pass = 0;; ++pass) {
-            const int ev = 4 + 2 * mb_launches;
-            if (events && ctx->profiling) HIP_TRY(hipEventRecord(ctx->events[ev], st));
-            HIP_TRY(vc1me::launch_mb_relax(ctx->d_jobs, n, pass, grid, lds, st));
-            if (events && ctx->profiling) {
-                HIP_TRY(hipEventRecord(ctx->events[ev + 1], st));
-                mb_events.emplace_back(ev, ev + 1);
-            }
-            ++mb_launches;
-            ++relax_passes;
-            HIP_TRY(hipMemcpyAsync(ctx->h_wl_n, ctx->d_wl_n + static_cast<size_t>(pass + 1) * n, sizeof(int32_t) * n,
-                                   hipMemcpyDeviceToHost, st));
-            HIP_TRY(hipStreamSynchronize(st));
-            grid = 0;
-            int64_t cnt = 0;
-            for (int i = 0; i < n; ++i) {
-                grid = std::max(grid, ctx->h_wl_n[i]);
-                cnt += ctx->h_wl_n[i];
-            }
-            if (grid == 0) return VC1ME_OK;
-            relax_analyses += static_cast<uint64_t>(cnt);
-            ctx->pass_analyses.push_back(cnt);
-            if (pass + 1 >= limit) break;
+    // relaxation passes [p0, p1): work-queue passes over the device lists,
+    // no host read in between
+    auto relax_passes = [&](bool events, int p0, int p1) -> int {
+        for (int p = p0; p < p1; ++p) {
+            const int rc = p == 0 ? timed(events, [&] { return vc1me::launch_mb_relax0(ctx->d_jobs, n, max_mbs_all, lds, Q, st); })
+                                  : timed(events, [&] { return vc1me::launch_mb_relaxq(ctx->d_jobs, p, ctx->relax_grid, lds, Q, st); });
+            if (rc) return rc;
         }
-        // not settled: the diagonal sweep is exact from any state
-        relax_fell_back = true;
-        return diagonal_all(events);
+        return VC1ME_OK;
     };
     // the step's launch sequence; every launch takes the job table and scalars only
     auto launch_all = [&](bool events) -> int {
+        HIP_TRY(vc1me::launch_step_begin(ctx->d_jobs, n, max_mbs_all, d_cnt, kRelaxCounters, st));
         HIP_TRY(vc1me::launch_phase_planes_jobs(ctx->d_jobs, n, max_pw, max_ph, st));
         HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st,
                                             events ? ctx->events[2] : nullptr, events ? ctx->events[3] : nullptr));
@@ -775,38 +775,36 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             if (int rc = after_global(st)) return rc;
         HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
         if (relax) {
-            if (int rc = relax_all(events)) return rc;
+            HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
+            if (relax_guess()) HIP_TRY(vc1me::launch_mb_guess(ctx->d_jobs, n, max_mbs_all, lds, st));
+            if (int rc = relax_passes(events, 0, passes)) return rc;
         } else {
             HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
             if (int rc = diagonal_all(events)) return rc;
         }
-        HIP_TRY(vc1me::launch_ops_sum(ctx->d_jobs, n, st));
+        HIP_TRY(vc1me::launch_step_end(ctx->d_jobs, n, d_sum, st));
         return VC1ME_OK;
     };
     ctx->timing.graph_replay = 0;
-    if (!relax && ctx->use_graphs && !ctx->profiling && !after_global) {
+    const bool graph = ctx->use_graphs && !ctx->profiling && !after_global;
+    if (graph) {
         // replay the step as one HIP graph, captured once per launch shape
-        std::string key;
-        auto put = [&key](long long v) { key += std::to_string(v); key += ','; };
-        put(reinterpret_cast<long long>(ctx->d_jobs)); put(n); put(lds); put(static_cast<long long>(max_cells));
-        put(max_mbs); put(max_diag); put(max_pw); put(max_ph);
-        for (int i = 0; i < n; ++i) {
-            const JobDesc& d = ctx->h_jobs[i];
-            put(d.W); put(d.H); put(d.range); put(d.local_range); put(d.mbw); put(d.mbh); put(d.max_diag);
-            put(d.mode); put(d.phase_owner); put(d.seed_mvs ? 1 : 0); put(d.motion_check);
-        }
+        const std::string key = shape + "p" + std::to_string(relax ? passes : 0) + "@" +
+                                std::to_string(reinterpret_cast<long long>(ctx->d_jobs)) + "," +
+                                std::to_string(reinterpret_cast<long long>(ctx->d_out)) + "," +
+                                std::to_string(reinterpret_cast<long long>(ctx->d_relax_wl));
         hipGraphExec_t exec = nullptr;
         for (auto& g : ctx->graphs)
             if (g.first == key) exec = g.second;
         if (!exec) {
             HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             const int rc = launch_all(false);
-            hipGraph_t graph = nullptr;
-            const hipError_t ec = hipStreamEndCapture(st, &graph);
-            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
             HIP_TRY(ec);
-            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
+            const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
             HIP_TRY(ei);
             if (ctx->graphs.size() >= 16) {
                 (void)hipGraphExecDestroy(ctx->graphs.front().second);
@@ -823,8 +821,54 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         HIP_TRY(hipEventRecord(ctx->events[0], st));
         if (int rc = launch_all(true)) return rc;
     }
+    // one copy of the counters and summaries (and the packed outputs)
+    const size_t head = pack_at;
+    HIP_TRY(hipMemcpyAsync(ctx->h_out, ctx->d_out, want_any ? out_bytes : head, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(ctx->events[1], st));
     HIP_TRY(hipEventSynchronize(ctx->events[1]));
+    const int32_t* cnt = reinterpret_cast<const int32_t*>(ctx->h_out);
+    bool relax_fell_back = false;
+    int used = 0;   // relaxation passes until the state settled
+    if (relax) {
+        // pass p's list is cnt[p]; the state settled once a list is empty
+        auto settled_at = [&](int upto) {
+            for (int p = 1; p <= upto; ++p)
+                if (cnt[p] == 0) return p;
+            return 0;
+        };
+        int launched = passes;
+        used = settled_at(launched);
+        while (!used && launched < limit) {
+            // not settled: more passes, eight at a time, then look again
+            const int more = std::min(limit, launched + 8);
+            if (int rc = relax_passes(true, launched, more)) return rc;
+            launched = more;
+            HIP_TRY(hipMemcpyAsync(ctx->h_out, ctx->d_out, head, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            used = settled_at(launched);
+        }
+        if (!used) {
+            // not settled within the limit: the diagonal sweep is exact from any state
+            relax_fell_back = true;
+            if (int rc = diagonal_all(true)) return rc;
+        }
+        if (!used || launched > passes) {
+            HIP_TRY(vc1me::launch_step_end(ctx->d_jobs, n, d_sum, st));
+            HIP_TRY(hipMemcpyAsync(ctx->h_out, ctx->d_out, want_any ? out_bytes : head, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipEventRecord(ctx->events[1], st));
+            HIP_TRY(hipEventSynchronize(ctx->events[1]));
+        }
+        // the most passes this shape has needed (a rise is a new captured
+        // graph at the next step; a fall changes nothing)
+        const int need = used ? used : limit;
+        bool found = false;
+        for (auto& h : ctx->pass_hints)
+            if (h.first == shape) { found = true; h.second = std::max(h.second, need); }
+        if (!found) {
+            if (ctx->pass_hints.size() >= 64) ctx->pass_hints.erase(ctx->pass_hints.begin());
+            ctx->pass_hints.emplace_back(shape, need);
+        }
+    }
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
     ctx->timing.step_ms = ms;
@@ -832,43 +876,66 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     if (!ctx->timing.graph_replay) HIP_TRY(hipEventElapsedTime(&gms, ctx->events[2], ctx->events[3]));
     ctx->timing.global_kernel_ms_total = gms;
     ctx->timing.global_kernel_launches = 9;
+    ctx->pass_analyses.clear();
+    ctx->pass_ms.clear();
+    uint64_t analyses = 0;
+    if (relax) {
+        const int np = used ? used : limit;
+        for (int p = 0; p < np; ++p) {
+            const int64_t a = p == 0 ? static_cast<int64_t>(total_mbs) : cnt[p];
+            ctx->pass_analyses.push_back(a);
+            analyses += static_cast<uint64_t>(a);
+        }
+    }
     double mbms = 0.0;
     for (size_t k = 0; k < mb_events.size(); ++k) {
         float x = 0.f;
         HIP_TRY(hipEventElapsedTime(&x, ctx->events[mb_events[k].first], ctx->events[mb_events[k].second]));
         mbms += x;
-        if (k < relax_passes) ctx->pass_ms.push_back(x);
+        if (relax && k < ctx->pass_analyses.size()) ctx->pass_ms.push_back(x);
     }
     if (ctx->timing.graph_replay) {
         mb_launches = 0;
-        for (int t = 0; t <= max_diag; ++t) mb_launches += per_diag(t) ? 1 : 0;
+        if (relax) mb_launches = passes;
+        else
+            for (int t = 0; t <= max_diag; ++t) mb_launches += per_diag(t) ? 1 : 0;
     }
     ctx->timing.mb_kernel_ms_total = mbms;
     ctx->timing.mb_kernel_launches = static_cast<uint64_t>(mb_launches);
     ctx->timing.schedule = relax && !relax_fell_back ? VC1ME_SCHEDULE_RELAX : VC1ME_SCHEDULE_DIAGONAL;
-    ctx->timing.relax_passes = relax_passes;
-    ctx->timing.relax_analyses = relax ? relax_analyses : 0;
-    // kernel-side error flags and work counters
+    ctx->timing.relax_passes = relax ? static_cast<uint64_t>(used ? used : limit) : 0;
+    ctx->timing.relax_analyses = relax ? analyses : 0;
+    // kernel-side error flags and work counters, from the one copy
+    const auto* sums = reinterpret_cast<const vc1me::StepSummary*>(ctx->h_out + sum_at);
     ctx->work = vc1me_work{};
     for (int i = 0; i < n; ++i) {
-        int32_t err = 0;
-        HIP_TRY(hipMemcpy(&err, ctx->slots[i].err, sizeof err, hipMemcpyDeviceToHost));
-        if (err) return fail(VC1ME_EKERNEL, "kernel reported internal error code " + std::to_string(err));
-        unsigned long long ops[8];
-        HIP_TRY(hipMemcpy(ops, ctx->slots[i].ops, sizeof ops, hipMemcpyDeviceToHost));
-        ctx->work.sad_px += ops[0];
-        ctx->work.interp_taps += ops[1];
-        ctx->work.satd_ops += ops[2];
-        ctx->work.sig_ops += ops[3];
-        ctx->work.frame_px += ops[4];
-        ctx->work.seeded_macroblocks += ops[5];
+        if (sums[i].err) return fail(VC1ME_EKERNEL, "kernel reported internal error code " + std::to_string(sums[i].err));
+        ctx->work.sad_px += sums[i].ops[0];
+        ctx->work.interp_taps += sums[i].ops[1];
+        ctx->work.satd_ops += sums[i].ops[2];
+        ctx->work.sig_ops += sums[i].ops[3];
+        ctx->work.frame_px += sums[i].ops[4];
+        ctx->work.seeded_macroblocks += sums[i].ops[5];
     }
     for (int i = 0; i < n; ++i) {
         ctx->counters.frames_analyzed++;
         ctx->counters.macroblocks_analyzed += static_cast<uint64_t>(ctx->h_jobs[i].mbw) * ctx->h_jobs[i].mbh;
-        if (jobs[i].out) {
-            const int rc = fetch_result(ctx, ctx->h_jobs[i], ctx->slots[i], jobs[i].out, seeds ? seeds[i] : nullptr);
-            if (rc) return rc;
+    }
+    if (want_any) {
+        // the caller's arrays from the pinned copy; large steps spread the
+        // copying over a few host threads
+        auto one = [&](int i) {
+            if (jobs[i].out)
+                scatter_result(ctx->h_jobs[i], sums[i], ctx->h_out + pack_off[i], jobs[i].out, seeds ? seeds[i] : nullptr);
+        };
+        const int workers = out_bytes > (size_t(8) << 20) ? std::min(8, n) : 1;
+        if (workers <= 1) {
+            for (int i = 0; i < n; ++i) one(i);
+        } else {
+            std::vector<std::thread> pool;
+            for (int w = 0; w < workers; ++w)
+                pool.emplace_back([&, w] { for (int i = w; i < n; i += workers) one(i); });
+            for (auto& t : pool) t.join();
         }
     }
     return VC1ME_OK;
@@ -1087,8 +1154,9 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_cands) (void)hipFree(ctx->d_cands);
     if (ctx->d_costs) (void)hipFree(ctx->d_costs);
     if (ctx->d_props) (void)hipFree(ctx->d_props);
-    if (ctx->d_wl_n) (void)hipFree(ctx->d_wl_n);
-    if (ctx->h_wl_n) (void)hipHostFree(ctx->h_wl_n);
+    if (ctx->d_relax_wl) (void)hipFree(ctx->d_relax_wl);
+    if (ctx->d_out) (void)hipFree(ctx->d_out);
+    if (ctx->h_out) (void)hipHostFree(ctx->h_out);
     if (ctx->d_ic) (void)hipFree(ctx->d_ic);
     if (ctx->d_b) (void)hipFree(ctx->d_b);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);

@@ -81,14 +81,12 @@ struct JobDesc {
     // sad_px, interp_taps, satd_ops, sig_ops, seeded (5 per macroblock),
     // summed into ops[] once the macroblocks are final
     uint64_t* mbops;
-    // relaxation schedule: two worklists of macroblock indices (pass p reads
-    // list p & 1, appends to the other), the queue stamp of each macroblock
-    // (p + 1 = already queued for pass p + 1), and the per-pass list lengths
-    // at wl_n[p * wl_stride]
-    int32_t* wl0; int32_t* wl1;
+    // relaxation schedule: the queue stamp of each macroblock (p = already
+    // queued for pass p); the lists themselves are the step's RelaxQueue
     uint32_t* mark;
-    int32_t* wl_n;
-    int32_t wl_stride;
+    // packed copy of the per-macroblock outputs written at the end of the
+    // step (PackedOut layout), or null when the results stay on the device
+    uint8_t* pack;
     // seeded core (null: the full search): each macroblock's 1-MV start
     // vector, quarter-pel (analyze_p_picture_core's shared seed)
     const int2* seed_mvs;
@@ -100,6 +98,62 @@ struct JobDesc {
 // Longest relaxation run before the step falls back to the anti-diagonal
 // sweep (which is exact from any starting state).
 constexpr int kMaxRelaxPasses = 96;
+
+// The relaxation's worklists over all jobs of a step: pass p reads
+// cnt[p] entries of wl[p & 1] ((job << 24) | macroblock) by ticket (tk[p])
+// and appends to wl[(p + 1) & 1]; cnt and tk hold kMaxRelaxPasses + 1
+// entries each, cap is the step's macroblock count.
+struct RelaxQueue {
+    int32_t* cnt;
+    int32_t* tk;
+    int32_t* wl[2];
+    int32_t cap;
+};
+
+// Per-job record of a finished step (k_step_end): error flag, global vector,
+// op totals (sad_px, interp_taps, satd_ops, sig_ops, frame_px, seeded) and
+// the frame aggregates as exact integers.
+struct StepSummary {
+    int32_t err;
+    int32_t gdx, gdy;
+    int32_t pad;
+    uint64_t global_sad;
+    unsigned long long ops[8];
+    uint64_t total_sad, rate_total, total_px, moved, frac, four, intra;
+    uint64_t pad2;
+};
+
+// Packed per-macroblock outputs of one job: each field contiguous, 256-byte
+// aligned, in the order below (layout shared by the kernel and the host).
+struct PackedOut {
+    int2* mvs;
+    int2* bmv;
+    uint64_t* screen;
+    double* ratio;
+    double* dl;
+    double* dc;
+    uint8_t* four;
+    uint8_t* intra;
+    uint8_t* decision;
+    size_t bytes;
+};
+__host__ __device__ inline size_t pack_align(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+__host__ __device__ inline PackedOut pack_layout(uint8_t* base, int mbs) {
+    PackedOut p;
+    const size_t n = static_cast<size_t>(mbs);
+    size_t o = 0;
+    p.mvs = reinterpret_cast<int2*>(base + o); o += pack_align(n * 8);
+    p.bmv = reinterpret_cast<int2*>(base + o); o += pack_align(n * 32);
+    p.screen = reinterpret_cast<uint64_t*>(base + o); o += pack_align(n * 8);
+    p.ratio = reinterpret_cast<double*>(base + o); o += pack_align(n * 8);
+    p.dl = reinterpret_cast<double*>(base + o); o += pack_align(n * 8);
+    p.dc = reinterpret_cast<double*>(base + o); o += pack_align(n * 8);
+    p.four = base + o; o += pack_align(n);
+    p.intra = base + o; o += pack_align(n);
+    p.decision = base + o; o += pack_align(n);
+    p.bytes = o;
+    return p;
+}
 
 // Phase-plane geometry of a W x H reference: sample (px, py), px in [-2, W],
 // py in [-2, H], lives at row py + 2, column px + 2.  The pitch leaves room
@@ -121,12 +175,22 @@ hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int ma
 // extended long-range search of every macroblock of the plain jobs, ahead of
 // the anti-diagonal sweep (the relaxation schedule computes it on demand)
 hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
-// relaxation schedule: starting state, one pass over a worklist, and the
-// per-job op totals of the final macroblock analyses
+// relaxation schedule: starting state, guess pass, pass 0 over every
+// macroblock, and a work-queue pass over pass p's list (grid: resident
+// workgroups, relax_queue_grid)
 hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipStream_t s);
 hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
-hipError_t launch_mb_relax(const JobDesc* d_jobs, int njobs, int pass, int grid_x, int lds_bytes, hipStream_t s);
-hipError_t launch_ops_sum(const JobDesc* d_jobs, int njobs, hipStream_t s);
+hipError_t launch_mb_relax0(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, const RelaxQueue& q,
+                            hipStream_t s);
+hipError_t launch_mb_relaxq(const JobDesc* d_jobs, int pass, int grid, int lds_bytes, const RelaxQueue& q,
+                            hipStream_t s);
+int relax_queue_grid(int lds_bytes, int cu_count);
+// start of a step (clears the per-job flags, counters and marks and the
+// first `ncounters` ints at `counters`) and end of a step (per-job summary
+// and, where J->pack is set, the packed outputs)
+hipError_t launch_step_begin(const JobDesc* d_jobs, int njobs, int max_mbs, int32_t* counters, int ncounters,
+                             hipStream_t s);
+hipError_t launch_step_end(const JobDesc* d_jobs, int njobs, StepSummary* out, hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);
 // phase planes of the step's owning jobs (k_phase_planes_jobs)
 hipError_t launch_phase_planes_jobs(const JobDesc* d_jobs, int njobs, int max_w, int max_h, hipStream_t s);

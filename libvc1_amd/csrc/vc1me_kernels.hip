@@ -2690,60 +2690,170 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     run_mb<true>(J, pos % J->mbw, pos / J->mbw, smem);
 }
 
-// One pass: pass 0 analyses every macroblock, pass p > 0 the worklist the
-// previous pass built; one workgroup per list entry (the host sizes the grid
-// from the list lengths).  The next list gets each dependent of a changed
-// macroblock once (the mark stamp).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_relax(const JobDesc* __restrict__ jobs, int pass) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const JobDesc* J = &jobs[blockIdx.y];
+// Appends the right, bottom-left, bottom and bottom-right neighbours of a
+// macroblock whose state changed to the list of the next pass (each once per
+// pass: the queue stamp), as (job << 24) | macroblock entries.
+__device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int pos, int next_pass, RelaxQueue Q) {
     const int mbw = J->mbw, mbh = J->mbh;
-    const int n = pass == 0 ? mbw * mbh : J->wl_n[pass * J->wl_stride];
-    const int i = static_cast<int>(blockIdx.x);
-    if (i >= n) return;
-    const int pos = pass == 0 ? i : ((pass & 1) ? J->wl1 : J->wl0)[i];
-    const NbState before = nb_state(J, pos);
-    const NbState after = run_mb(J, pos % mbw, pos / mbw, smem);
-    if (threadIdx.x == 0 && !nb_equal(before, after)) {
-        int32_t* next = (pass & 1) ? J->wl0 : J->wl1;
-        const int mx = pos % mbw, my = pos / mbw;
-        const int dxs[4] = {1, -1, 0, 1}, dys[4] = {0, 1, 1, 1};
-        for (int k = 0; k < 4; ++k) {
-            const int x = mx + dxs[k], y = my + dys[k];
-            if (x < 0 || x >= mbw || y >= mbh) continue;
-            const int d = y * mbw + x;
-            if (atomicExch(&J->mark[d], static_cast<uint32_t>(pass + 1)) != static_cast<uint32_t>(pass + 1)) {
-                const int at = atomicAdd(&J->wl_n[(pass + 1) * J->wl_stride], 1);
-                next[at] = d;
-            }
+    int32_t* next = Q.wl[next_pass & 1];
+    const int mx = pos % mbw, my = pos / mbw;
+    const int dxs[4] = {1, -1, 0, 1}, dys[4] = {0, 1, 1, 1};
+    for (int k = 0; k < 4; ++k) {
+        const int x = mx + dxs[k], y = my + dys[k];
+        if (x < 0 || x >= mbw || y >= mbh) continue;
+        const int d = y * mbw + x;
+        if (atomicExch(&J->mark[d], static_cast<uint32_t>(next_pass)) != static_cast<uint32_t>(next_pass)) {
+            const int at = atomicAdd(&Q.cnt[next_pass], 1);
+            if (at < Q.cap) next[at] = (job << 24) | d;
+            else atomicMax(J->err, 12);      // cannot happen: a pass lists each macroblock at most once
         }
     }
 }
 
-// per-job totals of the final analyses' op counts (block = job)
-__global__ void __launch_bounds__(256) k_ops_sum(const JobDesc* __restrict__ jobs) {
-    const JobDesc* J = &jobs[blockIdx.x];
-    const int mbs = J->mbw * J->mbh;
-    __shared__ unsigned long long part[5][256 / kWave];
-    unsigned long long s[5] = {0, 0, 0, 0, 0};
+// Pass 0: every macroblock of every job (one workgroup each, full
+// occupancy); the macroblocks whose neighbours' state changed form the list
+// of pass 1.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_relax0(const JobDesc* __restrict__ jobs,
+                                                                                       RelaxQueue Q) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int job = static_cast<int>(blockIdx.y);
+    const JobDesc* J = &jobs[job];
+    const int mbw = J->mbw;
+    const int pos = static_cast<int>(blockIdx.x);
+    if (pos >= mbw * J->mbh) return;
+    const NbState before = nb_state(J, pos);
+    const NbState after = run_mb(J, pos % mbw, pos / mbw, smem);
+    if (threadIdx.x == 0 && !nb_equal(before, after)) queue_dependents(J, job, pos, 1, Q);
+}
+
+// Pass p > 0 as a work queue: a grid of resident workgroups takes entries of
+// the pass's list by ticket until the list is exhausted (no workgroup ever
+// waits for another), and queues the dependents of the macroblocks that
+// changed for pass p + 1.  The list length was final when the previous pass
+// ended; an empty list ends every workgroup at once, so passes launched
+// after the state settled cost only their launch.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_relaxq(const JobDesc* __restrict__ jobs,
+                                                                                       RelaxQueue Q, int pass) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int n = imin(Q.cnt[pass], Q.cap);
+    if (n == 0) return;
+    const int32_t* list = Q.wl[pass & 1];
+    for (;;) {
+        int t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(&Q.tk[pass], 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (t >= n) return;
+        const int code = list[t];
+        const int job = code >> 24, pos = code & 0xffffff;
+        const JobDesc* J = &jobs[job];
+        const NbState before = nb_state(J, pos);
+        const NbState after = run_mb(J, pos % J->mbw, pos / J->mbw, smem);
+        if (threadIdx.x == 0 && !nb_equal(before, after)) queue_dependents(J, job, pos, pass + 1, Q);
+        wsync();
+    }
+}
+
+// Start of a step: clears the per-job error flag, op counters, extended
+// search marks and cost report, and the relaxation's list lengths and
+// ticket counters (a fill kernel, so a captured step has no memset nodes).
+__global__ void __launch_bounds__(256) k_step_begin(const JobDesc* __restrict__ jobs, int32_t* __restrict__ counters,
+                                                    int ncounters) {
+    const JobDesc* J = &jobs[blockIdx.y];
+    const int pos = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (blockIdx.x == 0 && blockIdx.y == 0)
+        for (int i = static_cast<int>(threadIdx.x); i < ncounters; i += 256) counters[i] = 0;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 8) J->ops[threadIdx.x] = 0;
+        if (threadIdx.x == 8) *J->err = 0;
+    }
+    if (pos >= J->mbw * J->mbh) return;
+    J->ext_done[pos] = 0;
+    if (J->cost20) {
+        J->cost20[pos] = 0;
+        J->one_mv[pos] = make_int2(0, 0);
+    }
+}
+
+// End of a step, one workgroup per job: the totals of the final macroblock
+// analyses' op counts, the frame aggregates of analyze_p_picture_core
+// (src/encoder_motion.cpp:416-441: selected SAD, temporal SAD, active
+// pixels, moved / fractional-chroma / 4-MV / intra macroblocks) as exact
+// integers, the global vector and the error flag, into the job's summary;
+// with J->pack set, every per-macroblock output field in the packed layout
+// the host copies back in one transfer.  Reads the state only, so it can run
+// again after more passes.
+__global__ void __launch_bounds__(256) k_step_end(const JobDesc* __restrict__ jobs, StepSummary* __restrict__ out) {
+    const int job = static_cast<int>(blockIdx.x);
+    const JobDesc* J = &jobs[job];
+    const int mbw = J->mbw, mbs = mbw * J->mbh;
+    constexpr int kN = 12;   // 5 op sums, then sad, rate, px, moved, frac, four, intra
+    unsigned long long s[kN] = {};
+    PackedOut P = pack_layout(J->pack, mbs);
     for (int pos = static_cast<int>(threadIdx.x); pos < mbs; pos += 256) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) s[k] += J->mbops[static_cast<size_t>(pos) * 5 + k];
+        const int mx = pos % mbw, my = pos / mbw;
+        const uint64_t px = static_cast<uint64_t>(imax(0, imin(16, J->W - mx * 16))) *
+                            static_cast<uint64_t>(imax(0, imin(16, J->H - my * 16)));
+        const uint8_t f = J->flags[pos];
+        bool moved = false, frac = false;
+        int2 b[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            b[k] = J->bmv[pos * 4 + k];
+            moved |= b[k].x != 0 || b[k].y != 0;
+            frac |= (b[k].x % 8) != 0 || (b[k].y % 8) != 0;
+        }
+        s[5] += J->chosen[pos];
+        s[6] += J->temporal[pos];
+        s[7] += px;
+        s[8] += moved;
+        s[9] += frac;
+        s[10] += (f & kFlag4mvChosen) ? 1 : 0;
+        s[11] += (f & kFlagIntra) ? 1 : 0;
+        if (J->pack) {
+            P.mvs[pos] = J->mvs[pos];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) P.bmv[pos * 4 + k] = b[k];
+            P.screen[pos] = J->screen[pos];
+            P.ratio[pos] = J->ratio[pos];
+            P.dl[pos] = J->dl_mae[pos];
+            P.dc[pos] = J->dc_mae[pos];
+            P.four[pos] = ((f & kFlag4mvChosen) && !(f & kFlagIntra)) ? 1 : 0;
+            P.intra[pos] = (f & kFlagIntra) ? 1 : 0;
+            P.decision[pos] = J->decision[pos];
+        }
     }
+    __shared__ unsigned long long part[kN][256 / kWave];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < kN; ++k) {
         unsigned long long v = s[k];
         for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
         if (lane == 0) part[k][w] = v;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long t[5] = {0, 0, 0, 0, 0};
-        for (int k = 0; k < 5; ++k)
+        unsigned long long t[kN] = {};
+        for (int k = 0; k < kN; ++k)
             for (int j = 0; j < 256 / kWave; ++j) t[k] += part[k][j];
-        for (int k = 0; k < 4; ++k) atomicAdd(&J->ops[k], t[k]);
-        atomicAdd(&J->ops[5], t[4]);
+        StepSummary& o = out[job];
+        o.err = *J->err;
+        o.gdx = J->g->dx;
+        o.gdy = J->g->dy;
+        o.global_sad = J->g->full_sad;
+        // ops[4] (frame SAD pixels) comes from the global search; the others
+        // are this step's final analyses
+        for (int k = 0; k < 8; ++k) o.ops[k] = J->ops[k];
+        for (int k = 0; k < 4; ++k) o.ops[k] += t[k];
+        o.ops[5] += t[4];
+        o.total_sad = t[5];
+        o.rate_total = t[6];
+        o.total_px = t[7];
+        o.moved = t[8];
+        o.frac = t[9];
+        o.four = t[10];
+        o.intra = t[11];
     }
 }
 
@@ -3745,16 +3855,39 @@ hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_mbs, int ld
     return hipGetLastError();
 }
 
-hipError_t launch_mb_relax(const JobDesc* d_jobs, int njobs, int pass, int grid_x, int lds_bytes, hipStream_t s) {
-    if (njobs <= 0 || grid_x <= 0) return hipSuccess;
-    if (pass < 0 || pass >= kMaxRelaxPasses) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_mb_relax, dim3(grid_x, njobs), dim3(64), lds_bytes, s, d_jobs, pass);
+hipError_t launch_mb_relax0(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, const RelaxQueue& q,
+                            hipStream_t s) {
+    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mb_relax0, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs, q);
     return hipGetLastError();
 }
 
-hipError_t launch_ops_sum(const JobDesc* d_jobs, int njobs, hipStream_t s) {
+int relax_queue_grid(int lds_bytes, int cu_count) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_mb_relaxq, 64, lds_bytes) != hipSuccess || per_cu <= 0)
+        per_cu = 8;
+    return per_cu * (cu_count > 0 ? cu_count : 256);
+}
+
+hipError_t launch_mb_relaxq(const JobDesc* d_jobs, int pass, int grid, int lds_bytes, const RelaxQueue& q,
+                            hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
+    if (pass <= 0 || pass >= kMaxRelaxPasses) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mb_relaxq, dim3(grid), dim3(64), lds_bytes, s, d_jobs, q, pass);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_begin(const JobDesc* d_jobs, int njobs, int max_mbs, int32_t* counters, int ncounters,
+                             hipStream_t s) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ops_sum, dim3(njobs), dim3(256), 0, s, d_jobs);
+    hipLaunchKernelGGL(k_step_begin, dim3(((max_mbs > 0 ? max_mbs : 1) + 255) / 256, njobs), dim3(256), 0, s, d_jobs,
+                       counters, ncounters);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_end(const JobDesc* d_jobs, int njobs, StepSummary* out, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_step_end, dim3(njobs), dim3(256), 0, s, d_jobs, out);
     return hipGetLastError();
 }
 

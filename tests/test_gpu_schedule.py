@@ -108,6 +108,43 @@ def test_relax_pass_limit_falls_back_to_the_sweep(backend):
     assert t.schedule == abi.SCHEDULE_DIAGONAL    # the sweep finished the step
 
 
+@pytest.mark.parametrize("graphs", [False, True], ids=["eager", "graph"])
+def test_relax_continues_when_the_launched_passes_do_not_settle(graphs):
+    """A new launch shape starts with a fixed number of device-driven passes
+    (here forced to 2); when the last launched pass still changed something,
+    the step launches more passes until it settles, exact either way, and
+    the shape's later steps start with enough passes."""
+    from libvc1_amd import Backend
+    d = synth.pan_rectangles(6, 4, width=320, height=176, frame_indices=[0, 3])
+    p = abi.default_params(320, 176, pqindex=8, motion_search_range=256, motion_local_search_range=32,
+                           mode=abi.MODE_MIXED_MV)
+    want = oracle_analyze(p, d[3], d[0])
+    _say(f"[case] relax with 2 first passes, graphs {graphs}")
+    os.environ["VC1ME_RELAX_FIRST_PASSES"] = "2"
+    be = Backend(0)       # a fresh context: no pass count known for the shape
+    try:
+        be.set_graphs(graphs)
+        fc, fr = be.upload(d[3]), be.upload(d[0])
+        try:
+            first = be.analyze_step([(fc, fr)], [p])[0]
+            t1 = be.timing()
+            again = be.analyze_step([(fc, fr)], [p])[0]
+            t2 = be.timing()
+        finally:
+            fc.release()
+            fr.release()
+    finally:
+        del os.environ["VC1ME_RELAX_FIRST_PASSES"]
+        be.close()
+    assert t1.schedule == abi.SCHEDULE_RELAX and t1.relax_passes > 2
+    assert t2.relax_passes == t1.relax_passes
+    assert t2.mb_kernel_launches == t1.relax_passes + 2 if graphs else True   # the need plus two, in the graph
+    for got in (first, again):
+        assert not got.diff(want), describe_diff(got, want, p.width)
+    if graphs:
+        assert t1.graph_replay == 2 and t2.graph_replay == 2    # recaptured with more passes
+
+
 def test_relax_learned_step_equals_diagonal(backend):
     """The proposal-seeded step runs the same macroblock analysis; both
     schedules give the same seeded result."""

@@ -85,7 +85,9 @@ def test_seeded_step_mixed_rasters_and_graphs(backend):
         for label, graphs, schedule in (("relax", False, abi.SCHEDULE_RELAX),
                                         ("diagonal", False, abi.SCHEDULE_DIAGONAL),
                                         ("graph capture", True, abi.SCHEDULE_DIAGONAL),
-                                        ("graph replay", True, abi.SCHEDULE_DIAGONAL)):
+                                        ("graph replay", True, abi.SCHEDULE_DIAGONAL),
+                                        ("relax graph capture", True, abi.SCHEDULE_RELAX),
+                                        ("relax graph replay", True, abi.SCHEDULE_RELAX)):
             _say(f"[case] seeded step of 3 rasters: {label}")
             backend.set_schedule(schedule)
             backend.set_graphs(graphs)
