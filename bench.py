@@ -95,6 +95,27 @@ def make_frames():
     return pairs, pics
 
 
+def host_description() -> dict:
+    """The host the CPU baseline runs on: logical CPUs the process may use,
+    logical CPUs of the machine, physical cores, the CPU model and the SIMD
+    extensions the reference's auto-selection can pick from."""
+    out = {"nproc": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count(),
+           "logical_cpus": os.cpu_count()}
+    try:
+        info = open("/proc/cpuinfo").read()
+        model = next((ln.split(":", 1)[1].strip() for ln in info.splitlines() if ln.startswith("model name")), None)
+        flags = next((ln.split(":", 1)[1].split() for ln in info.splitlines() if ln.startswith("flags")), [])
+        cores = {(blk.split("physical id")[1].split("\n")[0], blk.split("core id")[1].split("\n")[0])
+                 for blk in info.split("\n\n") if "core id" in blk and "physical id" in blk}
+        out["model"] = model
+        out["physical_cores"] = len(cores) or None
+        out["isa"] = [f for f in ("sse4_2", "avx2", "fma", "bmi2", "avx512f", "avx512bw", "avx512vl", "avx512_vnni")
+                      if f in flags]
+    except OSError:
+        pass
+    return out
+
+
 def cpu_baseline(pairs, pics, params):
     """Reference CPU path (oracle/_ref) on `threads` independent pairs, one
     whole frame pair per thread, SIMD tier auto-selected by the reference."""
@@ -103,7 +124,8 @@ def cpu_baseline(pairs, pics, params):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from libvc1_amd import abi
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libvc1ref.so")
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_description()
+    threads = max(1, min(16, host.get("nproc") or 1))
     n = min(len(pairs), threads)
     curs = [abi.PlaneSet.of(pics[c]) for c, _ in pairs[:n]]
     refs = [abi.PlaneSet.of(pics[r]) for _, r in pairs[:n]]
@@ -119,7 +141,9 @@ def cpu_baseline(pairs, pics, params):
         secs1 = lib.ref_time_p_frames(C.byref(params), 1, ca, ra, 1, 1)
         if secs > 0:
             out = {"value": round(n * mbs / secs, 1), "unit": "MB/s", "cores": threads, "kind": "reference",
-                   "sample": f"{n} of the 16 1080p P-frame pairs, one pair per thread, reference SIMD auto-selected",
+                   "threads": threads, "host": host,
+                   "sample": f"{n} of the 16 1080p P-frame pairs, one pair per thread on {threads} threads "
+                             f"(of {host.get('nproc')} usable logical CPUs), reference SIMD auto-selected",
                    "seconds": round(secs, 3),
                    "value_1_thread": round(mbs / secs1, 1) if secs1 > 0 else None}
             if all(k in pics for k in (0, 1, 3)):
@@ -467,6 +491,123 @@ def intensity_part(be, jobs, final_results, check_pair):
             "estimates": sorted({str(x) for x in ics})}
 
 
+def scorer_leg(be, pics, frames):
+    """Per-call latency of the drop-in batch scorer (the compute slot's
+    score_motion_batch) on the 1080p frames at the reference's batch sizes:
+    9 (a sub-pel stencil), 36 (UMH finalists), 289 (the rescue's coarse grid)
+    and 1089 (the dense grid), for vc1me_score_batch (host planes keyed by a
+    checksum), vc1me_score_batch_gen (keyed by generation numbers),
+    vc1me_score_batch_frames (resident frames) and its allocation-free form;
+    host-timed per call, every call synchronous as the slot's contract is."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    cur, ref = pics[3].y, pics[0].y
+    fc, fr = frames[3], frames[0]
+    forms = {
+        "score_batch": lambda c: be.score_batch(cur, ref, 960, 540, 16, 16, c),
+        "score_batch_gen": lambda c: be.score_batch_gen(cur, 3, ref, 0, 960, 540, 16, 16, c),
+        "score_batch_frames": lambda c: be.score_batch_frames(fc, fr, 960, 540, 16, 16, c),
+        "score_batch_frames_noalloc": lambda c: be.score_batch_frames(fc, fr, 960, 540, 16, 16, c, noalloc=True),
+    }
+    out = {}
+    for n in (9, 36, 289, 1089):
+        cands = (rng.integers(-32, 33, size=(n, 2)) * 4).astype(np.int32)
+        row = {}
+        for name, fn in forms.items():
+            fn(cands)
+            reps = 50 if name == "score_batch" else 200
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn(cands)
+            row[name] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+        out[str(n)] = row
+    log(f"scorer: microseconds per call {out}")
+    return {"us_per_call": out, "block": "16x16 at (960, 540) of the 1080p pair (3, 0), integer candidates within "
+                                          "+-32 px, SAD, stride 1"}
+
+
+HARD_STREAMS, HARD_SEED = 16, 9_500_000
+HARD_STEPS = (("mid-scene", 10), ("scene cut", 30))
+
+
+def _hard_stream(seed):
+    from libvc1_amd import synth
+    need = sorted({t - k for _, t in HARD_STEPS for k in (0, 1)})
+    d = synth.pan_rectangles(seed, 120, width=3840, height=2160, max_pan_px=200, max_obj_px=200, cut_every=30,
+                             dup_fraction=0.05, frame_indices=need)
+    return seed, d
+
+
+def hard_frames():
+    """The 2160p hard-motion workload's frames (velocities up to 200 px per
+    frame, a cut to a fresh scene every 30 frames, 5% exact duplicates; one
+    seed per stream) around two frame steps of 16 streams, generated in worker
+    processes before the GPU is touched."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    t0 = time.time()
+    seeds = [HARD_SEED + i for i in range(HARD_STREAMS)]
+    with ProcessPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)), mp_context=mp.get_context("fork")) as pool:
+        out = list(pool.map(_hard_stream, seeds, chunksize=1))
+    log(f"hard motion: generated {sum(len(d) for _, d in out)} 2160p pictures of {len(out)} streams in "
+        f"{time.time() - t0:.1f}s")
+    return out
+
+
+def hard_leg(be, gen, steps):
+    """The 2160p hard-motion workload: 16 concurrent streams, one frame step
+    of all of them per analysis step (16 pairs at 3840x2160, R 1024), plain
+    (one captured graph per step) and learned-seeded, at a mid-scene step and
+    at the step across the scene cut, with the learned form's cost rise."""
+    import numpy as np
+    from libvc1_amd import abi, learned
+    from libvc1_amd.backend import ProposalNet
+    p = abi.default_params(3840, 2160, pqindex=8, motion_search_range=1024, motion_local_search_range=32,
+                           mode=abi.MODE_MIXED_MV, me_quality=2, allow_intra=1)
+    mbs = 240 * 135
+    net = ProposalNet(be)
+    net.set_params(learned.load_weights())
+    res = {}
+    uploaded = []
+    try:
+        for label, t in HARD_STEPS:
+            jobs = []
+            for _, d in gen:
+                fc, fr = be.upload(d[t]), be.upload(d[t - 1])
+                uploaded += [fc, fr]
+                jobs.append((fc, fr))
+            plist = [p] * len(jobs)
+            be.set_graphs(True)
+            be.analyze_step(jobs, plist, want_results=False)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                be.analyze_step(jobs, plist, want_results=False)
+            plain_s = (time.perf_counter() - t0) / steps
+            be.set_graphs(False)
+            be.analyze_step_learned(net, jobs, plist, 0.5, want_results=False)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                be.analyze_step_learned(net, jobs, plist, 0.5, want_results=False)
+            learned_s = (time.perf_counter() - t0) / steps
+            cmp = learned.cost_compare(be, net, jobs, plist, 0.5)
+            n_mbs = len(jobs) * mbs
+            res[label] = {"frame": t, "ms_per_step": round(plain_s * 1e3, 2), "value": round(n_mbs / plain_s, 1),
+                          "learned_ms_per_step": round(learned_s * 1e3, 2),
+                          "learned_value": round(n_mbs / learned_s, 1),
+                          "learned_speedup": round(plain_s / learned_s, 4),
+                          "learned_cost_rise": round(cmp["cost_rise"], 6),
+                          "learned_seeded_fraction": round(cmp["seeded_fraction"], 4)}
+            log(f"hard motion {label}: {res[label]}")
+    finally:
+        be.set_graphs(False)
+        net.close()
+        for f in uploaded:
+            f.release()
+    return {"streams": HARD_STREAMS, "seed": HARD_SEED, "unit": "MB/s", "steps": res,
+            "config": "3840x2160, 16 streams in one step per frame step, R 1024, local 32, Mixed-MV qpel SATD, "
+                      "velocities up to 200 px/frame, cuts every 30 frames, 5% duplicates"}
+
+
 BATCHED_STREAMS, BATCHED_SEED = 64, 5
 
 
@@ -536,6 +677,8 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the diagonal-sweep comparison steps")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-stream batched leg")
+    ap.add_argument("--no-hard", action="store_true", help="skip the 2160p hard-motion leg")
+    ap.add_argument("--no-scorer", action="store_true", help="skip the batch-scorer latency leg")
     ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-core leg")
     ap.add_argument("--no-bframes", action="store_true", help="skip the B-picture leg")
     ap.add_argument("--batched-scale", type=float, default=0.1,
@@ -553,6 +696,7 @@ def main() -> int:
     params = workload_params()
     pairs, pics = make_frames()
     batched_streams = None if args.no_batched else batched_frames(args.batched_scale)
+    hard_streams = None if args.no_hard else hard_frames()
     be = Backend(0)
     log(f"device {be.device_name}")
     frames = {k: be.upload(v) for k, v in pics.items()}
@@ -560,32 +704,53 @@ def main() -> int:
     plist = [params] * len(jobs)
     mbs_per_step = len(jobs) * ((WIDTH + 15) // 16) * ((HEIGHT + 15) // 16)
 
+    # the timed steps: each step one captured HIP graph (relaxation passes
+    # driven from the device, every per-macroblock field and the frame
+    # aggregates computed; nothing copied back but the step summary)
+    be.set_graphs(True)
     for i in range(args.warmup):
         be.analyze_step(jobs, plist, want_results=False)
-        log(f"warmup step {i + 1}/{args.warmup}: {be.timing().step_ms:.1f} ms")
-
-    be.set_profiling(True)
-    mb_ms = 0.0
-    mb_launches = 0
-    work = None
+        log(f"warmup step {i + 1}/{args.warmup}: {be.timing().step_ms:.1f} ms (graph {be.timing().graph_replay})")
     passes, analyses = [], []
     t0 = time.perf_counter()
     for i in range(args.steps):
         be.analyze_step(jobs, plist, want_results=False)
         t = be.timing()
-        mb_ms += t.mb_kernel_ms_total
-        mb_launches += t.mb_kernel_launches
         passes.append(int(t.relax_passes))
         analyses.append(int(t.relax_analyses))
-        w = be.work()
-        work = w
     elapsed = time.perf_counter() - t0
-    be.set_profiling(False)
+    graph_replays = int(t.graph_replay)
     ms_per_step = elapsed * 1e3 / args.steps
     value = mbs_per_step * args.steps / elapsed
+    # download-inclusive rate (every field of every pair back to the host)
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        results = be.analyze_step(jobs, plist, want_results=True)
+    with_dl = mbs_per_step * args.steps / (time.perf_counter() - t1)
+    be.set_graphs(False)
     schedule = {"kind": "relax" if t.schedule == abi.SCHEDULE_RELAX else "diagonal",
+                "launch": "one captured HIP graph per step" if graph_replays == 1 else "eager",
                 "passes_per_step": passes, "analyses_per_step": analyses}
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s, schedule {schedule}")
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s, with download "
+        f"{with_dl:,.0f} MB/s, schedule {schedule}")
+
+    # the same steps launched one by one with HIP events around every launch
+    # of the macroblock kernel: its launch durations for the roofline
+    be.set_profiling(True)
+    mb_ms = 0.0
+    mb_launches = 0
+    work = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        be.analyze_step(jobs, plist, want_results=False)
+        tp = be.timing()
+        mb_ms += tp.mb_kernel_ms_total
+        mb_launches += tp.mb_kernel_launches
+        work = be.work()
+    eager_value = mbs_per_step * args.steps / (time.perf_counter() - t0)
+    be.set_profiling(False)
+    schedule["eager_profiled_value"] = round(eager_value, 1)
+    schedule["relax_pass_ms"] = [round(ms, 3) for _, ms in be.relax_passes()]
 
     # the anti-diagonal sweep on the same steps, for comparison
     if not args.no_compare:
@@ -598,11 +763,6 @@ def main() -> int:
         be.set_schedule(abi.SCHEDULE_RELAX)
         log(f"diagonal sweep: {schedule['diagonal_value']:,.0f} MB/s")
 
-    # download-inclusive rate (MV fields back to the host every step)
-    t1 = time.perf_counter()
-    results = be.analyze_step(jobs, plist, want_results=True)
-    with_dl = mbs_per_step / (time.perf_counter() - t1)
-
     parity = None
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -614,7 +774,8 @@ def main() -> int:
         parity = "bit-exact" if not bad else f"MISMATCH in {bad}"
         log(f"parity of pair {pairs[0]} vs oracle: {parity} ({time.time() - t2:.1f}s)")
 
-    # roofline of the macroblock kernel
+    # roofline of the macroblock kernel (relaxation: k_mb_relax0 for pass 0,
+    # k_mb_relaxq for the work-queue passes; profiled under one tag)
     mb_kernel = "k_mb_relax" if schedule["kind"] == "relax" else "k_mb_diagonal"
     cus, clk, peak = device_peak_ops()
     mb_ops = work.sad_px + work.interp_taps + work.satd_ops + work.sig_ops
@@ -628,7 +789,8 @@ def main() -> int:
         "unit": "TOP/s",
         "frac": round(achieved / peak, 5),
         "traffic": measured_traffic(mb_kernel),
-        "kernel": mb_kernel,
+        "kernel": "k_mb_relax0 + k_mb_relaxq (one launch per relaxation pass)" if mb_kernel == "k_mb_relax"
+                  else mb_kernel,
         "avg_launch_us": round(avg_launch_us, 2),
         "launches_per_step": mb_launches // max(1, args.steps),
         "ops_per_step": {"sad_px": work.sad_px, "interp_taps": work.interp_taps, "satd_ops": work.satd_ops,
@@ -657,6 +819,13 @@ def main() -> int:
     learned_out = None
     if not args.no_learned:
         learned_out = learned_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step)
+
+    scorer = None if args.no_scorer else scorer_leg(be, pics, frames)
+
+    hard = None
+    if hard_streams is not None:
+        hard = hard_leg(be, hard_streams, max(2, args.steps // 4))
+        hard_streams = None
 
     batched = None
     if batched_streams is not None:
@@ -696,6 +865,8 @@ def main() -> int:
         "seeded_core": seeded,
         "b_pictures": bpics,
         "batched": batched,
+        "hard_motion_2160p": hard,
+        "scorer": scorer,
     }
     for f in frames.values():
         f.release()

@@ -164,6 +164,21 @@ int vc1me_score_batch(vc1me_ctx* ctx, const uint8_t* cur, const uint8_t* ref,
                       const vc1me_candidate* candidates, size_t n,
                       uint64_t* costs_out);
 
+/* vc1me_score_batch with the device copies of the planes keyed by a
+ * generation number the caller keeps per plane instead of a checksum: no pass
+ * over the planes per call (the checksum of a 1080p plane costs about as
+ * much as scoring a batch).  The caller must change a plane's generation
+ * whenever it writes the plane (the reference's pointer-and-fingerprint
+ * cache, src/vulkan_compute.cpp:194-231, misses such writes when the sampled
+ * bytes stay the same; a generation does not).  An encoder can pass the
+ * picture's coding index.  No other difference. */
+int vc1me_score_batch_gen(vc1me_ctx* ctx, const uint8_t* cur, uint64_t cur_generation,
+                          const uint8_t* ref, uint64_t ref_generation,
+                          int width, int height, int x0, int y0, int bw, int bh,
+                          int sample_stride, int bilinear, int rnd, int metric,
+                          const vc1me_candidate* candidates, size_t n,
+                          uint64_t* costs_out);
+
 /* ---- frames resident in HBM (explicit handles instead of the reference's
  * pointer+fingerprint cache, src/vulkan_compute.cpp:194-231). */
 int vc1me_frame_upload(vc1me_ctx* ctx, const vc1me_planes* planes, vc1me_frame** out);

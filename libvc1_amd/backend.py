@@ -30,7 +30,7 @@ EXPORTS = (
     "vc1me_frame_download", "vc1me_analyze_b_frame", "vc1me_analyze_b_step", "vc1me_scene_change_score",
     "vc1me_has_good_motion_match", "vc1me_predict_p_frame", "vc1me_predict_bi_frame",
     "vc1me_set_learned_candidates", "vc1me_score_batch_frames_noalloc", "vc1me_predict_p_frame_padded",
-    "vc1me_predict_bi_frame_padded",
+    "vc1me_predict_bi_frame_padded", "vc1me_score_batch_gen",
 )
 
 
@@ -63,6 +63,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.vc1me_score_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
+    lib.vc1me_score_batch_gen.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_int,
+                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_int, P(abi.Candidate), C.c_size_t, P(C.c_uint64)]
     lib.vc1me_frame_upload.argtypes = [C.c_void_p, P(abi.Planes), P(C.c_void_p)]
     lib.vc1me_frame_release.argtypes = [C.c_void_p, C.c_void_p]
     lib.vc1me_frame_release.restype = None
@@ -385,6 +388,35 @@ class Backend:
         self._check(self.lib.vc1me_score_batch(
             self.ctx, cur.ctypes.data, ref.ctypes.data, w, h, x0, y0, bw, bh, sample_stride, bilinear, rnd,
             metric, cands.ctypes.data_as(C.POINTER(abi.Candidate)), len(cands),
+            out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
+    def score_batch_frames(self, cur: Frame, ref: Frame, x0: int, y0: int, bw: int, bh: int, cands: np.ndarray,
+                           sample_stride: int = 1, bilinear: int = 0, rnd: int = 0, metric: int = abi.METRIC_SAD,
+                           noalloc: bool = False) -> np.ndarray:
+        """vc1me_score_batch_frames (or its allocation-free form) on resident
+        frames."""
+        cands = np.ascontiguousarray(cands, np.int32).reshape(-1, 2)
+        out = np.zeros(len(cands), np.uint64)
+        fn = self.lib.vc1me_score_batch_frames_noalloc if noalloc else self.lib.vc1me_score_batch_frames
+        self._check(fn(self.ctx, cur.handle, ref.handle, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
+                       cands.ctypes.data_as(C.POINTER(abi.Candidate)), len(cands),
+                       out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out
+
+    def score_batch_gen(self, cur: np.ndarray, cur_gen: int, ref: np.ndarray, ref_gen: int, x0: int, y0: int,
+                        bw: int, bh: int, cands: np.ndarray, sample_stride: int = 1, bilinear: int = 0, rnd: int = 0,
+                        metric: int = abi.METRIC_SAD) -> np.ndarray:
+        """vc1me_score_batch_gen: the planes' device copies keyed by the
+        caller's generation numbers."""
+        cur = np.ascontiguousarray(cur, np.uint8)
+        ref = np.ascontiguousarray(ref, np.uint8)
+        h, w = cur.shape
+        cands = np.ascontiguousarray(cands, np.int32).reshape(-1, 2)
+        out = np.zeros(len(cands), np.uint64)
+        self._check(self.lib.vc1me_score_batch_gen(
+            self.ctx, cur.ctypes.data, cur_gen, ref.ctypes.data, ref_gen, w, h, x0, y0, bw, bh, sample_stride,
+            bilinear, rnd, metric, cands.ctypes.data_as(C.POINTER(abi.Candidate)), len(cands),
             out.ctypes.data_as(C.POINTER(C.c_uint64))))
         return out
 

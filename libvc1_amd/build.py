@@ -51,6 +51,22 @@ def build_backend(verbose: bool = True) -> str:
     return LIB
 
 
+ADAPTER_SRC = os.path.join(ROOT, "tests", "adapter", "adapter_test.cpp")
+ADAPTER_BIN = os.path.join(ROOT, "tests", "adapter", "adapter_test")
+
+
+def build_adapter_test() -> str:
+    """The C++ test program of the libvc1 adapter (integration/hip_compute.h),
+    compiled by the host compiler against include/vc1me.h and linked to the
+    built library (tests/adapter/adapter_test)."""
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}",
+          f"-I{os.path.join(HERE, 'integration')}", f"-I{os.path.dirname(ADAPTER_SRC)}", ADAPTER_SRC,
+          f"-L{HERE}", "-l:libvc1me.so", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../libvc1_amd",
+          "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-pthread", "-o", ADAPTER_BIN])
+    return ADAPTER_BIN
+
+
 def build_oracle(with_reference: bool = True) -> None:
     make = shutil.which("make") or "make"
     _run([make, "-s", "-C", os.path.join(ROOT, "oracle")])
@@ -60,6 +76,7 @@ def build_oracle(with_reference: bool = True) -> None:
 
 def main(argv: list[str]) -> int:
     build_backend()
+    build_adapter_test()
     if "--oracle" in argv:
         build_oracle()
     return 0

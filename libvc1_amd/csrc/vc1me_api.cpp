@@ -235,6 +235,7 @@ hipError_t ensure_slot(JobSlot& s, size_t mbs, size_t cells) {
 struct CachedPlane {
     const uint8_t* host = nullptr;
     int w = 0, h = 0;
+    bool by_generation = false;     // sum is the caller's generation, not a checksum
     uint64_t sum = 0;
     uint8_t* dev = nullptr;
     uint64_t last_use = 0;
@@ -323,13 +324,17 @@ int ensure_events(vc1me_ctx* ctx, size_t n) {
 }
 
 // Device copy of a host luma plane for the batch scorer; keyed by pointer,
-// size and a checksum of the whole plane, so in-place edits are never missed.
-int plane_for(vc1me_ctx* ctx, const uint8_t* host, int w, int h, uint8_t** out) {
+// size and either a checksum of the whole plane (so in-place edits are never
+// missed) or, with by_generation, the caller's generation number of it (no
+// pass over the plane: the caller changes the generation when it writes the
+// plane).
+int plane_for(vc1me_ctx* ctx, const uint8_t* host, int w, int h, uint8_t** out, bool by_generation = false,
+              uint64_t generation = 0) {
     const size_t n = static_cast<size_t>(w) * h;
-    const uint64_t sum = fnv64(host, n);
+    const uint64_t sum = by_generation ? generation : fnv64(host, n);
     ++ctx->use_clock;
     for (auto& p : ctx->planes)
-        if (p.dev && p.host == host && p.w == w && p.h == h && p.sum == sum) {
+        if (p.dev && p.host == host && p.w == w && p.h == h && p.by_generation == by_generation && p.sum == sum) {
             p.last_use = ctx->use_clock;
             *out = p.dev;
             return VC1ME_OK;
@@ -346,6 +351,7 @@ int plane_for(vc1me_ctx* ctx, const uint8_t* host, int w, int h, uint8_t** out) 
     if (!victim->dev) HIP_TRY(hipMalloc(&victim->dev, n));
     HIP_TRY(hipMemcpyAsync(victim->dev, host, n, hipMemcpyHostToDevice, ctx->stream));
     victim->host = host; victim->w = w; victim->h = h; victim->sum = sum; victim->last_use = ctx->use_clock;
+    victim->by_generation = by_generation;
     ctx->counters.frame_uploads++;
     ctx->counters.frame_upload_bytes += n;
     *out = victim->dev;
@@ -765,6 +771,12 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         }
         return VC1ME_OK;
     };
+    // end of the step: the cost report of the final field, then the summaries
+    auto finish = [&]() -> int {
+        if (ctx->cost_report) HIP_TRY(vc1me::launch_cost_final(ctx->d_jobs, n, max_mbs_all, lds, st));
+        HIP_TRY(vc1me::launch_step_end(ctx->d_jobs, n, d_sum, st));
+        return VC1ME_OK;
+    };
     // the step's launch sequence; every launch takes the job table and scalars only
     auto launch_all = [&](bool events) -> int {
         HIP_TRY(vc1me::launch_step_begin(ctx->d_jobs, n, max_mbs_all, d_cnt, kRelaxCounters, st));
@@ -782,8 +794,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
             if (int rc = diagonal_all(events)) return rc;
         }
-        HIP_TRY(vc1me::launch_step_end(ctx->d_jobs, n, d_sum, st));
-        return VC1ME_OK;
+        return finish();
     };
     ctx->timing.graph_replay = 0;
     const bool graph = ctx->use_graphs && !ctx->profiling && !after_global;
@@ -853,7 +864,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             if (int rc = diagonal_all(true)) return rc;
         }
         if (!used || launched > passes) {
-            HIP_TRY(vc1me::launch_step_end(ctx->d_jobs, n, d_sum, st));
+            if (int rc = finish()) return rc;
             HIP_TRY(hipMemcpyAsync(ctx->h_out, ctx->d_out, want_any ? out_bytes : head, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipEventRecord(ctx->events[1], st));
             HIP_TRY(hipEventSynchronize(ctx->events[1]));
@@ -1181,6 +1192,24 @@ int vc1me_score_batch(vc1me_ctx* ctx, const uint8_t* cur, const uint8_t* ref, in
     uint8_t *dc = nullptr, *dr = nullptr;
     if (int rc = plane_for(ctx, cur, width, height, &dc)) return rc;
     if (int rc = plane_for(ctx, ref, width, height, &dr)) return rc;
+    return score_on_device(ctx, dc, dr, width, height, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
+                           candidates, n, costs_out);
+}
+
+int vc1me_score_batch_gen(vc1me_ctx* ctx, const uint8_t* cur, uint64_t cur_generation, const uint8_t* ref,
+                          uint64_t ref_generation, int width, int height, int x0, int y0, int bw, int bh,
+                          int sample_stride, int bilinear, int rnd, int metric, const vc1me_candidate* candidates,
+                          size_t n, uint64_t* costs_out) {
+    if (!ctx) return fail(VC1ME_EINVAL, "null context");
+    if (!cur || !ref || width <= 0 || height <= 0 || bw <= 0 || bh <= 0 || !candidates || !n) return VC1ME_OK;
+    if (!costs_out) return fail(VC1ME_EINVAL, "null costs");
+    if (int rc = validate_score_args(width, height, n)) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (int rc = set_device(ctx)) return rc;
+    if (injected_failure()) return fail(VC1ME_EINJECTED, "failure injected by VC1ME_FAIL_AT");
+    uint8_t *dc = nullptr, *dr = nullptr;
+    if (int rc = plane_for(ctx, cur, width, height, &dc, true, cur_generation)) return rc;
+    if (int rc = plane_for(ctx, ref, width, height, &dr, true, ref_generation)) return rc;
     return score_on_device(ctx, dc, dr, width, height, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
                            candidates, n, costs_out);
 }
@@ -2397,17 +2426,33 @@ int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jo
             nn->props_cap[i] = need;
         }
     }
-    // the network runs inside the step, after the global search it is aligned by
+    // the network runs inside the step, after the global search it is aligned
+    // by: one batched forward pass per raster of the step
     auto forward = [&](hipStream_t st) -> int {
+        std::vector<bool> done(n, false);
         for (int i = 0; i < n; ++i) {
+            if (done[i]) continue;
+            std::vector<int> group;
+            for (int k = i; k < n; ++k)
+                if (!done[k] && jobs[k].cur->w == jobs[i].cur->w && jobs[k].cur->h == jobs[i].cur->h) {
+                    group.push_back(k);
+                    done[k] = true;
+                }
+            std::vector<const vc1me_frame*> cs, rs;
+            std::vector<const int32_t*> gs;
+            for (int k : group) {
+                cs.push_back(jobs[k].cur);
+                rs.push_back(jobs[k].ref);
+                gs.push_back(&ctx->slots[k].g->dx);
+            }
+            const int gn = static_cast<int>(group.size());
             int h4, w4, mbh, mbw;
-            const vc1me_frame* c = jobs[i].cur;
-            const vc1me_frame* r = jobs[i].ref;
-            const int32_t* g = &ctx->slots[i].g->dx;
-            if (int rc = nn_inputs(nn, &c, &r, 1, h4, w4, mbh, mbw, nullptr, &g)) return rc;
-            HIP_TRY(nn_forward(nn, 1, h4, w4, st));
-            HIP_TRY(hipMemcpyAsync(nn->props[i], nn->b.out, sizeof(float) * mbh * mbw * 5, hipMemcpyDeviceToDevice,
-                                   st));
+            if (int rc = nn_inputs(nn, cs.data(), rs.data(), gn, h4, w4, mbh, mbw, nullptr, gs.data())) return rc;
+            HIP_TRY(nn_forward(nn, gn, h4, w4, st));
+            const size_t per = static_cast<size_t>(mbh) * mbw * 5;
+            for (int j = 0; j < gn; ++j)
+                HIP_TRY(hipMemcpyAsync(nn->props[group[j]], nn->b.out + per * j, sizeof(float) * per,
+                                       hipMemcpyDeviceToDevice, st));
         }
         return VC1ME_OK;
     };

@@ -191,6 +191,8 @@ int relax_queue_grid(int lds_bytes, int cu_count);
 hipError_t launch_step_begin(const JobDesc* d_jobs, int njobs, int max_mbs, int32_t* counters, int ncounters,
                              hipStream_t s);
 hipError_t launch_step_end(const JobDesc* d_jobs, int njobs, StepSummary* out, hipStream_t s);
+// the cost report of the final field (jobs with cost20 set)
+hipError_t launch_cost_final(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
 int mb_kernel_lds_bytes(int local_range);
 // phase planes of the step's owning jobs (k_phase_planes_jobs)
 hipError_t launch_phase_planes_jobs(const JobDesc* d_jobs, int njobs, int max_w, int max_h, hipStream_t s);

@@ -2329,29 +2329,9 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         one = make_int2(0, 0);
         one_sad = lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, 0, 0, true, 0, 1, one_mode, 0, 0), 0);
     }
-    if (J->cost20) {
-        // judge of the learned form: 20 * (SAD + lambda * R) of the final 1-MV
-        // vector against the rate predictor (the rate-winner score of
-        // src/encoder_motion.cpp:3459-3471), 20 * lambda = lam20; work
-        // counters left as the search alone would leave them
-        const uint32_t s0 = C.ln_sad, s1 = C.ln_taps, s2 = C.ln_satd;
-        const uint64_t s = lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, one.x, one.y, true, 0, 1, one_mode, 0, 0), 0);
-        C.ln_sad = s0; C.ln_taps = s1; C.ln_satd = s2;
-        uint32_t r = 0;
-        int prx = rate_pi.px, pry = rate_pi.py;
-        if (rate_pi.hybrid) {
-            const bool use_a = modular_mv_distance(J->mvrange, one.x, one.y, rate_pi.ax, rate_pi.ay) <=
-                               modular_mv_distance(J->mvrange, one.x, one.y, rate_pi.cx, rate_pi.cy);
-            prx = use_a ? rate_pi.ax : rate_pi.cx;
-            pry = use_a ? rate_pi.ay : rate_pi.cy;
-            ++r;
-        }
-        r += mv_rate_bits(J->mvrange, one.x, one.y, prx, pry, one_mode, true);
-        if (C.lane == 0) {
-            J->cost20[pos] = 20u * s + static_cast<uint64_t>(J->lam20) * r;
-            J->one_mv[pos] = one;
-        }
-    }
+    // the cost report's vector: the final 1-MV (16x16) vector, costed by
+    // k_cost_final once the field is final
+    if (J->cost20 && C.lane == 0) J->one_mv[pos] = one;
     const uint64_t one_screen = one_sad;
     int2 outmv = one;
     // block vectors of the decision: one (1-MV), the four 8x8 results (4-MV,
@@ -2751,6 +2731,44 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
         if (threadIdx.x == 0 && !nb_equal(before, after)) queue_dependents(J, job, pos, pass + 1, Q);
         wsync();
     }
+}
+
+// The cost report of a finished step (judge of the learned form): per
+// macroblock C20 = 20 * (SAD + lambda * R) of its final 1-MV vector, SAD by
+// block_sad_motion at that vector and R against predictor_info_mixed_1mv of
+// the final field with the HYBRIDPRED bit (the rate-winner score of
+// src/encoder_motion.cpp:3459-3471, 20 * lambda = lam20).  Evaluated on the
+// final field, so that a schedule that stops before the raster-order fixed
+// point is judged by the field it returns; at the fixed point it is the score
+// the last analysis saw.  One workgroup per macroblock.
+__global__ void __launch_bounds__(64) k_cost_final(const JobDesc* __restrict__ jobs) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const JobDesc* J = &jobs[blockIdx.y];
+    const int pos = static_cast<int>(blockIdx.x);
+    if (!J->cost20 || J->range == 0 || J->motion_check || pos >= J->mbw * J->mbh) return;
+    Ctx C;
+    init_ctx(C, J, pos % J->mbw, pos / J->mbw, smem);
+    for (int i = C.lane; i < 256; i += kWave) {
+        const int x = i & 15, y = i >> 4;
+        C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
+    }
+    nb_load(C);
+    wsync();
+    const Pred rate_pi = predictor_mixed_1mv(C);
+    const int one_mode = J->mode == 3 ? 0 : J->mode;
+    const int2 one = J->one_mv[pos];
+    const uint64_t s = lane_read64(qpel_eval(C, C.x0, C.y0, 16, 16, one.x, one.y, true, 0, 1, one_mode, 0, 0), 0);
+    uint32_t r = 0;
+    int prx = rate_pi.px, pry = rate_pi.py;
+    if (rate_pi.hybrid) {
+        const bool use_a = modular_mv_distance(J->mvrange, one.x, one.y, rate_pi.ax, rate_pi.ay) <=
+                           modular_mv_distance(J->mvrange, one.x, one.y, rate_pi.cx, rate_pi.cy);
+        prx = use_a ? rate_pi.ax : rate_pi.cx;
+        pry = use_a ? rate_pi.ay : rate_pi.cy;
+        ++r;
+    }
+    r += mv_rate_bits(J->mvrange, one.x, one.y, prx, pry, one_mode, true);
+    if (C.lane == 0) J->cost20[pos] = 20u * s + static_cast<uint64_t>(J->lam20) * r;
 }
 
 // Start of a step: clears the per-job error flag, op counters, extended
@@ -3882,6 +3900,12 @@ hipError_t launch_step_begin(const JobDesc* d_jobs, int njobs, int max_mbs, int3
     if (njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_step_begin, dim3(((max_mbs > 0 ? max_mbs : 1) + 255) / 256, njobs), dim3(256), 0, s, d_jobs,
                        counters, ncounters);
+    return hipGetLastError();
+}
+
+hipError_t launch_cost_final(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s) {
+    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cost_final, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs);
     return hipGetLastError();
 }
 

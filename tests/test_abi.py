@@ -56,3 +56,19 @@ def test_create_fails_loudly_without_device():
     with pytest.raises(backend.BackendError) as e:
         backend.Backend(0)
     assert e.value.code == abi.VC1ME_EDEVICE
+
+
+def test_adapter_compiles_against_the_header():
+    """The libvc1 adapter (libvc1_amd/integration/hip_compute.h) and its test
+    program compile as C++17 against include/vc1me.h alone."""
+    import shutil
+    import subprocess
+    from libvc1_amd import build
+    cxx = shutil.which("g++")
+    if not cxx:
+        pytest.skip("no host C++ compiler")
+    r = subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(ROOT, 'libvc1_amd', 'integration')}",
+                        f"-I{os.path.join(ROOT, 'tests', 'adapter')}", build.ADAPTER_SRC],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

@@ -54,6 +54,25 @@ def test_score_batch_matches_kernel_law(backend):
                         assert np.array_equal(got, want)
 
 
+def test_score_batch_generations(backend):
+    """vc1me_score_batch_gen keys the planes' device copies by the caller's
+    generation numbers: the kernel law as before; a plane written in place
+    under a new generation is scored afresh, and the checksum-keyed
+    vc1me_score_batch notices the write by itself."""
+    rng = np.random.default_rng(12)
+    cur = rng.integers(0, 256, size=(96, 160), dtype=np.uint8)
+    ref = rng.integers(0, 256, size=(96, 160), dtype=np.uint8)
+    cands = rng.integers(-40, 41, size=(36, 2)).astype(np.int32)
+    _say("[case] score_batch_gen")
+    want = oracle_score_batch(cur, ref, 32, 16, 16, 16, cands, 1, 0, 0, 0)
+    assert np.array_equal(backend.score_batch_gen(cur, 1, ref, 1, 32, 16, 16, 16, cands), want)
+    ref[:, :] = 255 - ref                       # written in place
+    want2 = oracle_score_batch(cur, ref, 32, 16, 16, 16, cands, 1, 0, 0, 0)
+    assert not np.array_equal(want, want2)
+    assert np.array_equal(backend.score_batch_gen(cur, 1, ref, 2, 32, 16, 16, 16, cands), want2)
+    assert np.array_equal(backend.score_batch(cur, ref, 32, 16, 16, 16, cands), want2)
+
+
 def test_score_batch_empty_and_limits(backend):
     cur = np.zeros((32, 32), np.uint8)
     out = backend.score_batch(cur, cur, 0, 0, 16, 16, np.zeros((0, 2), np.int32))
