@@ -23,7 +23,7 @@ ARCH = "gfx950"
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    "-ffp-contract=off",
+    "-ffp-contract=off", "-mllvm", "-disable-machine-licm",
     # packed-FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -269,7 +269,12 @@ struct vc1me_ctx {
     int2* d_cands = nullptr;
     uint32_t* d_costs = nullptr;
     size_t cand_cap = 0;
-    std::vector<uint32_t> h_costs;
+    // pinned staging of the scorer's candidates and costs (the allocating
+    // forms grow it; the allocation-free form uses a fixed one made at
+    // creation, as large as its scratch array)
+    uint8_t* h_stage = nullptr;
+    size_t stage_cap = 0;
+    uint8_t* h_noalloc = nullptr;
     int cu_count = 256;
     // proposals handed in by the host (vc1me_analyze_step_proposals)
     float* d_props = nullptr;
@@ -370,15 +375,26 @@ int score_on_device(vc1me_ctx* ctx, const uint8_t* dcur, const uint8_t* dref, in
         HIP_TRY(hipMalloc(&ctx->d_costs, cap * sizeof(uint32_t)));
         ctx->cand_cap = cap;
     }
+    const size_t stage = n * (sizeof(int2) + sizeof(uint32_t));
+    if (ctx->stage_cap < stage) {
+        if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+        ctx->h_stage = nullptr;
+        ctx->stage_cap = 0;
+        const size_t cap = std::max(stage, size_t(4096) * (sizeof(int2) + sizeof(uint32_t)));
+        HIP_TRY(hipHostMalloc(&ctx->h_stage, cap, hipHostMallocDefault));
+        ctx->stage_cap = cap;
+    }
+    int2* h_c = reinterpret_cast<int2*>(ctx->h_stage);
+    uint32_t* h_k = reinterpret_cast<uint32_t*>(ctx->h_stage + n * sizeof(int2));
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(hipMemcpyAsync(ctx->d_cands, candidates, n * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
+    std::memcpy(h_c, candidates, n * sizeof(int2));
+    HIP_TRY(hipMemcpyAsync(ctx->d_cands, h_c, n * sizeof(int2), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(vc1me::launch_score_batch(dcur, dref, width, height, x0, y0, bw, bh, sample_stride, bilinear, rnd, metric,
                                       ctx->d_cands, static_cast<int>(n), ctx->d_costs, ctx->stream));
-    ctx->h_costs.resize(n);
-    HIP_TRY(hipMemcpyAsync(ctx->h_costs.data(), ctx->d_costs, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(h_k, ctx->d_costs, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const auto t1 = std::chrono::steady_clock::now();
-    for (size_t i = 0; i < n; ++i) costs_out[i] = ctx->h_costs[i];
+    for (size_t i = 0; i < n; ++i) costs_out[i] = h_k[i];
     ctx->counters.dispatches++;
     ctx->counters.candidates += n;
     ctx->counters.submit_wait_nanoseconds +=
@@ -1149,6 +1165,8 @@ int vc1me_create(int device, vc1me_ctx** out) {
     // the allocation-free scorer's scratch array on this device (looked up
     // once per device, with the device current)
     ctx->scratch = scratch_for_device(ctx->device, &ctx->scratch_error);
+    if (hipHostMalloc(&ctx->h_noalloc, kNoAllocScratchBytes, hipHostMallocDefault) != hipSuccess)
+        ctx->h_noalloc = nullptr;
     *out = ctx;
     return VC1ME_OK;
 }
@@ -1165,6 +1183,8 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     if (ctx->d_cands) (void)hipFree(ctx->d_cands);
     if (ctx->d_costs) (void)hipFree(ctx->d_costs);
     if (ctx->d_props) (void)hipFree(ctx->d_props);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+    if (ctx->h_noalloc) (void)hipHostFree(ctx->h_noalloc);
     if (ctx->d_relax_wl) (void)hipFree(ctx->d_relax_wl);
     if (ctx->d_out) (void)hipFree(ctx->d_out);
     if (ctx->h_out) (void)hipHostFree(ctx->h_out);
@@ -1293,13 +1313,15 @@ int vc1me_score_batch_frames_noalloc(vc1me_ctx* ctx, const vc1me_frame* cur, con
     std::lock_guard<std::mutex> dev_lock(g_scratch[ctx->device].mu);
     int2* d_cands = static_cast<int2*>(ctx->scratch);
     uint32_t* d_costs = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(ctx->scratch) + kNoAllocCandBytes);
-    // the 32-bit costs land in the upper half of costs_out and are widened
-    // in place from the front: entry i is read before any write reaches it
-    // (8i + 8 <= 4n + 4i + 4 for every i < n)
-    uint32_t* staged = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(costs_out) + 4 * n);
+    // candidates and costs pass through the context's fixed pinned staging
+    // (made at creation, the size of the scratch array)
+    if (!ctx->h_noalloc) return fail(VC1ME_EDEVICE, "pinned staging unavailable");
+    int2* h_c = reinterpret_cast<int2*>(ctx->h_noalloc);
+    uint32_t* staged = reinterpret_cast<uint32_t*>(ctx->h_noalloc + kNoAllocCandBytes);
     hipStream_t st = ctx->stream;
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(hipMemcpyAsync(d_cands, candidates, n * sizeof(int2), hipMemcpyHostToDevice, st));
+    std::memcpy(h_c, candidates, n * sizeof(int2));
+    HIP_TRY(hipMemcpyAsync(d_cands, h_c, n * sizeof(int2), hipMemcpyHostToDevice, st));
     HIP_TRY(vc1me::launch_score_batch(cur->y, ref->y, cur->w, cur->h, x0, y0, bw, bh, sample_stride, bilinear, rnd,
                                       metric, d_cands, static_cast<int>(n), d_costs, st));
     HIP_TRY(hipMemcpyAsync(staged, d_costs, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));

@@ -2158,8 +2158,46 @@ __device__ __forceinline__ int median4(int a, int b, int c, int d) {
 struct NbState {
     int2 mv;
     int2 b[4];
-    uint32_t fp;    // intra flag | propagated flag << 8
+    uint32_t fp;        // intra flag | propagated flag << 8
+    uint32_t changed;   // (lane 0) the analysis changed the macroblock's state
 };
+
+__device__ __forceinline__ NbState nb_state(const JobDesc* J, int pos) {
+    NbState s;
+    s.mv = J->mvs[pos];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s.b[k] = J->bmv[pos * 4 + k];
+    s.fp = static_cast<uint32_t>(J->flags[pos] & kFlagIntra) | (static_cast<uint32_t>(J->prop[pos]) << 8);
+    s.changed = 0;
+    return s;
+}
+
+__device__ __forceinline__ bool nb_equal(const NbState& a, const NbState& b) {
+    bool eq = a.mv.x == b.mv.x && a.mv.y == b.mv.y && a.fp == b.fp;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) eq = eq && a.b[k].x == b.b[k].x && a.b[k].y == b.b[k].y;
+    return eq;
+}
+
+// Lane 0 writes the macroblock's neighbour-visible state (vectors, flags,
+// propagated flag) and notes in st.changed whether it differs from the state
+// the macroblock held; the relaxation queues the dependents of a changed
+// macroblock.  No other workgroup writes this macroblock's state while it is
+// analysed, so the state read here is the one the analysis started from (and
+// no register holds it through the analysis).
+__device__ __forceinline__ void commit_state(const Ctx& C, NbState& st, uint8_t flags, uint8_t prop) {
+    st.changed = 0;
+    if (C.lane == 0) {
+        const JobDesc* J = C.J;
+        const int pos = C.pos;
+        st.changed = nb_equal(nb_state(J, pos), st) ? 0u : 1u;
+        J->mvs[pos] = st.mv;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = st.b[k];
+        J->flags[pos] = flags;
+        J->prop[pos] = prop;
+    }
+}
 
 // kGuess: the relaxation's starting state only (never a result): the
 // analysis without the 4-MV stage and the extended long-range search, the
@@ -2195,14 +2233,14 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         // SAD are both this SAD (src/encoder_motion.cpp:741-748)
         const uint64_t zs = key_cost(z), zpx = active_pixels(C);
         const bool zgood = J->motion_check && zpx && (zs <= zpx * 12u || (zs > 0 && zs * 10u <= zs * 9u));
+        st.mv = make_int2(0, 0);
+        for (int k = 0; k < 4; ++k) st.b[k] = make_int2(0, 0);
+        st.fp = 0;
+        commit_state(C, st, zgood ? kFlagGoodMatch : 0, 0);
         if (C.lane == 0) {
             J->screen[pos] = key_cost(z);
             J->chosen[pos] = key_cost(z);
             J->temporal[pos] = key_cost(z);
-            J->mvs[pos] = make_int2(0, 0);
-            for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = make_int2(0, 0);
-            J->flags[pos] = zgood ? kFlagGoodMatch : 0;
-            J->prop[pos] = 0;
             J->ratio[pos] = -1.0;
             J->dl_mae[pos] = -1.0;
             J->dc_mae[pos] = -1.0;
@@ -2210,9 +2248,6 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             uint64_t* o = J->mbops + static_cast<size_t>(pos) * 5;
             o[0] = C.n_sad; o[1] = 0; o[2] = 0; o[3] = 0; o[4] = 0;
         }
-        st.mv = make_int2(0, 0);
-        for (int k = 0; k < 4; ++k) st.b[k] = make_int2(0, 0);
-        st.fp = 0;
         return;
     }
 
@@ -2228,7 +2263,6 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         const Bounds qb = bounds_qpel(range);
         const int2 sv = J->seed_mvs[pos];
         const int sx = clampi(sv.x, qb.xmin, qb.xmax), sy = clampi(sv.y, qb.ymin, qb.ymax);
-        if (C.lane == 0) J->prop[pos] = 0;
         if (J->subpel) pw_load(C, C.x0 + floor_div4(sx) - kPwMargin, C.y0 + floor_div4(sy) - kPwMargin,
                                mode_bilinear(one_mode));
         SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true};
@@ -2290,7 +2324,6 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             if (ext.dx != im.dx || ext.dy != im.dy) { im = ext; selected_distant = true; }
         }
         prop_out = selected_distant ? 1 : 0;
-        if (C.lane == 0) J->prop[pos] = prop_out;
         if (J->motion_check) {
             // the motion-failure check (src/encoder_motion.cpp:731-748): the
             // SAD-only refinement, then the usable-match test against 12 per
@@ -2302,10 +2335,11 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             const bool good = px && (refined <= px * 12u || (zero > 0 && refined * 10u <= zero * 9u));
             C.n_sad += wave_sum32(C.ln_sad);
             C.n_taps += wave_sum32(C.ln_taps);
+            st.mv = r;
+            for (int k = 0; k < 4; ++k) st.b[k] = r;
+            st.fp = static_cast<uint32_t>(prop_out) << 8;
+            commit_state(C, st, good ? kFlagGoodMatch : 0, prop_out);
             if (C.lane == 0) {
-                J->mvs[pos] = r;
-                for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = r;
-                J->flags[pos] = good ? kFlagGoodMatch : 0;
                 J->screen[pos] = refined;
                 J->chosen[pos] = refined;
                 J->temporal[pos] = refined;
@@ -2316,9 +2350,6 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
                 uint64_t* o = J->mbops + static_cast<size_t>(pos) * 5;
                 o[0] = C.n_sad; o[1] = C.n_taps; o[2] = 0; o[3] = C.n_sig; o[4] = 0;
             }
-            st.mv = r;
-            for (int k = 0; k < 4; ++k) st.b[k] = r;
-            st.fp = static_cast<uint32_t>(prop_out) << 8;
             return;
         }
         if (J->subpel) pw_load(C, C.x0 + im.dx - kPwMargin, C.y0 + im.dy - kPwMargin, mode_bilinear(one_mode));
@@ -2481,10 +2512,11 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     C.n_sad += wave_sum32(C.ln_sad);
     C.n_taps += wave_sum32(C.ln_taps);
     C.n_satd += wave_sum32(C.ln_satd);
+    st.mv = outmv;
+    st.b[0] = ob0; st.b[1] = ob1; st.b[2] = ob2; st.b[3] = ob3;
+    st.fp = static_cast<uint32_t>(flags & kFlagIntra) | (static_cast<uint32_t>(prop_out) << 8);
+    commit_state(C, st, flags, prop_out);
     if (C.lane == 0) {
-        J->mvs[pos] = outmv;
-        J->bmv[pos * 4 + 0] = ob0; J->bmv[pos * 4 + 1] = ob1; J->bmv[pos * 4 + 2] = ob2; J->bmv[pos * 4 + 3] = ob3;
-        J->flags[pos] = flags;
         J->screen[pos] = temporal_screen;
         J->ratio[pos] = ratio;
         J->dl_mae[pos] = dl_mae;
@@ -2497,9 +2529,6 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         uint64_t* o = J->mbops + static_cast<size_t>(pos) * 5;
         o[0] = C.n_sad; o[1] = C.n_taps; o[2] = C.n_satd; o[3] = C.n_sig; o[4] = was_seeded ? 1 : 0;
     }
-    st.mv = outmv;
-    st.b[0] = ob0; st.b[1] = ob1; st.b[2] = ob2; st.b[3] = ob3;
-    st.fp = static_cast<uint32_t>(flags & kFlagIntra) | (static_cast<uint32_t>(prop_out) << 8);
 }
 
 __device__ __forceinline__ void init_ctx(Ctx& C, const JobDesc* J, int mx, int my, uint8_t* smem) {
@@ -2510,7 +2539,13 @@ __device__ __forceinline__ void init_ctx(Ctx& C, const JobDesc* J, int mx, int m
     C.aw = imax(0, imin(16, J->W - C.x0));
     C.ah = imax(0, imin(16, J->H - C.y0));
     C.pos = my * J->mbw + mx;
-    C.lane = threadIdx.x;
+    // the lane index passes through an empty asm: in the work-queue pass the
+    // analysis runs in a loop, and values derived from the lane alone would
+    // otherwise be hoisted out of it and held across the whole analysis
+    // (the kernel spilled)
+    int lane = threadIdx.x;
+    __asm__ volatile("" : "+v"(lane));
+    C.lane = lane;
     C.L = carve(smem, J->local_range);
     C.winCap = win_capacity(J->local_range);
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;
@@ -2627,21 +2662,6 @@ __global__ void __launch_bounds__(64) k_mb_diagonal(const JobDesc* __restrict__ 
 // rewritten in the same pass may see either value; the rewrite queues the
 // reader again, so the race cannot leak into the result.
 
-__device__ __forceinline__ NbState nb_state(const JobDesc* J, int pos) {
-    NbState s;
-    s.mv = J->mvs[pos];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) s.b[k] = J->bmv[pos * 4 + k];
-    s.fp = static_cast<uint32_t>(J->flags[pos] & kFlagIntra) | (static_cast<uint32_t>(J->prop[pos]) << 8);
-    return s;
-}
-
-__device__ __forceinline__ bool nb_equal(const NbState& a, const NbState& b) {
-    bool eq = a.mv.x == b.mv.x && a.mv.y == b.mv.y && a.fp == b.fp;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) eq = eq && a.b[k].x == b.b[k].x && a.b[k].y == b.b[k].y;
-    return eq;
-}
 
 // starting state: every macroblock at the frame's global vector, no intra,
 // nothing propagated; no macroblock queued yet
@@ -2701,9 +2721,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const int mbw = J->mbw;
     const int pos = static_cast<int>(blockIdx.x);
     if (pos >= mbw * J->mbh) return;
-    const NbState before = nb_state(J, pos);
     const NbState after = run_mb(J, pos % mbw, pos / mbw, smem);
-    if (threadIdx.x == 0 && !nb_equal(before, after)) queue_dependents(J, job, pos, 1, Q);
+    if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, 1, Q);
 }
 
 // Pass p > 0 as a work queue: a grid of resident workgroups takes entries of
@@ -2718,18 +2737,22 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
     const int n = imin(Q.cnt[pass], Q.cap);
     if (n == 0) return;
     const int32_t* list = Q.wl[pass & 1];
-    for (;;) {
-        int t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(&Q.tk[pass], 1);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
-        if (t >= n) return;
-        const int code = list[t];
+    // one wave per workgroup; the barrier at the end of each entry keeps the
+    // compiler from moving one macroblock's LDS accesses past the next one's
+    for (int t = __builtin_amdgcn_readfirstlane(threadIdx.x == 0 ? atomicAdd(&Q.tk[pass], 1) : 0); t < n;) {
+        // the entry is the same for every lane: read it into a scalar register,
+        // so the job and macroblock of the analysis are known to be uniform
+        // (loaded into vector registers, every job field the analysis reads
+        // became per-lane work and the kernel spilled)
+        const int code = __builtin_amdgcn_readfirstlane(list[t]);
         const int job = code >> 24, pos = code & 0xffffff;
         const JobDesc* J = &jobs[job];
-        const NbState before = nb_state(J, pos);
         const NbState after = run_mb(J, pos % J->mbw, pos / J->mbw, smem);
-        if (threadIdx.x == 0 && !nb_equal(before, after)) queue_dependents(J, job, pos, pass + 1, Q);
+        if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, pass + 1, Q);
         wsync();
+        int nt = 0;
+        if (threadIdx.x == 0) nt = atomicAdd(&Q.tk[pass], 1);
+        t = __builtin_amdgcn_readfirstlane(__shfl(nt, 0));
     }
 }
 
