@@ -18,6 +18,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 using vc1me::GlobalState;
@@ -1452,6 +1453,26 @@ vc1me::IntensityLuts intensity_luts(const vc1me_intensity& ic) {
 
 namespace {
 
+// The fade line cur ~ gain * ref + offset fitted by least squares to the
+// sample moments m = {sum ref, sum cur, sum ref^2, sum ref*cur} of `count`
+// samples; a singular system keeps unit gain and takes the mean difference
+// as the offset.  The long-double expressions are evaluated in the order
+// that fixes the rounding of the coded parameters.
+struct FadeLine {
+    long double gain, offset;
+};
+
+FadeLine fit_fade_line(const unsigned long long m[4], int count) {
+    const long double k = static_cast<long double>(count);
+    const long double s_ref = static_cast<long double>(m[0]), s_cur = static_cast<long double>(m[1]);
+    const long double det = k * static_cast<long double>(m[2]) - s_ref * s_ref;
+    if (std::fabs(det) > 1e-9L) {
+        const long double gain = (k * static_cast<long double>(m[3]) - s_ref * s_cur) / det;
+        return {gain, (s_cur - gain * s_ref) / k};
+    }
+    return {1.0L, (s_cur - s_ref) / k};
+}
+
 // estimate_intensity_comp on resident frames (context lock held)
 int estimate_ic(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, const int32_t* mvs,
                 vc1me_intensity* out) {
@@ -1486,43 +1507,35 @@ int estimate_ic(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, 
     unsigned long long sums[4] = {0, 0, 0, 0};
     for (int blk = 0; blk < sb; ++blk)
         for (int k = 0; k < 4; ++k) sums[k] += part[4 * blk + k];
-    // the least-squares fit in long double on the exact integer sums
-    // (src/encoder_motion_compensation.cpp:319-328)
-    const long double ln = static_cast<long double>(n), sx = static_cast<long double>(sums[0]),
-                      sy = static_cast<long double>(sums[1]);
-    const long double denom = ln * static_cast<long double>(sums[2]) - sx * sx;
-    long double a = 1.0L, b = 0.0L;
-    if (std::fabs(denom) > 1e-9L) {
-        a = (ln * static_cast<long double>(sums[3]) - sx * sy) / denom;
-        b = (sy - a * sx) / ln;
-    } else {
-        b = (sy - sx) / ln;
-    }
-    const int ls = std::clamp(static_cast<int>(std::llround(a * 64.0L - 32.0L)), 1, 63);
-    const int sh = std::clamp(static_cast<int>(std::llround(b)), -32, 31);
-    const int l0 = std::max(1, ls - 3), l1 = std::min(63, ls + 3);
-    const int z0 = std::max(-32, sh - 3), z1 = std::min(31, sh + 3);
+    // cur ~ gain * ref + offset over the samples, in long double on the exact
+    // integer moments (src/encoder_motion_compensation.cpp:319-328), coded as
+    // LUMSCALE/LUMSHIFT and refined over the 7 x 7 codes around them
+    const FadeLine line = fit_fade_line(sums, n);
+    const int scale0 = std::clamp(static_cast<int>(std::llround(line.gain * 64.0L - 32.0L)), 1, 63);
+    const int shift0 = std::clamp(static_cast<int>(std::llround(line.offset)), -32, 31);
+    const int l0 = std::max(1, scale0 - 3), l1 = std::min(63, scale0 + 3);
+    const int z0 = std::max(-32, shift0 - 3), z1 = std::min(31, shift0 + 3);
     HIP_TRY(vc1me::launch_ic_sads(d_smp, n, l0, z0, d_sad, st));
     HIP_TRY(hipMemcpyAsync(part.data(), d_sad, 49 * sizeof(unsigned long long) * gb, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     unsigned long long sads[49] = {};
     for (int blk = 0; blk < gb; ++blk)
         for (int k = 0; k < 49; ++k) sads[k] += part[49 * blk + k];
-    // smallest SAD, ties to the smaller |scale - 32| + |shift|, then the
-    // first in (scale, shift) order (:337-345)
-    uint64_t best = UINT64_MAX;
-    int best_ls = 32, best_sh = 0;
+    // the code pair of least sampled SAD; equal SADs go to the pair nearer
+    // the identity (scale code 32, shift 0) in L1 distance, then to the first
+    // in (scale, shift) scan order (:337-345)
+    std::tuple<uint64_t, int, int> least{UINT64_MAX, 0, 0};
+    int scale = 32, shift = 0;
     for (int l = l0; l <= l1; ++l)
         for (int z = z0; z <= z1; ++z) {
-            const uint64_t sad = sads[(l - l0) * 7 + (z - z0)];
-            if (sad < best || (sad == best && std::abs(l - 32) + std::abs(z) < std::abs(best_ls - 32) + std::abs(best_sh))) {
-                best = sad; best_ls = l; best_sh = z;
-            }
+            const int scan = (l - l0) * 7 + (z - z0);
+            const std::tuple<uint64_t, int, int> key{sads[scan], std::abs(l - 32) + std::abs(z), scan};
+            if (key < least) { least = key; scale = l; shift = z; }
         }
-    if (best_ls == 32 && best_sh == 0) return VC1ME_OK;
+    if (scale == 32 && shift == 0) return VC1ME_OK;
     out->enabled = 1;
-    out->lumscale = best_ls;
-    out->lumshift = best_sh < 0 ? best_sh + 64 : best_sh;
+    out->lumscale = scale;
+    out->lumshift = shift < 0 ? shift + 64 : shift;
     return VC1ME_OK;
 }
 
@@ -2016,6 +2029,46 @@ int global_only(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref, 
     return VC1ME_OK;
 }
 
+// Whether the interior of a 1-MV quarter-pel field (macroblocks off the
+// picture border) moves as one translation plus an isotropic zoom about the
+// centroid of their centres (src/encoder_motion.cpp:637-673): the zoom is
+// the least-squares slope of the centred displacements (pixels) against the
+// centred positions; the field is coherent when at least 60% of at least 12
+// interior macroblocks lie within 2.5 px of the fitted motion and the zoom's
+// magnitude is in [0.005, 0.08].  Sums run in raster order in long double,
+// the order that fixes their rounding.
+bool zoom_coherent(const int32_t* mv, int mbw, int mbh) {
+    struct Sample {
+        long double x, y, u, v;   // centre (pixels) and displacement (pixels)
+    };
+    std::vector<Sample> in;
+    for (int by = 1; by < mbh - 1; ++by)
+        for (int bx = 1; bx < mbw - 1; ++bx) {
+            const size_t at = 2 * (static_cast<size_t>(by) * mbw + bx);
+            in.push_back({bx * 16.0L + 8.0L, by * 16.0L + 8.0L, mv[at] / 4.0L, mv[at + 1] / 4.0L});
+        }
+    if (in.size() < 12) return false;
+    Sample mean{0.0L, 0.0L, 0.0L, 0.0L};
+    for (const Sample& e : in) { mean.x += e.x; mean.y += e.y; mean.u += e.u; mean.v += e.v; }
+    const size_t count = in.size();
+    mean.x /= count; mean.y /= count; mean.u /= count; mean.v /= count;
+    long double cross = 0.0L, spread = 0.0L;
+    for (const Sample& e : in) {
+        const long double x = e.x - mean.x, y = e.y - mean.y;
+        cross += x * (e.u - mean.u) + y * (e.v - mean.v);
+        spread += x * x + y * y;
+    }
+    const long double zoom = spread > 0.0L ? cross / spread : 0.0L;
+    size_t agree = 0;
+    for (const Sample& e : in) {
+        const long double x = e.x - mean.x, y = e.y - mean.y;
+        const long double ru = e.u - (mean.u + zoom * x), rv = e.v - (mean.v + zoom * y);
+        agree += ru * ru + rv * rv <= 6.25L ? 1 : 0;
+    }
+    const long double mag = std::fabs(zoom);
+    return agree * 5 >= count * 3 && mag >= 0.005L && mag <= 0.08L;
+}
+
 }  // namespace
 
 int vc1me_scene_change_score(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me_frame* ref,
@@ -2063,40 +2116,7 @@ int vc1me_scene_change_score(vc1me_ctx* ctx, const vc1me_frame* cur, const vc1me
         const vc1me_job job{cur, ref, &q, &local};
         if (int rc = run_jobs(ctx, &job, 1)) return rc;
         const double lm = local.mean_abs_residual;
-        long double mx = 0.0L, my = 0.0L, mdx = 0.0L, mdy = 0.0L;
-        size_t samples = 0;
-        for (int by = 1; by < mbh - 1; ++by)
-            for (int bx = 1; bx < mbw - 1; ++bx) {
-                const size_t pos = static_cast<size_t>(by) * mbw + bx;
-                mx += bx * 16.0L + 8.0L; my += by * 16.0L + 8.0L;
-                mdx += mv[2 * pos] / 4.0L; mdy += mv[2 * pos + 1] / 4.0L;
-                ++samples;
-            }
-        bool coherent = false;
-        if (samples >= 12) {
-            mx /= samples; my /= samples; mdx /= samples; mdy /= samples;
-            long double num = 0.0L, den = 0.0L;
-            for (int by = 1; by < mbh - 1; ++by)
-                for (int bx = 1; bx < mbw - 1; ++bx) {
-                    const size_t pos = static_cast<size_t>(by) * mbw + bx;
-                    const long double x = bx * 16.0L + 8.0L - mx, y = by * 16.0L + 8.0L - my;
-                    const long double dx = mv[2 * pos] / 4.0L - mdx, dy = mv[2 * pos + 1] / 4.0L - mdy;
-                    num += x * dx + y * dy;
-                    den += x * x + y * y;
-                }
-            const long double z = den > 0.0L ? num / den : 0.0L;
-            size_t inliers = 0;
-            for (int by = 1; by < mbh - 1; ++by)
-                for (int bx = 1; bx < mbw - 1; ++bx) {
-                    const size_t pos = static_cast<size_t>(by) * mbw + bx;
-                    const long double x = bx * 16.0L + 8.0L - mx, y = by * 16.0L + 8.0L - my;
-                    const long double ex = mv[2 * pos] / 4.0L - (mdx + z * x);
-                    const long double ey = mv[2 * pos + 1] / 4.0L - (mdy + z * y);
-                    if (ex * ex + ey * ey <= 6.25L) ++inliers;
-                }
-            const long double az = std::fabs(z);
-            coherent = inliers * 5 >= samples * 3 && az >= 0.005L && az <= 0.08L;
-        }
+        const bool coherent = zoom_coherent(mv.data(), mbw, mbh);
         if (coherent && lm < best * 0.72 && lm < scene_threshold * 0.86) best = lm;
     }
     *score = best;
