@@ -123,6 +123,41 @@ __device__ __forceinline__ uint64_t lane_shfl64(uint64_t v, int src) {
     return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
+// An entry of a winner search, ordered lexicographically by (cls, s, a, b).
+struct WinKey {
+    uint32_t cls;
+    unsigned __int128 s;
+    uint64_t a, b;
+};
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+    const uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+    return (static_cast<uint64_t>(dpp_mov<CTRL, 0xf>(hi, hi)) << 32) | dpp_mov<CTRL, 0xf>(lo, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ void win_step(WinKey& k) {
+    WinKey p;
+    p.cls = dpp_mov<CTRL, 0xf>(k.cls, k.cls);
+    p.s = (static_cast<unsigned __int128>(dpp64<CTRL>(static_cast<uint64_t>(k.s >> 64))) << 64) |
+          dpp64<CTRL>(static_cast<uint64_t>(k.s));
+    p.a = dpp64<CTRL>(k.a);
+    p.b = dpp64<CTRL>(k.b);
+    bool less;
+    if (p.cls != k.cls) less = p.cls < k.cls;
+    else if (p.s != k.s) less = p.s < k.s;
+    else if (p.a != k.a) less = p.a < k.a;
+    else less = p.b < k.b;
+    if (less) k = p;
+}
+// The least entry of lanes 0..7, in each of those lanes: a tournament over
+// lane pairs, quads and the two halves of the eight (every lane active).
+__device__ __forceinline__ WinKey lanes8_min(WinKey k) {
+    win_step<kDppXor1>(k);
+    win_step<kDppXor2>(k);
+    win_step<kDppHalfMirror>(k);
+    return k;
+}
+
 // Global-memory view of a plane pointer (lets loads be scheduled past LDS stores).
 typedef const __attribute__((address_space(1))) uint8_t* gptr_u8;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -2094,24 +2129,15 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
         rate += mv_rate_bits(J->mvrange, xq, yq, prx, pry, sp.mode, sp.more);
         score = x87_score(sad, rate, lm, lk);
     }
-    // the SAD+lambda*R winner: the candidate no other one precedes
-    int rw;
-    {
-        int before = 0;
-        for (int k = 0; k < n; ++k) {
-            const unsigned __int128 bs = lane_read128(score, k);
-            const uint64_t bsad = lane_read64(sad, k);
-            const int bx = __builtin_amdgcn_readlane(xq, k), by = __builtin_amdgcn_readlane(yq, k);
-            bool less;
-            if (bs != score) less = bs < score;
-            else if (bsad != sad) less = bsad < sad;
-            else less = tie_less(bx, by, xq, yq);
-            before += less ? 1 : 0;
-        }
-        rw = __ffsll(static_cast<long long>(__ballot(in && before == 0))) - 1;
-    }
-    int2 result = make_int2(__builtin_amdgcn_readlane(xq, rw), __builtin_amdgcn_readlane(yq, rw));
+    // the SAD+lambda*R winner: the least (score, SAD, tie order) entry of the
+    // shortlist (at most eight, lanes 0..7; lanes past it rank last).  key_q
+    // orders (SAD, tie order) exactly: SADs are below 2^21 except the
+    // hidden-padding veto, and the vector fields cover the quarter-pel bounds.
+    const WinKey w1 = lanes8_min(WinKey{in ? 0u : 1u, score, key_q(sad, xq, yq), 0});
+    const uint64_t k1 = lane_read64(w1.a, 0);
+    int2 result = make_int2(keyq_x(k1), keyq_y(k1));
     if (J->quality == 2) {
+        const int rw = __ffsll(static_cast<long long>(__ballot(in && xq == result.x && yq == result.y))) - 1;
         const uint64_t rws = lane_read64(sad, rw);
         const uint64_t pct3 = (rws / 100) * 3 + (((rws % 100) * 3 + 99) / 100);
         const uint64_t slack = pct3 > 4 ? pct3 : 4;
@@ -2119,22 +2145,11 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
         const bool el = in && sad < UINT64_MAX / 8 && sad <= limit;
         const uint64_t satd = qpel_eval(C, sp.bx0, sp.by0, sp.bw, sp.bh, xq, yq, el, 0, n, sp.mode, 1, kBadSad);
         const unsigned __int128 s2 = x87_score(satd, rate, lm, lk);
-        int before = 0;
-        for (int k = 0; k < n; ++k) {
-            const bool bel = ((__ballot(el) >> k) & 1) != 0;
-            const unsigned __int128 bs = lane_read128(s2, k);
-            const uint64_t bsatd = lane_read64(satd, k), bsad = lane_read64(sad, k);
-            const int bx = __builtin_amdgcn_readlane(xq, k), by = __builtin_amdgcn_readlane(yq, k);
-            bool less;
-            if (bel != el) less = bel;
-            else if (bs != s2) less = bs < s2;
-            else if (bsatd != satd) less = bsatd < satd;
-            else if (bsad != sad) less = bsad < sad;
-            else less = tie_less(bx, by, xq, yq);
-            before += less ? 1 : 0;
-        }
-        const int w = __ffsll(static_cast<long long>(__ballot(in && before == 0))) - 1;
-        result = make_int2(__builtin_amdgcn_readlane(xq, w), __builtin_amdgcn_readlane(yq, w));
+        // the SATD winner: eligible entries first, then (score, SATD, SAD,
+        // tie order)
+        const WinKey w2 = lanes8_min(WinKey{in ? (el ? 0u : 1u) : 2u, s2, satd, key_q(sad, xq, yq)});
+        const uint64_t k2 = lane_read64(w2.b, 0);
+        result = make_int2(keyq_x(k2), keyq_y(k2));
     }
     *best_sad = screen;
     return result;
