@@ -58,6 +58,10 @@ constexpr int kDppHalfMirror = 0x141; // row_half_mirror
 constexpr int kDppMirror = 0x140;     // row_mirror
 constexpr int kDppBcast15 = 0x142;    // row_bcast:15
 constexpr int kDppBcast31 = 0x143;    // row_bcast:31
+constexpr int kDppShr1 = 0x111;       // row_shr:1 (lane i takes lane i - 1 of its row)
+constexpr int kDppShr2 = 0x112;
+constexpr int kDppShr4 = 0x114;
+constexpr int kDppShr8 = 0x118;
 
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ uint64_t min64_step(uint64_t v) {
@@ -65,6 +69,25 @@ __device__ __forceinline__ uint64_t min64_step(uint64_t v) {
     const uint32_t plo = dpp_mov<CTRL, ROWMASK>(lo, lo), phi = dpp_mov<CTRL, ROWMASK>(hi, hi);
     const uint64_t p = (static_cast<uint64_t>(phi) << 32) | plo;
     return p < v ? p : v;
+}
+// min(v, the value of lane i - k of the row), where that lane exists
+template <int CTRL>
+__device__ __forceinline__ uint64_t min64_shr(uint64_t v) {
+    const uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+    const uint64_t p = (static_cast<uint64_t>(dpp_mov<CTRL, 0xf>(~0u, hi)) << 32) | dpp_mov<CTRL, 0xf>(~0u, lo);
+    return p < v ? p : v;
+}
+// Inclusive prefix minimum within each 16-lane row (every lane active).
+__device__ __forceinline__ uint64_t row_prefix_min64(uint64_t v) {
+    v = min64_shr<kDppShr1>(v);
+    v = min64_shr<kDppShr2>(v);
+    v = min64_shr<kDppShr4>(v);
+    return min64_shr<kDppShr8>(v);
+}
+// The value of lane i - 1 of the row (~0 in the row's first lane).
+__device__ __forceinline__ uint64_t row_prev64(uint64_t v) {
+    const uint32_t lo = static_cast<uint32_t>(v), hi = static_cast<uint32_t>(v >> 32);
+    return (static_cast<uint64_t>(dpp_mov<kDppShr1, 0xf>(~0u, hi)) << 32) | dpp_mov<kDppShr1, 0xf>(~0u, lo);
 }
 // Minimum over the wave, returned uniformly (read from lane 63).
 __device__ __forceinline__ uint64_t wave_min64(uint64_t v) {
@@ -917,7 +940,8 @@ __device__ __forceinline__ void pw_load(Ctx& C, int wx, int wy, bool bil) {
 // pass), an 8x8 block 4 lanes (sixteen per pass).  Returns the cost of its own
 // candidate in each lane first+c, `keep` in the other lanes.
 __device__ __forceinline__ uint64_t qpel_eval(Ctx& C, int bx0, int by0, int bw, int bh, int vx, int vy, bool want,
-                                              int first, int cnt, int mode, int metric, uint64_t keep) {
+                                              int first, int cnt, int mode, int metric, uint64_t keep,
+                                              bool count = true) {
     uint64_t res = keep;
     if (cnt <= 0) return res;
     const JobDesc* J = C.J;
@@ -1011,7 +1035,7 @@ __device__ __forceinline__ uint64_t qpel_eval(Ctx& C, int bx0, int by0, int bw, 
         }
         const uint32_t total = group_sum32(part, 1 << lshift);
         // op counts: the lead lane of each scored candidate counts it
-        if (!skip && t == 0) {
+        if (count && !skip && t == 0) {
             const uint32_t px = static_cast<uint32_t>(aw * ah);
             if (metric == 0) C.ln_sad += px;
             C.ln_taps += px * static_cast<uint32_t>(taps_per_sample(hm, vm, bil));
@@ -1037,6 +1061,17 @@ __device__ __forceinline__ void count_sad_eval(Ctx& C, int bx0, int by0, int bw,
         return;
     C.n_sad += static_cast<uint64_t>(aw) * ah;
     C.n_taps += static_cast<uint64_t>(aw) * ah * taps_per_sample(v.x & 3, v.y & 3, bil);
+}
+
+// The same op counts added to this lane's partial counters (lane-parallel callers).
+__device__ __forceinline__ void count_sad_eval_lane(Ctx& C, int bx0, int by0, int bw, int bh, int2 v, int mode) {
+    const bool bil = mode_bilinear(mode);
+    const int aw = imax(0, imin(bw, C.g.W - bx0)), ah = imax(0, imin(bh, C.g.H - by0));
+    if (hidden_padding(C.g, bx0, by0, bw, bh, v.x, v.y, bil ? mode : 0, chroma_block(bx0, by0, bw, bh)) ||
+        (bil && ((v.x & 1) || (v.y & 1))))
+        return;
+    C.ln_sad += static_cast<uint32_t>(aw * ah);
+    C.ln_taps += static_cast<uint32_t>(aw * ah * taps_per_sample(v.x & 3, v.y & 3, bil));
 }
 
 // ------------------------------------------------------------------ integer searches
@@ -2401,9 +2436,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
                 if (imax(iabs(m.x), iabs(m.y)) > limit) { m.x = (m.x / 4) * 4; m.y = (m.y / 4) * 4; }
                 return m;
             };
-            // the eleven seed vectors, one per lane: their SADs are scored
-            // together (unique vectors only), then replayed in the
-            // reference's order
+            // the eleven seed vectors, one per lane
             int sx = one.x, sy = one.y;
             {
                 const int s = C.lane - 1;
@@ -2424,39 +2457,25 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
                 const int2 w = within(make_int2(sx, sy));
                 sx = w.x; sy = w.y;
             }
-            uint64_t scost;
-            uint32_t repeat = 0;        // bit s: seed s repeats an earlier seed
-            {
-                SubList U;
-                U.n = 0; U.xq = 0; U.yq = 0; U.sad = kNone;
-                int slot = 0;
-                for (int j = 0; j < 11; ++j) {
-                    const int vx = __builtin_amdgcn_readlane(sx, j), vy = __builtin_amdgcn_readlane(sy, j);
-                    const uint64_t hit = __ballot(C.lane < U.n && U.xq == vx && U.yq == vy);
-                    int at = U.n;
-                    if (hit) {
-                        at = __ffsll(static_cast<long long>(hit)) - 1;
-                        repeat |= 1u << j;
-                    } else {
-                        if (C.lane == U.n) { U.xq = vx; U.yq = vy; }
-                        ++U.n;
-                    }
-                    if (C.lane == j) slot = at;
-                }
-                U.sad = qpel_eval(C, bx, by, 8, 8, U.xq, U.yq, true, 0, U.n, 0, 0, kNone);
-                scost = lane_shfl64(U.sad, slot);
-            }
-            int2 best_seed = make_int2(__builtin_amdgcn_readlane(sx, 0), __builtin_amdgcn_readlane(sy, 0));
-            uint64_t seed_cost = lane_read64(scost, 0);
-            for (int s = 1; s < 11; ++s) {
-                const int2 sd = make_int2(__builtin_amdgcn_readlane(sx, s), __builtin_amdgcn_readlane(sy, s));
-                if (sd.x == best_seed.x && sd.y == best_seed.y) continue;
-                // the reference scores this seed again if it repeats an earlier one
-                if ((repeat >> s) & 1) count_sad_eval(C, bx, by, 8, 8, sd, 0);
-                const uint64_t cc = lane_read64(scost, s);
-                const int cm = iabs(sd.x) + iabs(sd.y), bm = iabs(best_seed.x) + iabs(best_seed.y);
-                if (cc < seed_cost || (cc == seed_cost && cm < bm)) { seed_cost = cc; best_seed = sd; }
-            }
+            // all eleven seeds scored at once (lanes 0..10).  The reference's
+            // scan keeps the first seed of least (cost, |x|+|y|): the least
+            // (cost, |x|+|y|, seed) key.  It scores seed 0 and every later seed
+            // that differs from the best of the seeds before it (the prefix
+            // minimum), and those scorings are what the op counts record.
+            const bool is_seed = C.lane < 11;
+            const uint64_t scost = qpel_eval(C, bx, by, 8, 8, sx, sy, is_seed, 0, 11, 0, 0, kNone, false);
+            const uint64_t skey = is_seed ? (cost21(scost) << 24) | (static_cast<uint64_t>(iabs(sx) + iabs(sy)) << 4) |
+                                                static_cast<uint64_t>(C.lane)
+                                          : kNone;
+            const uint64_t upto = row_prefix_min64(skey);
+            const uint64_t before = row_prev64(upto);
+            const int prev_best = before == kNone ? 0 : static_cast<int>(before & 15);
+            const int pbx = __shfl(sx, prev_best), pby = __shfl(sy, prev_best);
+            if (is_seed && (C.lane == 0 || sx != pbx || sy != pby))
+                count_sad_eval_lane(C, bx, by, 8, 8, make_int2(sx, sy), 0);
+            const int win = static_cast<int>(lane_read64(upto, 10) & 15);
+            const int2 best_seed = make_int2(__builtin_amdgcn_readlane(sx, win), __builtin_amdgcn_readlane(sy, win));
+            const uint64_t seed_cost = lane_read64(scost, win);
             uint64_t bsad = 0;
             SubParams sp{bx, by, 8, 8, 3, range, true};
             const int2 mv = refine_staged(C, sp, best_seed.x, best_seed.y, pi4, &bsad, seed_cost);
