@@ -452,6 +452,28 @@ __device__ __forceinline__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int w
     const bool aligned_rows = (W & 3) == 0;
     gptr_u8 ry = as_global(C.ry);
     uint32_t* w4 = reinterpret_cast<uint32_t*>(C.L.win);
+    if (aligned_rows && ax0 >= 0 && ax0 + wp <= W) {
+        // every dword of every row (slack included) lies inside the plane's
+        // columns: copy global -> LDS directly, 64 consecutive dwords of the
+        // window per instruction (dword i lands at byte 4 i), all in flight
+        // at once; rows clamp to the plane.  Lane l copies dword base + l,
+        // whose row and column advance by (dr, dc) per instruction.
+        typedef const __attribute__((address_space(1))) void* gvoid;
+        typedef __attribute__((address_space(3))) void* lvoid;
+        const int dr = kWave / cpr, dc = kWave - dr * cpr;
+        int r = C.lane / cpr, c = C.lane - (C.lane / cpr) * cpr;
+        for (int base = 0; base < total; base += kWave) {
+            const int sy = clampi(wy0 + r, 0, H - 1);
+            const gptr_u8 src = ry + static_cast<size_t>(sy) * W + ax0 + c * 4;
+            if (base + C.lane < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(w4 + base), 4, 0, 0);
+            c += dc;
+            r += dr;
+            if (c >= cpr) { c -= cpr; ++r; }
+        }
+        C.wx0 = ax0; C.wy0 = wy0; C.ww = aw; C.wh = wh; C.wp = wp;
+        wsync();
+        return true;
+    }
 #pragma unroll 4
     for (int i = C.lane; i < total; i += kWave) {
         const int r = i / cpr, c = i - r * cpr;
