@@ -314,28 +314,18 @@ __host__ __device__ inline int seen_words(int local_range) {
 }
 constexpr int kSmallWin = 80 * 96;   // windows of the far (long-range) candidates
 
-// Sampled points the UMH search of local range R can log: its stages score at
-// most 5 seeds, a 5x5, the sparse lattice, the uneven cross, the hexagon
-// rings, 32 small-hexagon steps of 6 and a closing diamond of 4 (each point
-// logged once: the log only takes points the dedupe bitset has not seen).
-__host__ __device__ inline int umh_key_cap(int R) {
-    const int side = (2 * R) / 8 + 1;
-    const int lattice = R >= 8 ? side * side + 2 * side : 0;
-    const int cs = R >= 96 ? 8 : (R >= 32 ? 4 : 2);
-    const int vlimit = R < (R / 2 > 8 ? R / 2 : 8) ? R : (R / 2 > 8 ? R / 2 : 8);
-    const int cross = 2 * (R / cs + vlimit / cs);
-    const int hs = R >= 96 ? 8 : (R >= 24 ? 4 : 2);
-    return 5 + 25 + lattice + cross + 6 * (R / hs) + 32 * 6 + 4;
-}
-
 // Phase window: the interpolated reference (every phase) around a macroblock's
 // integer vector, staged in LDS for the sub-pel stages.  kPwSide samples
 // square per phase; rows hold kPwPitch bytes so that 4-byte aligned copies
 // of the phase planes (plus the 8-byte read of the block scorer) fit whatever
 // the alignment of the window origin.
-constexpr int kPwMargin = 4;
-constexpr int kPwSide = 16 + 2 * kPwMargin;
-constexpr int kPwPitch = 32;
+// The window starts kPwMargin samples before the integer vector and covers
+// kPwSide = 21 samples: the 16x16 sub-pel candidates (within 3 quarter-pels of
+// the vector) and every 4-MV seed (within 2 pels of the 1-MV vector) stay
+// inside it; a candidate outside reads the phase planes.
+constexpr int kPwMargin = 3;
+constexpr int kPwSide = 21;
+constexpr int kPwPitch = 28;
 static_assert(kPwSide + 3 <= kPwPitch && ((kPwSide - 4 + 3) & ~3) + 8 <= kPwPitch, "phase-window rows too short");
 constexpr int kPwBytes = (16 * kPwSide * kPwPitch + 255) & ~255;   // whole 256-byte copy blocks
 
@@ -352,8 +342,6 @@ struct Lds {
     uint8_t* pw;         // phase window
     uint32_t* seen_s;    // sampled-SAD dedupe bitset of the UMH search
     uint32_t* seen_f;    // full-SAD dedupe bitset
-    uint64_t* ukeys;     // keys of every sampled point the UMH search considered
-    uint64_t* top16;     // long-range top-16 keys
     int2* own;           // current MB's four block vectors
     uint32_t* nb;        // neighbour snapshot: kNbWords dwords each of left, top, top-left, top-right
 };
@@ -363,22 +351,19 @@ struct Lds {
 constexpr int kNbWords = 12;
 enum { NB_LEFT = 0, NB_TOP = 1, NB_TOPLEFT = 2, NB_TOPRIGHT = 3 };
 
-// The integer searches' window, dedupe bitsets and key log are dead once the
-// sub-pel stages start, so the phase window shares their bytes; so does the
-// extended search's staging (kXwRoundBytes, defined with it below).
+// The integer searches' window and dedupe bitsets are dead once the sub-pel
+// stages start, so the phase window shares their bytes; so does the extended
+// search's staging (kXwRoundBytes, defined with it below).
 __host__ __device__ inline int int_scratch_bytes(int local_range) {
-    int off = (win_capacity(local_range) + 15) & ~15;  // win
-    off += seen_words(local_range) * 4 * 2;             // seen_s, seen_f
-    off = (off + 15) & ~15;
-    off += umh_key_cap(local_range) * 8;                // ukeys
-    return off;
+    const int off = (win_capacity(local_range) + 15) & ~15;  // win
+    return off + seen_words(local_range) * 4 * 2;             // seen_s, seen_f
 }
 
 __host__ __device__ inline int shared_bytes(int local_range);
 
 __host__ __device__ inline int lds_bytes_for(int local_range) {
-    // cur, shared region, top16, own, neighbour snapshot, slack
-    return 256 + shared_bytes(local_range) + 16 * 8 + 4 * 8 + 4 * 12 * 4 + 64;
+    // cur, shared region, own, neighbour snapshot, slack
+    return 256 + shared_bytes(local_range) + 4 * 8 + 4 * 12 * 4 + 64;
 }
 
 __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
@@ -388,14 +373,11 @@ __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     int off = 0;
     L.win = region + off; off += (win_capacity(local_range) + 15) & ~15;
     L.seen_s = reinterpret_cast<uint32_t*>(region + off); off += seen_words(local_range) * 4;
-    L.seen_f = reinterpret_cast<uint32_t*>(region + off); off += seen_words(local_range) * 4;
-    off = (off + 15) & ~15;
-    L.ukeys = reinterpret_cast<uint64_t*>(region + off);
+    L.seen_f = reinterpret_cast<uint32_t*>(region + off);
     L.pw = region;
     const int shared = shared_bytes(local_range);
-    L.top16 = reinterpret_cast<uint64_t*>(region + shared);
-    L.own = reinterpret_cast<int2*>(region + shared + 16 * 8);
-    L.nb = reinterpret_cast<uint32_t*>(region + shared + 16 * 8 + 4 * 8);
+    L.own = reinterpret_cast<int2*>(region + shared);
+    L.nb = reinterpret_cast<uint32_t*>(region + shared + 4 * 8);
     return L;
 }
 
@@ -540,13 +522,36 @@ __device__ __forceinline__ bool in_window(const Ctx& C, int dx, int dy) {
 
 enum { KEY_TOTAL = 0, KEY_SEQ_MAN = 1, KEY_SEQ = 2 };
 
+// The four least keys considered so far, ascending (kNone where fewer):
+// wave-uniform values.
+struct Top4 {
+    uint64_t k0, k1, k2, k3;
+};
+__device__ __forceinline__ void top4_insert(Top4& t, uint64_t m) {
+    if (m < t.k0) { t.k3 = t.k2; t.k2 = t.k1; t.k1 = t.k0; t.k0 = m; }
+    else if (m < t.k1) { t.k3 = t.k2; t.k2 = t.k1; t.k1 = m; }
+    else if (m < t.k2) { t.k3 = t.k2; t.k2 = m; }
+    else if (m < t.k3) t.k3 = m;
+}
+// Merge a batch's keys (one per lane, kNone elsewhere; distinct from each
+// other and from the keys already held), whose least is `least`: its
+// smallest keys enter in ascending order until one cannot.
+__device__ __forceinline__ void top4_merge(Top4& t, uint64_t key, uint64_t least) {
+    uint64_t m = least;
+    for (int r = 0; r < 4 && m < t.k3; ++r) {
+        top4_insert(t, m);
+        if (r < 3) m = wave_min64(key > m ? key : kNone);
+    }
+}
+
 // Score n integer candidates (pt(i) -> (dx,dy), already clamped) at the given
 // sample stride from the current window, G lanes per candidate.  seen: dedupe
-// bitset or null.  ukeys/ucount: optional log of every scored key.  Returns
-// the minimum key of the batch (kNone if no candidate was scored).
+// bitset or null.  top: optional four least keys of every scored point, to
+// which the batch's keys are merged (with the bitset, every key is new).
+// Returns the minimum key of the batch (kNone if no candidate was scored).
 template <class PtFn>
 __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int keymode, int seq0, PtFn pt,
-                              uint64_t* ukeys = nullptr, int* ucount = nullptr) {
+                              Top4* top = nullptr) {
     const int rows = (C.ah + stride - 1) / stride;
     int G = 1;
     while (G * 2 * n <= kWave && G * 2 <= rows && G < 16) G *= 2;
@@ -595,16 +600,9 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
             else if (keymode == KEY_SEQ_MAN) key = key_seq(cost, iabs(dx) + iabs(dy), seq0 + i);
             else key = key_seq(cost, 0, seq0 + i);
         }
-        if (ukeys) {
-            const uint64_t m = __ballot(leader);
-            const int pos = *ucount + lane_prefix(m);
-            const int cap = umh_key_cap(C.J->local_range);
-            if (leader && pos < cap) ukeys[pos] = key;
-            *ucount += __popcll(m);
-            if (*ucount > cap) set_err(C, 8);
-        }
         const uint64_t mk = wave_min64(key);
         best = mk < best ? mk : best;
+        if (top) top4_merge(*top, key, mk);
     }
     return best;
 }
@@ -1118,11 +1116,10 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
     for (int i = C.lane; i < seen_words(C.J->local_range); i += kWave) { C.L.seen_s[i] = 0; C.L.seen_f[i] = 0; }
     wsync();
     uint64_t best = kNone;
-    int nkeys = 0;
-    uint64_t* uk = C.L.ukeys;
+    Top4 top{kNone, kNone, kNone, kNone};
     auto cl = [R](int v) VC1ME_INLINE { return clampi(v, -R, R); };
     auto stage = [&](int n, auto pt) VC1ME_INLINE {
-        const uint64_t m = int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, pt, uk, &nkeys);
+        const uint64_t m = int_batch(C, n, 2, C.L.seen_s, KEY_TOTAL, 0, pt, &top);
         best = m < best ? m : best;
     };
 
@@ -1217,25 +1214,8 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
         });
     }
     // the best four considered points (top[0..3] of the reference's list)
-    wsync();
-    uint64_t tk[4] = {kNone, kNone, kNone, kNone};
-    const int nk = imin(nkeys, umh_key_cap(C.J->local_range));
-    const int fin = imin(4, nk);
-    {
-        uint64_t prev = 0;
-        bool first = true;
-        for (int f = 0; f < fin; ++f) {
-            uint64_t m = kNone;
-            for (int i = C.lane; i < nk; i += kWave) {
-                const uint64_t k = uk[i];
-                if ((first || k > prev) && k < m) m = k;
-            }
-            m = wave_min64(m);
-            if (f == 0) tk[0] = m; else if (f == 1) tk[1] = m; else if (f == 2) tk[2] = m; else tk[3] = m;
-            prev = m;
-            first = false;
-        }
-    }
+    const uint64_t tk[4] = {top.k0, top.k1, top.k2, top.k3};
+    const int fin = (top.k0 != kNone) + (top.k1 != kNone) + (top.k2 != kNone) + (top.k3 != kNone);
     // full-resolution finalists: 3x3 around the best four
     uint64_t full = int_batch(C, fin * 9, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
         const int f = i / 9, o = i % 9;
@@ -1571,7 +1551,7 @@ constexpr int kXwPitch = 64;
 constexpr int kXwBytes = kXwSide * kXwPitch;
 // candidates staged per round: their windows and 25 keys each must fit the
 // shared LDS region, which is at least the phase window's size
-constexpr int kXwPerRound = 5;
+constexpr int kXwPerRound = 3;
 constexpr int kXwRoundBytes = kXwPerRound * (kXwBytes + 25 * 8);   // windows, then 25 keys per candidate
 
 // The shared LDS region: the largest of its three uses.
@@ -2738,7 +2718,7 @@ __global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ 
 // The starting state of the relaxation: every macroblock's guess analysis at
 // once (its outputs are all overwritten by pass 0, which analyses every
 // macroblock for real).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_guess(const JobDesc* __restrict__ jobs) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_guess(const JobDesc* __restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t smem[];
     const JobDesc* J = &jobs[blockIdx.y];
     const int pos = static_cast<int>(blockIdx.x);
@@ -2769,7 +2749,7 @@ __device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int 
 // Pass 0: every macroblock of every job (one workgroup each, full
 // occupancy); the macroblocks whose neighbours' state changed form the list
 // of pass 1.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_relax0(const JobDesc* __restrict__ jobs,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_relax0(const JobDesc* __restrict__ jobs,
                                                                                        RelaxQueue Q) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int job = static_cast<int>(blockIdx.y);
@@ -2787,7 +2767,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_
 // changed for pass p + 1.  The list length was final when the previous pass
 // ended; an empty list ends every workgroup at once, so passes launched
 // after the state settled cost only their launch.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) k_mb_relaxq(const JobDesc* __restrict__ jobs,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_relaxq(const JobDesc* __restrict__ jobs,
                                                                                        RelaxQueue Q, int pass) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int n = imin(Q.cnt[pass], Q.cap);
