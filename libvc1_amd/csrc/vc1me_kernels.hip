@@ -557,7 +557,8 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
     while (G * 2 * n <= kWave && G * 2 <= rows && G < 16) G *= 2;
     const int per = kWave / G;
     const int part = C.lane & (G - 1);
-    uint64_t best = kNone;
+    uint64_t best = kNone;        // with `top`: the least key, merged per batch
+    uint64_t lane_best = kNone;   // without: this lane's least key
     for (int base = 0; base < n; base += per) {
         const int i = base + C.lane / G;
         const bool act = i < n;
@@ -600,11 +601,15 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
             else if (keymode == KEY_SEQ_MAN) key = key_seq(cost, iabs(dx) + iabs(dy), seq0 + i);
             else key = key_seq(cost, 0, seq0 + i);
         }
-        const uint64_t mk = wave_min64(key);
-        best = mk < best ? mk : best;
-        if (top) top4_merge(*top, key, mk);
+        if (top) {
+            const uint64_t mk = wave_min64(key);
+            best = mk < best ? mk : best;
+            top4_merge(*top, key, mk);
+        } else {
+            lane_best = key < lane_best ? key : lane_best;
+        }
     }
-    return best;
+    return top ? best : wave_min64(lane_best);
 }
 
 // ------------------------------------------------------------------ geometry helpers
@@ -1532,14 +1537,13 @@ __device__ __forceinline__ uint64_t cur_signature(Ctx& C) {
 }
 
 // long_range_signature_distance, src/encoder_motion.cpp:1603-1613
+// = 2 * (sum of |a_i - b_i| over the four luma bytes) + 3 * (the same over the
+// two chroma bytes): two byte-SAD instructions.
 __device__ __forceinline__ uint32_t sig_distance(uint64_t a, uint64_t b) {
-    uint32_t d = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-        d += static_cast<uint32_t>(iabs(static_cast<int>((a >> (8 * i)) & 255) - static_cast<int>((b >> (8 * i)) & 255))) * 2u;
-    d += static_cast<uint32_t>(iabs(static_cast<int>((a >> 32) & 255) - static_cast<int>((b >> 32) & 255))) * 3u;
-    d += static_cast<uint32_t>(iabs(static_cast<int>((a >> 40) & 255) - static_cast<int>((b >> 40) & 255))) * 3u;
-    return d;
+    const uint32_t luma = __builtin_amdgcn_sad_u8(static_cast<uint32_t>(a), static_cast<uint32_t>(b), 0u);
+    const uint32_t chroma = __builtin_amdgcn_sad_u8(static_cast<uint32_t>(a >> 32) & 0xffffu,
+                                                    static_cast<uint32_t>(b >> 32) & 0xffffu, 0u);
+    return 2u * luma + 3u * chroma;
 }
 
 // Windows of the extended search: 36 x 36 frame samples around a candidate
