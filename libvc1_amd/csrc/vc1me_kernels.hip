@@ -1665,67 +1665,74 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
     // cells whose offset lies inside the bounds
     const int cx_lo = imax(0, (C.x0 + b.xmin + 7) >> 3), cx_hi = imin(J->ncx - 1, (C.x0 + b.xmax) >> 3);
     const int cy_lo = imax(0, (C.y0 + b.ymin + 7) >> 3), cy_hi = imin(J->ncy - 1, (C.y0 + b.ymax) >> 3);
-    // Scan: rows of the lattice are staged in LDS (16-byte aligned copies),
-    // every lane keys two cells of a row, and keys below the running 16th
-    // best are inserted into the top list that lanes 0..15 hold in order.
+    // Scan: lane l keys cells l, l + 64, l + 128, ... of the cell rectangle in
+    // row-major order, four per block, read straight from the lattice (it
+    // stays in L2); the next block's signatures are loaded while the current
+    // block is keyed.  Keys below the running 16th best are inserted into the
+    // top list that lanes 0..15 hold in order.  The top sixteen are a set of
+    // distinct keys, so the order in which cells are visited does not matter.
     uint64_t top = kNone;      // lane r < 16: r-th best key so far
     uint64_t thr = kNone;      // the 16th best (uniform)
     uint32_t scanned = 0;
     if (cx_lo <= cx_hi && cy_lo <= cy_hi) {
+        constexpr int kU = 4;
         const int nxc = cx_hi - cx_lo + 1;
-        const int chunks = (8 + nxc * 8 + 15) >> 4;           // per row, worst alignment
-        const int pitch = chunks * 16;
-        const int region = shared_bytes(J->local_range);
-        const int rows_per = imax(1, region / pitch);
-        const gptr_u8 sigb = as_global(reinterpret_cast<const uint8_t*>(J->sig));
-        typedef const __attribute__((address_space(1))) void* gvoid;
-        typedef __attribute__((address_space(3))) void* lvoid;
-        uint8_t* const stage = C.L.pw;
-        for (int r0 = cy_lo; r0 <= cy_hi; r0 += rows_per) {
-            const int nr = imin(rows_per, cy_hi - r0 + 1);
-            wsync();
-            for (int r = 0; r < nr; ++r) {
-                const size_t a = (static_cast<size_t>(r0 + r) * J->ncx + cx_lo) * 8;
-                const gptr_u8 src = sigb + (a & ~static_cast<size_t>(15));
-                for (int base = 0; base < chunks; base += kWave) {
-                    const int idx = base + C.lane;
-                    if (idx < chunks)
-                        __builtin_amdgcn_global_load_lds((gvoid)(src + idx * 16), (lvoid)(stage + r * pitch + base * 16),
-                                                         16, 0, 0);
+        const int total = nxc * (cy_hi - cy_lo + 1);
+        const int ncx = J->ncx;
+        typedef const __attribute__((address_space(1))) uint64_t* gptr_u64;
+        const gptr_u64 sig = (gptr_u64)(J->sig) + static_cast<size_t>(cy_lo) * ncx + cx_lo;
+        const int dr = kWave / nxc, dc = kWave - dr * nxc;
+        int r = C.lane / nxc, c = C.lane - (C.lane / nxc) * nxc;   // the lane's next cell
+        // loads without branches (a lane past the rectangle reads its first
+        // cell and keys nothing), so that the wait before keying a block
+        // leaves the next block's loads in flight
+        auto fetch = [&](uint64_t (&fv)[kU], int (&fr)[kU], int (&fc)[kU], int base) VC1ME_INLINE {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                fr[u] = r;
+                fc[u] = c;
+                const bool in = base + C.lane + u * kWave < total;
+                fv[u] = sig[in ? static_cast<size_t>(r) * ncx + c : 0];
+                c += dc;
+                r += dr;
+                if (c >= nxc) { c -= nxc; ++r; }
+            }
+        };
+        auto key_block = [&](const uint64_t (&v)[kU], const int (&vr)[kU], const int (&vc)[kU], int base) VC1ME_INLINE {
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int dx = (cx_lo + vc[u]) * 8 - C.x0, dy = (cy_lo + vr[u]) * 8 - C.y0;
+                const bool ok = base + C.lane + u * kWave < total && !(iabs(dy) <= 32 && iabs(dx) <= 32);
+                uint64_t k = kNone;
+                if (ok) {
+                    ++scanned;
+                    k = key_int(sig_distance(want, v[u]), dx, dy);
+                }
+                uint64_t hits = __ballot(k < thr);
+                while (hits) {
+                    const int l = __ffsll(static_cast<long long>(hits)) - 1;
+                    hits &= hits - 1;
+                    const uint64_t kk = lane_read64(k, l);
+                    if (kk >= thr) continue;
+                    // insert kk into the ordered list of lanes 0..15
+                    const int pos = __popcll(__ballot(C.lane < 16 && top < kk));
+                    const uint64_t up = lane_shfl64(top, C.lane > 0 ? C.lane - 1 : 0);
+                    if (C.lane < 16 && C.lane > pos) top = up;
+                    if (C.lane == pos) top = kk;
+                    thr = lane_read64(top, 15);
                 }
             }
-            C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;   // the integer window is overwritten
-            wsync();
-            for (int r = 0; r < nr; ++r) {
-                const int cyy = r0 + r;
-                const int xoff = static_cast<int>(((static_cast<size_t>(cyy) * J->ncx + cx_lo) * 8) & 15) >> 3;
-                const uint64_t* row = reinterpret_cast<const uint64_t*>(stage + r * pitch) + xoff;
-                const int dy = cyy * 8 - C.y0;
-                const bool inner_row = iabs(dy) <= 32;
-                for (int xb = 0; xb < nxc; xb += kWave) {
-                    const int xi = xb + C.lane;
-                    const int dx = (cx_lo + xi) * 8 - C.x0;
-                    const bool ok = xi < nxc && !(inner_row && iabs(dx) <= 32);
-                    uint64_t k = kNone;
-                    if (ok) {
-                        ++scanned;
-                        k = key_int(sig_distance(want, row[xi]), dx, dy);
-                    }
-                    uint64_t hits = __ballot(k < thr);
-                    while (hits) {
-                        const int l = __ffsll(static_cast<long long>(hits)) - 1;
-                        hits &= hits - 1;
-                        const uint64_t kk = lane_read64(k, l);
-                        if (kk >= thr) continue;
-                        // insert kk into the ordered list of lanes 0..15
-                        const int pos = __popcll(__ballot(C.lane < 16 && top < kk));
-                        const uint64_t up = lane_shfl64(top, C.lane > 0 ? C.lane - 1 : 0);
-                        if (C.lane < 16 && C.lane > pos) top = up;
-                        if (C.lane == pos) top = kk;
-                        thr = lane_read64(top, 15);
-                    }
-                }
-            }
+        };
+        // two blocks in turn: one is keyed while the other loads
+        uint64_t va[kU], vb[kU];
+        int ra[kU], ca[kU], rb[kU], cb[kU];
+        fetch(va, ra, ca, 0);
+        for (int base = 0; base < total; base += 2 * kU * kWave) {
+            fetch(vb, rb, cb, base + kU * kWave);
+            key_block(va, ra, ca, base);
+            if (base + kU * kWave >= total) break;
+            fetch(va, ra, ca, base + 2 * kU * kWave);
+            key_block(vb, rb, cb, base + kU * kWave);
         }
     }
     C.n_sig += wave_sum32(scanned);
