@@ -584,7 +584,8 @@ static_assert(kRelaxCounters * sizeof(int32_t) <= kOutSummaryAt, "relaxation cou
 // learned form runs its network there, on the global vectors in the slots).
 int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* nn_out = nullptr, float nn_logit = 0.f,
              const std::function<int(hipStream_t)>& after_global = {},
-             const vc1me_p_result* const* seeds = nullptr, bool motion_check = false, bool nn_candidates = true) {
+             const vc1me_p_result* const* seeds = nullptr, bool motion_check = false, bool nn_candidates = true,
+             const std::string& graph_tag = std::string()) {
     if (n <= 0) return VC1ME_OK;
     if (!jobs) return fail(VC1ME_EINVAL, "null jobs");
     if (n > kMaxJobs) return fail(VC1ME_ELIMIT, "at most " + std::to_string(kMaxJobs) + " jobs per step");
@@ -814,13 +815,15 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         return finish();
     };
     ctx->timing.graph_replay = 0;
-    const bool graph = ctx->use_graphs && !ctx->profiling && !after_global;
+    // a step with work after the global search is capturable when its caller
+    // tags the buffers that work uses (the learned step's network)
+    const bool graph = ctx->use_graphs && !ctx->profiling && (!after_global || !graph_tag.empty());
     if (graph) {
         // replay the step as one HIP graph, captured once per launch shape
         const std::string key = shape + "p" + std::to_string(relax ? passes : 0) + "@" +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_jobs)) + "," +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_out)) + "," +
-                                std::to_string(reinterpret_cast<long long>(ctx->d_relax_wl));
+                                std::to_string(reinterpret_cast<long long>(ctx->d_relax_wl)) + graph_tag;
         hipGraphExec_t exec = nullptr;
         for (auto& g : ctx->graphs)
             if (g.first == key) exec = g.second;
@@ -1003,8 +1006,11 @@ struct vc1me_nn {
     uint16_t* wtt[pnet::kLayers] = {};   // bf16 [9][ci][co] (layers with cin 32)
     bool packed = false;
     NnBuffers b;
-    std::vector<float*> props;               // per job of a learned step: 5 floats per MB
-    std::vector<size_t> props_cap;
+    // proposals of a learned step: 5 floats per macroblock, each raster
+    // group's jobs contiguous in one buffer (props[i] points at job i's)
+    float* props_all = nullptr;
+    size_t props_cap = 0;
+    std::vector<float*> props;
 };
 
 namespace {
@@ -1057,8 +1063,9 @@ hipError_t nn_pack(vc1me_nn* nn, hipStream_t st) {
     return hipSuccess;
 }
 
-// forward of nb images already in b.x0; outputs in b.out
-hipError_t nn_forward(vc1me_nn* nn, int nb, int h4, int w4, hipStream_t st) {
+// forward of nb images already in b.x0; outputs in out (null: b.out).  The
+// weights must be packed already when the launches are being captured.
+hipError_t nn_forward(vc1me_nn* nn, int nb, int h4, int w4, hipStream_t st, float* out = nullptr) {
     hipError_t e = nn_pack(nn, st);
     const uint16_t* in = nn->b.x0;
     for (int l = 0; l < pnet::kLayers && e == hipSuccess; ++l) {
@@ -1069,7 +1076,7 @@ hipError_t nn_forward(vc1me_nn* nn, int nb, int h4, int w4, hipStream_t st) {
     }
     if (e == hipSuccess)
         e = pnet::launch_pool_head(nn->b.act[pnet::kLayers - 1], nn->p + pnet::head_offset(), nn->b.pooled,
-                                       nn->b.out, nb, h4, w4, (h4 + 3) / 4, (w4 + 3) / 4, st);
+                                       out ? out : nn->b.out, nb, h4, w4, (h4 + 3) / 4, (w4 + 3) / 4, st);
     return e;
 }
 
@@ -2316,7 +2323,7 @@ void vc1me_nn_destroy(vc1me_nn* nn) {
     NnBuffers& b = nn->b;
     rel(b.x0); for (auto& a : b.act) rel(a); rel(b.dz[0]); rel(b.dz[1]); rel(b.pooled); rel(b.out); rel(b.dout);
     rel(b.tgt); rel(b.valid); rel(b.in32); rel(b.loss); rel(b.gxy);
-    for (auto& q : nn->props) rel(q);
+    rel(nn->props_all);
     delete nn;
 }
 
@@ -2454,52 +2461,69 @@ int vc1me_analyze_step_learned(vc1me_ctx* ctx, vc1me_nn* nn, const vc1me_job* jo
     if (!jobs || n > 255) return fail(VC1ME_EINVAL, "bad job list");
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (int rc = set_device(ctx)) return rc;
-    if (nn->props.size() < static_cast<size_t>(n)) { nn->props.resize(n, nullptr); nn->props_cap.resize(n, 0); }
-    for (int i = 0; i < n; ++i) {
+    for (int i = 0; i < n; ++i)
         if (!jobs[i].cur || !jobs[i].ref || !jobs[i].params) return fail(VC1ME_EINVAL, "null job field");
-        int h4, w4, mbh, mbw;
-        nn_dims(jobs[i].cur->w, jobs[i].cur->h, h4, w4, mbh, mbw);
-        const size_t need = static_cast<size_t>(mbh) * mbw * 5;
-        if (nn->props_cap[i] < need) {
-            if (nn->props[i]) (void)hipFree(nn->props[i]);
-            nn->props[i] = nullptr;
-            nn->props_cap[i] = 0;
-            HIP_TRY(hipMalloc(&nn->props[i], need * sizeof(float)));
-            nn->props_cap[i] = need;
-        }
-    }
-    // the network runs inside the step, after the global search it is aligned
-    // by: one batched forward pass per raster of the step
-    auto forward = [&](hipStream_t st) -> int {
+    // the jobs by raster: one batched forward pass per group, whose outputs
+    // land contiguously in the step's proposal buffer
+    struct Group { std::vector<int> jobs; int h4, w4, mbh, mbw; size_t at; };
+    std::vector<Group> groups;
+    {
         std::vector<bool> done(n, false);
         for (int i = 0; i < n; ++i) {
             if (done[i]) continue;
-            std::vector<int> group;
+            Group g{};
             for (int k = i; k < n; ++k)
                 if (!done[k] && jobs[k].cur->w == jobs[i].cur->w && jobs[k].cur->h == jobs[i].cur->h) {
-                    group.push_back(k);
+                    g.jobs.push_back(k);
                     done[k] = true;
                 }
-            std::vector<const vc1me_frame*> cs, rs;
-            std::vector<const int32_t*> gs;
-            for (int k : group) {
-                cs.push_back(jobs[k].cur);
-                rs.push_back(jobs[k].ref);
-                gs.push_back(&ctx->slots[k].g->dx);
-            }
-            const int gn = static_cast<int>(group.size());
-            int h4, w4, mbh, mbw;
-            if (int rc = nn_inputs(nn, cs.data(), rs.data(), gn, h4, w4, mbh, mbw, nullptr, gs.data())) return rc;
-            HIP_TRY(nn_forward(nn, gn, h4, w4, st));
-            const size_t per = static_cast<size_t>(mbh) * mbw * 5;
-            for (int j = 0; j < gn; ++j)
-                HIP_TRY(hipMemcpyAsync(nn->props[group[j]], nn->b.out + per * j, sizeof(float) * per,
-                                       hipMemcpyDeviceToDevice, st));
+            nn_dims(jobs[i].cur->w, jobs[i].cur->h, g.h4, g.w4, g.mbh, g.mbw);
+            groups.push_back(std::move(g));
+        }
+    }
+    size_t total = 0;
+    for (auto& g : groups) {
+        g.at = total;
+        total += g.jobs.size() * static_cast<size_t>(g.mbh) * g.mbw * 5;
+    }
+    if (nn->props_cap < total) {
+        if (nn->props_all) (void)hipFree(nn->props_all);
+        nn->props_all = nullptr;
+        nn->props_cap = 0;
+        HIP_TRY(hipMalloc(&nn->props_all, total * sizeof(float)));
+        nn->props_cap = total;
+    }
+    nn->props.assign(n, nullptr);
+    for (const auto& g : groups)
+        for (size_t j = 0; j < g.jobs.size(); ++j)
+            nn->props[g.jobs[j]] = nn->props_all + g.at + j * static_cast<size_t>(g.mbh) * g.mbw * 5;
+    // everything the forward passes use is allocated and packed before the
+    // step, so that the step (forward included) can be captured as a graph
+    for (const auto& g : groups) HIP_TRY(nn_reserve(nn, static_cast<int>(g.jobs.size()), g.h4, g.w4));
+    HIP_TRY(nn_pack(nn, ctx->stream));
+    // (the inputs are read through the step's job table: frames and global
+    // vectors, as every other kernel of the step)
+    auto forward = [&](hipStream_t st) -> int {
+        for (const auto& g : groups) {
+            pnet::JobIndex idx{};
+            for (size_t j = 0; j < g.jobs.size(); ++j) idx.k[j] = static_cast<uint8_t>(g.jobs[j]);
+            const int gn = static_cast<int>(g.jobs.size());
+            HIP_TRY(pnet::launch_downsample_jobs(ctx->d_jobs, idx, gn, g.h4, g.w4, nn->b.x0, st));
+            HIP_TRY(nn_forward(nn, gn, g.h4, g.w4, st, nn->props_all + g.at));
         }
         return VC1ME_OK;
     };
+    // the graph of such a step also depends on the network's buffers
+    std::string tag = "nn";
+    for (const void* q : {static_cast<const void*>(nn->b.x0), static_cast<const void*>(nn->b.act[0]),
+                          static_cast<const void*>(nn->b.act[1]), static_cast<const void*>(nn->b.act[2]),
+                          static_cast<const void*>(nn->b.act[3]), static_cast<const void*>(nn->b.pooled),
+                          static_cast<const void*>(nn->props_all), static_cast<const void*>(nn->p),
+                          static_cast<const void*>(nn->wt[0]), static_cast<const void*>(nn->wt[1]),
+                          static_cast<const void*>(nn->wt[2]), static_cast<const void*>(nn->wt[3])})
+        tag += "," + std::to_string(reinterpret_cast<uintptr_t>(q));
     const float logit = std::log(threshold / (1.0f - threshold));
-    return run_jobs(ctx, jobs, n, nn->props.data(), logit, forward, nullptr, false, ctx->learned_candidates != 0);
+    return run_jobs(ctx, jobs, n, nn->props.data(), logit, forward, nullptr, false, ctx->learned_candidates != 0, tag);
 }
 
 }  // extern "C"

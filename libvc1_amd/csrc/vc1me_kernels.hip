@@ -2729,10 +2729,30 @@ __global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ 
 // The starting state of the relaxation: every macroblock's guess analysis at
 // once (its outputs are all overwritten by pass 0, which analyses every
 // macroblock for real).
+//
+// Work of a (macroblocks x jobs) grid, XCD-aware: the dispatcher deals
+// workgroups to the eight XCDs in turn (linear id L runs on XCD L % 8), and
+// neighbouring macroblocks read mostly the same reference windows, phase
+// planes and signature index.  Workgroup L therefore takes item
+// band(L % 8) + L / 8 of the job-major item order, so that each XCD works
+// through one contiguous band of macroblocks (and of jobs) and its L2 holds
+// what its neighbours read.  A bijection of [0, T): XCDs 0..r-1 take q + 1
+// items, the others q (T = 8q + r).
+struct GridItem { int job, pos; };
+__device__ __forceinline__ GridItem xcd_banded_item() {
+    const int T = static_cast<int>(gridDim.x * gridDim.y);
+    const int L = static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x);
+    const int x = L & 7, k = L >> 3, q = T >> 3, r = T & 7;
+    const int item = x * q + imin(x, r) + k;
+    const int per = static_cast<int>(gridDim.x);
+    return {item / per, item - (item / per) * per};
+}
+
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_guess(const JobDesc* __restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t smem[];
-    const JobDesc* J = &jobs[blockIdx.y];
-    const int pos = static_cast<int>(blockIdx.x);
+    const GridItem it = xcd_banded_item();
+    const JobDesc* J = &jobs[it.job];
+    const int pos = it.pos;
     if (pos >= J->mbw * J->mbh) return;
     run_mb<true>(J, pos % J->mbw, pos / J->mbw, smem);
 }
@@ -2763,10 +2783,11 @@ __device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_relax0(const JobDesc* __restrict__ jobs,
                                                                                        RelaxQueue Q) {
     extern __shared__ __align__(16) uint8_t smem[];
-    const int job = static_cast<int>(blockIdx.y);
+    const GridItem it = xcd_banded_item();
+    const int job = it.job;
     const JobDesc* J = &jobs[job];
     const int mbw = J->mbw;
-    const int pos = static_cast<int>(blockIdx.x);
+    const int pos = it.pos;
     if (pos >= mbw * J->mbh) return;
     const NbState after = run_mb(J, pos % mbw, pos / mbw, smem);
     if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, 1, Q);

@@ -14,6 +14,8 @@
 // D[4(l/16)+j][l%16], j = 0..3.
 #include "vc1me_nn_internal.h"
 
+#include <atomic>
+
 namespace pnet {
 namespace {
 
@@ -47,12 +49,9 @@ __device__ __forceinline__ v4s load4(const uint16_t* p, bool ok) {
 // The ref block is taken displaced by the global vector gxy = {dx, dy} (px,
 // device memory, null = none), so the network sees ref aligned with the
 // frame's dominant motion and predicts residuals from it.
-__global__ void k_downsample(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ ref, int W, int H, int h4,
-                             int w4, const int32_t* __restrict__ gxy, uint16_t* __restrict__ x0) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= h4 * w4) return;
+__device__ __forceinline__ void downsample_cell(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ ref, int W,
+                                                int H, int w4, int i, int gx, int gy, uint16_t* __restrict__ x0) {
     const int y = i / w4, x = i % w4;
-    const int gx = gxy ? gxy[0] : 0, gy = gxy ? gxy[1] : 0;
     uint32_t sc = 0, sr = 0;
     for (int dy = 0; dy < 4; ++dy)
         for (int dx = 0; dx < 4; ++dx) {
@@ -66,6 +65,25 @@ __global__ void k_downsample(const uint8_t* __restrict__ cur, const uint8_t* __r
     x0[2 * i + 1] = to_bf16(static_cast<float>(sr) / (16.0f * 255.0f) - 0.5f);
 }
 
+__global__ void k_downsample(const uint8_t* __restrict__ cur, const uint8_t* __restrict__ ref, int W, int H, int h4,
+                             int w4, const int32_t* __restrict__ gxy, uint16_t* __restrict__ x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h4 * w4) return;
+    downsample_cell(cur, ref, W, H, w4, i, gxy ? gxy[0] : 0, gxy ? gxy[1] : 0, x0);
+}
+
+// The same for the pairs of an analysis step, read from its job table on the
+// device (frames and global vectors), so that a captured step replays on new
+// frames: image b of the batch is job idx.k[b]; grid (cells, images).
+__global__ void k_downsample_jobs(const vc1me::JobDesc* __restrict__ jobs, JobIndex idx, int h4, int w4,
+                                  uint16_t* __restrict__ x0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= h4 * w4) return;
+    const vc1me::JobDesc* J = &jobs[idx.k[blockIdx.y]];
+    downsample_cell(J->cy, J->ry, J->W, J->H, w4, i, J->g->dx, J->g->dy,
+                    x0 + static_cast<size_t>(blockIdx.y) * h4 * w4 * 2);
+}
+
 // fp32 host-provided input [b][h4][w4][2] -> bf16 (gradient checks)
 __global__ void k_to_bf16(const float* __restrict__ in, uint16_t* __restrict__ out, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,63 +92,84 @@ __global__ void k_to_bf16(const float* __restrict__ in, uint16_t* __restrict__ o
 
 // ------------------------------------------------------------------ convolution
 
-// Forward of one 3x3 convolution with dilation d: one wave computes 16 output
-// pixels of one row x 32 output channels.  K = 9 taps x Cin (Cin 32: 18 MFMA
-// steps; Cin 2: taps x channels packed into one 32-wide K, two steps).
-// wt: bf16 [cout][9][cin].  out = ReLU(conv + bias), bf16.
-__global__ void __launch_bounds__(64) k_conv_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wt,
-                                                 const float* __restrict__ bias, uint16_t* __restrict__ out, int nb,
-                                                 int h, int w, int cin, int dil) {
+// Forward of one 3x3 convolution with dilation dil, as D^T = W X^T: the wave
+// keeps the layer's weights in registers as MFMA A operands (M = output
+// channel, K = tap x input channel, tap-major) and
+// walks tiles of 16 output pixels of one row (N = pixel), issuing every input
+// load of a tile before its MFMAs.  D[4(l/16)+j][l%16] gives lane l four
+// consecutive output channels of one pixel: one 8-byte store per 16 channels.
+// Cin 32: 18 K steps (tap t = step / 2, channel half step % 2); Cin 2: taps x
+// channels packed into one K of 32 (18 used), two steps.
+template <int kCin>
+__global__ void __launch_bounds__(64) k_conv_fwd_regw(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wt,
+                                                      const float* __restrict__ bias, uint16_t* __restrict__ out,
+                                                      int nb, int h, int w, int dil) {
+    constexpr int kSteps = kCin == 32 ? 18 : 2;
     const int l = threadIdx.x;
-    const int tiles_x = (w + 15) / 16;
-    const int tile = blockIdx.x;
-    const int b = tile / (h * tiles_x), rem = tile - b * h * tiles_x;
-    const int y = rem / tiles_x, x0 = (rem - y * tiles_x) * 16;
-    if (b >= nb) return;
-    const int px = x0 + (l & 15);     // A row: output pixel
-    const int kq = 4 * (l >> 4);      // first K index of this lane's slice
-    v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    const uint16_t* xb = x + static_cast<size_t>(b) * h * w * cin;
-    if (cin == 32) {
-        for (int t = 0; t < 9; ++t) {
-            const int sy = y + (t / 3 - 1) * dil, sx = px + (t % 3 - 1) * dil;
-            const bool in = sy >= 0 && sy < h && sx >= 0 && sx < w && px < w;
-            for (int cb = 0; cb < 2; ++cb) {
-                const int c0 = cb * 16 + kq;
-                const v4s a = load4(xb + (static_cast<size_t>(sy) * w + sx) * 32 + c0, in);
-                const v4s b0 = load4(wt + (static_cast<size_t>(l & 15) * 9 + t) * 32 + c0, true);
-                const v4s b1 = load4(wt + (static_cast<size_t>(16 + (l & 15)) * 9 + t) * 32 + c0, true);
-                acc0 = mfma(a, b0, acc0);
-                acc1 = mfma(a, b1, acc1);
-            }
-        }
-    } else {
-        // cin == 2: K index k = tap * 2 + channel, 18 used of 32
-        for (int ks = 0; ks < 2; ++ks) {
-            v4s a = {0, 0, 0, 0}, b0 = {0, 0, 0, 0}, b1 = {0, 0, 0, 0};
+    const int m = l & 15, kq = 4 * (l >> 4);
+    v4s wa[kSteps], wb[kSteps];
+#pragma unroll
+    for (int ks = 0; ks < kSteps; ++ks) {
+        if (kCin == 32) {
+            const int t = ks >> 1, c0 = (ks & 1) * 16 + kq;
+            wa[ks] = load4(wt + (static_cast<size_t>(m) * 9 + t) * 32 + c0, true);
+            wb[ks] = load4(wt + (static_cast<size_t>(16 + m) * 9 + t) * 32 + c0, true);
+        } else {
+#pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int k = ks * 16 + kq + j;
-                if (k >= 18) continue;
-                const int t = k >> 1, c = k & 1;
-                const int sy = y + (t / 3 - 1) * dil, sx = px + (t % 3 - 1) * dil;
-                if (sy >= 0 && sy < h && sx >= 0 && sx < w && px < w)
-                    a[j] = static_cast<short>(xb[(static_cast<size_t>(sy) * w + sx) * 2 + c]);
-                b0[j] = static_cast<short>(wt[(static_cast<size_t>(l & 15) * 9 + t) * 2 + c]);
-                b1[j] = static_cast<short>(wt[(static_cast<size_t>(16 + (l & 15)) * 9 + t) * 2 + c]);
+                wa[ks][j] = k < 18 ? static_cast<short>(wt[m * 18 + k]) : 0;
+                wb[ks][j] = k < 18 ? static_cast<short>(wt[(16 + m) * 18 + k]) : 0;
             }
-            acc0 = mfma(a, b0, acc0);
-            acc1 = mfma(a, b1, acc1);
         }
     }
-    // D[pixel 4(l/16)+j][cout l%16]
-    uint16_t* ob = out + static_cast<size_t>(b) * h * w * 32;
-    for (int j = 0; j < 4; ++j) {
-        const int opx = x0 + 4 * (l >> 4) + j;
-        if (opx >= w) continue;
-        const int co = l & 15;
-        const float v0 = fmaxf(acc0[j] + bias[co], 0.f), v1 = fmaxf(acc1[j] + bias[16 + co], 0.f);
-        ob[(static_cast<size_t>(y) * w + opx) * 32 + co] = to_bf16(v0);
-        ob[(static_cast<size_t>(y) * w + opx) * 32 + 16 + co] = to_bf16(v1);
+    float b0[4], b1[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) { b0[j] = bias[kq + j]; b1[j] = bias[16 + kq + j]; }
+    const int tiles_x = (w + 15) / 16;
+    const int per = h * tiles_x, ntiles = nb * per;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int b = tile / per, rem = tile - b * per;
+        const int y = rem / tiles_x, px = (rem - y * tiles_x) * 16 + m;
+        const uint16_t* xb = x + static_cast<size_t>(b) * h * w * kCin;
+        v4s xv[kSteps];
+#pragma unroll
+        for (int ks = 0; ks < kSteps; ++ks) {
+            if (kCin == 32) {
+                const int t = ks >> 1, c0 = (ks & 1) * 16 + kq;
+                const int sy = y + (t / 3 - 1) * dil, sx = px + (t % 3 - 1) * dil;
+                const bool in = sy >= 0 && sy < h && sx >= 0 && sx < w && px < w;
+                xv[ks] = load4(xb + (static_cast<size_t>(sy) * w + sx) * 32 + c0, in);
+            } else {
+                v4s a = {0, 0, 0, 0};
+#pragma unroll
+                for (int pr = 0; pr < 2; ++pr) {
+                    const int k = ks * 16 + kq + 2 * pr, t = k >> 1;   // channels 0 and 1 of tap t
+                    const int sy = y + (t / 3 - 1) * dil, sx = px + (t % 3 - 1) * dil;
+                    if (k < 18 && sy >= 0 && sy < h && sx >= 0 && sx < w && px < w) {
+                        const uint32_t u = *reinterpret_cast<const uint32_t*>(xb + (static_cast<size_t>(sy) * w + sx) * 2);
+                        a[2 * pr] = static_cast<short>(u & 0xffffu);
+                        a[2 * pr + 1] = static_cast<short>(u >> 16);
+                    }
+                }
+                xv[ks] = a;
+            }
+        }
+        v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < kSteps; ++ks) {
+            acc0 = mfma(wa[ks], xv[ks], acc0);
+            acc1 = mfma(wb[ks], xv[ks], acc1);
+        }
+        if (px < w) {
+            uint16_t* o = out + ((static_cast<size_t>(b) * h + y) * w + px) * 32 + kq;
+            uint32_t lo0 = to_bf16(fmaxf(acc0[0] + b0[0], 0.f)) | (static_cast<uint32_t>(to_bf16(fmaxf(acc0[1] + b0[1], 0.f))) << 16);
+            uint32_t hi0 = to_bf16(fmaxf(acc0[2] + b0[2], 0.f)) | (static_cast<uint32_t>(to_bf16(fmaxf(acc0[3] + b0[3], 0.f))) << 16);
+            uint32_t lo1 = to_bf16(fmaxf(acc1[0] + b1[0], 0.f)) | (static_cast<uint32_t>(to_bf16(fmaxf(acc1[1] + b1[1], 0.f))) << 16);
+            uint32_t hi1 = to_bf16(fmaxf(acc1[2] + b1[2], 0.f)) | (static_cast<uint32_t>(to_bf16(fmaxf(acc1[3] + b1[3], 0.f))) << 16);
+            *reinterpret_cast<uint2*>(o) = make_uint2(lo0, hi0);
+            *reinterpret_cast<uint2*>(o + 16) = make_uint2(lo1, hi1);
+        }
     }
 }
 
@@ -359,6 +398,13 @@ hipError_t launch_downsample(const uint8_t* cur, const uint8_t* ref, int W, int 
     hipLaunchKernelGGL(k_downsample, dim3((n + 255) / 256), dim3(256), 0, s, cur, ref, W, H, h4, w4, gxy, x0);
     return hipGetLastError();
 }
+hipError_t launch_downsample_jobs(const vc1me::JobDesc* jobs, const JobIndex& idx, int nb, int h4, int w4, uint16_t* x0,
+                                  hipStream_t s) {
+    const int n = h4 * w4;
+    if (nb <= 0 || nb > 255 || n <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_downsample_jobs, dim3((n + 255) / 256, nb), dim3(256), 0, s, jobs, idx, h4, w4, x0);
+    return hipGetLastError();
+}
 hipError_t launch_to_bf16(const float* in, uint16_t* out, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
     return hipGetLastError();
@@ -366,7 +412,31 @@ hipError_t launch_to_bf16(const float* in, uint16_t* out, int n, hipStream_t s) 
 hipError_t launch_conv_fwd(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* out, int nb, int h, int w,
                            int cin, int dil, hipStream_t s) {
     const int tiles = nb * h * ((w + 15) / 16);
-    hipLaunchKernelGGL(k_conv_fwd, dim3(tiles), dim3(64), 0, s, x, wt, bias, out, nb, h, w, cin, dil);
+    if (tiles <= 0) return hipSuccess;
+    // resident waves that each keep the weights in registers and walk tiles
+    // (one round of the device's occupancy), fewer for small inputs
+    static std::atomic<int> resident[2] = {{0}, {0}};
+    int r = resident[cin == 32 ? 1 : 0].load(std::memory_order_relaxed);
+    if (!r) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, cin == 32 ? reinterpret_cast<const void*>(k_conv_fwd_regw<32>)
+                                   : reinterpret_cast<const void*>(k_conv_fwd_regw<2>), 64, 0) == hipSuccess &&
+            cus > 0 && per_cu > 0)
+            r = cus * per_cu;
+        else
+            r = 3072;
+        resident[cin == 32 ? 1 : 0].store(r, std::memory_order_relaxed);
+    }
+    const int grid = tiles < r ? tiles : r;
+    if (cin == 32)
+        hipLaunchKernelGGL(k_conv_fwd_regw<32>, dim3(grid), dim3(64), 0, s, x, wt, bias, out, nb, h, w, dil);
+    else if (cin == 2)
+        hipLaunchKernelGGL(k_conv_fwd_regw<2>, dim3(grid), dim3(64), 0, s, x, wt, bias, out, nb, h, w, dil);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 hipError_t launch_conv_bwd_data(const uint16_t* dz, const uint16_t* wtt, const uint16_t* xact, uint16_t* dx, int nb,

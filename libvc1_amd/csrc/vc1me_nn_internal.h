@@ -3,7 +3,14 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "vc1me_internal.h"
+
 namespace pnet {
+
+// the jobs of a step that form one batch of the network (by value)
+struct JobIndex {
+    uint8_t k[256];
+};
 
 constexpr int kChannels = 32;
 constexpr int kLayers = 4;            // 3x3 convolutions, dilations 1, 2, 4, 8
@@ -19,6 +26,8 @@ inline int head_offset() { return (kW1 + 32) + 3 * (kW + 32); }
 
 hipError_t launch_downsample(const uint8_t* cur, const uint8_t* ref, int W, int H, int h4, int w4, const int32_t* gxy,
                              uint16_t* x0, hipStream_t s);
+hipError_t launch_downsample_jobs(const vc1me::JobDesc* jobs, const JobIndex& idx, int nb, int h4, int w4, uint16_t* x0,
+                                  hipStream_t s);
 hipError_t launch_to_bf16(const float* in, uint16_t* out, int n, hipStream_t s);
 hipError_t launch_conv_fwd(const uint16_t* x, const uint16_t* wt, const float* bias, uint16_t* out, int nb, int h, int w,
                            int cin, int dil, hipStream_t s);
