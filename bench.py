@@ -221,29 +221,52 @@ def device_peak_ops():
     return cus, clk, cus * 64 * clk
 
 
-def learned_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, threshold=0.5):
+def learned_leg(lanes, jobs, plist, plain_value, plain_value_one, steps, warmup, mbs_per_step, threshold=0.5):
     """The learned-seeded form on the same step: timed like the plain form
-    (network forward included), then judged by the cost report against the
-    plain form, with two ablations that remove the network's contribution."""
+    (network forward inside each captured step, the same steps in flight),
+    then judged by the cost report against the plain form, with two
+    ablations that remove the network's contribution."""
     import numpy as np
-    from libvc1_amd import abi, learned
+    from libvc1_amd import abi, batching, learned
     from libvc1_amd.backend import ProposalNet
 
+    be = lanes[0]
     params = learned.load_weights()
-    net = ProposalNet(be)
-    net.set_params(params)
-    for _ in range(warmup):
-        be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
-    elapsed = time.perf_counter() - t0
+    nets = []
+    for ln in lanes:
+        net = ProposalNet(ln)
+        net.set_params(params)
+        nets.append(net)
+        ln.set_graphs(True)
+    idx = {id(ln): i for i, ln in enumerate(lanes)}
+
+    def learned_step(ln, k):
+        ln.analyze_step_learned(nets[idx[id(ln)]], jobs, plist, threshold, want_results=False)
+
+    def timed(fn):
+        batching.run_in_flight(lanes, max(1, warmup) * len(lanes), fn)
+        t0 = time.perf_counter()
+        batching.run_in_flight(lanes, steps, fn)
+        return time.perf_counter() - t0
+
+    elapsed = timed(learned_step)
     value = mbs_per_step * steps / elapsed
-    log(f"learned: {elapsed * 1e3 / steps:.1f} ms/step, {value:,.0f} MB/s ({value / plain_value:.3f}x plain)")
+    t0 = time.perf_counter()
+    for k in range(steps):
+        learned_step(be, k)
+    value_one = mbs_per_step * steps / (time.perf_counter() - t0)
+    log(f"learned: {elapsed * 1e3 / steps:.1f} ms/step, {value:,.0f} MB/s ({value / plain_value:.3f}x plain; "
+        f"one in flight {value_one / plain_value_one:.3f}x)")
+    for ln in lanes:
+        ln.set_graphs(False)
+    net = nets[0]
     judged = learned.cost_compare(be, net, jobs, plist, threshold)
     log(f"learned cost: {judged}")
 
     # ablations: the same seeded search without the network's candidates
+    # (timed one step at a time through vc1me_analyze_step_proposals, which
+    # uploads the proposals from the host: compared with the plain form one
+    # step at a time)
     mbw, mbh = abi.mb_grid(plist[0].width, plist[0].height)
     res = be.analyze_step(jobs, plist)
     glob = np.array([[r.struct.global_dx, r.struct.global_dy] for r in res], np.int32)
@@ -268,31 +291,36 @@ def learned_leg(be, jobs, plist, plain_value, steps, warmup, mbs_per_step, thres
             ca = np.concatenate([be.last_mb_costs(j, mbw * mbh)[0] for j in range(len(jobs))]).astype(np.float64)
         finally:
             be.set_cost_report(False)
-        abl[name] = {"value": round(v, 1), "speedup_vs_plain": round(v / plain_value, 4),
+        abl[name] = {"value_one_in_flight": round(v, 1), "speedup_vs_plain_one_in_flight": round(v / plain_value_one, 4),
                      "cost_rise": round(float(ca.sum() / cp.sum() - 1), 6),
                      "frac_worse": round(float(np.mean(ca > cp)), 6)}
         log(f"ablation {name}: {abl[name]}")
     # the network as a confidence gate only (vc1me_set_learned_candidates(0))
-    be.set_learned_candidates(False)
+    for ln in lanes:
+        ln.set_learned_candidates(False)
+        ln.set_graphs(True)
     try:
-        be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            be.analyze_step_learned(net, jobs, plist, threshold, want_results=False)
-        gv = mbs_per_step * steps / (time.perf_counter() - t0)
+        gv = mbs_per_step * steps / timed(learned_step)
+        for ln in lanes:
+            ln.set_graphs(False)
         gj = learned.cost_compare(be, net, jobs, plist, threshold)
     finally:
-        be.set_learned_candidates(True)
+        for ln in lanes:
+            ln.set_learned_candidates(True)
+            ln.set_graphs(False)
     gate_only = {"value": round(gv, 1), "speedup_vs_plain": round(gv / plain_value, 4),
                  "cost_rise": round(gj["cost_rise"], 6), "frac_worse": round(gj["frac_worse"], 6),
                  "seeded_fraction": round(gj["seeded_fraction"], 4)}
     log(f"learned, gate only: {gate_only}")
-    net.close()
+    for n in nets:
+        n.close()
     meta_path = learned.WEIGHTS + ".json"
     sha = json.load(open(meta_path)).get("sha256") if os.path.exists(meta_path) else None
     return {
         "value": round(value, 1), "unit": "MB/s", "speedup_vs_plain": round(value / plain_value, 4),
-        "ms_per_step": round(elapsed * 1e3 / steps, 3), "threshold": threshold,
+        "ms_per_step": round(elapsed * 1e3 / steps, 3), "steps_in_flight": len(lanes),
+        "value_one_in_flight": round(value_one, 1),
+        "speedup_vs_plain_one_in_flight": round(value_one / plain_value_one, 4), "threshold": threshold,
         "cost_rise": round(judged["cost_rise"], 6), "frac_worse": round(judged["frac_worse"], 6),
         "mean_excess_per_mb": round(judged["mean_excess"], 3), "seeded_fraction": round(judged["seeded_fraction"], 4),
         "cost": "sum over MBs of SAD + lambda*R of the final 1-MV vector (exact integers 20*C), learned vs plain",
@@ -554,22 +582,29 @@ def hard_frames():
     return out
 
 
-def hard_leg(be, gen, steps):
+def hard_leg(lanes, gen, steps):
     """The 2160p hard-motion workload: 16 concurrent streams, one frame step
     of all of them per analysis step (16 pairs at 3840x2160, R 1024), plain
     (one captured graph per step) and learned-seeded, at a mid-scene step and
     at the step across the scene cut, with the learned form's cost rise."""
     import numpy as np
-    from libvc1_amd import abi, learned
+    from libvc1_amd import abi, batching, learned
     from libvc1_amd.backend import ProposalNet
     p = abi.default_params(3840, 2160, pqindex=8, motion_search_range=1024, motion_local_search_range=32,
                            mode=abi.MODE_MIXED_MV, me_quality=2, allow_intra=1)
     mbs = 240 * 135
-    net = ProposalNet(be)
-    net.set_params(learned.load_weights())
+    be = lanes[0]
+    nets = []
+    for ln in lanes:
+        nets.append(ProposalNet(ln))
+        nets[-1].set_params(learned.load_weights())
+    net = nets[0]
+    idx = {id(ln): i for i, ln in enumerate(lanes)}
     res = {}
     uploaded = []
     try:
+        for ln in lanes:
+            ln.set_graphs(True)
         for label, t in HARD_STEPS:
             jobs = []
             for _, d in gen:
@@ -577,19 +612,25 @@ def hard_leg(be, gen, steps):
                 uploaded += [fc, fr]
                 jobs.append((fc, fr))
             plist = [p] * len(jobs)
-            be.set_graphs(True)
-            be.analyze_step(jobs, plist, want_results=False)
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                be.analyze_step(jobs, plist, want_results=False)
-            plain_s = (time.perf_counter() - t0) / steps
-            be.set_graphs(False)
-            be.analyze_step_learned(net, jobs, plist, 0.5, want_results=False)
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                be.analyze_step_learned(net, jobs, plist, 0.5, want_results=False)
-            learned_s = (time.perf_counter() - t0) / steps
+
+            def plain(ln, k):
+                ln.analyze_step(jobs, plist, want_results=False)
+
+            def seeded(ln, k):
+                ln.analyze_step_learned(nets[idx[id(ln)]], jobs, plist, 0.5, want_results=False)
+            # the same frame step repeated, `len(lanes)` steps in flight
+            secs = {}
+            for name, fn in (("plain", plain), ("learned", seeded)):
+                batching.run_in_flight(lanes, len(lanes), fn)
+                t0 = time.perf_counter()
+                batching.run_in_flight(lanes, steps, fn)
+                secs[name] = (time.perf_counter() - t0) / steps
+            plain_s, learned_s = secs["plain"], secs["learned"]
+            for ln in lanes:
+                ln.set_graphs(False)
             cmp = learned.cost_compare(be, net, jobs, plist, 0.5)
+            for ln in lanes:
+                ln.set_graphs(True)
             n_mbs = len(jobs) * mbs
             res[label] = {"frame": t, "ms_per_step": round(plain_s * 1e3, 2), "value": round(n_mbs / plain_s, 1),
                           "learned_ms_per_step": round(learned_s * 1e3, 2),
@@ -599,11 +640,13 @@ def hard_leg(be, gen, steps):
                           "learned_seeded_fraction": round(cmp["seeded_fraction"], 4)}
             log(f"hard motion {label}: {res[label]}")
     finally:
-        be.set_graphs(False)
-        net.close()
+        for ln in lanes:
+            ln.set_graphs(False)
+        for n in nets:
+            n.close()
         for f in uploaded:
             f.release()
-    return {"streams": HARD_STREAMS, "seed": HARD_SEED, "unit": "MB/s", "steps": res,
+    return {"streams": HARD_STREAMS, "seed": HARD_SEED, "unit": "MB/s", "steps": res, "steps_in_flight": len(lanes),
             "config": "3840x2160, 16 streams in one step per frame step, R 1024, local 32, Mixed-MV qpel SATD, "
                       "velocities up to 200 px/frame, cuts every 30 frames, 5% duplicates"}
 
@@ -628,16 +671,18 @@ def batched_frames(scale: float):
     return out
 
 
-def batched_leg(be, generated, scale: float, check: bool):
-    """Every P pair of the 64 streams through the batching runner (steps and
-    modes from the tuned table, frames resident), timed once after a warm-up
-    run; one 720x480 pair is checked against the oracle."""
+def batched_leg(lanes, generated, scale: float, check: bool):
+    """Every P pair of the 64 streams through the batching runner (steps,
+    modes and steps in flight from the tuned table, frames resident), timed
+    once after a warm-up run; one 720x480 pair is checked against the
+    oracle."""
     from libvc1_amd import batching
+    be = lanes[0]
     streams = [batching.Stream(w, h, [be.upload(p) for p in pics], batching.stream_params(w, h), seed)
                for seed, w, h, pics in generated]
     try:
-        batching.run(be, streams)                      # warm-up (graph capture, allocations)
-        out = batching.run(be, streams)
+        batching.run(lanes, streams)                   # warm-up (graph capture, allocations)
+        out = batching.run(lanes, streams)
         parity = None
         if check:
             sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -659,7 +704,8 @@ def batched_leg(be, generated, scale: float, check: bool):
                      "Mixed-MV qpel SATD; steps and modes from the tuned table",
            "per_resolution": {k: {"pairs": v["pairs"], "macroblocks": v["macroblocks"], "seconds": v["seconds"],
                                   "mode": f"{v['setting']['schedule']}{'+graph' if v['setting']['graphs'] else ''}"
-                                          f" x{v['setting']['jobs_per_step']}"}
+                                          f" x{v['setting']['jobs_per_step']}, {v['setting'].get('in_flight', 1)}"
+                                          " in flight"}
                               for k, v in out["per_resolution"].items()},
            "parity": parity}
     log(f"batched: {res['value']:,.0f} MB/s over {res['pairs']} pairs; {res['per_resolution']}; parity {parity}")
@@ -669,7 +715,7 @@ def batched_leg(be, generated, scale: float, check: bool):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -681,6 +727,8 @@ def main() -> int:
     ap.add_argument("--no-scorer", action="store_true", help="skip the batch-scorer latency leg")
     ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-core leg")
     ap.add_argument("--no-bframes", action="store_true", help="skip the B-picture leg")
+    ap.add_argument("--in-flight", type=int, default=0,
+                    help="steps kept in flight, one context and host thread each (0: the tuned setting)")
     ap.add_argument("--batched-scale", type=float, default=0.1,
                     help="stream lengths (8..120 frames) are scaled by this in the batched leg")
     args = ap.parse_args()
@@ -691,7 +739,7 @@ def main() -> int:
               "(streams and frame pairs shard across GPUs without communication)", file=sys.stderr)
         return 2
 
-    from libvc1_amd import Backend, abi
+    from libvc1_amd import Backend, abi, batching
 
     params = workload_params()
     pairs, pics = make_frames()
@@ -703,36 +751,46 @@ def main() -> int:
     jobs = [(frames[c], frames[r]) for c, r in pairs]
     plist = [params] * len(jobs)
     mbs_per_step = len(jobs) * ((WIDTH + 15) // 16) * ((HEIGHT + 15) // 16)
+    flight = args.in_flight or batching.best_in_flight(batching.load_table(be), WIDTH, HEIGHT, len(jobs))
+    # steps in flight: one context (own stream and device state) per host
+    # thread, all on the frames uploaded by the first
+    lanes = [be] + [Backend(0) for _ in range(flight - 1)]
 
     # the timed steps: each step one captured HIP graph (relaxation passes
     # driven from the device, every per-macroblock field and the frame
     # aggregates computed; nothing copied back but the step summary)
-    be.set_graphs(True)
-    for i in range(args.warmup):
-        be.analyze_step(jobs, plist, want_results=False)
-        log(f"warmup step {i + 1}/{args.warmup}: {be.timing().step_ms:.1f} ms (graph {be.timing().graph_replay})")
-    passes, analyses = [], []
+    for ln in lanes:
+        ln.set_graphs(True)
+
+    def plain_step(ln, k):
+        ln.analyze_step(jobs, plist, want_results=False)
+        t = ln.timing()
+        return int(t.relax_passes), int(t.relax_analyses), int(t.graph_replay), int(t.schedule)
+    batching.run_in_flight(lanes, max(1, args.warmup) * len(lanes), plain_step)
+    log(f"warmup: {max(1, args.warmup)} step(s) on each of {len(lanes)} context(s)")
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        be.analyze_step(jobs, plist, want_results=False)
-        t = be.timing()
-        passes.append(int(t.relax_passes))
-        analyses.append(int(t.relax_analyses))
+    recs = batching.run_in_flight(lanes, args.steps, plain_step)
     elapsed = time.perf_counter() - t0
-    graph_replays = int(t.graph_replay)
     ms_per_step = elapsed * 1e3 / args.steps
     value = mbs_per_step * args.steps / elapsed
+    # the same steps one at a time on one context
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        plain_step(be, k)
+    value_one = mbs_per_step * args.steps / (time.perf_counter() - t0)
     # download-inclusive rate (every field of every pair back to the host)
     t1 = time.perf_counter()
-    for i in range(args.steps):
-        results = be.analyze_step(jobs, plist, want_results=True)
+    batching.run_in_flight(lanes, args.steps, lambda ln, k: ln.analyze_step(jobs, plist, want_results=True))
     with_dl = mbs_per_step * args.steps / (time.perf_counter() - t1)
-    be.set_graphs(False)
-    schedule = {"kind": "relax" if t.schedule == abi.SCHEDULE_RELAX else "diagonal",
-                "launch": "one captured HIP graph per step" if graph_replays == 1 else "eager",
-                "passes_per_step": passes, "analyses_per_step": analyses}
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s, with download "
-        f"{with_dl:,.0f} MB/s, schedule {schedule}")
+    results = be.analyze_step(jobs, plist, want_results=True)
+    for ln in lanes:
+        ln.set_graphs(False)
+    schedule = {"kind": "relax" if recs[-1][3] == abi.SCHEDULE_RELAX else "diagonal",
+                "launch": "one captured HIP graph per step" if recs[-1][2] == 1 else "eager",
+                "steps_in_flight": len(lanes),
+                "passes_per_step": [r[0] for r in recs], "analyses_per_step": [r[1] for r in recs]}
+    log(f"timed {args.steps} steps, {len(lanes)} in flight: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s "
+        f"(one in flight {value_one:,.0f}), with download {with_dl:,.0f} MB/s, schedule {schedule}")
 
     # the same steps launched one by one with HIP events around every launch
     # of the macroblock kernel: its launch durations for the roofline
@@ -818,18 +876,18 @@ def main() -> int:
 
     learned_out = None
     if not args.no_learned:
-        learned_out = learned_leg(be, jobs, plist, value, args.steps, args.warmup, mbs_per_step)
+        learned_out = learned_leg(lanes, jobs, plist, value, value_one, args.steps, args.warmup, mbs_per_step)
 
     scorer = None if args.no_scorer else scorer_leg(be, pics, frames)
 
     hard = None
     if hard_streams is not None:
-        hard = hard_leg(be, hard_streams, max(2, args.steps // 4))
+        hard = hard_leg(lanes, hard_streams, max(2, args.steps // 4))
         hard_streams = None
 
     batched = None
     if batched_streams is not None:
-        batched = batched_leg(be, batched_streams, args.batched_scale, not args.no_check)
+        batched = batched_leg(lanes, batched_streams, args.batched_scale, not args.no_check)
         batched_streams = None
 
     cpu = None
@@ -855,6 +913,8 @@ def main() -> int:
                                "PQINDEX 8, plain GPU ME (bit-exact)", "width": WIDTH, "height": HEIGHT,
                    "p_frames_per_step": len(jobs), "macroblocks_per_step": mbs_per_step, "seed": SEED},
         "form": "plain",
+        "steps_in_flight": len(lanes),
+        "value_one_in_flight": round(value_one, 1),
         "schedule": schedule,
         "value_with_download": round(with_dl, 1),
         "parity": parity,
@@ -870,6 +930,8 @@ def main() -> int:
     }
     for f in frames.values():
         f.release()
+    for ln in lanes[1:]:
+        ln.close()
     be.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -141,6 +141,7 @@ class Backend:
     """One vc1me context on a HIP device."""
 
     def __init__(self, device: int = 0):
+        self.device = device
         self.lib = load_library()
         ctx = C.c_void_p()
         self._check(self.lib.vc1me_create(device, C.byref(ctx)))

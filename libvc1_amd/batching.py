@@ -12,13 +12,19 @@ be replayed from a captured HIP graph (vc1me_set_graphs).  Grouping by raster
 keeps a step's launch count equal to its own frames' diagonal count, so small
 frames never wait on large ones.
 
+Steps may also be kept in flight: with k contexts (each with its own stream
+and device state) driven by k host threads, the latency-bound end of one
+step (the relaxation's last passes, each a chain of dependent analyses; the
+global search's dependent stages) overlaps the full-occupancy passes of the
+next.
+
 The autotuner measures, per raster, macroblock searches per second for each
 candidate step size under each launch mode (relaxation; sweep; sweep replayed
-from a graph), and persists the winners in a JSON table per device
-(``tuning/<device>.json``).  The design has one wave per macroblock by
-construction, so the tuned knobs are the step composition, the schedule and
-the launch mode; every mode gives the same results, so the table changes
-speed only.
+from a graph) and each number of steps in flight, and persists the winners in
+a JSON table per device (``tuning/<device>.json``).  The design has one wave
+per macroblock by construction, so the tuned knobs are the step composition,
+the schedule, the launch mode and the steps in flight; every setting gives
+the same results, so the table changes speed only.
 
     python -m libvc1_amd.batching tune  [--resolutions 720x480,1280x720,1920x1080,3840x2160]
     python -m libvc1_amd.batching bench [--streams 64] [--seed 5] [--length-scale 1.0]
@@ -41,9 +47,13 @@ from .backend import Backend
 TUNING_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning")
 RESOLUTIONS = ((720, 480), (1280, 720), (1920, 1080), (3840, 2160))
 STEP_CANDIDATES = (4, 8, 16, 32, 64, 128, 255)
-DEFAULT_SETTING = {"jobs_per_step": 16, "schedule": "relax", "graphs": False}
+IN_FLIGHT_CANDIDATES = (1, 2, 3)
+DEFAULT_SETTING = {"jobs_per_step": 16, "schedule": "relax", "graphs": False, "in_flight": 1}
 # launch modes the tuner tries: (schedule, graphs)
-MODES = (("relax", False), ("diagonal", False), ("diagonal", True))
+MODES = (("relax", True), ("diagonal", False), ("diagonal", True))
+# a trial key: step size, "r" for the relaxation, "g" for graph replay, "xK"
+# for K steps in flight (absent: one)
+TRIAL_KEY = re.compile(r"(\d+)(r?)(g?)(?:x(\d+))?")
 
 
 def log(msg: str) -> None:
@@ -104,6 +114,45 @@ def load_table(backend: Backend) -> dict:
     return {}
 
 
+def run_in_flight(backends, count: int, fn) -> list:
+    """fn(backend, k) for k in range(count), with one host thread per backend
+    (a context with its own stream), each taking the next k as soon as its
+    previous call returned: up to len(backends) steps in flight on the device.
+    Returns fn's results in k order; the first exception is re-raised once
+    every thread has stopped."""
+    import threading
+    backends = list(backends)
+    if len(backends) <= 1 or count <= 1:
+        return [fn(backends[0], k) for k in range(count)]
+    out = [None] * count
+    state = {"next": 0}
+    errors = []
+    lock = threading.Lock()
+
+    def lane(be):
+        while True:
+            with lock:
+                if errors or state["next"] >= count:
+                    return
+                k = state["next"]
+                state["next"] += 1
+            try:
+                out[k] = fn(be, k)
+            except BaseException as e:  # noqa: BLE001 - re-raised on the calling thread
+                with lock:
+                    errors.append(e)
+                return
+
+    threads = [threading.Thread(target=lane, args=(be,)) for be in backends]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
+    return out
+
+
 def setting_for(table: dict, width: int, height: int, pairs: int | None = None) -> dict:
     """The tuned setting of a raster.  With `pairs` (the pairs the runner has
     for the raster) and the table's trials, the best measured mode among step
@@ -114,74 +163,108 @@ def setting_for(table: dict, width: int, height: int, pairs: int | None = None) 
     if pairs is not None and trials:
         best = None
         sizes = sorted({int(re.match(r"\d+", k).group()) for k in trials})
-        fill = [b for b in sizes if b <= pairs] or sizes[:1]
         for k, rate in trials.items():
-            m = re.fullmatch(r"(\d+)(r?)(g?)", k)
-            if not m or int(m.group(1)) not in fill:
+            m = TRIAL_KEY.fullmatch(k)
+            if not m:
+                continue
+            b, flight = int(m.group(1)), int(m.group(4) or 1)
+            # the pairs must fill `flight` steps of the trial's size
+            if b * flight > pairs and (b, flight) != (sizes[0], 1):
                 continue
             if best is None or rate > best[0]:
-                best = (rate, int(m.group(1)), "relax" if m.group(2) else "diagonal", bool(m.group(3)))
+                best = (rate, b, "relax" if m.group(2) else "diagonal", bool(m.group(3)), flight)
         if best is not None:
-            # every trial grew with the step size: take all the pairs (at most
-            # 255 per step) in the mode measured best at the filled size
-            return {"jobs_per_step": max(best[1], min(pairs, 255)), "schedule": best[2], "graphs": best[3],
+            # every trial grew with the step size: spread the pairs over the
+            # steps in flight (at most 255 per step) in the mode measured best
+            per = max(best[1], min(255, -(-pairs // best[4])))
+            return {"jobs_per_step": per, "schedule": best[2], "graphs": best[3], "in_flight": best[4],
                     "mbs_per_s": best[0]}
     st = dict(table.get("settings", {}).get(raster, DEFAULT_SETTING))
     st.setdefault("schedule", "diagonal")      # tables written before the relaxation schedule
+    st.setdefault("in_flight", 1)              # tables written before steps in flight
     return st
+
+
+def best_in_flight(table: dict, width: int, height: int, per_step: int, default: int = 2) -> int:
+    """Steps in flight measured best for captured relaxation steps of
+    `per_step` pairs of a raster (trials "<n>rg" and "<n>rgx<k>"), else
+    `default`."""
+    trials = table.get("trials", {}).get(f"{width}x{height}", {})
+    best = None
+    for k, rate in trials.items():
+        m = TRIAL_KEY.fullmatch(k)
+        if m and int(m.group(1)) == per_step and m.group(2) and m.group(3):
+            flight = int(m.group(4) or 1)
+            if best is None or rate > best[0]:
+                best = (rate, flight)
+    return best[1] if best else default
 
 
 def _schedule_code(name: str) -> int:
     return abi.SCHEDULE_RELAX if name == "relax" else abi.SCHEDULE_DIAGONAL
 
 
-def _run_steps(backend: Backend, jobs, params, per_step: int, graphs: bool, schedule: str = "relax",
+def _run_steps(backends, jobs, params, per_step: int, graphs: bool, schedule: str = "relax",
                collect: bool = False) -> list:
-    backend.set_graphs(graphs)
-    backend.set_schedule(_schedule_code(schedule))
-    out = []
+    """The jobs as steps of per_step pairs, as many in flight as there are
+    backends (a single Backend: one at a time)."""
+    backends = [backends] if isinstance(backends, Backend) else list(backends)
+    for be in backends:
+        be.set_graphs(graphs)
+        be.set_schedule(_schedule_code(schedule))
+    chunks = [jobs[i:i + per_step] for i in range(0, len(jobs), per_step)]
     try:
-        for i in range(0, len(jobs), per_step):
-            chunk = jobs[i:i + per_step]
-            out += backend.analyze_step(chunk, [params] * len(chunk), want_results=collect)
+        res = run_in_flight(backends, len(chunks),
+                            lambda be, k: be.analyze_step(chunks[k], [params] * len(chunks[k]), want_results=collect))
     finally:
-        backend.set_graphs(False)
-        backend.set_schedule(abi.SCHEDULE_RELAX)
-    return out
+        for be in backends:
+            be.set_graphs(False)
+            be.set_schedule(abi.SCHEDULE_RELAX)
+    return [r for part in res for r in part]
 
 
 def autotune(backend: Backend, resolutions=RESOLUTIONS, candidates=STEP_CANDIDATES, frames: int = 5,
-             seed: int = 77, repeats: int = 2) -> dict:
-    """Measures MB searches/s per raster for every step size and launch mode
-    (MODES) on `frames`-frame synthetic clips; returns the table (not yet
-    saved)."""
+             seed: int = 77, repeats: int = 2, flights=IN_FLIGHT_CANDIDATES) -> dict:
+    """Measures MB searches/s per raster for every step size, launch mode
+    (MODES) and number of steps in flight on `frames`-frame synthetic clips;
+    returns the table (not yet saved).  Steps in flight run on extra
+    contexts of the same device, created here and closed at the end."""
+    extra = [Backend(backend.device) for _ in range(max(flights) - 1)]
     settings, trials = {}, {}
-    for (w, h) in resolutions:
-        pics = synth.pan_rectangles(seed + w, frames, width=w, height=h)
-        fr = [backend.upload(pics[k]) for k in range(frames)]
-        base = [(fr[k], fr[k - 1]) for k in range(1, frames)]
-        params = stream_params(w, h)
-        mbs = int(np.prod(abi.mb_grid(w, h)))
-        best, res = None, {}
-        for b in candidates:
-            jobs = (base * ((b + len(base) - 1) // len(base)))[:b]   # b independent jobs
-            for schedule, graphs in MODES:
-                _run_steps(backend, jobs, params, b, graphs, schedule)          # warm up (and capture)
-                t0 = time.perf_counter()
-                for _ in range(repeats):
-                    _run_steps(backend, jobs, params, b, graphs, schedule)
-                rate = repeats * b * mbs / (time.perf_counter() - t0)
-                res[f"{b}{'r' if schedule == 'relax' else ''}{'g' if graphs else ''}"] = round(rate, 1)
-                if best is None or rate > best[0]:
-                    best = (rate, b, graphs, schedule)
-        for f in fr:
-            f.release()
-        settings[f"{w}x{h}"] = {"jobs_per_step": best[1], "schedule": best[3], "graphs": best[2],
-                                "mbs_per_s": round(best[0], 1)}
-        trials[f"{w}x{h}"] = res
-        log(f"{w}x{h}: best {settings[f'{w}x{h}']} of {res}")
+    try:
+        for (w, h) in resolutions:
+            pics = synth.pan_rectangles(seed + w, frames, width=w, height=h)
+            fr = [backend.upload(pics[k]) for k in range(frames)]
+            base = [(fr[k], fr[k - 1]) for k in range(1, frames)]
+            params = stream_params(w, h)
+            mbs = int(np.prod(abi.mb_grid(w, h)))
+            best, res = None, {}
+            for b in candidates:
+                jobs = (base * ((b + len(base) - 1) // len(base)))[:b]   # b independent jobs
+                for schedule, graphs in MODES:
+                    for k in flights:
+                        lanes = [backend] + extra[:k - 1]
+                        steps = jobs * (repeats * k)
+                        _run_steps(lanes, jobs * k, params, b, graphs, schedule)     # warm up (and capture)
+                        t0 = time.perf_counter()
+                        _run_steps(lanes, steps, params, b, graphs, schedule)
+                        rate = len(steps) * mbs / (time.perf_counter() - t0)
+                        key = f"{b}{'r' if schedule == 'relax' else ''}{'g' if graphs else ''}{f'x{k}' if k > 1 else ''}"
+                        res[key] = round(rate, 1)
+                        if best is None or rate > best[0]:
+                            best = (rate, b, graphs, schedule, k)
+            for f in fr:
+                f.release()
+            settings[f"{w}x{h}"] = {"jobs_per_step": best[1], "schedule": best[3], "graphs": best[2],
+                                    "in_flight": best[4], "mbs_per_s": round(best[0], 1)}
+            trials[f"{w}x{h}"] = res
+            log(f"{w}x{h}: best {settings[f'{w}x{h}']} of {res}")
+    finally:
+        for be in extra:
+            be.close()
     return {"device": backend.device_name, "settings": settings, "trials": trials,
-            "candidates": list(candidates), "created": time.strftime("%Y-%m-%d %H:%M:%S")}
+            "candidates": list(candidates), "in_flight": list(flights),
+            "created": time.strftime("%Y-%m-%d %H:%M:%S")}
 
 
 def save_table(backend: Backend, table: dict) -> str:
@@ -192,10 +275,14 @@ def save_table(backend: Backend, table: dict) -> str:
     return p
 
 
-def run(backend: Backend, streams: list[Stream], table: dict | None = None, collect: bool = False) -> dict:
+def run(backend, streams: list[Stream], table: dict | None = None, collect: bool = False) -> dict:
     """Analyses every P pair of every stream; returns counts and timing, and
-    with collect=True the results as {(stream index, pair index): result}."""
-    table = load_table(backend) if table is None else table
+    with collect=True the results as {(stream index, pair index): result}.
+    `backend` is one Backend or a list of contexts of one device (the first
+    owns the frames); a raster whose setting keeps k steps in flight uses the
+    first k of them."""
+    backends = [backend] if isinstance(backend, Backend) else list(backend)
+    table = load_table(backends[0]) if table is None else table
     groups: dict = {}
     keys: dict = {}
     for si, s in enumerate(streams):
@@ -206,7 +293,8 @@ def run(backend: Backend, streams: list[Stream], table: dict | None = None, coll
     for (w, h), jobs in sorted(groups.items()):
         st = setting_for(table, w, h, len(jobs))
         t1 = time.perf_counter()
-        res = _run_steps(backend, jobs, stream_params(w, h), int(st["jobs_per_step"]), bool(st["graphs"]),
+        lanes = backends[:max(1, min(len(backends), int(st.get("in_flight", 1))))]
+        res = _run_steps(lanes, jobs, stream_params(w, h), int(st["jobs_per_step"]), bool(st["graphs"]),
                          str(st["schedule"]), collect)
         results.update(zip(keys[(w, h)], res))
         mbs = len(jobs) * int(np.prod(abi.mb_grid(w, h)))
@@ -261,8 +349,10 @@ def main(argv=None) -> int:
     log(f"{len(streams)} streams, {sum(len(s.frames) for s in streams)} frames resident ({time.time() - t0:.0f}s)")
     if pool:
         pool.shutdown()
-    run(be, streams[:4], table)                       # warm up every code path once
-    out = run(be, streams, table)
+    flight = max([int(v.get("in_flight", 1)) for v in table.get("settings", {}).values()] + [1])
+    lanes = [be] + [Backend(0) for _ in range(flight - 1)]
+    run(lanes, streams[:4], table)                    # warm up every code path once
+    out = run(lanes, streams, table)
     out.update({"streams": len(streams), "pairs": sum(len(s.pairs) for s in streams),
                 "length_scale": a.length_scale, "seed": a.seed,
                 "resolutions": {f"{w}x{h}": sum(1 for s in streams if (s.width, s.height) == (w, h))
@@ -270,6 +360,8 @@ def main(argv=None) -> int:
     for s in streams:
         for f in s.frames:
             f.release()
+    for lane in lanes[1:]:
+        lane.close()
     be.close()
     print(json.dumps(out))
     return 0

@@ -420,6 +420,8 @@ int validate_params(const vc1me_params* p, const vc1me_frame* cur, const vc1me_f
     if (!p || !cur || !ref) return fail(VC1ME_EINVAL, "null argument");
     if (p->width <= 0 || p->height <= 0) return fail(VC1ME_EINVAL, "raster must be positive");
     if ((p->width & 1) || (p->height & 1)) return fail(VC1ME_EINVAL, "coded raster must be even (4:2:0)");
+    if (static_cast<int64_t>((p->width + 15) / 16) * ((p->height + 15) / 16) >= vc1me::kMaxRelaxMacroblocks)
+        return fail(VC1ME_ELIMIT, "at most 2^24 - 1 macroblocks per picture");
     if (cur->w != p->width || cur->h != p->height || ref->w != p->width || ref->h != p->height)
         return fail(VC1ME_EINVAL, "P-picture reference dimensions do not match encoder configuration");
     if (p->pqindex < 1 || p->pqindex > 31) return fail(VC1ME_EINVAL, "PQINDEX must be 1..31");

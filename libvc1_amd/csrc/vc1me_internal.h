@@ -100,9 +100,19 @@ struct JobDesc {
 constexpr int kMaxRelaxPasses = 96;
 
 // The relaxation's worklists over all jobs of a step: pass p reads
-// cnt[p] entries of wl[p & 1] ((job << 24) | macroblock) by ticket (tk[p])
+// cnt[p] entries of wl[p & 1] (relax_entry(job, macroblock)) by ticket (tk[p])
 // and appends to wl[(p + 1) & 1]; cnt and tk hold kMaxRelaxPasses + 1
 // entries each, cap is the step's macroblock count.
+// A worklist entry: job (0..254) in the top 8 bits, macroblock index (below
+// 2^24) in the low 24, packed and unpacked as unsigned (a job of 128 or more
+// sets the top bit).
+constexpr int kMaxRelaxMacroblocks = 1 << 24;
+__host__ __device__ inline uint32_t relax_entry(int job, int mb) {
+    return (static_cast<uint32_t>(job) << 24) | static_cast<uint32_t>(mb);
+}
+__host__ __device__ inline int relax_entry_job(uint32_t e) { return static_cast<int>(e >> 24); }
+__host__ __device__ inline int relax_entry_pos(uint32_t e) { return static_cast<int>(e & 0xffffffu); }
+
 struct RelaxQueue {
     int32_t* cnt;
     int32_t* tk;

@@ -2759,7 +2759,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
 
 // Appends the right, bottom-left, bottom and bottom-right neighbours of a
 // macroblock whose state changed to the list of the next pass (each once per
-// pass: the queue stamp), as (job << 24) | macroblock entries.
+// pass: the queue stamp), as relax_entry(job, macroblock) entries.
 __device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int pos, int next_pass, RelaxQueue Q) {
     const int mbw = J->mbw, mbh = J->mbh;
     int32_t* next = Q.wl[next_pass & 1];
@@ -2771,7 +2771,7 @@ __device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int 
         const int d = y * mbw + x;
         if (atomicExch(&J->mark[d], static_cast<uint32_t>(next_pass)) != static_cast<uint32_t>(next_pass)) {
             const int at = atomicAdd(&Q.cnt[next_pass], 1);
-            if (at < Q.cap) next[at] = (job << 24) | d;
+            if (at < Q.cap) next[at] = static_cast<int32_t>(relax_entry(job, d));
             else atomicMax(J->err, 12);      // cannot happen: a pass lists each macroblock at most once
         }
     }
@@ -2812,8 +2812,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         // so the job and macroblock of the analysis are known to be uniform
         // (loaded into vector registers, every job field the analysis reads
         // became per-lane work and the kernel spilled)
-        const int code = __builtin_amdgcn_readfirstlane(list[t]);
-        const int job = code >> 24, pos = code & 0xffffff;
+        const uint32_t code = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(list[t]));
+        const int job = relax_entry_job(code), pos = relax_entry_pos(code);
         const JobDesc* J = &jobs[job];
         const NbState after = run_mb(J, pos % J->mbw, pos / J->mbw, smem);
         if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, pass + 1, Q);

@@ -72,3 +72,39 @@ def test_adapter_compiles_against_the_header():
                         f"-I{os.path.join(ROOT, 'tests', 'adapter')}", build.ADAPTER_SRC],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_relaxation_worklist_entries_round_trip(tmp_path):
+    """The relaxation's worklist entry (job in the top 8 bits, macroblock in
+    the low 24, stored in an int32 list) unpacks to the same job and
+    macroblock for every job a step may hold (up to 254: the top bit set from
+    job 128 on) and every macroblock index below 2^24 (host build of the
+    device helpers)."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if not cxx or not os.path.isdir("/opt/rocm/include"):
+        pytest.skip("no host C++ compiler or HIP headers")
+    src = tmp_path / "entry.cpp"
+    src.write_text(r'''
+#include "vc1me_internal.h"
+#include <cstdio>
+int main() {
+    int bad = 0;
+    for (int job : {0, 1, 127, 128, 200, 254})
+        for (int mb : {0, 1, 8159, 32399, (1 << 24) - 1}) {
+            const int32_t stored = static_cast<int32_t>(vc1me::relax_entry(job, mb));
+            const uint32_t back = static_cast<uint32_t>(stored);
+            if (vc1me::relax_entry_job(back) != job || vc1me::relax_entry_pos(back) != mb) ++bad;
+        }
+    std::printf("%d\n", bad);
+    return bad != 0;
+}
+''')
+    exe = tmp_path / "entry"
+    r = subprocess.run([cxx, "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                        f"-I{os.path.join(ROOT, 'libvc1_amd', 'csrc')}", str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "0", r.stdout + r.stderr

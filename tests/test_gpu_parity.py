@@ -346,3 +346,33 @@ def test_batched_streams_runner_matches_oracle(backend):
             want = oracle_analyze(s.params, pics[si][c], pics[si][r])
             got = out["results"][(si, k)]
             assert not got.diff(want), describe_diff(got, want, s.width)
+
+
+def test_runner_with_steps_in_flight_matches_oracle(backend):
+    """Steps kept in flight on three contexts (host threads, one stream each)
+    over frames uploaded by the first: every pair equals the oracle's, under
+    the captured relaxation step and the sweep."""
+    from libvc1_amd import Backend, batching
+    specs = [(310, 176, 144, 6), (311, 320, 176, 5)]
+    streams, pics = [], []
+    for seed, w, h, n in specs:
+        d = synth.pan_rectangles(seed, n, width=w, height=h, max_pan_px=8, max_obj_px=16)
+        pics.append([d[k] for k in range(n)])
+        streams.append(batching.Stream(w, h, [backend.upload(x) for x in pics[-1]], batching.stream_params(w, h), seed))
+    table = {"settings": {"176x144": {"jobs_per_step": 2, "schedule": "relax", "graphs": True, "in_flight": 3},
+                          "320x176": {"jobs_per_step": 1, "schedule": "diagonal", "graphs": False, "in_flight": 2}}}
+    lanes = [backend, Backend(0), Backend(0)]
+    _say("[case] batching runner, steps in flight on 3 contexts")
+    try:
+        out = batching.run(lanes, streams, table, collect=True)
+    finally:
+        for lane in lanes[1:]:
+            lane.close()
+        for st in streams:
+            for f in st.frames:
+                f.release()
+    for si, st in enumerate(streams):
+        for k, (c, r) in enumerate(st.pairs):
+            want = oracle_analyze(st.params, pics[si][c], pics[si][r])
+            got = out["results"][(si, k)]
+            assert not got.diff(want), describe_diff(got, want, st.width)

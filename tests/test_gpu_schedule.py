@@ -174,3 +174,31 @@ def test_set_schedule_rejects_unknown(backend):
     from libvc1_amd.backend import BackendError
     with pytest.raises(BackendError):
         backend.set_schedule(7)
+
+
+@pytest.mark.parametrize("graphs", [False, True], ids=["eager", "graph"])
+def test_relax_step_of_255_jobs(backend, graphs):
+    """The largest step (255 jobs) under the relaxation schedule: the
+    work-queue passes carry entries of jobs 128..254 (the entry's top bit
+    set), and every job equals the oracle (five distinct pairs, repeated)."""
+    pics = [synth.pan_rectangles(400 + k, 3, width=96, height=64, max_pan_px=6, max_obj_px=20, frame_indices=[0, 2])
+            for k in range(5)]
+    p = abi.default_params(96, 64, pqindex=8, motion_search_range=128, motion_local_search_range=32,
+                           mode=abi.MODE_MIXED_MV)
+    want = [oracle_analyze(p, d[2], d[0]) for d in pics]
+    frames = [(backend.upload(d[2]), backend.upload(d[0])) for d in pics]
+    _say(f"[case] relaxation, 255 jobs of 96x64, graphs {graphs}")
+    backend.set_graphs(graphs)
+    try:
+        for _ in range(2 if graphs else 1):   # capture, then replay
+            got = backend.analyze_step([frames[j % 5] for j in range(255)], [p] * 255)
+            passes = backend.relax_passes()
+            assert backend.timing().schedule == abi.SCHEDULE_RELAX
+            assert len(passes) > 1 and passes[1][0] > 0, passes     # work-queue passes ran
+            for j, g in enumerate(got):
+                assert not g.diff(want[j % 5]), (j, describe_diff(g, want[j % 5], 96))
+    finally:
+        backend.set_graphs(False)
+        for a, b in frames:
+            a.release()
+            b.release()
