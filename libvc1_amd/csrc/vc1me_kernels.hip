@@ -2263,8 +2263,12 @@ __device__ __forceinline__ void commit_state(const Ctx& C, NbState& st, uint8_t 
 }
 
 // kGuess: the relaxation's starting state only (never a result): the
-// analysis without the 4-MV stage and the extended long-range search, the
-// two costliest parts whose outcome rarely changes a neighbour's predictors.
+// analysis without the 4-MV stage, the costliest part, whose outcome rarely
+// changes a neighbour's predictors.  The extended long-range search does run:
+// its result depends on no neighbour and is kept for the step (pass 0 reads
+// it back), and the long-range vectors it finds reach the neighbours'
+// predictors and propagated searches in pass 0 already, instead of spreading
+// one macroblock per pass.
 template <bool kGuess>
 __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     const JobDesc* J = C.J;
@@ -2376,7 +2380,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
                 }
             }
         }
-        if (!kGuess && range > local_range && !seeded_ok && !match_good(C, im.sad)) {
+        if (range > local_range && !seeded_ok && !match_good(C, im.sad)) {
             uint64_t ext_sad = UINT64_MAX;
             int ext_dec = 0;
             const IRes ext = extended_content(C, im, &ext_sad, &ext_dec);
