@@ -11,12 +11,16 @@ each GOP, each predicted from the anchor three frames back: 16 P frames,
 4-MV 8x8), quarter-pel, SATD finalists, PQINDEX 8, P-intra decision on.
 
 One step = vc1me_analyze_step over all 16 (cur, ref) pairs with the frames
-already resident in HBM, under the default relaxation schedule (every
-macroblock at once, then worklist passes until nothing changes; bit-exact
-with the raster-order reference); the MV fields stay on the device (their
-download is timed separately and reported as `value_with_download`).  The
-same steps under the anti-diagonal sweep are timed for comparison
-(`schedule.diagonal_value`).
+already resident in HBM, under the default relaxation schedule (a guess
+pass, every macroblock at once, then worklist passes until nothing changes;
+bit-exact with the raster-order reference), captured as one HIP graph; the
+MV fields stay on the device (their download is timed separately and
+reported as `value_with_download`).  `value` keeps the autotuned number of
+steps in flight (`steps_in_flight`: one context with its own stream and
+device state per host thread, all on the same resident frames), so that the
+latency-bound end of one step overlaps the next; `value_one_in_flight` is
+the same steps one at a time.  The same steps under the anti-diagonal sweep
+are timed for comparison (`schedule.diagonal_value`).
 
 The learned-seeded form (`learned`) runs the same steps through
 vc1me_analyze_step_learned with the trained weights of
@@ -42,14 +46,14 @@ The batched leg (`batched`) runs the 64-stream workload of uneven rasters
 and lengths (lengths scaled to keep the run short) through the batching
 runner with the tuned table; it is reported beside the headline, not as it.
 
-Also reported: the roofline of the dominant kernel (the macroblock analysis,
-k_mb_relax: one launch per relaxation pass) from HIP events recorded around
-each of its launches inside the timed region, and the reference's own
-multithreaded CPU path timed on this host on a bounded sample of the same
-frames.
+Also reported: the roofline of the dominant kernels (the macroblock
+analysis: k_mb_guess, then k_mb_relax0 and k_mb_relaxq, one launch per
+relaxation pass) from HIP events recorded around each of their launches in
+the same steps run one at a time, and the reference's own multithreaded CPU
+path timed on this host on a bounded sample of the same frames.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-check] [--no-cpu] [--no-learned] [--no-compare]
-                    [--no-seeded] [--no-bframes] [--no-batched] [--batched-scale S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--in-flight N] [--no-check] [--no-cpu] [--no-learned]
+                    [--no-compare] [--no-seeded] [--no-bframes] [--no-batched] [--batched-scale S]
     python bench.py --copy-calibration     # 256 MiB device copy, for HBM counter scaling
 
 `roofline.traffic` (HBM bytes per launch of the macroblock kernel) comes from
@@ -832,9 +836,11 @@ def main() -> int:
         parity = "bit-exact" if not bad else f"MISMATCH in {bad}"
         log(f"parity of pair {pairs[0]} vs oracle: {parity} ({time.time() - t2:.1f}s)")
 
-    # roofline of the macroblock kernel (relaxation: k_mb_relax0 for pass 0,
-    # k_mb_relaxq for the work-queue passes; profiled under one tag)
-    mb_kernel = "k_mb_relax" if schedule["kind"] == "relax" else "k_mb_diagonal"
+    # roofline of the macroblock analysis kernels (relaxation: k_mb_guess for
+    # the starting state, which also runs the extended long-range searches
+    # the passes read back, k_mb_relax0 for pass 0, k_mb_relaxq for the
+    # work-queue passes; profiled under one tag)
+    mb_kernel = "k_mb_analysis" if schedule["kind"] == "relax" else "k_mb_diagonal"
     cus, clk, peak = device_peak_ops()
     mb_ops = work.sad_px + work.interp_taps + work.satd_ops + work.sig_ops
     mb_sec_per_step = (mb_ms / args.steps) / 1e3
@@ -847,8 +853,8 @@ def main() -> int:
         "unit": "TOP/s",
         "frac": round(achieved / peak, 5),
         "traffic": measured_traffic(mb_kernel),
-        "kernel": "k_mb_relax0 + k_mb_relaxq (one launch per relaxation pass)" if mb_kernel == "k_mb_relax"
-                  else mb_kernel,
+        "kernel": "k_mb_guess + k_mb_relax0 + k_mb_relaxq (the guess pass, then one launch per relaxation pass)"
+                  if mb_kernel == "k_mb_analysis" else mb_kernel,
         "avg_launch_us": round(avg_launch_us, 2),
         "launches_per_step": mb_launches // max(1, args.steps),
         "ops_per_step": {"sad_px": work.sad_px, "interp_taps": work.interp_taps, "satd_ops": work.satd_ops,

@@ -180,7 +180,10 @@ int vc1me_score_batch_gen(vc1me_ctx* ctx, const uint8_t* cur, uint64_t cur_gener
                           uint64_t* costs_out);
 
 /* ---- frames resident in HBM (explicit handles instead of the reference's
- * pointer+fingerprint cache, src/vulkan_compute.cpp:194-231). */
+ * pointer+fingerprint cache, src/vulkan_compute.cpp:194-231).  A handle may
+ * be used in the jobs of every context of the same device (so that several
+ * contexts keep steps in flight over one set of frames); the context that
+ * uploaded it releases it, once no step that reads it is running. */
 int vc1me_frame_upload(vc1me_ctx* ctx, const vc1me_planes* planes, vc1me_frame** out);
 void vc1me_frame_release(vc1me_ctx* ctx, vc1me_frame* frame);
 
@@ -441,6 +444,10 @@ typedef struct vc1me_timing {
     uint64_t schedule;
     uint64_t relax_passes;
     uint64_t relax_analyses;
+    /* with profiling on: device ms of the macroblock pre-pass (the
+     * relaxation's guess pass, or the extended long-range pass ahead of the
+     * sweep), which is included in mb_kernel_ms_total / mb_kernel_launches */
+    double prepass_ms;
 } vc1me_timing;
 int vc1me_last_timing(vc1me_ctx* ctx, vc1me_timing* out);
 /* Record HIP events around every macroblock-kernel launch so that

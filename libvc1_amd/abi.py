@@ -90,7 +90,8 @@ class Timing(C.Structure):
     _fields_ = [("step_ms", C.c_double), ("mb_kernel_ms_total", C.c_double),
                 ("mb_kernel_launches", C.c_uint64), ("global_kernel_ms_total", C.c_double),
                 ("global_kernel_launches", C.c_uint64), ("graph_replay", C.c_uint64),
-                ("schedule", C.c_uint64), ("relax_passes", C.c_uint64), ("relax_analyses", C.c_uint64)]
+                ("schedule", C.c_uint64), ("relax_passes", C.c_uint64), ("relax_analyses", C.c_uint64),
+                ("prepass_ms", C.c_double)]
 
 
 class Work(C.Structure):

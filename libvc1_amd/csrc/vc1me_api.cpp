@@ -759,6 +759,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     // launches with events around them when profiling
     int mb_launches = 0;
     std::vector<std::pair<int, int>> mb_events;
+    int prepass_event = -1;   // mb_events entry of the guess pass (or the sweep's extended pre-pass)
     auto timed = [&](bool events, const std::function<hipError_t()>& launch) -> int {
         const int ev = 4 + 2 * mb_launches;
         const bool rec = events && ctx->profiling && static_cast<size_t>(ev + 1) < ctx->events.size();
@@ -808,10 +809,18 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
         if (relax) {
             HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
-            if (relax_guess()) HIP_TRY(vc1me::launch_mb_guess(ctx->d_jobs, n, max_mbs_all, lds, st));
+            if (relax_guess()) {
+                const size_t at = mb_events.size();
+                if (int rc = timed(events, [&] { return vc1me::launch_mb_guess(ctx->d_jobs, n, max_mbs_all, lds, st); }))
+                    return rc;
+                if (mb_events.size() > at) prepass_event = static_cast<int>(at);
+            }
             if (int rc = relax_passes(events, 0, passes)) return rc;
         } else {
-            HIP_TRY(vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st));
+            const size_t at = mb_events.size();
+            if (int rc = timed(events, [&] { return vc1me::launch_extended(ctx->d_jobs, n, max_mbs, lds, st); }))
+                return rc;
+            if (mb_events.size() > at) prepass_event = static_cast<int>(at);
             if (int rc = diagonal_all(events)) return rc;
         }
         return finish();
@@ -921,19 +930,24 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         }
     }
     double mbms = 0.0;
+    double prepass_ms = 0.0;
     for (size_t k = 0; k < mb_events.size(); ++k) {
         float x = 0.f;
         HIP_TRY(hipEventElapsedTime(&x, ctx->events[mb_events[k].first], ctx->events[mb_events[k].second]));
         mbms += x;
-        if (relax && k < ctx->pass_analyses.size()) ctx->pass_ms.push_back(x);
+        if (static_cast<int>(k) == prepass_event) { prepass_ms = x; continue; }
+        const size_t pass = prepass_event >= 0 && static_cast<int>(k) > prepass_event ? k - 1 : k;
+        if (relax && pass < ctx->pass_analyses.size()) ctx->pass_ms.push_back(x);
     }
     if (ctx->timing.graph_replay) {
         mb_launches = 0;
-        if (relax) mb_launches = passes;
+        if (relax) mb_launches = passes + (relax_guess() ? 1 : 0);
         else
             for (int t = 0; t <= max_diag; ++t) mb_launches += per_diag(t) ? 1 : 0;
+        if (!relax) ++mb_launches;   // the extended pre-pass
     }
     ctx->timing.mb_kernel_ms_total = mbms;
+    ctx->timing.prepass_ms = prepass_ms;
     ctx->timing.mb_kernel_launches = static_cast<uint64_t>(mb_launches);
     ctx->timing.schedule = relax && !relax_fell_back ? VC1ME_SCHEDULE_RELAX : VC1ME_SCHEDULE_DIAGONAL;
     ctx->timing.relax_passes = relax ? static_cast<uint64_t>(used ? used : limit) : 0;
