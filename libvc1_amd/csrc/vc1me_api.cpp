@@ -698,6 +698,15 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     int32_t* d_cnt = reinterpret_cast<int32_t*>(ctx->d_out);
     auto* d_sum = reinterpret_cast<vc1me::StepSummary*>(ctx->d_out + sum_at);
     const bool relax = ctx->schedule == VC1ME_SCHEDULE_RELAX;
+    int max_mbs = 0, max_mbs_all = 0;
+    for (int i = 0; i < n; ++i) {
+        const int m = ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh;
+        max_mbs_all = std::max(max_mbs_all, m);
+        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range) max_mbs = std::max(max_mbs, m);
+    }
+    // the whole-frame kernels' (macroblocks x jobs) grid is indexed in int
+    if (static_cast<int64_t>(max_mbs_all) * n >= (int64_t(1) << 30))
+        return fail(VC1ME_ELIMIT, "a step holds at most 2^30 macroblocks (largest picture's count x jobs)");
     vc1me::RelaxQueue Q{};
     const int lds = vc1me::mb_kernel_lds_bytes(max_local);
     if (relax) {
@@ -721,12 +730,6 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     }
     HIP_TRY(hipMemcpyAsync(ctx->d_jobs, ctx->h_jobs.data(), sizeof(JobDesc) * n, hipMemcpyHostToDevice, st));
     ctx->last_jobs = n;
-    int max_mbs = 0, max_mbs_all = 0;
-    for (int i = 0; i < n; ++i) {
-        const int m = ctx->h_jobs[i].mbw * ctx->h_jobs[i].mbh;
-        max_mbs_all = std::max(max_mbs_all, m);
-        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range) max_mbs = std::max(max_mbs, m);
-    }
     auto per_diag = [&](int t) {
         int per = 0;
         for (int i = 0; i < n; ++i)

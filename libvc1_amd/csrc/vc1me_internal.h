@@ -99,10 +99,6 @@ struct JobDesc {
 // sweep (which is exact from any starting state).
 constexpr int kMaxRelaxPasses = 96;
 
-// The relaxation's worklists over all jobs of a step: pass p reads
-// cnt[p] entries of wl[p & 1] (relax_entry(job, macroblock)) by ticket (tk[p])
-// and appends to wl[(p + 1) & 1]; cnt and tk hold kMaxRelaxPasses + 1
-// entries each, cap is the step's macroblock count.
 // A worklist entry: job (0..254) in the top 8 bits, macroblock index (below
 // 2^24) in the low 24, packed and unpacked as unsigned (a job of 128 or more
 // sets the top bit).
@@ -113,6 +109,27 @@ __host__ __device__ inline uint32_t relax_entry(int job, int mb) {
 __host__ __device__ inline int relax_entry_job(uint32_t e) { return static_cast<int>(e >> 24); }
 __host__ __device__ inline int relax_entry_pos(uint32_t e) { return static_cast<int>(e & 0xffffffu); }
 
+// XCD bands of a step's (macroblocks x jobs) grid of `total` items in
+// job-major order (item = job * per + macroblock): the dispatcher deals a
+// grid's workgroups to the eight XCDs in turn, and the whole-frame kernels
+// give the workgroups on XCD x the contiguous items of band x.  Bands
+// 0..r-1 hold q + 1 items, the others q (total = 8q + r); xcd_band_of is the
+// inverse map.
+constexpr int kXcds = 8;
+__host__ __device__ inline int xcd_band_start(int total, int x) {
+    const int q = total / kXcds, r = total % kXcds;
+    return x * q + (x < r ? x : r);
+}
+__host__ __device__ inline int xcd_band_size(int total, int x) { return total / kXcds + (x < total % kXcds ? 1 : 0); }
+__host__ __device__ inline int xcd_band_of(int total, int item) {
+    const int q = total / kXcds, r = total % kXcds, big = r * (q + 1);
+    return item < big ? item / (q + 1) : r + (item - big) / q;   // q > 0 whenever item >= big
+}
+
+// The relaxation's worklists over all jobs of a step: pass p reads cnt[p]
+// entries of wl[p & 1] (relax_entry(job, macroblock)) by ticket (tk[p]) and
+// appends to wl[(p + 1) & 1]; cnt and tk hold kMaxRelaxPasses + 1 entries
+// each, cap is the step's macroblock count.
 struct RelaxQueue {
     int32_t* cnt;
     int32_t* tk;

@@ -2740,14 +2740,12 @@ __global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ 
 // planes and signature index.  Workgroup L therefore takes item
 // band(L % 8) + L / 8 of the job-major item order, so that each XCD works
 // through one contiguous band of macroblocks (and of jobs) and its L2 holds
-// what its neighbours read.  A bijection of [0, T): XCDs 0..r-1 take q + 1
-// items, the others q (T = 8q + r).
+// what its neighbours read (xcd_band_start: a bijection of [0, T)).
 struct GridItem { int job, pos; };
 __device__ __forceinline__ GridItem xcd_banded_item() {
     const int T = static_cast<int>(gridDim.x * gridDim.y);
     const int L = static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x);
-    const int x = L & 7, k = L >> 3, q = T >> 3, r = T & 7;
-    const int item = x * q + imin(x, r) + k;
+    const int item = xcd_band_start(T, L % kXcds) + L / kXcds;
     const int per = static_cast<int>(gridDim.x);
     return {item / per, item - (item / per) * per};
 }

@@ -78,8 +78,10 @@ def test_relaxation_worklist_entries_round_trip(tmp_path):
     """The relaxation's worklist entry (job in the top 8 bits, macroblock in
     the low 24, stored in an int32 list) unpacks to the same job and
     macroblock for every job a step may hold (up to 254: the top bit set from
-    job 128 on) and every macroblock index below 2^24 (host build of the
-    device helpers)."""
+    job 128 on) and every macroblock index below 2^24; the XCD bands that
+    order the whole-frame kernels' work and segment the lists tile the grid,
+    and every item maps to the band that holds it (host build of the device
+    helpers)."""
     import shutil
     import subprocess
     cxx = shutil.which("g++")
@@ -96,6 +98,24 @@ int main() {
             const int32_t stored = static_cast<int32_t>(vc1me::relax_entry(job, mb));
             const uint32_t back = static_cast<uint32_t>(stored);
             if (vc1me::relax_entry_job(back) != job || vc1me::relax_entry_pos(back) != mb) ++bad;
+        }
+    // XCD bands of a grid: every item lies in its own band, and the bands
+    // tile [0, total) in order
+    for (int total : {1, 5, 7, 8, 9, 15, 16, 17, 130560, 255 * 8160 + 3, 255 * 32400})
+        for (int x = 0, next = 0; x < vc1me::kXcds; ++x) {
+            const int start = vc1me::xcd_band_start(total, x), size = vc1me::xcd_band_size(total, x);
+            if (start != next) ++bad;
+            next = start + size;
+            if (x == vc1me::kXcds - 1 && next != total) ++bad;
+            for (int item : {start, start + size / 2, start + size - 1})
+                if (size > 0 && vc1me::xcd_band_of(total, item) != x) ++bad;
+        }
+    for (int total : {1, 7, 8, 9, 23, 1000})
+        for (int item = 0; item < total; ++item) {
+            const int x = vc1me::xcd_band_of(total, item);
+            if (x < 0 || x >= vc1me::kXcds || item < vc1me::xcd_band_start(total, x) ||
+                item >= vc1me::xcd_band_start(total, x) + vc1me::xcd_band_size(total, x))
+                ++bad;
         }
     std::printf("%d\n", bad);
     return bad != 0;
