@@ -420,6 +420,43 @@ __device__ __forceinline__ void set_err(Ctx& C, int code) {
 // [wx0, wx0+ww) and rows [wy0, wy0+wh).  The stored origin is rounded down to
 // 16 bytes so whole rows move as 16-byte loads.  Returns false if the window
 // does not fit the buffer.
+//
+// win_issue only starts the direct copy (a window inside the picture's
+// columns) and returns whether it did; the next wsync waits for it.  An
+// analysis issues its integer-search window that way before it loads the
+// neighbours' state, so that the two loads' latencies overlap.
+__device__ __forceinline__ bool win_issue(Ctx& C, int wx0, int wy0, int ww, int wh) {
+    const int ax0 = wx0 & ~3;
+    const int aw = wx0 + ww - ax0;
+    const int wp = (aw + 8 + 3) & ~3;
+    const int W = C.g.W, H = C.g.H;
+    if (ww <= 0 || wh <= 0 || wp * wh > C.winCap || (W & 3) != 0 || ax0 < 0 || ax0 + wp > W) return false;
+    wsync();
+    const int cpr = wp >> 2;
+    const int total = cpr * wh;
+    gptr_u8 ry = as_global(C.ry);
+    uint32_t* w4 = reinterpret_cast<uint32_t*>(C.L.win);
+    typedef const __attribute__((address_space(1))) void* gvoid;
+    typedef __attribute__((address_space(3))) void* lvoid;
+    const int dr = kWave / cpr, dc = kWave - dr * cpr;
+    int r = C.lane / cpr, c = C.lane - (C.lane / cpr) * cpr;
+    for (int base = 0; base < total; base += kWave) {
+        const int sy = clampi(wy0 + r, 0, H - 1);
+        const gptr_u8 src = ry + static_cast<size_t>(sy) * W + ax0 + c * 4;
+        if (base + C.lane < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(w4 + base), 4, 0, 0);
+        c += dc;
+        r += dr;
+        if (c >= cpr) { c -= cpr; ++r; }
+    }
+    C.wx0 = ax0; C.wy0 = wy0; C.ww = aw; C.wh = wh; C.wp = wp;
+    return true;
+}
+// whether the window staged last (issued or loaded) is exactly this one
+__device__ __forceinline__ bool win_is(const Ctx& C, int wx0, int wy0, int ww, int wh) {
+    const int ax0 = wx0 & ~3;
+    return C.wx0 == ax0 && C.wy0 == wy0 && C.ww == wx0 + ww - ax0 && C.wh == wh;
+}
+
 __device__ __forceinline__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int wh) {
     // rows start at the dword holding column wx0 and carry 8 bytes of read
     // slack for the row SAD; copied one dword per lane
@@ -533,22 +570,26 @@ __device__ __forceinline__ void top4_insert(Top4& t, uint64_t m) {
     else if (m < t.k2) { t.k3 = t.k2; t.k2 = m; }
     else if (m < t.k3) t.k3 = m;
 }
-// Merge a batch's keys (one per lane, kNone elsewhere; distinct from each
-// other and from the keys already held), whose least is `least`: its
-// smallest keys enter in ascending order until one cannot.
-__device__ __forceinline__ void top4_merge(Top4& t, uint64_t key, uint64_t least) {
-    uint64_t m = least;
-    for (int r = 0; r < 4 && m < t.k3; ++r) {
-        top4_insert(t, m);
-        if (r < 3) m = wave_min64(key > m ? key : kNone);
+// The wave's four least keys from every lane's own four least (keys are
+// distinct across lanes: each scored point is one lane's), ascending: four
+// rounds of a wave minimum, the lane holding it dropping its head.
+__device__ __forceinline__ Top4 top4_wave(Top4 t) {
+    Top4 w{kNone, kNone, kNone, kNone};
+    for (int r = 0; r < 4; ++r) {
+        const uint64_t m = wave_min64(t.k0);
+        if (r == 0) w.k0 = m; else if (r == 1) w.k1 = m; else if (r == 2) w.k2 = m; else w.k3 = m;
+        if (m == kNone) break;
+        if (t.k0 == m) { t.k0 = t.k1; t.k1 = t.k2; t.k2 = t.k3; t.k3 = kNone; }
     }
+    return w;
 }
 
 // Score n integer candidates (pt(i) -> (dx,dy), already clamped) at the given
 // sample stride from the current window, G lanes per candidate.  seen: dedupe
-// bitset or null.  top: optional four least keys of every scored point, to
-// which the batch's keys are merged (with the bitset, every key is new).
-// Returns the minimum key of the batch (kNone if no candidate was scored).
+// bitset or null.  top: optional per-lane four least keys of the points this
+// lane scored, to which its keys of the batch are added (with the bitset,
+// every key is new; top4_wave merges the lanes' lists).  Returns the minimum
+// key of the batch (kNone if no candidate was scored).
 template <class PtFn>
 __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_t* seen, int keymode, int seq0, PtFn pt,
                               Top4* top = nullptr) {
@@ -604,7 +645,7 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
         if (top) {
             const uint64_t mk = wave_min64(key);
             best = mk < best ? mk : best;
-            top4_merge(*top, key, mk);
+            top4_insert(*top, key);
         } else {
             lane_best = key < lane_best ? key : lane_best;
         }
@@ -1114,7 +1155,9 @@ struct IRes {
 // from the log once, at the end.
 __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int gdy, int R) {
     // the whole search stays inside [-R,R]^2: one window serves every stage
-    if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
+    // (issued at the start of the analysis when it could be)
+    if (!win_is(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R) &&
+        !win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
         set_err(C, 5);
         return {0, 0, kBadSad};
     }
@@ -1143,18 +1186,18 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
             dy = i == 0 ? py : i == 1 ? 0 : i == 2 ? gy : use_a ? ay : qy;
         });
     }
-    // local 5x5
+    // local 5x5 around the seeds' best, and the sparse lattice (R >= 8) in
+    // the same batch: the lattice's points do not depend on the best so far,
+    // and the best and the top list the cross starts from are functions of
+    // the set of points considered, whatever their order
     {
         const int cx = key_dx(best), cy = key_dy(best);
-        stage(25, [=](int i, int& dx, int& dy) VC1ME_INLINE { dx = cl(cx + (i % 5) - 2); dy = cl(cy + (i / 5) - 2); });
-    }
-    // sparse lattice
-    if (R >= 8) {
-        const int side = (2 * R) / 8 + 1;
-        stage(side * side + 2 * side, [=](int i, int& dx, int& dy) VC1ME_INLINE {
-            if (i < side * side) { dx = -R + 8 * (i % side); dy = -R + 8 * (i / side); }
+        const int side = R >= 8 ? (2 * R) / 8 + 1 : 0;
+        stage(25 + side * side + 2 * side, [=](int i, int& dx, int& dy) VC1ME_INLINE {
+            if (i < 25) { dx = cx + (i % 5) - 2; dy = cy + (i / 5) - 2; }
+            else if (i < 25 + side * side) { dx = -R + 8 * ((i - 25) % side); dy = -R + 8 * ((i - 25) / side); }
             else {
-                const int j = i - side * side, d = -R + 8 * (j >> 1);
+                const int j = i - 25 - side * side, d = -R + 8 * (j >> 1);
                 if (j & 1) { dx = d; dy = R; } else { dx = R; dy = d; }
             }
             dx = cl(dx); dy = cl(dy);
@@ -1219,8 +1262,9 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
         });
     }
     // the best four considered points (top[0..3] of the reference's list)
-    const uint64_t tk[4] = {top.k0, top.k1, top.k2, top.k3};
-    const int fin = (top.k0 != kNone) + (top.k1 != kNone) + (top.k2 != kNone) + (top.k3 != kNone);
+    const Top4 wt = top4_wave(top);
+    const uint64_t tk[4] = {wt.k0, wt.k1, wt.k2, wt.k3};
+    const int fin = (wt.k0 != kNone) + (wt.k1 != kNone) + (wt.k2 != kNone) + (wt.k3 != kNone);
     // full-resolution finalists: 3x3 around the best four
     uint64_t full = int_batch(C, fin * 9, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
         const int f = i / 9, o = i % 9;
@@ -1271,7 +1315,8 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
 
 // integer_motion_diamond, src/encoder_motion_diamond.cpp:13-73
 __device__ __forceinline__ IRes diamond_search(Ctx& C, const Pred pi, int gdx, int gdy, int R) {
-    if (!win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
+    if (!win_is(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R) &&
+        !win_load(C, C.x0 - R, C.y0 - R, 16 + 2 * R, 16 + 2 * R)) {
         set_err(C, 5);
         return {0, 0, kBadSad};
     }
@@ -2279,6 +2324,12 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     const int local_range = J->local_range;
     const int pos = C.pos;
 
+    // the local integer search's window, issued first when the analysis will
+    // start with that search (its load overlaps the ones below; the wsync
+    // after them waits for all)
+    if (range > 0 && !J->seed_mvs && J->lr_mode != 2 &&
+        !(J->nn_out && J->nn_out[static_cast<size_t>(pos) * 5 + 4] >= J->nn_logit))
+        win_issue(C, C.x0 - local_range, C.y0 - local_range, 16 + 2 * local_range, 16 + 2 * local_range);
     // current block to LDS, zero outside the active area; the neighbours'
     // state snapshot
     for (int i = C.lane; i < 256; i += kWave) {
