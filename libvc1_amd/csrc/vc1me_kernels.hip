@@ -2194,14 +2194,14 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
         int rank = 0;
         for (int k = 0; k < n; ++k) rank += lane_read64(myk, k) < myk ? 1 : 0;
         if (n > keep) n = keep;
-        int src = 0;
-        for (int r = 0; r < n; ++r) {
-            const uint64_t at = __ballot(lane < L.n && rank == r);
-            if (lane == r) src = __ffsll(static_cast<long long>(at)) - 1;
-        }
-        xq = __shfl(L.xq, src);
-        yq = __shfl(L.yq, src);
-        sad = lane_shfl64(L.sad, src);
+        // every listed entry moves to the lane of its rank (the keys, and so
+        // the ranks, are distinct) by a forward permute; the lanes past the
+        // list all go to lane 63, which is never a shortlist lane
+        const int to = (lane < L.n ? rank : kWave - 1) * 4;
+        xq = __builtin_amdgcn_ds_permute(to, L.xq);
+        yq = __builtin_amdgcn_ds_permute(to, L.yq);
+        sad = (static_cast<uint64_t>(static_cast<uint32_t>(__builtin_amdgcn_ds_permute(to, static_cast<int>(L.sad >> 32)))) << 32) |
+              static_cast<uint32_t>(__builtin_amdgcn_ds_permute(to, static_cast<int>(L.sad)));
     }
     const uint64_t screen = lane_read64(sad, 0);
     // rates and x87 scores, one candidate per lane
