@@ -1746,13 +1746,15 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
         auto key_block = [&](const uint64_t (&v)[kU], const int (&vr)[kU], const int (&vc)[kU], int base) VC1ME_INLINE {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
+                // the distance is the key's most significant field: a cell
+                // farther than the 16th best cannot enter the list, so the
+                // whole key is built only when some lane's cell might
+                const uint32_t dist = sig_distance(want, v[u]);
+                const bool near = base + C.lane + u * kWave < total && dist <= static_cast<uint32_t>(thr >> 34);
+                if (!__ballot(near)) continue;
                 const int dx = (cx_lo + vc[u]) * 8 - C.x0, dy = (cy_lo + vr[u]) * 8 - C.y0;
-                const bool ok = base + C.lane + u * kWave < total && !(iabs(dy) <= 32 && iabs(dx) <= 32);
-                uint64_t k = kNone;
-                if (ok) {
-                    ++scanned;
-                    k = key_int(sig_distance(want, v[u]), dx, dy);
-                }
+                const bool ok = near && !(iabs(dy) <= 32 && iabs(dx) <= 32);
+                const uint64_t k = ok ? key_int(dist, dx, dy) : kNone;
                 uint64_t hits = __ballot(k < thr);
                 while (hits) {
                     const int l = __ffsll(static_cast<long long>(hits)) - 1;
@@ -1780,7 +1782,15 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
             key_block(vb, rb, cb, base + kU * kWave);
         }
     }
-    C.n_sig += wave_sum32(scanned);
+    // the cells scanned: the rectangle less those whose offset lies within
+    // +-32 px in both axes (macroblock origins are multiples of 16, so those
+    // are the cell columns x0 / 8 - 4 .. x0 / 8 + 4 and rows likewise)
+    if (cx_lo <= cx_hi && cy_lo <= cy_hi) {
+        const int ex = imax(0, imin(cx_hi, (C.x0 >> 3) + 4) - imax(cx_lo, (C.x0 >> 3) - 4) + 1);
+        const int ey = imax(0, imin(cy_hi, (C.y0 >> 3) + 4) - imax(cy_lo, (C.y0 >> 3) - 4) + 1);
+        scanned = static_cast<uint32_t>((cx_hi - cx_lo + 1) * (cy_hi - cy_lo + 1) - ex * ey);
+    }
+    C.n_sig += scanned;
     // the top 16 by (distance, |d|, dy, dx): lane r holds the r-th
     const uint64_t ckeys = top;
     const int tn = __popcll(__ballot(C.lane < 16 && top != kNone));
