@@ -1718,6 +1718,7 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
     // distinct keys, so the order in which cells are visited does not matter.
     uint64_t top = kNone;      // lane r < 16: r-th best key so far
     uint64_t thr = kNone;      // the 16th best (uniform)
+    bool empty = true;         // no key has been listed yet
     uint32_t scanned = 0;
     if (cx_lo <= cx_hi && cy_lo <= cy_hi) {
         constexpr int kU = 4;
@@ -1756,6 +1757,21 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
                 const bool ok = near && !(iabs(dy) <= 32 && iabs(dx) <= 32);
                 const uint64_t k = ok ? key_int(dist, dx, dy) : kNone;
                 uint64_t hits = __ballot(k < thr);
+                if (empty && hits) {
+                    // the list is empty: it becomes the (up to) sixteen least
+                    // of these keys, each moved to the lane of its rank (the
+                    // keys are distinct) by a forward permute
+                    empty = false;
+                    const int nk = __popcll(hits);
+                    int rank = 0;
+                    for (int j = 0; j < kWave; ++j) rank += lane_read64(k, j) < k ? 1 : 0;
+                    const int to = ((hits >> C.lane) & 1) && rank < 16 ? rank * 4 : (kWave - 1) * 4;
+                    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(to, static_cast<int>(k)));
+                    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(to, static_cast<int>(k >> 32)));
+                    top = C.lane < imin(nk, 16) ? (static_cast<uint64_t>(hi) << 32) | lo : kNone;
+                    thr = lane_read64(top, 15);
+                    continue;
+                }
                 while (hits) {
                     const int l = __ffsll(static_cast<long long>(hits)) - 1;
                     hits &= hits - 1;
