@@ -414,12 +414,16 @@ hipError_t launch_conv_fwd(const uint16_t* x, const uint16_t* wt, const float* b
     const int tiles = nb * h * ((w + 15) / 16);
     if (tiles <= 0) return hipSuccess;
     // resident waves that each keep the weights in registers and walk tiles
-    // (one round of the device's occupancy), fewer for small inputs
-    static std::atomic<int> resident[2] = {{0}, {0}};
-    int r = resident[cin == 32 ? 1 : 0].load(std::memory_order_relaxed);
+    // (one round of the current device's occupancy, looked up once per
+    // device), fewer for small inputs
+    constexpr int kDevices = 64;
+    static std::atomic<int> resident[kDevices][2] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) dev = -1;
+    int r = dev >= 0 ? resident[dev][cin == 32 ? 1 : 0].load(std::memory_order_relaxed) : 0;
     if (!r) {
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
+        int cus = 0, per_cu = 0;
+        if (dev >= 0 &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &per_cu, cin == 32 ? reinterpret_cast<const void*>(k_conv_fwd_regw<32>)
@@ -428,7 +432,7 @@ hipError_t launch_conv_fwd(const uint16_t* x, const uint16_t* wt, const float* b
             r = cus * per_cu;
         else
             r = 3072;
-        resident[cin == 32 ? 1 : 0].store(r, std::memory_order_relaxed);
+        if (dev >= 0) resident[dev][cin == 32 ? 1 : 0].store(r, std::memory_order_relaxed);
     }
     const int grid = tiles < r ? tiles : r;
     if (cin == 32)
