@@ -247,6 +247,11 @@ struct CachedPlane {
 struct vc1me_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of a step: work that depends on the frames alone (phase
+    // planes, signature index) runs there beside the global search, forked
+    // from and joined back to `stream` by the two events
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::mutex mu;
     std::string name;
     vc1me_counters counters{};
@@ -804,12 +809,20 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     // the step's launch sequence; every launch takes the job table and scalars only
     auto launch_all = [&](bool events) -> int {
         HIP_TRY(vc1me::launch_step_begin(ctx->d_jobs, n, max_mbs_all, d_cnt, kRelaxCounters, st));
-        HIP_TRY(vc1me::launch_phase_planes_jobs(ctx->d_jobs, n, max_pw, max_ph, st));
+        // the phase planes and the signature index depend on the frames
+        // alone: they run on the side stream beside the global search (and
+        // the network), and the macroblock passes wait for both (in a
+        // captured step, two branches of the graph)
+        HIP_TRY(hipEventRecord(ctx->fork, st));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+        HIP_TRY(vc1me::launch_phase_planes_jobs(ctx->d_jobs, n, max_pw, max_ph, ctx->side));
+        HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), ctx->side));
+        HIP_TRY(hipEventRecord(ctx->join, ctx->side));
         HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st,
                                             events ? ctx->events[2] : nullptr, events ? ctx->events[3] : nullptr));
         if (after_global)
             if (int rc = after_global(st)) return rc;
-        HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), st));
+        HIP_TRY(hipStreamWaitEvent(st, ctx->join, 0));
         if (relax) {
             HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
             if (relax_guess()) {
@@ -1188,7 +1201,17 @@ int vc1me_create(int device, vc1me_ctx** out) {
         if (prop.multiProcessorCount > 0) ctx->cu_count = prop.multiProcessorCount;
     }
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) { delete ctx; return fail(VC1ME_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (ctx->join) (void)hipEventDestroy(ctx->join);
+        if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(VC1ME_EDEVICE, std::string("stream set-up: ") + hipGetErrorString(e));
+    }
     // the allocation-free scorer's scratch array on this device (looked up
     // once per device, with the device current)
     ctx->scratch = scratch_for_device(ctx->device, &ctx->scratch_error);
@@ -1220,6 +1243,10 @@ void vc1me_destroy(vc1me_ctx* ctx) {
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& b : ctx->phase_pool) if (b.first) (void)hipFree(b.first);
     for (auto e : ctx->events) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ctx->join);
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
