@@ -2335,7 +2335,8 @@ __device__ __forceinline__ void commit_state(const Ctx& C, NbState& st, uint8_t 
 
 // kGuess: the relaxation's starting state only (never a result): the
 // analysis without the 4-MV stage, the costliest part, whose outcome rarely
-// changes a neighbour's predictors.  The extended long-range search does run:
+// changes a neighbour's predictors, and with the diamond integer search in
+// place of UMH.  The extended long-range search does run:
 // its result depends on no neighbour and is kept for the step (pass 0 reads
 // it back), and the long-range vectors it finds reach the neighbours'
 // predictors and propagated searches in pass 0 already, instead of spreading
@@ -2436,7 +2437,10 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             if (match_good(C, pr.sad)) { im = pr; sc = true; selected_distant = true; }
         }
         if (!sc) {
-            if (J->algo == 1) im = diamond_search(C, pi, gdx, gdy, local_range);
+            // (the guess pass takes the cheaper diamond search: its result is
+            // only the relaxation's starting state, and the diamond lands on
+            // the UMH result for most macroblocks)
+            if (J->algo == 1 || kGuess) im = diamond_search(C, pi, gdx, gdy, local_range);
             else im = umh_search(C, pi, gdx, gdy, local_range);
         }
         if (J->lr_mode != 2) {
