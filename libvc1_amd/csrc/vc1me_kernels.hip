@@ -1960,6 +1960,9 @@ __device__ __forceinline__ IRes extended_content(Ctx& C, IRes local, uint64_t* c
 struct SubParams {
     int bx0, by0, bw, bh, mode, range;
     bool more;
+    // the guess pass's refinement: the fast stencils and the SAD+lambda*R
+    // winner only (no SATD finalists), whatever the job's settings
+    bool cheap = false;
 };
 
 __device__ __forceinline__ int make_even(int v) { return (v & 1) ? (v > 0 ? v - 1 : v + 1) : v; }
@@ -2143,7 +2146,7 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
     }
     if (hid) sub_add(C, sp, L, 0, 0, 1);
     uint64_t half = key_q(lane_read64(L.sad, 0), sxq, syq);
-    const bool fast = J->fast != 0;
+    const bool fast = J->fast != 0 || sp.cheap;
     // half-pel stencil around the seed
     {
         int rx = sxq, ry = syq, nb;
@@ -2255,7 +2258,7 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
     const WinKey w1 = lanes8_min(WinKey{in ? 0u : 1u, score, key_q(sad, xq, yq), 0});
     const uint64_t k1 = lane_read64(w1.a, 0);
     int2 result = make_int2(keyq_x(k1), keyq_y(k1));
-    if (J->quality == 2) {
+    if (J->quality == 2 && !sp.cheap) {
         const int rw = __ffsll(static_cast<long long>(__ballot(in && xq == result.x && yq == result.y))) - 1;
         const uint64_t rws = lane_read64(sad, rw);
         const uint64_t pct3 = (rws / 100) * 3 + (((rws % 100) * 3 + 99) / 100);
@@ -2335,8 +2338,8 @@ __device__ __forceinline__ void commit_state(const Ctx& C, NbState& st, uint8_t 
 
 // kGuess: the relaxation's starting state only (never a result): the
 // analysis without the 4-MV stage, the costliest part, whose outcome rarely
-// changes a neighbour's predictors, and with the diamond integer search in
-// place of UMH.  The extended long-range search does run:
+// changes a neighbour's predictors, with the diamond integer search in place
+// of UMH and the fast sub-pel stencils without SATD finalists.  The extended long-range search does run:
 // its result depends on no neighbour and is kept for the step (pass 0 reads
 // it back), and the long-range vectors it finds reach the neighbours'
 // predictors and propagated searches in pass 0 already, instead of spreading
@@ -2501,7 +2504,7 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
             return;
         }
         if (J->subpel) pw_load(C, C.x0 + im.dx - kPwMargin, C.y0 + im.dy - kPwMargin, mode_bilinear(one_mode));
-        SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true};
+        SubParams sp{C.x0, C.y0, 16, 16, one_mode, range, true, kGuess};
         one = refine_staged(C, sp, im.dx * 4, im.dy * 4, rate_pi, &one_sad, im.sad);
     }
     if (hidden_padding(C.g, C.x0, C.y0, 16, 16, one.x, one.y, one_mode, true)) {
