@@ -1420,15 +1420,16 @@ __device__ __forceinline__ uint64_t far_batch(Ctx& C, int n, int stride, int key
 
 // Full SADs of up to four integer vectors at once, read straight from the
 // reference frame: lane = (candidate, row).  Hidden-padding vectors cost
-// kBadSad.  Returns the minimum key (cost, |d|, dy, dx) over the n vectors
-// held in lanes 0..n-1 of (vx, vy).
-__device__ __forceinline__ uint64_t global_sad_min(Ctx& C, int vx, int vy, int n) {
+// kBadSad.  Returns the minimum key (cost, |d|, dy, dx) over the vectors i
+// < n held in lanes i of (vx, vy) whose bit i of `which` is set.
+__device__ __forceinline__ uint64_t global_sad_min(Ctx& C, int vx, int vy, int n, uint32_t which = ~0u) {
     const int W = C.g.W, H = C.g.H;
     const gptr_u8 ry = as_global(C.ry);
     uint64_t best = kNone;
     for (int base = 0; base < n; base += 4) {
+        if (!((which >> base) & 15u)) continue;
         const int c = base + (C.lane >> 4), r = C.lane & 15;
-        const bool mine = c < n;
+        const bool mine = c < n && ((which >> c) & 1u);
         const int dx = __shfl(vx, mine ? c : 0), dy = __shfl(vy, mine ? c : 0);
         const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
         uint32_t sum = 0;
@@ -1505,7 +1506,38 @@ __device__ __forceinline__ IRes seeded_search(Ctx& C, const Pred pi, int gdx, in
     const Bounds b = bounds_px(range);
     int vx, vy;
     seed_lane(C, o, gdx, gdy, b, round_qpel_to_pixel(pi.px), round_qpel_to_pixel(pi.py), vx, vy);
-    uint64_t best = global_sad_min(C, vx, vy, 5);
+    // the five seeds, uniform; those inside the staged reference window (the
+    // local search's, issued at the start of the analysis) are scored from
+    // LDS, the others from the frame
+    const int s0x = __builtin_amdgcn_readlane(vx, 0), s0y = __builtin_amdgcn_readlane(vy, 0);
+    const int s1x = __builtin_amdgcn_readlane(vx, 1), s1y = __builtin_amdgcn_readlane(vy, 1);
+    const int s2x = __builtin_amdgcn_readlane(vx, 2), s2y = __builtin_amdgcn_readlane(vy, 2);
+    const int s3x = __builtin_amdgcn_readlane(vx, 3), s3y = __builtin_amdgcn_readlane(vy, 3);
+    const int s4x = __builtin_amdgcn_readlane(vx, 4), s4y = __builtin_amdgcn_readlane(vy, 4);
+    auto seed = [=](int i, int& dx, int& dy) VC1ME_INLINE {
+        dx = i == 0 ? s0x : i == 1 ? s1x : i == 2 ? s2x : i == 3 ? s3x : s4x;
+        dy = i == 0 ? s0y : i == 1 ? s1y : i == 2 ? s2y : i == 3 ? s3y : s4y;
+    };
+    uint32_t inw = 0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        int dx, dy;
+        seed(i, dx, dy);
+        if (in_window(C, dx, dy)) inw |= 1u << i;
+    }
+    uint64_t best = kNone;
+    if (inw) {
+        // a seed outside the window takes the place of the first inside it
+        // (scored twice, the same key)
+        const int first = __builtin_ctz(inw);
+        best = int_batch(C, 5, 1, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
+            seed(((inw >> i) & 1u) ? i : first, dx, dy);
+        });
+    }
+    if (inw != 31u) {
+        const uint64_t g = global_sad_min(C, vx, vy, 5, ~inw & 31u);
+        best = g < best ? g : best;
+    }
     for (int it = 0; it < 32; ++it) {
         const int cx = key_dx(best), cy = key_dy(best);
         // one 28x28 window serves the descent until it walks out of it
@@ -2355,10 +2387,9 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     const int pos = C.pos;
 
     // the local integer search's window, issued first when the analysis will
-    // start with that search (its load overlaps the ones below; the wsync
-    // after them waits for all)
-    if (range > 0 && !J->seed_mvs && J->lr_mode != 2 &&
-        !(J->nn_out && J->nn_out[static_cast<size_t>(pos) * 5 + 4] >= J->nn_logit))
+    // start with that search or the seeded one (its load overlaps the ones
+    // below; the wsync after them waits for all)
+    if (range > 0 && !J->seed_mvs && J->lr_mode != 2)
         win_issue(C, C.x0 - local_range, C.y0 - local_range, 16 + 2 * local_range, 16 + 2 * local_range);
     // current block to LDS, zero outside the active area; the neighbours'
     // state snapshot
