@@ -2328,7 +2328,7 @@ struct NbState {
     int2 mv;
     int2 b[4];
     uint32_t fp;        // intra flag | propagated flag << 8
-    uint32_t changed;   // (lane 0) the analysis changed the macroblock's state
+    uint32_t changed;   // (lane 0) mask of the dependents that read a changed field (requeue_mask)
 };
 
 __device__ __forceinline__ NbState nb_state(const JobDesc* J, int pos) {
@@ -2341,25 +2341,48 @@ __device__ __forceinline__ NbState nb_state(const JobDesc* J, int pos) {
     return s;
 }
 
-__device__ __forceinline__ bool nb_equal(const NbState& a, const NbState& b) {
-    bool eq = a.mv.x == b.mv.x && a.mv.y == b.mv.y && a.fp == b.fp;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) eq = eq && a.b[k].x == b.b[k].x && a.b[k].y == b.b[k].y;
-    return eq;
+// The dependents of macroblock (mx, my) whose analysis reads a field that
+// differs between its states o and n: bit k for dependent k of
+// queue_dependents (right, bottom-left, bottom, bottom-right).  What each
+// reads of this macroblock through its neighbour snapshot:
+//   right (this is its left):  the 1-MV vector (predictor_info), block 1
+//     (predictor_info_mixed_1mv, the 4-MV predictor of its block 0), block 3
+//     (that of its block 2), the intra flag (hybrid tests);
+//   bottom (this is its top):  the vector, blocks 2 and 3 (the mixed 1-MV
+//     and 4-MV predictors of its blocks 0 and 1), the intra flag;
+//   bottom-left (top-right):   the vector, block 2;
+//   bottom-right (top-left):   block 3 (4-MV predictor of its block 0); the
+//     vector and block 3 in predictor_info and the mixed 1-MV predictor only
+//     when it sits in the last column;
+// and every one the propagated flag, and the vector when that flag is set
+// (propagated_long_range_motion).  Block 0 is read by none of them.  A
+// dependent whose read fields are all unchanged would repeat its last
+// analysis exactly, so it is not queued.
+__device__ __forceinline__ uint32_t requeue_mask(const NbState& o, const NbState& n, int mx, int mbw) {
+    auto ne = [](int2 a, int2 b) VC1ME_INLINE { return a.x != b.x || a.y != b.y; };
+    const bool dmv = ne(o.mv, n.mv), db1 = ne(o.b[1], n.b[1]), db2 = ne(o.b[2], n.b[2]), db3 = ne(o.b[3], n.b[3]);
+    const bool dintra = ((o.fp ^ n.fp) & 0xffu) != 0, dprop = ((o.fp ^ n.fp) >> 8) != 0;
+    const bool prop = (n.fp >> 8) != 0;
+    uint32_t m = 0;
+    if (dmv || db1 || db3 || dintra || dprop) m |= 1u;                      // right
+    if (dmv || db2 || dprop) m |= 2u;                                        // bottom-left
+    if (dmv || db2 || db3 || dintra || dprop) m |= 4u;                       // bottom
+    if (db3 || dprop || (dmv && (prop || mx + 1 == mbw - 1))) m |= 8u;       // bottom-right
+    return m;
 }
 
 // Lane 0 writes the macroblock's neighbour-visible state (vectors, flags,
-// propagated flag) and notes in st.changed whether it differs from the state
-// the macroblock held; the relaxation queues the dependents of a changed
-// macroblock.  No other workgroup writes this macroblock's state while it is
-// analysed, so the state read here is the one the analysis started from (and
-// no register holds it through the analysis).
+// propagated flag) and notes in st.changed which dependents read a field
+// that differs from the state the macroblock held (requeue_mask); the
+// relaxation queues those.  No other workgroup writes this macroblock's
+// state while it is analysed, so the state read here is the one the analysis
+// started from (and no register holds it through the analysis).
 __device__ __forceinline__ void commit_state(const Ctx& C, NbState& st, uint8_t flags, uint8_t prop) {
     st.changed = 0;
     if (C.lane == 0) {
         const JobDesc* J = C.J;
         const int pos = C.pos;
-        st.changed = nb_equal(nb_state(J, pos), st) ? 0u : 1u;
+        st.changed = requeue_mask(nb_state(J, pos), st, C.mx, C.g.mbw);
         J->mvs[pos] = st.mv;
 #pragma unroll
         for (int k = 0; k < 4; ++k) J->bmv[pos * 4 + k] = st.b[k];
@@ -2857,6 +2880,12 @@ __global__ void __launch_bounds__(256) k_relax_init(const JobDesc* __restrict__ 
 // through one contiguous band of macroblocks (and of jobs) and its L2 holds
 // what its neighbours read (xcd_band_start: a bijection of [0, T)).
 struct GridItem { int job, pos; };
+// Waves per SIMD the macroblock kernels are compiled for (their register
+// budget: 512 / waves VGPRs per lane); their LDS allows four.
+#ifndef VC1ME_MB_WAVES
+#define VC1ME_MB_WAVES 4
+#endif
+
 __device__ __forceinline__ GridItem xcd_banded_item() {
     const int T = static_cast<int>(gridDim.x * gridDim.y);
     const int L = static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x);
@@ -2865,7 +2894,7 @@ __device__ __forceinline__ GridItem xcd_banded_item() {
     return {item / per, item - (item / per) * per};
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_guess(const JobDesc* __restrict__ jobs) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_MB_WAVES))) k_mb_guess(const JobDesc* __restrict__ jobs) {
     extern __shared__ __align__(16) uint8_t smem[];
     const GridItem it = xcd_banded_item();
     const JobDesc* J = &jobs[it.job];
@@ -2874,15 +2903,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     run_mb<true>(J, pos % J->mbw, pos / J->mbw, smem);
 }
 
-// Appends the right, bottom-left, bottom and bottom-right neighbours of a
-// macroblock whose state changed to the list of the next pass (each once per
-// pass: the queue stamp), as relax_entry(job, macroblock) entries.
-__device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int pos, int next_pass, RelaxQueue Q) {
+// Appends the dependents in `mask` (bit k: right, bottom-left, bottom,
+// bottom-right) of a macroblock whose state changed to the list of the next
+// pass (each once per pass: the queue stamp), as relax_entry(job,
+// macroblock) entries.
+__device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int pos, int next_pass, RelaxQueue Q,
+                                                 uint32_t mask) {
     const int mbw = J->mbw, mbh = J->mbh;
     int32_t* next = Q.wl[next_pass & 1];
     const int mx = pos % mbw, my = pos / mbw;
     const int dxs[4] = {1, -1, 0, 1}, dys[4] = {0, 1, 1, 1};
     for (int k = 0; k < 4; ++k) {
+        if (!((mask >> k) & 1u)) continue;
         const int x = mx + dxs[k], y = my + dys[k];
         if (x < 0 || x >= mbw || y >= mbh) continue;
         const int d = y * mbw + x;
@@ -2897,7 +2929,7 @@ __device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int 
 // Pass 0: every macroblock of every job (one workgroup each, full
 // occupancy); the macroblocks whose neighbours' state changed form the list
 // of pass 1.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_relax0(const JobDesc* __restrict__ jobs,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_MB_WAVES))) k_mb_relax0(const JobDesc* __restrict__ jobs,
                                                                                        RelaxQueue Q) {
     extern __shared__ __align__(16) uint8_t smem[];
     const GridItem it = xcd_banded_item();
@@ -2907,7 +2939,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const int pos = it.pos;
     if (pos >= mbw * J->mbh) return;
     const NbState after = run_mb(J, pos % mbw, pos / mbw, smem);
-    if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, 1, Q);
+    if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, 1, Q, after.changed);
 }
 
 // Pass p > 0 as a work queue: a grid of resident workgroups takes entries of
@@ -2916,7 +2948,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
 // changed for pass p + 1.  The list length was final when the previous pass
 // ended; an empty list ends every workgroup at once, so passes launched
 // after the state settled cost only their launch.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_mb_relaxq(const JobDesc* __restrict__ jobs,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_MB_WAVES))) k_mb_relaxq(const JobDesc* __restrict__ jobs,
                                                                                        RelaxQueue Q, int pass) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int n = imin(Q.cnt[pass], Q.cap);
@@ -2933,7 +2965,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         const int job = relax_entry_job(code), pos = relax_entry_pos(code);
         const JobDesc* J = &jobs[job];
         const NbState after = run_mb(J, pos % J->mbw, pos / J->mbw, smem);
-        if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, pass + 1, Q);
+        if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, pass + 1, Q, after.changed);
         wsync();
         int nt = 0;
         if (threadIdx.x == 0) nt = atomicAdd(&Q.tk[pass], 1);
