@@ -3118,39 +3118,93 @@ __global__ void __launch_bounds__(256) k_step_end(const JobDesc* __restrict__ jo
 // ------------------------------------------------------------------ global search
 
 // Frame SAD partial sums (frame_sad, src/encoder_motion.cpp:1565-1575):
-// block = (row chunk, point, job); 256 threads, 8 sampled rows per block.
-__global__ void __launch_bounds__(256) k_frame_sads(const JobDesc* __restrict__ jobs) {
+// block = (row chunk, point group, job), 256 threads, 8 sampled rows per
+// block and up to kFsPoints points per group.  Each thread reads its current
+// samples once and the reference at every point of the group; unsampled
+// (stride 1) rows of a width divisible by 4 are read as dwords, four pixels
+// per v_sad_u8, with the clamped byte reads only where the reference dword
+// would leave the row.
+constexpr int kFsPoints = 8;
+__global__ void __launch_bounds__(256) k_frame_sads(const JobDesc* __restrict__ jobs, int per) {
     const JobDesc* J = &jobs[blockIdx.z];
     GlobalState* g = J->g;
-    const int p = blockIdx.y;
-    if (p >= g->n) return;
+    const int p0 = static_cast<int>(blockIdx.y) * per;
+    const int np = imin(imin(per, kFsPoints), g->n - p0);
+    if (np <= 0) return;
     const int s = g->stride;
-    const int dx = g->pts[2 * p], dy = g->pts[2 * p + 1];
     const int W = J->W, H = J->H;
     const int rows = (H + s - 1) / s, cols = (W + s - 1) / s;
-    const int r0 = blockIdx.x * 8;
+    const int r0 = static_cast<int>(blockIdx.x) * 8;
     if (r0 >= rows) return;
-    uint32_t acc = 0;
-    for (int r = r0; r < imin(rows, r0 + 8); ++r) {
-        const int y = r * s;
-        const int sy = clampi(y + dy, 0, H - 1);
-        const uint8_t* cr = J->cy + static_cast<size_t>(y) * W;
-        const uint8_t* rr = J->ry + static_cast<size_t>(sy) * W;
-        for (int c = threadIdx.x; c < cols; c += 256) {
-            const int x = c * s;
-            acc += static_cast<uint32_t>(iabs(static_cast<int>(cr[x]) - static_cast<int>(rr[clampi(x + dx, 0, W - 1)])));
+    const int nr = imin(rows, r0 + 8) - r0;
+    int pdx[kFsPoints], pdy[kFsPoints];
+#pragma unroll
+    for (int k = 0; k < kFsPoints; ++k) {
+        pdx[k] = k < np ? g->pts[2 * (p0 + k)] : 0;
+        pdy[k] = k < np ? g->pts[2 * (p0 + k) + 1] : 0;
+    }
+    uint32_t acc[kFsPoints];
+#pragma unroll
+    for (int k = 0; k < kFsPoints; ++k) acc[k] = 0;
+    const gptr_u8 cy = as_global(J->cy), ry = as_global(J->ry);
+    typedef const __attribute__((address_space(1))) uint32_t* gptr_u32;
+    const bool dwords = s == 1 && (W & 3) == 0;
+    const int units = dwords ? W >> 2 : cols;     // per row: dwords or samples
+    const int total = nr * units;
+    int rr = static_cast<int>(threadIdx.x) / units, u = static_cast<int>(threadIdx.x) - rr * units;
+    for (int i = static_cast<int>(threadIdx.x); i < total; i += 256) {
+        const int y = (r0 + rr) * s;
+        if (dwords) {
+            const int x = 4 * u;
+            const uint32_t c = *reinterpret_cast<gptr_u32>(cy + static_cast<size_t>(y) * W + x);
+#pragma unroll
+            for (int k = 0; k < kFsPoints; ++k) {
+                if (k >= np) break;
+                const gptr_u8 row = ry + static_cast<size_t>(clampi(y + pdy[k], 0, H - 1)) * W;
+                const int a = x + pdx[k];
+                uint32_t r;
+                if (a >= 0 && (a & ~3) + 8 <= W) {
+                    const gptr_u32 q = reinterpret_cast<gptr_u32>(row + (a & ~3));
+                    r = __builtin_amdgcn_alignbyte(q[1], q[0], a & 3);
+                } else {
+                    r = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) r |= static_cast<uint32_t>(row[clampi(a + j, 0, W - 1)]) << (8 * j);
+                }
+                acc[k] = __builtin_amdgcn_sad_u8(r, c, acc[k]);
+            }
+        } else {
+            const int x = u * s;
+            const int c = cy[static_cast<size_t>(y) * W + x];
+#pragma unroll
+            for (int k = 0; k < kFsPoints; ++k) {
+                if (k >= np) break;
+                const int sy = clampi(y + pdy[k], 0, H - 1);
+                acc[k] += static_cast<uint32_t>(iabs(c - static_cast<int>(ry[static_cast<size_t>(sy) * W +
+                                                                               clampi(x + pdx[k], 0, W - 1)])));
+            }
         }
+        u += 256;
+        while (u >= units) { u -= units; ++rr; }
     }
-    // block reduction
-    __shared__ uint32_t red[4];
-    acc = wave_sum32(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    // block sums per point
+    __shared__ uint32_t red[kFsPoints][4];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+#pragma unroll
+    for (int k = 0; k < kFsPoints; ++k) {
+        if (k >= np) break;
+        const uint32_t t = wave_sum32(acc[k]);
+        if ((threadIdx.x & 63) == 0) red[k][wave] = t;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint64_t t = static_cast<uint64_t>(red[0]) + red[1] + red[2] + red[3];
-        atomicAdd(reinterpret_cast<unsigned long long*>(&g->cost[p]), static_cast<unsigned long long>(t));
-        atomicAdd(&J->ops[4], static_cast<unsigned long long>(imin(rows, r0 + 8) - r0) * static_cast<unsigned long long>(cols));
+    if (static_cast<int>(threadIdx.x) < np) {
+        const int k = static_cast<int>(threadIdx.x);
+        const uint64_t t = static_cast<uint64_t>(red[k][0]) + red[k][1] + red[k][2] + red[k][3];
+        atomicAdd(reinterpret_cast<unsigned long long*>(&g->cost[p0 + k]), static_cast<unsigned long long>(t));
     }
+    if (threadIdx.x == 0)
+        atomicAdd(&J->ops[4], static_cast<unsigned long long>(np) * static_cast<unsigned long long>(nr) *
+                                  static_cast<unsigned long long>(cols));
 }
 
 __device__ __forceinline__ bool g_better(uint64_t ac, int ax, int ay, uint64_t bc, int bx, int by) {
@@ -4021,7 +4075,10 @@ hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc*
     // up to four stages; each: frame SADs of the stage's points, then the replay
     for (int stage = 0; stage < 4; ++stage) {
         if (!pts[stage]) continue;
-        hipLaunchKernelGGL(k_frame_sads, dim3(chunks[stage], pts[stage], njobs), dim3(256), 0, s, d_jobs);
+        // point groups of at most kFsPoints, as even as the count allows
+        const int groups = (pts[stage] + kFsPoints - 1) / kFsPoints;
+        const int per = (pts[stage] + groups - 1) / groups;
+        hipLaunchKernelGGL(k_frame_sads, dim3(chunks[stage], groups, njobs), dim3(256), 0, s, d_jobs, per);
         hipLaunchKernelGGL(k_global_step, dim3(njobs), dim3(64), 0, s, d_jobs, njobs, stage);
     }
     if (ev1) (void)hipEventRecord(ev1, s);
