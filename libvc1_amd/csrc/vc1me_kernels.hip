@@ -4091,35 +4091,123 @@ hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hi
     return hipGetLastError();
 }
 
-// Phase planes of one reference (see qpel_eval): one thread per plane
-// position, every phase of the filter set.
-__device__ __forceinline__ void phase_point(const uint8_t* __restrict__ ref, int W, int H, uint8_t* __restrict__ out,
-                                            int pp, int ph, int bil) {
-    const int px = static_cast<int>(blockIdx.x) * 64 + static_cast<int>(threadIdx.x & 63) - 2;
-    const int py = static_cast<int>(blockIdx.y) * 4 + static_cast<int>(threadIdx.x >> 6) - 2;
-    if (px > W || py > H) return;
+// Phase planes of one reference (see qpel_eval).  Each thread produces four
+// consecutive plane positions px0 .. px0+3 of row py, px0 = 4 t - 2, which
+// fill one aligned dword of every plane (plane column px + 2).  Their
+// clamped 4x7 neighbourhood v[r][k] = ref(py - 1 + r, px0 - 1 + k) serves
+// every phase: position j reads n[r][c] = v[r][j + c] of luma_mc_sample
+// (mc_sample).  A vertical filter sum depends only on the column and vm, so
+// it is formed once per column and rounded for both horizontal classes
+// (shift 1 or 3 at vm 2, 3 or 5 at vm 1 and 3: hm 2 against hm 1 and 3).
+// Four byte values (each in [0, 255]) packed little-endian through v_perm_b32.
+// Packed with shifts and ors, the clamped filter outputs are fused by the
+// compiler into v_ashr_pk_u8_i32, which writes only the low half of its
+// destination (still holding the first unshifted sum, all ones above bit 15
+// when negative), and the two bytes or-ed in above came out as 255.
+__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
+    const uint32_t lo = __builtin_amdgcn_perm(static_cast<uint32_t>(b), static_cast<uint32_t>(a), 0x0c0c0400u);
+    const uint32_t hi = __builtin_amdgcn_perm(static_cast<uint32_t>(d), static_cast<uint32_t>(c), 0x0c0c0400u);
+    return __builtin_amdgcn_perm(hi, lo, 0x05040100u);
+}
+__device__ __forceinline__ void phase_quad(const uint8_t* __restrict__ ref, int W, int H, uint8_t* __restrict__ out,
+                                           int pp, int ph, int bil, int t, int py) {
+    const int px0 = 4 * t - 2;
+    if (px0 > W || py > H) return;
     const gptr_u8 r = as_global(ref);
-    int n[4][4];
+    typedef const __attribute__((address_space(1))) uint32_t* gptr_u32;
+    int v[4][7];
+    const int x0 = px0 - 1;              // = 4 t - 3
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         const size_t row = static_cast<size_t>(clampi(py - 1 + a, 0, H - 1)) * W;
+        if ((W & 3) == 0 && x0 - 1 >= 0 && x0 + 6 <= W - 1) {
+            // bytes 4t-4 .. 4t+3 of the row: two aligned dwords
+            const gptr_u32 q = reinterpret_cast<gptr_u32>(r + row + (x0 - 1));
+            const uint32_t d0 = q[0], d1 = q[1];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) n[a][b] = r[row + clampi(px - 1 + b, 0, W - 1)];
+            for (int k = 0; k < 7; ++k) v[a][k] = static_cast<int>(((k < 3 ? d0 >> (8 * (k + 1)) : d1 >> (8 * (k - 3)))) & 255u);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) v[a][k] = r[row + clampi(x0 + k, 0, W - 1)];
+        }
     }
     const size_t psz = static_cast<size_t>(pp) * ph;
-    uint8_t* o = out + static_cast<size_t>(py + 2) * pp + (px + 2);
+    uint8_t* o = out + static_cast<size_t>(py + 2) * pp + (px0 + 2);
+    uint32_t w[16];
+    int np;
     if (bil) {
+        np = 4;
+        int s1[4], s2[4], s3[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) o[k * psz] = static_cast<uint8_t>(mc_sample(n, (k & 1) * 2, (k >> 1) * 2, true));
+        for (int j = 0; j < 4; ++j) {
+            const int n11 = v[1][j + 1], n12 = v[1][j + 2], n21 = v[2][j + 1], n22 = v[2][j + 2];
+            s1[j] = (n11 + n12 + 1) >> 1;
+            s2[j] = (n11 + n21 + 1) >> 1;
+            s3[j] = (n11 + n12 + n21 + n22 + 2) >> 2;
+        }
+        w[0] = pack4(v[1][1], v[1][2], v[1][3], v[1][4]);
+        w[1] = pack4(s1[0], s1[1], s1[2], s1[3]);
+        w[2] = pack4(s2[0], s2[1], s2[2], s2[3]);
+        w[3] = pack4(s3[0], s3[1], s3[2], s3[3]);
     } else {
+        np = 16;
+        w[0] = pack4(v[1][1], v[1][2], v[1][3], v[1][4]);
+        // vm = 0: horizontal only
 #pragma unroll
-        for (int k = 0; k < 16; ++k) o[k * psz] = static_cast<uint8_t>(mc_sample(n, k & 3, k >> 2, false));
+        for (int hm = 1; hm < 4; ++hm) {
+            const int bias = hm == 2 ? 8 : 32, sh = hm == 2 ? 4 : 6;
+            int q[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                q[j] = clampi(arshift(mspel(v[1][j], v[1][j + 1], v[1][j + 2], v[1][j + 3], hm) + bias, sh), 0, 255);
+            w[hm] = pack4(q[0], q[1], q[2], q[3]);
+        }
+#pragma unroll
+        for (int vm = 1; vm < 4; ++vm) {
+            // hm = 0: vertical only
+            {
+                const int bias = vm == 2 ? 7 : 31, sh = vm == 2 ? 4 : 6;
+                int q[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    q[j] = clampi(arshift(mspel(v[0][j + 1], v[1][j + 1], v[2][j + 1], v[3][j + 1], vm) + bias, sh), 0, 255);
+                w[vm * 4] = pack4(q[0], q[1], q[2], q[3]);
+            }
+            // the vertical sums of the seven columns, rounded per class
+            int sv[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) sv[k] = mspel(v[0][k], v[1][k], v[2][k], v[3][k], vm);
+#pragma unroll
+            for (int hm = 1; hm < 4; ++hm) {
+                const int shift = ((hm == 2 ? 1 : 5) + (vm == 2 ? 1 : 5)) >> 1;
+                const int r1 = (1 << (shift - 1)) - 1;
+                int tv[7];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) tv[k] = arshift(sv[k] + r1, shift);
+                int q[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    q[j] = clampi(arshift(mspel(tv[j], tv[j + 1], tv[j + 2], tv[j + 3], hm) + 64, 7), 0, 255);
+                w[vm * 4 + hm] = pack4(q[0], q[1], q[2], q[3]);
+            }
+        }
     }
+    // the dword lies inside the row's pitch (pitch >= W + 11 and a multiple
+    // of 16; positions past W only fill columns no reader uses)
+    typedef uint32_t* u32p;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        if (k < np) *reinterpret_cast<u32p>(o + k * psz) = w[k];
 }
+
+// grid: x = groups of 64 quads of a row, y = groups of 4 rows (positions
+// px in [-2, W], py in [-2, H])
+__device__ __forceinline__ int phase_quad_t() { return static_cast<int>(blockIdx.x) * 64 + static_cast<int>(threadIdx.x & 63); }
+__device__ __forceinline__ int phase_quad_y() { return static_cast<int>(blockIdx.y) * 4 + static_cast<int>(threadIdx.x >> 6) - 2; }
 
 __global__ void __launch_bounds__(256) k_phase_planes(const uint8_t* __restrict__ ref, int W, int H,
                                                       uint8_t* __restrict__ out, int pp, int ph, int bil) {
-    phase_point(ref, W, H, out, pp, ph, bil);
+    phase_quad(ref, W, H, out, pp, ph, bil, phase_quad_t(), phase_quad_y());
 }
 
 // The phase planes of every job that owns its reference's planes in the step
@@ -4128,20 +4216,24 @@ __global__ void __launch_bounds__(256) k_phase_planes_jobs(const JobDesc* __rest
     const JobDesc* J = &jobs[blockIdx.z];
     if (!J->phase_owner) return;
     const bool bil = J->mode == 2;
-    phase_point(J->ry, J->W, J->H, const_cast<uint8_t*>(bil ? J->bp : J->qp), J->pp, J->ph, bil ? 1 : 0);
+    phase_quad(J->ry, J->W, J->H, const_cast<uint8_t*>(bil ? J->bp : J->qp), J->pp, J->ph, bil ? 1 : 0,
+               phase_quad_t(), phase_quad_y());
 }
+
+// quads of a row: positions -2 .. W, four per quad
+__host__ __device__ inline int phase_quads(int width) { return (width + 2) / 4 + 1; }
 
 hipError_t launch_phase_planes(const uint8_t* ref, int width, int height, uint8_t* out, int pitch, int rows,
                                int bilinear, hipStream_t s) {
-    if (rows < height + 3 || pitch < width + 3 + 8) return hipErrorInvalidValue;
-    const dim3 grid((width + 3 + 63) / 64, (height + 3 + 3) / 4);
+    if (rows < height + 3 || pitch < width + 3 + 8 || (pitch & 15)) return hipErrorInvalidValue;
+    const dim3 grid((phase_quads(width) + 63) / 64, (height + 3 + 3) / 4);
     hipLaunchKernelGGL(k_phase_planes, grid, dim3(256), 0, s, ref, width, height, out, pitch, rows, bilinear);
     return hipGetLastError();
 }
 
 hipError_t launch_phase_planes_jobs(const JobDesc* d_jobs, int njobs, int max_w, int max_h, hipStream_t s) {
     if (njobs <= 0 || max_w <= 0) return hipSuccess;
-    const dim3 grid((max_w + 3 + 63) / 64, (max_h + 3 + 3) / 4, njobs);
+    const dim3 grid((phase_quads(max_w) + 63) / 64, (max_h + 3 + 3) / 4, njobs);
     hipLaunchKernelGGL(k_phase_planes_jobs, grid, dim3(256), 0, s, d_jobs);
     return hipGetLastError();
 }

@@ -266,6 +266,41 @@ def test_block_cost_primitive_fuzz(backend):
         fr.release()
 
 
+def test_block_costs_on_noise_at_every_phase(backend):
+    """Interpolated SADs on white noise, where the bicubic filter sums leave
+    [0, 255 << shift] on both sides for many samples (the clamps of every
+    phase), at every quarter-pel phase of a 25x25 candidate grid for blocks
+    in the corners, on the edges and inside a 176x144 frame, bicubic and
+    bilinear, against the oracle's block_sad_motion."""
+    import ctypes as C
+    from _oracle import oracle
+    rng = np.random.default_rng(4242)
+    w, h = 176, 144
+    cur = synth.Picture(rng.integers(0, 256, (h, w), dtype=np.uint8), rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8),
+                      rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8))
+    ref = synth.Picture(rng.integers(0, 256, (h, w), dtype=np.uint8), rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8),
+                      rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8))
+    p = abi.default_params(w, h, pqindex=8, motion_search_range=64, motion_local_search_range=16)
+    cs, rs = abi.PlaneSet.of(cur), abi.PlaneSet.of(ref)
+    grid = np.array([(x, y) for y in range(-12, 13) for x in range(-12, 13)], np.int32)
+    fc, fr = backend.upload(cur), backend.upload(ref)
+    try:
+        for (x0, y0) in ((0, 0), (160, 128), (80, 64), (0, 64), (160, 0), (96, 128)):
+            for mode in (0, 2):
+                cands = grid & ~1 if mode == 2 else grid
+                _say(f"[case] block costs on noise at ({x0},{y0}) mode={mode}")
+                got = backend.block_costs(fc, fr, p, x0, y0, 16, 16, mode, 0, cands)
+                want = np.array([oracle().oracle_block_cost(C.byref(p), C.byref(cs.struct), C.byref(rs.struct),
+                                                            x0, y0, 16, 16, int(a), int(b), mode, 0)
+                                 for a, b in cands], np.uint64)
+                bad = np.nonzero(got != want)[0]
+                assert not len(bad), f"({x0},{y0}) mode={mode}: " + \
+                    ", ".join(f"{tuple(cands[i])}: {got[i]} vs {want[i]}" for i in bad[:6])
+    finally:
+        fc.release()
+        fr.release()
+
+
 @pytest.mark.parametrize("mode,rng_", [(abi.MODE_ONE_MV_QPEL, 512), (abi.MODE_MIXED_MV, 32), (abi.MODE_MIXED_MV, 128)],
                          ids=["qpel_r512", "mixed_r32", "mixed_r128"])
 def test_pan_variants(backend, mode, rng_):
