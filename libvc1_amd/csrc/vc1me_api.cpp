@@ -606,6 +606,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     hipStream_t st = ctx->stream;
     int max_local = 0, max_diag = 0;
     size_t max_cells = 0, total_mbs = 0;
+    int max_ncx = 0, max_ncy = 0;        // signature lattice of the jobs that search long range
     for (int i = 0; i < n; ++i) {
         const vc1me_params* p = jobs[i].params;
         const size_t mbs = static_cast<size_t>((p->width + 15) / 16) * ((p->height + 15) / 16);
@@ -634,7 +635,11 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             HIP_TRY(hipMemcpyAsync(ctx->slots[i].seed_mvs, sd->mvs, mbs * sizeof(int2), hipMemcpyHostToDevice, st));
             ctx->h_jobs[i].seed_mvs = ctx->slots[i].seed_mvs;
         }
-        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range && !seeds) max_cells = std::max(max_cells, cells);
+        if (ctx->h_jobs[i].range > ctx->h_jobs[i].local_range && !seeds) {
+            max_cells = std::max(max_cells, cells);
+            max_ncx = std::max(max_ncx, ctx->h_jobs[i].ncx);
+            max_ncy = std::max(max_ncy, ctx->h_jobs[i].ncy);
+        }
     }
     if (ctx->d_jobs_cap < static_cast<size_t>(n)) {
         if (ctx->d_jobs) (void)hipFree(ctx->d_jobs);
@@ -816,7 +821,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         HIP_TRY(hipEventRecord(ctx->fork, st));
         HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
         HIP_TRY(vc1me::launch_phase_planes_jobs(ctx->d_jobs, n, max_pw, max_ph, ctx->side));
-        HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, static_cast<int>(max_cells), ctx->side));
+        HIP_TRY(vc1me::launch_signatures(ctx->d_jobs, n, max_ncx, max_ncy, ctx->side));
         HIP_TRY(hipEventRecord(ctx->join, ctx->side));
         HIP_TRY(vc1me::launch_global_search(ctx->d_jobs, n, ctx->h_jobs.data(), st,
                                             events ? ctx->events[2] : nullptr, events ? ctx->events[3] : nullptr));

@@ -196,7 +196,7 @@ constexpr uint8_t kFlagGoodMatch = 4;   // motion-failure check: a usable match
 // Kernel launchers (vc1me_kernels.hip).
 hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc* h_jobs, hipStream_t s,
                                 hipEvent_t ev0, hipEvent_t ev1);
-hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hipStream_t s);
+hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_ncx, int max_ncy, hipStream_t s);
 hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int max_per_diag, int lds_bytes,
                               hipStream_t s);
 // extended long-range search of every macroblock of the plain jobs, ahead of

@@ -3494,35 +3494,75 @@ __global__ void __launch_bounds__(64) k_global_step(const JobDesc* __restrict__ 
 // ------------------------------------------------------------------ long-range index
 
 // long_range_signature of every 8-px lattice cell of the reference
-// (src/encoder_motion.cpp:1578-1625), packed 6 bytes per cell.
-__global__ void k_signatures(const JobDesc* __restrict__ jobs, int max_cells) {
-    const JobDesc* J = &jobs[blockIdx.y];
-    if (J->range <= J->local_range || J->range == 0) return;
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    const int total = J->ncx * J->ncy;
-    if (cell >= total) return;
-    const int x0 = (cell % J->ncx) * 8, y0 = (cell / J->ncx) * 8;
-    const int W = J->W, H = J->H, cw = W / 2, ch = H / 2;
-    auto mean = [](const uint8_t* p, int pw, int ph, int bx, int by) -> uint32_t {
-        uint32_t sum = 0, n = 0;
-        for (int y = 0; y < 8 && by + y < ph; ++y) {
-            if (by + y < 0) continue;
-            for (int x = 0; x < 8 && bx + x < pw; ++x) {
-                if (bx + x < 0) continue;
-                sum += p[static_cast<size_t>(by + y) * pw + bx + x];
-                ++n;
-            }
+// (src/encoder_motion.cpp:1578-1625), packed 6 bytes per cell: the rounded
+// means of the four 8x8 luma blocks of the cell's 16x16 area and of the 8x8
+// chroma blocks at half its position, each over the pixels inside the plane
+// (128 when none is).  A workgroup takes a tile of 64 x 4 cells: it first sums
+// the aligned 8x8 luma blocks and the 4x4 chroma sub-blocks the tile's cells
+// cover (65 x 5 of each, every pixel read once, rows as dwords where the
+// plane allows), then each thread forms one cell's means from those sums.
+constexpr int kSigTileX = 64, kSigTileY = 4;
+constexpr int kSigBx = kSigTileX + 1, kSigBy = kSigTileY + 1;
+
+// sum of the pixels of p (pitch pw, pw x ph) in [x0, x0 + n) x [y0, y0 + n)
+__device__ __forceinline__ uint32_t block_sum(gptr_u8 p, int pw, int ph, int x0, int y0, int n, bool dwords) {
+    typedef const __attribute__((address_space(1))) uint32_t* gptr_u32;
+    uint32_t sum = 0;
+    const int rows = imin(n, ph - y0);
+    if (dwords && x0 + n <= pw) {
+        for (int y = 0; y < rows; ++y) {
+            const gptr_u32 q = reinterpret_cast<gptr_u32>(p + static_cast<size_t>(y0 + y) * pw + x0);
+            for (int k = 0; k < n / 4; ++k) sum = __builtin_amdgcn_sad_u8(q[k], 0u, sum);
         }
-        return n ? (sum + n / 2) / n : 128u;
-    };
+    } else {
+        const int cols = imin(n, pw - x0);
+        for (int y = 0; y < rows; ++y)
+            for (int x = 0; x < cols; ++x) sum += p[static_cast<size_t>(y0 + y) * pw + x0 + x];
+    }
+    return sum;
+}
+__device__ __forceinline__ uint32_t rounded_mean(uint32_t sum, int n) {
+    return n > 0 ? (sum + static_cast<uint32_t>(n) / 2) / static_cast<uint32_t>(n) : 128u;
+}
+
+__global__ void __launch_bounds__(256) k_signatures(const JobDesc* __restrict__ jobs) {
+    const JobDesc* J = &jobs[blockIdx.z];
+    if (J->range <= J->local_range || J->range == 0) return;
+    const int ncx = J->ncx, ncy = J->ncy;
+    const int cx0 = static_cast<int>(blockIdx.x) * kSigTileX, cy0 = static_cast<int>(blockIdx.y) * kSigTileY;
+    if (cx0 >= ncx || cy0 >= ncy) return;
+    const int W = J->W, H = J->H, cw = W / 2, ch = H / 2;
+    __shared__ uint32_t ls[kSigBy][kSigBx], us[kSigBy][kSigBx], vs[kSigBy][kSigBx];
+    const gptr_u8 py = as_global(J->ry), pu = as_global(J->ru), pv = as_global(J->rv);
+    const bool ydw = (W & 3) == 0 && (reinterpret_cast<uintptr_t>(J->ry) & 3) == 0;
+    const bool udw = (cw & 3) == 0 && (reinterpret_cast<uintptr_t>(J->ru) & 3) == 0;
+    const bool vdw = (cw & 3) == 0 && (reinterpret_cast<uintptr_t>(J->rv) & 3) == 0;
+    for (int i = static_cast<int>(threadIdx.x); i < kSigBx * kSigBy; i += 256) {
+        const int by = i / kSigBx, bx = i - by * kSigBx;
+        const int gx = cx0 + bx, gy = cy0 + by;        // luma block / chroma sub-block index
+        uint32_t l = 0, u = 0, v = 0;
+        if (8 * gx < W && 8 * gy < H) l = block_sum(py, W, H, 8 * gx, 8 * gy, 8, ydw);
+        if (4 * gx < cw && 4 * gy < ch) {
+            u = block_sum(pu, cw, ch, 4 * gx, 4 * gy, 4, udw);
+            v = block_sum(pv, cw, ch, 4 * gx, 4 * gy, 4, vdw);
+        }
+        ls[by][bx] = l; us[by][bx] = u; vs[by][bx] = v;
+    }
+    __syncthreads();
+    const int tx = static_cast<int>(threadIdx.x) & (kSigTileX - 1), ty = static_cast<int>(threadIdx.x) / kSigTileX;
+    const int cx = cx0 + tx, cy = cy0 + ty;
+    if (cx >= ncx || cy >= ncy) return;
+    const int x0 = cx * 8, y0 = cy * 8;
+    auto lum_n = [&](int bx, int by) VC1ME_INLINE { return imax(0, imin(8, W - bx)) * imax(0, imin(8, H - by)); };
+    const int cn = imax(0, imin(8, cw - x0 / 2)) * imax(0, imin(8, ch - y0 / 2));
     uint64_t s = 0;
-    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0, y0));
-    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0 + 8, y0)) << 8;
-    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0, y0 + 8)) << 16;
-    s |= static_cast<uint64_t>(mean(J->ry, W, H, x0 + 8, y0 + 8)) << 24;
-    s |= static_cast<uint64_t>(mean(J->ru, cw, ch, x0 / 2, y0 / 2)) << 32;
-    s |= static_cast<uint64_t>(mean(J->rv, cw, ch, x0 / 2, y0 / 2)) << 40;
-    J->sig[cell] = s;
+    s |= static_cast<uint64_t>(rounded_mean(ls[ty][tx], lum_n(x0, y0)));
+    s |= static_cast<uint64_t>(rounded_mean(ls[ty][tx + 1], lum_n(x0 + 8, y0))) << 8;
+    s |= static_cast<uint64_t>(rounded_mean(ls[ty + 1][tx], lum_n(x0, y0 + 8))) << 16;
+    s |= static_cast<uint64_t>(rounded_mean(ls[ty + 1][tx + 1], lum_n(x0 + 8, y0 + 8))) << 24;
+    s |= static_cast<uint64_t>(rounded_mean(us[ty][tx] + us[ty][tx + 1] + us[ty + 1][tx] + us[ty + 1][tx + 1], cn)) << 32;
+    s |= static_cast<uint64_t>(rounded_mean(vs[ty][tx] + vs[ty][tx + 1] + vs[ty + 1][tx] + vs[ty + 1][tx + 1], cn)) << 40;
+    J->sig[static_cast<size_t>(cy) * ncx + cx] = s;
 }
 
 // ------------------------------------------------------------------ batch scorer
@@ -4085,9 +4125,10 @@ hipError_t launch_global_search(const JobDesc* d_jobs, int njobs, const JobDesc*
     return hipGetLastError();
 }
 
-hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_cells, hipStream_t s) {
-    if (max_cells <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_signatures, dim3((max_cells + 255) / 256, njobs), dim3(256), 0, s, d_jobs, max_cells);
+hipError_t launch_signatures(const JobDesc* d_jobs, int njobs, int max_ncx, int max_ncy, hipStream_t s) {
+    if (max_ncx <= 0 || max_ncy <= 0) return hipSuccess;
+    const dim3 grid((max_ncx + kSigTileX - 1) / kSigTileX, (max_ncy + kSigTileY - 1) / kSigTileY, njobs);
+    hipLaunchKernelGGL(k_signatures, grid, dim3(256), 0, s, d_jobs);
     return hipGetLastError();
 }
 
