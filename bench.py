@@ -15,11 +15,14 @@ already resident in HBM, under the default relaxation schedule (a guess
 pass, every macroblock at once, then worklist passes until nothing changes;
 bit-exact with the raster-order reference), captured as one HIP graph; the
 MV fields stay on the device (their download is timed separately and
-reported as `value_with_download`).  `value` keeps the autotuned number of
-steps in flight (`steps_in_flight`: one context with its own stream and
-device state per host thread, all on the same resident frames), so that the
-latency-bound end of one step overlaps the next; `value_one_in_flight` is
-the same steps one at a time.  The same steps under the anti-diagonal sweep
+reported as `value_with_download`).  `value` keeps several steps in flight
+(`steps_in_flight`: one context with its own stream and device state per
+host thread, all on the same resident frames), so that the latency-bound end
+of one step overlaps the next; their number is tuned on this workload before
+the timed steps (two rounds of K steps at each of 1, 2 and 3,
+`schedule.in_flight_trial`)
+unless --in-flight sets it; `value_one_in_flight` is the same steps one at a
+time.  The same steps under the anti-diagonal sweep
 are timed for comparison (`schedule.diagonal_value`).
 
 The learned-seeded form (`learned`) runs the same steps through
@@ -225,30 +228,59 @@ def device_peak_ops():
     return cus, clk, cus * 64 * clk
 
 
-def learned_leg(lanes, jobs, plist, plain_value, plain_value_one, steps, warmup, mbs_per_step, threshold=0.5):
+def warm(lanes, fn, n: int = 1) -> None:
+    """n steps on every context, one context after the other (run_in_flight
+    deals steps to whichever context is free, so a warm-up through it can
+    leave a context without its first step, graph capture included, until
+    the timed steps)."""
+    for ln in lanes:
+        for k in range(max(1, n)):
+            fn(ln, k)
+
+
+def learned_leg(lanes, jobs, plist, plain_value, plain_value_one, steps, warmup, mbs_per_step, threshold=0.5,
+                in_flight=0):
     """The learned-seeded form on the same step: timed like the plain form
-    (network forward inside each captured step, the same steps in flight),
-    then judged by the cost report against the plain form, with two
-    ablations that remove the network's contribution."""
+    (network forward inside each captured step; its number of steps in
+    flight tuned the same way on the contexts of `lanes`, unless `in_flight`
+    fixes it), then judged by the cost report against the plain form, with
+    two ablations that remove the network's contribution."""
     import numpy as np
     from libvc1_amd import abi, batching, learned
     from libvc1_amd.backend import ProposalNet
 
-    be = lanes[0]
+    pool = lanes
+    be = pool[0]
     params = learned.load_weights()
     nets = []
-    for ln in lanes:
+    for ln in pool:
         net = ProposalNet(ln)
         net.set_params(params)
         nets.append(net)
         ln.set_graphs(True)
-    idx = {id(ln): i for i, ln in enumerate(lanes)}
+    idx = {id(ln): i for i, ln in enumerate(pool)}
 
     def learned_step(ln, k):
         ln.analyze_step_learned(nets[idx[id(ln)]], jobs, plist, threshold, want_results=False)
 
+    # steps in flight tuned for this form as for the plain one (two rounds
+    # of `steps` steps at each count, each count's better round)
+    trial = {}
+    if in_flight:
+        lanes = pool[:in_flight]
+    else:
+        warm(pool[:max(batching.IN_FLIGHT_CANDIDATES)], learned_step)
+        for _ in range(2):
+            for k in batching.IN_FLIGHT_CANDIDATES:
+                t0 = time.perf_counter()
+                batching.run_in_flight(pool[:k], steps, learned_step)
+                trial[k] = max(trial.get(k, 0.0), mbs_per_step * steps / (time.perf_counter() - t0))
+        lanes = pool[:max(trial, key=trial.get)]
+        log("learned, steps in flight, trial MB/s: " + ", ".join(f"{k}: {v:,.0f}" for k, v in trial.items()) +
+            f" -> {len(lanes)}")
+
     def timed(fn):
-        batching.run_in_flight(lanes, max(1, warmup) * len(lanes), fn)
+        warm(lanes, fn, warmup)
         t0 = time.perf_counter()
         batching.run_in_flight(lanes, steps, fn)
         return time.perf_counter() - t0
@@ -261,7 +293,7 @@ def learned_leg(lanes, jobs, plist, plain_value, plain_value_one, steps, warmup,
     value_one = mbs_per_step * steps / (time.perf_counter() - t0)
     log(f"learned: {elapsed * 1e3 / steps:.1f} ms/step, {value:,.0f} MB/s ({value / plain_value:.3f}x plain; "
         f"one in flight {value_one / plain_value_one:.3f}x)")
-    for ln in lanes:
+    for ln in pool:
         ln.set_graphs(False)
     net = nets[0]
     judged = learned.cost_compare(be, net, jobs, plist, threshold)
@@ -323,6 +355,7 @@ def learned_leg(lanes, jobs, plist, plain_value, plain_value_one, steps, warmup,
     return {
         "value": round(value, 1), "unit": "MB/s", "speedup_vs_plain": round(value / plain_value, 4),
         "ms_per_step": round(elapsed * 1e3 / steps, 3), "steps_in_flight": len(lanes),
+        "in_flight_trial": {str(k): round(v, 1) for k, v in trial.items()},
         "value_one_in_flight": round(value_one, 1),
         "speedup_vs_plain_one_in_flight": round(value_one / plain_value_one, 4), "threshold": threshold,
         "cost_rise": round(judged["cost_rise"], 6), "frac_worse": round(judged["frac_worse"], 6),
@@ -625,7 +658,7 @@ def hard_leg(lanes, gen, steps):
             # the same frame step repeated, `len(lanes)` steps in flight
             secs = {}
             for name, fn in (("plain", plain), ("learned", seeded)):
-                batching.run_in_flight(lanes, len(lanes), fn)
+                warm(lanes, fn)
                 t0 = time.perf_counter()
                 batching.run_in_flight(lanes, steps, fn)
                 secs[name] = (time.perf_counter() - t0) / steps
@@ -732,7 +765,7 @@ def main() -> int:
     ap.add_argument("--no-seeded", action="store_true", help="skip the seeded-core leg")
     ap.add_argument("--no-bframes", action="store_true", help="skip the B-picture leg")
     ap.add_argument("--in-flight", type=int, default=0,
-                    help="steps kept in flight, one context and host thread each (0: the tuned setting)")
+                    help="steps kept in flight, one context and host thread each (0: tuned on this workload before the timed steps)")
     ap.add_argument("--batched-scale", type=float, default=0.1,
                     help="stream lengths (8..120 frames) are scaled by this in the batched leg")
     args = ap.parse_args()
@@ -755,22 +788,38 @@ def main() -> int:
     jobs = [(frames[c], frames[r]) for c, r in pairs]
     plist = [params] * len(jobs)
     mbs_per_step = len(jobs) * ((WIDTH + 15) // 16) * ((HEIGHT + 15) // 16)
-    flight = args.in_flight or batching.best_in_flight(batching.load_table(be), WIDTH, HEIGHT, len(jobs))
     # steps in flight: one context (own stream and device state) per host
-    # thread, all on the frames uploaded by the first
-    lanes = [be] + [Backend(0) for _ in range(flight - 1)]
-
-    # the timed steps: each step one captured HIP graph (relaxation passes
-    # driven from the device, every per-macroblock field and the frame
-    # aggregates computed; nothing copied back but the step summary)
-    for ln in lanes:
+    # thread, all on the frames uploaded by the first; the timed steps: each
+    # step one captured HIP graph (relaxation passes driven from the device,
+    # every per-macroblock field and the frame aggregates computed; nothing
+    # copied back but the step summary)
+    pool = [be] + [Backend(0) for _ in range(max(args.in_flight, max(batching.IN_FLIGHT_CANDIDATES)) - 1)]
+    for ln in pool:
         ln.set_graphs(True)
 
     def plain_step(ln, k):
         ln.analyze_step(jobs, plist, want_results=False)
         t = ln.timing()
         return int(t.relax_passes), int(t.relax_analyses), int(t.graph_replay), int(t.schedule)
-    batching.run_in_flight(lanes, max(1, args.warmup) * len(lanes), plain_step)
+    flight = args.in_flight
+    trial = {}
+    if not flight:
+        # the number in flight, tuned on this workload before the timed
+        # steps (the table's trials run other clips and parameters): two
+        # interleaved rounds at each candidate count, as many steps as the
+        # timed region (fewer, and the start and the tail, when not every
+        # context has a step, weigh more the more contexts there are), after
+        # one warm-up step per context; each count's better round
+        warm(pool[:max(batching.IN_FLIGHT_CANDIDATES)], plain_step)
+        for _ in range(2):
+            for k in batching.IN_FLIGHT_CANDIDATES:
+                t0 = time.perf_counter()
+                batching.run_in_flight(pool[:k], args.steps, plain_step)
+                trial[k] = max(trial.get(k, 0.0), mbs_per_step * args.steps / (time.perf_counter() - t0))
+        flight = max(trial, key=trial.get)
+        log("steps in flight, trial MB/s: " + ", ".join(f"{k}: {v:,.0f}" for k, v in trial.items()) + f" -> {flight}")
+    lanes = pool[:flight]
+    warm(lanes, plain_step, args.warmup)
     log(f"warmup: {max(1, args.warmup)} step(s) on each of {len(lanes)} context(s)")
     t0 = time.perf_counter()
     recs = batching.run_in_flight(lanes, args.steps, plain_step)
@@ -787,11 +836,12 @@ def main() -> int:
     batching.run_in_flight(lanes, args.steps, lambda ln, k: ln.analyze_step(jobs, plist, want_results=True))
     with_dl = mbs_per_step * args.steps / (time.perf_counter() - t1)
     results = be.analyze_step(jobs, plist, want_results=True)
-    for ln in lanes:
+    for ln in pool:
         ln.set_graphs(False)
     schedule = {"kind": "relax" if recs[-1][3] == abi.SCHEDULE_RELAX else "diagonal",
                 "launch": "one captured HIP graph per step" if recs[-1][2] == 1 else "eager",
                 "steps_in_flight": len(lanes),
+                "in_flight_trial": {str(k): round(v, 1) for k, v in trial.items()},
                 "passes_per_step": [r[0] for r in recs], "analyses_per_step": [r[1] for r in recs]}
     log(f"timed {args.steps} steps, {len(lanes)} in flight: {ms_per_step:.1f} ms/step, {value:,.0f} MB/s "
         f"(one in flight {value_one:,.0f}), with download {with_dl:,.0f} MB/s, schedule {schedule}")
@@ -882,7 +932,8 @@ def main() -> int:
 
     learned_out = None
     if not args.no_learned:
-        learned_out = learned_leg(lanes, jobs, plist, value, value_one, args.steps, args.warmup, mbs_per_step)
+        learned_out = learned_leg(pool, jobs, plist, value, value_one, args.steps, args.warmup, mbs_per_step,
+                                  in_flight=args.in_flight)
 
     scorer = None if args.no_scorer else scorer_leg(be, pics, frames)
 
@@ -893,7 +944,7 @@ def main() -> int:
 
     batched = None
     if batched_streams is not None:
-        batched = batched_leg(lanes, batched_streams, args.batched_scale, not args.no_check)
+        batched = batched_leg(pool, batched_streams, args.batched_scale, not args.no_check)
         batched_streams = None
 
     cpu = None

@@ -224,10 +224,12 @@ def _run_steps(backends, jobs, params, per_step: int, graphs: bool, schedule: st
 
 
 def autotune(backend: Backend, resolutions=RESOLUTIONS, candidates=STEP_CANDIDATES, frames: int = 5,
-             seed: int = 77, repeats: int = 2, flights=IN_FLIGHT_CANDIDATES) -> dict:
+             seed: int = 77, repeats: int = 6, flights=IN_FLIGHT_CANDIDATES) -> dict:
     """Measures MB searches/s per raster for every step size, launch mode
-    (MODES) and number of steps in flight on `frames`-frame synthetic clips;
-    returns the table (not yet saved).  Steps in flight run on extra
+    (MODES) and number of steps in flight on `frames`-frame synthetic clips,
+    `repeats` timed steps per context (enough that the first and last steps,
+    which overlap nothing, do not decide the number in flight); returns the
+    table (not yet saved).  Steps in flight run on extra
     contexts of the same device, created here and closed at the end."""
     extra = [Backend(backend.device) for _ in range(max(flights) - 1)]
     settings, trials = {}, {}
@@ -245,7 +247,11 @@ def autotune(backend: Backend, resolutions=RESOLUTIONS, candidates=STEP_CANDIDAT
                     for k in flights:
                         lanes = [backend] + extra[:k - 1]
                         steps = jobs * (repeats * k)
-                        _run_steps(lanes, jobs * k, params, b, graphs, schedule)     # warm up (and capture)
+                        # one step on every context (and its capture): dealt
+                        # through run_in_flight, a context could miss it and
+                        # capture inside the timed steps
+                        for ln in lanes:
+                            _run_steps(ln, jobs, params, b, graphs, schedule)
                         t0 = time.perf_counter()
                         _run_steps(lanes, steps, params, b, graphs, schedule)
                         rate = len(steps) * mbs / (time.perf_counter() - t0)
