@@ -1625,14 +1625,16 @@ __device__ __forceinline__ uint32_t sig_distance(uint64_t a, uint64_t b) {
 
 // Windows of the extended search: 36 x 36 frame samples around a candidate
 // (its 5x5 grid at step 4 and the 5x5 refinement around the grid's best stay
-// within +-10 px), rows of 64 bytes holding the 16-byte aligned chunks that
-// cover the window plus the 4-dword read slack of the row SAD.
+// within +-10 px), rows of 40 bytes: the dword-aligned columns that cover the
+// window (its first sample at byte 0..3) and the five-dword read of the row
+// SAD at an offset of up to 20 px.
 constexpr int kXwSide = 36;
-constexpr int kXwPitch = 64;
+constexpr int kXwPitch = 40;
 constexpr int kXwBytes = kXwSide * kXwPitch;
 // candidates staged per round: their windows and 25 keys each must fit the
-// shared LDS region, which is at least the phase window's size
-constexpr int kXwPerRound = 3;
+// shared LDS region, which is at least the phase window's size; five fill
+// the two wave passes of a round's 125 points
+constexpr int kXwPerRound = 5;
 constexpr int kXwRoundBytes = kXwPerRound * (kXwBytes + 25 * 8);   // windows, then 25 keys per candidate
 
 // The shared LDS region: the largest of its three uses.
@@ -1646,45 +1648,39 @@ static_assert(kXwPerRound <= 8 && kXwPerRound * 25 <= 4 * kWave, "one round's po
 // Stage the windows of candidates t0 .. t0+cnt-1 (offsets in `cand` lanes
 // t0..) into slots 0..cnt-1 of the shared LDS region.
 __device__ __forceinline__ void xw_load(Ctx& C, uint64_t ckeys, int t0, int cnt) {
-    const JobDesc* J = C.J;
     const int W = C.g.W, H = C.g.H;
     uint8_t* const xw = C.L.pw;
-    constexpr int kChunks = kXwPitch / 16;
+    constexpr int kDw = kXwPitch / 4;    // dwords per window row
     wsync();
     for (int k = 0; k < cnt; ++k) {
         const uint64_t ck = lane_read64(ckeys, t0 + k);
         const int wx = C.x0 + key_dx(ck) - 10, wy = C.y0 + key_dy(ck) - 10;
-        const int a0 = wx & ~15;
+        const int a0 = wx & ~3;
         const gptr_u8 ry = as_global(C.ry);
         uint8_t* const dst = xw + k * kXwBytes;
-        if (a0 >= 0 && a0 + kXwPitch <= W && (W & 15) == 0) {
+        if (a0 >= 0 && a0 + kXwPitch <= W && (W & 3) == 0) {
             typedef const __attribute__((address_space(1))) void* gvoid;
             typedef __attribute__((address_space(3))) void* lvoid;
 #pragma unroll
-            for (int base = 0; base < kXwSide * kChunks; base += kWave) {
+            for (int base = 0; base < kXwSide * kDw; base += kWave) {
                 const int i = base + C.lane;
-                const int r = i / kChunks, ch = i - r * kChunks;
+                const int r = i / kDw, d = i - r * kDw;
                 const int sy = clampi(wy + r, 0, H - 1);
-                // lanes past the window issue no copy (a copy lands at byte 16 * lane)
-                if (i < kXwSide * kChunks)
-                    __builtin_amdgcn_global_load_lds((gvoid)(ry + static_cast<size_t>(sy) * W + a0 + ch * 16),
-                                                     (lvoid)(dst + base * 16), 16, 0, 0);
+                // lanes past the window issue no copy (a copy lands at byte 4 * lane)
+                if (i < kXwSide * kDw)
+                    __builtin_amdgcn_global_load_lds((gvoid)(ry + static_cast<size_t>(sy) * W + a0 + d * 4),
+                                                     (lvoid)(dst + base * 4), 4, 0, 0);
             }
         } else {
-            for (int i = C.lane; i < kXwSide * kChunks; i += kWave) {
-                const int r = i / kChunks, ch = i - r * kChunks;
+            for (int i = C.lane; i < kXwSide * kDw; i += kWave) {
+                const int r = i / kDw, d = i - r * kDw;
                 const int sy = clampi(wy + r, 0, H - 1);
                 const gptr_u8 row = ry + static_cast<size_t>(sy) * W;
-                uint32_t wv[4];
+                uint32_t acc = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint32_t acc = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b)
-                        acc |= static_cast<uint32_t>(row[clampi(a0 + ch * 16 + q * 4 + b, 0, W - 1)]) << (8 * b);
-                    wv[q] = acc;
-                }
-                *reinterpret_cast<uint4*>(dst + r * kXwPitch + ch * 16) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+                for (int b = 0; b < 4; ++b)
+                    acc |= static_cast<uint32_t>(row[clampi(a0 + d * 4 + b, 0, W - 1)]) << (8 * b);
+                *reinterpret_cast<uint32_t*>(dst + r * kXwPitch + d * 4) = acc;
             }
         }
     }
@@ -1694,10 +1690,10 @@ __device__ __forceinline__ void xw_load(Ctx& C, uint64_t ckeys, int t0, int cnt)
 
 // SAD part of rows part, part+step, ... (sampled at `stride`) of the 16x16 block
 // at offset (dx,dy) read from an extended-search window whose sample (0,0) is
-// frame position (wx, wy) and whose first chunk starts at column wx - sh.
+// frame position (wx, wy) and whose first dword starts at column wx - sh.
 __device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win, int wx, int wy, int dx, int dy,
                                                 int stride, int part, int step) {
-    const int sh = wx & 15;
+    const int sh = wx & 3;
     const int ox = C.x0 + dx - wx + sh, oy = C.y0 + dy - wy;
     const int nrows = (C.ah + stride - 1) / stride;
     const uint32_t smask = stride == 1 ? 0xffffffffu : 0x00ff00ffu;
