@@ -280,6 +280,12 @@ __device__ __forceinline__ bool hidden_chroma(const Geo& g, int x0, int y0, int 
 }
 __device__ __forceinline__ bool hidden_padding(const Geo& g, int x0, int y0, int bw, int bh, int xq, int yq,
                                                int mode, bool chroma) {
+    // Early out: neither test can hold when the block, moved by the integer
+    // part of the vector plus the filter's right and bottom reach (at most 2),
+    // ends before the visible width and height; the chroma block's reach in
+    // chroma samples, cx + 8 + floor(uvx / 4), stays below ceil(vw / 2) under
+    // the same bound (floor(uvx / 4) <= (floor(xq / 4) + 1) / 2).
+    if (x0 + bw + floor_div4(xq) + 1 < g.vw && y0 + bh + floor_div4(yq) + 1 < g.vh) return false;
     return hidden_luma(g, x0, y0, bw, bh, xq, yq, mode) || (chroma && hidden_chroma(g, x0, y0, bw, bh, xq, yq));
 }
 __device__ __forceinline__ bool chroma_block(int x0, int y0, int bw, int bh) {
@@ -598,6 +604,12 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
     while (G * 2 * n <= kWave && G * 2 <= rows && G < 16) G *= 2;
     const int per = kWave / G;
     const int part = C.lane & (G - 1);
+    // A scored point's block lies inside the staged window; when the window
+    // ends before the visible width and height, no point can read hidden
+    // padding (the luma block's last column is at most the window's last,
+    // E < vw, and the chroma block's at most ceil((E - 1) / 2) < ceil(vw / 2);
+    // rows likewise), and the per-point test is skipped.
+    const bool may_hide = C.ww <= 0 || C.wx0 + C.ww > C.g.vw || C.wy0 + C.wh > C.g.vh;
     uint64_t best = kNone;        // with `top`: the least key, merged per batch
     uint64_t lane_best = kNone;   // without: this lane's least key
     for (int base = 0; base < n; base += per) {
@@ -618,7 +630,7 @@ __device__ __forceinline__ uint64_t int_batch(Ctx& C, int n, int stride, uint32_
         uint32_t sum = 0;
         bool hid = false;
         if (valid) {
-            hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
+            hid = may_hide && hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
             if (!hid) {
                 if (!in_window(C, dx, dy)) {
                     set_err(C, 2);
@@ -1871,7 +1883,12 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
                     dy = clampi(oy + (i / 5) - 2, b.ymin, b.ymax);
                 }
                 if (ok) {
-                    const bool hid = hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
+                    // every point of the candidate lies within +-10 px of it:
+                    // its block ends before column x0 + cdx + 26 and row
+                    // y0 + cdy + 26, inside the visible picture or not (the
+                    // bound of int_batch)
+                    const bool may_hide = C.x0 + cdx + 26 > C.g.vw || C.y0 + cdy + 26 > C.g.vh;
+                    const bool hid = may_hide && hidden_padding(C.g, C.x0, C.y0, 16, 16, dx * 4, dy * 4, 0, true);
                     uint64_t cost = kBadSad;
                     if (!hid) {
                         const int stride = stage ? 1 : 2;
