@@ -224,6 +224,60 @@ def test_hidden_padding_truth_table():
     assert not hp(C.byref(odd), 64, 48, 16, 16, 0, 0, 0, 1)
 
 
+def test_hidden_padding_skip_rules_never_skip_a_hidden_vector():
+    """The GPU kernels skip the hidden-padding test where it cannot hold:
+    (1) for a block at (x0, y0) of size bw x bh and a quarter-pel vector
+    (xq, yq), when x0 + bw + floor(xq / 4) + 1 < visible width and likewise
+    for rows; (2) for every integer vector of a batch whose blocks lie inside
+    a window ending before the visible width and height.  Against the
+    oracle's motion_uses_hidden_padding on random rasters (display sizes
+    below the coded ones), blocks, modes and vectors, and exhaustively at
+    the bottom-right edges: a skipped vector is never a hidden one."""
+    hp = O.oracle().oracle_hidden_padding
+    rng = np.random.default_rng(2024)
+    rasters = [(1920, 1080, 0, 0), (176, 144, 170, 139), (48, 40, 47, 39), (1280, 720, 1279, 717),
+               (160, 120, 0, 0), (66, 50, 65, 49), (352, 288, 0, 0)]
+    checked = 0
+    for (w, h, dw, dh) in rasters:
+        p = abi.default_params(w, h, display_width=dw, display_height=dh)
+        vw, vh = dw or w, dh or h
+        mbw, mbh = (w + 15) // 16, (h + 15) // 16
+        # rule (1): random blocks and vectors, and the edge blocks exhaustively
+        for _ in range(3000):
+            bw = int(rng.choice([16, 8]))
+            x0 = int(rng.integers(0, mbw)) * 16 + (int(rng.choice([0, 8])) if bw == 8 else 0)
+            y0 = int(rng.integers(0, mbh)) * 16 + (int(rng.choice([0, 8])) if bw == 8 else 0)
+            span = int(rng.choice([8, 40, 400]))
+            xq, yq = (int(v) for v in rng.integers(-4 * span, 4 * span + 1, 2))
+            mode = int(rng.choice([0, 1, 2, 3]))
+            chroma = int(bw == 16)
+            if x0 + bw + (xq >> 2) + 1 < vw and y0 + bw + (yq >> 2) + 1 < vh:
+                checked += 1
+                assert not hp(C.byref(p), x0, y0, bw, bw, xq, yq, mode, chroma), (w, h, x0, y0, bw, xq, yq, mode)
+        for bw in (16, 8):
+            for x0 in range(max(0, mbw * 16 - 32), mbw * 16, bw):
+                for y0 in range(max(0, mbh * 16 - 32), mbh * 16, bw):
+                    for xq in range(-24, 25):
+                        for yq in range(-24, 25, 5):
+                            if x0 + bw + (xq >> 2) + 1 < vw and y0 + bw + (yq >> 2) + 1 < vh:
+                                for mode in (0, 2):
+                                    checked += 1
+                                    assert not hp(C.byref(p), x0, y0, bw, bw, xq, yq, mode, int(bw == 16))
+        # rule (2): integer vectors of 16x16 blocks inside a window ending at
+        # column ex < vw and row ey < vh
+        for _ in range(60):
+            mx, my = int(rng.integers(0, mbw)), int(rng.integers(0, mbh))
+            x0, y0 = mx * 16, my * 16
+            if x0 + 15 >= vw or y0 + 15 >= vh:
+                continue
+            ex = int(rng.integers(x0 + 15, vw)); ey = int(rng.integers(y0 + 15, vh))
+            for dx in range(ex - 15 - x0 - 20, ex - 15 - x0 + 1):
+                for dy in range(ey - 15 - y0 - 20, ey - 15 - y0 + 1, 4):
+                    checked += 1
+                    assert not hp(C.byref(p), x0, y0, 16, 16, 4 * dx, 4 * dy, 0, 1), (w, h, x0, y0, dx, dy)
+    assert checked > 50000
+
+
 def test_hidden_padding_final_vectors_never_read_padding():
     # tests/hidden_reference_guard.cpp:59-78: final P vectors stay out of the padding
     z = np.load(os.path.join(GOLDEN, "hidden_160x120_mixed.npz"), allow_pickle=False)
