@@ -1613,7 +1613,9 @@ __device__ __forceinline__ uint64_t cur_signature(Ctx& C) {
         if (b < 4) { px = C.x0 + (b & 1) * 8 + lx; py = C.y0 + (b >> 1) * 8 + ly; pw = W; ph = H; p = J->cy; }
         else { px = C.x0 / 2 + lx; py = C.y0 / 2 + ly; pw = cw; ph = ch; p = b == 4 ? J->cu : J->cv; }
         const bool in = px < pw && py < ph && px >= 0 && py >= 0;
-        v[b] = wave_sum32(in ? p[static_cast<size_t>(py) * pw + px] : 0u);
+        // luma from the current block in LDS (every in-frame pixel of the macroblock)
+        const uint32_t pix = !in ? 0u : b < 4 ? C.L.cur[(py - C.y0) * 16 + (px - C.x0)] : p[static_cast<size_t>(py) * pw + px];
+        v[b] = wave_sum32(pix);
         n[b] = wave_sum32(in ? 1u : 0u);
     }
     uint64_t s = 0;
@@ -2427,11 +2429,19 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
     // below; the wsync after them waits for all)
     if (range > 0 && !J->seed_mvs && J->lr_mode != 2)
         win_issue(C, C.x0 - local_range, C.y0 - local_range, 16 + 2 * local_range, 16 + 2 * local_range);
-    // current block to LDS, zero outside the active area; the neighbours'
-    // state snapshot
-    for (int i = C.lane; i < 256; i += kWave) {
-        const int x = i & 15, y = i >> 4;
-        C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
+    // current block to LDS, zero outside the active area (a whole block:
+    // one direct global-to-LDS dword copy, lane i the dword of row i / 4 at
+    // column 4 (i % 4)); the neighbours' state snapshot
+    if (C.aw == 16 && C.ah == 16 && (C.g.W & 3) == 0) {
+        typedef const __attribute__((address_space(1))) void* gvoid;
+        typedef __attribute__((address_space(3))) void* lvoid;
+        const gptr_u8 src = as_global(J->cy) + static_cast<size_t>(C.y0 + (C.lane >> 2)) * C.g.W + C.x0 + (C.lane & 3) * 4;
+        __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)C.L.cur, 4, 0, 0);
+    } else {
+        for (int i = C.lane; i < 256; i += kWave) {
+            const int x = i & 15, y = i >> 4;
+            C.L.cur[i] = (x < C.aw && y < C.ah) ? J->cy[static_cast<size_t>(C.y0 + y) * C.g.W + C.x0 + x] : 0;
+        }
     }
     nb_load(C);
     wsync();
@@ -2694,7 +2704,8 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         for (int k = 0; k < 4; ++k) {
             const int bx = C.x0 + (k & 1) * 8, by = C.y0 + ((k >> 1) & 1) * 8;
             const bool in = bx + lx < C.g.W && by + ly < C.g.H;
-            const uint32_t v = in ? J->cy[static_cast<size_t>(by + ly) * C.g.W + bx + lx] : 0u;
+            // the current block in LDS holds every in-frame pixel of the macroblock
+            const uint32_t v = in ? C.L.cur[(by - C.y0 + ly) * 16 + (bx - C.x0 + lx)] : 0u;
             const uint32_t sum = wave_sum32(v);
             const uint32_t n = wave_sum32(in ? 1u : 0u);
             const int mean = n ? static_cast<int>((sum + n / 2) / n) : 128;
