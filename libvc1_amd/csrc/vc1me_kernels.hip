@@ -350,6 +350,7 @@ struct Lds {
     uint32_t* seen_f;    // full-SAD dedupe bitset
     int2* own;           // current MB's four block vectors
     uint32_t* nb;        // neighbour snapshot: kNbWords dwords each of left, top, top-left, top-right
+    uint32_t* cs;        // sampled current rows: stride 2 (8 rows x 2 dwords), then stride 4 (4 rows x 1 dword)
 };
 
 // Neighbour snapshot per neighbour: 1-MV vector (2 dwords), the four block
@@ -368,8 +369,8 @@ __host__ __device__ inline int int_scratch_bytes(int local_range) {
 __host__ __device__ inline int shared_bytes(int local_range);
 
 __host__ __device__ inline int lds_bytes_for(int local_range) {
-    // cur, shared region, own, neighbour snapshot, slack
-    return 256 + shared_bytes(local_range) + 4 * 8 + 4 * 12 * 4 + 64;
+    // cur, shared region, own, neighbour snapshot, sampled rows, slack
+    return 256 + shared_bytes(local_range) + 4 * 8 + 4 * 12 * 4 + 20 * 4 + 64;
 }
 
 __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
@@ -384,6 +385,7 @@ __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     const int shared = shared_bytes(local_range);
     L.own = reinterpret_cast<int2*>(region + shared);
     L.nb = reinterpret_cast<uint32_t*>(region + shared + 4 * 8);
+    L.cs = reinterpret_cast<uint32_t*>(region + shared + 4 * 8 + 4 * 12 * 4);
     return L;
 }
 
@@ -525,35 +527,83 @@ __device__ __forceinline__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int w
 // Partial SAD of sampled rows part, part+G, ... of the 16x16 block at integer
 // offset (dx,dy).  Rows at or beyond the active height and columns at or
 // beyond the active width contribute nothing (mb_sad, src/encoder_motion.cpp:2274-2282).
+// The sampled current rows of the stride-2 and stride-4 SADs, packed once per
+// analysis from the current block (zero outside the active area): dword
+// 2 r + h of stride 2 holds row 2 r, columns 8 h + 0, 2, 4, 6; dword 16 + r of
+// stride 4 holds row 4 r, columns 0, 4, 8, 12.
+__device__ __forceinline__ void pack_sampled_rows(Ctx& C) {
+    const int l = C.lane;
+    if (l < 20) {
+        const uint8_t* row = C.L.cur + (l < 16 ? (l >> 1) * 2 * 16 + (l & 1) * 8 : (l - 16) * 4 * 16);
+        const int step = l < 16 ? 2 : 4;
+        C.L.cs[l] = static_cast<uint32_t>(row[0]) | (static_cast<uint32_t>(row[step]) << 8) |
+                    (static_cast<uint32_t>(row[2 * step]) << 16) | (static_cast<uint32_t>(row[3 * step]) << 24);
+    }
+}
+
+// Byte mask of the packed samples at columns c0, c0 + step, c0 + 2 step,
+// c0 + 3 step that lie inside the active width.
+__device__ __forceinline__ uint32_t sample_mask(int aw, int c0, int step) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (c0 + k * step < aw) m |= 0xffu << (8 * k);
+    return m;
+}
+
+// SAD of the block at window offset (dx, dy) over rows part, part + G, ...
+// of the sampled rows (stride 1, 2 or 4 in both axes).  Stride 1 reads four
+// window dwords per row against the current row; strides 2 and 4 pack the
+// sampled window bytes of a row (alignbyte then v_perm, or v_perm straight
+// from the window dwords) and compare them with the packed current rows.
+// The current block is zero outside the active area, so only the window
+// side is masked (and only when the block is not whole).
 __device__ __forceinline__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, int stride, int part, int G) {
     const int ox = C.x0 + dx - C.wx0, oy = C.y0 + dy - C.wy0;
     const int nrows = (C.ah + stride - 1) / stride;
-    const uint32_t smask = stride == 1 ? 0xffffffffu : (stride == 2 ? 0x00ff00ffu : 0x000000ffu);
-    uint32_t m[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        uint32_t cm = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            if (j * 4 + b < C.aw) cm |= 0xffu << (8 * b);
-        m[j] = cm & smask;
-    }
-    if (stride == 4) { m[0] &= 0xff; m[1] &= 0xff; m[2] &= 0xff; m[3] &= 0xff; }
     const int base = ox & ~3, sh = ox & 3;
+    const bool whole = C.aw == 16;
     uint32_t acc = 0;
-    for (int r = part; r < nrows; r += G) {
-        const int y = r * stride;
-        const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + y) * C.wp + base);
-        const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + y * 16);
-        const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
-        const uint32_t r0 = __builtin_amdgcn_alignbyte(d1, d0, sh);
-        const uint32_t r1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
-        const uint32_t r2 = __builtin_amdgcn_alignbyte(d3, d2, sh);
-        const uint32_t r3 = __builtin_amdgcn_alignbyte(d4, d3, sh);
-        acc = __builtin_amdgcn_sad_u8(r0 & m[0], cr[0] & m[0], acc);
-        acc = __builtin_amdgcn_sad_u8(r1 & m[1], cr[1] & m[1], acc);
-        acc = __builtin_amdgcn_sad_u8(r2 & m[2], cr[2] & m[2], acc);
-        acc = __builtin_amdgcn_sad_u8(r3 & m[3], cr[3] & m[3], acc);
+    if (stride == 1) {
+        uint32_t m[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m[j] = sample_mask(C.aw, 4 * j, 1);
+        for (int r = part; r < nrows; r += G) {
+            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + r) * C.wp + base);
+            const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + r * 16);
+            const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
+            uint32_t r0 = __builtin_amdgcn_alignbyte(d1, d0, sh), r1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
+            uint32_t r2 = __builtin_amdgcn_alignbyte(d3, d2, sh), r3 = __builtin_amdgcn_alignbyte(d4, d3, sh);
+            if (!whole) { r0 &= m[0]; r1 &= m[1]; r2 &= m[2]; r3 &= m[3]; }
+            acc = __builtin_amdgcn_sad_u8(r0, cr[0], acc);
+            acc = __builtin_amdgcn_sad_u8(r1, cr[1], acc);
+            acc = __builtin_amdgcn_sad_u8(r2, cr[2], acc);
+            acc = __builtin_amdgcn_sad_u8(r3, cr[3], acc);
+        }
+    } else if (stride == 2) {
+        const uint32_t pm0 = sample_mask(C.aw, 0, 2), pm1 = sample_mask(C.aw, 8, 2);
+        for (int r = part; r < nrows; r += G) {
+            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + 2 * r) * C.wp + base);
+            const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
+            const uint32_t r0 = __builtin_amdgcn_alignbyte(d1, d0, sh), r1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
+            const uint32_t r2 = __builtin_amdgcn_alignbyte(d3, d2, sh), r3 = __builtin_amdgcn_alignbyte(d4, d3, sh);
+            // bytes 0 and 2 of each: columns 0, 2, 4, 6 and 8, 10, 12, 14
+            const uint32_t e0 = __builtin_amdgcn_perm(r1, r0, 0x06040200u) & pm0;
+            const uint32_t e1 = __builtin_amdgcn_perm(r3, r2, 0x06040200u) & pm1;
+            acc = __builtin_amdgcn_sad_u8(e0, C.L.cs[2 * r], acc);
+            acc = __builtin_amdgcn_sad_u8(e1, C.L.cs[2 * r + 1], acc);
+        }
+    } else {
+        // byte sh of each of the four window dwords: columns 0, 4, 8, 12
+        const uint32_t lo_sel = static_cast<uint32_t>(sh) | (static_cast<uint32_t>(sh + 4) << 8) | 0x0c0c0000u;
+        const uint32_t hi_sel = 0x00000c0cu | (static_cast<uint32_t>(sh) << 16) | (static_cast<uint32_t>(sh + 4) << 24);
+        const uint32_t pm = sample_mask(C.aw, 0, 4);
+        for (int r = part; r < nrows; r += G) {
+            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + 4 * r) * C.wp + base);
+            const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3];
+            const uint32_t e = (__builtin_amdgcn_perm(d1, d0, lo_sel) | __builtin_amdgcn_perm(d3, d2, hi_sel)) & pm;
+            acc = __builtin_amdgcn_sad_u8(e, C.L.cs[16 + r], acc);
+        }
     }
     return acc;
 }
@@ -2444,6 +2494,8 @@ __device__ __forceinline__ void analyze_mb(Ctx& C, NbState& st) {
         }
     }
     nb_load(C);
+    wsync();
+    pack_sampled_rows(C);
     wsync();
 
     uint64_t one_sad = 0;
