@@ -172,13 +172,36 @@ __device__ __forceinline__ void win_step(WinKey& k) {
     else less = p.b < k.b;
     if (less) k = p;
 }
-// The least entry of lanes 0..7, in each of those lanes: a tournament over
-// lane pairs, quads and the two halves of the eight (every lane active).
+// One tournament step on the leading fields (cls, s) only.
+template <int CTRL>
+__device__ __forceinline__ void win_step_cs(uint32_t& cls, unsigned __int128& s) {
+    const uint32_t pc = dpp_mov<CTRL, 0xf>(cls, cls);
+    const unsigned __int128 ps = (static_cast<unsigned __int128>(dpp64<CTRL>(static_cast<uint64_t>(s >> 64))) << 64) |
+                                 dpp64<CTRL>(static_cast<uint64_t>(s));
+    const bool less = pc != cls ? pc < cls : ps < s;
+    if (less) { cls = pc; s = ps; }
+}
+// The least entry of lanes 0..7 (every lane active), in lane 0: a tournament
+// over lane pairs, quads and the two halves of the eight on (cls, s); the
+// entry's other fields come from the one lane holding that (cls, s), and
+// only a tie on it runs the full tournament among the tying lanes.
 __device__ __forceinline__ WinKey lanes8_min(WinKey k) {
-    win_step<kDppXor1>(k);
-    win_step<kDppXor2>(k);
-    win_step<kDppHalfMirror>(k);
-    return k;
+    uint32_t cls = k.cls;
+    unsigned __int128 s = k.s;
+    win_step_cs<kDppXor1>(cls, s);
+    win_step_cs<kDppXor2>(cls, s);
+    win_step_cs<kDppHalfMirror>(cls, s);
+    const int lane = static_cast<int>(threadIdx.x & (kWave - 1));
+    const uint64_t tied = __ballot(lane < 8 && k.cls == cls && k.s == s) & 0xffull;
+    if (__popcll(tied) == 1) {
+        const int l = __ffsll(static_cast<long long>(tied)) - 1;
+        return WinKey{cls, s, lane_read64(k.a, l), lane_read64(k.b, l)};
+    }
+    WinKey t = ((tied >> lane) & 1ull) ? k : WinKey{~0u, ~static_cast<unsigned __int128>(0), ~0ull, ~0ull};
+    win_step<kDppXor1>(t);
+    win_step<kDppXor2>(t);
+    win_step<kDppHalfMirror>(t);
+    return t;
 }
 
 // Global-memory view of a plane pointer (lets loads be scheduled past LDS stores).
@@ -942,10 +965,16 @@ __device__ __forceinline__ uint32_t mv_rate_bits(int mvrange, int xq, int yq, in
 }
 // modular_mv_distance, src/encoder_motion.cpp:1299-1310
 __device__ __forceinline__ int mod_dist(int a, int b, int range) {
-    int d = a - b;
+    // ((d + range) % span + span) % span is the residue of t = d + range in
+    // [0, span): t itself when it already lies there (the vectors of one
+    // picture differ by less than the span), the general form otherwise
     const int span = range * 2;
-    d = ((d + range) % span + span) % span - range;
-    return iabs(d);
+    int t = a - b + range;
+    if (t < 0 || t >= span) {
+        t %= span;
+        if (t < 0) t += span;
+    }
+    return iabs(t - range);
 }
 __device__ __forceinline__ int modular_mv_distance(int mvrange, int ax, int ay, int bx, int by) {
     return mod_dist(ax, bx, mvrange_xq(mvrange)) + mod_dist(ay, by, mvrange_yq(mvrange));
