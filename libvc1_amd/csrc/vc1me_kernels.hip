@@ -1238,6 +1238,19 @@ struct IRes {
     uint64_t sad;
 };
 
+// Division by a divisor fixed for a batch: q = floor(i / d) as the high word
+// of i * ceil(2^32 / d), exact whenever i * d < 2^32 (the grid indices here
+// stay below 2^17 and the divisors below 2^11).
+struct FastDiv {
+    uint32_t d, m;
+};
+__device__ __forceinline__ FastDiv fast_div(int d) {
+    return {static_cast<uint32_t>(d), d > 1 ? static_cast<uint32_t>((0x100000000ull + d - 1) / d) : 0u};
+}
+__device__ __forceinline__ int fdiv(int i, FastDiv f) {
+    return f.d > 1 ? static_cast<int>(__umulhi(static_cast<uint32_t>(i), f.m)) : i;
+}
+
 // integer_motion_umh, src/encoder_motion.cpp:1781-2023.  The reference keeps
 // a running best and a top-8 list; with distinct points both are functions of
 // the set of points considered so far, so each stage scores its points
@@ -1284,9 +1297,13 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
     {
         const int cx = key_dx(best), cy = key_dy(best);
         const int side = R >= 8 ? (2 * R) / 8 + 1 : 0;
+        const FastDiv fs = fast_div(side > 0 ? side : 1);
         stage(25 + side * side + 2 * side, [=](int i, int& dx, int& dy) VC1ME_INLINE {
             if (i < 25) { dx = cx + (i % 5) - 2; dy = cy + (i / 5) - 2; }
-            else if (i < 25 + side * side) { dx = -R + 8 * ((i - 25) % side); dy = -R + 8 * ((i - 25) / side); }
+            else if (i < 25 + side * side) {
+                const int q = fdiv(i - 25, fs);
+                dx = -R + 8 * (i - 25 - q * side); dy = -R + 8 * q;
+            }
             else {
                 const int j = i - 25 - side * side, d = -R + 8 * (j >> 1);
                 if (j & 1) { dx = d; dy = R; } else { dx = R; dy = d; }
@@ -1370,9 +1387,10 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
     if (R > 0 && !match_good(C, key_cost(full))) {
         const int step = R >= 8 ? 4 : (R >= 3 ? 2 : 1);
         const int side = (2 * R) / step + 1;
+        const FastDiv fs = fast_div(side);
         const uint64_t coarse = int_batch(C, side * side + 2 * side, 4, nullptr, KEY_TOTAL, 0,
                                           [=](int i, int& dx, int& dy) VC1ME_INLINE {
-            if (i < side * side) { dx = -R + step * (i % side); dy = -R + step * (i / side); }
+            if (i < side * side) { const int q = fdiv(i, fs); dx = -R + step * (i - q * side); dy = -R + step * q; }
             else {
                 const int j = i - side * side, d = -R + step * (j >> 1);
                 if (j & 1) { dx = d; dy = R; } else { dx = R; dy = d; }
@@ -1382,21 +1400,27 @@ __device__ __forceinline__ IRes umh_search(Ctx& C, const Pred pi, int gdx, int g
         const int rr = imax(1, step - 1);
         const int xa = imax(-R, cdx - rr), xb = imin(R, cdx + rr), ya = imax(-R, cdy - rr), yb = imin(R, cdy + rr);
         const int rw = xb - xa + 1, rh = yb - ya + 1;
+        const FastDiv frw = fast_div(rw);
         const uint64_t rescue = int_batch(C, rw * rh, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
-            dx = xa + i % rw; dy = ya + i / rw;
+            const int q = fdiv(i, frw);
+            dx = xa + i - q * rw; dy = ya + q;
         });
         if (rescue < full) full = rescue;
         const uint64_t px = active_pixels(C);
         if (px && key_cost(full) > px * 24u && R <= 32) {
             const int ds = (2 * R) / 2 + 1;
+            const FastDiv fds = fast_div(ds);
             const uint64_t dense = int_batch(C, ds * ds, 2, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
-                dx = -R + 2 * (i % ds); dy = -R + 2 * (i / ds);
+                const int q = fdiv(i, fds);
+                dx = -R + 2 * (i - q * ds); dy = -R + 2 * q;
             });
             const int ddx = key_dx(dense), ddy = key_dy(dense);
             const int xa2 = imax(-R, ddx - 2), xb2 = imin(R, ddx + 2), ya2 = imax(-R, ddy - 2), yb2 = imin(R, ddy + 2);
             const int w2 = xb2 - xa2 + 1, h2 = yb2 - ya2 + 1;
+            const FastDiv fw2 = fast_div(w2);
             const uint64_t dfull = int_batch(C, w2 * h2, 1, C.L.seen_f, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
-                dx = xa2 + i % w2; dy = ya2 + i / w2;
+                const int q = fdiv(i, fw2);
+                dx = xa2 + i - q * w2; dy = ya2 + q;
             });
             if (dfull < full) full = dfull;
         }
@@ -2387,7 +2411,11 @@ __device__ __forceinline__ int2 refine_staged(Ctx& C, const SubParams& sp, int s
     if (J->quality == 2 && !sp.cheap) {
         const int rw = __ffsll(static_cast<long long>(__ballot(in && xq == result.x && yq == result.y))) - 1;
         const uint64_t rws = lane_read64(sad, rw);
-        const uint64_t pct3 = (rws / 100) * 3 + (((rws % 100) * 3 + 99) / 100);
+        // (rws / 100) * 3 + ceil((rws % 100) * 3 / 100) = ceil(3 rws / 100),
+        // in 32 bits for the SADs that fit
+        const uint64_t pct3 = rws < (1ull << 30)
+                                  ? static_cast<uint64_t>((static_cast<uint32_t>(rws) * 3u + 99u) / 100u)
+                                  : (rws / 100) * 3 + (((rws % 100) * 3 + 99) / 100);
         const uint64_t slack = pct3 > 4 ? pct3 : 4;
         const uint64_t limit = rws > UINT64_MAX - slack ? UINT64_MAX : rws + slack;
         const bool el = in && sad < UINT64_MAX / 8 && sad <= limit;
