@@ -374,6 +374,7 @@ struct Lds {
     int2* own;           // current MB's four block vectors
     uint32_t* nb;        // neighbour snapshot: kNbWords dwords each of left, top, top-left, top-right
     uint32_t* cs;        // sampled current rows: stride 2 (8 rows x 2 dwords), then stride 4 (4 rows x 1 dword)
+    int* sd;             // a short seed list read by lane index (8 ints)
 };
 
 // Neighbour snapshot per neighbour: 1-MV vector (2 dwords), the four block
@@ -392,8 +393,8 @@ __host__ __device__ inline int int_scratch_bytes(int local_range) {
 __host__ __device__ inline int shared_bytes(int local_range);
 
 __host__ __device__ inline int lds_bytes_for(int local_range) {
-    // cur, shared region, own, neighbour snapshot, sampled rows, slack
-    return 256 + shared_bytes(local_range) + 4 * 8 + 4 * 12 * 4 + 20 * 4 + 64;
+    // cur, shared region, own, neighbour snapshot, sampled rows, seed list, slack
+    return 256 + shared_bytes(local_range) + 4 * 8 + 4 * 12 * 4 + 20 * 4 + 8 * 4 + 64;
 }
 
 __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
@@ -409,6 +410,7 @@ __device__ __forceinline__ Lds carve(uint8_t* smem, int local_range) {
     L.own = reinterpret_cast<int2*>(region + shared);
     L.nb = reinterpret_cast<uint32_t*>(region + shared + 4 * 8);
     L.cs = reinterpret_cast<uint32_t*>(region + shared + 4 * 8 + 4 * 12 * 4);
+    L.sd = reinterpret_cast<int*>(region + shared + 4 * 8 + 4 * 12 * 4 + 20 * 4);
     return L;
 }
 
@@ -1455,25 +1457,22 @@ __device__ __forceinline__ IRes diamond_search(Ctx& C, const Pred pi, int gdx, i
         keep = (k1 ? 1u : 0u) | (k2 ? 2u : 0u) | (k3 ? 4u : 0u) | (k4 ? 8u : 0u);
     }
     const int sn = 1 + __builtin_popcount(keep);
-    // kept seeds, compacted in order
-    int s1x = 0, s1y = 0, s2x = 0, s2y = 0, s3x = 0, s3y = 0, s4x = 0, s4y = 0;
-    {
-        unsigned m = keep;
-        auto take = [=](unsigned mm, int& x, int& y) VC1ME_INLINE {
-            const int k = __builtin_ctz(mm);
-            x = k == 0 ? p1x : k == 1 ? p2x : k == 2 ? p3x : p4x;
-            y = k == 0 ? p1y : k == 1 ? p2y : k == 2 ? p3y : p4y;
-        };
-        if (m) { take(m, s1x, s1y); m &= m - 1; }
-        if (m) { take(m, s2x, s2y); m &= m - 1; }
-        if (m) { take(m, s3x, s3y); m &= m - 1; }
-        if (m) { take(m, s4x, s4y); }
-    }
+    // the kept seeds, compacted in order into the LDS seed list and read
+    // there by lane index (held in registers and picked by a lane's index,
+    // the list went to scratch memory)
     if (sn > 1) {
+        if (C.lane == 0) {
+            int n = 0;
+            if (keep & 1u) { C.L.sd[2 * n] = p1x; C.L.sd[2 * n + 1] = p1y; ++n; }
+            if (keep & 2u) { C.L.sd[2 * n] = p2x; C.L.sd[2 * n + 1] = p2y; ++n; }
+            if (keep & 4u) { C.L.sd[2 * n] = p3x; C.L.sd[2 * n + 1] = p3y; ++n; }
+            if (keep & 8u) { C.L.sd[2 * n] = p4x; C.L.sd[2 * n + 1] = p4y; }
+        }
+        wsync();
+        const int* sd = C.L.sd;
         const uint64_t s = int_batch(C, sn - 1, 1, nullptr, KEY_TOTAL, 0, [=](int i, int& dx, int& dy) VC1ME_INLINE {
-            const int j = i + 1;
-            dx = j == 1 ? s1x : j == 2 ? s2x : j == 3 ? s3x : s4x;
-            dy = j == 1 ? s1y : j == 2 ? s2y : j == 3 ? s3y : s4y;
+            dx = sd[2 * i];
+            dy = sd[2 * i + 1];
         });
         if (s < best) best = s;
     }
@@ -3043,7 +3042,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_M
 __device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int pos, int next_pass, RelaxQueue Q,
                                                  uint32_t mask) {
     const int mbw = J->mbw, mbh = J->mbh;
-    int32_t* next = Q.wl[next_pass & 1];
+    int32_t* next = (next_pass & 1) ? Q.wl[1] : Q.wl[0];     // a select: indexed, the pair went to scratch
     const int mx = pos % mbw, my = pos / mbw;
     const int dxs[4] = {1, -1, 0, 1}, dys[4] = {0, 1, 1, 1};
     for (int k = 0; k < 4; ++k) {
@@ -3086,7 +3085,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_M
     extern __shared__ __align__(16) uint8_t smem[];
     const int n = imin(Q.cnt[pass], Q.cap);
     if (n == 0) return;
-    const int32_t* list = Q.wl[pass & 1];
+    const int32_t* list = (pass & 1) ? Q.wl[1] : Q.wl[0];
     // one wave per workgroup; the barrier at the end of each entry keeps the
     // compiler from moving one macroblock's LDS accesses past the next one's
     for (int t = __builtin_amdgcn_readfirstlane(threadIdx.x == 0 ? atomicAdd(&Q.tk[pass], 1) : 0); t < n;) {
