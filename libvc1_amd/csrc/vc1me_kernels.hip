@@ -1876,26 +1876,35 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
         // loads without branches (a lane past the rectangle reads its first
         // cell and keys nothing), so that the wait before keying a block
         // leaves the next block's loads in flight
+        // (cell rows and columns, the lattice width and the rectangle's size
+        // are below 2^24: row * width is a 24-bit product and the cell's
+        // index a 32-bit offset; the lattice's top two signature bytes are
+        // zero, so the chroma SAD needs no mask)
+        const uint32_t uncx = static_cast<uint32_t>(ncx);
         auto fetch = [&](uint64_t (&fv)[kU], int (&fr)[kU], int (&fc)[kU], int base) VC1ME_INLINE {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 fr[u] = r;
                 fc[u] = c;
-                const bool in = base + C.lane + u * kWave < total;
-                fv[u] = sig[in ? static_cast<size_t>(r) * ncx + c : 0];
+                const bool in = C.lane < total - base - u * kWave;
+                const uint32_t at = __umul24(static_cast<uint32_t>(r), uncx) + static_cast<uint32_t>(c);
+                fv[u] = sig[in ? at : 0u];
                 c += dc;
                 r += dr;
                 if (c >= nxc) { c -= nxc; ++r; }
             }
         };
+        const uint32_t want_lo = static_cast<uint32_t>(want), want_hi = static_cast<uint32_t>(want >> 32);
+        uint32_t thr_d = 0xffffffffu >> 2;   // the distance field of thr (uniform)
         auto key_block = [&](const uint64_t (&v)[kU], const int (&vr)[kU], const int (&vc)[kU], int base) VC1ME_INLINE {
 #pragma unroll
             for (int u = 0; u < kU; ++u) {
                 // the distance is the key's most significant field: a cell
                 // farther than the 16th best cannot enter the list, so the
                 // whole key is built only when some lane's cell might
-                const uint32_t dist = sig_distance(want, v[u]);
-                const bool near = base + C.lane + u * kWave < total && dist <= static_cast<uint32_t>(thr >> 34);
+                const uint32_t dist = 2u * __builtin_amdgcn_sad_u8(want_lo, static_cast<uint32_t>(v[u]), 0u) +
+                                      3u * __builtin_amdgcn_sad_u8(want_hi, static_cast<uint32_t>(v[u] >> 32), 0u);
+                const bool near = C.lane < total - base - u * kWave && dist <= thr_d;
                 if (!__ballot(near)) continue;
                 const int dx = (cx_lo + vc[u]) * 8 - C.x0, dy = (cy_lo + vr[u]) * 8 - C.y0;
                 const bool ok = near && !(iabs(dy) <= 32 && iabs(dx) <= 32);
@@ -1914,6 +1923,7 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
                     const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_ds_permute(to, static_cast<int>(k >> 32)));
                     top = C.lane < imin(nk, 16) ? (static_cast<uint64_t>(hi) << 32) | lo : kNone;
                     thr = lane_read64(top, 15);
+                    thr_d = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(thr >> 34)));
                     continue;
                 }
                 while (hits) {
@@ -1927,6 +1937,7 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
                     if (C.lane < 16 && C.lane > pos) top = up;
                     if (C.lane == pos) top = kk;
                     thr = lane_read64(top, 15);
+                    thr_d = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(thr >> 34)));
                 }
             }
         };
