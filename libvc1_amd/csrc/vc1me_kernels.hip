@@ -475,8 +475,9 @@ __device__ __forceinline__ bool win_issue(Ctx& C, int wx0, int wy0, int ww, int 
     int r = C.lane / cpr, c = C.lane - (C.lane / cpr) * cpr;
     for (int base = 0; base < total; base += kWave) {
         const int sy = clampi(wy0 + r, 0, H - 1);
-        const gptr_u8 src = ry + static_cast<size_t>(sy) * W + ax0 + c * 4;
-        if (base + C.lane < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(w4 + base), 4, 0, 0);
+        // (rows and the picture width are below 2^24: a 24-bit product)
+        const gptr_u8 src = ry + (__umul24(static_cast<uint32_t>(sy), static_cast<uint32_t>(W)) + static_cast<uint32_t>(ax0 + c * 4));
+        if (C.lane < total - base) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(w4 + base), 4, 0, 0);
         c += dc;
         r += dr;
         if (c >= cpr) { c -= cpr; ++r; }
@@ -516,8 +517,8 @@ __device__ __forceinline__ bool win_load(Ctx& C, int wx0, int wy0, int ww, int w
         int r = C.lane / cpr, c = C.lane - (C.lane / cpr) * cpr;
         for (int base = 0; base < total; base += kWave) {
             const int sy = clampi(wy0 + r, 0, H - 1);
-            const gptr_u8 src = ry + static_cast<size_t>(sy) * W + ax0 + c * 4;
-            if (base + C.lane < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(w4 + base), 4, 0, 0);
+            const gptr_u8 src = ry + (__umul24(static_cast<uint32_t>(sy), static_cast<uint32_t>(W)) + static_cast<uint32_t>(ax0 + c * 4));
+            if (C.lane < total - base) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(w4 + base), 4, 0, 0);
             c += dc;
             r += dr;
             if (c >= cpr) { c -= cpr; ++r; }
@@ -587,6 +588,10 @@ __device__ __forceinline__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, i
     const int ox = C.x0 + dx - C.wx0, oy = C.y0 + dy - C.wy0;
     const int nrows = (C.ah + stride - 1) / stride;
     const int base = ox & ~3, sh = ox & 3;
+    // (the point lies inside the window: its row and column offsets are
+    // non-negative, and with the pitch below 2^24 a row's offset is a 24-bit
+    // product)
+    const uint32_t uwp = static_cast<uint32_t>(C.wp), ubase = static_cast<uint32_t>(base);
     const bool whole = C.aw == 16;
     uint32_t acc = 0;
     if (stride == 1) {
@@ -594,7 +599,7 @@ __device__ __forceinline__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, i
 #pragma unroll
         for (int j = 0; j < 4; ++j) m[j] = sample_mask(C.aw, 4 * j, 1);
         for (int r = part; r < nrows; r += G) {
-            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + r) * C.wp + base);
+            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (__umul24(static_cast<uint32_t>(oy + r), uwp) + ubase));
             const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + r * 16);
             const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
             uint32_t r0 = __builtin_amdgcn_alignbyte(d1, d0, sh), r1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
@@ -608,7 +613,7 @@ __device__ __forceinline__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, i
     } else if (stride == 2) {
         const uint32_t pm0 = sample_mask(C.aw, 0, 2), pm1 = sample_mask(C.aw, 8, 2);
         for (int r = part; r < nrows; r += G) {
-            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + 2 * r) * C.wp + base);
+            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (__umul24(static_cast<uint32_t>(oy + 2 * r), uwp) + ubase));
             const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
             const uint32_t r0 = __builtin_amdgcn_alignbyte(d1, d0, sh), r1 = __builtin_amdgcn_alignbyte(d2, d1, sh);
             const uint32_t r2 = __builtin_amdgcn_alignbyte(d3, d2, sh), r3 = __builtin_amdgcn_alignbyte(d4, d3, sh);
@@ -624,7 +629,7 @@ __device__ __forceinline__ uint32_t sad_int_part(const Ctx& C, int dx, int dy, i
         const uint32_t hi_sel = 0x00000c0cu | (static_cast<uint32_t>(sh) << 16) | (static_cast<uint32_t>(sh + 4) << 24);
         const uint32_t pm = sample_mask(C.aw, 0, 4);
         for (int r = part; r < nrows; r += G) {
-            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (oy + 4 * r) * C.wp + base);
+            const uint32_t* wr = reinterpret_cast<const uint32_t*>(C.L.win + (__umul24(static_cast<uint32_t>(oy + 4 * r), uwp) + ubase));
             const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3];
             const uint32_t e = (__builtin_amdgcn_perm(d1, d0, lo_sel) | __builtin_amdgcn_perm(d3, d2, hi_sel)) & pm;
             acc = __builtin_amdgcn_sad_u8(e, C.L.cs[16 + r], acc);
@@ -1062,14 +1067,22 @@ __device__ __forceinline__ void pw_load(Ctx& C, int wx, int wy, bool bil) {
         // directly, 64 dwords (256 bytes, dword i at byte 4 i) per instruction
         typedef const __attribute__((address_space(1))) void* gvoid;
         typedef __attribute__((address_space(3))) void* lvoid;
+        // the lane's dword base + lane of the window as (plane, row r,
+        // dword d), advanced by 64 dwords (9 rows and 1 dword) per copy:
+        // no division, and the row offset a 24-bit product (rows and the
+        // plane pitch are below 2^24)
+        static_assert(kWave == 9 * kDwords + 1 && kPwSide > 10, "the window walk advances 9 rows and 1 dword");
+        int r = C.lane / kDwords, d = C.lane - (C.lane / kDwords) * kDwords;
+        gptr_u8 pl = planes;
+        const uint32_t upp = static_cast<uint32_t>(J->pp);
         for (int base = 0; base < total; base += kWave) {
-            const int i = base + C.lane;
-            const int ph = i / (kPwSide * kDwords), rem = i - ph * (kPwSide * kDwords);
-            const int r = rem / kDwords, d = rem - r * kDwords;
             const int prow = clampi(wy + r, -2, H) + 2;
-            const gptr_u8 src = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp + a0 + d * 4;
+            const gptr_u8 src = pl + (__umul24(static_cast<uint32_t>(prow), upp) + static_cast<uint32_t>(a0 + d * 4));
             // lanes past the window issue no copy (a copy lands at byte 4 * lane)
-            if (i < total) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(C.L.pw + base * 4), 4, 0, 0);
+            if (C.lane < total - base) __builtin_amdgcn_global_load_lds((gvoid)src, (lvoid)(C.L.pw + base * 4), 4, 0, 0);
+            r += 9;
+            if (++d >= kDwords) { d -= kDwords; ++r; }
+            if (r >= kPwSide) { r -= kPwSide; pl += psz; }
         }
     } else
     for (int i = C.lane; i < total; i += kWave) {
@@ -1151,7 +1164,9 @@ __device__ __forceinline__ uint64_t qpel_eval(Ctx& C, int bx0, int by0, int bw, 
                                by0 + iy >= C.pwy && by0 + iy + bh <= C.pwy + kPwSide;
             if (in_pw) {
                 const int a = x0 - C.pwx + C.pwsh;
-                const uint8_t* base_row = C.L.pw + (phase * kPwSide + (by0 + ty + iy - C.pwy)) * kPwPitch + (a & ~3);
+                const uint8_t* base_row =
+                    C.L.pw + (__umul24(static_cast<uint32_t>(phase * kPwSide + (by0 + ty + iy - C.pwy)), static_cast<uint32_t>(kPwPitch)) +
+                              static_cast<uint32_t>(a & ~3));
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const uint32_t* q = reinterpret_cast<const uint32_t*>(base_row + r * kPwPitch);
@@ -1826,7 +1841,8 @@ __device__ __forceinline__ uint32_t xw_sad_part(const Ctx& C, const uint8_t* win
     uint32_t acc = 0;
     for (int r = part; r < nrows; r += step) {
         const int y = r * stride;
-        const uint32_t* wr = reinterpret_cast<const uint32_t*>(win + (oy + y) * kXwPitch + base);
+        const uint32_t* wr = reinterpret_cast<const uint32_t*>(win + (__umul24(static_cast<uint32_t>(oy + y), static_cast<uint32_t>(kXwPitch)) +
+                                                                   static_cast<uint32_t>(base)));
         const uint32_t* cr = reinterpret_cast<const uint32_t*>(C.L.cur + y * 16);
         const uint32_t d0 = wr[0], d1 = wr[1], d2 = wr[2], d3 = wr[3], d4 = wr[4];
         acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(d1, d0, al) & m[0], cr[0] & m[0], acc);
