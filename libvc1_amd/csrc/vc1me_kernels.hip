@@ -1084,17 +1084,25 @@ __device__ __forceinline__ void pw_load(Ctx& C, int wx, int wy, bool bil) {
             if (++d >= kDwords) { d -= kDwords; ++r; }
             if (r >= kPwSide) { r -= kPwSide; pl += psz; }
         }
-    } else
-    for (int i = C.lane; i < total; i += kWave) {
-        const int ph = i / (kPwSide * kDwords), rem = i - ph * (kPwSide * kDwords);
-        const int r = rem / kDwords, d = rem - r * kDwords;
-        const int prow = clampi(wy + r, -2, H) + 2;
-        const int col = a0 + d * 4;
-        const gptr_u8 row = planes + static_cast<size_t>(ph) * psz + static_cast<size_t>(prow) * J->pp;
-        uint32_t v = 0;
+    } else {
+        // the same walk; the columns clamp to the plane per byte
+        int r = C.lane / kDwords, d = C.lane - (C.lane / kDwords) * kDwords;
+        gptr_u8 pl = planes;
+        const uint32_t upp = static_cast<uint32_t>(J->pp);
+        for (int base = 0; base < total; base += kWave) {
+            const int prow = clampi(wy + r, -2, H) + 2;
+            const gptr_u8 row = pl + __umul24(static_cast<uint32_t>(prow), upp);
+            const int col = a0 + d * 4;
+            if (C.lane < total - base) {
+                uint32_t v = 0;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) v |= static_cast<uint32_t>(row[clampi(col + b, 0, W + 2)]) << (8 * b);
-        *reinterpret_cast<uint32_t*>(C.L.pw + (ph * kPwSide + r) * kPwPitch + d * 4) = v;
+                for (int b = 0; b < 4; ++b) v |= static_cast<uint32_t>(row[clampi(col + b, 0, W + 2)]) << (8 * b);
+                reinterpret_cast<uint32_t*>(C.L.pw)[base + C.lane] = v;
+            }
+            r += 9;
+            if (++d >= kDwords) { d -= kDwords; ++r; }
+            if (r >= kPwSide) { r -= kPwSide; pl += psz; }
+        }
     }
     C.pwx = wx; C.pwy = wy; C.pwsh = c0 - a0; C.pw_ok = true; C.pw_bil = bil;
     C.wx0 = C.wy0 = -(1 << 30); C.ww = C.wh = C.wp = 0;   // the integer window is overwritten
@@ -1792,15 +1800,22 @@ __device__ __forceinline__ void xw_load(Ctx& C, uint64_t ckeys, int t0, int cnt)
         if (a0 >= 0 && a0 + kXwPitch <= W && (W & 3) == 0) {
             typedef const __attribute__((address_space(1))) void* gvoid;
             typedef __attribute__((address_space(3))) void* lvoid;
+            // the lane's dword base + lane as (row r, dword d), advanced by
+            // 64 dwords (6 rows and 4 dwords) per copy; a row's offset is a
+            // 24-bit product (rows and the picture width are below 2^24)
+            static_assert(kWave == 6 * kDw + 4, "the window walk advances 6 rows and 4 dwords");
+            int r = C.lane / kDw, d = C.lane - (C.lane / kDw) * kDw;
 #pragma unroll
             for (int base = 0; base < kXwSide * kDw; base += kWave) {
-                const int i = base + C.lane;
-                const int r = i / kDw, d = i - r * kDw;
                 const int sy = clampi(wy + r, 0, H - 1);
                 // lanes past the window issue no copy (a copy lands at byte 4 * lane)
-                if (i < kXwSide * kDw)
-                    __builtin_amdgcn_global_load_lds((gvoid)(ry + static_cast<size_t>(sy) * W + a0 + d * 4),
-                                                     (lvoid)(dst + base * 4), 4, 0, 0);
+                if (C.lane < kXwSide * kDw - base)
+                    __builtin_amdgcn_global_load_lds(
+                        (gvoid)(ry + (__umul24(static_cast<uint32_t>(sy), static_cast<uint32_t>(W)) + static_cast<uint32_t>(a0 + d * 4))),
+                        (lvoid)(dst + base * 4), 4, 0, 0);
+                r += 6;
+                d += 4;
+                if (d >= kDw) { d -= kDw; ++r; }
             }
         } else {
             for (int i = C.lane; i < kXwSide * kDw; i += kWave) {
@@ -2002,16 +2017,19 @@ __device__ __forceinline__ void extended_search(Ctx& C, int range, uint64_t* out
             for (int j = 0; j < 4; ++j) {
                 const int c = C.lane + j * kWave;
                 const bool ok = c < m;
-                const int g = ok ? c / 25 : 0, i = c - g * 25;
+                // c / 25 = (1311 c) >> 15 for c < 4699, i / 5 = (13 i) >> 6
+                // for i < 64: 24-bit products
+                const int g = ok ? static_cast<int>(__umul24(static_cast<uint32_t>(c), 1311u) >> 15) : 0, i = c - g * 25;
+                const int iy = static_cast<int>(__umul24(static_cast<uint32_t>(i), 13u) >> 6), ix = i - 5 * iy;
                 const int cdx = __shfl(mydx, g), cdy = __shfl(mydy, g);
                 const int ox = stage ? __shfl(sxv, g) : 0, oy = stage ? __shfl(syv, g) : 0;
                 int dx, dy;
                 if (stage == 0) {
-                    dx = clampi(cdx + 4 * (i % 5) - 8, b.xmin, b.xmax);
-                    dy = clampi(cdy + 4 * (i / 5) - 8, b.ymin, b.ymax);
+                    dx = clampi(cdx + 4 * ix - 8, b.xmin, b.xmax);
+                    dy = clampi(cdy + 4 * iy - 8, b.ymin, b.ymax);
                 } else {
-                    dx = clampi(ox + (i % 5) - 2, b.xmin, b.xmax);
-                    dy = clampi(oy + (i / 5) - 2, b.ymin, b.ymax);
+                    dx = clampi(ox + ix - 2, b.xmin, b.xmax);
+                    dy = clampi(oy + iy - 2, b.ymin, b.ymax);
                 }
                 if (ok) {
                     // every point of the candidate lies within +-10 px of it:
