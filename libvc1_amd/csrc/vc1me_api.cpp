@@ -83,6 +83,16 @@ int relax_first_passes() {
     return std::max(1, std::min(v, vc1me::kMaxRelaxPasses));
 }
 
+// Launches of the relaxation's pass 0, by rows (VC1ME_RELAX_PASS0_PHASES;
+// read at every step): launch r analyses the rows y with y % phases == r, so
+// that the macroblocks of later launches read their top neighbours' pass-0
+// state and are not queued again for it.
+int relax_pass0_phases() {
+    const char* e = std::getenv("VC1ME_RELAX_PASS0_PHASES");
+    const int v = e && *e ? std::atoi(e) : 4;
+    return std::max(1, std::min(v, 16));
+}
+
 // Guess pass before the relaxation's pass 0 (VC1ME_RELAX_GUESS=0 turns it off).
 bool relax_guess() {
     const char* e = std::getenv("VC1ME_RELAX_GUESS");
@@ -647,7 +657,13 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         HIP_TRY(hipMalloc(&ctx->d_jobs, sizeof(JobDesc) * n));
         ctx->d_jobs_cap = n;
     }
-    if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1 + vc1me::kMaxRelaxPasses))) return rc;
+    // pass 0 by rows: the grid of one launch covers the largest job's rows of one phase
+    const int phases0 = relax_pass0_phases();
+    int max_items0 = 0;
+    for (int i = 0; i < n; ++i)
+        max_items0 = std::max(max_items0, ctx->h_jobs[i].mbw * ((ctx->h_jobs[i].mbh + phases0 - 1) / phases0));
+    if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1 + vc1me::kMaxRelaxPasses + phases0)))
+        return rc;
     // interpolated references: one owning job per distinct (frame, filter),
     // each with a buffer of the context's step pool (planes live for the step)
     int max_pw = 0, max_ph = 0;
@@ -799,9 +815,16 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     // no host read in between
     auto relax_passes = [&](bool events, int p0, int p1) -> int {
         for (int p = p0; p < p1; ++p) {
-            const int rc = p == 0 ? timed(events, [&] { return vc1me::launch_mb_relax0(ctx->d_jobs, n, max_mbs_all, lds, Q, st); })
-                                  : timed(events, [&] { return vc1me::launch_mb_relaxq(ctx->d_jobs, p, ctx->relax_grid, lds, Q, st); });
-            if (rc) return rc;
+            if (p == 0) {
+                for (int r = 0; r < phases0; ++r)
+                    if (int rc = timed(events, [&] {
+                            return vc1me::launch_mb_relax0(ctx->d_jobs, n, max_items0, r, phases0, lds, Q, st);
+                        }))
+                        return rc;
+                continue;
+            }
+            if (int rc = timed(events, [&] { return vc1me::launch_mb_relaxq(ctx->d_jobs, p, ctx->relax_grid, lds, Q, st); }))
+                return rc;
         }
         return VC1ME_OK;
     };
@@ -852,7 +875,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     const bool graph = ctx->use_graphs && !ctx->profiling && (!after_global || !graph_tag.empty());
     if (graph) {
         // replay the step as one HIP graph, captured once per launch shape
-        const std::string key = shape + "p" + std::to_string(relax ? passes : 0) + "@" +
+        const std::string key = shape + "p" + std::to_string(relax ? passes : 0) + "r" + std::to_string(phases0) + "@" +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_jobs)) + "," +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_out)) + "," +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_relax_wl)) + graph_tag;
@@ -957,12 +980,17 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         HIP_TRY(hipEventElapsedTime(&x, ctx->events[mb_events[k].first], ctx->events[mb_events[k].second]));
         mbms += x;
         if (static_cast<int>(k) == prepass_event) { prepass_ms = x; continue; }
-        const size_t pass = prepass_event >= 0 && static_cast<int>(k) > prepass_event ? k - 1 : k;
-        if (relax && pass < ctx->pass_analyses.size()) ctx->pass_ms.push_back(x);
+        // the relaxation's pass 0 is its first phases0 launches
+        const size_t rel = prepass_event >= 0 && static_cast<int>(k) > prepass_event ? k - 1 : k;
+        const size_t pass = !relax ? rel : rel < static_cast<size_t>(phases0) ? 0 : rel - phases0 + 1;
+        if (relax && pass < ctx->pass_analyses.size()) {
+            if (rel > 0 && rel < static_cast<size_t>(phases0) && !ctx->pass_ms.empty()) ctx->pass_ms.back() += x;
+            else ctx->pass_ms.push_back(x);
+        }
     }
     if (ctx->timing.graph_replay) {
         mb_launches = 0;
-        if (relax) mb_launches = passes + (relax_guess() ? 1 : 0);
+        if (relax) mb_launches = passes + phases0 - 1 + (relax_guess() ? 1 : 0);
         else
             for (int t = 0; t <= max_diag; ++t) mb_launches += per_diag(t) ? 1 : 0;
         if (!relax) ++mb_launches;   // the extended pre-pass

@@ -3104,19 +3104,32 @@ __device__ __forceinline__ void queue_dependents(const JobDesc* J, int job, int 
 }
 
 // Pass 0: every macroblock of every job (one workgroup each, full
-// occupancy); the macroblocks whose neighbours' state changed form the list
-// of pass 1.
+// occupancy), in `phases` launches by rows: launch `phase` analyses the rows
+// y with y % phases == phase of every job.  A macroblock of a later launch
+// reads the state its top-left, top and top-right neighbours committed in
+// this pass (the launches are ordered on the stream), so a macroblock whose
+// state changed queues its bottom-left, bottom and bottom-right dependents
+// only when their row came first (its own row is of the last phase, the row
+// below of phase 0); its right neighbour is analysed in the same launch, in
+// no set order, and is queued as in a single launch.  The dependents that
+// read a changed field and were analysed before it form the list of pass 1.
+// Grid: (max over jobs of mbw * ceil(mbh / phases), jobs).
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_MB_WAVES))) k_mb_relax0(const JobDesc* __restrict__ jobs,
-                                                                                       RelaxQueue Q) {
+                                                                                       RelaxQueue Q, int phase,
+                                                                                       int phases) {
     extern __shared__ __align__(16) uint8_t smem[];
     const GridItem it = xcd_banded_item();
     const int job = it.job;
     const JobDesc* J = &jobs[job];
     const int mbw = J->mbw;
-    const int pos = it.pos;
-    if (pos >= mbw * J->mbh) return;
-    const NbState after = run_mb(J, pos % mbw, pos / mbw, smem);
-    if (threadIdx.x == 0 && after.changed) queue_dependents(J, job, pos, 1, Q, after.changed);
+    const int band = it.pos / mbw;
+    const int mx = it.pos - band * mbw;
+    const int my = phase + phases * band;
+    if (my >= J->mbh) return;
+    const int pos = my * mbw + mx;
+    const NbState after = run_mb(J, mx, my, smem);
+    const uint32_t mask = phase + 1 < phases ? (after.changed & 1u) : after.changed;
+    if (threadIdx.x == 0 && mask) queue_dependents(J, job, pos, 1, Q, mask);
 }
 
 // Pass p > 0 as a work queue: a grid of resident workgroups takes entries of
@@ -4480,10 +4493,11 @@ hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_mbs, int ld
     return hipGetLastError();
 }
 
-hipError_t launch_mb_relax0(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, const RelaxQueue& q,
-                            hipStream_t s) {
-    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mb_relax0, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs, q);
+hipError_t launch_mb_relax0(const JobDesc* d_jobs, int njobs, int max_items, int phase, int phases, int lds_bytes,
+                            const RelaxQueue& q, hipStream_t s) {
+    if (njobs <= 0 || max_items <= 0) return hipSuccess;
+    if (phases < 1 || phase < 0 || phase >= phases) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mb_relax0, dim3(max_items, njobs), dim3(64), lds_bytes, s, d_jobs, q, phase, phases);
     return hipGetLastError();
 }
 
