@@ -93,6 +93,14 @@ int relax_pass0_phases() {
     return std::max(1, std::min(v, 16));
 }
 
+// Launches of the guess pass, by rows as pass 0 (VC1ME_RELAX_GUESS_PHASES;
+// read at every step).
+int relax_guess_phases() {
+    const char* e = std::getenv("VC1ME_RELAX_GUESS_PHASES");
+    const int v = e && *e ? std::atoi(e) : 4;
+    return std::max(1, std::min(v, 16));
+}
+
 // Guess pass before the relaxation's pass 0 (VC1ME_RELAX_GUESS=0 turns it off).
 bool relax_guess() {
     const char* e = std::getenv("VC1ME_RELAX_GUESS");
@@ -658,11 +666,14 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
         ctx->d_jobs_cap = n;
     }
     // pass 0 by rows: the grid of one launch covers the largest job's rows of one phase
-    const int phases0 = relax_pass0_phases();
-    int max_items0 = 0;
-    for (int i = 0; i < n; ++i)
+    const int phases0 = relax_pass0_phases(), phasesg = relax_guess_phases();
+    int max_items0 = 0, max_itemsg = 0;
+    for (int i = 0; i < n; ++i) {
         max_items0 = std::max(max_items0, ctx->h_jobs[i].mbw * ((ctx->h_jobs[i].mbh + phases0 - 1) / phases0));
-    if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1 + vc1me::kMaxRelaxPasses + phases0)))
+        max_itemsg = std::max(max_itemsg, ctx->h_jobs[i].mbw * ((ctx->h_jobs[i].mbh + phasesg - 1) / phasesg));
+    }
+    if (int rc = ensure_events(ctx, 4 + 2 * static_cast<size_t>(max_diag + 1 + vc1me::kMaxRelaxPasses + phases0 +
+                                                                phasesg)))
         return rc;
     // interpolated references: one owning job per distinct (frame, filter),
     // each with a buffer of the context's step pool (planes live for the step)
@@ -789,7 +800,8 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     int mb_launches = 0;
     std::vector<std::pair<int, int>> mb_events;
     int prepass_event = -1;   // mb_events entry of the guess pass (or the sweep's extended pre-pass)
-    auto timed = [&](bool events, const std::function<hipError_t()>& launch) -> int {
+    // one event pair around `launch`, which makes `count` kernel launches
+    auto timed = [&](bool events, const std::function<hipError_t()>& launch, int count = 1) -> int {
         const int ev = 4 + 2 * mb_launches;
         const bool rec = events && ctx->profiling && static_cast<size_t>(ev + 1) < ctx->events.size();
         if (rec) HIP_TRY(hipEventRecord(ctx->events[ev], st));
@@ -798,7 +810,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             HIP_TRY(hipEventRecord(ctx->events[ev + 1], st));
             mb_events.emplace_back(ev, ev + 1);
         }
-        ++mb_launches;
+        mb_launches += count;
         return VC1ME_OK;
     };
     // one launch per anti-diagonal: every macroblock after its neighbours
@@ -855,7 +867,16 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
             HIP_TRY(vc1me::launch_relax_init(ctx->d_jobs, n, max_mbs_all, st));
             if (relax_guess()) {
                 const size_t at = mb_events.size();
-                if (int rc = timed(events, [&] { return vc1me::launch_mb_guess(ctx->d_jobs, n, max_mbs_all, lds, st); }))
+                // the guess's launches by rows are timed as one interval
+                if (int rc = timed(
+                        events,
+                        [&] {
+                            for (int r = 0; r < phasesg; ++r)
+                                if (hipError_t e = vc1me::launch_mb_guess(ctx->d_jobs, n, max_itemsg, r, phasesg, lds, st))
+                                    return e;
+                            return hipSuccess;
+                        },
+                        phasesg))
                     return rc;
                 if (mb_events.size() > at) prepass_event = static_cast<int>(at);
             }
@@ -875,7 +896,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     const bool graph = ctx->use_graphs && !ctx->profiling && (!after_global || !graph_tag.empty());
     if (graph) {
         // replay the step as one HIP graph, captured once per launch shape
-        const std::string key = shape + "p" + std::to_string(relax ? passes : 0) + "r" + std::to_string(phases0) + "@" +
+        const std::string key = shape + "p" + std::to_string(relax ? passes : 0) + "r" + std::to_string(phases0) + "g" + std::to_string(phasesg) + "@" +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_jobs)) + "," +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_out)) + "," +
                                 std::to_string(reinterpret_cast<long long>(ctx->d_relax_wl)) + graph_tag;
@@ -990,7 +1011,7 @@ int run_jobs(vc1me_ctx* ctx, const vc1me_job* jobs, int n, const float* const* n
     }
     if (ctx->timing.graph_replay) {
         mb_launches = 0;
-        if (relax) mb_launches = passes + phases0 - 1 + (relax_guess() ? 1 : 0);
+        if (relax) mb_launches = passes + phases0 - 1 + (relax_guess() ? phasesg : 0);
         else
             for (int t = 0; t <= max_diag; ++t) mb_launches += per_diag(t) ? 1 : 0;
         if (!relax) ++mb_launches;   // the extended pre-pass

@@ -202,12 +202,13 @@ hipError_t launch_mb_diagonal(const JobDesc* d_jobs, int njobs, int diag, int ma
 // extended long-range search of every macroblock of the plain jobs, ahead of
 // the anti-diagonal sweep (the relaxation schedule computes it on demand)
 hipError_t launch_extended(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
-// relaxation schedule: starting state, guess pass, pass 0 over every
+// relaxation schedule: starting state, guess pass (by rows, as pass 0), pass 0 over every
 // macroblock (launch `phase` of `phases` by rows; max_items = max over jobs
 // of mbw * ceil(mbh / phases)), and a work-queue pass over pass p's list
 // (grid: resident workgroups, relax_queue_grid)
 hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipStream_t s);
-hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s);
+hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_items, int phase, int phases, int lds_bytes,
+                           hipStream_t s);
 hipError_t launch_mb_relax0(const JobDesc* d_jobs, int njobs, int max_items, int phase, int phases, int lds_bytes,
                             const RelaxQueue& q, hipStream_t s);
 hipError_t launch_mb_relaxq(const JobDesc* d_jobs, int pass, int grid, int lds_bytes, const RelaxQueue& q,

@@ -3071,13 +3071,19 @@ __device__ __forceinline__ GridItem xcd_banded_item() {
     return {item / per, item - (item / per) * per};
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_MB_WAVES))) k_mb_guess(const JobDesc* __restrict__ jobs) {
+// In `phases` launches by rows like pass 0 (launch `phase` takes the rows y
+// with y % phases == phase), so that the guesses of later rows start from
+// the guesses of the rows above them.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(VC1ME_MB_WAVES))) k_mb_guess(const JobDesc* __restrict__ jobs,
+                                                                                      int phase, int phases) {
     extern __shared__ __align__(16) uint8_t smem[];
     const GridItem it = xcd_banded_item();
     const JobDesc* J = &jobs[it.job];
-    const int pos = it.pos;
-    if (pos >= J->mbw * J->mbh) return;
-    run_mb<true>(J, pos % J->mbw, pos / J->mbw, smem);
+    const int band = it.pos / J->mbw;
+    const int mx = it.pos - band * J->mbw;
+    const int my = phase + phases * band;
+    if (my >= J->mbh) return;
+    run_mb<true>(J, mx, my, smem);
 }
 
 // Appends the dependents in `mask` (bit k: right, bottom-left, bottom,
@@ -4487,9 +4493,11 @@ hipError_t launch_relax_init(const JobDesc* d_jobs, int njobs, int max_mbs, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_mbs, int lds_bytes, hipStream_t s) {
-    if (njobs <= 0 || max_mbs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mb_guess, dim3(max_mbs, njobs), dim3(64), lds_bytes, s, d_jobs);
+hipError_t launch_mb_guess(const JobDesc* d_jobs, int njobs, int max_items, int phase, int phases, int lds_bytes,
+                           hipStream_t s) {
+    if (njobs <= 0 || max_items <= 0) return hipSuccess;
+    if (phases < 1 || phase < 0 || phase >= phases) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mb_guess, dim3(max_items, njobs), dim3(64), lds_bytes, s, d_jobs, phase, phases);
     return hipGetLastError();
 }
 

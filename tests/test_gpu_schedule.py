@@ -122,6 +122,7 @@ def test_relax_continues_when_the_launched_passes_do_not_settle(graphs):
     _say(f"[case] relax with 2 first passes, graphs {graphs}")
     os.environ["VC1ME_RELAX_FIRST_PASSES"] = "2"
     os.environ["VC1ME_RELAX_PASS0_PHASES"] = "3"
+    os.environ["VC1ME_RELAX_GUESS_PHASES"] = "2"
     be = Backend(0)       # a fresh context: no pass count known for the shape
     try:
         be.set_graphs(graphs)
@@ -137,31 +138,34 @@ def test_relax_continues_when_the_launched_passes_do_not_settle(graphs):
     finally:
         del os.environ["VC1ME_RELAX_FIRST_PASSES"]
         del os.environ["VC1ME_RELAX_PASS0_PHASES"]
+        del os.environ["VC1ME_RELAX_GUESS_PHASES"]
         be.close()
     assert t1.schedule == abi.SCHEDULE_RELAX and t1.relax_passes > 2
     assert t2.relax_passes == t1.relax_passes
-    # in the graph: the guess pass, pass 0 as three launches by rows, then
-    # the other passes the shape needed plus two
-    assert t2.mb_kernel_launches == 1 + t1.relax_passes + 2 + 2 if graphs else True
+    # in the graph: the guess pass as two launches by rows, pass 0 as three,
+    # then the other passes the shape needed plus two
+    assert t2.mb_kernel_launches == 2 + t1.relax_passes + 2 + 2 if graphs else True
     for got in (first, again):
         assert not got.diff(want), describe_diff(got, want, p.width)
     if graphs:
         assert t1.graph_replay == 2 and t2.graph_replay == 2    # recaptured with more passes
 
 
-@pytest.mark.parametrize("phases", [1, 2, 5, 16])
-def test_relax_pass0_by_rows_is_exact(phases):
+@pytest.mark.parametrize("phases,guess_phases", [(1, 1), (2, 3), (5, 2), (16, 16)])
+def test_relax_pass0_by_rows_is_exact(phases, guess_phases):
     """Pass 0 launched by rows (launch r analyses the rows y with
     y % phases == r; a changed macroblock queues its bottom dependents only
     when their row came first) relaxes to the oracle's field for every phase
-    count, including more phases than the picture has macroblock rows."""
+    count, including more phases than the picture has macroblock rows, and
+    whatever the guess pass's own launches by rows left as starting state."""
     from libvc1_amd import Backend
     d = synth.pan_rectangles(7, 4, width=320, height=176, frame_indices=[0, 3])
     p = abi.default_params(320, 176, pqindex=8, motion_search_range=256, motion_local_search_range=32,
                            mode=abi.MODE_MIXED_MV)
     want = oracle_analyze(p, d[3], d[0])
-    _say(f"[case] relax, pass 0 in {phases} launches by rows")
+    _say(f"[case] relax, pass 0 in {phases} launches by rows, guess in {guess_phases}")
     os.environ["VC1ME_RELAX_PASS0_PHASES"] = str(phases)
+    os.environ["VC1ME_RELAX_GUESS_PHASES"] = str(guess_phases)
     be = Backend(0)
     try:
         fc, fr = be.upload(d[3]), be.upload(d[0])
@@ -173,6 +177,7 @@ def test_relax_pass0_by_rows_is_exact(phases):
             fr.release()
     finally:
         del os.environ["VC1ME_RELAX_PASS0_PHASES"]
+        del os.environ["VC1ME_RELAX_GUESS_PHASES"]
         be.close()
     assert t.schedule == abi.SCHEDULE_RELAX
     assert not got.diff(want), describe_diff(got, want, p.width)
